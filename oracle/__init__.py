@@ -1,0 +1,361 @@
+"""CPU oracle for the 2DGS surfel training step -- test infrastructure only.
+
+A ctypes view of ``oracle/liboracle.so`` (built from ``cpuref.cpp`` by
+``oracle/Makefile``). Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this package; the product
+(``splattar_amd``) never does.
+
+The C++ restates, function by function, the reference oracle
+``crates/gs-cpu-ref/src/{forward,backward,math,scene}.rs`` in FP64, plus an
+fp32 mirror of the integer stages, the L1 + D-SSIM loss of
+``first_attempt/crates/gs-train/src/shaders/ssim.wgsl`` and the Adam step of
+``first_attempt/crates/gs-train/src/shaders/adam.wgsl``. See ``cpuref.h`` for
+the per-function citations and the checks that pin each of them.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_u32p = C.POINTER(C.c_uint32)
+_u64p = C.POINTER(C.c_uint64)
+_i64p = C.POINTER(C.c_int64)
+
+
+class _Scene(C.Structure):
+    _fields_ = [("n", C.c_int64), ("sh_degree", C.c_int32), ("pos", _dp), ("scales", _dp),
+                ("quat", _dp), ("opacity", _dp), ("sh", _dp)]
+
+
+class _Camera(C.Structure):
+    _fields_ = [("center", C.c_double * 3), ("quat", C.c_double * 4), ("focal", C.c_double),
+                ("width", C.c_int32), ("height", C.c_int32)]
+
+
+class _Grads(C.Structure):
+    _fields_ = [("pos", _dp), ("scales", _dp), ("quat", _dp), ("opacity", _dp), ("sh", _dp),
+                ("cam_center", C.c_double * 3), ("cam_quat", C.c_double * 4), ("focal", C.c_double)]
+
+
+class _Cam32(C.Structure):
+    _fields_ = [("rot", C.c_float * 9), ("center", C.c_float * 3), ("focal", C.c_float),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("tiles_x", C.c_uint32),
+                ("tiles_y", C.c_uint32)]
+
+
+def lib():
+    """Load (once) and return the oracle shared library."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {_LIB_PATH} (run `make -C oracle`)")
+        L = C.CDLL(_LIB_PATH)
+        L.oref_quat_to_mat.argtypes = [_dp, _dp]
+        L.oref_quat_grad.argtypes = [_dp, _dp, _dp]
+        L.oref_sh_basis.argtypes = [C.c_int, _dp, _dp]
+        L.oref_sh_basis_grad.argtypes = [C.c_int, _dp, _dp]
+        L.oref_render.argtypes = [C.POINTER(_Scene), C.POINTER(_Camera), C.c_int, _dp, _dp, _dp,
+                                  _dp, _i64p]
+        L.oref_distortion_loss.argtypes = [C.POINTER(_Scene), C.POINTER(_Camera), C.c_int]
+        L.oref_distortion_loss.restype = C.c_double
+        L.oref_gradients.argtypes = [C.POINTER(_Scene), C.POINTER(_Camera), C.c_int, _dp, _dp,
+                                     _dp, C.c_double, C.POINTER(_Grads)]
+        L.oref_prepare_order.argtypes = [C.POINTER(_Scene), C.POINTER(_Camera), _i64p]
+        L.oref_prepare_order.restype = C.c_int64
+        L.oref_prep_f32.argtypes = [C.c_int64, _fp, _fp, _fp, C.POINTER(_Cam32), _u32p, _u32p,
+                                    _u32p]
+        L.oref_exclusive_scan_u32.argtypes = [C.c_int64, _u32p, _u32p]
+        L.oref_exclusive_scan_u32.restype = C.c_uint32
+        L.oref_emit_keys.argtypes = [C.c_int64, _u32p, _u32p, C.c_uint32, C.c_uint32, _u64p,
+                                     _u32p, C.c_int64]
+        L.oref_emit_keys.restype = C.c_int64
+        L.oref_stable_sort_u64.argtypes = [C.c_int64, _u64p, _u32p]
+        L.oref_tile_ranges.argtypes = [C.c_int64, _u64p, C.c_uint32, C.c_uint32, _u32p]
+        L.oref_l1_ssim.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_double, _dp, _dp,
+                                   _dp]
+        L.oref_l1_ssim.restype = C.c_double
+        L.oref_blur.argtypes = [C.c_int, C.c_int, _dp, _dp]
+        L.oref_adam_step_f32.argtypes = [C.c_int64, _fp, _fp, _fp, _fp, _fp, C.c_float, C.c_float,
+                                         C.c_int, C.c_int]
+        L.oref_num_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# --------------------------------------------------------------- scene types
+
+@dataclass
+class Scene:
+    """FP64 micro-scene (crates/gs-cpu-ref/src/scene.rs:23-52).
+
+    sh is (n, ncoef, 3), coefficient-major rgb.
+    """
+    pos: np.ndarray
+    scales: np.ndarray
+    quat: np.ndarray
+    opacity: np.ndarray
+    sh: np.ndarray
+    sh_degree: int
+
+    @property
+    def n(self) -> int:
+        return int(self.pos.shape[0])
+
+    def copy(self) -> "Scene":
+        return Scene(self.pos.copy(), self.scales.copy(), self.quat.copy(), self.opacity.copy(),
+                     self.sh.copy(), self.sh_degree)
+
+    def _c(self):
+        keep = (_d(self.pos), _d(self.scales), _d(self.quat), _d(self.opacity), _d(self.sh))
+        s = _Scene(self.n, self.sh_degree, *[_ptr(k, _dp) for k in keep])
+        return s, keep
+
+
+@dataclass
+class Camera:
+    """FP64 camera (scene.rs:36-45): camera-to-world quat xyzw, center, focal."""
+    center: np.ndarray
+    quat: np.ndarray
+    focal: float
+    width: int
+    height: int
+
+    def copy(self) -> "Camera":
+        return Camera(self.center.copy(), self.quat.copy(), float(self.focal), self.width,
+                      self.height)
+
+    def _c(self):
+        c = _Camera()
+        for k in range(3):
+            c.center[k] = float(self.center[k])
+        for k in range(4):
+            c.quat[k] = float(self.quat[k])
+        c.focal = float(self.focal)
+        c.width = int(self.width)
+        c.height = int(self.height)
+        return c
+
+
+@dataclass
+class Gradients:
+    """Gradients of every parameter class (scene.rs:66-97)."""
+    pos: np.ndarray
+    scales: np.ndarray
+    quat: np.ndarray
+    opacity: np.ndarray
+    sh: np.ndarray
+    cam_center: np.ndarray = field(default_factory=lambda: np.zeros(3))
+    cam_quat: np.ndarray = field(default_factory=lambda: np.zeros(4))
+    focal: float = 0.0
+
+
+# ------------------------------------------------------------------ math.rs
+
+def quat_to_mat(q) -> np.ndarray:
+    """math.rs:14-34; returns R (3x3, R[:, k] = image of e_k)."""
+    out = np.zeros(9)
+    lib().oref_quat_to_mat(_ptr(_d(q), _dp), _ptr(out, _dp))
+    return out.reshape(3, 3).T.copy()
+
+
+def quat_grad(q, dl_dr: np.ndarray) -> np.ndarray:
+    """math.rs:38-88; dl_dr is 3x3 with dl_dr[j, k] = dL/dR[j][k]."""
+    cm = _d(np.asarray(dl_dr, dtype=np.float64).T.reshape(9))
+    out = np.zeros(4)
+    lib().oref_quat_grad(_ptr(_d(q), _dp), _ptr(cm, _dp), _ptr(out, _dp))
+    return out
+
+
+def sh_basis(degree: int, d) -> np.ndarray:
+    out = np.zeros(16)
+    k = lib().oref_sh_basis(degree, _ptr(_d(d), _dp), _ptr(out, _dp))
+    return out[:k]
+
+
+def sh_basis_grad(degree: int, d) -> np.ndarray:
+    out = np.zeros(48)
+    k = lib().oref_sh_basis_grad(degree, _ptr(_d(d), _dp), _ptr(out, _dp))
+    return out[:3 * k].reshape(k, 3)
+
+
+# -------------------------------------------------------- forward / backward
+
+def render(scene: Scene, cam: Camera, mode: int = 1, with_hits: bool = False):
+    """forward.rs:207-246. Returns dict color (H,W,3), alpha, depth, normal."""
+    P = cam.width * cam.height
+    color = np.zeros(P * 3)
+    alpha = np.zeros(P)
+    depth = np.zeros(P)
+    normal = np.zeros(P * 3)
+    hits = np.zeros(1, dtype=np.int64)
+    s, keep = scene._c()
+    c = cam._c()
+    rc = lib().oref_render(C.byref(s), C.byref(c), mode, _ptr(color, _dp), _ptr(alpha, _dp),
+                           _ptr(depth, _dp), _ptr(normal, _dp), _ptr(hits, _i64p))
+    if rc != 0:
+        raise RuntimeError("oref_render failed")
+    H, W = cam.height, cam.width
+    out = dict(color=color.reshape(H, W, 3), alpha=alpha.reshape(H, W),
+               depth=depth.reshape(H, W), normal=normal.reshape(H, W, 3))
+    if with_hits:
+        out["hits"] = int(hits[0])
+    return out
+
+
+def distortion_loss(scene: Scene, cam: Camera, mode: int = 1) -> float:
+    s, keep = scene._c()
+    c = cam._c()
+    return float(lib().oref_distortion_loss(C.byref(s), C.byref(c), mode))
+
+
+def gradients(scene: Scene, cam: Camera, dl_dcolor, dl_ddepth=None, dl_dnormal=None,
+              lambda_dist: float = 0.0, mode: int = 1) -> Gradients:
+    """backward.rs:57-249 (gradients_full)."""
+    n = scene.n
+    ncoef = (scene.sh_degree + 1) ** 2
+    g = Gradients(np.zeros((n, 3)), np.zeros((n, 2)), np.zeros((n, 4)), np.zeros(n),
+                  np.zeros((n, ncoef, 3)))
+    cg = _Grads(_ptr(g.pos, _dp), _ptr(g.scales, _dp), _ptr(g.quat, _dp), _ptr(g.opacity, _dp),
+                _ptr(g.sh, _dp))
+    s, keep = scene._c()
+    c = cam._c()
+    dlc = _d(dl_dcolor).reshape(-1)
+    dld = _d(dl_ddepth).reshape(-1) if dl_ddepth is not None else None
+    dln = _d(dl_dnormal).reshape(-1) if dl_dnormal is not None else None
+    rc = lib().oref_gradients(C.byref(s), C.byref(c), mode, _ptr(dlc, _dp), _ptr(dld, _dp),
+                              _ptr(dln, _dp), float(lambda_dist), C.byref(cg))
+    if rc != 0:
+        raise RuntimeError("oref_gradients failed")
+    g.cam_center = np.array(cg.cam_center[:])
+    g.cam_quat = np.array(cg.cam_quat[:])
+    g.focal = float(cg.focal)
+    return g
+
+
+def prepare_order(scene: Scene, cam: Camera) -> np.ndarray:
+    s, keep = scene._c()
+    c = cam._c()
+    order = np.zeros(scene.n, dtype=np.int64)
+    k = lib().oref_prepare_order(C.byref(s), C.byref(c), _ptr(order, _i64p))
+    return order[:k]
+
+
+# ------------------------------------------------------ fp32 integer mirror
+
+def cam32(rot: np.ndarray, center, focal: float, width: int, height: int) -> _Cam32:
+    """Build the fp32 camera the mirror and the GPU share. rot is R (3x3)."""
+    c = _Cam32()
+    r = np.asarray(rot, dtype=np.float32)
+    for k in range(3):
+        for j in range(3):
+            c.rot[3 * k + j] = float(r[j, k])
+    for k in range(3):
+        c.center[k] = float(np.float32(center[k]))
+    c.focal = float(np.float32(focal))
+    c.width, c.height = int(width), int(height)
+    c.tiles_x, c.tiles_y = (width + 15) // 16, (height + 15) // 16
+    return c
+
+
+def prep_f32(pos4, scales2, quat4, cam: _Cam32):
+    """fp32 mirror of surfel_prep_fwd.wgsl:64-132 -> (rects n*4, keys, counts)."""
+    pos4 = np.ascontiguousarray(pos4, dtype=np.float32)
+    scales2 = np.ascontiguousarray(scales2, dtype=np.float32)
+    quat4 = np.ascontiguousarray(quat4, dtype=np.float32)
+    n = pos4.shape[0]
+    rects = np.zeros((n, 4), dtype=np.uint32)
+    keys = np.zeros(n, dtype=np.uint32)
+    counts = np.zeros(n, dtype=np.uint32)
+    lib().oref_prep_f32(n, _ptr(pos4, _fp), _ptr(scales2, _fp), _ptr(quat4, _fp), C.byref(cam),
+                        _ptr(rects, _u32p), _ptr(keys, _u32p), _ptr(counts, _u32p))
+    return rects, keys, counts
+
+
+def exclusive_scan(values) -> tuple[np.ndarray, int]:
+    v = np.ascontiguousarray(values, dtype=np.uint32)
+    out = np.zeros_like(v)
+    total = lib().oref_exclusive_scan_u32(v.shape[0], _ptr(v, _u32p), _ptr(out, _u32p))
+    return out, int(total)
+
+
+def emit_keys(rects, depth_keys, tiles_x: int, tile_base: int = 0, max_entries: int | None = None):
+    rects = np.ascontiguousarray(rects, dtype=np.uint32)
+    depth_keys = np.ascontiguousarray(depth_keys, dtype=np.uint32)
+    n = depth_keys.shape[0]
+    r = rects.reshape(n, 4).astype(np.int64)
+    valid = (r[:, 2] >= r[:, 0]) & (r[:, 3] >= r[:, 1])
+    total = int(((r[:, 2] - r[:, 0] + 1) * (r[:, 3] - r[:, 1] + 1) * valid).sum())
+    cap = total if max_entries is None else int(max_entries)
+    keys = np.zeros(max(cap, 1), dtype=np.uint64)
+    payloads = np.zeros(max(cap, 1), dtype=np.uint32)
+    e = lib().oref_emit_keys(n, _ptr(rects, _u32p), _ptr(depth_keys, _u32p), tiles_x, tile_base,
+                             _ptr(keys, _u64p), _ptr(payloads, _u32p), cap)
+    m = min(e, cap)
+    return keys[:m].copy(), payloads[:m].copy(), int(e)
+
+
+def stable_sort(keys, payloads):
+    k = np.ascontiguousarray(keys, dtype=np.uint64).copy()
+    p = np.ascontiguousarray(payloads, dtype=np.uint32).copy()
+    lib().oref_stable_sort_u64(k.shape[0], _ptr(k, _u64p), _ptr(p, _u32p))
+    return k, p
+
+
+def tile_ranges(sorted_keys, num_tiles: int, tile_base: int = 0) -> np.ndarray:
+    k = np.ascontiguousarray(sorted_keys, dtype=np.uint64)
+    r = np.zeros((num_tiles, 2), dtype=np.uint32)
+    lib().oref_tile_ranges(k.shape[0], _ptr(k, _u64p), num_tiles, tile_base, _ptr(r, _u32p))
+    return r
+
+
+# -------------------------------------------------------------- loss / Adam
+
+def l1_ssim(img, target, lam: float = 0.2, dssim_scale: float = 0.5, grad: bool = True):
+    """ssim.wgsl:41-143. img/target (H, W, 3). Returns (loss, dl_dimg, l1, ssim)."""
+    img = _d(img)
+    target = _d(target)
+    H, W = img.shape[:2]
+    dl = np.zeros(H * W * 3) if grad else None
+    l1 = np.zeros(1)
+    ss = np.zeros(1)
+    loss = lib().oref_l1_ssim(W, H, _ptr(img.reshape(-1), _dp), _ptr(target.reshape(-1), _dp),
+                              lam, dssim_scale, _ptr(dl, _dp), _ptr(l1, _dp), _ptr(ss, _dp))
+    return (float(loss), dl.reshape(H, W, 3) if grad else None, float(l1[0]), float(ss[0]))
+
+
+def blur(channel: np.ndarray) -> np.ndarray:
+    a = _d(channel)
+    H, W = a.shape
+    out = np.zeros_like(a)
+    lib().oref_blur(W, H, _ptr(a.reshape(-1), _dp), _ptr(out.reshape(-1), _dp))
+    return out
+
+
+def adam_step_f32(raw, grad_act, m, v, lr: float, activation: int, t: int, reg: float = 0.0):
+    """adam.wgsl:44-74 in f32 (in place). Returns the activated copy."""
+    act = np.zeros_like(raw)
+    lib().oref_adam_step_f32(raw.shape[0], _ptr(raw, _fp), _ptr(grad_act, _fp), _ptr(m, _fp),
+                             _ptr(v, _fp), _ptr(act, _fp), lr, reg, activation, t)
+    return act
+
+
+def num_threads() -> int:
+    return int(lib().oref_num_threads())

@@ -1,0 +1,153 @@
+/*
+ * CPU oracle for the differentiable 2D-Gaussian-surfel training step.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker the HIP path is
+ * compared against; nothing in the product (splattar_amd/, include/) links,
+ * loads or calls it. Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may use it.
+ *
+ * Contents (every function cites the reference code it restates):
+ *   1. FP64 restatement of crates/gs-cpu-ref (forward.rs, backward.rs,
+ *      math.rs, scene.rs): prepare / covers_pixel / evaluate / render /
+ *      distortion_loss / gradients_full, quat_to_mat / quat_grad /
+ *      sh_basis / sh_basis_grad. Literal O(P*N) mode and a tile-binned mode
+ *      that walks exactly the same surfels in the same order.
+ *   2. An fp32 mirror of the integer stages of the GPU path
+ *      (surfel_prep_fwd.wgsl:64-132 rect + depth key, tile_binning.wgsl
+ *      count/expand/ranges, the stable (tile||depth) sort of binning.rs:5-8)
+ *      with the operation order pinned, for bit-exact key/offset/range checks.
+ *   3. FP64 L1 + D-SSIM loss and its gradient (first_attempt/crates/gs-train/
+ *      src/shaders/ssim.wgsl:1-143, ssim.rs:1-4,84).
+ *   4. f32 Adam with the raw->activated chain (first_attempt/crates/gs-train/
+ *      src/shaders/adam.wgsl:33-74, optim.rs:179-197).
+ *
+ * Parity pins (what checks each part; see tests/):
+ *   - quat_grad / sh_basis_grad / triple products: central FD at the
+ *     reference's own points (math.rs:200-267).
+ *   - sh_basis: known answers of crates/gs-core/src/sh.rs:80-126.
+ *   - render / gradients_full: central FD over every parameter class at the
+ *     reference gradcheck seeds (crates/gs-cpu-ref/tests/gradcheck.rs:214-326),
+ *     closed forms (single fronto-parallel surfel, two stacked surfels,
+ *     T_final = prod(1-a), sum w = 1 - T_final), literal == binned.
+ *   - integer mirror: std::stable_sort / exclusive scan / brute-force
+ *     range extraction, the reference property-test case lists.
+ *   - SSIM: FD in FP64 on small images, SSIM(x,x) = 1, blur self-adjointness.
+ *   - Adam: the reference's own CPU loop (gs-train/tests/adam.rs:37-98).
+ */
+#ifndef SPLATTAR_ORACLE_CPUREF_H
+#define SPLATTAR_ORACLE_CPUREF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Scene in FP64, flat arrays. sh is coefficient-major rgb:
+ * sh[(i*ncoef + k)*3 + ch], ncoef = (deg+1)^2. */
+typedef struct {
+    int64_t n;
+    int32_t sh_degree;
+    const double* pos;      /* n*3 */
+    const double* scales;   /* n*2, activated */
+    const double* quat;     /* n*4, xyzw, unnormalized */
+    const double* opacity;  /* n, activated */
+    const double* sh;       /* n*ncoef*3 */
+} oref_scene;
+
+typedef struct {
+    double center[3];
+    double quat[4];  /* camera-to-world, xyzw, unnormalized */
+    double focal;
+    int32_t width;
+    int32_t height;
+} oref_camera;
+
+typedef struct {
+    double* pos;        /* n*3 */
+    double* scales;     /* n*2 */
+    double* quat;       /* n*4 */
+    double* opacity;    /* n */
+    double* sh;         /* n*ncoef*3 */
+    double cam_center[3];
+    double cam_quat[4];
+    double focal;
+} oref_grads;
+
+/* ---- math.rs ---- */
+void oref_quat_to_mat(const double q[4], double r_colmajor[9]);
+void oref_quat_grad(const double q[4], const double dl_dr_colmajor[9], double out[4]);
+int oref_sh_basis(int degree, const double d[3], double* out /* 16 */);
+int oref_sh_basis_grad(int degree, const double d[3], double* out /* 16*3 */);
+
+/* ---- forward.rs ---- */
+/* mode 0 = literal O(P*N) walk, 1 = tile-binned walk (identical results).
+ * Any output pointer may be NULL. color/normal are P*3, alpha/depth P. */
+int oref_render(const oref_scene* s, const oref_camera* cam, int mode,
+                double* color, double* alpha, double* depth, double* normal,
+                int64_t* n_hits /* optional: total evaluated (pixel, surfel) pairs */);
+double oref_distortion_loss(const oref_scene* s, const oref_camera* cam, int mode);
+
+/* ---- backward.rs ---- */
+/* dl_ddepth / dl_dnormal may be NULL (zero). Grad arrays are overwritten. */
+int oref_gradients(const oref_scene* s, const oref_camera* cam, int mode,
+                   const double* dl_dcolor, const double* dl_ddepth,
+                   const double* dl_dnormal, double lambda_dist, oref_grads* g);
+
+/* Number of surfels that survive the near cull and their prepare() order
+ * (indices sorted by (f32 depth bits, idx)). order may be NULL. */
+int64_t oref_prepare_order(const oref_scene* s, const oref_camera* cam, int64_t* order);
+
+/* ---- fp32 mirror of the integer stages ---- */
+typedef struct {
+    float rot[9];     /* camera-to-world rotation, column-major (rot[3*k+j] = R[j][k]) */
+    float center[3];
+    float focal;
+    uint32_t width, height;
+    uint32_t tiles_x, tiles_y;
+} oref_cam32;
+
+/* surfel_prep_fwd.wgsl:64-132 with the operation order of the HIP project
+ * kernel. rects n*4 (x0,y0,x1,y1; culled = 1,1,0,0), keys n, counts n. */
+void oref_prep_f32(int64_t n, const float* pos4, const float* scales2, const float* quat4,
+                   const oref_cam32* cam, uint32_t* rects, uint32_t* depth_keys,
+                   uint32_t* counts);
+/* Exclusive scan (mod 2^32); returns the total. */
+uint32_t oref_exclusive_scan_u32(int64_t n, const uint32_t* in, uint32_t* out);
+/* Expansion (tile_binning.wgsl:46-66) into 64-bit (tile<<32 | depth) keys with
+ * surfel-id payloads, entries in surfel order, row-major within each rect.
+ * tile_base is added to every tile id (multi-view batches). Returns E. */
+int64_t oref_emit_keys(int64_t n, const uint32_t* rects, const uint32_t* depth_keys,
+                       uint32_t tiles_x, uint32_t tile_base, uint64_t* keys,
+                       uint32_t* payloads, int64_t max_entries);
+/* Stable sort of (key, payload) by key. */
+void oref_stable_sort_u64(int64_t n, uint64_t* keys, uint32_t* payloads);
+/* tile_binning.wgsl:89-103 over sorted 64-bit keys: ranges[2*t] = start,
+ * ranges[2*t+1] = end, tile t = key >> 32 (minus tile_base); empty tiles 0,0. */
+void oref_tile_ranges(int64_t e, const uint64_t* keys, uint32_t num_tiles,
+                      uint32_t tile_base, uint32_t* ranges);
+
+/* ---- ssim.wgsl / ssim.rs ---- */
+/* img/target are P*3 (rgb). dl_dimg P*3 (may be NULL). Returns the loss
+ * (1-lambda)*mean|x-y| + lambda*dssim_scale*mean(1-SSIM); l1_mean and
+ * ssim_mean are optional outputs. */
+double oref_l1_ssim(int w, int h, const double* img, const double* target, double lambda,
+                    double dssim_scale, double* dl_dimg, double* l1_mean, double* ssim_mean);
+/* Zero-padded separable 11-tap blur of one P-sized channel map. */
+void oref_blur(int w, int h, const double* in, double* out);
+
+/* ---- adam.wgsl / optim.rs ---- */
+/* One f32 Adam step (t is the 1-based step index) with activation mode
+ * 0 = identity, 1 = exp, 2 = sigmoid. Updates raw/m/v in place and writes
+ * activated (may be NULL). */
+void oref_adam_step_f32(int64_t n, float* raw, const float* grad_act, float* m, float* v,
+                        float* activated, float lr, float reg, int activation, int t);
+
+/* ---- timing helper for the CPU baseline ---- */
+int oref_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
