@@ -1,0 +1,303 @@
+/*
+ * gs_hip.h -- C ABI of the MI355X (gfx950) differentiable 2D-Gaussian-surfel
+ * training step.
+ *
+ * Drop-in boundary for splattar's GPU op layer: every entry point below
+ * replaces one method of the reference Rust API (crates/gs-wgpu,
+ * crates/gs-kernels, first_attempt/crates/gs-train), cited per declaration.
+ * The reference records deferred work on a wgpu CommandEncoder; here an
+ * encode call enqueues asynchronously on a HIP stream (passed as void*, a
+ * hipStream_t; NULL = the context's stream) and is capturable into a
+ * hipGraph. Device buffers are exposed as raw pointers exactly where the
+ * reference exposes wgpu::Buffer fields; the caller wires objects together
+ * by passing one object's pointer to another, as in the reference.
+ *
+ * Errors: every call returns a gs_status; no abort crosses the ABI. The
+ * reference panics on misuse (assert!) -- here that is GS_ERR_INVALID /
+ * GS_ERR_CAPACITY with a message in gs_last_error() (thread-local).
+ * Threading: a handle is not thread-safe; distinct handles on distinct
+ * streams are independent. All device buffers stay allocated at capacity
+ * for the handle's lifetime (no reallocation mid-training).
+ */
+#ifndef GS_HIP_H
+#define GS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GS_ABI_VERSION 1
+
+typedef enum gs_status {
+    GS_OK = 0,
+    GS_ERR_INVALID = 1,   /* bad argument / misuse (reference: assert! panic) */
+    GS_ERR_CAPACITY = 2,  /* request exceeds a capacity fixed at creation */
+    GS_ERR_OOM = 3,       /* device allocation failed */
+    GS_ERR_HIP = 4,       /* HIP runtime error (message names the call) */
+    GS_ERR_OVERFLOW = 5,  /* tile entries exceeded max_entries (reference: unchecked) */
+    GS_ERR_NO_DEVICE = 6  /* no HIP device (reference: GpuError::NoAdapter, gs-wgpu/src/lib.rs:19-27) */
+} gs_status;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* gs_last_error(void);
+/* ABI version (GS_ABI_VERSION) and build string. */
+int gs_abi_version(void);
+const char* gs_build_info(void);
+
+/* ------------------------------------------------------------------ context
+ * GpuContext (crates/gs-wgpu/src/context.rs:7-96): one device, one queue. */
+typedef struct gs_context* gs_ctx;
+gs_status gs_ctx_create(int device_index, gs_ctx* out);
+void gs_ctx_destroy(gs_ctx ctx);
+/* The context's stream (hipStream_t): the reference's single queue. */
+void* gs_ctx_stream(gs_ctx ctx);
+gs_status gs_ctx_synchronize(gs_ctx ctx);
+/* Device name and compute-unit count (for logs / the autotuner key). */
+gs_status gs_ctx_device_info(gs_ctx ctx, char* name, size_t name_len, int* compute_units);
+
+/* ------------------------------------------------------------------ buffers
+ * crates/gs-wgpu/src/buffers.rs: storage_empty / queue.write_buffer /
+ * readback / encoder.clear_buffer. */
+gs_status gs_buffer_alloc(gs_ctx ctx, size_t bytes, void** out); /* zero-filled */
+gs_status gs_buffer_free(gs_ctx ctx, void* ptr);
+/* queue.write_buffer: host -> device, ordered on the context stream. */
+gs_status gs_write_buffer(gs_ctx ctx, void* dst, const void* src, size_t bytes);
+/* buffers::readback (buffers.rs:24-51): blocking device -> host. */
+gs_status gs_read_buffer(gs_ctx ctx, void* dst, const void* src, size_t bytes);
+/* encoder.clear_buffer: zero `bytes` (multiple of 4) with a kernel
+ * (graph-capturable; no memset node is recorded). */
+gs_status gs_clear_buffer(gs_ctx ctx, void* stream, void* dst, size_t bytes);
+
+/* ---------------------------------------------------------------- PrefixSum
+ * crates/gs-wgpu/src/prefix_sum.rs:19-164: exclusive u32 scan in place,
+ * wrapping mod 2^32, grand total into a 1-element device buffer. */
+typedef struct gs_scan_s* gs_scan;
+gs_status gs_scan_create(gs_ctx ctx, uint32_t capacity, gs_scan* out); /* PrefixSum::new */
+void gs_scan_destroy(gs_scan scan);
+uint32_t* gs_scan_data(gs_scan scan);   /* PrefixSum::data  (prefix_sum.rs) */
+uint32_t* gs_scan_total(gs_scan scan);  /* PrefixSum::total */
+/* PrefixSum::encode(ctx, encoder, n) (prefix_sum.rs:134-164); n <= capacity. */
+gs_status gs_scan_encode(gs_scan scan, void* stream, uint32_t n);
+
+/* -------------------------------------------------------------- RadixSorter
+ * crates/gs-wgpu/src/sort.rs:17-327: stable LSD sort of the first counts[0]
+ * (key, u32 payload) pairs in place; the count lives in device memory.
+ * Keys are 32- or 64-bit (key_bits); the reference sorts 32-bit keys. */
+typedef struct gs_sorter_s* gs_sorter;
+gs_status gs_sorter_create(gs_ctx ctx, uint32_t capacity, uint32_t key_bits, gs_sorter* out); /* RadixSorter::new */
+void gs_sorter_destroy(gs_sorter s);
+void* gs_sorter_keys(gs_sorter s);          /* RadixSorter::keys     (sort.rs:101-104) */
+uint32_t* gs_sorter_payloads(gs_sorter s);  /* RadixSorter::payloads (sort.rs:106-109) */
+uint32_t* gs_sorter_counts(gs_sorter s);    /* RadixSorter::counts   (sort.rs:111-114) */
+/* RadixSorter::encode (sort.rs:117-119): sorts key bits [begin_bit,
+ * end_bit) with 4- or 8-bit digits; the result is back in keys()/payloads(). */
+gs_status gs_sorter_encode(gs_sorter s, void* stream, uint32_t begin_bit, uint32_t end_bit,
+                           uint32_t digit_bits);
+
+/* --------------------------------------------------------------- TileBinner
+ * crates/gs-kernels/src/binning.rs:22-238. Items carry an inclusive
+ * tile-space rect (x1 < x0 or y1 < y0 = culled) and a u32 depth key; encode
+ * counts, scans, expands into 64-bit (tile << 32 | depth) keys with item-id
+ * payloads, sorts stably and extracts per-tile [start, end) ranges. The
+ * reference's payload is an entry index mapped through entry_items; here the
+ * payload is the item id itself (same order: an item's entries lie in
+ * distinct tiles). */
+typedef struct gs_binner_s* gs_binner;
+gs_status gs_binner_create(gs_ctx ctx, uint32_t max_items, uint32_t max_entries, uint32_t num_tiles,
+                           gs_binner* out);                 /* TileBinner::new (binning.rs:46) */
+void gs_binner_destroy(gs_binner b);
+uint32_t* gs_binner_rects(gs_binner b);           /* pub rects  (binning.rs:27), 4 u32 per item */
+uint32_t* gs_binner_depths(gs_binner b);          /* pub depths (binning.rs:28) */
+uint32_t* gs_binner_ranges(gs_binner b);          /* pub ranges (binning.rs:32), 2 u32 per tile */
+uint32_t* gs_binner_sorted_entries(gs_binner b);  /* sorted_entries() (binning.rs:148) */
+uint64_t* gs_binner_sorted_keys(gs_binner b);     /* sorted 64-bit keys (no reference counterpart) */
+uint32_t* gs_binner_entry_count(gs_binner b);     /* entry_count() (binning.rs:153) */
+/* TileBinner::encode (binning.rs:159-238); digit_bits 4 or 8. */
+gs_status gs_binner_encode(gs_binner b, void* stream, uint32_t num_items, uint32_t tiles_x,
+                           uint32_t digit_bits);
+/* Blocking: *overflowed = 1 if any encode since the last call produced more
+ * entries than max_entries (entries beyond it were dropped, memory-safe). */
+gs_status gs_binner_overflowed(gs_binner b, int* overflowed);
+
+/* --------------------------------------------------------------- Rasterizer
+ * crates/gs-kernels/src/rasterizer.rs:34-448. */
+typedef struct gs_raster_s* gs_raster;
+
+/* RasterCamera (rasterizer.rs:34-41): camera-to-world quat (xyzw, normalized
+ * on use), centre, focal in pixels, active SH degree. */
+typedef struct gs_raster_camera {
+    float center[3];
+    float quat[4];
+    float focal;
+    uint32_t sh_degree;
+} gs_raster_camera;
+
+/* SceneInput (rasterizer.rs:43-54): ACTIVATED parameters, host memory.
+ * positions n*3, scales n*2 (world units), quats n*4 (xyzw, unnormalized),
+ * opacities n (0..1), sh n*sh_coeffs*3 coefficient-major rgb (padded to 48
+ * floats per surfel on upload). */
+typedef struct gs_scene_input {
+    const float* positions;
+    const float* scales;
+    const float* quats;
+    const float* opacities;
+    const float* sh;
+    uint32_t sh_coeffs;
+    uint32_t n;
+} gs_scene_input;
+
+/* The public device buffers of Rasterizer (rasterizer.rs:56-98). P = W*H. */
+typedef struct gs_raster_buffers {
+    float* positions;      /* capacity * 4 (xyz, w = 1) */
+    float* scales;         /* capacity * 2 */
+    float* quats;          /* capacity * 4 */
+    float* opacities;      /* capacity */
+    float* sh;             /* capacity * 48 */
+    float* out_color;      /* P * 4: composited rgb, accumulated alpha */
+    float* out_aux;        /* P * 4: final transmittance, alpha-weighted depth, 0, 0 */
+    float* out_normal;     /* P * 4: alpha-weighted normal, 0 */
+    uint32_t* out_ncontrib;/* P: tile-list index just past the last contributor */
+    float* dl_dcolor;      /* P * 4: dL/d(rgb), w = dL/d(depth image) -- backward input */
+    float* dl_dnormal;     /* P * 4: dL/d(normal image) -- backward input */
+    float* grad_pos;       /* capacity * 4 */
+    float* grad_scales;    /* capacity * 2 */
+    float* grad_quat;      /* capacity * 4 */
+    float* grad_opacity;   /* capacity */
+    float* grad_sh;        /* capacity * 48 */
+    float* grad_cam;       /* 16: dL/dR (9, column-major), dL/dC (3), dL/df, pad 3 */
+} gs_raster_buffers;
+
+/* Rasterizer::new (rasterizer.rs:108). */
+gs_status gs_raster_create(gs_ctx ctx, uint32_t capacity, uint32_t width, uint32_t height,
+                           uint32_t max_entries, gs_raster* out);
+void gs_raster_destroy(gs_raster r);
+gs_status gs_raster_get_buffers(gs_raster r, gs_raster_buffers* out);
+/* Rasterizer::upload_scene (rasterizer.rs:359-386). */
+gs_status gs_raster_upload_scene(gs_raster r, const gs_scene_input* scene);
+/* pub lambda_dist (rasterizer.rs:60): depth-distortion weight for backward. */
+gs_status gs_raster_set_lambda_dist(gs_raster r, float lambda_dist);
+/* Radix digit width for the binning sort (4 or 8; autotuned by default). */
+gs_status gs_raster_set_digit_bits(gs_raster r, uint32_t digit_bits);
+/* Rasterizer::forward (rasterizer.rs:389-448): project, bin, sort, ranges,
+ * composite. */
+gs_status gs_raster_forward(gs_raster r, void* stream, const gs_raster_camera* camera,
+                            uint32_t num_surfels);
+/* Rasterizer::backward (rasterizer.rs:320-356): clears the gradient buffers
+ * and differentiates colour, depth (dl_dcolor.w), normal and distortion
+ * losses into grad_* and grad_cam. */
+gs_status gs_raster_backward(gs_raster r, void* stream, uint32_t num_surfels);
+#define GS_BWD_ACCUMULATE 1u  /* add into grad_* instead of overwriting */
+#define GS_BWD_COLOR_ONLY 2u  /* only dL/d(rgb) is non-zero (faster kernel) */
+#define GS_BWD_NO_CAMERA 4u   /* skip grad_cam */
+gs_status gs_raster_backward_ex(gs_raster r, void* stream, uint32_t num_surfels, uint32_t flags);
+/* Blocking: entries produced by the last forward; GS_ERR_OVERFLOW if it
+ * exceeded max_entries (the reference does not check, binning.rs:186). */
+gs_status gs_raster_entry_count(gs_raster r, uint32_t* entries);
+
+/* Binning internals of the last forward, for parity tests and tooling. */
+typedef struct gs_raster_debug {
+    const uint64_t* sorted_keys;    /* entry_count: (tile << 32) | f32 depth bits */
+    const uint32_t* sorted_payloads;/* entry_count: surfel ids */
+    const uint32_t* ranges;         /* tiles * 2 */
+    const uint32_t* rects;          /* capacity * 4 */
+    const uint32_t* depth_keys;     /* capacity */
+    const uint32_t* offsets;        /* capacity: exclusive scan of tile counts */
+    uint32_t num_tiles;
+} gs_raster_debug;
+gs_status gs_raster_debug_buffers(gs_raster r, gs_raster_debug* out);
+
+/* ----------------------------------------------------------------- SsimLoss
+ * first_attempt/crates/gs-train/src/ssim.rs:25-298: fused L1 + D-SSIM,
+ * writes dL/d(image) rgb (w = 0) into dl_out. Loss =
+ * (1-lambda) mean|x-y| + lambda * dssim_scale * mean(1 - SSIM); the
+ * reference's dssim_scale is 0.5 (ssim.rs:4), the default here too. */
+typedef struct gs_loss_s* gs_loss;
+/* SsimLoss::new(ctx, w, h, lambda, img, dl_out) (ssim.rs:63-70). */
+gs_status gs_loss_create(gs_ctx ctx, uint32_t width, uint32_t height, float lambda,
+                         const float* img, float* dl_out, gs_loss* out);
+void gs_loss_destroy(gs_loss l);
+float* gs_loss_target(gs_loss l);   /* pub target (ssim.rs:31): P * 4, rgb used */
+gs_status gs_loss_set_dssim_scale(gs_loss l, float scale);
+gs_status gs_loss_encode(gs_loss l, void* stream);   /* SsimLoss::encode (ssim.rs:231) */
+/* SsimLoss::read_losses (ssim.rs:282-298): blocking (l1_mean, dssim_mean),
+ * dssim_mean = dssim_scale * mean(1 - SSIM). */
+gs_status gs_loss_read(gs_loss l, double* l1_mean, double* dssim_mean);
+
+/* ---------------------------------------------------------------- Optimizer
+ * first_attempt/crates/gs-train/src/optim.rs:50-241. */
+typedef struct gs_optim_s* gs_optim;
+#define GS_ACT_IDENTITY 0
+#define GS_ACT_EXP 1      /* raw = log(activated): scales */
+#define GS_ACT_SIGMOID 2  /* raw = logit(activated): opacity */
+gs_status gs_optim_create(gs_ctx ctx, gs_optim* out);   /* Optimizer::new (optim.rs:58) */
+void gs_optim_destroy(gs_optim o);
+/* Optimizer::add_class (optim.rs:87-141): grads = dL/d(activated). */
+gs_status gs_optim_add_class(gs_optim o, const char* name, uint32_t n, int activation, float lr,
+                             const float* grads, float* activated);
+float* gs_optim_class_raw(gs_optim o, const char* name);  /* class(name).raw (optim.rs:41) */
+gs_status gs_optim_set_lr(gs_optim o, const char* name, float lr);    /* optim.rs:171 */
+gs_status gs_optim_set_reg(gs_optim o, const char* name, float reg);  /* optim.rs:175 */
+/* Optimizer::encode_step (optim.rs:200-227): one Adam step on every class,
+ * then the activated copies. One fused launch. */
+gs_status gs_optim_encode_step(gs_optim o, void* stream);
+gs_status gs_optim_encode_activate(gs_optim o, void* stream);  /* optim.rs:230-241 */
+gs_status gs_optim_step_count(gs_optim o, uint32_t* t);        /* blocking */
+
+/* ------------------------------------------------------------------ Trainer
+ * Trainer::step (first_attempt/crates/gs-train/src/trainer.rs:1139-1332):
+ * render -> L1 + D-SSIM -> backward -> Adam with raw -> activated chains.
+ * A step here covers a set of views whose losses are summed (each view's
+ * loss normalised by its own pixel count, ssim.rs:84) before ONE Adam step.
+ *   GS_STEP_PLAIN:   every kernel launched per view (the reference's shape);
+ *   GS_STEP_BATCHED: every stage launched once for all views of the step;
+ *   GS_STEP_GRAPH:   GS_STEP_BATCHED captured once into a hipGraph and
+ *                    replayed (view descriptors live in device memory). */
+typedef struct gs_trainer_s* gs_trainer;
+
+typedef struct gs_train_view {
+    uint32_t width, height;
+    gs_raster_camera camera;
+    const float* target_rgba;   /* host, width*height*4 (rgb used) */
+} gs_train_view;
+
+typedef struct gs_train_config {
+    float lambda;               /* D-SSIM weight (trainer.rs:470: 0.2) */
+    float dssim_scale;          /* 0.5 = reference (1-SSIM)/2, 1.0 = (1-SSIM) */
+    float lr_pos, lr_scales, lr_quat, lr_opacity, lr_sh;   /* trainer.rs:471-476 */
+    uint32_t max_views_per_step;
+    uint32_t entries_per_view;  /* binning capacity per view (0 = 64 * n) */
+    uint32_t digit_bits;        /* 0 = autotune, else 4 or 8 */
+} gs_train_config;
+
+#define GS_STEP_PLAIN 0u
+#define GS_STEP_BATCHED 1u
+#define GS_STEP_GRAPH 2u
+
+/* Trainer::new (trainer.rs:722-792): `init` holds ACTIVATED values; raw
+ * parameters are log(scales), logit(opacity), identity otherwise. Targets
+ * are uploaded once and stay resident. */
+gs_status gs_trainer_create(gs_ctx ctx, const gs_train_config* cfg, const gs_scene_input* init,
+                            uint32_t n_views, const gs_train_view* views, gs_trainer* out);
+void gs_trainer_destroy(gs_trainer t);
+gs_status gs_trainer_step(gs_trainer t, void* stream, const uint32_t* view_ids, uint32_t n_ids,
+                          uint32_t mode);
+/* Blocking: per-view (l1_mean, dssim_mean) of the last step, 2 per view. */
+gs_status gs_trainer_last_losses(gs_trainer t, double* out, uint32_t n_ids);
+/* Blocking: render view `view` with the current parameters into rgba (W*H*4). */
+gs_status gs_trainer_render(gs_trainer t, uint32_t view, float* rgba);
+/* Activated parameters (positions n*4, scales n*2, quats n*4, opacities n,
+ * sh n*48) and parameter gradients of the last step. */
+gs_status gs_trainer_params(gs_trainer t, gs_raster_buffers* out);
+/* Blocking: entries of the last step (sum over its views); GS_ERR_OVERFLOW
+ * if any view exceeded its binning capacity. */
+gs_status gs_trainer_entry_count(gs_trainer t, uint32_t* entries);
+/* Selected radix digit width after autotuning (4 or 8). */
+gs_status gs_trainer_digit_bits(gs_trainer t, uint32_t* bits);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GS_HIP_H */

@@ -88,6 +88,7 @@ def lib():
         L.oref_adam_step_f32.argtypes = [C.c_int64, _fp, _fp, _fp, _fp, _fp, C.c_float, C.c_float,
                                          C.c_int, C.c_int]
         L.oref_num_threads.restype = C.c_int
+        L.oref_xorshift32_fill.argtypes = [C.c_uint32, C.c_int64, C.c_uint32, _u32p]
         _lib = L
     return _lib
 
@@ -355,6 +356,13 @@ def adam_step_f32(raw, grad_act, m, v, lr: float, activation: int, t: int, reg: 
     lib().oref_adam_step_f32(raw.shape[0], _ptr(raw, _fp), _ptr(grad_act, _fp), _ptr(m, _fp),
                              _ptr(v, _fp), _ptr(act, _fp), lr, reg, activation, t)
     return act
+
+
+def xorshift32(seed: int, n: int, modulus: int = 0) -> np.ndarray:
+    """xorshift32 stream of crates/gs-wgpu/tests/sort.rs:18-25 (mod modulus)."""
+    out = np.empty(n, dtype=np.uint32)
+    lib().oref_xorshift32_fill(seed & 0xFFFFFFFF, n, modulus, _ptr(out, _u32p))
+    return out
 
 
 def num_threads() -> int:

@@ -1060,6 +1060,16 @@ void oref_adam_step_f32(int64_t n, float* raw, const float* grad_act, float* m, 
     }
 }
 
+void oref_xorshift32_fill(uint32_t seed, int64_t n, uint32_t modulus, uint32_t* out) {
+    uint32_t x = seed;
+    for (int64_t i = 0; i < n; ++i) {
+        x ^= x << 13;
+        x ^= x >> 17;
+        x ^= x << 5;
+        out[i] = modulus ? x % modulus : x;
+    }
+}
+
 int oref_num_threads(void) { return threads_available(); }
 
 }  // extern "C"

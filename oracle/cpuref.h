@@ -144,6 +144,11 @@ void oref_blur(int w, int h, const double* in, double* out);
 void oref_adam_step_f32(int64_t n, float* raw, const float* grad_act, float* m, float* v,
                         float* activated, float lr, float reg, int activation, int t);
 
+/* ---- test-input generators ---- */
+/* xorshift32 (shifts 13/17/5) stream of crates/gs-wgpu/tests/sort.rs:18-25;
+ * out[i] = (i-th draw) % modulus (modulus 0 = full range). */
+void oref_xorshift32_fill(uint32_t seed, int64_t n, uint32_t modulus, uint32_t* out);
+
 /* ---- timing helper for the CPU baseline ---- */
 int oref_num_threads(void);
 

@@ -1,0 +1,164 @@
+// Tile binning: key emission (A5) and tile ranges (A7).
+//
+// emit_keys restates `expand` of crates/gs-kernels/src/shaders/
+// tile_binning.wgsl:46-66, with one 64-bit key per entry instead of the
+// reference's separate 32-bit tile and depth keys (two stable 32-bit sorts,
+// binning.rs:5-8): key = (global_tile << 32) | f32_bits(depth), payload =
+// global record index (view * N + surfel). Entries are written in record
+// order and row-major within each rect, so a stable sort reproduces the
+// oracle's (depth bits, surfel index) tie-break (forward.rs:95). Unlike the
+// reference it bounds-checks against max_entries and raises an overflow flag.
+//
+// tile_ranges restates tile_binning.wgsl:89-103 over the sorted 64-bit keys.
+
+#include "gs_internal.h"
+
+namespace gs {
+
+namespace {
+
+// One wave per item-group: each lane owns one item; the wave then emits the
+// union of its lanes' entries cooperatively so that stores of consecutive
+// entries come from consecutive lanes (coalesced 8-B key and 4-B payload
+// writes) instead of each lane streaming its own run.
+__global__ __launch_bounds__(256) void emit_keys_kernel(
+    uint32_t n_items, const uint4* __restrict__ rects, const uint32_t* __restrict__ depth_keys,
+    const uint32_t* __restrict__ offsets, const CamParams* __restrict__ cams, uint32_t items_per_view,
+    uint64_t* __restrict__ keys, uint32_t* __restrict__ payloads, uint32_t max_entries,
+    const uint32_t* __restrict__ total, uint32_t* __restrict__ sort_count,
+    uint32_t* __restrict__ overflow_flag) {
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid == 0) {
+        const uint32_t t = *total;
+        *sort_count = t < max_entries ? t : max_entries;
+        if (t > max_entries) atomicOr(overflow_flag, 1u);
+    }
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave_first = gid - (uint32_t)lane;
+    if (wave_first >= n_items) return;
+
+    uint32_t cnt = 0, base = 0, w = 0, x0 = 0, y0 = 0, tiles_x = 0, tile_base = 0, dk = 0;
+    if (gid < n_items) {
+        const uint4 r = rects[gid];
+        if (r.z >= r.x && r.w >= r.y) {
+            w = r.z - r.x + 1u;
+            cnt = w * (r.w - r.y + 1u);
+            x0 = r.x;
+            y0 = r.y;
+            base = offsets[gid];
+            dk = depth_keys[gid];
+            const CamParams& cam = cams[gid / items_per_view];
+            tiles_x = cam.tiles_x;
+            tile_base = cam.tile_base;
+        }
+    }
+    // Inclusive prefix of counts across the wave (ordering within the wave
+    // follows item order, matching the scan's offsets).
+    uint32_t incl = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t v = __shfl_up(incl, o);
+        if (lane >= o) incl += v;
+    }
+    const uint32_t wave_total = __shfl(incl, 63);
+    const uint32_t excl = incl - cnt;
+    // Cooperative emission: entry k of the wave's run belongs to the lane
+    // whose [excl, incl) interval holds k.
+    for (uint32_t k0 = 0; k0 < wave_total; k0 += 64) {
+        const uint32_t k = k0 + (uint32_t)lane;
+        // Find the owner lane: largest lane with excl <= k (binary search over
+        // the wave via shuffles).
+        int owner = 0;
+        for (int step = 32; step >= 1; step >>= 1) {
+            const int cand = owner + step;
+            const uint32_t ex = __shfl(excl, cand & 63);
+            if (cand < 64 && ex <= k) owner = cand;
+        }
+        const uint32_t o_excl = __shfl(excl, owner);
+        const uint32_t o_cnt = __shfl(cnt, owner);
+        const uint32_t o_base = __shfl(base, owner);
+        const uint32_t o_w = __shfl(w, owner);
+        const uint32_t o_x0 = __shfl(x0, owner);
+        const uint32_t o_y0 = __shfl(y0, owner);
+        const uint32_t o_tx = __shfl(tiles_x, owner);
+        const uint32_t o_tb = __shfl(tile_base, owner);
+        const uint32_t o_dk = __shfl(dk, owner);
+        if (k < wave_total && o_cnt > 0) {
+            const uint32_t local = k - o_excl;
+            const uint32_t e = o_base + local;
+            if (e < max_entries) {
+                const uint32_t ty = o_y0 + local / o_w;
+                const uint32_t tx = o_x0 + local % o_w;
+                const uint64_t tile = (uint64_t)(o_tb + ty * o_tx + tx);
+                keys[e] = (tile << 32) | (uint64_t)o_dk;
+                payloads[e] = wave_first + (uint32_t)owner;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void tile_ranges_kernel(const uint64_t* __restrict__ keys,
+                                                          const uint32_t* __restrict__ count,
+                                                          uint32_t* __restrict__ ranges) {
+    const uint32_t n = *count;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t tile = (uint32_t)(keys[i] >> 32);
+        if (i == 0 || (uint32_t)(keys[i - 1] >> 32) != tile) ranges[2 * (size_t)tile] = i;
+        if (i == n - 1 || (uint32_t)(keys[i + 1] >> 32) != tile) ranges[2 * (size_t)tile + 1] = i + 1;
+    }
+}
+
+// count_tiles of tile_binning.wgsl:30-43 (standalone binner; the raster path
+// fuses it into project).
+__global__ __launch_bounds__(256) void count_tiles_kernel(uint32_t n, const uint4* __restrict__ rects,
+                                                          uint32_t* __restrict__ counts) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint4 r = rects[i];
+    counts[i] = (r.z >= r.x && r.w >= r.y) ? (r.z - r.x + 1u) * (r.w - r.y + 1u) : 0u;
+}
+
+__global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t* __restrict__ dst, uint32_t value,
+                                                       size_t count) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = value;
+}
+
+}  // namespace
+
+hipError_t launch_emit_keys(hipStream_t st, uint32_t n_items, const uint4* rects,
+                            const uint32_t* depth_keys, const uint32_t* offsets,
+                            const CamParams* cams_dev, uint32_t items_per_view,
+                            uint32_t /*tiles_x_fixed*/, uint64_t* keys, uint32_t* payloads,
+                            uint32_t max_entries, const uint32_t* total, uint32_t* sort_count,
+                            uint32_t* overflow_flag) {
+    uint32_t blocks = div_up(n_items > 0 ? n_items : 1, 256);
+    hipLaunchKernelGGL(emit_keys_kernel, dim3(blocks), dim3(256), 0, st, n_items, rects, depth_keys,
+                       offsets, cams_dev, items_per_view > 0 ? items_per_view : 1, keys, payloads,
+                       max_entries, total, sort_count, overflow_flag);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_ranges(hipStream_t st, const uint64_t* keys, const uint32_t* count,
+                              uint32_t capacity, uint32_t* ranges) {
+    uint32_t blocks = div_up(capacity > 0 ? capacity : 1, 256);
+    if (blocks > 8192) blocks = 8192;  // grid-stride beyond this
+    hipLaunchKernelGGL(tile_ranges_kernel, dim3(blocks), dim3(256), 0, st, keys, count, ranges);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_tiles(hipStream_t st, uint32_t n, const uint4* rects, uint32_t* counts) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(count_tiles_kernel, dim3(div_up(n, 256)), dim3(256), 0, st, n, rects, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(hipStream_t st, uint32_t* dst, uint32_t value, size_t count) {
+    if (count == 0) return hipSuccess;
+    uint64_t blocks = (count + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((uint32_t)blocks), dim3(256), 0, st, dst, value, count);
+    return hipGetLastError();
+}
+
+}  // namespace gs

@@ -1,0 +1,312 @@
+// Geometry backward (A11): one thread per surfel, all views of a batch.
+//
+// Restates crates/gs-kernels/src/shaders/surfel_prep_bwd.wgsl:129-284 (the
+// oracle's backward.rs:176-245 and quat_grad of math.rs:38-88). The
+// rasterizer backward hands over gradients w.r.t. the record coefficients
+// (A = tv x c, B = c x tu, M = tu x tv, Nt = -M.c, px, py, depth, colour,
+// opacity, normal); this kernel first maps them to the camera-space
+// (dc, dtu, dtv) of the reference by the transposed cross-product rules, then
+// chains to position, scales, quaternion, opacity and SH exactly as the
+// reference does. Gradients of every view in the batch are summed per surfel
+// before one write, so the parameters are read once per step, not per view.
+// With cam_partials set, the per-view camera gradients [dL/dR (9, column
+// major), dL/dC (3), dL/df, 0, 0, 0] are reduced per block into partials that
+// cam_reduce folds (no contended atomics on the 16 hot addresses).
+
+#include "gs_internal.h"
+
+namespace gs {
+
+namespace {
+
+constexpr int kGeomThreads = 256;
+
+struct V3f {
+    float x, y, z;
+};
+__device__ __forceinline__ V3f v3(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ V3f operator+(V3f a, V3f b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3f operator-(V3f a, V3f b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3f operator*(V3f a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V3f crs(V3f a, V3f b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float dt(V3f a, V3f b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// quat_grad (math.rs:38-88 / surfel_prep_bwd.wgsl:75-112) for dL/dR with only
+// columns 0 and 1 non-zero (the tangent axes).
+__device__ __forceinline__ void quat_grad_cols01(float qx, float qy, float qz, float qw, float n,
+                                                 V3f c0, V3f c1, float out[4]) {
+    const float x = qx, y = qy, z = qz, w = qw;
+    // <dL/dR, dR/dq_k> over columns 0 and 1.
+    const float g0 = dt(c0, v3(0.f, 2.f * y, 2.f * z)) + dt(c1, v3(2.f * y, -4.f * x, 2.f * w));
+    const float g1 = dt(c0, v3(-4.f * y, 2.f * x, -2.f * w)) + dt(c1, v3(2.f * x, 0.f, 2.f * z));
+    const float g2 = dt(c0, v3(-4.f * z, 2.f * w, 2.f * x)) + dt(c1, v3(-2.f * w, -4.f * z, 2.f * y));
+    const float g3 = dt(c0, v3(0.f, 2.f * z, -2.f * y)) + dt(c1, v3(-2.f * z, 0.f, 2.f * x));
+    const float d = g0 * x + g1 * y + g2 * z + g3 * w;
+    const float inv_n = 1.0f / n;
+    out[0] = (g0 - x * d) * inv_n;
+    out[1] = (g1 - y * d) * inv_n;
+    out[2] = (g2 - z * d) * inv_n;
+    out[3] = (g3 - w * d) * inv_n;
+}
+
+__global__ __launch_bounds__(kGeomThreads) void geom_bwd_kernel(
+    const CamParams* __restrict__ cams, uint32_t n_views, uint32_t n, const float4* __restrict__ pos,
+    const float2* __restrict__ scales, const float4* __restrict__ quats, const float* __restrict__ sh,
+    const Record* __restrict__ recs, const float* __restrict__ grad_coef, ParamGrads g,
+    int accumulate, float* __restrict__ cam_partials) {
+    __shared__ float s_cam[kGeomThreads / 64][16];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const bool valid = i < n;
+
+    float4 p = make_float4(0.f, 0.f, 0.f, 0.f), q = make_float4(0.f, 0.f, 0.f, 1.f);
+    float2 s = make_float2(0.f, 0.f);
+    if (valid) {
+        p = pos[i];
+        s = scales[i];
+        q = quats[i];
+    }
+    float acc_pos[3] = {0.f, 0.f, 0.f}, acc_sc[2] = {0.f, 0.f}, acc_q[4] = {0.f, 0.f, 0.f, 0.f};
+    float acc_op = 0.f;
+    float acc_sh[kShFloats];
+#pragma unroll
+    for (int k = 0; k < kShFloats; ++k) acc_sh[k] = 0.f;
+
+    for (uint32_t v = 0; v < n_views; ++v) {
+        const CamParams& cam = cams[v];
+        float cg[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) cg[k] = 0.f;
+        bool any = false;
+        float gc[kGradStride];
+        if (valid) {
+            const float4* src = reinterpret_cast<const float4*>(grad_coef + ((size_t)v * n + i) * kGradStride);
+#pragma unroll
+            for (int k = 0; k < kGradStride / 4; ++k) {
+                const float4 t = src[k];
+                gc[4 * k + 0] = t.x;
+                gc[4 * k + 1] = t.y;
+                gc[4 * k + 2] = t.z;
+                gc[4 * k + 3] = t.w;
+            }
+#pragma unroll
+            for (int k = 0; k < kGradStride; ++k) any |= gc[k] != 0.0f;
+        }
+        if (any) {
+            const CamFrame f = cam_frame(cam, p, s, q);
+            const V3f c = v3(f.cx_, f.cy_, f.cz_);
+            const V3f tu = v3(f.tux, f.tuy, f.tuz);
+            const V3f tv = v3(f.tvx, f.tvy, f.tvz);
+            const V3f A = crs(tv, c), B = crs(c, tu), M = crs(tu, tv);
+            const V3f gA = v3(gc[kGA], gc[kGA + 1], gc[kGA + 2]);
+            const V3f gB = v3(gc[kGB], gc[kGB + 1], gc[kGB + 2]);
+            const V3f gMs = v3(gc[kGM], gc[kGM + 1], gc[kGM + 2]);
+            const float gNt = gc[kGNt];
+            const float w = -c.z;
+            const float fo = cam.focal;
+            // A = tv x c, B = c x tu, M = tu x tv, Nt = -M.c (transposed rules).
+            const V3f gM = gMs - c * gNt;
+            V3f dc = crs(gA, tv) + crs(tu, gB) - M * gNt;
+            V3f dtu = crs(gB, c) + crs(tv, gM);
+            V3f dtv = crs(c, gA) + crs(gM, tu);
+            // px = cx + f c.x / w, py = cy - f c.y / w, w = -c.z (backward.rs:329-336).
+            const float gpx = gc[kGPx], gpy = gc[kGPy];
+            dc.x += gpx * fo / w;
+            dc.z += gpx * fo * c.x / (w * w);
+            dc.y += -gpy * fo / w;
+            dc.z += -gpy * fo * c.y / (w * w);
+            dc.z -= gc[kGDep];  // screen-branch t = centre depth = -c.z
+            const float dfocal = -(A.x * gA.x + A.y * gA.y + B.x * gB.x + B.y * gB.y + M.x * gMs.x +
+                                   M.y * gMs.y) / fo +
+                                 gpx * c.x / w - gpy * c.y / w;
+            // Normal chain (backward.rs:186-196).
+            const V3f gn = v3(gc[kGNrm], gc[kGNrm + 1], gc[kGNrm + 2]);
+            if (gn.x != 0.f || gn.y != 0.f || gn.z != 0.f) {
+                const float len = sqrtf(dt(M, M));
+                if (len > 1e-12f) {
+                    const V3f mh = M * (1.0f / len);
+                    const float sign = dt(mh, c) > 0.f ? -1.f : 1.f;
+                    const V3f dm = (gn - mh * dt(mh, gn)) * (sign / len);
+                    dtu = dtu + crs(tv, dm);
+                    dtv = dtv + crs(dm, tu);
+                }
+            }
+            // c = R^T (p - C): dpos += R dc (backward.rs:198-203).
+            const float* R = cam.rot;
+            auto rmul = [&](V3f a) {
+                return v3(R[0] * a.x + R[3] * a.y + R[6] * a.z, R[1] * a.x + R[4] * a.y + R[7] * a.z,
+                          R[2] * a.x + R[5] * a.y + R[8] * a.z);
+            };
+            V3f dpos = rmul(dc);
+            // tangent axes: tau = R^T (s * Rq e_axis) (backward.rs:206-219).
+            const float x = f.qx, y = f.qy, z = f.qz, ww = f.qw;
+            const V3f ax_u = v3(1.f - 2.f * (y * y + z * z), 2.f * (x * y + ww * z), 2.f * (x * z - ww * y));
+            const V3f ax_v = v3(2.f * (x * y - ww * z), 1.f - 2.f * (x * x + z * z), 2.f * (y * z + ww * x));
+            const V3f dy_u = rmul(dtu), dy_v = rmul(dtv);
+            acc_sc[0] += dt(dy_u, ax_u);
+            acc_sc[1] += dt(dy_v, ax_v);
+            float dq[4];
+            quat_grad_cols01(x, y, z, ww, f.qn, dy_u * s.x, dy_v * s.y, dq);
+            acc_q[0] += dq[0];
+            acc_q[1] += dq[1];
+            acc_q[2] += dq[2];
+            acc_q[3] += dq[3];
+            // Colour -> SH (+ view direction), clamp mask from the record
+            // (a clamped channel was stored as exactly 0).
+            const float4 col = recs[(size_t)v * n + i].col_op;
+            V3f dcol = v3(col.x == 0.f ? 0.f : gc[kGCol], col.y == 0.f ? 0.f : gc[kGCol + 1],
+                          col.z == 0.f ? 0.f : gc[kGCol + 2]);
+            V3f dcenter = v3(0.f, 0.f, 0.f) - dpos;
+            if (dcol.x != 0.f || dcol.y != 0.f || dcol.z != 0.f) {
+                const float len = sqrtf(f.xx * f.xx + f.xy * f.xy + f.xz * f.xz);
+                const float il = 1.0f / len;
+                const V3f dir = v3(f.xx * il, f.xy * il, f.xz * il);
+                float basis[16], bgrad[48];
+                const int nb = sh_basis((int)cam.sh_degree, dir.x, dir.y, dir.z, basis);
+                sh_basis_grad((int)cam.sh_degree, dir.x, dir.y, dir.z, bgrad);
+                const float* shp = sh + (size_t)i * kShFloats;
+                V3f ddir = v3(0.f, 0.f, 0.f);
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    if (k < nb) {
+                        acc_sh[3 * k + 0] += dcol.x * basis[k];
+                        acc_sh[3 * k + 1] += dcol.y * basis[k];
+                        acc_sh[3 * k + 2] += dcol.z * basis[k];
+                        const float sdot = shp[3 * k] * dcol.x + shp[3 * k + 1] * dcol.y + shp[3 * k + 2] * dcol.z;
+                        ddir = ddir + v3(bgrad[3 * k], bgrad[3 * k + 1], bgrad[3 * k + 2]) * sdot;
+                    }
+                }
+                const V3f dv = (ddir - dir * dt(dir, ddir)) * il;
+                dpos = dpos + dv;
+                dcenter = dcenter - dv;
+            }
+            acc_pos[0] += dpos.x;
+            acc_pos[1] += dpos.y;
+            acc_pos[2] += dpos.z;
+            acc_op += gc[kGOp];
+            if (cam_partials) {
+                // dL/dR_cam[j][k] += y_j da_k for a = R^T y (backward.rs:252-257).
+                const V3f xw = v3(f.xx, f.xy, f.xz);
+                const V3f yu = ax_u * s.x, yv = ax_v * s.y;
+                const float das[3][3] = {{dc.x, dc.y, dc.z}, {dtu.x, dtu.y, dtu.z}, {dtv.x, dtv.y, dtv.z}};
+                const V3f ys[3] = {xw, yu, yv};
+#pragma unroll
+                for (int kk = 0; kk < 3; ++kk)
+#pragma unroll
+                    for (int src = 0; src < 3; ++src) {
+                        cg[3 * kk + 0] += ys[src].x * das[src][kk];
+                        cg[3 * kk + 1] += ys[src].y * das[src][kk];
+                        cg[3 * kk + 2] += ys[src].z * das[src][kk];
+                    }
+                cg[9] = dcenter.x;
+                cg[10] = dcenter.y;
+                cg[11] = dcenter.z;
+                cg[12] = dfocal;
+            }
+        }
+        if (cam_partials) {
+            // Block reduction of the 16 camera slots (every thread takes part).
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                float r = cg[k];
+                for (int o = 32; o >= 1; o >>= 1) r += __shfl_xor(r, o);
+                if (lane == 0) s_cam[wave][k] = r;
+            }
+            __syncthreads();
+            if (threadIdx.x < 16) {
+                float r = 0.f;
+                for (int w2 = 0; w2 < kGeomThreads / 64; ++w2) r += s_cam[w2][threadIdx.x];
+                cam_partials[((size_t)v * gridDim.x + blockIdx.x) * 16 + threadIdx.x] = r;
+            }
+            __syncthreads();
+        }
+    }
+    if (!valid) return;
+    float4 gp = make_float4(acc_pos[0], acc_pos[1], acc_pos[2], 0.f);
+    float2 gs = make_float2(acc_sc[0], acc_sc[1]);
+    float4 gq = make_float4(acc_q[0], acc_q[1], acc_q[2], acc_q[3]);
+    float go = acc_op;
+    float* gsh = g.sh + (size_t)i * kShFloats;
+    if (accumulate) {
+        const float4 op = g.pos[i];
+        gp.x += op.x;
+        gp.y += op.y;
+        gp.z += op.z;
+        const float2 os = g.scales[i];
+        gs.x += os.x;
+        gs.y += os.y;
+        const float4 oq = g.quat[i];
+        gq.x += oq.x;
+        gq.y += oq.y;
+        gq.z += oq.z;
+        gq.w += oq.w;
+        go += g.opacity[i];
+#pragma unroll
+        for (int k = 0; k < kShFloats; ++k) acc_sh[k] += gsh[k];
+    }
+    g.pos[i] = gp;
+    g.scales[i] = gs;
+    g.quat[i] = gq;
+    g.opacity[i] = go;
+    float4* gsh4 = reinterpret_cast<float4*>(gsh);
+#pragma unroll
+    for (int k = 0; k < kShFloats / 4; ++k)
+        gsh4[k] = make_float4(acc_sh[4 * k], acc_sh[4 * k + 1], acc_sh[4 * k + 2], acc_sh[4 * k + 3]);
+}
+
+// Folds per-block camera partials into 16 floats per view (accumulating into
+// g_cam when accumulate != 0).
+__global__ __launch_bounds__(256) void cam_reduce_kernel(const float* __restrict__ partials,
+                                                         uint32_t nblocks, float* __restrict__ g_cam,
+                                                         int accumulate) {
+    __shared__ float s[256 / 64][16];
+    const uint32_t v = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    float acc[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc[k] = 0.f;
+    for (uint32_t b = threadIdx.x; b < nblocks; b += blockDim.x) {
+        const float* src = partials + ((size_t)v * nblocks + b) * 16;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) acc[k] += src[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        float r = acc[k];
+        for (int o = 32; o >= 1; o >>= 1) r += __shfl_xor(r, o);
+        if (lane == 0) s[wave][k] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        float r = 0.f;
+        for (int w = 0; w < 4; ++w) r += s[w][threadIdx.x];
+        float* dst = g_cam + (size_t)v * 16 + threadIdx.x;
+        *dst = accumulate ? *dst + r : r;
+    }
+}
+
+}  // namespace
+
+size_t geom_cam_partials_bytes(uint32_t n, uint32_t n_views) {
+    return (size_t)div_up(n ? n : 1, kGeomThreads) * n_views * 16 * sizeof(float);
+}
+
+hipError_t launch_geom_bwd(hipStream_t st, const CamParams* cams_dev, uint32_t n_views, uint32_t n,
+                           const float4* pos, const float2* scales, const float4* quats,
+                           const float* sh, const Record* recs, const float* grad_coef,
+                           ParamGrads g, bool accumulate, float* cam_partials) {
+    if (n == 0) return hipSuccess;
+    const uint32_t blocks = div_up(n, kGeomThreads);
+    float* parts = g.cam ? cam_partials : nullptr;
+    hipLaunchKernelGGL(geom_bwd_kernel, dim3(blocks), dim3(kGeomThreads), 0, st, cams_dev, n_views, n,
+                       pos, scales, quats, sh, recs, grad_coef, g, accumulate ? 1 : 0, parts);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !parts) return e;
+    hipLaunchKernelGGL(cam_reduce_kernel, dim3(n_views), dim3(256), 0, st, parts, blocks, g.cam,
+                       accumulate ? 1 : 0);
+    return hipGetLastError();
+}
+
+}  // namespace gs

@@ -1,0 +1,416 @@
+// Internal declarations shared by the gfx950 kernels and the C-ABI runtime.
+// Everything here is MI355X-only HIP: wave64, hardware FP32 atomics, no
+// alternative backends.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "gs_hip.h"
+
+namespace gs {
+
+// ------------------------------------------------------------ model constants
+// crates/gs-cpu-ref/src/scene.rs:17-21 and raster_common.wgsl:34-39.
+constexpr float kAlphaSkip = 1.0f / 255.0f;
+constexpr float kAlphaClamp = 0.995f;
+constexpr float kTTerminate = 1e-4f;
+constexpr float kNearDepth = 0.05f;
+constexpr int kTile = 16;        // coverage granularity: part of the forward model
+constexpr int kShFloats = 48;    // SH storage padded to degree 3 (rasterizer.rs:13-15)
+constexpr int kWave = 64;
+
+// Per-surfel, per-view raster record written by the project kernel: the
+// ray-splat setup in the form the per-pixel evaluation needs. With
+// s = (-(x+0.5-cx)/f, (y+0.5-cy)/f, 1) the negated pixel ray, the Cramer
+// numerators of forward.rs:136-147 are linear in s:
+//   D  = det[tu, tv, s] = M.s,  M = tu x tv
+//   Nu = det[-c, tv, s] = A.s,  A = tv x c
+//   Nv = det[tu, -c, s] = B.s,  B = c x tu
+//   Nt = det[tu, tv, -c] = -M.c (per surfel constant)
+// so u = A.s/D, v = B.s/D, t = Nt/D.
+struct __align__(16) Record {
+    float4 m_nt;     // M.xyz, Nt
+    float4 a_px;     // A.xyz, projected center x (pixels)
+    float4 b_py;     // B.xyz, projected center y (pixels)
+    float4 col_op;   // clamped SH color rgb, opacity
+    float4 nrm_dep;  // camera-facing unit normal xyz, center depth
+};
+static_assert(sizeof(Record) == 80, "record layout");
+
+// Coefficient-space gradient accumulator per (view, surfel), summed over the
+// pixels of every tile the surfel reaches (raster_bwd), chained to parameter
+// space once per surfel (geom_bwd). Slots:
+enum GradSlot : int {
+    kGA = 0,        // dL/dA (3)
+    kGB = 3,        // dL/dB (3)
+    kGM = 6,        // dL/dM (3)
+    kGCol = 9,      // dL/dcolor (3)
+    kGOp = 12,      // dL/dopacity
+    kGPx = 13,      // dL/dpx
+    kGPy = 14,      // dL/dpy
+    kGNt = 15,      // dL/dNt        (depth / distortion losses only)
+    kGDep = 16,     // dL/ddepth     (screen-branch depth, depth losses only)
+    kGNrm = 17,     // dL/dnormal (3) (normal loss only)
+    kGradStride = 20,
+};
+
+// Camera as the kernels see it (crates/gs-kernels/src/shaders/
+// raster_common.wgsl:5-20 RasterCamera), one per view.
+struct __align__(16) CamParams {
+    float rot[9];          // camera-to-world R, column-major: rot[3k + j] = R[j][k]
+    float center[3];
+    float focal;
+    float cx, cy;          // principal point = (W/2, H/2)
+    uint32_t width, height;
+    uint32_t tiles_x, tiles_y;
+    uint32_t tile_base;    // first global tile id of this view (batched views)
+    uint32_t pix_base;     // first pixel of this view in batched image buffers
+    uint32_t rec_base;     // first record (= view * N) of this view
+    uint32_t sh_degree;
+    float lambda_dist;
+    uint32_t target_base;  // first pixel of this view's loss target (trainer)
+    uint32_t pad1, pad2;
+};
+
+// ------------------------------------------------------------- device math
+
+// The projection quantities that decide tile rects and depth keys are
+// computed with a pinned operation order and no FMA contraction, so the CPU
+// fp32 mirror (oracle/cpuref.cpp oref_prep_f32) reproduces them bit for bit.
+struct CamFrame {
+    float cx_, cy_, cz_;     // c = R^T (p - C)
+    float tux, tuy, tuz;     // tu = R^T (R(q) e0 s_u)
+    float tvx, tvy, tvz;     // tv = R^T (R(q) e1 s_v)
+    float xx, xy, xz;        // p - C (world)
+    float qx, qy, qz, qw;    // normalized quaternion
+    float qn;                // |q|
+};
+
+__device__ __forceinline__ float pinned_dot3(const float* a, float x, float y, float z) {
+#pragma clang fp contract(off)
+    float p0 = a[0] * x;
+    float p1 = a[1] * y;
+    float p2 = a[2] * z;
+    return (p0 + p1) + p2;
+}
+
+// surfel_prep_fwd.wgsl:168-181 (camera-space centre and tangent axes).
+__device__ __forceinline__ CamFrame cam_frame(const CamParams& cam, float4 p, float2 s, float4 q) {
+#pragma clang fp contract(off)
+    CamFrame f;
+    f.xx = p.x - cam.center[0];
+    f.xy = p.y - cam.center[1];
+    f.xz = p.z - cam.center[2];
+    f.cx_ = pinned_dot3(cam.rot + 0, f.xx, f.xy, f.xz);
+    f.cy_ = pinned_dot3(cam.rot + 3, f.xx, f.xy, f.xz);
+    f.cz_ = pinned_dot3(cam.rot + 6, f.xx, f.xy, f.xz);
+    float qq = ((q.x * q.x + q.y * q.y) + q.z * q.z) + q.w * q.w;
+    f.qn = __fsqrt_rn(qq);
+    f.qx = __fdiv_rn(q.x, f.qn);
+    f.qy = __fdiv_rn(q.y, f.qn);
+    f.qz = __fdiv_rn(q.z, f.qn);
+    f.qw = __fdiv_rn(q.w, f.qn);
+    const float x = f.qx, y = f.qy, z = f.qz, w = f.qw;
+    float a0 = 1.0f - 2.0f * (y * y + z * z);
+    float a1 = 2.0f * (x * y + w * z);
+    float a2 = 2.0f * (x * z - w * y);
+    float b0 = 2.0f * (x * y - w * z);
+    float b1 = 1.0f - 2.0f * (x * x + z * z);
+    float b2 = 2.0f * (y * z + w * x);
+    float ux = a0 * s.x, uy = a1 * s.x, uz = a2 * s.x;
+    float vx = b0 * s.y, vy = b1 * s.y, vz = b2 * s.y;
+    f.tux = pinned_dot3(cam.rot + 0, ux, uy, uz);
+    f.tuy = pinned_dot3(cam.rot + 3, ux, uy, uz);
+    f.tuz = pinned_dot3(cam.rot + 6, ux, uy, uz);
+    f.tvx = pinned_dot3(cam.rot + 0, vx, vy, vz);
+    f.tvy = pinned_dot3(cam.rot + 3, vx, vy, vz);
+    f.tvz = pinned_dot3(cam.rot + 6, vx, vy, vz);
+    return f;
+}
+
+// surfel_prep_fwd.wgsl:196-224: projected centre, conservative 16-px tile rect
+// (radius min(3 f (|tu|+|tv|)/depth + 3, 64)), depth key. Returns false if
+// the rect is degenerate (off screen); depth >= near is the caller's check.
+struct RectOut {
+    float px, py;
+    uint32_t x0, y0, x1, y1;
+    bool on_screen;
+};
+
+__device__ __forceinline__ RectOut tile_rect(const CamParams& cam, const CamFrame& f, float depth) {
+#pragma clang fp contract(off)
+    RectOut r;
+    float lu = __fsqrt_rn((f.tux * f.tux + f.tuy * f.tuy) + f.tuz * f.tuz);
+    float lv = __fsqrt_rn((f.tvx * f.tvx + f.tvy * f.tvy) + f.tvz * f.tvz);
+    r.px = cam.cx + __fdiv_rn(cam.focal * f.cx_, depth);
+    r.py = cam.cy - __fdiv_rn(cam.focal * f.cy_, depth);
+    float rad = __fdiv_rn((3.0f * cam.focal) * (lu + lv), depth) + 3.0f;
+    rad = rad < 64.0f ? rad : 64.0f;
+    const float fw = (float)cam.width, fh = (float)cam.height;
+    r.on_screen = r.px + rad >= 0.0f && r.px - rad < fw && r.py + rad >= 0.0f && r.py - rad < fh;
+    const float mx = (float)(cam.tiles_x - 1), my = (float)(cam.tiles_y - 1);
+    auto cl = [](float v, float hi) {
+        float c = v > 0.0f ? v : 0.0f;
+        c = c < hi ? c : hi;
+        return (uint32_t)c;
+    };
+    r.x0 = cl(__fdiv_rn(r.px - rad, 16.0f), mx);
+    r.y0 = cl(__fdiv_rn(r.py - rad, 16.0f), my);
+    r.x1 = cl(__fdiv_rn(r.px + rad, 16.0f), mx);
+    r.y1 = cl(__fdiv_rn(r.py + rad, 16.0f), my);
+    return r;
+}
+
+// SH constants, crates/gs-core/src/sh.rs:6-18 (f32 values).
+constexpr float kShC0 = 0.28209479f;
+constexpr float kShC1 = 0.48860252f;
+constexpr float kShC2_0 = 1.0925484f, kShC2_1 = -1.0925484f, kShC2_2 = 0.31539157f,
+                kShC2_3 = -1.0925484f, kShC2_4 = 0.5462742f;
+constexpr float kShC3_0 = -0.5900436f, kShC3_1 = 2.8906114f, kShC3_2 = -0.4570458f,
+                kShC3_3 = 0.37317634f, kShC3_4 = -0.4570458f, kShC3_5 = 1.4453057f,
+                kShC3_6 = -0.5900436f;
+
+// Basis values b_k(dir) for degree <= 3 (math.rs:127-156); returns count.
+__device__ __forceinline__ int sh_basis(int degree, float x, float y, float z, float* b) {
+    b[0] = kShC0;
+    if (degree < 1) return 1;
+    b[1] = -kShC1 * y;
+    b[2] = kShC1 * z;
+    b[3] = -kShC1 * x;
+    if (degree < 2) return 4;
+    const float xx = x * x, yy = y * y, zz = z * z;
+    b[4] = kShC2_0 * x * y;
+    b[5] = kShC2_1 * y * z;
+    b[6] = kShC2_2 * (2.0f * zz - xx - yy);
+    b[7] = kShC2_3 * x * z;
+    b[8] = kShC2_4 * (xx - yy);
+    if (degree < 3) return 9;
+    b[9] = kShC3_0 * y * (3.0f * xx - yy);
+    b[10] = kShC3_1 * x * y * z;
+    b[11] = kShC3_2 * y * (4.0f * zz - xx - yy);
+    b[12] = kShC3_3 * z * (2.0f * zz - 3.0f * xx - 3.0f * yy);
+    b[13] = kShC3_4 * x * (4.0f * zz - xx - yy);
+    b[14] = kShC3_5 * z * (xx - yy);
+    b[15] = kShC3_6 * x * (xx - 3.0f * yy);
+    return 16;
+}
+
+// d b_k / d dir (math.rs:159-191), 3 floats per basis function.
+__device__ __forceinline__ void sh_basis_grad(int degree, float x, float y, float z, float* g) {
+    g[0] = g[1] = g[2] = 0.0f;
+    if (degree < 1) return;
+    g[3] = 0.0f; g[4] = -kShC1; g[5] = 0.0f;
+    g[6] = 0.0f; g[7] = 0.0f; g[8] = kShC1;
+    g[9] = -kShC1; g[10] = 0.0f; g[11] = 0.0f;
+    if (degree < 2) return;
+    const float xx = x * x, yy = y * y, zz = z * z;
+    g[12] = kShC2_0 * y; g[13] = kShC2_0 * x; g[14] = 0.0f;
+    g[15] = 0.0f; g[16] = kShC2_1 * z; g[17] = kShC2_1 * y;
+    g[18] = kShC2_2 * (-2.0f * x); g[19] = kShC2_2 * (-2.0f * y); g[20] = kShC2_2 * (4.0f * z);
+    g[21] = kShC2_3 * z; g[22] = 0.0f; g[23] = kShC2_3 * x;
+    g[24] = kShC2_4 * (2.0f * x); g[25] = kShC2_4 * (-2.0f * y); g[26] = 0.0f;
+    if (degree < 3) return;
+    g[27] = kShC3_0 * (6.0f * x * y); g[28] = kShC3_0 * (3.0f * xx - 3.0f * yy); g[29] = 0.0f;
+    g[30] = kShC3_1 * (y * z); g[31] = kShC3_1 * (x * z); g[32] = kShC3_1 * (x * y);
+    g[33] = kShC3_2 * (-2.0f * x * y); g[34] = kShC3_2 * (4.0f * zz - xx - 3.0f * yy);
+    g[35] = kShC3_2 * (8.0f * y * z);
+    g[36] = kShC3_3 * (-6.0f * x * z); g[37] = kShC3_3 * (-6.0f * y * z);
+    g[38] = kShC3_3 * (6.0f * zz - 3.0f * xx - 3.0f * yy);
+    g[39] = kShC3_4 * (4.0f * zz - 3.0f * xx - yy); g[40] = kShC3_4 * (-2.0f * x * y);
+    g[41] = kShC3_4 * (8.0f * x * z);
+    g[42] = kShC3_5 * (2.0f * x * z); g[43] = kShC3_5 * (-2.0f * y * z); g[44] = kShC3_5 * (xx - yy);
+    g[45] = kShC3_6 * (3.0f * xx - 3.0f * yy); g[46] = kShC3_6 * (-6.0f * x * y); g[47] = 0.0f;
+}
+
+__device__ __forceinline__ float3 cross3(float3 a, float3 b) {
+    return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// Negated pixel ray s for pixel (x, y) (forward.rs:32-40 with s = -d).
+__device__ __forceinline__ void pixel_s(const CamParams& cam, int x, int y, float& sx, float& sy) {
+    sx = -(((float)x + 0.5f - cam.cx) / cam.focal);
+    sy = ((float)y + 0.5f - cam.cy) / cam.focal;
+}
+
+// One pixel-surfel evaluation (forward.rs:136-169), shared verbatim by the
+// forward and backward kernels so the backward replays the forward's alpha
+// decisions bit for bit. Returns alpha (0 when skipped).
+struct HitEval {
+    float alpha;     // min(o * ghat, 0.995), 0 if below 1/255
+    float ghat;
+    float u, v, t;   // ray-branch coordinates (t = centre depth on the screen branch)
+    float inv_d;     // 1 / D (ray branch)
+    bool ray;
+    bool clamped;    // o * ghat > 0.995
+};
+
+__device__ __forceinline__ HitEval eval_hit(const Record& r, float sx, float sy, float pxc, float pyc) {
+    HitEval h;
+    const float D = r.m_nt.x * sx + r.m_nt.y * sy + r.m_nt.z;
+    const float Nu = r.a_px.x * sx + r.a_px.y * sy + r.a_px.z;
+    const float Nv = r.b_py.x * sx + r.b_py.y * sy + r.b_py.z;
+    float e_ray = -INFINITY;
+    h.u = h.v = h.t = 0.0f;
+    h.inv_d = 0.0f;
+    if (fabsf(D) > 1e-12f) {
+        const float inv = 1.0f / D;
+        h.u = Nu * inv;
+        h.v = Nv * inv;
+        h.t = r.m_nt.w * inv;
+        h.inv_d = inv;
+        if (h.t > kNearDepth) e_ray = -(h.u * h.u + h.v * h.v) * 0.5f;
+    }
+    const float dx = pxc - r.a_px.w, dy = pyc - r.b_py.w;
+    const float e_scr = -(dx * dx + dy * dy);   // 2 sigma^2 = 1 (LOWPASS_SIGMA2 = 0.5)
+    h.ray = e_ray >= e_scr;
+    const float e = h.ray ? e_ray : e_scr;
+    if (!h.ray) h.t = r.nrm_dep.w;
+    h.ghat = __expf(e);
+    const float raw = r.col_op.w * h.ghat;
+    h.clamped = raw > kAlphaClamp;
+    float a = fminf(raw, kAlphaClamp);
+    h.alpha = a < kAlphaSkip ? 0.0f : a;
+    return h;
+}
+
+// ------------------------------------------------------------ runtime glue
+
+void set_error(const std::string& msg);
+gs_status fail(gs_status st, const std::string& msg);
+gs_status hip_fail(hipError_t e, const char* what);
+
+#define GS_HIP(call)                                          \
+    do {                                                      \
+        hipError_t e_ = (call);                               \
+        if (e_ != hipSuccess) return ::gs::hip_fail(e_, #call); \
+    } while (0)
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// ------------------------------------------------------------- launchers
+// Each returns hipSuccess or the launch error. Shapes are validated by the
+// callers in runtime.cpp before any launch.
+
+// project.hip
+hipError_t launch_project(hipStream_t st, const CamParams* cams_dev, uint32_t n_views, uint32_t n,
+                          const float4* pos, const float2* scales, const float4* quats,
+                          const float* opac, const float* sh, Record* recs, uint4* rects,
+                          uint32_t* depth_keys, uint32_t* counts, uint32_t* ranges_to_clear,
+                          uint32_t num_ranges);
+
+// scan.hip
+size_t scan_scratch_bytes(uint32_t capacity);
+hipError_t launch_exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t n,
+                                 uint32_t* total_out, void* scratch);
+
+// binning.hip
+hipError_t launch_emit_keys(hipStream_t st, uint32_t n_items, const uint4* rects,
+                            const uint32_t* depth_keys, const uint32_t* offsets,
+                            const CamParams* cams_dev, uint32_t items_per_view,
+                            uint32_t tiles_x_fixed, uint64_t* keys, uint32_t* payloads,
+                            uint32_t max_entries, const uint32_t* total, uint32_t* sort_count,
+                            uint32_t* overflow_flag);
+hipError_t launch_tile_ranges(hipStream_t st, const uint64_t* keys, const uint32_t* count,
+                              uint32_t capacity, uint32_t* ranges);
+hipError_t launch_fill_u32(hipStream_t st, uint32_t* dst, uint32_t value, size_t count);
+hipError_t launch_count_tiles(hipStream_t st, uint32_t n, const uint4* rects, uint32_t* counts);
+
+// sort.hip
+size_t sort_scratch_bytes(uint32_t capacity);
+hipError_t launch_radix_sort_u64(hipStream_t st, uint64_t* keys0, uint32_t* vals0, uint64_t* keys1,
+                                 uint32_t* vals1, const uint32_t* count, uint32_t capacity,
+                                 uint32_t begin_bit, uint32_t end_bit, uint32_t digit_bits,
+                                 void* scratch, int* result_in_alt);
+hipError_t launch_copy_pairs(hipStream_t st, const void* src_keys, const uint32_t* src_vals,
+                             void* dst_keys, uint32_t* dst_vals, uint32_t key_bytes,
+                             const uint32_t* count, uint32_t capacity);
+hipError_t launch_radix_sort_u32(hipStream_t st, uint32_t* keys0, uint32_t* vals0, uint32_t* keys1,
+                                 uint32_t* vals1, const uint32_t* count, uint32_t capacity,
+                                 uint32_t begin_bit, uint32_t end_bit, uint32_t digit_bits,
+                                 void* scratch, int* result_in_alt);
+
+// raster_fwd.hip
+struct FwdOut {
+    float4* color;     // rgb, sum w
+    float4* aux;       // T_final, sum w t, 0, 0
+    float4* normal;    // sum w n
+    uint32_t* ncontrib;
+};
+hipError_t launch_raster_fwd(hipStream_t st, const CamParams* cams_dev, const CamParams* cams_host,
+                             uint32_t n_views, uint32_t total_tiles, const uint32_t* ranges,
+                             const uint32_t* payloads, const Record* recs, FwdOut out);
+
+// raster_bwd.hip
+struct BwdIn {
+    const float4* dl_dcolor;   // rgb, w = depth-loss gradient
+    const float4* dl_dnormal;  // may be null
+    const float4* out_color;   // forward sum w in .w
+    const float4* out_aux;     // T_final, sum w t
+    const uint32_t* ncontrib;
+};
+hipError_t launch_raster_bwd(hipStream_t st, const CamParams* cams_dev, const CamParams* cams_host,
+                             uint32_t n_views, uint32_t total_tiles, const uint32_t* ranges,
+                             const uint32_t* payloads, const Record* recs, BwdIn in,
+                             float* grad_coef, bool aux);
+
+// geom_bwd.hip
+struct ParamGrads {
+    float4* pos;
+    float2* scales;
+    float4* quat;
+    float* opacity;
+    float* sh;
+    float* cam;   // 16 floats per view, or null
+};
+size_t geom_cam_partials_bytes(uint32_t n, uint32_t n_views);
+hipError_t launch_geom_bwd(hipStream_t st, const CamParams* cams_dev, uint32_t n_views, uint32_t n,
+                           const float4* pos, const float2* scales, const float4* quats,
+                           const float* sh, const Record* recs, const float* grad_coef,
+                           ParamGrads g, bool accumulate, float* cam_partials);
+
+// loss.hip
+// With `view` set, the target is read at target + view->target_base (the
+// per-view descriptor lives in device memory, so a captured graph replays
+// with whatever views the descriptor table holds).
+hipError_t launch_l1_ssim(hipStream_t st, uint32_t w, uint32_t h, float lambda, float dssim_scale,
+                          const float4* img, const float4* target, float4* dl_out,
+                          float* coef_scratch, double* partial_sums, uint32_t n_partials,
+                          const CamParams* view = nullptr);
+uint32_t l1_ssim_partials(uint32_t w, uint32_t h);
+
+// adam.hip
+struct AdamClass {
+    float* raw;
+    const float* grad;
+    float* m;
+    float* v;
+    float* activated;
+    uint32_t n;
+    int activation;   // 0 identity, 1 exp, 2 sigmoid
+    float lr;
+    float reg;
+    uint32_t block_begin;  // first block of this class in the fused grid
+};
+struct AdamState {
+    uint32_t t;
+    float p1, p2;          // beta1^t, beta2^t (running f32 products)
+    float bc1_inv, bc2_inv;
+};
+constexpr int kMaxAdamClasses = 8;
+struct AdamTable {
+    AdamClass c[kMaxAdamClasses];
+    uint32_t n_classes;
+};
+uint32_t adam_blocks_for(uint32_t n);
+hipError_t launch_adam_tick(hipStream_t st, AdamState* state);
+// The class table travels by value in the kernel arguments (a captured graph
+// keeps the learning rates it was captured with).
+hipError_t launch_adam(hipStream_t st, const AdamTable& table, uint32_t total_blocks,
+                       const AdamState* state, bool step);
+
+}  // namespace gs

@@ -1,0 +1,109 @@
+// Surfel projection (A2) with tile counting (A3) fused: one thread per
+// (view, surfel). Restates crates/gs-kernels/src/shaders/surfel_prep_fwd.wgsl:
+// 64-132 and count_tiles of tile_binning.wgsl:30-43, and additionally emits
+// the per-surfel ray-splat setup (M, A, B, Nt) so the rasterizer's per-pixel
+// intersection is three 2-FMA dot products instead of four triple products.
+//
+// HBM traffic per (view, surfel): reads 16+8+16+4+12*(d+1)^2 B of parameters,
+// writes an 80-B record, a 16-B rect, a 4-B depth key and a 4-B count.
+
+#include "gs_internal.h"
+
+namespace gs {
+
+namespace {
+
+__global__ __launch_bounds__(256) void project_kernel(
+    const CamParams* __restrict__ cams, uint32_t n, const float4* __restrict__ pos,
+    const float2* __restrict__ scales, const float4* __restrict__ quats,
+    const float* __restrict__ opac, const float* __restrict__ sh, Record* __restrict__ recs,
+    uint4* __restrict__ rects, uint32_t* __restrict__ depth_keys, uint32_t* __restrict__ counts,
+    uint32_t* __restrict__ ranges_to_clear, uint32_t num_ranges) {
+    const uint32_t view = blockIdx.y;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    // The tile-range table of the following binning stage starts cleared
+    // (binning.rs:228); the first view's grid does it (stream order makes it
+    // visible to tile_ranges).
+    if (view == 0 && ranges_to_clear != nullptr) {
+        for (uint32_t k = i; k < num_ranges; k += gridDim.x * blockDim.x) ranges_to_clear[k] = 0u;
+    }
+    if (i >= n) return;
+    const CamParams cam = cams[view];
+    const uint32_t g = view * n + i;
+
+    const float4 p = pos[i];
+    const float2 s = scales[i];
+    const float4 q = quats[i];
+    const CamFrame f = cam_frame(cam, p, s, q);
+    const float depth = -f.cz_;
+    uint4 rect = make_uint4(1u, 1u, 0u, 0u);
+    if (!(depth >= kNearDepth)) {  // near cull (surfel_prep_fwd.wgsl:171-176); NaN culls too
+        rects[g] = rect;
+        depth_keys[g] = 0u;
+        counts[g] = 0u;
+        return;
+    }
+    const RectOut r = tile_rect(cam, f, depth);
+    uint32_t count = 0u;
+    if (r.on_screen) {
+        rect = make_uint4(r.x0, r.y0, r.x1, r.y1);
+        count = (r.x1 - r.x0 + 1u) * (r.y1 - r.y0 + 1u);
+    }
+    rects[g] = rect;
+    depth_keys[g] = __float_as_uint(depth);
+    counts[g] = count;
+    if (count == 0u) return;  // no entry will ever read the record
+
+    // Ray-splat setup.
+    const float3 c = make_float3(f.cx_, f.cy_, f.cz_);
+    const float3 tu = make_float3(f.tux, f.tuy, f.tuz);
+    const float3 tv = make_float3(f.tvx, f.tvy, f.tvz);
+    const float3 M = cross3(tu, tv);
+    const float3 A = cross3(tv, c);
+    const float3 B = cross3(c, tu);
+    const float Nt = -dot3(M, c);
+
+    // View-dependent colour (surfel_prep_fwd.wgsl:113-147, 183-186).
+    const float inv_len = 1.0f / sqrtf(f.xx * f.xx + f.xy * f.xy + f.xz * f.xz);
+    float basis[16];
+    const int nb = sh_basis((int)cam.sh_degree, f.xx * inv_len, f.xy * inv_len, f.xz * inv_len, basis);
+    const float* shp = sh + (size_t)i * kShFloats;
+    float cr = 0.5f, cg = 0.5f, cb = 0.5f;
+    for (int k = 0; k < nb; ++k) {
+        cr += basis[k] * shp[3 * k + 0];
+        cg += basis[k] * shp[3 * k + 1];
+        cb += basis[k] * shp[3 * k + 2];
+    }
+    // Camera-facing unit normal (surfel_prep_fwd.wgsl:188-194).
+    float3 nrm = M;
+    const float mm = dot3(M, M);
+    if (mm > 0.0f) {
+        const float inv = 1.0f / sqrtf(mm);
+        nrm = make_float3(M.x * inv, M.y * inv, M.z * inv);
+        if (dot3(nrm, c) > 0.0f) nrm = make_float3(-nrm.x, -nrm.y, -nrm.z);
+    }
+    Record rec;
+    rec.m_nt = make_float4(M.x, M.y, M.z, Nt);
+    rec.a_px = make_float4(A.x, A.y, A.z, r.px);
+    rec.b_py = make_float4(B.x, B.y, B.z, r.py);
+    rec.col_op = make_float4(fmaxf(cr, 0.0f), fmaxf(cg, 0.0f), fmaxf(cb, 0.0f), opac[i]);
+    rec.nrm_dep = make_float4(nrm.x, nrm.y, nrm.z, depth);
+    recs[g] = rec;
+}
+
+}  // namespace
+
+hipError_t launch_project(hipStream_t st, const CamParams* cams_dev, uint32_t n_views, uint32_t n,
+                          const float4* pos, const float2* scales, const float4* quats,
+                          const float* opac, const float* sh, Record* recs, uint4* rects,
+                          uint32_t* depth_keys, uint32_t* counts, uint32_t* ranges_to_clear,
+                          uint32_t num_ranges) {
+    if (n == 0 && num_ranges == 0) return hipSuccess;
+    uint32_t blocks = div_up(n > num_ranges ? n : num_ranges, 256);
+    dim3 grid(blocks, n_views);
+    hipLaunchKernelGGL(project_kernel, grid, dim3(256), 0, st, cams_dev, n, pos, scales, quats, opac,
+                       sh, recs, rects, depth_keys, counts, ranges_to_clear, num_ranges);
+    return hipGetLastError();
+}
+
+}  // namespace gs

@@ -1,0 +1,269 @@
+// Tile rasterizer backward (A10): the hand-written adjoint of raster_fwd.
+//
+// Restates crates/gs-kernels/src/shaders/rasterize_bwd.wgsl:113-312 (the
+// oracle's backward.rs:67-173): per pixel, walk the tile list back to front
+// from the stored final transmittance and last-contributor index, recover
+// T_before = T / (1 - alpha) instead of storing per-fragment state, keep
+// suffix sums, and differentiate the compositing and the ray-splat
+// intersection. Differences from the reference, all MI355X-driven:
+//   * gradients are taken w.r.t. the per-surfel record coefficients (A, B, M,
+//     Nt, px, py, depth, colour, opacity, normal); the linear chain to
+//     camera-space (c, tu, tv) happens once per surfel in geom_bwd instead of
+//     four triple-product gradients per pixel-hit;
+//   * per record, each wave64 reduces its 64 pixels' contributions with a
+//     transposed butterfly (17 lane exchanges for 16 values instead of 96),
+//     adds them into an LDS slot (ds_add_f32), and the batch's slots are
+//     flushed with hardware global FP32 atomics (no CAS loops).
+// AUX = true adds the depth / normal image losses and the in-walk depth
+// distortion loss (backward.rs:126-153).
+
+#include "gs_internal.h"
+
+namespace gs {
+
+namespace {
+
+constexpr int kBatch = 256;
+
+__device__ __forceinline__ uint32_t view_of_tile(const CamParams* cams, uint32_t n_views, uint32_t t) {
+    uint32_t v = 0;
+    for (uint32_t k = 1; k < n_views; ++k)
+        if (cams[k].tile_base <= t) v = k;
+    return v;
+}
+
+// Sum NV values (power of two) over the 64 lanes of a wave. On return lane l
+// holds the total of value index (l >> log2(64/NV)) & (NV-1); lanes with
+// (l & (64/NV - 1)) == 0 are the designated writers.
+template <int NV>
+__device__ __forceinline__ float wave_reduce_transpose(float (&v)[NV], int lane) {
+    int off = 32;
+#pragma unroll
+    for (int count = NV; count > 1; count >>= 1, off >>= 1) {
+        const bool upper = (lane & off) != 0;
+#pragma unroll
+        for (int i = 0; i < count / 2; ++i) {
+            const float lo = v[i], hi = v[count / 2 + i];
+            const float send = upper ? lo : hi;
+            const float keep = upper ? hi : lo;
+            v[i] = keep + __shfl_xor(send, off);
+        }
+    }
+    float r = v[0];
+#pragma unroll
+    for (; off >= 1; off >>= 1) r += __shfl_xor(r, off);
+    return r;
+}
+
+template <bool AUX>
+__global__ __launch_bounds__(256) void raster_bwd_kernel(const CamParams* __restrict__ cams,
+                                                         uint32_t n_views,
+                                                         const uint32_t* __restrict__ ranges,
+                                                         const uint32_t* __restrict__ payloads,
+                                                         const Record* __restrict__ recs, BwdIn in,
+                                                         float* __restrict__ grad_coef) {
+    constexpr int NS = AUX ? 20 : 16;  // LDS slots per record
+    __shared__ float4 s_m[kBatch], s_a[kBatch], s_b[kBatch], s_c[kBatch], s_n[kBatch];
+    __shared__ uint32_t s_pay[kBatch];
+    __shared__ float s_grad[kBatch * NS];
+    __shared__ uint32_t s_view, s_maxlast;
+
+    const uint32_t gtile = blockIdx.x;
+    const uint32_t start = ranges[2 * (size_t)gtile], end = ranges[2 * (size_t)gtile + 1];
+    if (end <= start) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) {
+        s_view = view_of_tile(cams, n_views, gtile);
+        s_maxlast = 0u;
+    }
+    __syncthreads();
+    const CamParams& cam = cams[s_view];
+    const uint32_t local = gtile - cam.tile_base;
+    const uint32_t tx = local % cam.tiles_x, ty = local / cam.tiles_x;
+    const int px = (int)(tx * kTile) + (wave & 1) * 8 + (lane & 7);
+    const int py = (int)(ty * kTile) + (wave >> 1) * 8 + (lane >> 3);
+    const bool inside = px < (int)cam.width && py < (int)cam.height;
+    float sx, sy;
+    pixel_s(cam, px, py, sx, sy);
+    const float pxc = (float)px + 0.5f, pyc = (float)py + 0.5f;
+    const size_t pi = (size_t)cam.pix_base + (size_t)py * cam.width + (size_t)px;
+
+    float dlr = 0.f, dlg = 0.f, dlb = 0.f, dld = 0.f, dnx = 0.f, dny = 0.f, dnz = 0.f;
+    float T = 1.0f, w_total = 0.f, wt_total = 0.f;
+    uint32_t last = 0;
+    if (inside) {
+        const float4 dl = in.dl_dcolor[pi];
+        dlr = dl.x;
+        dlg = dl.y;
+        dlb = dl.z;
+        const float4 aux = in.out_aux[pi];
+        T = aux.x;
+        last = in.ncontrib[pi];
+        if (AUX) {
+            dld = dl.w;
+            if (in.dl_dnormal) {
+                const float4 dn = in.dl_dnormal[pi];
+                dnx = dn.x;
+                dny = dn.y;
+                dnz = dn.z;
+            }
+            w_total = in.out_color[pi].w;
+            wt_total = aux.y;
+        }
+    }
+    const float lambda = AUX ? cam.lambda_dist : 0.0f;
+    bool has_loss = inside && (dlr != 0.f || dlg != 0.f || dlb != 0.f);
+    if (AUX) has_loss = inside && (has_loss || dld != 0.f || dnx != 0.f || dny != 0.f ||
+                                   dnz != 0.f || lambda != 0.f);
+    if (!has_loss) last = 0;
+    if (last) atomicMax(&s_maxlast, last);
+    __syncthreads();
+    const uint32_t maxlast = s_maxlast;
+    if (maxlast == 0) return;
+
+    float suf_r = 0.f, suf_g = 0.f, suf_b = 0.f;
+    float suf_d = 0.f, suf_nx = 0.f, suf_ny = 0.f, suf_nz = 0.f, suf_w = 0.f, suf_cw = 0.f;
+
+    const int nbatch = (int)((maxlast + kBatch - 1) / kBatch);
+    for (int b = nbatch - 1; b >= 0; --b) {
+        const uint32_t base = start + (uint32_t)b * kBatch;
+        const uint32_t cnt = min((uint32_t)kBatch, end - base);
+        if ((uint32_t)tid < cnt) {
+            const uint32_t pay = payloads[base + tid];
+            const Record& r = recs[pay];
+            s_m[tid] = r.m_nt;
+            s_a[tid] = r.a_px;
+            s_b[tid] = r.b_py;
+            s_c[tid] = r.col_op;
+            s_n[tid] = r.nrm_dep;
+            s_pay[tid] = pay;
+        }
+        for (int k = tid; k < (int)cnt * NS; k += 256) s_grad[k] = 0.0f;
+        __syncthreads();
+
+        for (int j = (int)cnt - 1; j >= 0; --j) {
+            const uint32_t rel = (uint32_t)b * kBatch + (uint32_t)j;
+            if (!__any(rel < last)) continue;
+            Record r;
+            r.m_nt = s_m[j];
+            r.a_px = s_a[j];
+            r.b_py = s_b[j];
+            r.col_op = s_c[j];
+            r.nrm_dep = s_n[j];
+            float g[16];
+            float g2[4];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) g[k] = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) g2[k] = 0.0f;
+            bool contrib = false;
+            if (rel < last) {
+                const HitEval h = eval_hit(r, sx, sy, pxc, pyc);
+                if (h.alpha > 0.0f) {
+                    contrib = true;
+                    const float one_m = 1.0f - h.alpha;
+                    const float inv_one_m = 1.0f / one_m;
+                    const float t_before = T * inv_one_m;
+                    const float w = t_before * h.alpha;
+                    const float t_hit = h.t;
+                    g[kGCol + 0] = dlr * w;
+                    g[kGCol + 1] = dlg * w;
+                    g[kGCol + 2] = dlb * w;
+                    float dl_dt = 0.0f;
+                    float dl_dalpha = (dlr * r.col_op.x + dlg * r.col_op.y + dlb * r.col_op.z) * t_before -
+                                      (dlr * suf_r + dlg * suf_g + dlb * suf_b) * inv_one_m;
+                    suf_r += r.col_op.x * w;
+                    suf_g += r.col_op.y * w;
+                    suf_b += r.col_op.z * w;
+                    if (AUX) {
+                        float dt_dist = 0.0f, coef = 0.0f;
+                        if (lambda != 0.0f) {
+                            const float a_pre = w_total - suf_w - w;
+                            const float b_pre = wt_total - suf_d - w * t_hit;
+                            dt_dist = lambda * 2.0f * w * (a_pre - suf_w);
+                            coef = lambda * (2.0f * (t_hit * a_pre - b_pre) +
+                                             2.0f * (suf_d - t_hit * suf_w));
+                        }
+                        g2[kGNrm - 16 + 0] = dnx * w;
+                        g2[kGNrm - 16 + 1] = dny * w;
+                        g2[kGNrm - 16 + 2] = dnz * w;
+                        dl_dt = dld * w + dt_dist;
+                        dl_dalpha += dld * (t_hit * t_before - suf_d * inv_one_m) +
+                                     (dnx * r.nrm_dep.x + dny * r.nrm_dep.y + dnz * r.nrm_dep.z) * t_before -
+                                     (dnx * suf_nx + dny * suf_ny + dnz * suf_nz) * inv_one_m +
+                                     coef * t_before - suf_cw * inv_one_m;
+                        suf_d += t_hit * w;
+                        suf_nx += r.nrm_dep.x * w;
+                        suf_ny += r.nrm_dep.y * w;
+                        suf_nz += r.nrm_dep.z * w;
+                        suf_w += w;
+                        suf_cw += coef * w;
+                    }
+                    T = t_before;
+                    if (!h.clamped) {
+                        g[kGOp] = dl_dalpha * h.ghat;
+                        const float dl_dghat = dl_dalpha * r.col_op.w;
+                        if (h.ray) {
+                            const float dl_du = -h.u * h.ghat * dl_dghat;
+                            const float dl_dv = -h.v * h.ghat * dl_dghat;
+                            const float dnu = dl_du * h.inv_d;
+                            const float dnv = dl_dv * h.inv_d;
+                            const float dD = -(h.u * dl_du + h.v * dl_dv + t_hit * dl_dt) * h.inv_d;
+                            g[kGA + 0] = dnu * sx;
+                            g[kGA + 1] = dnu * sy;
+                            g[kGA + 2] = dnu;
+                            g[kGB + 0] = dnv * sx;
+                            g[kGB + 1] = dnv * sy;
+                            g[kGB + 2] = dnv;
+                            g[kGM + 0] = dD * sx;
+                            g[kGM + 1] = dD * sy;
+                            g[kGM + 2] = dD;
+                            if (AUX) g[kGNt] = dl_dt * h.inv_d;
+                        } else {
+                            if (AUX) g2[kGDep - 16] = dl_dt;
+                            const float dx = pxc - r.a_px.w, dy = pyc - r.b_py.w;
+                            g[kGPx] = 2.0f * dx * h.ghat * dl_dghat;
+                            g[kGPy] = 2.0f * dy * h.ghat * dl_dghat;
+                        }
+                    }
+                }
+            }
+            if (!__any(contrib)) continue;
+            const float red = wave_reduce_transpose<16>(g, lane);
+            if ((lane & 3) == 0) {
+                if (red != 0.0f) atomicAdd(&s_grad[j * NS + (lane >> 2)], red);
+            }
+            if (AUX) {
+                const float red2 = wave_reduce_transpose<4>(g2, lane);
+                if ((lane & 15) == 0 && red2 != 0.0f) atomicAdd(&s_grad[j * NS + 16 + (lane >> 4)], red2);
+            }
+        }
+        __syncthreads();
+        for (int k = tid; k < (int)cnt * NS; k += 256) {
+            const float v = s_grad[k];
+            if (v != 0.0f) {
+                const int e = k / NS, slot = k - e * NS;
+                atomicAdd(&grad_coef[(size_t)s_pay[e] * kGradStride + slot], v);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+hipError_t launch_raster_bwd(hipStream_t st, const CamParams* cams_dev, const CamParams* /*cams_host*/,
+                             uint32_t n_views, uint32_t total_tiles, const uint32_t* ranges,
+                             const uint32_t* payloads, const Record* recs, BwdIn in,
+                             float* grad_coef, bool aux) {
+    if (total_tiles == 0) return hipSuccess;
+    if (aux)
+        hipLaunchKernelGGL(raster_bwd_kernel<true>, dim3(total_tiles), dim3(256), 0, st, cams_dev,
+                           n_views, ranges, payloads, recs, in, grad_coef);
+    else
+        hipLaunchKernelGGL(raster_bwd_kernel<false>, dim3(total_tiles), dim3(256), 0, st, cams_dev,
+                           n_views, ranges, payloads, recs, in, grad_coef);
+    return hipGetLastError();
+}
+
+}  // namespace gs

@@ -1,0 +1,121 @@
+// Tile rasterizer forward (A8): one 256-thread workgroup per 16x16 tile
+// (of any view in a batch), front-to-back alpha compositing.
+//
+// Restates crates/gs-kernels/src/shaders/rasterize_fwd.wgsl:20-85 and
+// evaluate_hit of raster_common.wgsl:69-95 (the oracle's forward.rs:207-246).
+// MI355X layout: the four wave64s of a workgroup each own an 8x8 quadrant of
+// the tile; surfel records are staged through LDS in batches of 256 (one
+// record per thread, gathered through the sorted payloads); a wave leaves the
+// batch loop once all 64 of its pixels are saturated, and the workgroup stops
+// as soon as every pixel's transmittance is below 1e-4.
+//
+// Outputs per pixel (same buffers as the reference Rasterizer):
+//   color  = (sum w c, sum w),  aux = (T_final, sum w t, 0, 0),
+//   normal = (sum w n, 0),      ncontrib = index (within the tile list) just
+//   past the last contributing record.
+
+#include "gs_internal.h"
+
+namespace gs {
+
+namespace {
+
+constexpr int kBatch = 256;
+
+// Locate the view owning a global tile id (views are few; tile_base ascends).
+__device__ __forceinline__ uint32_t view_of_tile(const CamParams* cams, uint32_t n_views, uint32_t t) {
+    uint32_t v = 0;
+    for (uint32_t k = 1; k < n_views; ++k)
+        if (cams[k].tile_base <= t) v = k;
+    return v;
+}
+
+__global__ __launch_bounds__(256) void raster_fwd_kernel(const CamParams* __restrict__ cams,
+                                                         uint32_t n_views,
+                                                         const uint32_t* __restrict__ ranges,
+                                                         const uint32_t* __restrict__ payloads,
+                                                         const Record* __restrict__ recs,
+                                                         FwdOut out) {
+    __shared__ float4 s_m[kBatch], s_a[kBatch], s_b[kBatch], s_c[kBatch], s_n[kBatch];
+    __shared__ uint32_t s_view;
+
+    const uint32_t gtile = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) s_view = view_of_tile(cams, n_views, gtile);
+    __syncthreads();
+    const CamParams& cam = cams[s_view];
+    const uint32_t local = gtile - cam.tile_base;
+    const uint32_t tx = local % cam.tiles_x, ty = local / cam.tiles_x;
+    const int px = (int)(tx * kTile) + (wave & 1) * 8 + (lane & 7);
+    const int py = (int)(ty * kTile) + (wave >> 1) * 8 + (lane >> 3);
+    const bool inside = px < (int)cam.width && py < (int)cam.height;
+    float sx, sy;
+    pixel_s(cam, px, py, sx, sy);
+    const float pxc = (float)px + 0.5f, pyc = (float)py + 0.5f;
+
+    const uint32_t start = ranges[2 * (size_t)gtile], end = ranges[2 * (size_t)gtile + 1];
+    float T = 1.0f;
+    float cr = 0.f, cg = 0.f, cb = 0.f, wsum = 0.f, dsum = 0.f, nx = 0.f, ny = 0.f, nz = 0.f;
+    uint32_t last = 0;
+    bool done = !inside;
+
+    for (uint32_t base = start; base < end; base += kBatch) {
+        const uint32_t cnt = min((uint32_t)kBatch, end - base);
+        if ((uint32_t)tid < cnt) {
+            const Record& r = recs[payloads[base + tid]];
+            s_m[tid] = r.m_nt;
+            s_a[tid] = r.a_px;
+            s_b[tid] = r.b_py;
+            s_c[tid] = r.col_op;
+            s_n[tid] = r.nrm_dep;
+        }
+        __syncthreads();
+        if (__any(!done)) {
+            for (uint32_t j = 0; j < cnt; ++j) {
+                Record r;
+                r.m_nt = s_m[j];
+                r.a_px = s_a[j];
+                r.b_py = s_b[j];
+                r.col_op = s_c[j];
+                r.nrm_dep = s_n[j];
+                const HitEval h = eval_hit(r, sx, sy, pxc, pyc);
+                if (!done && h.alpha > 0.0f) {
+                    const float w = T * h.alpha;
+                    cr += r.col_op.x * w;
+                    cg += r.col_op.y * w;
+                    cb += r.col_op.z * w;
+                    wsum += w;
+                    dsum += h.t * w;
+                    nx += r.nrm_dep.x * w;
+                    ny += r.nrm_dep.y * w;
+                    nz += r.nrm_dep.z * w;
+                    T *= 1.0f - h.alpha;
+                    last = base - start + j + 1u;
+                    if (T < kTTerminate) done = true;
+                }
+                if (!__any(!done)) break;
+            }
+        }
+        if (__syncthreads_count(!done) == 0) break;
+    }
+    if (inside) {
+        const size_t pi = (size_t)cam.pix_base + (size_t)py * cam.width + (size_t)px;
+        out.color[pi] = make_float4(cr, cg, cb, wsum);
+        out.aux[pi] = make_float4(T, dsum, 0.0f, 0.0f);
+        out.normal[pi] = make_float4(nx, ny, nz, 0.0f);
+        out.ncontrib[pi] = last;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_raster_fwd(hipStream_t st, const CamParams* cams_dev, const CamParams* /*cams_host*/,
+                             uint32_t n_views, uint32_t total_tiles, const uint32_t* ranges,
+                             const uint32_t* payloads, const Record* recs, FwdOut out) {
+    if (total_tiles == 0) return hipSuccess;
+    hipLaunchKernelGGL(raster_fwd_kernel, dim3(total_tiles), dim3(256), 0, st, cams_dev, n_views,
+                       ranges, payloads, recs, out);
+    return hipGetLastError();
+}
+
+}  // namespace gs

@@ -1,0 +1,1432 @@
+// C-ABI runtime for the MI355X surfel training step (include/gs_hip.h).
+//
+// Host-side orchestration only: every handle allocates its device buffers
+// once at capacity (crates/gs-wgpu CLAUDE-style "no mid-training
+// reallocation"), validates shapes before any launch, and enqueues the
+// gfx950 kernels of the sibling .hip files on the caller's stream.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "gs_internal.h"
+
+namespace gs {
+
+// ------------------------------------------------------------------ errors
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+gs_status fail(gs_status st, const std::string& msg) {
+    g_last_error = msg;
+    return st;
+}
+
+gs_status hip_fail(hipError_t e, const char* what) {
+    g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? GS_ERR_OOM : GS_ERR_HIP;
+}
+
+#define GS_CHECK(expr)                        \
+    do {                                      \
+        gs_status s_ = (expr);                \
+        if (s_ != GS_OK) return s_;           \
+    } while (0)
+
+#define GS_REQUIRE(cond, msg)                                    \
+    do {                                                         \
+        if (!(cond)) return ::gs::fail(GS_ERR_INVALID, (msg));   \
+    } while (0)
+
+// --------------------------------------------------------------- allocation
+
+template <typename T>
+static gs_status dalloc(T** out, size_t count, const char* what) {
+    *out = nullptr;
+    if (count == 0) count = 1;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) return hip_fail(e, what);
+    e = hipMemset(p, 0, count * sizeof(T));
+    if (e != hipSuccess) {
+        (void)hipFree(p);
+        return hip_fail(e, what);
+    }
+    *out = static_cast<T*>(p);
+    return GS_OK;
+}
+
+template <typename T>
+static void dfree(T*& p) {
+    if (p) (void)hipFree(static_cast<void*>(p));
+    p = nullptr;
+}
+
+static hipStream_t as_stream(void* s, hipStream_t fallback) {
+    return s ? static_cast<hipStream_t>(s) : fallback;
+}
+
+// Camera-to-world rotation from an unnormalized quaternion (math.rs:14-34),
+// evaluated in double and rounded once: the same f32 matrix feeds every
+// kernel and the CPU mirror.
+static void quat_to_rot(const float q[4], float rot[9]) {
+    const double n = std::sqrt((double)q[0] * q[0] + (double)q[1] * q[1] + (double)q[2] * q[2] +
+                               (double)q[3] * q[3]);
+    const double x = q[0] / n, y = q[1] / n, z = q[2] / n, w = q[3] / n;
+    const double c[3][3] = {{1 - 2 * (y * y + z * z), 2 * (x * y + w * z), 2 * (x * z - w * y)},
+                            {2 * (x * y - w * z), 1 - 2 * (x * x + z * z), 2 * (y * z + w * x)},
+                            {2 * (x * z + w * y), 2 * (y * z - w * x), 1 - 2 * (x * x + y * y)}};
+    for (int k = 0; k < 3; ++k)
+        for (int j = 0; j < 3; ++j) rot[3 * k + j] = (float)c[k][j];
+}
+
+static CamParams make_cam(const gs_raster_camera& c, uint32_t w, uint32_t h) {
+    CamParams p;
+    std::memset(&p, 0, sizeof(p));
+    quat_to_rot(c.quat, p.rot);
+    p.center[0] = c.center[0];
+    p.center[1] = c.center[1];
+    p.center[2] = c.center[2];
+    p.focal = c.focal;
+    p.cx = (float)w * 0.5f;
+    p.cy = (float)h * 0.5f;
+    p.width = w;
+    p.height = h;
+    p.tiles_x = div_up(w, kTile);
+    p.tiles_y = div_up(h, kTile);
+    p.sh_degree = c.sh_degree;
+    return p;
+}
+
+static gs_status check_camera(const gs_raster_camera* c) {
+    GS_REQUIRE(c != nullptr, "camera is null");
+    GS_REQUIRE(std::isfinite(c->focal) && c->focal > 0.0f, "camera focal must be finite and > 0");
+    GS_REQUIRE(c->sh_degree <= 3, "sh_degree must be <= 3");
+    const double qn = (double)c->quat[0] * c->quat[0] + (double)c->quat[1] * c->quat[1] +
+                      (double)c->quat[2] * c->quat[2] + (double)c->quat[3] * c->quat[3];
+    GS_REQUIRE(qn > 0.0 && std::isfinite(qn), "camera quaternion must be non-zero and finite");
+    return GS_OK;
+}
+
+}  // namespace gs
+
+using namespace gs;
+
+// =================================================================== context
+
+struct gs_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipDeviceProp_t props{};
+};
+
+extern "C" {
+
+const char* gs_last_error(void) { return g_last_error.c_str(); }
+int gs_abi_version(void) { return GS_ABI_VERSION; }
+const char* gs_build_info(void) { return "splattar_amd gfx950 HIP " __DATE__; }
+
+gs_status gs_ctx_create(int device_index, gs_ctx* out) {
+    GS_REQUIRE(out != nullptr, "out is null");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) return fail(GS_ERR_NO_DEVICE, "no HIP device available");
+    GS_REQUIRE(device_index >= 0 && device_index < count, "device index out of range");
+    GS_HIP(hipSetDevice(device_index));
+    auto ctx = std::make_unique<gs_context>();
+    ctx->device = device_index;
+    GS_HIP(hipGetDeviceProperties(&ctx->props, device_index));
+    GS_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    *out = ctx.release();
+    return GS_OK;
+}
+
+void gs_ctx_destroy(gs_ctx ctx) {
+    if (!ctx) return;
+    if (ctx->stream) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+}
+
+void* gs_ctx_stream(gs_ctx ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+gs_status gs_ctx_synchronize(gs_ctx ctx) {
+    GS_REQUIRE(ctx != nullptr, "ctx is null");
+    GS_HIP(hipStreamSynchronize(ctx->stream));
+    GS_HIP(hipDeviceSynchronize());
+    return GS_OK;
+}
+
+gs_status gs_ctx_device_info(gs_ctx ctx, char* name, size_t name_len, int* compute_units) {
+    GS_REQUIRE(ctx != nullptr, "ctx is null");
+    if (name && name_len) {
+        std::snprintf(name, name_len, "%s (%s)", ctx->props.name, ctx->props.gcnArchName);
+    }
+    if (compute_units) *compute_units = ctx->props.multiProcessorCount;
+    return GS_OK;
+}
+
+gs_status gs_buffer_alloc(gs_ctx ctx, size_t bytes, void** out) {
+    GS_REQUIRE(ctx != nullptr && out != nullptr, "null argument");
+    uint8_t* p = nullptr;
+    GS_CHECK(dalloc(&p, bytes, "gs_buffer_alloc"));
+    *out = p;
+    return GS_OK;
+}
+
+gs_status gs_buffer_free(gs_ctx ctx, void* ptr) {
+    GS_REQUIRE(ctx != nullptr, "ctx is null");
+    if (ptr) GS_HIP(hipFree(ptr));
+    return GS_OK;
+}
+
+gs_status gs_write_buffer(gs_ctx ctx, void* dst, const void* src, size_t bytes) {
+    GS_REQUIRE(ctx != nullptr && dst != nullptr && (src != nullptr || bytes == 0), "null argument");
+    if (bytes == 0) return GS_OK;
+    // Pageable source: the copy is complete (staged) when the call returns,
+    // so the caller may reuse `src`, as with queue.write_buffer.
+    GS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    GS_HIP(hipStreamSynchronize(ctx->stream));
+    return GS_OK;
+}
+
+gs_status gs_read_buffer(gs_ctx ctx, void* dst, const void* src, size_t bytes) {
+    GS_REQUIRE(ctx != nullptr && dst != nullptr && (src != nullptr || bytes == 0), "null argument");
+    if (bytes == 0) return GS_OK;
+    GS_HIP(hipStreamSynchronize(ctx->stream));
+    GS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return GS_OK;
+}
+
+gs_status gs_clear_buffer(gs_ctx ctx, void* stream, void* dst, size_t bytes) {
+    GS_REQUIRE(ctx != nullptr && dst != nullptr, "null argument");
+    GS_REQUIRE(bytes % 4 == 0, "clear size must be a multiple of 4 bytes");
+    GS_HIP(launch_fill_u32(as_stream(stream, ctx->stream), static_cast<uint32_t*>(dst), 0u, bytes / 4));
+    return GS_OK;
+}
+
+}  // extern "C"
+
+// ================================================================ PrefixSum
+
+struct gs_scan_s {
+    gs_ctx ctx = nullptr;
+    uint32_t capacity = 0;
+    uint32_t* data = nullptr;
+    uint32_t* total = nullptr;
+    uint8_t* scratch = nullptr;
+    ~gs_scan_s() {
+        dfree(data);
+        dfree(total);
+        dfree(scratch);
+    }
+};
+
+extern "C" {
+
+gs_status gs_scan_create(gs_ctx ctx, uint32_t capacity, gs_scan* out) {
+    GS_REQUIRE(ctx && out, "null argument");
+    *out = nullptr;
+    auto s = std::make_unique<gs_scan_s>();
+    s->ctx = ctx;
+    s->capacity = capacity;
+    GS_CHECK(dalloc(&s->data, capacity, "scan data"));
+    GS_CHECK(dalloc(&s->total, 1, "scan total"));
+    GS_CHECK(dalloc(&s->scratch, scan_scratch_bytes(capacity), "scan scratch"));
+    *out = s.release();
+    return GS_OK;
+}
+
+void gs_scan_destroy(gs_scan s) { delete s; }
+uint32_t* gs_scan_data(gs_scan s) { return s ? s->data : nullptr; }
+uint32_t* gs_scan_total(gs_scan s) { return s ? s->total : nullptr; }
+
+gs_status gs_scan_encode(gs_scan s, void* stream, uint32_t n) {
+    GS_REQUIRE(s != nullptr, "scan is null");
+    if (n > s->capacity) return fail(GS_ERR_CAPACITY, "scan: n exceeds capacity");  // prefix_sum.rs:135
+    GS_HIP(launch_exclusive_scan(as_stream(stream, s->ctx->stream), s->data, s->data, n, s->total,
+                                 s->scratch));
+    return GS_OK;
+}
+
+}  // extern "C"
+
+// ============================================================== RadixSorter
+
+struct gs_sorter_s {
+    gs_ctx ctx = nullptr;
+    uint32_t capacity = 0;
+    uint32_t key_bits = 64;
+    void* keys[2] = {nullptr, nullptr};
+    uint32_t* vals[2] = {nullptr, nullptr};
+    uint32_t* counts = nullptr;
+    uint8_t* scratch = nullptr;
+    ~gs_sorter_s() {
+        for (int i = 0; i < 2; ++i) {
+            if (keys[i]) (void)hipFree(keys[i]);
+            dfree(vals[i]);
+        }
+        dfree(counts);
+        dfree(scratch);
+    }
+};
+
+extern "C" {
+
+gs_status gs_sorter_create(gs_ctx ctx, uint32_t capacity, uint32_t key_bits, gs_sorter* out) {
+    GS_REQUIRE(ctx && out, "null argument");
+    GS_REQUIRE(key_bits == 32 || key_bits == 64, "key_bits must be 32 or 64");
+    GS_REQUIRE(capacity < (1u << 30), "sorter capacity must be < 2^30");
+    *out = nullptr;
+    auto s = std::make_unique<gs_sorter_s>();
+    s->ctx = ctx;
+    s->capacity = capacity;
+    s->key_bits = key_bits;
+    for (int i = 0; i < 2; ++i) {
+        uint8_t* k = nullptr;
+        GS_CHECK(dalloc(&k, (size_t)(capacity ? capacity : 1) * (key_bits / 8), "sorter keys"));
+        s->keys[i] = k;
+        GS_CHECK(dalloc(&s->vals[i], capacity, "sorter payloads"));
+    }
+    GS_CHECK(dalloc(&s->counts, 1, "sorter counts"));
+    GS_CHECK(dalloc(&s->scratch, sort_scratch_bytes(capacity), "sorter scratch"));
+    *out = s.release();
+    return GS_OK;
+}
+
+void gs_sorter_destroy(gs_sorter s) { delete s; }
+void* gs_sorter_keys(gs_sorter s) { return s ? s->keys[0] : nullptr; }
+uint32_t* gs_sorter_payloads(gs_sorter s) { return s ? s->vals[0] : nullptr; }
+uint32_t* gs_sorter_counts(gs_sorter s) { return s ? s->counts : nullptr; }
+
+gs_status gs_sorter_encode(gs_sorter s, void* stream, uint32_t begin_bit, uint32_t end_bit,
+                           uint32_t digit_bits) {
+    GS_REQUIRE(s != nullptr, "sorter is null");
+    GS_REQUIRE(digit_bits == 4 || digit_bits == 8, "digit_bits must be 4 or 8");
+    GS_REQUIRE(begin_bit < end_bit && end_bit <= s->key_bits, "bad bit range");
+    hipStream_t st = as_stream(stream, s->ctx->stream);
+    int alt = 0;
+    if (s->key_bits == 64)
+        GS_HIP(launch_radix_sort_u64(st, static_cast<uint64_t*>(s->keys[0]), s->vals[0],
+                                     static_cast<uint64_t*>(s->keys[1]), s->vals[1], s->counts,
+                                     s->capacity, begin_bit, end_bit, digit_bits, s->scratch, &alt));
+    else
+        GS_HIP(launch_radix_sort_u32(st, static_cast<uint32_t*>(s->keys[0]), s->vals[0],
+                                     static_cast<uint32_t*>(s->keys[1]), s->vals[1], s->counts,
+                                     s->capacity, begin_bit, end_bit, digit_bits, s->scratch, &alt));
+    if (alt)  // odd pass count: bring the result back (sort.rs:7: results land in keys())
+        GS_HIP(launch_copy_pairs(st, s->keys[1], s->vals[1], s->keys[0], s->vals[0], s->key_bits / 8,
+                                 s->counts, s->capacity));
+    return GS_OK;
+}
+
+}  // extern "C"
+
+// =============================================================== TileBinner
+
+struct gs_binner_s {
+    gs_ctx ctx = nullptr;
+    uint32_t max_items = 0, max_entries = 0, num_tiles = 0;
+    uint4* rects = nullptr;
+    uint32_t* depths = nullptr;
+    uint32_t* offsets = nullptr;
+    uint32_t* total = nullptr;
+    uint32_t* count = nullptr;
+    uint32_t* overflow = nullptr;
+    uint64_t* keys[2] = {nullptr, nullptr};
+    uint32_t* vals[2] = {nullptr, nullptr};
+    uint32_t* ranges = nullptr;
+    uint8_t* scan_scratch = nullptr;
+    uint8_t* sort_scratch = nullptr;
+    CamParams* cam = nullptr;
+    uint32_t cam_tiles_x = 0;
+    ~gs_binner_s() {
+        dfree(rects);
+        dfree(depths);
+        dfree(offsets);
+        dfree(total);
+        dfree(count);
+        dfree(overflow);
+        for (int i = 0; i < 2; ++i) {
+            dfree(keys[i]);
+            dfree(vals[i]);
+        }
+        dfree(ranges);
+        dfree(scan_scratch);
+        dfree(sort_scratch);
+        dfree(cam);
+    }
+};
+
+static uint32_t bits_for(uint32_t v) {
+    uint32_t b = 1;
+    while (b < 32 && (1ull << b) < (uint64_t)v) ++b;
+    return b;
+}
+
+extern "C" {
+
+gs_status gs_binner_create(gs_ctx ctx, uint32_t max_items, uint32_t max_entries, uint32_t num_tiles,
+                           gs_binner* out) {
+    GS_REQUIRE(ctx && out, "null argument");
+    GS_REQUIRE(max_entries < (1u << 30), "max_entries must be < 2^30");
+    GS_REQUIRE(num_tiles > 0, "num_tiles must be > 0");
+    *out = nullptr;
+    auto b = std::make_unique<gs_binner_s>();
+    b->ctx = ctx;
+    b->max_items = max_items;
+    b->max_entries = max_entries;
+    b->num_tiles = num_tiles;
+    GS_CHECK(dalloc(&b->rects, max_items, "binner rects"));
+    GS_CHECK(dalloc(&b->depths, max_items, "binner depths"));
+    GS_CHECK(dalloc(&b->offsets, max_items, "binner offsets"));
+    GS_CHECK(dalloc(&b->total, 1, "binner total"));
+    GS_CHECK(dalloc(&b->count, 1, "binner count"));
+    GS_CHECK(dalloc(&b->overflow, 1, "binner overflow"));
+    for (int i = 0; i < 2; ++i) {
+        GS_CHECK(dalloc(&b->keys[i], max_entries, "binner keys"));
+        GS_CHECK(dalloc(&b->vals[i], max_entries, "binner payloads"));
+    }
+    GS_CHECK(dalloc(&b->ranges, (size_t)num_tiles * 2, "binner ranges"));
+    GS_CHECK(dalloc(&b->scan_scratch, scan_scratch_bytes(max_items), "binner scan scratch"));
+    GS_CHECK(dalloc(&b->sort_scratch, sort_scratch_bytes(max_entries), "binner sort scratch"));
+    GS_CHECK(dalloc(&b->cam, 1, "binner cam"));
+    *out = b.release();
+    return GS_OK;
+}
+
+void gs_binner_destroy(gs_binner b) { delete b; }
+uint32_t* gs_binner_rects(gs_binner b) { return b ? reinterpret_cast<uint32_t*>(b->rects) : nullptr; }
+uint32_t* gs_binner_depths(gs_binner b) { return b ? b->depths : nullptr; }
+uint32_t* gs_binner_ranges(gs_binner b) { return b ? b->ranges : nullptr; }
+uint32_t* gs_binner_sorted_entries(gs_binner b) { return b ? b->vals[0] : nullptr; }
+uint64_t* gs_binner_sorted_keys(gs_binner b) { return b ? b->keys[0] : nullptr; }
+uint32_t* gs_binner_entry_count(gs_binner b) { return b ? b->count : nullptr; }
+
+gs_status gs_binner_encode(gs_binner b, void* stream, uint32_t num_items, uint32_t tiles_x,
+                           uint32_t digit_bits) {
+    GS_REQUIRE(b != nullptr, "binner is null");
+    if (num_items > b->max_items) return fail(GS_ERR_CAPACITY, "binner: num_items > max_items");
+    // binning.rs:181
+    GS_REQUIRE(tiles_x > 0 && b->num_tiles % tiles_x == 0, "binner: num_tiles must be a multiple of tiles_x");
+    GS_REQUIRE(digit_bits == 4 || digit_bits == 8, "digit_bits must be 4 or 8");
+    hipStream_t st = as_stream(stream, b->ctx->stream);
+    if (b->cam_tiles_x != tiles_x) {
+        CamParams c;
+        std::memset(&c, 0, sizeof(c));
+        c.tiles_x = tiles_x;
+        c.tiles_y = b->num_tiles / tiles_x;
+        GS_HIP(hipMemcpyAsync(b->cam, &c, sizeof(c), hipMemcpyHostToDevice, st));
+        GS_HIP(hipStreamSynchronize(st));
+        b->cam_tiles_x = tiles_x;
+    }
+    GS_HIP(launch_count_tiles(st, num_items, b->rects, b->offsets));
+    GS_HIP(launch_exclusive_scan(st, b->offsets, b->offsets, num_items, b->total, b->scan_scratch));
+    GS_HIP(launch_emit_keys(st, num_items, b->rects, b->depths, b->offsets, b->cam,
+                            num_items ? num_items : 1, tiles_x, b->keys[0], b->vals[0],
+                            b->max_entries, b->total, b->count, b->overflow));
+    const uint32_t end_bit = 32 + bits_for(b->num_tiles);
+    int alt = 0;
+    GS_HIP(launch_radix_sort_u64(st, b->keys[0], b->vals[0], b->keys[1], b->vals[1], b->count,
+                                 b->max_entries, 0, end_bit, digit_bits, b->sort_scratch, &alt));
+    if (alt)
+        GS_HIP(launch_copy_pairs(st, b->keys[1], b->vals[1], b->keys[0], b->vals[0], 8, b->count,
+                                 b->max_entries));
+    GS_HIP(launch_fill_u32(st, b->ranges, 0u, (size_t)b->num_tiles * 2));  // binning.rs:228
+    GS_HIP(launch_tile_ranges(st, b->keys[0], b->count, b->max_entries, b->ranges));
+    return GS_OK;
+}
+
+gs_status gs_binner_overflowed(gs_binner b, int* overflowed) {
+    GS_REQUIRE(b && overflowed, "null argument");
+    uint32_t f = 0;
+    GS_HIP(hipStreamSynchronize(b->ctx->stream));
+    GS_HIP(hipMemcpy(&f, b->overflow, 4, hipMemcpyDeviceToHost));
+    *overflowed = f ? 1 : 0;
+    const uint32_t zero = 0;
+    GS_HIP(hipMemcpy(b->overflow, &zero, 4, hipMemcpyHostToDevice));
+    return GS_OK;
+}
+
+}  // extern "C"
+
+// ============================================================== raster core
+
+namespace gs {
+
+// Device state of the projection -> binning -> raster pipeline for up to
+// cap_views views of up to cap_n surfels. Used by the single-view Rasterizer
+// handle and by the trainer's plain and batched steps.
+struct RasterCore {
+    gs_ctx ctx = nullptr;
+    uint32_t cap_n = 0, cap_views = 0, cap_entries = 0, cap_tiles = 0, cap_pixels = 0;
+    CamParams* cams_dev = nullptr;
+    CamParams* cams_pinned = nullptr;
+    hipEvent_t cams_done = nullptr;
+    bool cams_pending = false;
+    Record* recs = nullptr;
+    uint4* rects = nullptr;
+    uint32_t* depth_keys = nullptr;
+    uint32_t* offsets = nullptr;
+    uint32_t* total = nullptr;       // per view slot (plain) or [0] (batched)
+    uint32_t* sort_count = nullptr;  // per view slot
+    uint32_t* overflow = nullptr;
+    uint64_t* keys[2] = {nullptr, nullptr};
+    uint32_t* pays[2] = {nullptr, nullptr};
+    uint8_t* scan_scratch = nullptr;
+    uint8_t* sort_scratch = nullptr;
+    uint32_t* ranges = nullptr;
+    float4* out_color = nullptr;
+    float4* out_aux = nullptr;
+    float4* out_normal = nullptr;
+    uint32_t* ncontrib = nullptr;
+    float* grad_coef = nullptr;
+    float* cam_partials = nullptr;
+    uint32_t digit_bits = 8;
+    int result_alt = 0;              // which ping-pong buffer holds the sorted entries
+    // Current view set (host mirror of cams_dev).
+    std::vector<CamParams> cams;
+    uint32_t total_tiles = 0, total_pixels = 0;
+
+    ~RasterCore() {
+        if (cams_done) (void)hipEventDestroy(cams_done);
+        if (cams_pinned) (void)hipHostFree(cams_pinned);
+        dfree(cams_dev);
+        dfree(recs);
+        dfree(rects);
+        dfree(depth_keys);
+        dfree(offsets);
+        dfree(total);
+        dfree(sort_count);
+        dfree(overflow);
+        for (int i = 0; i < 2; ++i) {
+            dfree(keys[i]);
+            dfree(pays[i]);
+        }
+        dfree(scan_scratch);
+        dfree(sort_scratch);
+        dfree(ranges);
+        dfree(out_color);
+        dfree(out_aux);
+        dfree(out_normal);
+        dfree(ncontrib);
+        dfree(grad_coef);
+        dfree(cam_partials);
+    }
+
+    gs_status init(gs_ctx c, uint32_t n, uint32_t views, uint32_t entries, uint32_t tiles,
+                   uint32_t pixels) {
+        ctx = c;
+        if (entries >= (1u << 30)) return fail(GS_ERR_CAPACITY, "max_entries must be < 2^30");
+        if ((uint64_t)n * views >= (1ull << 32)) return fail(GS_ERR_CAPACITY, "views * surfels must be < 2^32");
+        cap_n = n;
+        cap_views = views;
+        cap_entries = entries;
+        cap_tiles = tiles;
+        cap_pixels = pixels;
+        const size_t nv = (size_t)n * views;
+        GS_CHECK(dalloc(&cams_dev, views, "cams"));
+        GS_HIP(hipHostMalloc((void**)&cams_pinned, sizeof(CamParams) * views, hipHostMallocDefault));
+        GS_HIP(hipEventCreateWithFlags(&cams_done, hipEventDisableTiming));
+        GS_CHECK(dalloc(&recs, nv, "records"));
+        GS_CHECK(dalloc(&rects, nv, "rects"));
+        GS_CHECK(dalloc(&depth_keys, nv, "depth keys"));
+        GS_CHECK(dalloc(&offsets, nv, "offsets"));
+        GS_CHECK(dalloc(&total, views, "totals"));
+        GS_CHECK(dalloc(&sort_count, views, "sort counts"));
+        GS_CHECK(dalloc(&overflow, 1, "overflow flag"));
+        for (int i = 0; i < 2; ++i) {
+            GS_CHECK(dalloc(&keys[i], entries, "keys"));
+            GS_CHECK(dalloc(&pays[i], entries, "payloads"));
+        }
+        GS_CHECK(dalloc(&scan_scratch, scan_scratch_bytes((uint32_t)std::min<size_t>(nv, 0xFFFFFFFFu)),
+                        "scan scratch"));
+        GS_CHECK(dalloc(&sort_scratch, sort_scratch_bytes(entries), "sort scratch"));
+        GS_CHECK(dalloc(&ranges, (size_t)tiles * 2, "ranges"));
+        GS_CHECK(dalloc(&out_color, pixels, "out_color"));
+        GS_CHECK(dalloc(&out_aux, pixels, "out_aux"));
+        GS_CHECK(dalloc(&out_normal, pixels, "out_normal"));
+        GS_CHECK(dalloc(&ncontrib, pixels, "ncontrib"));
+        GS_CHECK(dalloc(&grad_coef, nv * kGradStride, "grad_coef"));
+        uint8_t* cp = nullptr;
+        GS_CHECK(dalloc(&cp, geom_cam_partials_bytes(n, views), "camera partials"));
+        cam_partials = reinterpret_cast<float*>(cp);
+        return GS_OK;
+    }
+
+    // Lay out a view set: batched = cumulative tile / pixel / record bases,
+    // plain = every view at base 0 (processed one at a time). Uploads the
+    // table from pinned memory (the previous upload must have landed before
+    // the staging buffer is rewritten).
+    gs_status set_views(hipStream_t st, const std::vector<CamParams>& in, bool batched) {
+        if (in.size() > cap_views) return fail(GS_ERR_CAPACITY, "too many views for this capacity");
+        cams = in;
+        uint32_t tb = 0, pb = 0;
+        total_tiles = 0;
+        total_pixels = 0;
+        for (size_t v = 0; v < cams.size(); ++v) {
+            CamParams& c = cams[v];
+            const uint32_t t = c.tiles_x * c.tiles_y, p = c.width * c.height;
+            if (batched) {
+                c.tile_base = tb;
+                c.pix_base = pb;
+                c.rec_base = (uint32_t)v * cap_n;
+                tb += t;
+                pb += p;
+                total_tiles = tb;
+                total_pixels = pb;
+            } else {
+                c.tile_base = 0;
+                c.pix_base = 0;
+                c.rec_base = 0;
+                total_tiles = std::max(total_tiles, t);
+                total_pixels = std::max(total_pixels, p);
+            }
+        }
+        if (total_tiles > cap_tiles || total_pixels > cap_pixels)
+            return fail(GS_ERR_CAPACITY, "view set exceeds the tile/pixel capacity");
+        if (cams_pending) GS_HIP(hipEventSynchronize(cams_done));
+        std::memcpy(cams_pinned, cams.data(), sizeof(CamParams) * cams.size());
+        GS_HIP(hipMemcpyAsync(cams_dev, cams_pinned, sizeof(CamParams) * cams.size(),
+                              hipMemcpyHostToDevice, st));
+        GS_HIP(hipEventRecord(cams_done, st));
+        cams_pending = true;
+        return GS_OK;
+    }
+
+    const uint64_t* sorted_keys() const { return keys[result_alt]; }
+    const uint32_t* sorted_pays() const { return pays[result_alt]; }
+
+    // Forward over views [v0, v0 + nv) of the current set (nv > 1 only when
+    // the set was laid out batched). Parameters are the activated buffers.
+    gs_status forward(hipStream_t st, uint32_t v0, uint32_t nv, uint32_t n, const float4* pos,
+                      const float2* sc, const float4* q, const float* op, const float* sh) {
+        if (n > cap_n) return fail(GS_ERR_CAPACITY, "num_surfels exceeds capacity");
+        const CamParams* cd = cams_dev + v0;
+        uint32_t tiles = 0;
+        for (uint32_t v = v0; v < v0 + nv; ++v) tiles += cams[v].tiles_x * cams[v].tiles_y;
+        uint32_t max_tile_id = 0;
+        for (uint32_t v = v0; v < v0 + nv; ++v)
+            max_tile_id = std::max(max_tile_id, cams[v].tile_base + cams[v].tiles_x * cams[v].tiles_y);
+        const uint32_t items = n * nv;
+        // Records / counts are indexed view-major from this view set's first view.
+        Record* r = recs;
+        GS_HIP(launch_project(st, cd, nv, n, pos, sc, q, op, sh, r, rects, depth_keys, offsets,
+                              ranges, 2 * max_tile_id));
+        GS_HIP(launch_exclusive_scan(st, offsets, offsets, items, total + v0, scan_scratch));
+        GS_HIP(launch_emit_keys(st, items, rects, depth_keys, offsets, cd, n ? n : 1, 0, keys[0],
+                                pays[0], cap_entries, total + v0, sort_count + v0, overflow));
+        const uint32_t end_bit = 32 + bits_for(max_tile_id);
+        GS_HIP(launch_radix_sort_u64(st, keys[0], pays[0], keys[1], pays[1], sort_count + v0,
+                                     cap_entries, 0, end_bit, digit_bits, sort_scratch, &result_alt));
+        GS_HIP(launch_tile_ranges(st, keys[result_alt], sort_count + v0, cap_entries, ranges));
+        FwdOut o{out_color, out_aux, out_normal, ncontrib};
+        GS_HIP(launch_raster_fwd(st, cd, cams.data() + v0, nv, tiles, ranges, pays[result_alt], recs, o));
+        return GS_OK;
+    }
+
+    gs_status backward(hipStream_t st, uint32_t v0, uint32_t nv, uint32_t n, const float4* pos,
+                       const float2* sc, const float4* q, const float* sh, const float4* dl_dcolor,
+                       const float4* dl_dnormal, ParamGrads g, bool accumulate, bool aux) {
+        const CamParams* cd = cams_dev + v0;
+        uint32_t tiles = 0;
+        for (uint32_t v = v0; v < v0 + nv; ++v) tiles += cams[v].tiles_x * cams[v].tiles_y;
+        GS_HIP(launch_fill_u32(st, reinterpret_cast<uint32_t*>(grad_coef), 0u,
+                               (size_t)n * nv * kGradStride));
+        BwdIn in{dl_dcolor, dl_dnormal, out_color, out_aux, ncontrib};
+        GS_HIP(launch_raster_bwd(st, cd, cams.data() + v0, nv, tiles, ranges, pays[result_alt], recs, in,
+                                 grad_coef, aux));
+        GS_HIP(launch_geom_bwd(st, cd, nv, n, pos, sc, q, sh, recs, grad_coef, g, accumulate,
+                               cam_partials));
+        return GS_OK;
+    }
+};
+
+}  // namespace gs
+
+// =============================================================== Rasterizer
+
+struct gs_raster_s {
+    gs_ctx ctx = nullptr;
+    uint32_t capacity = 0, width = 0, height = 0, max_entries = 0;
+    float lambda_dist = 0.0f;
+    gs::RasterCore core;
+    float4* positions = nullptr;
+    float2* scales = nullptr;
+    float4* quats = nullptr;
+    float* opacities = nullptr;
+    float* sh = nullptr;
+    float4* dl_dcolor = nullptr;
+    float4* dl_dnormal = nullptr;
+    float4* grad_pos = nullptr;
+    float2* grad_scales = nullptr;
+    float4* grad_quat = nullptr;
+    float* grad_opacity = nullptr;
+    float* grad_sh = nullptr;
+    float* grad_cam = nullptr;
+    uint32_t last_n = 0;
+    bool have_forward = false;
+    ~gs_raster_s() {
+        dfree(positions);
+        dfree(scales);
+        dfree(quats);
+        dfree(opacities);
+        dfree(sh);
+        dfree(dl_dcolor);
+        dfree(dl_dnormal);
+        dfree(grad_pos);
+        dfree(grad_scales);
+        dfree(grad_quat);
+        dfree(grad_opacity);
+        dfree(grad_sh);
+        dfree(grad_cam);
+    }
+};
+
+static gs_status upload_scene_into(gs_ctx ctx, const gs_scene_input* s, uint32_t capacity, float4* pos,
+                                   float2* sc, float4* q, float* op, float* sh) {
+    GS_REQUIRE(s != nullptr, "scene is null");
+    if (s->n > capacity) return fail(GS_ERR_CAPACITY, "scene larger than capacity");  // rasterizer.rs:361
+    GS_REQUIRE(s->sh_coeffs >= 1 && s->sh_coeffs <= 16, "sh_coeffs must be in 1..16");
+    GS_REQUIRE(s->n == 0 || (s->positions && s->scales && s->quats && s->opacities && s->sh),
+               "scene arrays must be non-null");
+    const size_t n = s->n;
+    std::vector<float> p4(n * 4), sh48(n * kShFloats, 0.0f);
+    for (size_t i = 0; i < n; ++i) {
+        p4[4 * i] = s->positions[3 * i];
+        p4[4 * i + 1] = s->positions[3 * i + 1];
+        p4[4 * i + 2] = s->positions[3 * i + 2];
+        p4[4 * i + 3] = 1.0f;
+        std::memcpy(&sh48[i * kShFloats], s->sh + i * s->sh_coeffs * 3, sizeof(float) * s->sh_coeffs * 3);
+    }
+    hipStream_t st = ctx->stream;
+    GS_HIP(hipMemcpyAsync(pos, p4.data(), n * 16, hipMemcpyHostToDevice, st));
+    GS_HIP(hipMemcpyAsync(sc, s->scales, n * 8, hipMemcpyHostToDevice, st));
+    GS_HIP(hipMemcpyAsync(q, s->quats, n * 16, hipMemcpyHostToDevice, st));
+    GS_HIP(hipMemcpyAsync(op, s->opacities, n * 4, hipMemcpyHostToDevice, st));
+    GS_HIP(hipMemcpyAsync(sh, sh48.data(), n * kShFloats * 4, hipMemcpyHostToDevice, st));
+    GS_HIP(hipStreamSynchronize(st));
+    return GS_OK;
+}
+
+extern "C" {
+
+gs_status gs_raster_create(gs_ctx ctx, uint32_t capacity, uint32_t width, uint32_t height,
+                           uint32_t max_entries, gs_raster* out) {
+    GS_REQUIRE(ctx && out, "null argument");
+    GS_REQUIRE(width > 0 && height > 0, "image size must be > 0");
+    GS_REQUIRE((uint64_t)width * height < (1ull << 31), "image too large");
+    *out = nullptr;
+    auto r = std::make_unique<gs_raster_s>();
+    r->ctx = ctx;
+    r->capacity = capacity;
+    r->width = width;
+    r->height = height;
+    r->max_entries = max_entries;
+    const uint32_t tiles = div_up(width, kTile) * div_up(height, kTile);
+    const uint32_t px = width * height;
+    GS_CHECK(r->core.init(ctx, capacity, 1, max_entries, tiles, px));
+    GS_CHECK(dalloc(&r->positions, capacity, "positions"));
+    GS_CHECK(dalloc(&r->scales, capacity, "scales"));
+    GS_CHECK(dalloc(&r->quats, capacity, "quats"));
+    GS_CHECK(dalloc(&r->opacities, capacity, "opacities"));
+    GS_CHECK(dalloc(&r->sh, (size_t)capacity * kShFloats, "sh"));
+    GS_CHECK(dalloc(&r->dl_dcolor, px, "dl_dcolor"));
+    GS_CHECK(dalloc(&r->dl_dnormal, px, "dl_dnormal"));
+    GS_CHECK(dalloc(&r->grad_pos, capacity, "grad_pos"));
+    GS_CHECK(dalloc(&r->grad_scales, capacity, "grad_scales"));
+    GS_CHECK(dalloc(&r->grad_quat, capacity, "grad_quat"));
+    GS_CHECK(dalloc(&r->grad_opacity, capacity, "grad_opacity"));
+    GS_CHECK(dalloc(&r->grad_sh, (size_t)capacity * kShFloats, "grad_sh"));
+    GS_CHECK(dalloc(&r->grad_cam, 16, "grad_cam"));
+    *out = r.release();
+    return GS_OK;
+}
+
+void gs_raster_destroy(gs_raster r) {
+    if (!r) return;
+    (void)hipStreamSynchronize(r->ctx->stream);
+    delete r;
+}
+
+gs_status gs_raster_get_buffers(gs_raster r, gs_raster_buffers* o) {
+    GS_REQUIRE(r && o, "null argument");
+    o->positions = reinterpret_cast<float*>(r->positions);
+    o->scales = reinterpret_cast<float*>(r->scales);
+    o->quats = reinterpret_cast<float*>(r->quats);
+    o->opacities = r->opacities;
+    o->sh = r->sh;
+    o->out_color = reinterpret_cast<float*>(r->core.out_color);
+    o->out_aux = reinterpret_cast<float*>(r->core.out_aux);
+    o->out_normal = reinterpret_cast<float*>(r->core.out_normal);
+    o->out_ncontrib = r->core.ncontrib;
+    o->dl_dcolor = reinterpret_cast<float*>(r->dl_dcolor);
+    o->dl_dnormal = reinterpret_cast<float*>(r->dl_dnormal);
+    o->grad_pos = reinterpret_cast<float*>(r->grad_pos);
+    o->grad_scales = reinterpret_cast<float*>(r->grad_scales);
+    o->grad_quat = reinterpret_cast<float*>(r->grad_quat);
+    o->grad_opacity = r->grad_opacity;
+    o->grad_sh = r->grad_sh;
+    o->grad_cam = r->grad_cam;
+    return GS_OK;
+}
+
+gs_status gs_raster_upload_scene(gs_raster r, const gs_scene_input* scene) {
+    GS_REQUIRE(r != nullptr, "raster is null");
+    return upload_scene_into(r->ctx, scene, r->capacity, r->positions, r->scales, r->quats,
+                             r->opacities, r->sh);
+}
+
+gs_status gs_raster_set_lambda_dist(gs_raster r, float lambda_dist) {
+    GS_REQUIRE(r != nullptr, "raster is null");
+    GS_REQUIRE(std::isfinite(lambda_dist), "lambda_dist must be finite");
+    r->lambda_dist = lambda_dist;
+    return GS_OK;
+}
+
+gs_status gs_raster_set_digit_bits(gs_raster r, uint32_t digit_bits) {
+    GS_REQUIRE(r != nullptr, "raster is null");
+    GS_REQUIRE(digit_bits == 4 || digit_bits == 8, "digit_bits must be 4 or 8");
+    r->core.digit_bits = digit_bits;
+    return GS_OK;
+}
+
+gs_status gs_raster_forward(gs_raster r, void* stream, const gs_raster_camera* camera,
+                            uint32_t num_surfels) {
+    GS_REQUIRE(r != nullptr, "raster is null");
+    GS_CHECK(check_camera(camera));
+    if (num_surfels > r->capacity) return fail(GS_ERR_CAPACITY, "num_surfels exceeds capacity");  // rasterizer.rs:409
+    hipStream_t st = as_stream(stream, r->ctx->stream);
+    CamParams c = make_cam(*camera, r->width, r->height);
+    c.lambda_dist = r->lambda_dist;
+    GS_CHECK(r->core.set_views(st, {c}, true));
+    GS_CHECK(r->core.forward(st, 0, 1, num_surfels, r->positions, r->scales, r->quats, r->opacities, r->sh));
+    r->last_n = num_surfels;
+    r->have_forward = true;
+    return GS_OK;
+}
+
+gs_status gs_raster_backward_ex(gs_raster r, void* stream, uint32_t num_surfels, uint32_t flags) {
+    GS_REQUIRE(r != nullptr, "raster is null");
+    GS_REQUIRE(r->have_forward, "backward needs a forward on this rasterizer first");
+    GS_REQUIRE(num_surfels == r->last_n, "backward num_surfels must match the forward");
+    hipStream_t st = as_stream(stream, r->ctx->stream);
+    const bool accumulate = (flags & GS_BWD_ACCUMULATE) != 0;
+    const bool aux = (flags & GS_BWD_COLOR_ONLY) == 0;
+    ParamGrads g{r->grad_pos, r->grad_scales, r->grad_quat, r->grad_opacity, r->grad_sh,
+                 (flags & GS_BWD_NO_CAMERA) ? nullptr : r->grad_cam};
+    GS_CHECK(r->core.backward(st, 0, 1, num_surfels, r->positions, r->scales, r->quats, r->sh,
+                              r->dl_dcolor, r->dl_dnormal, g, accumulate, aux));
+    return GS_OK;
+}
+
+gs_status gs_raster_backward(gs_raster r, void* stream, uint32_t num_surfels) {
+    return gs_raster_backward_ex(r, stream, num_surfels, 0u);
+}
+
+gs_status gs_raster_entry_count(gs_raster r, uint32_t* entries) {
+    GS_REQUIRE(r && entries, "null argument");
+    GS_HIP(hipStreamSynchronize(r->ctx->stream));
+    GS_HIP(hipDeviceSynchronize());
+    uint32_t t = 0, of = 0;
+    GS_HIP(hipMemcpy(&t, r->core.total, 4, hipMemcpyDeviceToHost));
+    GS_HIP(hipMemcpy(&of, r->core.overflow, 4, hipMemcpyDeviceToHost));
+    *entries = t;
+    if (of) {
+        const uint32_t zero = 0;
+        GS_HIP(hipMemcpy(r->core.overflow, &zero, 4, hipMemcpyHostToDevice));
+        return fail(GS_ERR_OVERFLOW, "tile entries exceeded max_entries (" + std::to_string(t) + " > " +
+                                         std::to_string(r->max_entries) + ")");
+    }
+    return GS_OK;
+}
+
+gs_status gs_raster_debug_buffers(gs_raster r, gs_raster_debug* o) {
+    GS_REQUIRE(r && o, "null argument");
+    o->sorted_keys = r->core.sorted_keys();
+    o->sorted_payloads = r->core.sorted_pays();
+    o->ranges = r->core.ranges;
+    o->rects = reinterpret_cast<const uint32_t*>(r->core.rects);
+    o->depth_keys = r->core.depth_keys;
+    o->offsets = r->core.offsets;
+    o->num_tiles = div_up(r->width, kTile) * div_up(r->height, kTile);
+    return GS_OK;
+}
+
+}  // extern "C"
+
+// ================================================================= SsimLoss
+
+struct gs_loss_s {
+    gs_ctx ctx = nullptr;
+    uint32_t width = 0, height = 0;
+    float lambda = 0.2f, dssim_scale = 0.5f;
+    const float4* img = nullptr;
+    float4* dl_out = nullptr;
+    float4* target = nullptr;
+    float* coef = nullptr;
+    double* partials = nullptr;
+    uint32_t n_partials = 0;
+    ~gs_loss_s() {
+        dfree(target);
+        dfree(coef);
+        dfree(partials);
+    }
+};
+
+extern "C" {
+
+gs_status gs_loss_create(gs_ctx ctx, uint32_t width, uint32_t height, float lambda, const float* img,
+                         float* dl_out, gs_loss* out) {
+    GS_REQUIRE(ctx && out && img && dl_out, "null argument");
+    GS_REQUIRE(width > 0 && height > 0, "image size must be > 0");
+    GS_REQUIRE(lambda >= 0.0f && lambda <= 1.0f, "lambda must be in [0, 1]");
+    *out = nullptr;
+    auto l = std::make_unique<gs_loss_s>();
+    l->ctx = ctx;
+    l->width = width;
+    l->height = height;
+    l->lambda = lambda;
+    l->img = reinterpret_cast<const float4*>(img);
+    l->dl_out = reinterpret_cast<float4*>(dl_out);
+    const size_t P = (size_t)width * height;
+    GS_CHECK(dalloc(&l->target, P, "loss target"));
+    GS_CHECK(dalloc(&l->coef, P * 12, "loss coefficient maps"));
+    l->n_partials = l1_ssim_partials(width, height);
+    GS_CHECK(dalloc(&l->partials, (size_t)l->n_partials * 2, "loss partials"));
+    *out = l.release();
+    return GS_OK;
+}
+
+void gs_loss_destroy(gs_loss l) { delete l; }
+float* gs_loss_target(gs_loss l) { return l ? reinterpret_cast<float*>(l->target) : nullptr; }
+
+gs_status gs_loss_set_dssim_scale(gs_loss l, float scale) {
+    GS_REQUIRE(l != nullptr, "loss is null");
+    GS_REQUIRE(std::isfinite(scale) && scale >= 0.0f, "dssim_scale must be finite and >= 0");
+    l->dssim_scale = scale;
+    return GS_OK;
+}
+
+gs_status gs_loss_encode(gs_loss l, void* stream) {
+    GS_REQUIRE(l != nullptr, "loss is null");
+    GS_HIP(launch_l1_ssim(as_stream(stream, l->ctx->stream), l->width, l->height, l->lambda,
+                          l->dssim_scale, l->img, l->target, l->dl_out, l->coef, l->partials,
+                          l->n_partials));
+    return GS_OK;
+}
+
+static gs_status read_loss_partials(hipStream_t st, const double* partials, uint32_t n, uint32_t w,
+                                    uint32_t h, float dssim_scale, double* l1_mean, double* dssim_mean) {
+    std::vector<double> p((size_t)n * 2);
+    GS_HIP(hipStreamSynchronize(st));
+    GS_HIP(hipMemcpy(p.data(), partials, p.size() * sizeof(double), hipMemcpyDeviceToHost));
+    double l1 = 0.0, ss = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        l1 += p[2 * i];
+        ss += p[2 * i + 1];
+    }
+    const double n3 = (double)w * h * 3.0;
+    if (l1_mean) *l1_mean = l1 / n3;
+    if (dssim_mean) *dssim_mean = (double)dssim_scale * (1.0 - ss / n3);
+    return GS_OK;
+}
+
+gs_status gs_loss_read(gs_loss l, double* l1_mean, double* dssim_mean) {
+    GS_REQUIRE(l != nullptr, "loss is null");
+    return read_loss_partials(l->ctx->stream, l->partials, l->n_partials, l->width, l->height,
+                              l->dssim_scale, l1_mean, dssim_mean);
+}
+
+}  // extern "C"
+
+// ================================================================ Optimizer
+
+struct OptimClass {
+    std::string name;
+    uint32_t n = 0;
+    int activation = 0;
+    float lr = 0.0f, reg = 0.0f;
+    float* raw = nullptr;
+    float* m = nullptr;
+    float* v = nullptr;
+    const float* grads = nullptr;
+    float* activated = nullptr;
+};
+
+struct gs_optim_s {
+    gs_ctx ctx = nullptr;
+    std::vector<OptimClass> classes;
+    AdamState* state = nullptr;
+    ~gs_optim_s() {
+        for (auto& c : classes) {
+            dfree(c.raw);
+            dfree(c.m);
+            dfree(c.v);
+        }
+        dfree(state);
+    }
+    AdamTable table(uint32_t* total_blocks) const {
+        AdamTable t;
+        std::memset(&t, 0, sizeof(t));
+        uint32_t b = 0;
+        for (size_t k = 0; k < classes.size(); ++k) {
+            const OptimClass& c = classes[k];
+            t.c[k] = AdamClass{c.raw, c.grads, c.m, c.v, c.activated, c.n, c.activation, c.lr, c.reg, b};
+            b += adam_blocks_for(c.n);
+        }
+        t.n_classes = (uint32_t)classes.size();
+        *total_blocks = b;
+        return t;
+    }
+    OptimClass* find(const char* name) {
+        if (!name) return nullptr;
+        for (auto& c : classes)
+            if (c.name == name) return &c;
+        return nullptr;
+    }
+};
+
+extern "C" {
+
+gs_status gs_optim_create(gs_ctx ctx, gs_optim* out) {
+    GS_REQUIRE(ctx && out, "null argument");
+    *out = nullptr;
+    auto o = std::make_unique<gs_optim_s>();
+    o->ctx = ctx;
+    GS_CHECK(dalloc(&o->state, 1, "adam state"));
+    AdamState s0{0u, 1.0f, 1.0f, 1.0f, 1.0f};
+    GS_HIP(hipMemcpy(o->state, &s0, sizeof(s0), hipMemcpyHostToDevice));
+    *out = o.release();
+    return GS_OK;
+}
+
+void gs_optim_destroy(gs_optim o) { delete o; }
+
+gs_status gs_optim_add_class(gs_optim o, const char* name, uint32_t n, int activation, float lr,
+                             const float* grads, float* activated) {
+    GS_REQUIRE(o && name && grads && activated, "null argument");
+    GS_REQUIRE(activation >= 0 && activation <= 2, "activation must be 0, 1 or 2");
+    GS_REQUIRE(o->classes.size() < (size_t)kMaxAdamClasses, "too many optimizer classes");
+    GS_REQUIRE(o->find(name) == nullptr, "duplicate optimizer class name");
+    OptimClass c;
+    c.name = name;
+    c.n = n;
+    c.activation = activation;
+    c.lr = lr;
+    c.grads = grads;
+    c.activated = activated;
+    GS_CHECK(dalloc(&c.raw, n, "adam raw"));
+    GS_CHECK(dalloc(&c.m, n, "adam m"));
+    GS_CHECK(dalloc(&c.v, n, "adam v"));
+    o->classes.push_back(c);
+    return GS_OK;
+}
+
+float* gs_optim_class_raw(gs_optim o, const char* name) {
+    if (!o) return nullptr;
+    OptimClass* c = o->find(name);
+    return c ? c->raw : nullptr;
+}
+
+gs_status gs_optim_set_lr(gs_optim o, const char* name, float lr) {
+    GS_REQUIRE(o != nullptr, "optim is null");
+    OptimClass* c = o->find(name);
+    GS_REQUIRE(c != nullptr, "unknown optimizer class");
+    c->lr = lr;
+    return GS_OK;
+}
+
+gs_status gs_optim_set_reg(gs_optim o, const char* name, float reg) {
+    GS_REQUIRE(o != nullptr, "optim is null");
+    OptimClass* c = o->find(name);
+    GS_REQUIRE(c != nullptr, "unknown optimizer class");
+    c->reg = reg;
+    return GS_OK;
+}
+
+gs_status gs_optim_encode_step(gs_optim o, void* stream) {
+    GS_REQUIRE(o != nullptr, "optim is null");
+    hipStream_t st = as_stream(stream, o->ctx->stream);
+    uint32_t blocks = 0;
+    AdamTable t = o->table(&blocks);
+    GS_HIP(launch_adam_tick(st, o->state));
+    GS_HIP(launch_adam(st, t, blocks, o->state, true));
+    return GS_OK;
+}
+
+gs_status gs_optim_encode_activate(gs_optim o, void* stream) {
+    GS_REQUIRE(o != nullptr, "optim is null");
+    uint32_t blocks = 0;
+    AdamTable t = o->table(&blocks);
+    GS_HIP(launch_adam(as_stream(stream, o->ctx->stream), t, blocks, o->state, false));
+    return GS_OK;
+}
+
+gs_status gs_optim_step_count(gs_optim o, uint32_t* t) {
+    GS_REQUIRE(o && t, "null argument");
+    AdamState s;
+    GS_HIP(hipStreamSynchronize(o->ctx->stream));
+    GS_HIP(hipMemcpy(&s, o->state, sizeof(s), hipMemcpyDeviceToHost));
+    *t = s.t;
+    return GS_OK;
+}
+
+}  // extern "C"
+
+// ================================================================== Trainer
+
+struct TrainViewDev {
+    uint32_t width, height;
+    gs_raster_camera camera;
+    size_t target_offset;   // in float4 units
+};
+
+struct gs_trainer_s {
+    gs_ctx ctx = nullptr;
+    gs_train_config cfg{};
+    uint32_t n = 0;
+    std::vector<TrainViewDev> views;
+    float4* targets = nullptr;
+    // activated parameters + gradients (layout of gs_raster_buffers)
+    float4* positions = nullptr;
+    float2* scales = nullptr;
+    float4* quats = nullptr;
+    float* opacities = nullptr;
+    float* sh = nullptr;
+    float4* grad_pos = nullptr;
+    float2* grad_scales = nullptr;
+    float4* grad_quat = nullptr;
+    float* grad_opacity = nullptr;
+    float* grad_sh = nullptr;
+    float4* dl_dcolor = nullptr;       // cap_pixels (batched layout)
+    float* loss_coef = nullptr;        // 12 floats per pixel of the largest view
+    double* loss_partials = nullptr;   // per view slot
+    uint32_t partials_per_view = 0;
+    gs_optim optim = nullptr;
+    gs::RasterCore core;
+    uint32_t max_w = 0, max_h = 0;
+    std::vector<uint32_t> last_ids;
+    bool last_batched = false;
+    uint32_t sh_degree = 3;
+    // graph mode
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<uint32_t> graph_shape;   // (w, h) per view of the captured step
+    bool tuned = false;
+
+    ~gs_trainer_s() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (optim) gs_optim_destroy(optim);
+        dfree(targets);
+        dfree(positions);
+        dfree(scales);
+        dfree(quats);
+        dfree(opacities);
+        dfree(sh);
+        dfree(grad_pos);
+        dfree(grad_scales);
+        dfree(grad_quat);
+        dfree(grad_opacity);
+        dfree(grad_sh);
+        dfree(dl_dcolor);
+        dfree(loss_coef);
+        dfree(loss_partials);
+    }
+
+    gs_status enqueue_step(hipStream_t st, const uint32_t* ids, uint32_t k, bool batched);
+};
+
+gs_status gs_trainer_s::enqueue_step(hipStream_t st, const uint32_t* ids, uint32_t k, bool batched) {
+    ParamGrads g{grad_pos, grad_scales, grad_quat, grad_opacity, grad_sh, nullptr};
+    if (batched) {
+        GS_CHECK(core.forward(st, 0, k, n, positions, scales, quats, opacities, sh));
+        for (uint32_t j = 0; j < k; ++j) {
+            const TrainViewDev& tv = views[ids[j]];
+            const CamParams& c = core.cams[j];
+            GS_HIP(launch_l1_ssim(st, tv.width, tv.height, cfg.lambda, cfg.dssim_scale,
+                                  core.out_color + c.pix_base, targets, dl_dcolor + c.pix_base, loss_coef,
+                                  loss_partials + (size_t)j * partials_per_view * 2, partials_per_view,
+                                  core.cams_dev + j));
+        }
+        GS_CHECK(core.backward(st, 0, k, n, positions, scales, quats, sh, dl_dcolor, nullptr, g, false,
+                               false));
+    } else {
+        for (uint32_t j = 0; j < k; ++j) {
+            const TrainViewDev& tv = views[ids[j]];
+            GS_CHECK(core.forward(st, j, 1, n, positions, scales, quats, opacities, sh));
+            GS_HIP(launch_l1_ssim(st, tv.width, tv.height, cfg.lambda, cfg.dssim_scale, core.out_color,
+                                  targets, dl_dcolor, loss_coef,
+                                  loss_partials + (size_t)j * partials_per_view * 2, partials_per_view,
+                                  core.cams_dev + j));
+            GS_CHECK(core.backward(st, j, 1, n, positions, scales, quats, sh, dl_dcolor, nullptr, g,
+                                   j > 0, false));
+        }
+    }
+    GS_CHECK(gs_optim_encode_step(optim, st));
+    return GS_OK;
+}
+
+extern "C" {
+
+gs_status gs_trainer_create(gs_ctx ctx, const gs_train_config* cfg, const gs_scene_input* init,
+                            uint32_t n_views, const gs_train_view* views, gs_trainer* out) {
+    GS_REQUIRE(ctx && cfg && init && views && out, "null argument");
+    GS_REQUIRE(n_views > 0, "need at least one view");
+    GS_REQUIRE(cfg->max_views_per_step >= 1 && cfg->max_views_per_step <= 64,
+               "max_views_per_step must be in 1..64");
+    GS_REQUIRE(cfg->digit_bits == 0 || cfg->digit_bits == 4 || cfg->digit_bits == 8,
+               "digit_bits must be 0, 4 or 8");
+    GS_REQUIRE(init->n > 0, "need at least one surfel");
+    *out = nullptr;
+    auto t = std::make_unique<gs_trainer_s>();
+    t->ctx = ctx;
+    t->cfg = *cfg;
+    t->n = init->n;
+    const uint32_t n = init->n;
+    size_t toff = 0;
+    uint32_t max_tiles = 0, max_px = 0;
+    for (uint32_t v = 0; v < n_views; ++v) {
+        const gs_train_view& tv = views[v];
+        GS_REQUIRE(tv.width > 0 && tv.height > 0 && tv.target_rgba != nullptr, "bad view");
+        GS_REQUIRE(toff + (size_t)tv.width * tv.height < (1ull << 32), "targets exceed 2^32 pixels");
+        GS_CHECK(check_camera(&tv.camera));
+        t->views.push_back(TrainViewDev{tv.width, tv.height, tv.camera, toff});
+        toff += (size_t)tv.width * tv.height;
+        max_tiles = std::max(max_tiles, div_up(tv.width, kTile) * div_up(tv.height, kTile));
+        max_px = std::max(max_px, tv.width * tv.height);
+        t->max_w = std::max(t->max_w, tv.width);
+        t->max_h = std::max(t->max_h, tv.height);
+    }
+    const uint32_t V = cfg->max_views_per_step;
+    const uint64_t per_view_entries = cfg->entries_per_view ? cfg->entries_per_view : (uint64_t)n * 64;
+    const uint64_t entries = per_view_entries * V;
+    if (entries >= (1ull << 30)) return fail(GS_ERR_CAPACITY, "entries per step must be < 2^30");
+    GS_CHECK(t->core.init(ctx, n, V, (uint32_t)entries, max_tiles * V, max_px * V));
+    t->core.digit_bits = cfg->digit_bits ? cfg->digit_bits : 8;
+    GS_CHECK(dalloc(&t->targets, toff, "targets"));
+    {
+        std::vector<float> buf;
+        for (uint32_t v = 0; v < n_views; ++v) {
+            const size_t P = (size_t)views[v].width * views[v].height;
+            GS_HIP(hipMemcpy(t->targets + t->views[v].target_offset, views[v].target_rgba, P * 16,
+                             hipMemcpyHostToDevice));
+        }
+    }
+    GS_CHECK(dalloc(&t->positions, n, "positions"));
+    GS_CHECK(dalloc(&t->scales, n, "scales"));
+    GS_CHECK(dalloc(&t->quats, n, "quats"));
+    GS_CHECK(dalloc(&t->opacities, n, "opacities"));
+    GS_CHECK(dalloc(&t->sh, (size_t)n * kShFloats, "sh"));
+    GS_CHECK(dalloc(&t->grad_pos, n, "grad_pos"));
+    GS_CHECK(dalloc(&t->grad_scales, n, "grad_scales"));
+    GS_CHECK(dalloc(&t->grad_quat, n, "grad_quat"));
+    GS_CHECK(dalloc(&t->grad_opacity, n, "grad_opacity"));
+    GS_CHECK(dalloc(&t->grad_sh, (size_t)n * kShFloats, "grad_sh"));
+    GS_CHECK(dalloc(&t->dl_dcolor, (size_t)max_px * V, "dl_dcolor"));
+    GS_CHECK(dalloc(&t->loss_coef, (size_t)max_px * 12, "loss coefficient maps"));
+    t->partials_per_view = div_up(t->max_w, 16) * div_up(t->max_h, 16);
+    GS_CHECK(dalloc(&t->loss_partials, (size_t)t->partials_per_view * 2 * V, "loss partials"));
+    GS_CHECK(upload_scene_into(ctx, init, n, t->positions, t->scales, t->quats, t->opacities, t->sh));
+    t->sh_degree = 0;
+    while ((t->sh_degree + 1) * (t->sh_degree + 1) < init->sh_coeffs) ++t->sh_degree;
+    // Optimizer classes (trainer.rs:757-761) with raw = log / logit / identity
+    // of the activated init (trainer.rs:763-792).
+    GS_CHECK(gs_optim_create(ctx, &t->optim));
+    struct Cls {
+        const char* name;
+        uint32_t count;
+        int act;
+        float lr;
+        const float* grads;
+        float* activated;
+    } cls[5] = {
+        {"pos", n * 4, GS_ACT_IDENTITY, cfg->lr_pos, (const float*)t->grad_pos, (float*)t->positions},
+        {"scales", n * 2, GS_ACT_EXP, cfg->lr_scales, (const float*)t->grad_scales, (float*)t->scales},
+        {"quat", n * 4, GS_ACT_IDENTITY, cfg->lr_quat, (const float*)t->grad_quat, (float*)t->quats},
+        {"opacity", n, GS_ACT_SIGMOID, cfg->lr_opacity, t->grad_opacity, t->opacities},
+        {"sh", n * kShFloats, GS_ACT_IDENTITY, cfg->lr_sh, t->grad_sh, t->sh},
+    };
+    for (auto& c : cls) GS_CHECK(gs_optim_add_class(t->optim, c.name, c.count, c.act, c.lr, c.grads, c.activated));
+    {
+        // raw init on the host (log / logit with the reference's clamps).
+        std::vector<float> pos4(n * 4), sc(n * 2), q(init->quats, init->quats + n * 4), op(n),
+            sh48((size_t)n * kShFloats, 0.0f);
+        for (uint32_t i = 0; i < n; ++i) {
+            pos4[4 * i] = init->positions[3 * i];
+            pos4[4 * i + 1] = init->positions[3 * i + 1];
+            pos4[4 * i + 2] = init->positions[3 * i + 2];
+            pos4[4 * i + 3] = 1.0f;
+            sc[2 * i] = std::log(std::max(init->scales[2 * i], 1e-6f));
+            sc[2 * i + 1] = std::log(std::max(init->scales[2 * i + 1], 1e-6f));
+            const float o = std::min(std::max(init->opacities[i], 1e-4f), 1.0f - 1e-4f);
+            op[i] = std::log(o / (1.0f - o));
+            std::memcpy(&sh48[(size_t)i * kShFloats], init->sh + (size_t)i * init->sh_coeffs * 3,
+                        sizeof(float) * init->sh_coeffs * 3);
+        }
+        GS_HIP(hipMemcpy(gs_optim_class_raw(t->optim, "pos"), pos4.data(), pos4.size() * 4, hipMemcpyHostToDevice));
+        GS_HIP(hipMemcpy(gs_optim_class_raw(t->optim, "scales"), sc.data(), sc.size() * 4, hipMemcpyHostToDevice));
+        GS_HIP(hipMemcpy(gs_optim_class_raw(t->optim, "quat"), q.data(), q.size() * 4, hipMemcpyHostToDevice));
+        GS_HIP(hipMemcpy(gs_optim_class_raw(t->optim, "opacity"), op.data(), op.size() * 4, hipMemcpyHostToDevice));
+        GS_HIP(hipMemcpy(gs_optim_class_raw(t->optim, "sh"), sh48.data(), sh48.size() * 4, hipMemcpyHostToDevice));
+    }
+    // Activated buffers are the activation of raw (the "re-run activation"
+    // contract of the reference optimizer).
+    GS_CHECK(gs_optim_encode_activate(t->optim, ctx->stream));
+    GS_HIP(hipStreamSynchronize(ctx->stream));
+    *out = t.release();
+    return GS_OK;
+}
+
+void gs_trainer_destroy(gs_trainer t) {
+    if (!t) return;
+    (void)hipStreamSynchronize(t->ctx->stream);
+    delete t;
+}
+
+gs_status gs_trainer_step(gs_trainer t, void* stream, const uint32_t* view_ids, uint32_t k,
+                          uint32_t mode) {
+    GS_REQUIRE(t && view_ids, "null argument");
+    GS_REQUIRE(k >= 1 && k <= t->cfg.max_views_per_step, "view count out of range");
+    GS_REQUIRE(mode <= GS_STEP_GRAPH, "unknown step mode");
+    for (uint32_t j = 0; j < k; ++j) GS_REQUIRE(view_ids[j] < t->views.size(), "view id out of range");
+    hipStream_t st = as_stream(stream, t->ctx->stream);
+    const bool batched = mode != GS_STEP_PLAIN;
+    std::vector<CamParams> cams;
+    for (uint32_t j = 0; j < k; ++j) {
+        const TrainViewDev& tv = t->views[view_ids[j]];
+        gs_raster_camera c = tv.camera;
+        c.sh_degree = t->sh_degree;
+        cams.push_back(make_cam(c, tv.width, tv.height));
+        cams.back().target_base = (uint32_t)tv.target_offset;
+    }
+    t->last_ids.assign(view_ids, view_ids + k);
+    t->last_batched = batched;
+    if (mode == GS_STEP_GRAPH) {
+        // The graph bakes grid sizes, so it is valid for one sequence of view
+        // sizes; the camera table is refreshed in device memory before each
+        // replay (set_views copies it on the stream ahead of the launch).
+        std::vector<uint32_t> shape;
+        for (uint32_t j = 0; j < k; ++j) {
+            shape.push_back(t->views[view_ids[j]].width);
+            shape.push_back(t->views[view_ids[j]].height);
+        }
+        GS_CHECK(t->core.set_views(st, cams, true));
+        if (!t->graph_exec || shape != t->graph_shape) {
+            if (t->graph_exec) {
+                (void)hipGraphExecDestroy(t->graph_exec);
+                t->graph_exec = nullptr;
+            }
+            if (t->graph) {
+                (void)hipGraphDestroy(t->graph);
+                t->graph = nullptr;
+            }
+            GS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            gs_status s = t->enqueue_step(st, view_ids, k, true);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(st, &g);
+            if (s != GS_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return s;
+            }
+            if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+            t->graph = g;
+            GS_HIP(hipGraphInstantiate(&t->graph_exec, g, nullptr, nullptr, 0));
+            t->graph_shape = shape;
+        }
+        GS_HIP(hipGraphLaunch(t->graph_exec, st));
+        return GS_OK;
+    }
+    GS_CHECK(t->core.set_views(st, cams, batched));
+    return t->enqueue_step(st, view_ids, k, batched);
+}
+
+gs_status gs_trainer_last_losses(gs_trainer t, double* out, uint32_t n_ids) {
+    GS_REQUIRE(t && out, "null argument");
+    GS_REQUIRE(n_ids <= t->last_ids.size(), "more views requested than the last step had");
+    for (uint32_t j = 0; j < n_ids; ++j) {
+        const TrainViewDev& tv = t->views[t->last_ids[j]];
+        const uint32_t np = div_up(tv.width, 16) * div_up(tv.height, 16);
+        GS_CHECK(read_loss_partials(t->ctx->stream, t->loss_partials + (size_t)j * t->partials_per_view * 2,
+                                    np, tv.width, tv.height, t->cfg.dssim_scale, &out[2 * j],
+                                    &out[2 * j + 1]));
+    }
+    return GS_OK;
+}
+
+gs_status gs_trainer_render(gs_trainer t, uint32_t view, float* rgba) {
+    GS_REQUIRE(t && rgba, "null argument");
+    GS_REQUIRE(view < t->views.size(), "view id out of range");
+    const TrainViewDev& tv = t->views[view];
+    hipStream_t st = t->ctx->stream;
+    gs_raster_camera c = tv.camera;
+    c.sh_degree = t->sh_degree;
+    GS_CHECK(t->core.set_views(st, {make_cam(c, tv.width, tv.height)}, true));
+    GS_CHECK(t->core.forward(st, 0, 1, t->n, t->positions, t->scales, t->quats, t->opacities, t->sh));
+    GS_HIP(hipStreamSynchronize(st));
+    GS_HIP(hipMemcpy(rgba, t->core.out_color, (size_t)tv.width * tv.height * 16, hipMemcpyDeviceToHost));
+    return GS_OK;
+}
+
+gs_status gs_trainer_params(gs_trainer t, gs_raster_buffers* o) {
+    GS_REQUIRE(t && o, "null argument");
+    std::memset(o, 0, sizeof(*o));
+    o->positions = reinterpret_cast<float*>(t->positions);
+    o->scales = reinterpret_cast<float*>(t->scales);
+    o->quats = reinterpret_cast<float*>(t->quats);
+    o->opacities = t->opacities;
+    o->sh = t->sh;
+    o->out_color = reinterpret_cast<float*>(t->core.out_color);
+    o->out_aux = reinterpret_cast<float*>(t->core.out_aux);
+    o->out_normal = reinterpret_cast<float*>(t->core.out_normal);
+    o->out_ncontrib = t->core.ncontrib;
+    o->dl_dcolor = reinterpret_cast<float*>(t->dl_dcolor);
+    o->grad_pos = reinterpret_cast<float*>(t->grad_pos);
+    o->grad_scales = reinterpret_cast<float*>(t->grad_scales);
+    o->grad_quat = reinterpret_cast<float*>(t->grad_quat);
+    o->grad_opacity = t->grad_opacity;
+    o->grad_sh = t->grad_sh;
+    return GS_OK;
+}
+
+gs_status gs_trainer_entry_count(gs_trainer t, uint32_t* entries) {
+    GS_REQUIRE(t && entries, "null argument");
+    GS_HIP(hipStreamSynchronize(t->ctx->stream));
+    std::vector<uint32_t> tot(t->core.cap_views);
+    uint32_t of = 0;
+    GS_HIP(hipMemcpy(tot.data(), t->core.total, 4 * tot.size(), hipMemcpyDeviceToHost));
+    GS_HIP(hipMemcpy(&of, t->core.overflow, 4, hipMemcpyDeviceToHost));
+    // A batched step scans all its views at once (one total); a plain step
+    // leaves one total per view.
+    uint64_t s = 0;
+    const size_t nt = t->last_batched ? 1 : std::max<size_t>(1, t->last_ids.size());
+    for (size_t v = 0; v < nt && v < tot.size(); ++v) s += tot[v];
+    *entries = (uint32_t)std::min<uint64_t>(s, 0xFFFFFFFFu);
+    if (of) {
+        const uint32_t zero = 0;
+        GS_HIP(hipMemcpy(t->core.overflow, &zero, 4, hipMemcpyHostToDevice));
+        return fail(GS_ERR_OVERFLOW, "tile entries exceeded the binning capacity");
+    }
+    return GS_OK;
+}
+
+gs_status gs_trainer_digit_bits(gs_trainer t, uint32_t* bits) {
+    GS_REQUIRE(t && bits, "null argument");
+    *bits = t->core.digit_bits;
+    return GS_OK;
+}
+
+}  // extern "C"

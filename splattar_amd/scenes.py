@@ -1,0 +1,165 @@
+"""Synthetic surfel scenes and cameras for the benchmark configurations.
+
+Distributions follow the benchmark definitions: centres U[-1, 1]^3, two
+log-normal scales per surfel, quaternions uniform on S^3 (normalised
+N(0, 1)^4), opacity sigmoid(N(0, 1)), colour RGB U[0, 1] stored as the SH DC
+coefficient (rgb - 0.5) / C0 (surfel_prep_fwd.wgsl colour = 0.5 + C0 * dc),
+higher SH bands N(0, 0.2). Cameras use the reference conventions:
+right-handed, y up, looking down -z (crates/gs-core/src/camera.rs:40-49
+look_at) with focal 0.5 H / tan(30 deg) (the 60-degree default FOV,
+camera.rs:57, focal_px :28-32).
+
+Seeds: numpy PCG64 seeded 0x5EED0000 + config index; the target scene of a
+config jitters the centres with seed ^ 1.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+SH_C0 = 0.28209479177387814
+
+
+@dataclass
+class SceneArrays:
+    positions: np.ndarray   # (n, 3) f32
+    scales: np.ndarray      # (n, 2) f32, activated
+    quats: np.ndarray       # (n, 4) f32, xyzw
+    opacities: np.ndarray   # (n,) f32, activated
+    sh: np.ndarray          # (n, (deg+1)^2 * 3) f32, coefficient-major rgb
+    sh_degree: int
+
+    @property
+    def n(self) -> int:
+        return int(self.positions.shape[0])
+
+    def as_init(self) -> dict:
+        return dict(positions=self.positions, scales=self.scales, quats=self.quats,
+                    opacities=self.opacities, sh=self.sh)
+
+
+def make_scene(n: int, sh_degree: int, log_scale_mu: float, log_scale_sigma: float, seed: int,
+               rest_sigma: float = 0.2) -> SceneArrays:
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-1.0, 1.0, (n, 3))
+    scales = np.exp(rng.normal(log_scale_mu, log_scale_sigma, (n, 2)))
+    q = rng.normal(0.0, 1.0, (n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    op = 1.0 / (1.0 + np.exp(-rng.normal(0.0, 1.0, n)))
+    ncoef = (sh_degree + 1) ** 2
+    sh = np.zeros((n, ncoef, 3))
+    rgb = rng.uniform(0.0, 1.0, (n, 3))
+    sh[:, 0, :] = (rgb - 0.5) / SH_C0
+    if ncoef > 1:
+        sh[:, 1:, :] = rng.normal(0.0, rest_sigma, (n, ncoef - 1, 3))
+    f = np.float32
+    return SceneArrays(pos.astype(f), scales.astype(f), q.astype(f), op.astype(f),
+                       sh.reshape(n, ncoef * 3).astype(f), sh_degree)
+
+
+def jittered(scene: SceneArrays, sigma: float, seed: int) -> SceneArrays:
+    rng = np.random.default_rng(seed)
+    pos = scene.positions.astype(np.float64) + rng.normal(0.0, sigma, scene.positions.shape)
+    return SceneArrays(pos.astype(np.float32), scene.scales, scene.quats, scene.opacities, scene.sh,
+                       scene.sh_degree)
+
+
+def quat_from_mat(m: np.ndarray) -> np.ndarray:
+    """Unit quaternion (xyzw) of a rotation matrix (Shepperd's method)."""
+    t = np.trace(m)
+    if t > 0:
+        s = np.sqrt(t + 1.0) * 2
+        w = 0.25 * s
+        x = (m[2, 1] - m[1, 2]) / s
+        y = (m[0, 2] - m[2, 0]) / s
+        z = (m[1, 0] - m[0, 1]) / s
+    elif m[0, 0] > m[1, 1] and m[0, 0] > m[2, 2]:
+        s = np.sqrt(1.0 + m[0, 0] - m[1, 1] - m[2, 2]) * 2
+        w = (m[2, 1] - m[1, 2]) / s
+        x = 0.25 * s
+        y = (m[0, 1] + m[1, 0]) / s
+        z = (m[0, 2] + m[2, 0]) / s
+    elif m[1, 1] > m[2, 2]:
+        s = np.sqrt(1.0 + m[1, 1] - m[0, 0] - m[2, 2]) * 2
+        w = (m[0, 2] - m[2, 0]) / s
+        x = (m[0, 1] + m[1, 0]) / s
+        y = 0.25 * s
+        z = (m[1, 2] + m[2, 1]) / s
+    else:
+        s = np.sqrt(1.0 + m[2, 2] - m[0, 0] - m[1, 1]) * 2
+        w = (m[1, 0] - m[0, 1]) / s
+        x = (m[0, 2] + m[2, 0]) / s
+        y = (m[1, 2] + m[2, 1]) / s
+        z = 0.25 * s
+    q = np.array([x, y, z, w])
+    return q / np.linalg.norm(q)
+
+
+def look_at(eye, target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0)) -> np.ndarray:
+    """camera.rs:40-49: camera-to-world quaternion looking from eye to target."""
+    eye = np.asarray(eye, float)
+    fwd = np.asarray(target, float) - eye
+    fwd /= np.linalg.norm(fwd)
+    right = np.cross(fwd, np.asarray(up, float))
+    right /= np.linalg.norm(right)
+    up2 = np.cross(right, fwd)
+    return quat_from_mat(np.stack([right, up2, -fwd], axis=1))
+
+
+def focal_for(height: int, fov_y_deg: float = 60.0) -> float:
+    """camera.rs:28-32 focal_px with the 60-degree default FOV."""
+    return 0.5 * height / np.tan(np.deg2rad(fov_y_deg) * 0.5)
+
+
+@dataclass
+class ViewSpec:
+    width: int
+    height: int
+    center: np.ndarray
+    quat: np.ndarray
+    focal: float
+
+
+def front_view(width: int, height: int, dist: float = 3.0) -> ViewSpec:
+    eye = np.array([0.0, 0.0, -dist])
+    return ViewSpec(width, height, eye, look_at(eye), focal_for(height))
+
+
+def sphere_views(count: int, seed: int, sizes, radius: float = 3.0):
+    """Cameras on a sphere of radius 3 looking at the origin, yaw U[0, 2pi),
+    pitch U[-pi/3, pi/3]; sizes is a list of (w, h) (cycled)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        yaw = rng.uniform(0.0, 2 * np.pi)
+        pitch = rng.uniform(-np.pi / 3, np.pi / 3)
+        eye = radius * np.array([np.cos(pitch) * np.sin(yaw), np.sin(pitch), np.cos(pitch) * np.cos(yaw)])
+        w, h = sizes[k % len(sizes)]
+        out.append(ViewSpec(w, h, eye, look_at(eye), focal_for(h)))
+    return out
+
+
+def uneven_sizes(count: int, seed: int):
+    """16:9 sizes between 640x360 and 1920x1080."""
+    rng = np.random.default_rng(seed)
+    hs = rng.integers(360, 1081, count)
+    return [(int(round(h * 16 / 9)), int(h)) for h in hs]
+
+
+CONFIG_SEED = 0x5EED0000
+
+
+def config_scene(index: int, n: int | None = None) -> SceneArrays:
+    """Scene of benchmark configuration `index` (0..4); n overrides the count."""
+    seed = CONFIG_SEED + index
+    if index == 0:
+        return make_scene(n or 10_000, 0, -4.0, 0.5, seed)
+    if index in (1, 2):
+        return make_scene(n or 100_000, 3, -5.0, 0.5, seed)
+    if index == 3:
+        return make_scene(n or 1_000_000, 3, -5.0, 1.0, seed)
+    if index == 4:
+        return make_scene(n or 500_000, 3, -5.0, 0.5, seed)
+    raise ValueError(index)

@@ -1,0 +1,279 @@
+"""GPU rasterizer forward / backward parity against the FP64 oracle.
+
+Restates crates/gs-kernels/tests/raster_parity.rs: the same seeds, scene
+generator (tests/common/mod.rs) and loss weights; the oracle is evaluated on
+the f32-rounded inputs the GPU sees. Tolerances are written per assertion:
+  * image: max |gpu - fp64| <= 1e-5 except at isolated decision flips
+    (alpha vs 1/255, alpha clamp, T < 1e-4, branch choice, N5 of the
+    design notes), which are counted and bounded by the reference's own
+    3e-3 per-channel limit (raster_parity.rs:59-69);
+  * gradients: per class normwise relative error <= 1e-4, and per element
+    err <= 1e-5 + 2e-3 max(|a|,|b|) (raster_parity.rs:101-113, 219);
+    aux losses 6e-3 / camera 3e-2 per element (raster_parity.rs:251).
+Projection rects / depth keys, sorted 64-bit keys and tile ranges are
+bit-exact against the fp32 mirror.
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import announce
+from microscene import XorShift64, make_scene, scene_from_f32, camera_from_f32
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_inputs(scene):
+    n = scene.n
+    return (scene.pos.astype(np.float32), scene.scales.astype(np.float32),
+            scene.quat.astype(np.float32), scene.opacity.astype(np.float32),
+            scene.sh.reshape(n, -1).astype(np.float32))
+
+
+def _camera(cam, deg):
+    import splattar_amd as sa
+    return sa.Camera(cam.center.astype(np.float32), cam.quat.astype(np.float32), float(np.float32(cam.focal)),
+                     deg)
+
+
+def _rasterizer(gpu, scene, cam, max_entries=None):
+    import splattar_amd as sa
+    n = scene.n
+    r = sa.Rasterizer(gpu, n, cam.width, cam.height, max_entries or max(n * 64, 1024))
+    r.upload_scene(*_gpu_inputs(scene))
+    return r
+
+
+def _forward_case(gpu, seed, n, deg, size, flavor="parity"):
+    scene, cam = make_scene(seed, n, deg, size, flavor=flavor)
+    s32, c32 = scene_from_f32(scene), camera_from_f32(cam)
+    announce(f"forward seed={seed} n={n} deg={deg} size={size}")
+    r = _rasterizer(gpu, scene, cam)
+    r.forward(_camera(cam, deg), n)
+    color = r.out_color.read()
+    aux = r.out_aux.read()
+    normal = r.out_normal.read()
+    dbg = r.debug()
+    ref = oracle.render(s32, c32)
+    r.close()
+    return scene, cam, color, aux, normal, dbg, ref
+
+
+def _image_stats(got, want):
+    d = np.abs(got - want)
+    return float(d.max()), float(d.mean()), int((d > 1e-5).sum())
+
+
+@pytest.mark.parametrize("seed,n,deg,size", [(101, 20, 0, 64), (202, 20, 1, 64), (303, 12, 3, 48),
+                                             (404, 50, 1, 128)])
+def test_forward_matches_fp64_oracle(gpu, seed, n, deg, size):
+    scene, cam, color, aux, normal, dbg, ref = _forward_case(gpu, seed, n, deg, size)
+    mx, mean, over = _image_stats(color[..., :3], ref["color"])
+    amx, _, _ = _image_stats(color[..., 3], ref["alpha"])
+    dmx, _, _ = _image_stats(aux[..., 1], ref["depth"])
+    nmx, _, _ = _image_stats(normal[..., :3], ref["normal"])
+    print(f"seed {seed}: color max {mx:.2e} mean {mean:.2e} px>1e-5 {over}; alpha {amx:.2e} "
+          f"depth {dmx:.2e} normal {nmx:.2e}")
+    assert mx < 3e-3 and mean < 3e-6      # raster_parity.rs:59-69 bound; mean far tighter
+    assert over <= max(3, color.size // 1000)  # >1e-5 only at isolated decision flips
+    assert amx < 3e-3 and nmx < 3e-3
+    # T_final + sum(w) = 1 (the compositing invariant) up to f32 rounding.
+    assert np.abs(aux[..., 0] + color[..., 3] - 1.0).max() < 2e-6
+
+
+def test_projection_and_binning_bit_exact(gpu):
+    scene, cam, color, aux, normal, dbg, ref = _forward_case(gpu, 404, 50, 1, 128)
+    rot = oracle.quat_to_mat(cam.quat.astype(np.float32).astype(np.float64))
+    c32 = oracle.cam32(rot, cam.center.astype(np.float32), cam.focal, cam.width, cam.height)
+    pos4 = np.concatenate([scene.pos, np.ones((scene.n, 1))], 1).astype(np.float32)
+    rects, keys, counts = oracle.prep_f32(pos4, scene.scales.astype(np.float32),
+                                          scene.quat.astype(np.float32), c32)
+    assert np.array_equal(dbg["rects"][:scene.n], rects)
+    assert np.array_equal(dbg["depth_keys"][:scene.n], keys)
+    k, p, e = oracle.emit_keys(rects, keys, c32.tiles_x)
+    k, p = oracle.stable_sort(k, p)
+    assert dbg["entries"] == e
+    assert np.array_equal(dbg["sorted_keys"], k)
+    assert np.array_equal(dbg["sorted_payloads"], p)
+    assert np.array_equal(dbg["ranges"], oracle.tile_ranges(k, c32.tiles_x * c32.tiles_y))
+
+
+def _weights(seed, size, aux):
+    # raster_parity.rs:129-161 (rng = seed ^ 0xfeed).
+    rng = XorShift64(seed ^ 0xFEED)
+    P = size * size
+    w = np.array([[rng.uniform(-1, 1) for _ in range(3)] for _ in range(P)])
+    wd = np.array([rng.uniform(-0.3, 0.3) if aux else 0.0 for _ in range(P)])
+    wn = np.array([[rng.uniform(-0.5, 0.5) for _ in range(3)] if aux else [0.0, 0.0, 0.0]
+                   for _ in range(P)])
+    return w, wd, wn
+
+
+def _grad_case(gpu, seed, n, deg, size, aux, flags=0):
+    scene, cam = make_scene(seed, n, deg, size, flavor="parity")
+    s32, c32 = scene_from_f32(scene), camera_from_f32(cam)
+    w, wd, wn = _weights(seed, size, aux)
+    w32, wd32, wn32 = w.astype(np.float32), wd.astype(np.float32), wn.astype(np.float32)
+    lam = 0.35 if aux else 0.0
+    ref = oracle.gradients(s32, c32, w32.astype(np.float64), wd32.astype(np.float64),
+                           wn32.astype(np.float64), lam)
+    announce(f"backward seed={seed} n={n} deg={deg} size={size} aux={aux} flags={flags}")
+    r = _rasterizer(gpu, scene, cam)
+    dl = np.zeros((size, size, 4), np.float32)
+    dl[..., :3] = w32.reshape(size, size, 3)
+    dl[..., 3] = wd32.reshape(size, size)
+    dn = np.zeros((size, size, 4), np.float32)
+    dn[..., :3] = wn32.reshape(size, size, 3)
+    r.dl_dcolor.write(dl)
+    r.dl_dnormal.write(dn)
+    r.set_lambda_dist(lam)
+    r.forward(_camera(cam, deg), n)
+    r.backward(n, flags=flags)
+    ncoef = (deg + 1) ** 2
+    got = dict(pos=r.grad_pos.read()[:, :3], scales=r.grad_scales.read(), quat=r.grad_quat.read(),
+               opacity=r.grad_opacity.read(), sh=r.grad_sh.read()[:, :ncoef * 3].reshape(n, ncoef, 3),
+               cam=r.grad_cam.read())
+    r.close()
+    return scene, cam, ref, got
+
+
+def _check_class(label, gpu_v, cpu_v, rel_tol, norm_tol=1e-4):
+    g = np.asarray(gpu_v, np.float64).ravel()
+    c = np.asarray(cpu_v, np.float64).ravel()
+    err = np.abs(g - c)
+    denom = np.maximum(np.abs(g), np.abs(c))
+    bad = err > 1e-5 + rel_tol * denom
+    assert not bad.any(), f"{label}: {bad.sum()} elements off, worst {err.max():.3e}"
+    cn = np.linalg.norm(c)
+    nrm = np.linalg.norm(g - c) / cn if cn > 0 else np.linalg.norm(g)
+    print(f"  {label}: normwise rel {nrm:.2e}")
+    assert nrm <= norm_tol, f"{label}: normwise relative error {nrm:.3e}"
+    return nrm
+
+
+def _check_all(scene, cam, ref, got, rel_tol, cam_tol, norm_tol=1e-4):
+    _check_class("pos", got["pos"], ref.pos, rel_tol, norm_tol)
+    _check_class("scales", got["scales"], ref.scales, rel_tol, norm_tol)
+    _check_class("quat", got["quat"], ref.quat, rel_tol, norm_tol)
+    _check_class("opacity", got["opacity"], ref.opacity, rel_tol, norm_tol)
+    _check_class("sh", got["sh"], ref.sh, rel_tol, norm_tol)
+    gc = got["cam"]
+    _check_class("cam_center", gc[9:12], ref.cam_center, cam_tol, max(norm_tol, 1e-3))
+    _check_class("focal", gc[12:13], [ref.focal], cam_tol, max(norm_tol, 1e-3))
+    dl_dr = gc[:9].reshape(3, 3).T.astype(np.float64)  # column-major -> dl_dr[j, k]
+    q = oracle.quat_grad(cam.quat.astype(np.float32).astype(np.float64), dl_dr)
+    _check_class("cam_quat", q, ref.cam_quat, cam_tol, max(norm_tol, 1e-3))
+
+
+@pytest.mark.parametrize("seed,n,deg,size", [(111, 15, 0, 48), (222, 15, 1, 48), (333, 10, 3, 40)])
+def test_backward_matches_fp64_oracle(gpu, seed, n, deg, size):
+    scene, cam, ref, got = _grad_case(gpu, seed, n, deg, size, aux=False)
+    _check_all(scene, cam, ref, got, 2e-3, 2e-3)
+
+
+@pytest.mark.parametrize("seed,n,deg,size", [(444, 12, 0, 40), (555, 12, 1, 40)])
+def test_backward_aux_losses_match_fp64_oracle(gpu, seed, n, deg, size):
+    scene, cam, ref, got = _grad_case(gpu, seed, n, deg, size, aux=True)
+    _check_all(scene, cam, ref, got, 6e-3, 3e-2, norm_tol=1e-3)
+
+
+def test_color_only_kernel_matches_full_kernel(gpu):
+    import splattar_amd as sa
+    _, _, ref, full = _grad_case(gpu, 222, 15, 1, 48, aux=False)
+    _, _, _, fast = _grad_case(gpu, 222, 15, 1, 48, aux=False, flags=sa._lib.GS_BWD_COLOR_ONLY)
+    for k in ("pos", "scales", "quat", "opacity", "sh"):
+        np.testing.assert_allclose(fast[k], full[k], rtol=1e-5, atol=1e-7)
+
+
+def test_backward_accumulates(gpu):
+    import splattar_amd as sa
+    scene, cam = make_scene(111, 15, 0, 48, flavor="parity")
+    w, _, _ = _weights(111, 48, False)
+    r = _rasterizer(gpu, scene, cam)
+    dl = np.zeros((48, 48, 4), np.float32)
+    dl[..., :3] = w.reshape(48, 48, 3)
+    r.dl_dcolor.write(dl)
+    announce("backward accumulate")
+    r.forward(_camera(cam, 0), 15)
+    r.backward(15, flags=sa._lib.GS_BWD_COLOR_ONLY)
+    one = r.grad_pos.read().copy()
+    r.backward(15, flags=sa._lib.GS_BWD_COLOR_ONLY | sa._lib.GS_BWD_ACCUMULATE)
+    two = r.grad_pos.read()
+    np.testing.assert_allclose(two, 2 * one, rtol=1e-5, atol=1e-8)
+    r.close()
+
+
+def _dense_scene(n, size, deg, seed):
+    """Many small surfels: tile lists longer than one 256-record LDS batch,
+    early termination, off-screen and behind-camera surfels."""
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-1, 1, (n, 3))
+    pos[: n // 50, 2] = -3.5  # behind the camera at z = -3 (culled)
+    scales = np.exp(rng.normal(-3.3, 0.5, (n, 2)))
+    q = rng.normal(size=(n, 4))
+    op = 1 / (1 + np.exp(-rng.normal(0.5, 1.0, n)))
+    ncoef = (deg + 1) ** 2
+    sh = rng.normal(0, 0.3, (n, ncoef, 3))
+    scene = oracle.Scene(pos, scales, q, op, sh, deg)
+    from splattar_amd.scenes import look_at, focal_for
+    eye = np.array([0.0, 0.0, -3.0])
+    cam = oracle.Camera(eye, look_at(eye), focal_for(size), size, size)
+    return scene, cam
+
+
+def test_dense_scene_forward_backward(gpu):
+    n, size, deg = 4000, 96, 2
+    scene, cam = _dense_scene(n, size, deg, 5)
+    s32, c32 = scene_from_f32(scene), camera_from_f32(cam)
+    ref = oracle.render(s32, c32)
+    rng = np.random.default_rng(9)
+    w = rng.uniform(-1, 1, (size * size, 3)).astype(np.float32)
+    refg = oracle.gradients(s32, c32, w.astype(np.float64))
+    announce(f"dense forward/backward n={n} size={size}")
+    r = _rasterizer(gpu, scene, cam)
+    dl = np.zeros((size, size, 4), np.float32)
+    dl[..., :3] = w.reshape(size, size, 3)
+    r.dl_dcolor.write(dl)
+    r.forward(_camera(cam, deg), n)
+    color = r.out_color.read()
+    ranges = r.debug()["ranges"]
+    assert (ranges[:, 1] - ranges[:, 0]).max() > 256  # multi-batch tiles exercised
+    mx, mean, over = _image_stats(color[..., :3], ref["color"])
+    print(f"dense: color max {mx:.2e} mean {mean:.2e} px>1e-5 {over}")
+    assert mean < 2e-5 and over <= color.size // 200
+    r.backward(n, flags=0)
+    gpos = r.grad_pos.read()[:, :3]
+    gop = r.grad_opacity.read()
+    for label, g, c in (("pos", gpos, refg.pos), ("opacity", gop, refg.opacity)):
+        nrm = np.linalg.norm(g - c) / np.linalg.norm(c)
+        print(f"  dense {label}: normwise rel {nrm:.2e}")
+        assert nrm < 2e-3
+    r.close()
+
+
+def test_overflow_reported(gpu):
+    import splattar_amd as sa
+    scene, cam = make_scene(404, 50, 1, 128, flavor="parity")
+    r = _rasterizer(gpu, scene, cam, max_entries=16)
+    announce("forward overflow")
+    r.forward(_camera(cam, 1), 50)
+    with pytest.raises(sa.GsError) as ei:
+        r.entry_count()
+    assert ei.value.status == 5
+    r.close()
+
+
+def test_invalid_arguments_rejected(gpu):
+    import splattar_amd as sa
+    scene, cam = make_scene(101, 20, 0, 64, flavor="parity")
+    r = _rasterizer(gpu, scene, cam)
+    with pytest.raises(sa.GsError):
+        r.forward(_camera(cam, 0), 21)          # beyond capacity
+    bad = _camera(cam, 0)
+    bad.focal = 0.0
+    with pytest.raises(sa.GsError):
+        r.forward(bad, 20)
+    with pytest.raises(sa.GsError):
+        r.backward(20)                           # backward before any forward
+    r.close()
