@@ -1,0 +1,260 @@
+#!/usr/bin/env python3
+"""Benchmark of the differentiable 2D-Gaussian-surfel training step on one MI355X.
+
+Workload (default): 1M surfels, SH degree 3, heavy-tailed log-normal scales
+(mu -5, sigma 1), 4 views of 1920x1080 per step drawn from 16 cameras on a
+sphere of radius 3, loss 0.8 L1 + 0.2 (1 - SSIM) against targets rendered
+from a jittered copy of the scene, one Adam step per training step.
+
+A "step" = project, scan, emit keys, 64-bit radix sort, tile ranges,
+rasterize forward, L1 + D-SSIM, rasterize backward, geometry backward and
+fused Adam over the 4 views. `value` is steps/s of the batched step captured
+in a hipGraph; the plain per-view step (every kernel launched per view, no
+graph) is timed beside it. Inputs are resident in HBM before timing starts.
+
+Roofline: the dominant stage's algorithmic bytes (per-stage formulas of the
+design notes) divided by its duration measured with HIP events on the step
+stream (profiled batched steps after the timed region).
+CPU baseline: the FP64 oracle (oracle/, OpenMP tile-binned mode) on a
+bounded sample -- one view's forward + loss + backward over a centred crop,
+scaled to a full step -- timed on this host.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "train steps/s, 1M surfels @1080p, batched vs plain at equal loss; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--n", type=int, default=1_000_000, help="surfels")
+    p.add_argument("--width", type=int, default=1920)
+    p.add_argument("--height", type=int, default=1080)
+    p.add_argument("--views-per-step", type=int, default=4)
+    p.add_argument("--num-views", type=int, default=16)
+    p.add_argument("--plain-steps", type=int, default=None)
+    p.add_argument("--profile-steps", type=int, default=3)
+    p.add_argument("--digit-bits", type=int, default=8)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-crop", default="480x270")
+    return p.parse_args()
+
+
+def stage_bytes(n, V, E, P, T, sort_passes, S=232):
+    """Algorithmic HBM bytes per stage for one step over V views (E, P, T
+    summed over the views): compulsory traffic only."""
+    return {
+        "project": V * n * (S + 60),
+        "scan": 8 * V * n,
+        "emit_keys": 24 * V * n + 12 * E,
+        "sort": (8 + 24 * sort_passes) * E,
+        "tile_ranges": 8 * E + 8 * T,
+        "raster_fwd": 56 * E + 20 * P,
+        "l1_ssim": 36 * P,
+        "grad_clear": 80 * V * n,
+        "raster_bwd": 56 * E + 20 * P + 52 * V * n,
+        "geom_bwd": V * n * (2 * S + 52),
+        "adam": 7 * S * n,
+    }
+
+
+def cpu_baseline(scene, spec, crop, views_per_step):
+    import oracle
+    n = scene.n
+    deg = scene.sh_degree
+    ncoef = (deg + 1) ** 2
+    s64 = oracle.Scene(scene.positions.astype(np.float64), scene.scales.astype(np.float64),
+                       scene.quats.astype(np.float64), scene.opacities.astype(np.float64),
+                       scene.sh.astype(np.float64).reshape(n, ncoef, 3), deg)
+    W, H = spec.width, spec.height
+    cw, ch = crop
+    x0, y0 = (W - cw) // 2, (H - ch) // 2
+
+    def cam(c):
+        return oracle.Camera(np.asarray(spec.center, np.float64), np.asarray(spec.quat, np.float64),
+                             float(spec.focal), W, H, c)
+
+    # Fixed per-view cost (prepare: projection + SH + depth sort; tile lists).
+    t0 = time.perf_counter()
+    oracle.render(s64, cam((x0, y0, 1, 1)))
+    oracle.gradients(s64, cam((x0, y0, 1, 1)), np.zeros((1, 3)))
+    t_fixed = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    img = oracle.render(s64, cam((x0, y0, cw, ch)))["color"]
+    tgt = np.clip(img + 0.01, 0, 1)
+    _, dl, _, _ = oracle.l1_ssim(img, tgt, 0.2, 1.0)
+    oracle.gradients(s64, cam((x0, y0, cw, ch)), dl.reshape(-1, 3))
+    t_crop = time.perf_counter() - t0
+    raw = np.zeros(n * 58, np.float32)
+    g = np.full(n * 58, 1e-3, np.float32)
+    m = np.zeros_like(raw)
+    v = np.zeros_like(raw)
+    t0 = time.perf_counter()
+    oracle.adam_step_f32(raw, g, m, v, 1e-3, 0, 1)
+    t_adam = time.perf_counter() - t0
+    scale = (W * H) / float(cw * ch)
+    t_view = t_fixed + max(t_crop - t_fixed, 0.0) * scale
+    t_step = views_per_step * t_view + t_adam
+    return {"value": 1.0 / t_step, "unit": "steps/s", "cores": oracle.num_threads(), "kind": "port",
+            "sample": (f"FP64 oracle (tile-binned, OpenMP) on view 0: fixed per-view cost {t_fixed:.2f}s + "
+                       f"centred {cw}x{ch} crop fwd+L1/SSIM+bwd {t_crop:.2f}s scaled x{scale:.1f} to "
+                       f"{W}x{H}; x{views_per_step} views + f32 Adam over {n}x58 params "
+                       f"{t_adam:.2f}s = {t_step:.1f}s/step"),
+            "sample_seconds": round(t_fixed + t_crop + t_adam, 2)}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on exactly one GPU (asked for --gpus {a.gpus}); multi-GPU sharding is "
+              "described in DESIGN.md and not built.", file=sys.stderr)
+        sys.exit(2)
+    import splattar_amd as sa
+    from splattar_amd import scenes
+
+    ctx = sa.GpuContext(0)
+    name, cus = ctx.device_info()
+    log(f"device {name}, {cus} CUs")
+    n, W, H, V = a.n, a.width, a.height, a.views_per_step
+    scene = scenes.config_scene(3, n)
+    target_scene = scenes.jittered(scene, 0.01, (scenes.CONFIG_SEED + 3) ^ 1)
+    specs = scenes.sphere_views(a.num_views, scenes.CONFIG_SEED + 3, [(W, H)])
+    deg = scene.sh_degree
+
+    # Targets rendered on the GPU from the jittered scene (setup, untimed).
+    t0 = time.perf_counter()
+    cap_entries = min(64 * n, (1 << 30) // 2)
+    r = sa.Rasterizer(ctx, n, W, H, cap_entries)
+    r.upload_scene(target_scene.positions, target_scene.scales, target_scene.quats,
+                   target_scene.opacities, target_scene.sh)
+    targets, ents = [], []
+    for sp in specs:
+        r.forward(sa.Camera(sp.center, sp.quat, sp.focal, deg), n)
+        ents.append(r.entry_count())
+        targets.append(r.out_color.read().copy())
+    r.close()
+    log(f"targets rendered in {time.perf_counter() - t0:.1f}s; entries per view "
+        f"min {min(ents)} mean {int(np.mean(ents))} max {max(ents)}")
+
+    extent = float(np.linalg.norm(scene.positions.max(0) - scene.positions.min(0)) * 0.5)
+    per_view = int(max(ents) * 1.3) + 1024
+    cfg = sa.TrainConfig(lam=0.2, dssim_scale=1.0, lr_pos=1.6e-4 * extent, lr_scales=5e-3, lr_quat=1e-3,
+                         lr_opacity=5e-2, lr_sh=2.5e-3, max_views_per_step=V, entries_per_view=per_view,
+                         digit_bits=a.digit_bits)
+    views = [(sa.Camera(sp.center, sp.quat, sp.focal, deg), t) for sp, t in zip(specs, targets)]
+    del targets
+
+    def ids_for(step):
+        return [(step * V + j) % a.num_views for j in range(V)]
+
+    def run(trainer, mode, steps, start):
+        for s in range(steps):
+            trainer.step(ids_for(start + s), mode=mode)
+
+    # ---- batched + graph (the headline)
+    tr = sa.Trainer(ctx, cfg, scene.as_init(), views)
+    run(tr, tr.GRAPH, a.warmup, 0)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    run(tr, tr.GRAPH, a.steps, a.warmup)
+    ctx.synchronize()
+    t_graph = time.perf_counter() - t0
+    graph_losses = tr.last_losses(V)
+    entries_step = tr.entry_count()
+    log(f"graph: {a.steps} steps in {t_graph:.3f}s -> {a.steps / t_graph:.2f} steps/s; "
+        f"entries/step {entries_step}")
+
+    # ---- batched without graph, then profiled batched steps (stage times)
+    run(tr, tr.BATCHED, 1, 0)
+    ctx.synchronize()
+    kb = max(2, a.steps // 4)
+    t0 = time.perf_counter()
+    run(tr, tr.BATCHED, kb, 1)
+    ctx.synchronize()
+    t_batched = (time.perf_counter() - t0) / kb
+    tr.set_profiling(True)
+    run(tr, tr.BATCHED, a.profile_steps, 3)
+    stages, psteps = tr.stage_times()
+    tr.set_profiling(False)
+    stage_ms = {k: v / max(psteps, 1) for k, v in stages.items()}
+    log("stage ms/step (batched): " + ", ".join(f"{k} {v:.3f}" for k, v in stage_ms.items()))
+
+    # ---- plain per-view steps (same views, same math, no graph)
+    kp = a.plain_steps if a.plain_steps is not None else max(2, a.steps // 4)
+    run(tr, tr.PLAIN, 1, 0)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    run(tr, tr.PLAIN, kp, 1)
+    ctx.synchronize()
+    t_plain = (time.perf_counter() - t0) / kp
+    tr.set_profiling(True)
+    run(tr, tr.PLAIN, 1, 2)
+    plain_stages, _ = tr.stage_times()
+    tr.set_profiling(False)
+    tr.close()
+
+    ms_graph = 1000.0 * t_graph / a.steps
+    value = a.steps / t_graph
+    # Roofline of the dominant kernel.
+    P = V * W * H
+    T = V * ((W + 15) // 16) * ((H + 15) // 16)
+    import math
+    tile_bits = max(1, math.ceil(math.log2(T)))
+    passes = math.ceil((32 + tile_bits) / a.digit_bits)
+    sb = stage_bytes(n, V, entries_step, P, T, passes)
+    dom = max(stage_ms, key=stage_ms.get)
+    achieved = sb[dom] / (stage_ms[dom] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
+                "ms_per_launch": round(stage_ms[dom], 4), "algorithmic_bytes": int(sb[dom])}
+    total_bytes = sum(sb.values())
+    result = {
+        "metric": METRIC, "value": round(value, 4), "unit": "steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(ms_graph, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"C3: {n} surfels SH3 heavy-tailed scales, {V}x{W}x{H} views/step, "
+                               f"L1+SSIM, Adam", "surfels": n, "views_per_step": V, "width": W,
+                   "height": H, "entries_per_step": entries_step, "digit_bits": a.digit_bits,
+                   "parallelism": "single-gpu"},
+        "plain_steps_per_s": round(1.0 / t_plain, 4),
+        "batched_nograph_steps_per_s": round(1.0 / t_batched, 4),
+        "batched_vs_plain": round(value * t_plain, 3),
+        "step_hbm_frac": round(total_bytes / (ms_graph * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+        "mpix_per_s": round(P * value / 1e6, 2),
+        "sorted_keys_per_s": round(entries_step / (stage_ms["sort"] * 1e-3), 1),
+        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "plain_stage_ms": {k: round(v, 4) for k, v in plain_stages.items()},
+        "losses_l1_dssim": graph_losses.round(6).tolist(),
+        "roofline": roofline,
+    }
+    if not a.no_cpu_baseline:
+        cw, chh = (int(x) for x in a.cpu_crop.split("x"))
+        log("timing CPU oracle baseline ...")
+        result["cpu_baseline"] = cpu_baseline(scene, specs[0], (cw, chh), V)
+    else:
+        result["cpu_baseline"] = None
+    ctx.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

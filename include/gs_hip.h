@@ -297,6 +297,28 @@ gs_status gs_trainer_entry_count(gs_trainer t, uint32_t* entries);
 /* Selected radix digit width after autotuning (4 or 8). */
 gs_status gs_trainer_digit_bits(gs_trainer t, uint32_t* bits);
 
+/* Per-stage timing (no reference counterpart; the reference's GpuTimer
+ * timestamp scopes, crates/gs-wgpu/src/profile.rs:16-133): with profiling
+ * on, plain and batched steps record a HIP event on their stream before
+ * each stage; gs_trainer_stage_times (blocking) returns the summed
+ * milliseconds per stage since the last call and the number of profiled
+ * steps, then resets. Graph steps are not instrumented. */
+#define GS_STAGE_PROJECT 0u
+#define GS_STAGE_SCAN 1u
+#define GS_STAGE_EMIT 2u
+#define GS_STAGE_SORT 3u
+#define GS_STAGE_RANGES 4u
+#define GS_STAGE_RASTER_FWD 5u
+#define GS_STAGE_LOSS 6u
+#define GS_STAGE_GRAD_CLEAR 7u
+#define GS_STAGE_RASTER_BWD 8u
+#define GS_STAGE_GEOM_BWD 9u
+#define GS_STAGE_ADAM 10u
+#define GS_STAGE_COUNT 11u
+const char* gs_stage_name(uint32_t stage);
+gs_status gs_trainer_set_profiling(gs_trainer t, int enable);
+gs_status gs_trainer_stage_times(gs_trainer t, double* ms, uint32_t* steps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,7 +39,8 @@ class _Scene(C.Structure):
 
 class _Camera(C.Structure):
     _fields_ = [("center", C.c_double * 3), ("quat", C.c_double * 4), ("focal", C.c_double),
-                ("width", C.c_int32), ("height", C.c_int32)]
+                ("width", C.c_int32), ("height", C.c_int32), ("crop_x0", C.c_int32),
+                ("crop_y0", C.c_int32), ("crop_w", C.c_int32), ("crop_h", C.c_int32)]
 
 
 class _Grads(C.Structure):
@@ -138,10 +139,19 @@ class Camera:
     focal: float
     width: int
     height: int
+    crop: tuple | None = None   # (x0, y0, w, h) pixel window; None = whole image
 
     def copy(self) -> "Camera":
         return Camera(self.center.copy(), self.quat.copy(), float(self.focal), self.width,
-                      self.height)
+                      self.height, self.crop)
+
+    @property
+    def out_size(self) -> tuple[int, int]:
+        """(width, height) of the evaluated window."""
+        if self.crop is None:
+            return self.width, self.height
+        x0, y0, w, h = self.crop
+        return min(w, self.width - x0), min(h, self.height - y0)
 
     def _c(self):
         c = _Camera()
@@ -152,6 +162,8 @@ class Camera:
         c.focal = float(self.focal)
         c.width = int(self.width)
         c.height = int(self.height)
+        if self.crop is not None:
+            c.crop_x0, c.crop_y0, c.crop_w, c.crop_h = (int(v) for v in self.crop)
         return c
 
 
@@ -201,7 +213,8 @@ def sh_basis_grad(degree: int, d) -> np.ndarray:
 
 def render(scene: Scene, cam: Camera, mode: int = 1, with_hits: bool = False):
     """forward.rs:207-246. Returns dict color (H,W,3), alpha, depth, normal."""
-    P = cam.width * cam.height
+    W, H = cam.out_size
+    P = W * H
     color = np.zeros(P * 3)
     alpha = np.zeros(P)
     depth = np.zeros(P)
@@ -213,7 +226,6 @@ def render(scene: Scene, cam: Camera, mode: int = 1, with_hits: bool = False):
                            _ptr(depth, _dp), _ptr(normal, _dp), _ptr(hits, _i64p))
     if rc != 0:
         raise RuntimeError("oref_render failed")
-    H, W = cam.height, cam.width
     out = dict(color=color.reshape(H, W, 3), alpha=alpha.reshape(H, W),
                depth=depth.reshape(H, W), normal=normal.reshape(H, W, 3))
     if with_hits:

@@ -359,6 +359,18 @@ TileLists build_tile_lists(const std::vector<SurfelCam>& surfels, const oref_cam
     return tl;
 }
 
+// Pixel window evaluated by render / gradients (the whole image unless the
+// camera carries a crop).
+struct Window {
+    int x0, y0, w, h;
+};
+inline Window window_of(const oref_camera* cam) {
+    if (cam->crop_w <= 0 || cam->crop_h <= 0) return {0, 0, cam->width, cam->height};
+    int x0 = std::max(0, std::min(cam->crop_x0, cam->width - 1));
+    int y0 = std::max(0, std::min(cam->crop_y0, cam->height - 1));
+    return {x0, y0, std::min(cam->crop_w, cam->width - x0), std::min(cam->crop_h, cam->height - y0)};
+}
+
 // Calls f(list_idx) for every surfel covers_pixel() admits at (x, y), in
 // prepare() order. Literal mode tests every surfel (forward.rs:224-227).
 template <class F>
@@ -599,14 +611,15 @@ int oref_render(const oref_scene* s, const oref_camera* cam, int mode, double* c
         tl = build_tile_lists(surfels, cam);
         tlp = &tl;
     }
-    const int W = cam->width, H = cam->height;
+    const Window win = window_of(cam);
     int64_t hits = 0;
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : hits) if (mode == 1)
-    for (int y = 0; y < H; ++y) {
-        for (int x = 0; x < W; ++x) {
+    for (int yy = 0; yy < win.h; ++yy) {
+        for (int xx = 0; xx < win.w; ++xx) {
+            const int x = win.x0 + xx, y = win.y0 + yy;
             V3 d = pixel_ray(cam, x, y);
             double pix_x = x + 0.5, pix_y = y + 0.5;
-            int64_t i = (int64_t)y * W + x;
+            int64_t i = (int64_t)yy * win.w + xx;
             double transmittance = 1.0, a_acc = 0.0, d_acc = 0.0;
             V3 c_acc, n_acc;
             walk_pixel(surfels, tlp, x, y, [&](int64_t li) {
@@ -688,20 +701,21 @@ int oref_gradients(const oref_scene* s, const oref_camera* cam, int mode,
         tl = build_tile_lists(surfels, cam);
         tlp = &tl;
     }
-    const int W = cam->width, H = cam->height;
+    const Window win = window_of(cam);
     // Per-thread accumulators, reduced in thread order (literal mode is
     // single-threaded and so accumulates in exactly the reference's order).
     int nthreads = mode == 1 ? threads_available() : 1;
     std::vector<Workspace> ws(nthreads);
     for (auto& w : ws) w.acc.assign(surfels.size(), CamSpaceAcc());
 #pragma omp parallel for schedule(static, 1) num_threads(nthreads) if (mode == 1)
-    for (int y = 0; y < H; ++y) {
+    for (int yy = 0; yy < win.h; ++yy) {
         int tid = 0;
 #ifdef _OPENMP
         tid = omp_get_thread_num();
 #endif
-        for (int x = 0; x < W; ++x) {
-            int64_t pi = (int64_t)y * W + x;
+        for (int xx = 0; xx < win.w; ++xx) {
+            const int x = win.x0 + xx, y = win.y0 + yy;
+            int64_t pi = (int64_t)yy * win.w + xx;
             V3 dlc(dl_dcolor[3 * pi], dl_dcolor[3 * pi + 1], dl_dcolor[3 * pi + 2]);
             double dld = dl_ddepth ? dl_ddepth[pi] : 0.0;
             V3 dln = dl_dnormal ? V3(dl_dnormal[3 * pi], dl_dnormal[3 * pi + 1], dl_dnormal[3 * pi + 2])

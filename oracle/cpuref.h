@@ -62,6 +62,11 @@ typedef struct {
     double focal;
     int32_t width;
     int32_t height;
+    /* Optional pixel window [x0, x0+w) x [y0, y0+h) of the full image (all 0
+     * = whole image). Outputs and dL/d(image) inputs are window-sized; the
+     * model (principal point, tile rects) stays that of the full image. Used
+     * to time a bounded sample for the CPU baseline. */
+    int32_t crop_x0, crop_y0, crop_w, crop_h;
 } oref_camera;
 
 typedef struct {

@@ -139,7 +139,11 @@ _PROTOS = {
     "gs_trainer_params": ([_vp, C.POINTER(RasterBuffers)], C.c_int),
     "gs_trainer_entry_count": ([_vp, C.POINTER(_u32)], C.c_int),
     "gs_trainer_digit_bits": ([_vp, C.POINTER(_u32)], C.c_int),
+    "gs_stage_name": ([_u32], C.c_char_p),
+    "gs_trainer_set_profiling": ([_vp, C.c_int], C.c_int),
+    "gs_trainer_stage_times": ([_vp, C.POINTER(C.c_double), C.POINTER(_u32)], C.c_int),
 }
+GS_STAGE_COUNT = 11
 
 _lib = None
 

@@ -460,6 +460,17 @@ class Trainer:
         check(lib().gs_trainer_digit_bits(self.h, C.byref(b)))
         return int(b.value)
 
+    def set_profiling(self, enable: bool):
+        check(lib().gs_trainer_set_profiling(self.h, 1 if enable else 0))
+
+    def stage_times(self) -> tuple[dict, int]:
+        """Summed ms per stage since the last call, and the profiled step count."""
+        ms = (C.c_double * L.GS_STAGE_COUNT)()
+        steps = C.c_uint32()
+        check(lib().gs_trainer_stage_times(self.h, ms, C.byref(steps)))
+        names = [lib().gs_stage_name(i).decode() for i in range(L.GS_STAGE_COUNT)]
+        return dict(zip(names, ms[:])), int(steps.value)
+
     def close(self):
         if self.h:
             lib().gs_trainer_destroy(self.h)

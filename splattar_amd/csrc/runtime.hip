@@ -466,6 +466,46 @@ gs_status gs_binner_overflowed(gs_binner b, int* overflowed) {
 
 namespace gs {
 
+// Stage timing with HIP events recorded on the step's stream: mark(stage)
+// before each stage, mark(-1) at the end; collect() attributes the time
+// between consecutive marks to the earlier mark's stage.
+struct Profiler {
+    bool enabled = false;
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<size_t, int>> marks;
+    double acc[GS_STAGE_COUNT] = {};
+    uint32_t steps = 0;
+
+    ~Profiler() {
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
+    gs_status mark(hipStream_t st, int stage) {
+        if (!enabled) return GS_OK;
+        const size_t idx = marks.size();
+        if (idx == pool.size()) {
+            hipEvent_t e;
+            GS_HIP(hipEventCreate(&e));
+            pool.push_back(e);
+        }
+        GS_HIP(hipEventRecord(pool[idx], st));
+        marks.emplace_back(idx, stage);
+        return GS_OK;
+    }
+    gs_status collect() {
+        if (marks.empty()) return GS_OK;
+        GS_HIP(hipEventSynchronize(pool[marks.back().first]));
+        for (size_t i = 0; i + 1 < marks.size(); ++i) {
+            const int s = marks[i].second;
+            if (s < 0) continue;
+            float ms = 0.0f;
+            GS_HIP(hipEventElapsedTime(&ms, pool[marks[i].first], pool[marks[i + 1].first]));
+            acc[s] += ms;
+        }
+        marks.clear();
+        return GS_OK;
+    }
+};
+
 // Device state of the projection -> binning -> raster pipeline for up to
 // cap_views views of up to cap_n surfels. Used by the single-view Rasterizer
 // handle and by the trainer's plain and batched steps.
@@ -612,8 +652,11 @@ struct RasterCore {
     // Forward over views [v0, v0 + nv) of the current set (nv > 1 only when
     // the set was laid out batched). Parameters are the activated buffers.
     gs_status forward(hipStream_t st, uint32_t v0, uint32_t nv, uint32_t n, const float4* pos,
-                      const float2* sc, const float4* q, const float* op, const float* sh) {
+                      const float2* sc, const float4* q, const float* op, const float* sh,
+                      Profiler* prof = nullptr) {
         if (n > cap_n) return fail(GS_ERR_CAPACITY, "num_surfels exceeds capacity");
+        Profiler dummy;
+        Profiler& P = prof ? *prof : dummy;
         const CamParams* cd = cams_dev + v0;
         uint32_t tiles = 0;
         for (uint32_t v = v0; v < v0 + nv; ++v) tiles += cams[v].tiles_x * cams[v].tiles_y;
@@ -622,32 +665,43 @@ struct RasterCore {
             max_tile_id = std::max(max_tile_id, cams[v].tile_base + cams[v].tiles_x * cams[v].tiles_y);
         const uint32_t items = n * nv;
         // Records / counts are indexed view-major from this view set's first view.
-        Record* r = recs;
-        GS_HIP(launch_project(st, cd, nv, n, pos, sc, q, op, sh, r, rects, depth_keys, offsets,
+        GS_CHECK(P.mark(st, GS_STAGE_PROJECT));
+        GS_HIP(launch_project(st, cd, nv, n, pos, sc, q, op, sh, recs, rects, depth_keys, offsets,
                               ranges, 2 * max_tile_id));
+        GS_CHECK(P.mark(st, GS_STAGE_SCAN));
         GS_HIP(launch_exclusive_scan(st, offsets, offsets, items, total + v0, scan_scratch));
+        GS_CHECK(P.mark(st, GS_STAGE_EMIT));
         GS_HIP(launch_emit_keys(st, items, rects, depth_keys, offsets, cd, n ? n : 1, 0, keys[0],
                                 pays[0], cap_entries, total + v0, sort_count + v0, overflow));
         const uint32_t end_bit = 32 + bits_for(max_tile_id);
+        GS_CHECK(P.mark(st, GS_STAGE_SORT));
         GS_HIP(launch_radix_sort_u64(st, keys[0], pays[0], keys[1], pays[1], sort_count + v0,
                                      cap_entries, 0, end_bit, digit_bits, sort_scratch, &result_alt));
+        GS_CHECK(P.mark(st, GS_STAGE_RANGES));
         GS_HIP(launch_tile_ranges(st, keys[result_alt], sort_count + v0, cap_entries, ranges));
         FwdOut o{out_color, out_aux, out_normal, ncontrib};
+        GS_CHECK(P.mark(st, GS_STAGE_RASTER_FWD));
         GS_HIP(launch_raster_fwd(st, cd, cams.data() + v0, nv, tiles, ranges, pays[result_alt], recs, o));
         return GS_OK;
     }
 
     gs_status backward(hipStream_t st, uint32_t v0, uint32_t nv, uint32_t n, const float4* pos,
                        const float2* sc, const float4* q, const float* sh, const float4* dl_dcolor,
-                       const float4* dl_dnormal, ParamGrads g, bool accumulate, bool aux) {
+                       const float4* dl_dnormal, ParamGrads g, bool accumulate, bool aux,
+                       Profiler* prof = nullptr) {
+        Profiler dummy;
+        Profiler& P = prof ? *prof : dummy;
         const CamParams* cd = cams_dev + v0;
         uint32_t tiles = 0;
         for (uint32_t v = v0; v < v0 + nv; ++v) tiles += cams[v].tiles_x * cams[v].tiles_y;
+        GS_CHECK(P.mark(st, GS_STAGE_GRAD_CLEAR));
         GS_HIP(launch_fill_u32(st, reinterpret_cast<uint32_t*>(grad_coef), 0u,
                                (size_t)n * nv * kGradStride));
         BwdIn in{dl_dcolor, dl_dnormal, out_color, out_aux, ncontrib};
+        GS_CHECK(P.mark(st, GS_STAGE_RASTER_BWD));
         GS_HIP(launch_raster_bwd(st, cd, cams.data() + v0, nv, tiles, ranges, pays[result_alt], recs, in,
                                  grad_coef, aux));
+        GS_CHECK(P.mark(st, GS_STAGE_GEOM_BWD));
         GS_HIP(launch_geom_bwd(st, cd, nv, n, pos, sc, q, sh, recs, grad_coef, g, accumulate,
                                cam_partials));
         return GS_OK;
@@ -1126,6 +1180,7 @@ struct gs_trainer_s {
     hipGraphExec_t graph_exec = nullptr;
     std::vector<uint32_t> graph_shape;   // (w, h) per view of the captured step
     bool tuned = false;
+    gs::Profiler prof;
 
     ~gs_trainer_s() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -1147,13 +1202,17 @@ struct gs_trainer_s {
         dfree(loss_partials);
     }
 
-    gs_status enqueue_step(hipStream_t st, const uint32_t* ids, uint32_t k, bool batched);
+    gs_status enqueue_step(hipStream_t st, const uint32_t* ids, uint32_t k, bool batched,
+                           bool profile = false);
 };
 
-gs_status gs_trainer_s::enqueue_step(hipStream_t st, const uint32_t* ids, uint32_t k, bool batched) {
+gs_status gs_trainer_s::enqueue_step(hipStream_t st, const uint32_t* ids, uint32_t k, bool batched,
+                                     bool profile) {
     ParamGrads g{grad_pos, grad_scales, grad_quat, grad_opacity, grad_sh, nullptr};
+    gs::Profiler* P = profile ? &prof : nullptr;
     if (batched) {
-        GS_CHECK(core.forward(st, 0, k, n, positions, scales, quats, opacities, sh));
+        GS_CHECK(core.forward(st, 0, k, n, positions, scales, quats, opacities, sh, P));
+        if (P) GS_CHECK(P->mark(st, GS_STAGE_LOSS));
         for (uint32_t j = 0; j < k; ++j) {
             const TrainViewDev& tv = views[ids[j]];
             const CamParams& c = core.cams[j];
@@ -1163,20 +1222,26 @@ gs_status gs_trainer_s::enqueue_step(hipStream_t st, const uint32_t* ids, uint32
                                   core.cams_dev + j));
         }
         GS_CHECK(core.backward(st, 0, k, n, positions, scales, quats, sh, dl_dcolor, nullptr, g, false,
-                               false));
+                               false, P));
     } else {
         for (uint32_t j = 0; j < k; ++j) {
             const TrainViewDev& tv = views[ids[j]];
-            GS_CHECK(core.forward(st, j, 1, n, positions, scales, quats, opacities, sh));
+            GS_CHECK(core.forward(st, j, 1, n, positions, scales, quats, opacities, sh, P));
+            if (P) GS_CHECK(P->mark(st, GS_STAGE_LOSS));
             GS_HIP(launch_l1_ssim(st, tv.width, tv.height, cfg.lambda, cfg.dssim_scale, core.out_color,
                                   targets, dl_dcolor, loss_coef,
                                   loss_partials + (size_t)j * partials_per_view * 2, partials_per_view,
                                   core.cams_dev + j));
             GS_CHECK(core.backward(st, j, 1, n, positions, scales, quats, sh, dl_dcolor, nullptr, g,
-                                   j > 0, false));
+                                   j > 0, false, P));
         }
     }
+    if (P) GS_CHECK(P->mark(st, GS_STAGE_ADAM));
     GS_CHECK(gs_optim_encode_step(optim, st));
+    if (P) {
+        GS_CHECK(P->mark(st, -1));
+        P->steps += 1;
+    }
     return GS_OK;
 }
 
@@ -1335,7 +1400,7 @@ gs_status gs_trainer_step(gs_trainer t, void* stream, const uint32_t* view_ids, 
                 t->graph = nullptr;
             }
             GS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            gs_status s = t->enqueue_step(st, view_ids, k, true);
+            gs_status s = t->enqueue_step(st, view_ids, k, true, false);
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(st, &g);
             if (s != GS_OK) {
@@ -1351,7 +1416,10 @@ gs_status gs_trainer_step(gs_trainer t, void* stream, const uint32_t* view_ids, 
         return GS_OK;
     }
     GS_CHECK(t->core.set_views(st, cams, batched));
-    return t->enqueue_step(st, view_ids, k, batched);
+    GS_CHECK(t->enqueue_step(st, view_ids, k, batched, t->prof.enabled));
+    // Fold finished events so the event pool stays bounded.
+    if (t->prof.enabled) GS_CHECK(t->prof.collect());
+    return GS_OK;
 }
 
 gs_status gs_trainer_last_losses(gs_trainer t, double* out, uint32_t n_ids) {
@@ -1420,6 +1488,32 @@ gs_status gs_trainer_entry_count(gs_trainer t, uint32_t* entries) {
         GS_HIP(hipMemcpy(t->core.overflow, &zero, 4, hipMemcpyHostToDevice));
         return fail(GS_ERR_OVERFLOW, "tile entries exceeded the binning capacity");
     }
+    return GS_OK;
+}
+
+const char* gs_stage_name(uint32_t stage) {
+    static const char* names[GS_STAGE_COUNT] = {"project",    "scan",        "emit_keys", "sort",
+                                                "tile_ranges", "raster_fwd", "l1_ssim",   "grad_clear",
+                                                "raster_bwd", "geom_bwd",    "adam"};
+    return stage < GS_STAGE_COUNT ? names[stage] : "";
+}
+
+gs_status gs_trainer_set_profiling(gs_trainer t, int enable) {
+    GS_REQUIRE(t != nullptr, "trainer is null");
+    GS_CHECK(t->prof.collect());
+    t->prof.enabled = enable != 0;
+    return GS_OK;
+}
+
+gs_status gs_trainer_stage_times(gs_trainer t, double* ms, uint32_t* steps) {
+    GS_REQUIRE(t && ms, "null argument");
+    GS_CHECK(t->prof.collect());
+    for (uint32_t s = 0; s < GS_STAGE_COUNT; ++s) {
+        ms[s] = t->prof.acc[s];
+        t->prof.acc[s] = 0.0;
+    }
+    if (steps) *steps = t->prof.steps;
+    t->prof.steps = 0;
     return GS_OK;
 }
 
