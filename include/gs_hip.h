@@ -204,10 +204,22 @@ typedef struct gs_raster_debug {
     const uint32_t* ranges;         /* tiles * 2 */
     const uint32_t* rects;          /* capacity * 4 */
     const uint32_t* depth_keys;     /* capacity */
-    const uint32_t* offsets;        /* capacity: exclusive scan of tile counts */
+    const uint32_t* offsets;        /* capacity: per-surfel tile counts (presort binning) or
+                                       their exclusive scan (64-bit binning) */
     uint32_t num_tiles;
 } gs_raster_debug;
+/* Blocking (composes the 64-bit keys when the presort binning ran). */
 gs_status gs_raster_debug_buffers(gs_raster r, gs_raster_debug* out);
+
+/* Binning strategy; both give the same entry order, ranges and 64-bit keys:
+ *   GS_BINNING_SORT64:  one stable sort of 64-bit (tile << 32 | depth) keys
+ *                       over every varying bit (the reference's composite
+ *                       of two stable 32-bit sorts, binning.rs:5-8);
+ *   GS_BINNING_PRESORT: stable sort of the N surfel depth keys, emission in
+ *                       depth order, stable sort of 32-bit tile ids (default). */
+#define GS_BINNING_SORT64 0u
+#define GS_BINNING_PRESORT 1u
+gs_status gs_raster_set_binning(gs_raster r, uint32_t mode);
 
 /* ----------------------------------------------------------------- SsimLoss
  * first_attempt/crates/gs-train/src/ssim.rs:25-298: fused L1 + D-SSIM,
@@ -270,6 +282,7 @@ typedef struct gs_train_config {
     uint32_t max_views_per_step;
     uint32_t entries_per_view;  /* binning capacity per view (0 = 64 * n) */
     uint32_t digit_bits;        /* 0 = autotune, else 4 or 8 */
+    uint32_t binning;           /* GS_BINNING_PRESORT (default) or GS_BINNING_SORT64 */
 } gs_train_config;
 
 #define GS_STEP_PLAIN 0u

@@ -23,6 +23,7 @@ GS_BWD_COLOR_ONLY = 2
 GS_BWD_NO_CAMERA = 4
 GS_STEP_PLAIN, GS_STEP_BATCHED, GS_STEP_GRAPH = 0, 1, 2
 GS_ACT_IDENTITY, GS_ACT_EXP, GS_ACT_SIGMOID = 0, 1, 2
+GS_BINNING_SORT64, GS_BINNING_PRESORT = 0, 1
 
 
 class GsError(RuntimeError):
@@ -64,7 +65,7 @@ class TrainConfig(C.Structure):
     _fields_ = [("lambda_", C.c_float), ("dssim_scale", C.c_float), ("lr_pos", C.c_float),
                 ("lr_scales", C.c_float), ("lr_quat", C.c_float), ("lr_opacity", C.c_float),
                 ("lr_sh", C.c_float), ("max_views_per_step", C.c_uint32),
-                ("entries_per_view", C.c_uint32), ("digit_bits", C.c_uint32)]
+                ("entries_per_view", C.c_uint32), ("digit_bits", C.c_uint32), ("binning", C.c_uint32)]
 
 
 _vp = C.c_void_p
@@ -110,6 +111,7 @@ _PROTOS = {
     "gs_raster_upload_scene": ([_vp, C.POINTER(SceneInput)], C.c_int),
     "gs_raster_set_lambda_dist": ([_vp, C.c_float], C.c_int),
     "gs_raster_set_digit_bits": ([_vp, _u32], C.c_int),
+    "gs_raster_set_binning": ([_vp, _u32], C.c_int),
     "gs_raster_forward": ([_vp, _vp, C.POINTER(RasterCamera), _u32], C.c_int),
     "gs_raster_backward": ([_vp, _vp, _u32], C.c_int),
     "gs_raster_backward_ex": ([_vp, _vp, _u32, _u32], C.c_int),

@@ -270,6 +270,10 @@ class Rasterizer:
     def set_digit_bits(self, bits: int):
         check(lib().gs_raster_set_digit_bits(self.h, bits))
 
+    def set_binning(self, mode: int):
+        """L.GS_BINNING_PRESORT (default) or L.GS_BINNING_SORT64."""
+        check(lib().gs_raster_set_binning(self.h, mode))
+
     def forward(self, camera: Camera, num_surfels: int, stream=None):
         rc = camera.c()
         check(lib().gs_raster_forward(self.h, stream, C.byref(rc), num_surfels))
@@ -388,6 +392,7 @@ class TrainConfig:
     max_views_per_step: int = 4
     entries_per_view: int = 0
     digit_bits: int = 8
+    binning: int = L.GS_BINNING_PRESORT
 
 
 class Trainer:
@@ -415,7 +420,8 @@ class Trainer:
             tv[k].camera = cam.c()
             tv[k].target_rgba = t.ctypes.data
         c = L.TrainConfig(cfg.lam, cfg.dssim_scale, cfg.lr_pos, cfg.lr_scales, cfg.lr_quat, cfg.lr_opacity,
-                          cfg.lr_sh, cfg.max_views_per_step, cfg.entries_per_view, cfg.digit_bits)
+                          cfg.lr_sh, cfg.max_views_per_step, cfg.entries_per_view, cfg.digit_bits,
+                          cfg.binning)
         h = C.c_void_p()
         check(lib().gs_trainer_create(ctx.h, C.byref(c), C.byref(si), len(views), tv, C.byref(h)))
         self.h = h.value

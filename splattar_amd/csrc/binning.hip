@@ -10,6 +10,13 @@
 // reference it bounds-checks against max_entries and raises an overflow flag.
 //
 // tile_ranges restates tile_binning.wgsl:89-103 over the sorted 64-bit keys.
+//
+// The training path uses the presorted variant (emit_tile_keys): surfels are
+// first stably sorted by depth key (N items, 32-bit keys), entries are
+// emitted in that order with 32-bit tile ids only, and a stable sort over the
+// tile bits alone finishes the job -- the same entry order, ranges and
+// (reconstructible) 64-bit keys as the 64-bit sort, with 2 instead of 6
+// passes over the entries.
 
 #include "gs_internal.h"
 
@@ -96,6 +103,102 @@ __global__ __launch_bounds__(256) void emit_keys_kernel(
     }
 }
 
+// Depth-presorted emission: items are visited in the order of a stable sort
+// of their depth keys (order[k] = item), so entries come out ordered by
+// (depth bits, record index) and a stable sort on the tile id alone yields
+// exactly the (tile, depth, index) order of the 64-bit path. Keys are the
+// 32-bit global tile ids; payloads the record indices.
+__global__ __launch_bounds__(256) void emit_tile_keys_kernel(
+    uint32_t n_items, const uint32_t* __restrict__ order, const uint4* __restrict__ rects,
+    const uint32_t* __restrict__ offsets_sorted, const CamParams* __restrict__ cams,
+    uint32_t items_per_view, uint32_t* __restrict__ tkeys, uint32_t* __restrict__ payloads,
+    uint32_t max_entries, const uint32_t* __restrict__ total, uint32_t* __restrict__ sort_count,
+    uint32_t* __restrict__ overflow_flag) {
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid == 0) {
+        const uint32_t t = *total;
+        *sort_count = t < max_entries ? t : max_entries;
+        if (t > max_entries) atomicOr(overflow_flag, 1u);
+    }
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave_first = gid - (uint32_t)lane;
+    if (wave_first >= n_items) return;
+
+    uint32_t cnt = 0, base = 0, w = 0, x0 = 0, y0 = 0, tiles_x = 0, tile_base = 0, item = 0;
+    if (gid < n_items) {
+        item = order[gid];
+        const uint4 r = rects[item];
+        if (r.z >= r.x && r.w >= r.y) {
+            w = r.z - r.x + 1u;
+            cnt = w * (r.w - r.y + 1u);
+            x0 = r.x;
+            y0 = r.y;
+            base = offsets_sorted[gid];
+            const CamParams& cam = cams[item / items_per_view];
+            tiles_x = cam.tiles_x;
+            tile_base = cam.tile_base;
+        }
+    }
+    uint32_t incl = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t v = __shfl_up(incl, o);
+        if (lane >= o) incl += v;
+    }
+    const uint32_t wave_total = __shfl(incl, 63);
+    const uint32_t excl = incl - cnt;
+    for (uint32_t k0 = 0; k0 < wave_total; k0 += 64) {
+        const uint32_t k = k0 + (uint32_t)lane;
+        int owner = 0;
+        for (int step = 32; step >= 1; step >>= 1) {
+            const int cand = owner + step;
+            const uint32_t ex = __shfl(excl, cand & 63);
+            if (cand < 64 && ex <= k) owner = cand;
+        }
+        const uint32_t o_excl = __shfl(excl, owner);
+        const uint32_t o_cnt = __shfl(cnt, owner);
+        const uint32_t o_base = __shfl(base, owner);
+        const uint32_t o_w = __shfl(w, owner);
+        const uint32_t o_x0 = __shfl(x0, owner);
+        const uint32_t o_y0 = __shfl(y0, owner);
+        const uint32_t o_tx = __shfl(tiles_x, owner);
+        const uint32_t o_tb = __shfl(tile_base, owner);
+        const uint32_t o_item = __shfl(item, owner);
+        if (k < wave_total && o_cnt > 0) {
+            const uint32_t local = k - o_excl;
+            const uint32_t e = o_base + local;
+            if (e < max_entries) {
+                const uint32_t ty = o_y0 + local / o_w;
+                const uint32_t tx = o_x0 + local % o_w;
+                tkeys[e] = o_tb + ty * o_tx + tx;
+                payloads[e] = o_item;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void tile_ranges_u32_kernel(const uint32_t* __restrict__ keys,
+                                                              const uint32_t* __restrict__ count,
+                                                              uint32_t* __restrict__ ranges) {
+    const uint32_t n = *count;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t tile = keys[i];
+        if (i == 0 || keys[i - 1] != tile) ranges[2 * (size_t)tile] = i;
+        if (i == n - 1 || keys[i + 1] != tile) ranges[2 * (size_t)tile + 1] = i + 1;
+    }
+}
+
+// 64-bit (tile << 32 | depth bits) keys of the presorted path, for parity
+// checks and tooling.
+__global__ __launch_bounds__(256) void compose_keys_kernel(const uint32_t* __restrict__ tkeys,
+                                                           const uint32_t* __restrict__ payloads,
+                                                           const uint32_t* __restrict__ depth_keys,
+                                                           const uint32_t* __restrict__ count,
+                                                           uint64_t* __restrict__ keys) {
+    const uint32_t n = *count;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        keys[i] = ((uint64_t)tkeys[i] << 32) | (uint64_t)depth_keys[payloads[i]];
+}
+
 __global__ __launch_bounds__(256) void tile_ranges_kernel(const uint64_t* __restrict__ keys,
                                                           const uint32_t* __restrict__ count,
                                                           uint32_t* __restrict__ ranges) {
@@ -144,6 +247,36 @@ hipError_t launch_tile_ranges(hipStream_t st, const uint64_t* keys, const uint32
     uint32_t blocks = div_up(capacity > 0 ? capacity : 1, 256);
     if (blocks > 8192) blocks = 8192;  // grid-stride beyond this
     hipLaunchKernelGGL(tile_ranges_kernel, dim3(blocks), dim3(256), 0, st, keys, count, ranges);
+    return hipGetLastError();
+}
+
+hipError_t launch_emit_tile_keys(hipStream_t st, uint32_t n_items, const uint32_t* order,
+                                 const uint4* rects, const uint32_t* offsets_sorted,
+                                 const CamParams* cams_dev, uint32_t items_per_view, uint32_t* tkeys,
+                                 uint32_t* payloads, uint32_t max_entries, const uint32_t* total,
+                                 uint32_t* sort_count, uint32_t* overflow_flag) {
+    uint32_t blocks = div_up(n_items > 0 ? n_items : 1, 256);
+    hipLaunchKernelGGL(emit_tile_keys_kernel, dim3(blocks), dim3(256), 0, st, n_items, order, rects,
+                       offsets_sorted, cams_dev, items_per_view > 0 ? items_per_view : 1, tkeys,
+                       payloads, max_entries, total, sort_count, overflow_flag);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_ranges_u32(hipStream_t st, const uint32_t* keys, const uint32_t* count,
+                                  uint32_t capacity, uint32_t* ranges) {
+    uint32_t blocks = div_up(capacity > 0 ? capacity : 1, 256);
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(tile_ranges_u32_kernel, dim3(blocks), dim3(256), 0, st, keys, count, ranges);
+    return hipGetLastError();
+}
+
+hipError_t launch_compose_keys(hipStream_t st, const uint32_t* tkeys, const uint32_t* payloads,
+                               const uint32_t* depth_keys, const uint32_t* count, uint32_t capacity,
+                               uint64_t* keys) {
+    uint32_t blocks = div_up(capacity > 0 ? capacity : 1, 256);
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(compose_keys_kernel, dim3(blocks), dim3(256), 0, st, tkeys, payloads, depth_keys,
+                       count, keys);
     return hipGetLastError();
 }
 

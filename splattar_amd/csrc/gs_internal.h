@@ -297,16 +297,20 @@ inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) /
 // callers in runtime.cpp before any launch.
 
 // project.hip
+// presort_keys / presort_vals (optional) receive a copy of the depth key and
+// the item index for the depth presort; item_count (optional) receives
+// n * n_views.
 hipError_t launch_project(hipStream_t st, const CamParams* cams_dev, uint32_t n_views, uint32_t n,
                           const float4* pos, const float2* scales, const float4* quats,
                           const float* opac, const float* sh, Record* recs, uint4* rects,
                           uint32_t* depth_keys, uint32_t* counts, uint32_t* ranges_to_clear,
-                          uint32_t num_ranges);
+                          uint32_t num_ranges, uint32_t* presort_keys = nullptr,
+                          uint32_t* presort_vals = nullptr, uint32_t* item_count = nullptr);
 
 // scan.hip
 size_t scan_scratch_bytes(uint32_t capacity);
 hipError_t launch_exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t n,
-                                 uint32_t* total_out, void* scratch);
+                                 uint32_t* total_out, void* scratch, const uint32_t* perm = nullptr);
 
 // binning.hip
 hipError_t launch_emit_keys(hipStream_t st, uint32_t n_items, const uint4* rects,
@@ -319,6 +323,16 @@ hipError_t launch_tile_ranges(hipStream_t st, const uint64_t* keys, const uint32
                               uint32_t capacity, uint32_t* ranges);
 hipError_t launch_fill_u32(hipStream_t st, uint32_t* dst, uint32_t value, size_t count);
 hipError_t launch_count_tiles(hipStream_t st, uint32_t n, const uint4* rects, uint32_t* counts);
+hipError_t launch_emit_tile_keys(hipStream_t st, uint32_t n_items, const uint32_t* order,
+                                 const uint4* rects, const uint32_t* offsets_sorted,
+                                 const CamParams* cams_dev, uint32_t items_per_view, uint32_t* tkeys,
+                                 uint32_t* payloads, uint32_t max_entries, const uint32_t* total,
+                                 uint32_t* sort_count, uint32_t* overflow_flag);
+hipError_t launch_tile_ranges_u32(hipStream_t st, const uint32_t* keys, const uint32_t* count,
+                                  uint32_t capacity, uint32_t* ranges);
+hipError_t launch_compose_keys(hipStream_t st, const uint32_t* tkeys, const uint32_t* payloads,
+                               const uint32_t* depth_keys, const uint32_t* count, uint32_t capacity,
+                               uint64_t* keys);
 
 // sort.hip
 size_t sort_scratch_bytes(uint32_t capacity);

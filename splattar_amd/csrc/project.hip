@@ -18,7 +18,8 @@ __global__ __launch_bounds__(256) void project_kernel(
     const float2* __restrict__ scales, const float4* __restrict__ quats,
     const float* __restrict__ opac, const float* __restrict__ sh, Record* __restrict__ recs,
     uint4* __restrict__ rects, uint32_t* __restrict__ depth_keys, uint32_t* __restrict__ counts,
-    uint32_t* __restrict__ ranges_to_clear, uint32_t num_ranges) {
+    uint32_t* __restrict__ ranges_to_clear, uint32_t num_ranges, uint32_t* __restrict__ presort_keys,
+    uint32_t* __restrict__ presort_vals, uint32_t* __restrict__ item_count) {
     const uint32_t view = blockIdx.y;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     // The tile-range table of the following binning stage starts cleared
@@ -27,9 +28,11 @@ __global__ __launch_bounds__(256) void project_kernel(
     if (view == 0 && ranges_to_clear != nullptr) {
         for (uint32_t k = i; k < num_ranges; k += gridDim.x * blockDim.x) ranges_to_clear[k] = 0u;
     }
+    if (view == 0 && i == 0 && item_count != nullptr) *item_count = n * gridDim.y;
     if (i >= n) return;
     const CamParams cam = cams[view];
     const uint32_t g = view * n + i;
+    if (presort_vals) presort_vals[g] = g;
 
     const float4 p = pos[i];
     const float2 s = scales[i];
@@ -41,6 +44,7 @@ __global__ __launch_bounds__(256) void project_kernel(
         rects[g] = rect;
         depth_keys[g] = 0u;
         counts[g] = 0u;
+        if (presort_keys) presort_keys[g] = 0u;
         return;
     }
     const RectOut r = tile_rect(cam, f, depth);
@@ -51,6 +55,7 @@ __global__ __launch_bounds__(256) void project_kernel(
     }
     rects[g] = rect;
     depth_keys[g] = __float_as_uint(depth);
+    if (presort_keys) presort_keys[g] = __float_as_uint(depth);
     counts[g] = count;
     if (count == 0u) return;  // no entry will ever read the record
 
@@ -97,12 +102,14 @@ hipError_t launch_project(hipStream_t st, const CamParams* cams_dev, uint32_t n_
                           const float4* pos, const float2* scales, const float4* quats,
                           const float* opac, const float* sh, Record* recs, uint4* rects,
                           uint32_t* depth_keys, uint32_t* counts, uint32_t* ranges_to_clear,
-                          uint32_t num_ranges) {
-    if (n == 0 && num_ranges == 0) return hipSuccess;
-    uint32_t blocks = div_up(n > num_ranges ? n : num_ranges, 256);
+                          uint32_t num_ranges, uint32_t* presort_keys, uint32_t* presort_vals,
+                          uint32_t* item_count) {
+    if (n == 0 && num_ranges == 0 && item_count == nullptr) return hipSuccess;
+    uint32_t blocks = div_up(n > num_ranges ? (n ? n : 1) : num_ranges, 256);
     dim3 grid(blocks, n_views);
     hipLaunchKernelGGL(project_kernel, grid, dim3(256), 0, st, cams_dev, n, pos, scales, quats, opac,
-                       sh, recs, rects, depth_keys, counts, ranges_to_clear, num_ranges);
+                       sh, recs, rects, depth_keys, counts, ranges_to_clear, num_ranges, presort_keys,
+                       presort_vals, item_count);
     return hipGetLastError();
 }
 

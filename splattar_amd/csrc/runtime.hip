@@ -534,6 +534,14 @@ struct RasterCore {
     uint32_t* ncontrib = nullptr;
     float* grad_coef = nullptr;
     float* cam_partials = nullptr;
+    // Depth-presort path (binning = GS_BINNING_PRESORT).
+    uint32_t* pre_keys[2] = {nullptr, nullptr};
+    uint32_t* pre_vals[2] = {nullptr, nullptr};
+    uint32_t* offs_sorted = nullptr;
+    uint32_t* item_count = nullptr;
+    uint32_t* tkeys[2] = {nullptr, nullptr};  // 32-bit tile keys, stored inside keys[1]
+    uint32_t binning = GS_BINNING_PRESORT;
+    bool keys64_valid = false;       // keys[result_alt] holds 64-bit keys of the last forward
     uint32_t digit_bits = 8;
     int result_alt = 0;              // which ping-pong buffer holds the sorted entries
     // Current view set (host mirror of cams_dev).
@@ -564,6 +572,12 @@ struct RasterCore {
         dfree(ncontrib);
         dfree(grad_coef);
         dfree(cam_partials);
+        for (int i = 0; i < 2; ++i) {
+            dfree(pre_keys[i]);
+            dfree(pre_vals[i]);
+        }
+        dfree(offs_sorted);
+        dfree(item_count);
     }
 
     gs_status init(gs_ctx c, uint32_t n, uint32_t views, uint32_t entries, uint32_t tiles,
@@ -591,9 +605,22 @@ struct RasterCore {
             GS_CHECK(dalloc(&keys[i], entries, "keys"));
             GS_CHECK(dalloc(&pays[i], entries, "payloads"));
         }
+        // The presorted path's two 32-bit tile-key buffers live in keys[1]
+        // (8 B per entry = two u32 arrays); keys[0] stays free for the
+        // composed 64-bit keys the debug interface reports.
+        tkeys[0] = reinterpret_cast<uint32_t*>(keys[1]);
+        tkeys[1] = tkeys[0] + (entries ? entries : 1);
         GS_CHECK(dalloc(&scan_scratch, scan_scratch_bytes((uint32_t)std::min<size_t>(nv, 0xFFFFFFFFu)),
                         "scan scratch"));
-        GS_CHECK(dalloc(&sort_scratch, sort_scratch_bytes(entries), "sort scratch"));
+        GS_CHECK(dalloc(&sort_scratch,
+                        sort_scratch_bytes((uint32_t)std::max<size_t>(entries, std::min<size_t>(nv, 0xFFFFFFFFu))),
+                        "sort scratch"));
+        for (int i = 0; i < 2; ++i) {
+            GS_CHECK(dalloc(&pre_keys[i], nv, "presort keys"));
+            GS_CHECK(dalloc(&pre_vals[i], nv, "presort order"));
+        }
+        GS_CHECK(dalloc(&offs_sorted, nv, "presorted offsets"));
+        GS_CHECK(dalloc(&item_count, 1, "item count"));
         GS_CHECK(dalloc(&ranges, (size_t)tiles * 2, "ranges"));
         GS_CHECK(dalloc(&out_color, pixels, "out_color"));
         GS_CHECK(dalloc(&out_aux, pixels, "out_aux"));
@@ -646,7 +673,20 @@ struct RasterCore {
         return GS_OK;
     }
 
-    const uint64_t* sorted_keys() const { return keys[result_alt]; }
+    const uint32_t* last_sort_count = nullptr;
+
+    // 64-bit (tile << 32 | depth) keys of the last forward (composed on
+    // demand for the presorted path) and its sorted payloads.
+    gs_status sorted_keys(hipStream_t st, const uint64_t** out) {
+        if (!keys64_valid && last_sort_count) {
+            GS_HIP(launch_compose_keys(st, tkeys[result_alt], pays[result_alt], depth_keys, last_sort_count,
+                                       cap_entries, keys[0]));
+            *out = keys[0];
+            return GS_OK;
+        }
+        *out = keys[result_alt];
+        return GS_OK;
+    }
     const uint32_t* sorted_pays() const { return pays[result_alt]; }
 
     // Forward over views [v0, v0 + nv) of the current set (nv > 1 only when
@@ -664,7 +704,36 @@ struct RasterCore {
         for (uint32_t v = v0; v < v0 + nv; ++v)
             max_tile_id = std::max(max_tile_id, cams[v].tile_base + cams[v].tiles_x * cams[v].tiles_y);
         const uint32_t items = n * nv;
-        // Records / counts are indexed view-major from this view set's first view.
+        FwdOut o{out_color, out_aux, out_normal, ncontrib};
+        if (binning == GS_BINNING_PRESORT) {
+            // Stable depth presort of the items, emission in that order, then
+            // a stable sort on the tile bits only (binning.hip).
+            GS_CHECK(P.mark(st, GS_STAGE_PROJECT));
+            GS_HIP(launch_project(st, cd, nv, n, pos, sc, q, op, sh, recs, rects, depth_keys, offsets,
+                                  ranges, 2 * max_tile_id, pre_keys[0], pre_vals[0], item_count));
+            GS_CHECK(P.mark(st, GS_STAGE_SORT));
+            int alt = 0;
+            GS_HIP(launch_radix_sort_u32(st, pre_keys[0], pre_vals[0], pre_keys[1], pre_vals[1],
+                                         item_count, items, 0, 32, digit_bits, sort_scratch, &alt));
+            const uint32_t* order = pre_vals[alt];
+            GS_CHECK(P.mark(st, GS_STAGE_SCAN));
+            GS_HIP(launch_exclusive_scan(st, offsets, offs_sorted, items, total + v0, scan_scratch, order));
+            GS_CHECK(P.mark(st, GS_STAGE_EMIT));
+            GS_HIP(launch_emit_tile_keys(st, items, order, rects, offs_sorted, cd, n ? n : 1, tkeys[0],
+                                         pays[0], cap_entries, total + v0, sort_count + v0, overflow));
+            GS_CHECK(P.mark(st, GS_STAGE_SORT));
+            GS_HIP(launch_radix_sort_u32(st, tkeys[0], pays[0], tkeys[1], pays[1], sort_count + v0,
+                                         cap_entries, 0, bits_for(max_tile_id), digit_bits, sort_scratch,
+                                         &result_alt));
+            GS_CHECK(P.mark(st, GS_STAGE_RANGES));
+            GS_HIP(launch_tile_ranges_u32(st, tkeys[result_alt], sort_count + v0, cap_entries, ranges));
+            keys64_valid = false;
+            last_sort_count = sort_count + v0;
+            GS_CHECK(P.mark(st, GS_STAGE_RASTER_FWD));
+            GS_HIP(launch_raster_fwd(st, cd, cams.data() + v0, nv, tiles, ranges, pays[result_alt], recs, o));
+            return GS_OK;
+        }
+        // 64-bit (tile || depth) key path: one sort over every key bit that varies.
         GS_CHECK(P.mark(st, GS_STAGE_PROJECT));
         GS_HIP(launch_project(st, cd, nv, n, pos, sc, q, op, sh, recs, rects, depth_keys, offsets,
                               ranges, 2 * max_tile_id));
@@ -679,7 +748,8 @@ struct RasterCore {
                                      cap_entries, 0, end_bit, digit_bits, sort_scratch, &result_alt));
         GS_CHECK(P.mark(st, GS_STAGE_RANGES));
         GS_HIP(launch_tile_ranges(st, keys[result_alt], sort_count + v0, cap_entries, ranges));
-        FwdOut o{out_color, out_aux, out_normal, ncontrib};
+        keys64_valid = true;
+        last_sort_count = sort_count + v0;
         GS_CHECK(P.mark(st, GS_STAGE_RASTER_FWD));
         GS_HIP(launch_raster_fwd(st, cd, cams.data() + v0, nv, tiles, ranges, pays[result_alt], recs, o));
         return GS_OK;
@@ -857,6 +927,13 @@ gs_status gs_raster_set_digit_bits(gs_raster r, uint32_t digit_bits) {
     return GS_OK;
 }
 
+gs_status gs_raster_set_binning(gs_raster r, uint32_t mode) {
+    GS_REQUIRE(r != nullptr, "raster is null");
+    GS_REQUIRE(mode == GS_BINNING_SORT64 || mode == GS_BINNING_PRESORT, "unknown binning mode");
+    r->core.binning = mode;
+    return GS_OK;
+}
+
 gs_status gs_raster_forward(gs_raster r, void* stream, const gs_raster_camera* camera,
                             uint32_t num_surfels) {
     GS_REQUIRE(r != nullptr, "raster is null");
@@ -909,7 +986,9 @@ gs_status gs_raster_entry_count(gs_raster r, uint32_t* entries) {
 
 gs_status gs_raster_debug_buffers(gs_raster r, gs_raster_debug* o) {
     GS_REQUIRE(r && o, "null argument");
-    o->sorted_keys = r->core.sorted_keys();
+    GS_REQUIRE(r->have_forward, "no forward has run on this rasterizer");
+    GS_CHECK(r->core.sorted_keys(r->ctx->stream, &o->sorted_keys));
+    GS_HIP(hipStreamSynchronize(r->ctx->stream));
     o->sorted_payloads = r->core.sorted_pays();
     o->ranges = r->core.ranges;
     o->rects = reinterpret_cast<const uint32_t*>(r->core.rects);
@@ -1253,6 +1332,8 @@ gs_status gs_trainer_create(gs_ctx ctx, const gs_train_config* cfg, const gs_sce
     GS_REQUIRE(n_views > 0, "need at least one view");
     GS_REQUIRE(cfg->max_views_per_step >= 1 && cfg->max_views_per_step <= 64,
                "max_views_per_step must be in 1..64");
+    GS_REQUIRE(cfg->binning == GS_BINNING_SORT64 || cfg->binning == GS_BINNING_PRESORT,
+               "unknown binning mode");
     GS_REQUIRE(cfg->digit_bits == 0 || cfg->digit_bits == 4 || cfg->digit_bits == 8,
                "digit_bits must be 0, 4 or 8");
     GS_REQUIRE(init->n > 0, "need at least one surfel");
@@ -1282,6 +1363,7 @@ gs_status gs_trainer_create(gs_ctx ctx, const gs_train_config* cfg, const gs_sce
     if (entries >= (1ull << 30)) return fail(GS_ERR_CAPACITY, "entries per step must be < 2^30");
     GS_CHECK(t->core.init(ctx, n, V, (uint32_t)entries, max_tiles * V, max_px * V));
     t->core.digit_bits = cfg->digit_bits ? cfg->digit_bits : 8;
+    t->core.binning = cfg->binning;
     GS_CHECK(dalloc(&t->targets, toff, "targets"));
     {
         std::vector<float> buf;

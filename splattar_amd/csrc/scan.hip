@@ -51,11 +51,14 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v, int lane) {
 }
 
 // `in` and `out` may alias (in-place scan, as PrefixSum::encode): every block
-// reads its whole partition into registers before writing it back.
+// reads its whole partition into registers before writing it back. With
+// `perm`, element i of the scanned sequence is in[perm[i]] (a gather; then
+// in and out must not alias).
 __global__ __launch_bounds__(kScanThreads) void scan_kernel(const uint32_t* in, uint32_t* out,
                                                             uint32_t n,
                                                             uint32_t* __restrict__ total,
-                                                            ScanScratch* __restrict__ sc) {
+                                                            ScanScratch* __restrict__ sc,
+                                                            const uint32_t* __restrict__ perm) {
     __shared__ uint32_t s_part;
     __shared__ uint32_t s_wave[kScanThreads / 64];
     __shared__ uint32_t s_prefix;
@@ -74,7 +77,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(const uint32_t* in, 
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         const uint32_t idx = first + k;
-        v[k] = idx < n ? in[idx] : 0u;
+        v[k] = idx < n ? in[perm ? perm[idx] : idx] : 0u;
         sum += v[k];
     }
     const uint32_t incl = wave_inclusive_scan(sum, lane);
@@ -132,13 +135,14 @@ size_t scan_scratch_bytes(uint32_t capacity) {
 }
 
 hipError_t launch_exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t n,
-                                 uint32_t* total_out, void* scratch) {
+                                 uint32_t* total_out, void* scratch, const uint32_t* perm) {
     ScanScratch* sc = static_cast<ScanScratch*>(scratch);
     hipLaunchKernelGGL(scan_init_kernel, dim3(1), dim3(1), 0, st, sc, total_out);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || n == 0) return e;
     const uint32_t parts = (n + kScanPart - 1) / kScanPart;
-    hipLaunchKernelGGL(scan_kernel, dim3(parts), dim3(kScanThreads), 0, st, in, out, n, total_out, sc);
+    hipLaunchKernelGGL(scan_kernel, dim3(parts), dim3(kScanThreads), 0, st, in, out, n, total_out, sc,
+                       perm);
     return hipGetLastError();
 }
 

@@ -45,11 +45,13 @@ def _rasterizer(gpu, scene, cam, max_entries=None):
     return r
 
 
-def _forward_case(gpu, seed, n, deg, size, flavor="parity"):
+def _forward_case(gpu, seed, n, deg, size, flavor="parity", binning=None):
     scene, cam = make_scene(seed, n, deg, size, flavor=flavor)
     s32, c32 = scene_from_f32(scene), camera_from_f32(cam)
-    announce(f"forward seed={seed} n={n} deg={deg} size={size}")
+    announce(f"forward seed={seed} n={n} deg={deg} size={size} binning={binning}")
     r = _rasterizer(gpu, scene, cam)
+    if binning is not None:
+        r.set_binning(binning)
     r.forward(_camera(cam, deg), n)
     color = r.out_color.read()
     aux = r.out_aux.read()
@@ -82,8 +84,9 @@ def test_forward_matches_fp64_oracle(gpu, seed, n, deg, size):
     assert np.abs(aux[..., 0] + color[..., 3] - 1.0).max() < 2e-6
 
 
-def test_projection_and_binning_bit_exact(gpu):
-    scene, cam, color, aux, normal, dbg, ref = _forward_case(gpu, 404, 50, 1, 128)
+@pytest.mark.parametrize("binning", [0, 1])
+def test_projection_and_binning_bit_exact(gpu, binning):
+    scene, cam, color, aux, normal, dbg, ref = _forward_case(gpu, 404, 50, 1, 128, binning=binning)
     rot = oracle.quat_to_mat(cam.quat.astype(np.float32).astype(np.float64))
     c32 = oracle.cam32(rot, cam.center.astype(np.float32), cam.focal, cam.width, cam.height)
     pos4 = np.concatenate([scene.pos, np.ones((scene.n, 1))], 1).astype(np.float32)
@@ -97,6 +100,40 @@ def test_projection_and_binning_bit_exact(gpu):
     assert np.array_equal(dbg["sorted_keys"], k)
     assert np.array_equal(dbg["sorted_payloads"], p)
     assert np.array_equal(dbg["ranges"], oracle.tile_ranges(k, c32.tiles_x * c32.tiles_y))
+
+
+@pytest.mark.parametrize("digit_bits", [8, 4])
+def test_binning_modes_agree_on_config0_scene(gpu, digit_bits):
+    """Presort and 64-bit binning give identical keys, payloads, ranges and
+    images on the 10k-surfel 256x256 benchmark scene, equal to the mirror."""
+    import splattar_amd as sa
+    from splattar_amd import scenes
+    sc = scenes.config_scene(0)
+    sp = scenes.front_view(256, 256)
+    cam = sa.Camera(sp.center.astype(np.float32), sp.quat.astype(np.float32), float(np.float32(sp.focal)), 0)
+    outs = {}
+    for mode in (0, 1):
+        announce(f"config0 binning={mode} digit={digit_bits}")
+        r = sa.Rasterizer(gpu, sc.n, 256, 256, sc.n * 64)
+        r.upload_scene(sc.positions, sc.scales, sc.quats, sc.opacities, sc.sh)
+        r.set_binning(mode)
+        r.set_digit_bits(digit_bits)
+        r.forward(cam, sc.n)
+        outs[mode] = (r.debug(), r.out_color.read(), r.out_ncontrib.read())
+        r.close()
+    (d0, c0, n0), (d1, c1, n1) = outs[0], outs[1]
+    for k in ("sorted_keys", "sorted_payloads", "ranges", "rects", "depth_keys"):
+        assert np.array_equal(d0[k], d1[k]), k
+    assert np.array_equal(c0, c1) and np.array_equal(n0, n1)
+    rot = oracle.quat_to_mat(cam.quat.astype(np.float64))
+    c32 = oracle.cam32(rot, cam.center, cam.focal, 256, 256)
+    pos4 = np.concatenate([sc.positions, np.ones((sc.n, 1), np.float32)], 1)
+    rects, keys, _ = oracle.prep_f32(pos4, sc.scales, sc.quats, c32)
+    k, p, e = oracle.emit_keys(rects, keys, c32.tiles_x)
+    k, p = oracle.stable_sort(k, p)
+    assert d1["entries"] == e and np.array_equal(d1["sorted_keys"], k)
+    assert np.array_equal(d1["sorted_payloads"], p)
+    assert np.array_equal(d1["ranges"], oracle.tile_ranges(k, c32.tiles_x * c32.tiles_y))
 
 
 def _weights(seed, size, aux):
