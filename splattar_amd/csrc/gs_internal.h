@@ -38,8 +38,9 @@ struct __align__(16) Record {
     float4 b_py;     // B.xyz, projected center y (pixels)
     float4 col_op;   // clamped SH color rgb, opacity
     float4 nrm_dep;  // camera-facing unit normal xyz, center depth
+    float4 box;      // conservative alpha >= 1/255 footprint (x0, y0, x1, y1), pixel centres
 };
-static_assert(sizeof(Record) == 80, "record layout");
+static_assert(sizeof(Record) == 96, "record layout");
 
 // Coefficient-space gradient accumulator per (view, surfel), summed over the
 // pixels of every tile the surfel reaches (raster_bwd), chained to parameter
@@ -249,7 +250,81 @@ struct HitEval {
     bool clamped;    // o * ghat > 0.995
 };
 
-__device__ __forceinline__ HitEval eval_hit(const Record& r, float sx, float sy, float pxc, float pyc) {
+// Exponent threshold below which a record cannot reach alpha >= 1/255:
+// o * exp(e) >= 1/255 needs e >= -ln(255 o); the 1e-3 slack (a 0.1 % factor)
+// dwarfs the error of __expf, so skipping e < lthr never drops a
+// contribution the exact test would keep.
+__device__ __forceinline__ float record_lthr(float opacity) {
+    return opacity * 255.0f > 1.0f ? -(logf(opacity * 255.0f) + 1e-3f) : 1.0f;  // 1 > any e: never
+}
+
+// Conservative box (pixel-centre coordinates) of the pixels at which a record
+// can reach alpha >= 1/255: the screen low-pass disk dx^2 + dy^2 <= L around
+// (px, py), united with the image of the ray-branch disk u^2 + v^2 <= 2L.
+// With s = (sx, sy, 1), the latter is the conic s^T Q s <= 0,
+// Q = A A^T + B B^T - 2L M M^T; when its quadratic part is positive definite
+// (the disk lies in front of the camera plane) the box comes from the dual
+// conic adj(Q): the tangent lines sx = k solve adj00 - 2k adj02 + k^2 adj22 = 0
+// (and likewise for sy). Otherwise the ray part is unbounded. Computed in
+// double with 1 px of margin, once per (view, surfel) at projection time.
+// The result is (x0, y0, x1, y1); a record that never contributes gets an
+// empty box (x0 > x1).
+__device__ inline float4 record_box(const Record& r, const CamParams& cam) {
+    const float op = r.col_op.w;
+    if (!(op * 255.0f > 1.0f)) return make_float4(1e30f, 1e30f, -1e30f, -1e30f);
+    const double L = log((double)op * 255.0) + 2e-3;
+    const double rs = sqrt(L) + 0.5;
+    double x0 = r.a_px.w - rs, x1 = r.a_px.w + rs, y0 = r.b_py.w - rs, y1 = r.b_py.w + rs;
+    const double k = 2.0 * L;
+    const double A[3] = {r.a_px.x, r.a_px.y, r.a_px.z};
+    const double B[3] = {r.b_py.x, r.b_py.y, r.b_py.z};
+    const double M[3] = {r.m_nt.x, r.m_nt.y, r.m_nt.z};
+    double Q[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Q[i][j] = A[i] * A[j] + B[i] * B[j] - k * M[i] * M[j];
+    const double det2 = Q[0][0] * Q[1][1] - Q[0][1] * Q[0][1];
+    const double mag = Q[0][0] * Q[1][1] + Q[0][1] * Q[0][1];
+    bool bounded = Q[0][0] > 0.0 && det2 > 1e-9 * mag;
+    if (bounded) {
+        const double a00 = Q[1][1] * Q[2][2] - Q[1][2] * Q[1][2];
+        const double a11 = Q[0][0] * Q[2][2] - Q[0][2] * Q[0][2];
+        const double a02 = Q[0][1] * Q[1][2] - Q[0][2] * Q[1][1];
+        const double a12 = Q[0][1] * Q[0][2] - Q[0][0] * Q[1][2];
+        const double dx = a02 * a02 - a00 * det2, dy = a12 * a12 - a11 * det2;
+        bounded = dx >= 0.0 && dy >= 0.0;
+        if (bounded) {
+            const double sdx = sqrt(dx), sdy = sqrt(dy);
+            const double sx_lo = (a02 - sdx) / det2, sx_hi = (a02 + sdx) / det2;
+            const double sy_lo = (a12 - sdy) / det2, sy_hi = (a12 + sdy) / det2;
+            // X = cx - f sx (pixel centre), Y = cy + f sy.
+            const double f = cam.focal;
+            x0 = fmin(x0, cam.cx - f * sx_hi - 1.0);
+            x1 = fmax(x1, cam.cx - f * sx_lo + 1.0);
+            y0 = fmin(y0, cam.cy + f * sy_lo - 1.0);
+            y1 = fmax(y1, cam.cy + f * sy_hi + 1.0);
+        }
+    }
+    if (!bounded || !(x0 == x0) || !(y0 == y0) || !(x1 == x1) || !(y1 == y1))
+        return make_float4(-1e30f, -1e30f, 1e30f, 1e30f);
+    return make_float4((float)x0, (float)y0, (float)x1, (float)y1);
+}
+
+// Bit q (q = 0..3, quadrant (q & 1, q >> 1) of the 16x16 tile at (tx, ty))
+// is set when the box (x0, y0, x1, y1) reaches that 8x8 pixel block.
+__device__ __forceinline__ uint32_t quadrant_mask(const float4 b, uint32_t tx, uint32_t ty) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float qx = (float)(tx * kTile + (q & 1) * 8), qy = (float)(ty * kTile + (q >> 1) * 8);
+        if (b.z >= qx + 0.5f && b.x <= qx + 7.5f && b.w >= qy + 0.5f && b.y <= qy + 7.5f) m |= 1u << q;
+    }
+    return m;
+}
+
+__device__ __forceinline__ HitEval eval_hit(const Record& r, float sx, float sy, float pxc, float pyc,
+                                            float lthr = -INFINITY) {
     HitEval h;
     const float D = r.m_nt.x * sx + r.m_nt.y * sy + r.m_nt.z;
     const float Nu = r.a_px.x * sx + r.a_px.y * sy + r.a_px.z;
@@ -270,6 +345,12 @@ __device__ __forceinline__ HitEval eval_hit(const Record& r, float sx, float sy,
     h.ray = e_ray >= e_scr;
     const float e = h.ray ? e_ray : e_scr;
     if (!h.ray) h.t = r.nrm_dep.w;
+    if (e < lthr) {  // cannot reach 1/255 (see record_lthr)
+        h.ghat = 0.0f;
+        h.clamped = false;
+        h.alpha = 0.0f;
+        return h;
+    }
     h.ghat = __expf(e);
     const float raw = r.col_op.w * h.ghat;
     h.clamped = raw > kAlphaClamp;

@@ -93,6 +93,7 @@ __global__ __launch_bounds__(256) void project_kernel(
     rec.b_py = make_float4(B.x, B.y, B.z, r.py);
     rec.col_op = make_float4(fmaxf(cr, 0.0f), fmaxf(cg, 0.0f), fmaxf(cb, 0.0f), opac[i]);
     rec.nrm_dep = make_float4(nrm.x, nrm.y, nrm.z, depth);
+    rec.box = record_box(rec, cam);
     recs[g] = rec;
 }
 

@@ -64,6 +64,9 @@ __global__ __launch_bounds__(256) void raster_bwd_kernel(const CamParams* __rest
                                                          float* __restrict__ grad_coef) {
     constexpr int NS = AUX ? 20 : 16;  // LDS slots per record
     __shared__ float4 s_m[kBatch], s_a[kBatch], s_b[kBatch], s_c[kBatch], s_n[kBatch];
+    __shared__ float s_lthr[kBatch];
+    __shared__ uint16_t s_list[4][kBatch];
+    __shared__ uint32_t s_qcnt[4][4];
     __shared__ uint32_t s_pay[kBatch];
     __shared__ float s_grad[kBatch * NS];
     __shared__ uint32_t s_view, s_maxlast;
@@ -124,24 +127,46 @@ __global__ __launch_bounds__(256) void raster_bwd_kernel(const CamParams* __rest
     float suf_r = 0.f, suf_g = 0.f, suf_b = 0.f;
     float suf_d = 0.f, suf_nx = 0.f, suf_ny = 0.f, suf_nz = 0.f, suf_w = 0.f, suf_cw = 0.f;
 
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const int nbatch = (int)((maxlast + kBatch - 1) / kBatch);
     for (int b = nbatch - 1; b >= 0; --b) {
         const uint32_t base = start + (uint32_t)b * kBatch;
-        const uint32_t cnt = min((uint32_t)kBatch, end - base);
+        const uint32_t cnt = min(min((uint32_t)kBatch, end - base), maxlast - (uint32_t)b * kBatch);
+        uint32_t qmask = 0;
         if ((uint32_t)tid < cnt) {
             const uint32_t pay = payloads[base + tid];
-            const Record& r = recs[pay];
+            Record r = recs[pay];
             s_m[tid] = r.m_nt;
             s_a[tid] = r.a_px;
             s_b[tid] = r.b_py;
             s_c[tid] = r.col_op;
             s_n[tid] = r.nrm_dep;
             s_pay[tid] = pay;
+            s_lthr[tid] = record_lthr(r.col_op.w);
+            qmask = quadrant_mask(r.box, tx, ty);
         }
         for (int k = tid; k < (int)cnt * NS; k += 256) s_grad[k] = 0.0f;
+        uint32_t qpos[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint64_t bal = __ballot((qmask >> q) & 1u);
+            qpos[q] = (uint32_t)__popcll(bal & lt_mask);
+            if (lane == 0) s_qcnt[wave][q] = (uint32_t)__popcll(bal);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if ((qmask >> q) & 1u) {
+                uint32_t off = 0;
+                for (int w = 0; w < wave; ++w) off += s_qcnt[w][q];
+                s_list[q][off + qpos[q]] = (uint16_t)tid;
+            }
+        }
+        const int my_count = (int)(s_qcnt[0][wave] + s_qcnt[1][wave] + s_qcnt[2][wave] + s_qcnt[3][wave]);
         __syncthreads();
 
-        for (int j = (int)cnt - 1; j >= 0; --j) {
+        for (int kk = my_count - 1; kk >= 0; --kk) {
+            const int j = s_list[wave][kk];
             const uint32_t rel = (uint32_t)b * kBatch + (uint32_t)j;
             if (!__any(rel < last)) continue;
             Record r;
@@ -158,7 +183,7 @@ __global__ __launch_bounds__(256) void raster_bwd_kernel(const CamParams* __rest
             for (int k = 0; k < 4; ++k) g2[k] = 0.0f;
             bool contrib = false;
             if (rel < last) {
-                const HitEval h = eval_hit(r, sx, sy, pxc, pyc);
+                const HitEval h = eval_hit(r, sx, sy, pxc, pyc, s_lthr[j]);
                 if (h.alpha > 0.0f) {
                     contrib = true;
                     const float one_m = 1.0f - h.alpha;

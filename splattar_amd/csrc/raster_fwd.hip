@@ -4,10 +4,14 @@
 // Restates crates/gs-kernels/src/shaders/rasterize_fwd.wgsl:20-85 and
 // evaluate_hit of raster_common.wgsl:69-95 (the oracle's forward.rs:207-246).
 // MI355X layout: the four wave64s of a workgroup each own an 8x8 quadrant of
-// the tile; surfel records are staged through LDS in batches of 256 (one
-// record per thread, gathered through the sorted payloads); a wave leaves the
-// batch loop once all 64 of its pixels are saturated, and the workgroup stops
-// as soon as every pixel's transmittance is below 1e-4.
+// the tile. Surfel records are staged through LDS in batches of 256 (one
+// record per thread, gathered through the sorted payloads); while staging,
+// each thread tests its record's alpha >= 1/255 footprint box (computed at
+// projection) against the four quadrants and the batch is split into four order-preserving per-quadrant lists, so a wave
+// only walks the records that can touch its 64 pixels. Per pixel, an
+// exponent test rejects non-contributing records before exp(). A wave leaves
+// the batch once its pixels are saturated; the workgroup stops when every
+// pixel's transmittance is below 1e-4.
 //
 // Outputs per pixel (same buffers as the reference Rasterizer):
 //   color  = (sum w c, sum w),  aux = (T_final, sum w t, 0, 0),
@@ -37,6 +41,9 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const CamParams* __rest
                                                          const Record* __restrict__ recs,
                                                          FwdOut out) {
     __shared__ float4 s_m[kBatch], s_a[kBatch], s_b[kBatch], s_c[kBatch], s_n[kBatch];
+    __shared__ float s_lthr[kBatch];
+    __shared__ uint16_t s_list[4][kBatch];
+    __shared__ uint32_t s_qcnt[4][4];
     __shared__ uint32_t s_view;
 
     const uint32_t gtile = blockIdx.x;
@@ -52,6 +59,7 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const CamParams* __rest
     float sx, sy;
     pixel_s(cam, px, py, sx, sy);
     const float pxc = (float)px + 0.5f, pyc = (float)py + 0.5f;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
     const uint32_t start = ranges[2 * (size_t)gtile], end = ranges[2 * (size_t)gtile + 1];
     float T = 1.0f;
@@ -61,24 +69,46 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const CamParams* __rest
 
     for (uint32_t base = start; base < end; base += kBatch) {
         const uint32_t cnt = min((uint32_t)kBatch, end - base);
+        uint32_t qmask = 0;
         if ((uint32_t)tid < cnt) {
-            const Record& r = recs[payloads[base + tid]];
+            Record r = recs[payloads[base + tid]];
             s_m[tid] = r.m_nt;
             s_a[tid] = r.a_px;
             s_b[tid] = r.b_py;
             s_c[tid] = r.col_op;
             s_n[tid] = r.nrm_dep;
+            s_lthr[tid] = record_lthr(r.col_op.w);
+            qmask = quadrant_mask(r.box, tx, ty);
+        }
+        // Order-preserving split of the batch into per-quadrant lists.
+        uint32_t qpos[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint64_t bal = __ballot((qmask >> q) & 1u);
+            qpos[q] = (uint32_t)__popcll(bal & lt_mask);
+            if (lane == 0) s_qcnt[wave][q] = (uint32_t)__popcll(bal);
         }
         __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if ((qmask >> q) & 1u) {
+                uint32_t off = 0;
+                for (int w = 0; w < wave; ++w) off += s_qcnt[w][q];
+                s_list[q][off + qpos[q]] = (uint16_t)tid;
+            }
+        }
+        const uint32_t my_count = s_qcnt[0][wave] + s_qcnt[1][wave] + s_qcnt[2][wave] + s_qcnt[3][wave];
+        __syncthreads();
         if (__any(!done)) {
-            for (uint32_t j = 0; j < cnt; ++j) {
+            for (uint32_t k = 0; k < my_count; ++k) {
+                const uint32_t j = s_list[wave][k];
                 Record r;
                 r.m_nt = s_m[j];
                 r.a_px = s_a[j];
                 r.b_py = s_b[j];
                 r.col_op = s_c[j];
                 r.nrm_dep = s_n[j];
-                const HitEval h = eval_hit(r, sx, sy, pxc, pyc);
+                const HitEval h = eval_hit(r, sx, sy, pxc, pyc, s_lthr[j]);
                 if (!done && h.alpha > 0.0f) {
                     const float w = T * h.alpha;
                     cr += r.col_op.x * w;
