@@ -54,6 +54,9 @@ def parse():
     p.add_argument("--plain-steps", type=int, default=None)
     p.add_argument("--profile-steps", type=int, default=3)
     p.add_argument("--digit-bits", type=int, default=8)
+    p.add_argument("--binning", choices=["tight", "presort", "sort64"], default="tight",
+                   help="tile binning: presort with footprint-clipped rects (same images), "
+                        "depth presort + 32-bit tile sort, or one 64-bit key sort")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-crop", default="480x270")
     return p.parse_args()
@@ -129,6 +132,7 @@ def main():
               "described in DESIGN.md and not built.", file=sys.stderr)
         sys.exit(2)
     import splattar_amd as sa
+    from splattar_amd._lib import GS_BINNING_PRESORT, GS_BINNING_SORT64, GS_BINNING_TIGHT
     from splattar_amd import scenes
 
     ctx = sa.GpuContext(0)
@@ -159,7 +163,9 @@ def main():
     per_view = int(max(ents) * 1.3) + 1024
     cfg = sa.TrainConfig(lam=0.2, dssim_scale=1.0, lr_pos=1.6e-4 * extent, lr_scales=5e-3, lr_quat=1e-3,
                          lr_opacity=5e-2, lr_sh=2.5e-3, max_views_per_step=V, entries_per_view=per_view,
-                         digit_bits=a.digit_bits)
+                         digit_bits=a.digit_bits,
+                         binning={"tight": GS_BINNING_TIGHT, "presort": GS_BINNING_PRESORT,
+                                  "sort64": GS_BINNING_SORT64}[a.binning])
     views = [(sa.Camera(sp.center, sp.quat, sp.focal, deg), t) for sp, t in zip(specs, targets)]
     del targets
 
@@ -233,7 +239,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"C3: {n} surfels SH3 heavy-tailed scales, {V}x{W}x{H} views/step, "
                                f"L1+SSIM, Adam", "surfels": n, "views_per_step": V, "width": W,
-                   "height": H, "entries_per_step": entries_step, "digit_bits": a.digit_bits,
+                   "height": H, "entries_per_step": entries_step, "digit_bits": a.digit_bits, "binning": a.binning,
                    "parallelism": "single-gpu"},
         "plain_steps_per_s": round(1.0 / t_plain, 4),
         "batched_nograph_steps_per_s": round(1.0 / t_batched, 4),

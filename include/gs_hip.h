@@ -211,14 +211,21 @@ typedef struct gs_raster_debug {
 /* Blocking (composes the 64-bit keys when the presort binning ran). */
 gs_status gs_raster_debug_buffers(gs_raster r, gs_raster_debug* out);
 
-/* Binning strategy; both give the same entry order, ranges and 64-bit keys:
+/* Binning strategy. SORT64 and PRESORT give the reference's entry list
+ * exactly (same order, ranges and 64-bit keys):
  *   GS_BINNING_SORT64:  one stable sort of 64-bit (tile << 32 | depth) keys
  *                       over every varying bit (the reference's composite
  *                       of two stable 32-bit sorts, binning.rs:5-8);
  *   GS_BINNING_PRESORT: stable sort of the N surfel depth keys, emission in
- *                       depth order, stable sort of 32-bit tile ids (default). */
+ *                       depth order, stable sort of 32-bit tile ids (default);
+ *   GS_BINNING_TIGHT:   PRESORT with each surfel's tile rect cut to the tiles
+ *                       its conservative alpha >= 1/255 footprint box reaches.
+ *                       It drops only entries that contribute to no pixel, so
+ *                       images and gradients equal PRESORT's; rects, entry
+ *                       counts and ncontrib (list positions) differ. */
 #define GS_BINNING_SORT64 0u
 #define GS_BINNING_PRESORT 1u
+#define GS_BINNING_TIGHT 2u
 gs_status gs_raster_set_binning(gs_raster r, uint32_t mode);
 
 /* ----------------------------------------------------------------- SsimLoss
@@ -282,7 +289,7 @@ typedef struct gs_train_config {
     uint32_t max_views_per_step;
     uint32_t entries_per_view;  /* binning capacity per view (0 = 64 * n) */
     uint32_t digit_bits;        /* 0 = autotune, else 4 or 8 */
-    uint32_t binning;           /* GS_BINNING_PRESORT (default) or GS_BINNING_SORT64 */
+    uint32_t binning;           /* GS_BINNING_SORT64 (0), GS_BINNING_PRESORT or GS_BINNING_TIGHT */
 } gs_train_config;
 
 #define GS_STEP_PLAIN 0u

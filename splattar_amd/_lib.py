@@ -23,7 +23,7 @@ GS_BWD_COLOR_ONLY = 2
 GS_BWD_NO_CAMERA = 4
 GS_STEP_PLAIN, GS_STEP_BATCHED, GS_STEP_GRAPH = 0, 1, 2
 GS_ACT_IDENTITY, GS_ACT_EXP, GS_ACT_SIGMOID = 0, 1, 2
-GS_BINNING_SORT64, GS_BINNING_PRESORT = 0, 1
+GS_BINNING_SORT64, GS_BINNING_PRESORT, GS_BINNING_TIGHT = 0, 1, 2
 
 
 class GsError(RuntimeError):

@@ -271,7 +271,7 @@ class Rasterizer:
         check(lib().gs_raster_set_digit_bits(self.h, bits))
 
     def set_binning(self, mode: int):
-        """L.GS_BINNING_PRESORT (default) or L.GS_BINNING_SORT64."""
+        """L.GS_BINNING_PRESORT (default), L.GS_BINNING_SORT64 or L.GS_BINNING_TIGHT."""
         check(lib().gs_raster_set_binning(self.h, mode))
 
     def forward(self, camera: Camera, num_surfels: int, stream=None):
@@ -392,7 +392,7 @@ class TrainConfig:
     max_views_per_step: int = 4
     entries_per_view: int = 0
     digit_bits: int = 8
-    binning: int = L.GS_BINNING_PRESORT
+    binning: int = L.GS_BINNING_TIGHT
 
 
 class Trainer:

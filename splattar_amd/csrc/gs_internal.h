@@ -323,6 +323,19 @@ __device__ __forceinline__ uint32_t quadrant_mask(const float4 b, uint32_t tx, u
     return m;
 }
 
+// The tiles of rect (x0, y0, x1, y1, inclusive) whose pixel centres
+// 16 t + 0.5 .. 16 t + 15.5 meet the box; x0 > x1 (or y0 > y1) when none do.
+__device__ __forceinline__ uint4 clip_rect_to_box(uint4 rect, const float4 b) {
+    const float bx0 = fminf(fmaxf(b.x, -1e6f), 1e6f), bx1 = fminf(fmaxf(b.z, -1e6f), 1e6f);
+    const float by0 = fminf(fmaxf(b.y, -1e6f), 1e6f), by1 = fminf(fmaxf(b.w, -1e6f), 1e6f);
+    const int tx0 = max((int)rect.x, (int)ceilf((bx0 - 15.5f) * (1.0f / kTile)));
+    const int tx1 = min((int)rect.z, (int)floorf((bx1 - 0.5f) * (1.0f / kTile)));
+    const int ty0 = max((int)rect.y, (int)ceilf((by0 - 15.5f) * (1.0f / kTile)));
+    const int ty1 = min((int)rect.w, (int)floorf((by1 - 0.5f) * (1.0f / kTile)));
+    if (tx0 > tx1 || ty0 > ty1) return make_uint4(1u, 1u, 0u, 0u);
+    return make_uint4((uint32_t)tx0, (uint32_t)ty0, (uint32_t)tx1, (uint32_t)ty1);
+}
+
 __device__ __forceinline__ HitEval eval_hit(const Record& r, float sx, float sy, float pxc, float pyc,
                                             float lthr = -INFINITY) {
     HitEval h;
@@ -386,7 +399,8 @@ hipError_t launch_project(hipStream_t st, const CamParams* cams_dev, uint32_t n_
                           const float* opac, const float* sh, Record* recs, uint4* rects,
                           uint32_t* depth_keys, uint32_t* counts, uint32_t* ranges_to_clear,
                           uint32_t num_ranges, uint32_t* presort_keys = nullptr,
-                          uint32_t* presort_vals = nullptr, uint32_t* item_count = nullptr);
+                          uint32_t* presort_vals = nullptr, uint32_t* item_count = nullptr,
+                          bool clip_to_box = false);
 
 // scan.hip
 size_t scan_scratch_bytes(uint32_t capacity);

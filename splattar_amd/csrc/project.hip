@@ -5,7 +5,9 @@
 // intersection is three 2-FMA dot products instead of four triple products.
 //
 // HBM traffic per (view, surfel): reads 16+8+16+4+12*(d+1)^2 B of parameters,
-// writes an 80-B record, a 16-B rect, a 4-B depth key and a 4-B count.
+// writes a 96-B record, a 16-B rect, a 4-B depth key and a 4-B count.
+// With clip_to_box the tile rect is cut down to the tiles the record's
+// conservative alpha >= 1/255 footprint box reaches (tight binning).
 
 #include "gs_internal.h"
 
@@ -19,7 +21,7 @@ __global__ __launch_bounds__(256) void project_kernel(
     const float* __restrict__ opac, const float* __restrict__ sh, Record* __restrict__ recs,
     uint4* __restrict__ rects, uint32_t* __restrict__ depth_keys, uint32_t* __restrict__ counts,
     uint32_t* __restrict__ ranges_to_clear, uint32_t num_ranges, uint32_t* __restrict__ presort_keys,
-    uint32_t* __restrict__ presort_vals, uint32_t* __restrict__ item_count) {
+    uint32_t* __restrict__ presort_vals, uint32_t* __restrict__ item_count, bool clip_to_box) {
     const uint32_t view = blockIdx.y;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     // The tile-range table of the following binning stage starts cleared
@@ -48,16 +50,6 @@ __global__ __launch_bounds__(256) void project_kernel(
         return;
     }
     const RectOut r = tile_rect(cam, f, depth);
-    uint32_t count = 0u;
-    if (r.on_screen) {
-        rect = make_uint4(r.x0, r.y0, r.x1, r.y1);
-        count = (r.x1 - r.x0 + 1u) * (r.y1 - r.y0 + 1u);
-    }
-    rects[g] = rect;
-    depth_keys[g] = __float_as_uint(depth);
-    if (presort_keys) presort_keys[g] = __float_as_uint(depth);
-    counts[g] = count;
-    if (count == 0u) return;  // no entry will ever read the record
 
     // Ray-splat setup.
     const float3 c = make_float3(f.cx_, f.cy_, f.cz_);
@@ -67,6 +59,29 @@ __global__ __launch_bounds__(256) void project_kernel(
     const float3 A = cross3(tv, c);
     const float3 B = cross3(c, tu);
     const float Nt = -dot3(M, c);
+    Record rec;
+    rec.m_nt = make_float4(M.x, M.y, M.z, Nt);
+    rec.a_px = make_float4(A.x, A.y, A.z, r.px);
+    rec.b_py = make_float4(B.x, B.y, B.z, r.py);
+    rec.col_op.w = opac[i];
+
+    uint32_t count = 0u;
+    if (r.on_screen) {
+        rect = make_uint4(r.x0, r.y0, r.x1, r.y1);
+        rec.box = record_box(rec, cam);
+        // Tight binning keeps only the tiles the alpha >= 1/255 footprint
+        // reaches; the dropped entries would contribute nothing anywhere.
+        if (clip_to_box) rect = clip_rect_to_box(rect, rec.box);
+        if (rect.x <= rect.z && rect.y <= rect.w)
+            count = (rect.z - rect.x + 1u) * (rect.w - rect.y + 1u);
+        else
+            rect = make_uint4(1u, 1u, 0u, 0u);
+    }
+    rects[g] = rect;
+    depth_keys[g] = __float_as_uint(depth);
+    if (presort_keys) presort_keys[g] = __float_as_uint(depth);
+    counts[g] = count;
+    if (count == 0u) return;  // no entry will ever read the record
 
     // View-dependent colour (surfel_prep_fwd.wgsl:113-147, 183-186).
     const float inv_len = 1.0f / sqrtf(f.xx * f.xx + f.xy * f.xy + f.xz * f.xz);
@@ -87,13 +102,8 @@ __global__ __launch_bounds__(256) void project_kernel(
         nrm = make_float3(M.x * inv, M.y * inv, M.z * inv);
         if (dot3(nrm, c) > 0.0f) nrm = make_float3(-nrm.x, -nrm.y, -nrm.z);
     }
-    Record rec;
-    rec.m_nt = make_float4(M.x, M.y, M.z, Nt);
-    rec.a_px = make_float4(A.x, A.y, A.z, r.px);
-    rec.b_py = make_float4(B.x, B.y, B.z, r.py);
     rec.col_op = make_float4(fmaxf(cr, 0.0f), fmaxf(cg, 0.0f), fmaxf(cb, 0.0f), opac[i]);
     rec.nrm_dep = make_float4(nrm.x, nrm.y, nrm.z, depth);
-    rec.box = record_box(rec, cam);
     recs[g] = rec;
 }
 
@@ -104,13 +114,13 @@ hipError_t launch_project(hipStream_t st, const CamParams* cams_dev, uint32_t n_
                           const float* opac, const float* sh, Record* recs, uint4* rects,
                           uint32_t* depth_keys, uint32_t* counts, uint32_t* ranges_to_clear,
                           uint32_t num_ranges, uint32_t* presort_keys, uint32_t* presort_vals,
-                          uint32_t* item_count) {
+                          uint32_t* item_count, bool clip_to_box) {
     if (n == 0 && num_ranges == 0 && item_count == nullptr) return hipSuccess;
     uint32_t blocks = div_up(n > num_ranges ? (n ? n : 1) : num_ranges, 256);
     dim3 grid(blocks, n_views);
     hipLaunchKernelGGL(project_kernel, grid, dim3(256), 0, st, cams_dev, n, pos, scales, quats, opac,
                        sh, recs, rects, depth_keys, counts, ranges_to_clear, num_ranges, presort_keys,
-                       presort_vals, item_count);
+                       presort_vals, item_count, clip_to_box);
     return hipGetLastError();
 }
 

@@ -11,8 +11,9 @@
 //     camera-space (c, tu, tv) happens once per surfel in geom_bwd instead of
 //     four triple-product gradients per pixel-hit;
 //   * per record, each wave64 reduces its 64 pixels' contributions with a
-//     transposed butterfly (17 lane exchanges for 16 values instead of 96),
-//     adds them into an LDS slot (ds_add_f32), and the batch's slots are
+//     transposed butterfly (17 lane exchanges for 16 values instead of 96,
+//     on permlane swaps and DPP rather than LDS permutes), adds them into an
+//     LDS slot (ds_add_f32), and the batch's slots are
 //     flushed with hardware global FP32 atomics (no CAS loops).
 // AUX = true adds the depth / normal image losses and the in-walk depth
 // distortion loss (backward.rs:126-153).
@@ -32,27 +33,64 @@ __device__ __forceinline__ uint32_t view_of_tile(const CamParams* cams, uint32_t
     return v;
 }
 
-// Sum NV values (power of two) over the 64 lanes of a wave. On return lane l
-// holds the total of value index (l >> log2(64/NV)) & (NV-1); lanes with
-// (l & (64/NV - 1)) == 0 are the designated writers.
+// Cross-lane exchange without the LDS crossbar: DPP for partners inside a
+// 16-lane row, the gfx950 permlane swaps across rows. xlane<OFF>(x) returns x
+// of a partner lane whose index differs from this lane's in bit OFF (and
+// possibly in lower bits): row_mirror (i <-> 15-i) for 8, row_half_mirror
+// (i <-> 7-i) for 4, quad permutations (xor) for 2 and 1.
+template <int OFF>
+__device__ __forceinline__ float xlane(float x) {
+    static_assert(OFF == 8 || OFF == 4 || OFF == 2 || OFF == 1, "row-local exchange only");
+    constexpr int ctrl = OFF == 8 ? 0x140 : OFF == 4 ? 0x141 : OFF == 2 ? 0x4E : 0xB1;
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), ctrl, 0xF, 0xF, false));
+}
+
+// One transposition step at lane bit OFF: lanes with the bit clear end up
+// with lo(self) + lo(partner), lanes with it set with hi(self) + hi(partner).
+// For OFF = 32 / 16 one v_permlane{32,16}_swap moves the upper half of lo
+// and the lower half of hi across in a single VALU op.
+template <int OFF>
+__device__ __forceinline__ float pair_sum(float lo, float hi, int lane) {
+    if constexpr (OFF == 32) {
+        const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(lo), __float_as_uint(hi), false, false);
+        return __uint_as_float(p[0]) + __uint_as_float(p[1]);
+    } else if constexpr (OFF == 16) {
+        const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(lo), __float_as_uint(hi), false, false);
+        return __uint_as_float(p[0]) + __uint_as_float(p[1]);
+    } else {
+        const bool upper = (lane & OFF) != 0;
+        const float send = upper ? lo : hi;
+        const float keep = upper ? hi : lo;
+        return keep + xlane<OFF>(send);
+    }
+}
+
+template <int OFF>
+__device__ __forceinline__ float finish_sum(float r) {
+    if constexpr (OFF >= 1) return finish_sum<OFF / 2>(r + xlane<OFF>(r));
+    else return r;
+}
+
+template <int COUNT, int OFF, int NV>
+__device__ __forceinline__ float transpose_reduce(float (&v)[NV], int lane) {
+    if constexpr (COUNT > 1) {
+#pragma unroll
+        for (int i = 0; i < COUNT / 2; ++i) v[i] = pair_sum<OFF>(v[i], v[COUNT / 2 + i], lane);
+        return transpose_reduce<COUNT / 2, OFF / 2, NV>(v, lane);
+    } else {
+        return finish_sum<OFF>(v[0]);
+    }
+}
+
+// Sum NV values (power of two, 4 <= NV <= 16) over the 64 lanes of a wave:
+// NV - 1 exchanges halve the value count per step, then the remaining lane
+// bits are summed. On return lane l holds the total of value index
+// (l >> log2(64/NV)) & (NV-1); lanes with (l & (64/NV - 1)) == 0 are the
+// designated writers.
 template <int NV>
 __device__ __forceinline__ float wave_reduce_transpose(float (&v)[NV], int lane) {
-    int off = 32;
-#pragma unroll
-    for (int count = NV; count > 1; count >>= 1, off >>= 1) {
-        const bool upper = (lane & off) != 0;
-#pragma unroll
-        for (int i = 0; i < count / 2; ++i) {
-            const float lo = v[i], hi = v[count / 2 + i];
-            const float send = upper ? lo : hi;
-            const float keep = upper ? hi : lo;
-            v[i] = keep + __shfl_xor(send, off);
-        }
-    }
-    float r = v[0];
-#pragma unroll
-    for (; off >= 1; off >>= 1) r += __shfl_xor(r, off);
-    return r;
+    static_assert(NV >= 4 && NV <= 16 && (NV & (NV - 1)) == 0, "4, 8 or 16 values");
+    return transpose_reduce<NV, 32, NV>(v, lane);
 }
 
 template <bool AUX>

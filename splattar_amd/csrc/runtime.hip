@@ -534,7 +534,7 @@ struct RasterCore {
     uint32_t* ncontrib = nullptr;
     float* grad_coef = nullptr;
     float* cam_partials = nullptr;
-    // Depth-presort path (binning = GS_BINNING_PRESORT).
+    // Depth-presort path (binning = GS_BINNING_PRESORT or GS_BINNING_TIGHT).
     uint32_t* pre_keys[2] = {nullptr, nullptr};
     uint32_t* pre_vals[2] = {nullptr, nullptr};
     uint32_t* offs_sorted = nullptr;
@@ -705,12 +705,13 @@ struct RasterCore {
             max_tile_id = std::max(max_tile_id, cams[v].tile_base + cams[v].tiles_x * cams[v].tiles_y);
         const uint32_t items = n * nv;
         FwdOut o{out_color, out_aux, out_normal, ncontrib};
-        if (binning == GS_BINNING_PRESORT) {
+        if (binning != GS_BINNING_SORT64) {
             // Stable depth presort of the items, emission in that order, then
             // a stable sort on the tile bits only (binning.hip).
             GS_CHECK(P.mark(st, GS_STAGE_PROJECT));
             GS_HIP(launch_project(st, cd, nv, n, pos, sc, q, op, sh, recs, rects, depth_keys, offsets,
-                                  ranges, 2 * max_tile_id, pre_keys[0], pre_vals[0], item_count));
+                                  ranges, 2 * max_tile_id, pre_keys[0], pre_vals[0], item_count,
+                                  binning == GS_BINNING_TIGHT));
             GS_CHECK(P.mark(st, GS_STAGE_SORT));
             int alt = 0;
             GS_HIP(launch_radix_sort_u32(st, pre_keys[0], pre_vals[0], pre_keys[1], pre_vals[1],
@@ -929,7 +930,7 @@ gs_status gs_raster_set_digit_bits(gs_raster r, uint32_t digit_bits) {
 
 gs_status gs_raster_set_binning(gs_raster r, uint32_t mode) {
     GS_REQUIRE(r != nullptr, "raster is null");
-    GS_REQUIRE(mode == GS_BINNING_SORT64 || mode == GS_BINNING_PRESORT, "unknown binning mode");
+    GS_REQUIRE(mode <= GS_BINNING_TIGHT, "unknown binning mode");
     r->core.binning = mode;
     return GS_OK;
 }
@@ -1332,8 +1333,7 @@ gs_status gs_trainer_create(gs_ctx ctx, const gs_train_config* cfg, const gs_sce
     GS_REQUIRE(n_views > 0, "need at least one view");
     GS_REQUIRE(cfg->max_views_per_step >= 1 && cfg->max_views_per_step <= 64,
                "max_views_per_step must be in 1..64");
-    GS_REQUIRE(cfg->binning == GS_BINNING_SORT64 || cfg->binning == GS_BINNING_PRESORT,
-               "unknown binning mode");
+    GS_REQUIRE(cfg->binning <= GS_BINNING_TIGHT, "unknown binning mode");
     GS_REQUIRE(cfg->digit_bits == 0 || cfg->digit_bits == 4 || cfg->digit_bits == 8,
                "digit_bits must be 0, 4 or 8");
     GS_REQUIRE(init->n > 0, "need at least one surfel");
