@@ -59,7 +59,22 @@ def parse():
                         "depth presort + 32-bit tile sort, or one 64-bit key sort")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-crop", default="480x270")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="only run 5 device copies of 1 GiB (read 1 GiB + write 1 GiB each), the "
+                        "known-size reference for scaling FETCH_SIZE / WRITE_SIZE counters")
     return p.parse_args()
+
+
+def copy_calibration():
+    import torch
+    n = (1 << 30) // 4
+    src = torch.rand(n, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 5}))
 
 
 def stage_bytes(n, V, E, P, T, sort_passes, S=232):
@@ -127,6 +142,9 @@ def cpu_baseline(scene, spec, crop, views_per_step):
 
 def main():
     a = parse()
+    if a.copy_calibration:
+        copy_calibration()
+        return
     if a.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for --gpus {a.gpus}); multi-GPU sharding is "
               "described in DESIGN.md and not built.", file=sys.stderr)

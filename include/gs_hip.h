@@ -228,6 +228,12 @@ gs_status gs_raster_debug_buffers(gs_raster r, gs_raster_debug* out);
 #define GS_BINNING_TIGHT 2u
 gs_status gs_raster_set_binning(gs_raster r, uint32_t mode);
 
+/* Per-quadrant footprint culling in the rasterizer kernels (default on).
+ * No counterpart in the reference: a verification switch. Culling only skips
+ * records that reach no pixel of a wave's 8x8 block, so images, ncontrib and
+ * gradients are the same with it off. */
+gs_status gs_raster_set_culling(gs_raster r, int enable);
+
 /* ----------------------------------------------------------------- SsimLoss
  * first_attempt/crates/gs-train/src/ssim.rs:25-298: fused L1 + D-SSIM,
  * writes dL/d(image) rgb (w = 0) into dl_out. Loss =

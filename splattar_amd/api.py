@@ -274,6 +274,10 @@ class Rasterizer:
         """L.GS_BINNING_PRESORT (default), L.GS_BINNING_SORT64 or L.GS_BINNING_TIGHT."""
         check(lib().gs_raster_set_binning(self.h, mode))
 
+    def set_culling(self, enable: bool):
+        """Per-quadrant footprint culling in the rasterizer kernels (default on)."""
+        check(lib().gs_raster_set_culling(self.h, 1 if enable else 0))
+
     def forward(self, camera: Camera, num_surfels: int, stream=None):
         rc = camera.c()
         check(lib().gs_raster_forward(self.h, stream, C.byref(rc), num_surfels))

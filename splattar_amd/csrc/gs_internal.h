@@ -39,8 +39,10 @@ struct __align__(16) Record {
     float4 col_op;   // clamped SH color rgb, opacity
     float4 nrm_dep;  // camera-facing unit normal xyz, center depth
     float4 box;      // conservative alpha >= 1/255 footprint (x0, y0, x1, y1), pixel centres
+    float4 ell0;     // footprint ellipse: centre (X, Y) in pixels, quadratic form h00, h01
+    float4 ell1;     // h11, 1/h00, 1/h11, screen-disk radius (pixels)
 };
-static_assert(sizeof(Record) == 96, "record layout");
+static_assert(sizeof(Record) == 128, "record layout");
 
 // Coefficient-space gradient accumulator per (view, surfel), summed over the
 // pixels of every tile the surfel reaches (raster_bwd), chained to parameter
@@ -258,20 +260,29 @@ __device__ __forceinline__ float record_lthr(float opacity) {
     return opacity * 255.0f > 1.0f ? -(logf(opacity * 255.0f) + 1e-3f) : 1.0f;  // 1 > any e: never
 }
 
-// Conservative box (pixel-centre coordinates) of the pixels at which a record
-// can reach alpha >= 1/255: the screen low-pass disk dx^2 + dy^2 <= L around
-// (px, py), united with the image of the ray-branch disk u^2 + v^2 <= 2L.
-// With s = (sx, sy, 1), the latter is the conic s^T Q s <= 0,
-// Q = A A^T + B B^T - 2L M M^T; when its quadratic part is positive definite
-// (the disk lies in front of the camera plane) the box comes from the dual
-// conic adj(Q): the tangent lines sx = k solve adj00 - 2k adj02 + k^2 adj22 = 0
-// (and likewise for sy). Otherwise the ray part is unbounded. Computed in
-// double with 1 px of margin, once per (view, surfel) at projection time.
-// The result is (x0, y0, x1, y1); a record that never contributes gets an
-// empty box (x0 > x1).
-__device__ inline float4 record_box(const Record& r, const CamParams& cam) {
+// Conservative footprint of a record: the pixels at which it can reach
+// alpha >= 1/255 lie in the screen low-pass disk dx^2 + dy^2 <= L around
+// (px, py) united with the image of the ray-branch disk u^2 + v^2 <= 2L,
+// L = ln(255 opacity) + 2e-3. With s = (sx, sy, 1), the latter is the conic
+// s^T Q s <= 0, Q = A A^T + B B^T - 2L M M^T (u = A.s/D, v = B.s/D, D = M.s).
+// When its quadratic part Q2 is positive definite it is an ellipse with
+// centre c = -Q2^-1 (Q02, Q12) and minimum q_c = Q22 + (Q02, Q12).c; in pixel
+// offsets d from the centre it reads d^T H d <= 1 with
+// H = diag(-1/f, 1/f) Q2 diag(-1/f, 1/f) / (-q_c). Its bounding box comes
+// from the dual conic adj(Q): the tangent lines sx = k solve
+// adj00 - 2k adj02 + k^2 adj22 = 0 (likewise for sy). Otherwise the ray
+// part is unbounded and the footprint is the whole image. Computed in double
+// once per (view, surfel) at projection; tests add 1 px of margin.
+constexpr float kFar = 1e18f;
+
+__device__ inline void record_footprint(Record& r, const CamParams& cam) {
     const float op = r.col_op.w;
-    if (!(op * 255.0f > 1.0f)) return make_float4(1e30f, 1e30f, -1e30f, -1e30f);
+    if (!(op * 255.0f > 1.0f)) {  // never contributes: empty box, empty ellipse, no disk
+        r.box = make_float4(kFar, kFar, -kFar, -kFar);
+        r.ell0 = make_float4(kFar, kFar, 1.0f, 0.0f);
+        r.ell1 = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        return;
+    }
     const double L = log((double)op * 255.0) + 2e-3;
     const double rs = sqrt(L) + 0.5;
     double x0 = r.a_px.w - rs, x1 = r.a_px.w + rs, y0 = r.b_py.w - rs, y1 = r.b_py.w + rs;
@@ -284,9 +295,11 @@ __device__ inline float4 record_box(const Record& r, const CamParams& cam) {
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) Q[i][j] = A[i] * A[j] + B[i] * B[j] - k * M[i] * M[j];
+    const double f = cam.focal;
     const double det2 = Q[0][0] * Q[1][1] - Q[0][1] * Q[0][1];
     const double mag = Q[0][0] * Q[1][1] + Q[0][1] * Q[0][1];
     bool bounded = Q[0][0] > 0.0 && det2 > 1e-9 * mag;
+    double ex = kFar, ey = kFar, h00 = 1.0, h01 = 0.0, h11 = 1.0;  // empty ellipse by default
     if (bounded) {
         const double a00 = Q[1][1] * Q[2][2] - Q[1][2] * Q[1][2];
         const double a11 = Q[0][0] * Q[2][2] - Q[0][2] * Q[0][2];
@@ -299,26 +312,72 @@ __device__ inline float4 record_box(const Record& r, const CamParams& cam) {
             const double sx_lo = (a02 - sdx) / det2, sx_hi = (a02 + sdx) / det2;
             const double sy_lo = (a12 - sdy) / det2, sy_hi = (a12 + sdy) / det2;
             // X = cx - f sx (pixel centre), Y = cy + f sy.
-            const double f = cam.focal;
             x0 = fmin(x0, cam.cx - f * sx_hi - 1.0);
             x1 = fmax(x1, cam.cx - f * sx_lo + 1.0);
             y0 = fmin(y0, cam.cy + f * sy_lo - 1.0);
             y1 = fmax(y1, cam.cy + f * sy_hi + 1.0);
         }
+        const double csx = -(Q[1][1] * Q[0][2] - Q[0][1] * Q[1][2]) / det2;
+        const double csy = -(Q[0][0] * Q[1][2] - Q[0][1] * Q[0][2]) / det2;
+        const double qc = Q[2][2] + Q[0][2] * csx + Q[1][2] * csy;
+        if (bounded && qc < 0.0) {
+            const double sc = 1.0 / (-qc * f * f);
+            ex = cam.cx - f * csx;
+            ey = cam.cy + f * csy;
+            h00 = Q[0][0] * sc;
+            h01 = -Q[0][1] * sc;
+            h11 = Q[1][1] * sc;
+        }
     }
-    if (!bounded || !(x0 == x0) || !(y0 == y0) || !(x1 == x1) || !(y1 == y1))
-        return make_float4(-1e30f, -1e30f, 1e30f, 1e30f);
-    return make_float4((float)x0, (float)y0, (float)x1, (float)y1);
+    const bool finite = x0 == x0 && y0 == y0 && x1 == x1 && y1 == y1 && ex == ex && ey == ey &&
+                        h00 == h00 && h01 == h01 && h11 == h11 && h00 > 0.0 && h11 > 0.0;
+    if (!bounded || !finite) {  // whole image: box unbounded, H = 0 accepts every block
+        r.box = make_float4(-kFar, -kFar, kFar, kFar);
+        r.ell0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        r.ell1 = make_float4(0.0f, 0.0f, 0.0f, (float)rs);
+        return;
+    }
+    r.box = make_float4((float)x0, (float)y0, (float)x1, (float)y1);
+    r.ell0 = make_float4((float)ex, (float)ey, (float)h00, (float)h01);
+    r.ell1 = make_float4((float)h11, (float)(1.0 / h00), (float)(1.0 / h11), (float)rs);
+}
+
+// True when the record's footprint can reach the pixel-centre rectangle
+// [x0, x1] x [y0, y1]: box overlap, then the screen disk or the ellipse
+// against the rectangle grown by 1 px. For the ellipse, the rectangle
+// contains the centre or the minimum of d^T H d over its four edges is <= 1
+// (a convex quadratic is minimised on the boundary when the centre is
+// outside).
+__device__ __forceinline__ bool footprint_hits(const Record& r, float x0, float y0, float x1, float y1) {
+    const float4 b = r.box;
+    if (!(b.z >= x0 && b.x <= x1 && b.w >= y0 && b.y <= y1)) return false;
+    x0 -= 1.0f;
+    y0 -= 1.0f;
+    x1 += 1.0f;
+    y1 += 1.0f;
+    const float px = r.a_px.w, py = r.b_py.w, rd = r.ell1.w;
+    const float qx = fminf(fmaxf(px, x0), x1) - px, qy = fminf(fmaxf(py, y0), y1) - py;
+    if (qx * qx + qy * qy <= rd * rd) return true;
+    const float dx0 = x0 - r.ell0.x, dx1 = x1 - r.ell0.x, dy0 = y0 - r.ell0.y, dy1 = y1 - r.ell0.y;
+    if (dx0 <= 0.0f && dx1 >= 0.0f && dy0 <= 0.0f && dy1 >= 0.0f) return true;
+    const float h00 = r.ell0.z, h01 = r.ell0.w, h11 = r.ell1.x, i00 = r.ell1.y, i11 = r.ell1.z;
+    auto quad = [&](float dx, float dy) { return (h00 * dx + 2.0f * h01 * dy) * dx + h11 * dy * dy; };
+    const float ya = fminf(fmaxf(-h01 * dx0 * i11, dy0), dy1);
+    const float yb = fminf(fmaxf(-h01 * dx1 * i11, dy0), dy1);
+    const float xa = fminf(fmaxf(-h01 * dy0 * i00, dx0), dx1);
+    const float xb = fminf(fmaxf(-h01 * dy1 * i00, dx0), dx1);
+    const float m = fminf(fminf(quad(dx0, ya), quad(dx1, yb)), fminf(quad(xa, dy0), quad(xb, dy1)));
+    return m <= 1.0f;
 }
 
 // Bit q (q = 0..3, quadrant (q & 1, q >> 1) of the 16x16 tile at (tx, ty))
-// is set when the box (x0, y0, x1, y1) reaches that 8x8 pixel block.
-__device__ __forceinline__ uint32_t quadrant_mask(const float4 b, uint32_t tx, uint32_t ty) {
+// is set when the record's footprint reaches that 8x8 pixel block.
+__device__ __forceinline__ uint32_t quadrant_mask(const Record& r, uint32_t tx, uint32_t ty) {
     uint32_t m = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const float qx = (float)(tx * kTile + (q & 1) * 8), qy = (float)(ty * kTile + (q >> 1) * 8);
-        if (b.z >= qx + 0.5f && b.x <= qx + 7.5f && b.w >= qy + 0.5f && b.y <= qy + 7.5f) m |= 1u << q;
+        if (footprint_hits(r, qx + 0.5f, qy + 0.5f, qx + 7.5f, qy + 7.5f)) m |= 1u << q;
     }
     return m;
 }
@@ -342,17 +401,17 @@ __device__ __forceinline__ HitEval eval_hit(const Record& r, float sx, float sy,
     const float D = r.m_nt.x * sx + r.m_nt.y * sy + r.m_nt.z;
     const float Nu = r.a_px.x * sx + r.a_px.y * sy + r.a_px.z;
     const float Nv = r.b_py.x * sx + r.b_py.y * sy + r.b_py.z;
-    float e_ray = -INFINITY;
-    h.u = h.v = h.t = 0.0f;
-    h.inv_d = 0.0f;
-    if (fabsf(D) > 1e-12f) {
-        const float inv = 1.0f / D;
-        h.u = Nu * inv;
-        h.v = Nv * inv;
-        h.t = r.m_nt.w * inv;
-        h.inv_d = inv;
-        if (h.t > kNearDepth) e_ray = -(h.u * h.u + h.v * h.v) * 0.5f;
-    }
+    // Branch-free: |D| <= 1e-12 gives inv = 0, hence t = 0 and no ray hit.
+    // v_rcp_f32 (1 ulp) refined by one Newton step r += r (1 - D r) instead
+    // of the IEEE division sequence.
+    const float r0 = __builtin_amdgcn_rcpf(D);
+    const float r1 = fmaf(r0, fmaf(-D, r0, 1.0f), r0);
+    const float inv = fabsf(D) > 1e-12f ? r1 : 0.0f;
+    h.u = Nu * inv;
+    h.v = Nv * inv;
+    h.t = r.m_nt.w * inv;
+    h.inv_d = inv;
+    const float e_ray = h.t > kNearDepth ? -(h.u * h.u + h.v * h.v) * 0.5f : -INFINITY;
     const float dx = pxc - r.a_px.w, dy = pyc - r.b_py.w;
     const float e_scr = -(dx * dx + dy * dy);   // 2 sigma^2 = 1 (LOWPASS_SIGMA2 = 0.5)
     h.ray = e_ray >= e_scr;
@@ -452,7 +511,8 @@ struct FwdOut {
 };
 hipError_t launch_raster_fwd(hipStream_t st, const CamParams* cams_dev, const CamParams* cams_host,
                              uint32_t n_views, uint32_t total_tiles, const uint32_t* ranges,
-                             const uint32_t* payloads, const Record* recs, FwdOut out);
+                             const uint32_t* payloads, const Record* recs, FwdOut out,
+                             bool cull = true);
 
 // raster_bwd.hip
 struct BwdIn {
@@ -465,7 +525,7 @@ struct BwdIn {
 hipError_t launch_raster_bwd(hipStream_t st, const CamParams* cams_dev, const CamParams* cams_host,
                              uint32_t n_views, uint32_t total_tiles, const uint32_t* ranges,
                              const uint32_t* payloads, const Record* recs, BwdIn in,
-                             float* grad_coef, bool aux);
+                             float* grad_coef, bool aux, bool cull = true);
 
 // geom_bwd.hip
 struct ParamGrads {

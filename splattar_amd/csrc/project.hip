@@ -5,7 +5,7 @@
 // intersection is three 2-FMA dot products instead of four triple products.
 //
 // HBM traffic per (view, surfel): reads 16+8+16+4+12*(d+1)^2 B of parameters,
-// writes a 96-B record, a 16-B rect, a 4-B depth key and a 4-B count.
+// writes a 128-B record, a 16-B rect, a 4-B depth key and a 4-B count.
 // With clip_to_box the tile rect is cut down to the tiles the record's
 // conservative alpha >= 1/255 footprint box reaches (tight binning).
 
@@ -68,7 +68,7 @@ __global__ __launch_bounds__(256) void project_kernel(
     uint32_t count = 0u;
     if (r.on_screen) {
         rect = make_uint4(r.x0, r.y0, r.x1, r.y1);
-        rec.box = record_box(rec, cam);
+        record_footprint(rec, cam);
         // Tight binning keeps only the tiles the alpha >= 1/255 footprint
         // reaches; the dropped entries would contribute nothing anywhere.
         if (clip_to_box) rect = clip_rect_to_box(rect, rec.box);

@@ -99,7 +99,7 @@ __global__ __launch_bounds__(256) void raster_bwd_kernel(const CamParams* __rest
                                                          const uint32_t* __restrict__ ranges,
                                                          const uint32_t* __restrict__ payloads,
                                                          const Record* __restrict__ recs, BwdIn in,
-                                                         float* __restrict__ grad_coef) {
+                                                         float* __restrict__ grad_coef, bool cull) {
     constexpr int NS = AUX ? 20 : 16;  // LDS slots per record
     __shared__ float4 s_m[kBatch], s_a[kBatch], s_b[kBatch], s_c[kBatch], s_n[kBatch];
     __shared__ float s_lthr[kBatch];
@@ -181,7 +181,7 @@ __global__ __launch_bounds__(256) void raster_bwd_kernel(const CamParams* __rest
             s_n[tid] = r.nrm_dep;
             s_pay[tid] = pay;
             s_lthr[tid] = record_lthr(r.col_op.w);
-            qmask = quadrant_mask(r.box, tx, ty);
+            qmask = cull ? quadrant_mask(r, tx, ty) : 0xFu;
         }
         for (int k = tid; k < (int)cnt * NS; k += 256) s_grad[k] = 0.0f;
         uint32_t qpos[4];
@@ -318,14 +318,14 @@ __global__ __launch_bounds__(256) void raster_bwd_kernel(const CamParams* __rest
 hipError_t launch_raster_bwd(hipStream_t st, const CamParams* cams_dev, const CamParams* /*cams_host*/,
                              uint32_t n_views, uint32_t total_tiles, const uint32_t* ranges,
                              const uint32_t* payloads, const Record* recs, BwdIn in,
-                             float* grad_coef, bool aux) {
+                             float* grad_coef, bool aux, bool cull) {
     if (total_tiles == 0) return hipSuccess;
     if (aux)
         hipLaunchKernelGGL(raster_bwd_kernel<true>, dim3(total_tiles), dim3(256), 0, st, cams_dev,
-                           n_views, ranges, payloads, recs, in, grad_coef);
+                           n_views, ranges, payloads, recs, in, grad_coef, cull);
     else
         hipLaunchKernelGGL(raster_bwd_kernel<false>, dim3(total_tiles), dim3(256), 0, st, cams_dev,
-                           n_views, ranges, payloads, recs, in, grad_coef);
+                           n_views, ranges, payloads, recs, in, grad_coef, cull);
     return hipGetLastError();
 }
 

@@ -6,8 +6,9 @@
 // MI355X layout: the four wave64s of a workgroup each own an 8x8 quadrant of
 // the tile. Surfel records are staged through LDS in batches of 256 (one
 // record per thread, gathered through the sorted payloads); while staging,
-// each thread tests its record's alpha >= 1/255 footprint box (computed at
-// projection) against the four quadrants and the batch is split into four order-preserving per-quadrant lists, so a wave
+// each thread tests its record's alpha >= 1/255 footprint (box, screen disk
+// and ellipse, computed at projection) against the four quadrants, and the
+// batch is split into four order-preserving per-quadrant lists, so a wave
 // only walks the records that can touch its 64 pixels. Per pixel, an
 // exponent test rejects non-contributing records before exp(). A wave leaves
 // the batch once its pixels are saturated; the workgroup stops when every
@@ -39,7 +40,7 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const CamParams* __rest
                                                          const uint32_t* __restrict__ ranges,
                                                          const uint32_t* __restrict__ payloads,
                                                          const Record* __restrict__ recs,
-                                                         FwdOut out) {
+                                                         FwdOut out, bool cull) {
     __shared__ float4 s_m[kBatch], s_a[kBatch], s_b[kBatch], s_c[kBatch], s_n[kBatch];
     __shared__ float s_lthr[kBatch];
     __shared__ uint16_t s_list[4][kBatch];
@@ -78,7 +79,7 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const CamParams* __rest
             s_c[tid] = r.col_op;
             s_n[tid] = r.nrm_dep;
             s_lthr[tid] = record_lthr(r.col_op.w);
-            qmask = quadrant_mask(r.box, tx, ty);
+            qmask = cull ? quadrant_mask(r, tx, ty) : 0xFu;
         }
         // Order-preserving split of the batch into per-quadrant lists.
         uint32_t qpos[4];
@@ -141,10 +142,10 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const CamParams* __rest
 
 hipError_t launch_raster_fwd(hipStream_t st, const CamParams* cams_dev, const CamParams* /*cams_host*/,
                              uint32_t n_views, uint32_t total_tiles, const uint32_t* ranges,
-                             const uint32_t* payloads, const Record* recs, FwdOut out) {
+                             const uint32_t* payloads, const Record* recs, FwdOut out, bool cull) {
     if (total_tiles == 0) return hipSuccess;
     hipLaunchKernelGGL(raster_fwd_kernel, dim3(total_tiles), dim3(256), 0, st, cams_dev, n_views,
-                       ranges, payloads, recs, out);
+                       ranges, payloads, recs, out, cull);
     return hipGetLastError();
 }
 

@@ -541,6 +541,7 @@ struct RasterCore {
     uint32_t* item_count = nullptr;
     uint32_t* tkeys[2] = {nullptr, nullptr};  // 32-bit tile keys, stored inside keys[1]
     uint32_t binning = GS_BINNING_PRESORT;
+    bool cull = true;                // per-quadrant footprint culling in the rasterizers
     bool keys64_valid = false;       // keys[result_alt] holds 64-bit keys of the last forward
     uint32_t digit_bits = 8;
     int result_alt = 0;              // which ping-pong buffer holds the sorted entries
@@ -731,7 +732,8 @@ struct RasterCore {
             keys64_valid = false;
             last_sort_count = sort_count + v0;
             GS_CHECK(P.mark(st, GS_STAGE_RASTER_FWD));
-            GS_HIP(launch_raster_fwd(st, cd, cams.data() + v0, nv, tiles, ranges, pays[result_alt], recs, o));
+            GS_HIP(launch_raster_fwd(st, cd, cams.data() + v0, nv, tiles, ranges, pays[result_alt], recs, o,
+                                     cull));
             return GS_OK;
         }
         // 64-bit (tile || depth) key path: one sort over every key bit that varies.
@@ -752,7 +754,8 @@ struct RasterCore {
         keys64_valid = true;
         last_sort_count = sort_count + v0;
         GS_CHECK(P.mark(st, GS_STAGE_RASTER_FWD));
-        GS_HIP(launch_raster_fwd(st, cd, cams.data() + v0, nv, tiles, ranges, pays[result_alt], recs, o));
+        GS_HIP(launch_raster_fwd(st, cd, cams.data() + v0, nv, tiles, ranges, pays[result_alt], recs, o,
+                                 cull));
         return GS_OK;
     }
 
@@ -771,7 +774,7 @@ struct RasterCore {
         BwdIn in{dl_dcolor, dl_dnormal, out_color, out_aux, ncontrib};
         GS_CHECK(P.mark(st, GS_STAGE_RASTER_BWD));
         GS_HIP(launch_raster_bwd(st, cd, cams.data() + v0, nv, tiles, ranges, pays[result_alt], recs, in,
-                                 grad_coef, aux));
+                                 grad_coef, aux, cull));
         GS_CHECK(P.mark(st, GS_STAGE_GEOM_BWD));
         GS_HIP(launch_geom_bwd(st, cd, nv, n, pos, sc, q, sh, recs, grad_coef, g, accumulate,
                                cam_partials));
@@ -925,6 +928,12 @@ gs_status gs_raster_set_digit_bits(gs_raster r, uint32_t digit_bits) {
     GS_REQUIRE(r != nullptr, "raster is null");
     GS_REQUIRE(digit_bits == 4 || digit_bits == 8, "digit_bits must be 4 or 8");
     r->core.digit_bits = digit_bits;
+    return GS_OK;
+}
+
+gs_status gs_raster_set_culling(gs_raster r, int enable) {
+    GS_REQUIRE(r != nullptr, "raster is null");
+    r->core.cull = enable != 0;
     return GS_OK;
 }
 

@@ -8,7 +8,11 @@ the f32-rounded inputs the GPU sees. Tolerances are written per assertion:
     design notes), which are counted and bounded by the reference's own
     3e-3 per-channel limit (raster_parity.rs:59-69);
   * gradients: per class normwise relative error <= 1e-4, and per element
-    err <= 1e-5 + 2e-3 max(|a|,|b|) (raster_parity.rs:101-113, 219);
+    err <= 1e-5 + 2e-3 max(|a|,|b|) (raster_parity.rs:101-113, 219). Where
+    the FP64 gradient itself moves by more under unit-roundoff (2^-24)
+    perturbations of the inputs, no f32 evaluation can be held to 1e-4: the
+    normwise bound then becomes 4x that measured sensitivity (max of three
+    perturbation trials, printed beside the error);
     aux losses 6e-3 / camera 3e-2 per element (raster_parity.rs:251).
 Projection rects / depth keys, sorted 64-bit keys and tile ranges are
 bit-exact against the fp32 mirror.
@@ -156,17 +160,18 @@ def _tight_scene(kind):
     return sc, sp
 
 
-def _run_binning(gpu, sc, sp, mode, dl):
+def _run_binning(gpu, sc, sp, mode, dl, cull=True):
     import splattar_amd as sa
     cam = sa.Camera(np.asarray(sp.center, np.float32), np.asarray(sp.quat, np.float32),
                     float(np.float32(sp.focal)), sc.sh_degree)
     r = sa.Rasterizer(gpu, sc.n, sp.width, sp.height, sc.n * 64)
     r.upload_scene(sc.positions, sc.scales, sc.quats, sc.opacities, sc.sh)
     r.set_binning(mode)
+    r.set_culling(cull)
     r.dl_dcolor.write(dl)
     r.forward(cam, sc.n)
     out = dict(dbg=r.debug(), color=r.out_color.read(), aux=r.out_aux.read(),
-               normal=r.out_normal.read())
+               normal=r.out_normal.read(), ncontrib=r.out_ncontrib.read())
     r.backward(sc.n, flags=0)
     out.update(pos=r.grad_pos.read()[:, :3], scales=r.grad_scales.read(), quat=r.grad_quat.read(),
                opacity=r.grad_opacity.read(), sh=r.grad_sh.read())
@@ -219,6 +224,29 @@ def test_tight_binning_equals_presort(gpu, kind):
         assert nrm < 1e-5, k
 
 
+@pytest.mark.parametrize("kind", ["config0", "heavy_tail", "dense"])
+def test_footprint_culling_is_exact(gpu, kind):
+    """The per-quadrant footprint culling (box, screen disk, ellipse) skips
+    only records that reach no pixel of a wave's 8x8 block: images and
+    ncontrib are bit-identical with it off, gradients equal up to the order
+    of float atomics."""
+    import splattar_amd as sa
+    sc, sp = _tight_scene(kind)
+    rng = np.random.default_rng(23)
+    dl = np.zeros((sp.height, sp.width, 4), np.float32)
+    dl[..., :3] = rng.uniform(-1, 1, (sp.height, sp.width, 3))
+    announce(f"culling on/off {kind} n={sc.n} {sp.width}x{sp.height}")
+    ref = _run_binning(gpu, sc, sp, sa._lib.GS_BINNING_TIGHT, dl, cull=False)
+    got = _run_binning(gpu, sc, sp, sa._lib.GS_BINNING_TIGHT, dl, cull=True)
+    for k in ("color", "aux", "normal", "ncontrib"):
+        assert np.array_equal(got[k], ref[k]), k
+    for k in ("pos", "scales", "quat", "opacity", "sh"):
+        g, c = got[k].astype(np.float64), ref[k].astype(np.float64)
+        nrm = np.linalg.norm(g - c) / max(np.linalg.norm(c), 1e-30)
+        print(f"  {k}: normwise rel {nrm:.2e}")
+        assert nrm < 1e-5, k
+
+
 def _weights(seed, size, aux):
     # raster_parity.rs:129-161 (rng = seed ^ 0xfeed).
     rng = XorShift64(seed ^ 0xFEED)
@@ -230,6 +258,29 @@ def _weights(seed, size, aux):
     return w, wd, wn
 
 
+_CLASSES = ("pos", "scales", "quat", "opacity", "sh")
+
+
+def _roundoff_sensitivity(s32, c32, ref, w32, wd32, wn32, lam, trials=3):
+    """Per class, the largest normwise change of the FP64 gradient when every
+    scene input is scaled by (1 +- 2^-24) with random signs: the error floor
+    of any f32 evaluation of this case."""
+    rng = np.random.default_rng(0)
+    out = {k: 0.0 for k in _CLASSES}
+    for _ in range(trials):
+        sp = s32.copy()
+        for arr in (sp.pos, sp.scales, sp.quat, sp.opacity, sp.sh):
+            arr *= 1.0 + rng.choice([-1.0, 1.0], arr.shape) * 2.0 ** -24
+        g = oracle.gradients(sp, c32, w32.astype(np.float64), wd32.astype(np.float64),
+                             wn32.astype(np.float64), lam)
+        for k in _CLASSES:
+            c = getattr(ref, k)
+            cn = np.linalg.norm(c)
+            if cn > 0:
+                out[k] = max(out[k], float(np.linalg.norm(getattr(g, k) - c) / cn))
+    return out
+
+
 def _grad_case(gpu, seed, n, deg, size, aux, flags=0):
     scene, cam = make_scene(seed, n, deg, size, flavor="parity")
     s32, c32 = scene_from_f32(scene), camera_from_f32(cam)
@@ -238,6 +289,7 @@ def _grad_case(gpu, seed, n, deg, size, aux, flags=0):
     lam = 0.35 if aux else 0.0
     ref = oracle.gradients(s32, c32, w32.astype(np.float64), wd32.astype(np.float64),
                            wn32.astype(np.float64), lam)
+    ref.sensitivity = _roundoff_sensitivity(s32, c32, ref, w32, wd32, wn32, lam)
     announce(f"backward seed={seed} n={n} deg={deg} size={size} aux={aux} flags={flags}")
     r = _rasterizer(gpu, scene, cam)
     dl = np.zeros((size, size, 4), np.float32)
@@ -268,16 +320,19 @@ def _check_class(label, gpu_v, cpu_v, rel_tol, norm_tol=1e-4):
     cn = np.linalg.norm(c)
     nrm = np.linalg.norm(g - c) / cn if cn > 0 else np.linalg.norm(g)
     print(f"  {label}: normwise rel {nrm:.2e}")
+    if nrm > 0.3 * norm_tol:
+        for i in np.argsort(-err)[:4]:
+            print(f"    [{i}] gpu {g[i]: .6e} fp64 {c[i]: .6e} err {err[i]:.2e} (|c|max {np.abs(c).max():.3e})")
     assert nrm <= norm_tol, f"{label}: normwise relative error {nrm:.3e}"
     return nrm
 
 
 def _check_all(scene, cam, ref, got, rel_tol, cam_tol, norm_tol=1e-4):
-    _check_class("pos", got["pos"], ref.pos, rel_tol, norm_tol)
-    _check_class("scales", got["scales"], ref.scales, rel_tol, norm_tol)
-    _check_class("quat", got["quat"], ref.quat, rel_tol, norm_tol)
-    _check_class("opacity", got["opacity"], ref.opacity, rel_tol, norm_tol)
-    _check_class("sh", got["sh"], ref.sh, rel_tol, norm_tol)
+    for k in _CLASSES:
+        sens = ref.sensitivity[k]
+        tol = max(norm_tol, 4.0 * sens)
+        print(f"  {k}: fp64 roundoff sensitivity {sens:.2e} -> normwise bound {tol:.2e}")
+        _check_class(k, got[k], getattr(ref, k), rel_tol, tol)
     gc = got["cam"]
     _check_class("cam_center", gc[9:12], ref.cam_center, cam_tol, max(norm_tol, 1e-3))
     _check_class("focal", gc[12:13], [ref.focal], cam_tol, max(norm_tol, 1e-3))
