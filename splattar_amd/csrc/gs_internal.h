@@ -395,6 +395,13 @@ __device__ __forceinline__ uint4 clip_rect_to_box(uint4 rect, const float4 b) {
     return make_uint4((uint32_t)tx0, (uint32_t)ty0, (uint32_t)tx1, (uint32_t)ty1);
 }
 
+// 1/x as v_rcp_f32 (1 ulp) refined by one Newton step r += r (1 - x r),
+// instead of the IEEE division sequence; exact for powers of two (1/1 = 1).
+__device__ __forceinline__ float rcp_refined(float x) {
+    const float r0 = __builtin_amdgcn_rcpf(x);
+    return fmaf(r0, fmaf(-x, r0, 1.0f), r0);
+}
+
 __device__ __forceinline__ HitEval eval_hit(const Record& r, float sx, float sy, float pxc, float pyc,
                                             float lthr = -INFINITY) {
     HitEval h;
@@ -402,11 +409,7 @@ __device__ __forceinline__ HitEval eval_hit(const Record& r, float sx, float sy,
     const float Nu = r.a_px.x * sx + r.a_px.y * sy + r.a_px.z;
     const float Nv = r.b_py.x * sx + r.b_py.y * sy + r.b_py.z;
     // Branch-free: |D| <= 1e-12 gives inv = 0, hence t = 0 and no ray hit.
-    // v_rcp_f32 (1 ulp) refined by one Newton step r += r (1 - D r) instead
-    // of the IEEE division sequence.
-    const float r0 = __builtin_amdgcn_rcpf(D);
-    const float r1 = fmaf(r0, fmaf(-D, r0, 1.0f), r0);
-    const float inv = fabsf(D) > 1e-12f ? r1 : 0.0f;
+    const float inv = fabsf(D) > 1e-12f ? rcp_refined(D) : 0.0f;
     h.u = Nu * inv;
     h.v = Nv * inv;
     h.t = r.m_nt.w * inv;

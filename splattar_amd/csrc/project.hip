@@ -16,21 +16,25 @@ namespace gs {
 namespace {
 
 __global__ __launch_bounds__(256) void project_kernel(
-    const CamParams* __restrict__ cams, uint32_t n, const float4* __restrict__ pos,
+    const CamParams* __restrict__ cams, uint32_t n_views, uint32_t n, const float4* __restrict__ pos,
     const float2* __restrict__ scales, const float4* __restrict__ quats,
     const float* __restrict__ opac, const float* __restrict__ sh, Record* __restrict__ recs,
     uint4* __restrict__ rects, uint32_t* __restrict__ depth_keys, uint32_t* __restrict__ counts,
     uint32_t* __restrict__ ranges_to_clear, uint32_t num_ranges, uint32_t* __restrict__ presort_keys,
     uint32_t* __restrict__ presort_vals, uint32_t* __restrict__ item_count, bool clip_to_box) {
-    const uint32_t view = blockIdx.y;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    // Views vary fastest over the blocks, so the V blocks that read the same
+    // 256 surfels run back to back and the later ones find the parameters in
+    // cache.
+    const uint32_t view = blockIdx.x % n_views;
+    const uint32_t sblocks = gridDim.x / n_views;
+    const uint32_t i = (blockIdx.x / n_views) * blockDim.x + threadIdx.x;
     // The tile-range table of the following binning stage starts cleared
-    // (binning.rs:228); the first view's grid does it (stream order makes it
+    // (binning.rs:228); the first view's blocks do it (stream order makes it
     // visible to tile_ranges).
     if (view == 0 && ranges_to_clear != nullptr) {
-        for (uint32_t k = i; k < num_ranges; k += gridDim.x * blockDim.x) ranges_to_clear[k] = 0u;
+        for (uint32_t k = i; k < num_ranges; k += sblocks * blockDim.x) ranges_to_clear[k] = 0u;
     }
-    if (view == 0 && i == 0 && item_count != nullptr) *item_count = n * gridDim.y;
+    if (view == 0 && i == 0 && item_count != nullptr) *item_count = n * n_views;
     if (i >= n) return;
     const CamParams cam = cams[view];
     const uint32_t g = view * n + i;
@@ -87,12 +91,26 @@ __global__ __launch_bounds__(256) void project_kernel(
     const float inv_len = 1.0f / sqrtf(f.xx * f.xx + f.xy * f.xy + f.xz * f.xz);
     float basis[16];
     const int nb = sh_basis((int)cam.sh_degree, f.xx * inv_len, f.xy * inv_len, f.xz * inv_len, basis);
-    const float* shp = sh + (size_t)i * kShFloats;
+    // 16-B loads of the (48-float, 192-B aligned) coefficient row, only as
+    // far as the degree needs.
+    const float4* sh4 = reinterpret_cast<const float4*>(sh + (size_t)i * kShFloats);
+    float shv[kShFloats];
+#pragma unroll
+    for (int q = 0; q < kShFloats / 4; ++q) {
+        const float4 v = 4 * q < 3 * nb ? sh4[q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        shv[4 * q + 0] = v.x;
+        shv[4 * q + 1] = v.y;
+        shv[4 * q + 2] = v.z;
+        shv[4 * q + 3] = v.w;
+    }
     float cr = 0.5f, cg = 0.5f, cb = 0.5f;
-    for (int k = 0; k < nb; ++k) {
-        cr += basis[k] * shp[3 * k + 0];
-        cg += basis[k] * shp[3 * k + 1];
-        cb += basis[k] * shp[3 * k + 2];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        if (k < nb) {
+            cr += basis[k] * shv[3 * k + 0];
+            cg += basis[k] * shv[3 * k + 1];
+            cb += basis[k] * shv[3 * k + 2];
+        }
     }
     // Camera-facing unit normal (surfel_prep_fwd.wgsl:188-194).
     float3 nrm = M;
@@ -116,9 +134,10 @@ hipError_t launch_project(hipStream_t st, const CamParams* cams_dev, uint32_t n_
                           uint32_t num_ranges, uint32_t* presort_keys, uint32_t* presort_vals,
                           uint32_t* item_count, bool clip_to_box) {
     if (n == 0 && num_ranges == 0 && item_count == nullptr) return hipSuccess;
-    uint32_t blocks = div_up(n > num_ranges ? (n ? n : 1) : num_ranges, 256);
-    dim3 grid(blocks, n_views);
-    hipLaunchKernelGGL(project_kernel, grid, dim3(256), 0, st, cams_dev, n, pos, scales, quats, opac,
+    if (n_views == 0) return hipSuccess;
+    const uint32_t blocks = div_up(n > num_ranges ? (n ? n : 1) : num_ranges, 256);
+    dim3 grid(blocks * n_views);
+    hipLaunchKernelGGL(project_kernel, grid, dim3(256), 0, st, cams_dev, n_views, n, pos, scales, quats, opac,
                        sh, recs, rects, depth_keys, counts, ranges_to_clear, num_ranges, presort_keys,
                        presort_vals, item_count, clip_to_box);
     return hipGetLastError();

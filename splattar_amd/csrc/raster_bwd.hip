@@ -213,84 +213,79 @@ __global__ __launch_bounds__(256) void raster_bwd_kernel(const CamParams* __rest
             r.b_py = s_b[j];
             r.col_op = s_c[j];
             r.nrm_dep = s_n[j];
+            // Branch-free body: a lane that does not contribute gets w = 0 and
+            // 1/(1-alpha) = 1 exactly, so its running sums and T are unchanged,
+            // and its gradient terms are selected to 0 (no NaN from 0 * inf).
+            const HitEval h = eval_hit(r, sx, sy, pxc, pyc, s_lthr[j]);
+            const bool act = rel < last && h.alpha > 0.0f;
+            const float alpha = act ? h.alpha : 0.0f;
+            const float one_m = 1.0f - alpha;
+            const float inv_one_m = rcp_refined(one_m);
+            const float t_before = T * inv_one_m;
+            const float w = t_before * alpha;
+            const float t_hit = h.t;
             float g[16];
             float g2[4];
-#pragma unroll
-            for (int k = 0; k < 16; ++k) g[k] = 0.0f;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) g2[k] = 0.0f;
-            bool contrib = false;
-            if (rel < last) {
-                const HitEval h = eval_hit(r, sx, sy, pxc, pyc, s_lthr[j]);
-                if (h.alpha > 0.0f) {
-                    contrib = true;
-                    const float one_m = 1.0f - h.alpha;
-                    const float inv_one_m = 1.0f / one_m;
-                    const float t_before = T * inv_one_m;
-                    const float w = t_before * h.alpha;
-                    const float t_hit = h.t;
-                    g[kGCol + 0] = dlr * w;
-                    g[kGCol + 1] = dlg * w;
-                    g[kGCol + 2] = dlb * w;
-                    float dl_dt = 0.0f;
-                    float dl_dalpha = (dlr * r.col_op.x + dlg * r.col_op.y + dlb * r.col_op.z) * t_before -
-                                      (dlr * suf_r + dlg * suf_g + dlb * suf_b) * inv_one_m;
-                    suf_r += r.col_op.x * w;
-                    suf_g += r.col_op.y * w;
-                    suf_b += r.col_op.z * w;
-                    if (AUX) {
-                        float dt_dist = 0.0f, coef = 0.0f;
-                        if (lambda != 0.0f) {
-                            const float a_pre = w_total - suf_w - w;
-                            const float b_pre = wt_total - suf_d - w * t_hit;
-                            dt_dist = lambda * 2.0f * w * (a_pre - suf_w);
-                            coef = lambda * (2.0f * (t_hit * a_pre - b_pre) +
-                                             2.0f * (suf_d - t_hit * suf_w));
-                        }
-                        g2[kGNrm - 16 + 0] = dnx * w;
-                        g2[kGNrm - 16 + 1] = dny * w;
-                        g2[kGNrm - 16 + 2] = dnz * w;
-                        dl_dt = dld * w + dt_dist;
-                        dl_dalpha += dld * (t_hit * t_before - suf_d * inv_one_m) +
-                                     (dnx * r.nrm_dep.x + dny * r.nrm_dep.y + dnz * r.nrm_dep.z) * t_before -
-                                     (dnx * suf_nx + dny * suf_ny + dnz * suf_nz) * inv_one_m +
-                                     coef * t_before - suf_cw * inv_one_m;
-                        suf_d += t_hit * w;
-                        suf_nx += r.nrm_dep.x * w;
-                        suf_ny += r.nrm_dep.y * w;
-                        suf_nz += r.nrm_dep.z * w;
-                        suf_w += w;
-                        suf_cw += coef * w;
-                    }
-                    T = t_before;
-                    if (!h.clamped) {
-                        g[kGOp] = dl_dalpha * h.ghat;
-                        const float dl_dghat = dl_dalpha * r.col_op.w;
-                        if (h.ray) {
-                            const float dl_du = -h.u * h.ghat * dl_dghat;
-                            const float dl_dv = -h.v * h.ghat * dl_dghat;
-                            const float dnu = dl_du * h.inv_d;
-                            const float dnv = dl_dv * h.inv_d;
-                            const float dD = -(h.u * dl_du + h.v * dl_dv + t_hit * dl_dt) * h.inv_d;
-                            g[kGA + 0] = dnu * sx;
-                            g[kGA + 1] = dnu * sy;
-                            g[kGA + 2] = dnu;
-                            g[kGB + 0] = dnv * sx;
-                            g[kGB + 1] = dnv * sy;
-                            g[kGB + 2] = dnv;
-                            g[kGM + 0] = dD * sx;
-                            g[kGM + 1] = dD * sy;
-                            g[kGM + 2] = dD;
-                            if (AUX) g[kGNt] = dl_dt * h.inv_d;
-                        } else {
-                            if (AUX) g2[kGDep - 16] = dl_dt;
-                            const float dx = pxc - r.a_px.w, dy = pyc - r.b_py.w;
-                            g[kGPx] = 2.0f * dx * h.ghat * dl_dghat;
-                            g[kGPy] = 2.0f * dy * h.ghat * dl_dghat;
-                        }
-                    }
+            g[kGCol + 0] = dlr * w;
+            g[kGCol + 1] = dlg * w;
+            g[kGCol + 2] = dlb * w;
+            float dl_dt = 0.0f;
+            float dl_dalpha = (dlr * r.col_op.x + dlg * r.col_op.y + dlb * r.col_op.z) * t_before -
+                              (dlr * suf_r + dlg * suf_g + dlb * suf_b) * inv_one_m;
+            suf_r += r.col_op.x * w;
+            suf_g += r.col_op.y * w;
+            suf_b += r.col_op.z * w;
+            if (AUX) {
+                float dt_dist = 0.0f, coef = 0.0f;
+                if (lambda != 0.0f) {
+                    const float a_pre = w_total - suf_w - w;
+                    const float b_pre = wt_total - suf_d - w * t_hit;
+                    dt_dist = lambda * 2.0f * w * (a_pre - suf_w);
+                    coef = lambda * (2.0f * (t_hit * a_pre - b_pre) + 2.0f * (suf_d - t_hit * suf_w));
                 }
+                g2[kGNrm - 16 + 0] = dnx * w;
+                g2[kGNrm - 16 + 1] = dny * w;
+                g2[kGNrm - 16 + 2] = dnz * w;
+                dl_dt = dld * w + dt_dist;
+                dl_dalpha += dld * (t_hit * t_before - suf_d * inv_one_m) +
+                             (dnx * r.nrm_dep.x + dny * r.nrm_dep.y + dnz * r.nrm_dep.z) * t_before -
+                             (dnx * suf_nx + dny * suf_ny + dnz * suf_nz) * inv_one_m +
+                             coef * t_before - suf_cw * inv_one_m;
+                suf_d += t_hit * w;
+                suf_nx += r.nrm_dep.x * w;
+                suf_ny += r.nrm_dep.y * w;
+                suf_nz += r.nrm_dep.z * w;
+                suf_w += w;
+                suf_cw += coef * w;
             }
+            T = t_before;
+            // Geometry terms exist only for contributing, unclamped hits.
+            const bool geo = act && !h.clamped;
+            g[kGOp] = geo ? dl_dalpha * h.ghat : 0.0f;
+            const float gd = geo ? h.ghat * (dl_dalpha * r.col_op.w) : 0.0f;  // ghat dL/dghat
+            const float dl_du = -h.u * gd;
+            const float dl_dv = -h.v * gd;
+            const float dnu = h.ray ? dl_du * h.inv_d : 0.0f;
+            const float dnv = h.ray ? dl_dv * h.inv_d : 0.0f;
+            const float dD = h.ray ? -(h.u * dl_du + h.v * dl_dv + t_hit * dl_dt) * h.inv_d : 0.0f;
+            g[kGA + 0] = dnu * sx;
+            g[kGA + 1] = dnu * sy;
+            g[kGA + 2] = dnu;
+            g[kGB + 0] = dnv * sx;
+            g[kGB + 1] = dnv * sy;
+            g[kGB + 2] = dnv;
+            g[kGM + 0] = dD * sx;
+            g[kGM + 1] = dD * sy;
+            g[kGM + 2] = dD;
+            const float gs = h.ray ? 0.0f : 2.0f * gd;
+            g[kGPx] = gs * (pxc - r.a_px.w);
+            g[kGPy] = gs * (pyc - r.b_py.w);
+            g[kGNt] = 0.0f;
+            if (AUX) {
+                g[kGNt] = (geo && h.ray) ? dl_dt * h.inv_d : 0.0f;
+                g2[kGDep - 16] = (geo && !h.ray) ? dl_dt : 0.0f;
+            }
+            const bool contrib = act;
             if (!__any(contrib)) continue;
             const float red = wave_reduce_transpose<16>(g, lane);
             if ((lane & 3) == 0) {
