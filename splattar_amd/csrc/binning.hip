@@ -108,12 +108,38 @@ __global__ __launch_bounds__(256) void emit_keys_kernel(
 // (depth bits, record index) and a stable sort on the tile id alone yields
 // exactly the (tile, depth, index) order of the 64-bit path. Keys are the
 // 32-bit global tile ids; payloads the record indices.
+// Position of the k-th (0-based) set bit of the 64-bit mask (hi:lo); k is
+// below the mask's population count.
+__device__ __forceinline__ uint32_t select_set_bit(uint32_t lo, uint32_t hi, uint32_t k) {
+    uint32_t pos = 0;
+    uint32_t v = lo;
+    const uint32_t plo = (uint32_t)__popc(lo);
+    if (k >= plo) {
+        k -= plo;
+        v = hi;
+        pos = 32;
+    }
+#pragma unroll
+    for (uint32_t width = 16; width >= 1; width >>= 1) {
+        const uint32_t low = v & ((1u << width) - 1u);
+        const uint32_t c = (uint32_t)__popc(low);
+        if (k >= c) {
+            k -= c;
+            v >>= width;
+            pos += width;
+        } else {
+            v = low;
+        }
+    }
+    return pos;
+}
+
 __global__ __launch_bounds__(256) void emit_tile_keys_kernel(
     uint32_t n_items, const uint32_t* __restrict__ order, const uint4* __restrict__ rects,
     const uint32_t* __restrict__ offsets_sorted, const CamParams* __restrict__ cams,
     uint32_t items_per_view, uint32_t* __restrict__ tkeys, uint32_t* __restrict__ payloads,
     uint32_t max_entries, const uint32_t* __restrict__ total, uint32_t* __restrict__ sort_count,
-    uint32_t* __restrict__ overflow_flag) {
+    uint32_t* __restrict__ overflow_flag, const uint64_t* __restrict__ tile_masks) {
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid == 0) {
         const uint32_t t = *total;
@@ -125,12 +151,21 @@ __global__ __launch_bounds__(256) void emit_tile_keys_kernel(
     if (wave_first >= n_items) return;
 
     uint32_t cnt = 0, base = 0, w = 0, x0 = 0, y0 = 0, tiles_x = 0, tile_base = 0, item = 0;
+    uint64_t mask = 0;
+    bool masked = false;
     if (gid < n_items) {
         item = order[gid];
         const uint4 r = rects[item];
         if (r.z >= r.x && r.w >= r.y) {
             w = r.z - r.x + 1u;
             cnt = w * (r.w - r.y + 1u);
+            // Tight binning: only the rect tiles whose bit is set (rects of
+            // at most 64 tiles; project wrote the same count).
+            if (tile_masks != nullptr && cnt <= 64u) {
+                mask = tile_masks[item];
+                masked = true;
+                cnt = (uint32_t)__popcll(mask);
+            }
             x0 = r.x;
             y0 = r.y;
             base = offsets_sorted[gid];
@@ -163,9 +198,13 @@ __global__ __launch_bounds__(256) void emit_tile_keys_kernel(
         const uint32_t o_tx = __shfl(tiles_x, owner);
         const uint32_t o_tb = __shfl(tile_base, owner);
         const uint32_t o_item = __shfl(item, owner);
+        const uint32_t o_mlo = __shfl((uint32_t)mask, owner);
+        const uint32_t o_mhi = __shfl((uint32_t)(mask >> 32), owner);
+        const bool o_masked = __shfl((int)masked, owner) != 0;
         if (k < wave_total && o_cnt > 0) {
-            const uint32_t local = k - o_excl;
+            uint32_t local = k - o_excl;
             const uint32_t e = o_base + local;
+            if (o_masked) local = select_set_bit(o_mlo, o_mhi, local);
             if (e < max_entries) {
                 const uint32_t ty = o_y0 + local / o_w;
                 const uint32_t tx = o_x0 + local % o_w;
@@ -174,6 +213,24 @@ __global__ __launch_bounds__(256) void emit_tile_keys_kernel(
             }
         }
     }
+}
+
+__global__ __launch_bounds__(256) void tile_order_keys_kernel(const uint32_t* __restrict__ ranges,
+                                                              uint32_t total_tiles,
+                                                              uint32_t* __restrict__ keys,
+                                                              uint32_t* __restrict__ vals,
+                                                              uint32_t* __restrict__ count,
+                                                              uint32_t* __restrict__ cursors) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) {
+        *count = total_tiles;
+        cursors[0] = 0u;
+        cursors[1] = 0u;
+    }
+    if (t >= total_tiles) return;
+    const uint32_t len = ranges[2 * (size_t)t + 1] - ranges[2 * (size_t)t];
+    keys[t] = 0xFFFFu - min(len, 0xFFFFu);
+    vals[t] = t;
 }
 
 __global__ __launch_bounds__(256) void tile_ranges_u32_kernel(const uint32_t* __restrict__ keys,
@@ -254,11 +311,20 @@ hipError_t launch_emit_tile_keys(hipStream_t st, uint32_t n_items, const uint32_
                                  const uint4* rects, const uint32_t* offsets_sorted,
                                  const CamParams* cams_dev, uint32_t items_per_view, uint32_t* tkeys,
                                  uint32_t* payloads, uint32_t max_entries, const uint32_t* total,
-                                 uint32_t* sort_count, uint32_t* overflow_flag) {
+                                 uint32_t* sort_count, uint32_t* overflow_flag,
+                                 const uint64_t* tile_masks) {
     uint32_t blocks = div_up(n_items > 0 ? n_items : 1, 256);
     hipLaunchKernelGGL(emit_tile_keys_kernel, dim3(blocks), dim3(256), 0, st, n_items, order, rects,
                        offsets_sorted, cams_dev, items_per_view > 0 ? items_per_view : 1, tkeys,
-                       payloads, max_entries, total, sort_count, overflow_flag);
+                       payloads, max_entries, total, sort_count, overflow_flag, tile_masks);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_order_keys(hipStream_t st, const uint32_t* ranges, uint32_t total_tiles,
+                                  uint32_t* keys, uint32_t* vals, uint32_t* count, uint32_t* cursors) {
+    const uint32_t blocks = div_up(total_tiles > 0 ? total_tiles : 1, 256);
+    hipLaunchKernelGGL(tile_order_keys_kernel, dim3(blocks), dim3(256), 0, st, ranges, total_tiles, keys,
+                       vals, count, cursors);
     return hipGetLastError();
 }
 

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstddef>
 #include <cstdint>
 #include <string>
@@ -405,9 +406,10 @@ __device__ __forceinline__ float rcp_refined(float x) {
 __device__ __forceinline__ HitEval eval_hit(const Record& r, float sx, float sy, float pxc, float pyc,
                                             float lthr = -INFINITY) {
     HitEval h;
-    const float D = r.m_nt.x * sx + r.m_nt.y * sy + r.m_nt.z;
-    const float Nu = r.a_px.x * sx + r.a_px.y * sy + r.a_px.z;
-    const float Nv = r.b_py.x * sx + r.b_py.y * sy + r.b_py.z;
+    // D = M.s, Nu = A.s, Nv = B.s with s = (sx, sy, 1): two FMAs each.
+    const float D = fmaf(r.m_nt.x, sx, fmaf(r.m_nt.y, sy, r.m_nt.z));
+    const float Nu = fmaf(r.a_px.x, sx, fmaf(r.a_px.y, sy, r.a_px.z));
+    const float Nv = fmaf(r.b_py.x, sx, fmaf(r.b_py.y, sy, r.b_py.z));
     // Branch-free: |D| <= 1e-12 gives inv = 0, hence t = 0 and no ray hit.
     const float inv = fabsf(D) > 1e-12f ? rcp_refined(D) : 0.0f;
     h.u = Nu * inv;
@@ -462,7 +464,7 @@ hipError_t launch_project(hipStream_t st, const CamParams* cams_dev, uint32_t n_
                           uint32_t* depth_keys, uint32_t* counts, uint32_t* ranges_to_clear,
                           uint32_t num_ranges, uint32_t* presort_keys = nullptr,
                           uint32_t* presort_vals = nullptr, uint32_t* item_count = nullptr,
-                          bool clip_to_box = false);
+                          bool clip_to_box = false, uint64_t* tile_masks = nullptr);
 
 // scan.hip
 size_t scan_scratch_bytes(uint32_t capacity);
@@ -484,7 +486,13 @@ hipError_t launch_emit_tile_keys(hipStream_t st, uint32_t n_items, const uint32_
                                  const uint4* rects, const uint32_t* offsets_sorted,
                                  const CamParams* cams_dev, uint32_t items_per_view, uint32_t* tkeys,
                                  uint32_t* payloads, uint32_t max_entries, const uint32_t* total,
-                                 uint32_t* sort_count, uint32_t* overflow_flag);
+                                 uint32_t* sort_count, uint32_t* overflow_flag,
+                                 const uint64_t* tile_masks = nullptr);
+// Longest-list-first tile order: keys[t] = 0xFFFF - min(list length, 0xFFFF),
+// vals[t] = t for the total_tiles tiles, *count = total_tiles and
+// cursors[0..1] = 0 (sort the pairs by 16 key bits, stably, to get the order).
+hipError_t launch_tile_order_keys(hipStream_t st, const uint32_t* ranges, uint32_t total_tiles,
+                                  uint32_t* keys, uint32_t* vals, uint32_t* count, uint32_t* cursors);
 hipError_t launch_tile_ranges_u32(hipStream_t st, const uint32_t* keys, const uint32_t* count,
                                   uint32_t capacity, uint32_t* ranges);
 hipError_t launch_compose_keys(hipStream_t st, const uint32_t* tkeys, const uint32_t* payloads,
@@ -512,10 +520,19 @@ struct FwdOut {
     float4* normal;    // sum w n
     uint32_t* ncontrib;
 };
+// Tile scheduling for the raster passes: with a queue (cursor != nullptr),
+// grid_blocks resident workgroups pull tile positions k from *cursor (which
+// must be 0 at launch) and process tile order[k] (longest list first) until
+// k >= total_tiles; without one, workgroup b processes tile b.
+struct TileQueue {
+    const uint32_t* order;
+    uint32_t* cursor;
+    uint32_t grid_blocks;
+};
 hipError_t launch_raster_fwd(hipStream_t st, const CamParams* cams_dev, const CamParams* cams_host,
                              uint32_t n_views, uint32_t total_tiles, const uint32_t* ranges,
                              const uint32_t* payloads, const Record* recs, FwdOut out,
-                             bool cull = true);
+                             bool cull = true, TileQueue queue = {nullptr, nullptr, 0});
 
 // raster_bwd.hip
 struct BwdIn {
@@ -528,7 +545,8 @@ struct BwdIn {
 hipError_t launch_raster_bwd(hipStream_t st, const CamParams* cams_dev, const CamParams* cams_host,
                              uint32_t n_views, uint32_t total_tiles, const uint32_t* ranges,
                              const uint32_t* payloads, const Record* recs, BwdIn in,
-                             float* grad_coef, bool aux, bool cull = true);
+                             float* grad_coef, bool aux, bool cull = true,
+                             TileQueue queue = {nullptr, nullptr, 0});
 
 // geom_bwd.hip
 struct ParamGrads {

@@ -7,11 +7,11 @@
 // 0.5 in the reference ((1 - SSIM)/2) and selectable here.
 // The reference runs 19 dispatches over 16 P-sized intermediates; here:
 //   ssim_fwd: per 16x16 tile (+5 px halo in LDS) compute the five blurred
-//             moments, SSIM, and the three coefficient maps
+//             moments, SSIM, and per channel the three coefficient maps
 //             C_mu = dw dS/dmu_x, C_x2 = dw dS/dE[x^2], C_xy = dw dS/dE[xy],
 //             plus per-block L1 / SSIM partial sums (deterministic reduction);
-//   ssim_bwd: per tile (+halo) blur the coefficient maps (the zero-padded
-//             symmetric blur is self-adjoint) and combine
+//   ssim_bwd: per tile (+halo), one channel at a time, blur the coefficient
+//             maps (the zero-padded symmetric blur is self-adjoint) and combine
 //             dL/dx = (1-l) sign(x-y)/3P + B(C_mu) + 2x B(C_x2) + y B(C_xy).
 
 #include "gs_internal.h"
@@ -23,10 +23,18 @@ namespace {
 constexpr int kT = 16;          // output tile
 constexpr int kH = 5;           // halo (11-tap window)
 constexpr int kS = kT + 2 * kH; // staged side = 26
+// LDS pitches chosen for conflict-free access: in both blur passes a 32-lane
+// half-wave covers two rows of 16 consecutive columns, so row pitches are
+// congruent to 16 modulo the 32 banks (staged inputs: 48; blurred rows: 16).
+constexpr int kPin = 48;
+constexpr int kPh = 16;
 __constant__ float c_w[6] = {0.26601818f, 0.21300237f, 0.10936069f,
                              0.03599398f, 0.00759866f, 0.00102838f};
 
 constexpr float kC1 = 0.0001f, kC2 = 0.0009f;
+
+// Coefficient maps between the two kernels: per channel ch and pixel p one
+// float4 (C_mu, C_x2, C_xy, 0) at coef[ch * P + p].
 
 __global__ __launch_bounds__(256) void ssim_fwd_kernel(uint32_t W, uint32_t H, float lambda,
                                                        float dssim_scale,
@@ -36,9 +44,9 @@ __global__ __launch_bounds__(256) void ssim_fwd_kernel(uint32_t W, uint32_t H, f
                                                        double* __restrict__ partials,
                                                        const CamParams* __restrict__ view) {
     if (view) tgt += view->target_base;
-    __shared__ float sx[3][kS][kS + 1];
-    __shared__ float sy[3][kS][kS + 1];
-    __shared__ float hb[5][kS][kT + 1];
+    __shared__ float sx[3][kS][kPin];
+    __shared__ float sy[3][kS][kPin];
+    __shared__ float hb[5][kS][kPh];
     __shared__ double s_red[2][4];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     const int x0 = (int)blockIdx.x * kT - kH, y0 = (int)blockIdx.y * kT - kH;
@@ -62,7 +70,7 @@ __global__ __launch_bounds__(256) void ssim_fwd_kernel(uint32_t W, uint32_t H, f
     const bool inside = gx < (int)W && gy < (int)H;
     const float inv_n3 = 1.0f / ((float)W * (float)H * 3.0f);
     const float dw = -lambda * dssim_scale * inv_n3;
-    float cm[3], cx2[3], cxy[3];
+    const size_t P = (size_t)W * H;
     double l1 = 0.0, ss = 0.0;
     for (int ch = 0; ch < 3; ++ch) {
         // Horizontal pass over all staged rows for the 16 output columns.
@@ -106,20 +114,12 @@ __global__ __launch_bounds__(256) void ssim_fwd_kernel(uint32_t W, uint32_t H, f
         const float cs = (2.0f * sxy + kC2) / q;
         const float s = l * cs;
         const float ds_dmux = cs * (2.0f * mu_y - 2.0f * mu_x * l) / p + l * (-2.0f * mu_y + 2.0f * mu_x * cs) / q;
-        cm[ch] = ds_dmux * dw;
-        cx2[ch] = (-l * cs / q) * dw;
-        cxy[ch] = (2.0f * l / q) * dw;
         if (inside) {
+            coef[ch * P + (size_t)gy * W + gx] =
+                make_float4(ds_dmux * dw, (-l * cs / q) * dw, (2.0f * l / q) * dw, 0.0f);
             ss += (double)s;
             l1 += (double)fabsf(sx[ch][ty + kH][tx + kH] - sy[ch][ty + kH][tx + kH]);
         }
-    }
-    if (inside) {
-        const size_t pix = (size_t)gy * W + gx;
-        const size_t P = (size_t)W * H;
-        coef[pix] = make_float4(cm[0], cm[1], cm[2], 0.f);
-        coef[P + pix] = make_float4(cx2[0], cx2[1], cx2[2], 0.f);
-        coef[2 * P + pix] = make_float4(cxy[0], cxy[1], cxy[2], 0.f);
     }
     // Deterministic block reduction of the loss sums.
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -139,6 +139,9 @@ __global__ __launch_bounds__(256) void ssim_fwd_kernel(uint32_t W, uint32_t H, f
     }
 }
 
+// One channel at a time: stage its three coefficient maps (one float4 per
+// pixel), blur them (the zero-padded symmetric blur is self-adjoint) and
+// combine dL/dx = (1-l) sign(x-y)/3P + B(C_mu) + 2x B(C_x2) + y B(C_xy).
 __global__ __launch_bounds__(256) void ssim_bwd_kernel(uint32_t W, uint32_t H, float lambda,
                                                        const float4* __restrict__ img,
                                                        const float4* __restrict__ tgt,
@@ -146,56 +149,56 @@ __global__ __launch_bounds__(256) void ssim_bwd_kernel(uint32_t W, uint32_t H, f
                                                        float4* __restrict__ dl_out,
                                                        const CamParams* __restrict__ view) {
     if (view) tgt += view->target_base;
-    __shared__ float sc[9][kS][kS + 1];
-    __shared__ float hb[9][kS][kT + 1];
+    __shared__ float sc[3][kS][kPin];
+    __shared__ float hb[3][kS][kPh];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     const int x0 = (int)blockIdx.x * kT - kH, y0 = (int)blockIdx.y * kT - kH;
     const size_t P = (size_t)W * H;
-    for (int k = threadIdx.x; k < kS * kS; k += 256) {
-        const int r = k / kS, c = k - r * kS;
-        const int gx = x0 + c, gy = y0 + r;
-        float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a, d = a;
-        if (gx >= 0 && gy >= 0 && gx < (int)W && gy < (int)H) {
-            const size_t pix = (size_t)gy * W + gx;
-            a = coef[pix];
-            b = coef[P + pix];
-            d = coef[2 * P + pix];
+    const int gx = (int)blockIdx.x * kT + tx, gy = (int)blockIdx.y * kT + ty;
+    const bool inside = gx < (int)W && gy < (int)H;
+    float b[3][3];  // [channel][map]
+    for (int ch = 0; ch < 3; ++ch) {
+        for (int k = threadIdx.x; k < kS * kS; k += 256) {
+            const int r = k / kS, c = k - r * kS;
+            const int sxp = x0 + c, syp = y0 + r;
+            float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (sxp >= 0 && syp >= 0 && sxp < (int)W && syp < (int)H)
+                a = coef[ch * P + (size_t)syp * W + sxp];
+            sc[0][r][c] = a.x;
+            sc[1][r][c] = a.y;
+            sc[2][r][c] = a.z;
         }
-        sc[0][r][c] = a.x; sc[1][r][c] = a.y; sc[2][r][c] = a.z;
-        sc[3][r][c] = b.x; sc[4][r][c] = b.y; sc[5][r][c] = b.z;
-        sc[6][r][c] = d.x; sc[7][r][c] = d.y; sc[8][r][c] = d.z;
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < kS * kT; k += 256) {
-        const int r = k / kT, c = k - r * kT;
+        __syncthreads();
+        for (int k = threadIdx.x; k < kS * kT; k += 256) {
+            const int r = k / kT, c = k - r * kT;
 #pragma unroll
-        for (int m = 0; m < 9; ++m) {
+            for (int m = 0; m < 3; ++m) {
+                float acc = 0.f;
+#pragma unroll
+                for (int d = -kH; d <= kH; ++d) acc += c_w[d < 0 ? -d : d] * sc[m][r][c + kH + d];
+                hb[m][r][c] = acc;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
             float acc = 0.f;
 #pragma unroll
-            for (int d = -kH; d <= kH; ++d) acc += c_w[d < 0 ? -d : d] * sc[m][r][c + kH + d];
-            hb[m][r][c] = acc;
+            for (int d = -kH; d <= kH; ++d) acc += c_w[d < 0 ? -d : d] * hb[m][ty + kH + d][tx];
+            b[ch][m] = acc;
         }
+        __syncthreads();  // sc / hb are reused by the next channel
     }
-    __syncthreads();
-    const int gx = (int)blockIdx.x * kT + tx, gy = (int)blockIdx.y * kT + ty;
-    if (gx >= (int)W || gy >= (int)H) return;
-    float b[9];
-#pragma unroll
-    for (int m = 0; m < 9; ++m) {
-        float acc = 0.f;
-#pragma unroll
-        for (int d = -kH; d <= kH; ++d) acc += c_w[d < 0 ? -d : d] * hb[m][ty + kH + d][tx];
-        b[m] = acc;
-    }
+    if (!inside) return;
     const size_t pix = (size_t)gy * W + gx;
     const float4 x = img[pix], y = tgt[pix];
     const float inv_n3 = 1.0f / ((float)W * (float)H * 3.0f);
     const float l1w = (1.0f - lambda) * inv_n3;
     auto sgn = [](float v) { return v > 0.f ? 1.f : (v < 0.f ? -1.f : 0.f); };
     float4 o;
-    o.x = sgn(x.x - y.x) * l1w + b[0] + 2.0f * x.x * b[3] + y.x * b[6];
-    o.y = sgn(x.y - y.y) * l1w + b[1] + 2.0f * x.y * b[4] + y.y * b[7];
-    o.z = sgn(x.z - y.z) * l1w + b[2] + 2.0f * x.z * b[5] + y.z * b[8];
+    o.x = sgn(x.x - y.x) * l1w + b[0][0] + 2.0f * x.x * b[0][1] + y.x * b[0][2];
+    o.y = sgn(x.y - y.y) * l1w + b[1][0] + 2.0f * x.y * b[1][1] + y.y * b[1][2];
+    o.z = sgn(x.z - y.z) * l1w + b[2][0] + 2.0f * x.z * b[2][1] + y.z * b[2][2];
     o.w = 0.0f;
     dl_out[pix] = o;
 }

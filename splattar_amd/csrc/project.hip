@@ -21,7 +21,8 @@ __global__ __launch_bounds__(256) void project_kernel(
     const float* __restrict__ opac, const float* __restrict__ sh, Record* __restrict__ recs,
     uint4* __restrict__ rects, uint32_t* __restrict__ depth_keys, uint32_t* __restrict__ counts,
     uint32_t* __restrict__ ranges_to_clear, uint32_t num_ranges, uint32_t* __restrict__ presort_keys,
-    uint32_t* __restrict__ presort_vals, uint32_t* __restrict__ item_count, bool clip_to_box) {
+    uint32_t* __restrict__ presort_vals, uint32_t* __restrict__ item_count, bool clip_to_box,
+    uint64_t* __restrict__ tile_masks) {
     // Views vary fastest over the blocks, so the V blocks that read the same
     // 256 surfels run back to back and the later ones find the parameters in
     // cache.
@@ -76,10 +77,28 @@ __global__ __launch_bounds__(256) void project_kernel(
         // Tight binning keeps only the tiles the alpha >= 1/255 footprint
         // reaches; the dropped entries would contribute nothing anywhere.
         if (clip_to_box) rect = clip_rect_to_box(rect, rec.box);
-        if (rect.x <= rect.z && rect.y <= rect.w)
-            count = (rect.z - rect.x + 1u) * (rect.w - rect.y + 1u);
-        else
-            rect = make_uint4(1u, 1u, 0u, 0u);
+        if (rect.x <= rect.z && rect.y <= rect.w) {
+            const uint32_t w = rect.z - rect.x + 1u;
+            count = w * (rect.w - rect.y + 1u);
+            if (tile_masks != nullptr) {
+                // Per-tile footprint test (box, disk, ellipse) for rects of at
+                // most 64 tiles: bit t = row-major tile t of the rect. Larger
+                // rects keep every tile (mask all ones, not consulted).
+                uint64_t m = ~0ull;
+                if (count <= 64u) {
+                    m = 0ull;
+                    for (uint32_t t = 0; t < count; ++t) {
+                        const float tx = (float)((rect.x + t % w) * kTile);
+                        const float ty = (float)((rect.y + t / w) * kTile);
+                        if (footprint_hits(rec, tx + 0.5f, ty + 0.5f, tx + 15.5f, ty + 15.5f))
+                            m |= 1ull << t;
+                    }
+                    count = (uint32_t)__popcll(m);
+                }
+                tile_masks[g] = m;
+            }
+        }
+        if (count == 0u) rect = make_uint4(1u, 1u, 0u, 0u);
     }
     rects[g] = rect;
     depth_keys[g] = __float_as_uint(depth);
@@ -132,14 +151,14 @@ hipError_t launch_project(hipStream_t st, const CamParams* cams_dev, uint32_t n_
                           const float* opac, const float* sh, Record* recs, uint4* rects,
                           uint32_t* depth_keys, uint32_t* counts, uint32_t* ranges_to_clear,
                           uint32_t num_ranges, uint32_t* presort_keys, uint32_t* presort_vals,
-                          uint32_t* item_count, bool clip_to_box) {
+                          uint32_t* item_count, bool clip_to_box, uint64_t* tile_masks) {
     if (n == 0 && num_ranges == 0 && item_count == nullptr) return hipSuccess;
     if (n_views == 0) return hipSuccess;
     const uint32_t blocks = div_up(n > num_ranges ? (n ? n : 1) : num_ranges, 256);
     dim3 grid(blocks * n_views);
     hipLaunchKernelGGL(project_kernel, grid, dim3(256), 0, st, cams_dev, n_views, n, pos, scales, quats, opac,
                        sh, recs, rects, depth_keys, counts, ranges_to_clear, num_ranges, presort_keys,
-                       presort_vals, item_count, clip_to_box);
+                       presort_vals, item_count, clip_to_box, tile_masks);
     return hipGetLastError();
 }
 

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -539,6 +540,16 @@ struct RasterCore {
     uint32_t* pre_vals[2] = {nullptr, nullptr};
     uint32_t* offs_sorted = nullptr;
     uint32_t* item_count = nullptr;
+    uint64_t* tile_masks = nullptr;  // tight binning: per-item mask of reached rect tiles
+    // Tile queue: longest-list-first order of the last forward's tiles and
+    // the two work cursors (forward, backward) of the persistent raster passes.
+    uint32_t* order_keys[2] = {nullptr, nullptr};
+    uint32_t* order_vals[2] = {nullptr, nullptr};
+    uint32_t* order_count = nullptr;
+    uint32_t* cursors = nullptr;
+    bool tile_queue = true;
+    uint32_t queue_blocks = 0;       // resident workgroups of a persistent raster launch
+    uint32_t queue_tiles = 0;        // tiles covered by the current order
     uint32_t* tkeys[2] = {nullptr, nullptr};  // 32-bit tile keys, stored inside keys[1]
     uint32_t binning = GS_BINNING_PRESORT;
     bool cull = true;                // per-quadrant footprint culling in the rasterizers
@@ -579,6 +590,13 @@ struct RasterCore {
         }
         dfree(offs_sorted);
         dfree(item_count);
+        dfree(tile_masks);
+        for (int i = 0; i < 2; ++i) {
+            dfree(order_keys[i]);
+            dfree(order_vals[i]);
+        }
+        dfree(order_count);
+        dfree(cursors);
     }
 
     gs_status init(gs_ctx c, uint32_t n, uint32_t views, uint32_t entries, uint32_t tiles,
@@ -614,13 +632,29 @@ struct RasterCore {
         GS_CHECK(dalloc(&scan_scratch, scan_scratch_bytes((uint32_t)std::min<size_t>(nv, 0xFFFFFFFFu)),
                         "scan scratch"));
         GS_CHECK(dalloc(&sort_scratch,
-                        sort_scratch_bytes((uint32_t)std::max<size_t>(entries, std::min<size_t>(nv, 0xFFFFFFFFu))),
+                        sort_scratch_bytes((uint32_t)std::max<size_t>(
+                            std::max<size_t>(entries, tiles), std::min<size_t>(nv, 0xFFFFFFFFu))),
                         "sort scratch"));
+        for (int i = 0; i < 2; ++i) {
+            GS_CHECK(dalloc(&order_keys[i], tiles ? tiles : 1, "tile order keys"));
+            GS_CHECK(dalloc(&order_vals[i], tiles ? tiles : 1, "tile order"));
+        }
+        GS_CHECK(dalloc(&order_count, 1, "tile order count"));
+        GS_CHECK(dalloc(&cursors, 2, "tile cursors"));
+        {
+            // Persistent raster launches: 8 workgroups per CU (more than can
+            // be resident; the rest start as slots free up). GS_TILE_QUEUE=0
+            // selects one workgroup per tile in tile order instead.
+            const char* env = std::getenv("GS_TILE_QUEUE");
+            tile_queue = !(env != nullptr && env[0] == '0');
+            queue_blocks = 8u * (uint32_t)std::max(1, ctx->props.multiProcessorCount);
+        }
         for (int i = 0; i < 2; ++i) {
             GS_CHECK(dalloc(&pre_keys[i], nv, "presort keys"));
             GS_CHECK(dalloc(&pre_vals[i], nv, "presort order"));
         }
         GS_CHECK(dalloc(&offs_sorted, nv, "presorted offsets"));
+        GS_CHECK(dalloc(&tile_masks, nv, "tile masks"));
         GS_CHECK(dalloc(&item_count, 1, "item count"));
         GS_CHECK(dalloc(&ranges, (size_t)tiles * 2, "ranges"));
         GS_CHECK(dalloc(&out_color, pixels, "out_color"));
@@ -712,7 +746,8 @@ struct RasterCore {
             GS_CHECK(P.mark(st, GS_STAGE_PROJECT));
             GS_HIP(launch_project(st, cd, nv, n, pos, sc, q, op, sh, recs, rects, depth_keys, offsets,
                                   ranges, 2 * max_tile_id, pre_keys[0], pre_vals[0], item_count,
-                                  binning == GS_BINNING_TIGHT));
+                                  binning == GS_BINNING_TIGHT,
+                                  binning == GS_BINNING_TIGHT ? tile_masks : nullptr));
             GS_CHECK(P.mark(st, GS_STAGE_SORT));
             int alt = 0;
             GS_HIP(launch_radix_sort_u32(st, pre_keys[0], pre_vals[0], pre_keys[1], pre_vals[1],
@@ -722,7 +757,8 @@ struct RasterCore {
             GS_HIP(launch_exclusive_scan(st, offsets, offs_sorted, items, total + v0, scan_scratch, order));
             GS_CHECK(P.mark(st, GS_STAGE_EMIT));
             GS_HIP(launch_emit_tile_keys(st, items, order, rects, offs_sorted, cd, n ? n : 1, tkeys[0],
-                                         pays[0], cap_entries, total + v0, sort_count + v0, overflow));
+                                         pays[0], cap_entries, total + v0, sort_count + v0, overflow,
+                                         binning == GS_BINNING_TIGHT ? tile_masks : nullptr));
             GS_CHECK(P.mark(st, GS_STAGE_SORT));
             GS_HIP(launch_radix_sort_u32(st, tkeys[0], pays[0], tkeys[1], pays[1], sort_count + v0,
                                          cap_entries, 0, bits_for(max_tile_id), digit_bits, sort_scratch,
@@ -731,9 +767,10 @@ struct RasterCore {
             GS_HIP(launch_tile_ranges_u32(st, tkeys[result_alt], sort_count + v0, cap_entries, ranges));
             keys64_valid = false;
             last_sort_count = sort_count + v0;
+            GS_CHECK(schedule_tiles(st, tiles));
             GS_CHECK(P.mark(st, GS_STAGE_RASTER_FWD));
             GS_HIP(launch_raster_fwd(st, cd, cams.data() + v0, nv, tiles, ranges, pays[result_alt], recs, o,
-                                     cull));
+                                     cull, queue(0)));
             return GS_OK;
         }
         // 64-bit (tile || depth) key path: one sort over every key bit that varies.
@@ -753,10 +790,31 @@ struct RasterCore {
         GS_HIP(launch_tile_ranges(st, keys[result_alt], sort_count + v0, cap_entries, ranges));
         keys64_valid = true;
         last_sort_count = sort_count + v0;
+        GS_CHECK(schedule_tiles(st, tiles));
         GS_CHECK(P.mark(st, GS_STAGE_RASTER_FWD));
         GS_HIP(launch_raster_fwd(st, cd, cams.data() + v0, nv, tiles, ranges, pays[result_alt], recs, o,
-                                 cull));
+                                 cull, queue(0)));
         return GS_OK;
+    }
+
+    // Longest-list-first order of the tiles just binned (16-bit keys, two
+    // stable 8-bit passes) and both work cursors reset to 0.
+    gs_status schedule_tiles(hipStream_t st, uint32_t tiles) {
+        queue_tiles = 0;
+        if (!tile_queue || tiles == 0) return GS_OK;
+        GS_HIP(launch_tile_order_keys(st, ranges, tiles, order_keys[0], order_vals[0], order_count, cursors));
+        int alt = 0;
+        GS_HIP(launch_radix_sort_u32(st, order_keys[0], order_vals[0], order_keys[1], order_vals[1],
+                                     order_count, tiles, 0, 16, 8, sort_scratch, &alt));
+        if (alt != 0) return fail(GS_ERR_INVALID, "tile order sort ended in the alternate buffer");
+        queue_tiles = tiles;
+        return GS_OK;
+    }
+
+    // Queue of a raster pass (cursor 0 = forward, 1 = backward), or none.
+    TileQueue queue(int which) const {
+        if (queue_tiles == 0) return TileQueue{nullptr, nullptr, 0};
+        return TileQueue{order_vals[0], cursors + which, queue_blocks};
     }
 
     gs_status backward(hipStream_t st, uint32_t v0, uint32_t nv, uint32_t n, const float4* pos,
@@ -773,8 +831,10 @@ struct RasterCore {
                                (size_t)n * nv * kGradStride));
         BwdIn in{dl_dcolor, dl_dnormal, out_color, out_aux, ncontrib};
         GS_CHECK(P.mark(st, GS_STAGE_RASTER_BWD));
+        const TileQueue qb = queue_tiles == tiles ? queue(1) : TileQueue{nullptr, nullptr, 0};
+        if (qb.cursor != nullptr) GS_HIP(launch_fill_u32(st, qb.cursor, 0u, 1));
         GS_HIP(launch_raster_bwd(st, cd, cams.data() + v0, nv, tiles, ranges, pays[result_alt], recs, in,
-                                 grad_coef, aux, cull));
+                                 grad_coef, aux, cull, qb));
         GS_CHECK(P.mark(st, GS_STAGE_GEOM_BWD));
         GS_HIP(launch_geom_bwd(st, cd, nv, n, pos, sc, q, sh, recs, grad_coef, g, accumulate,
                                cam_partials));

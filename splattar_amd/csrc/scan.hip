@@ -74,12 +74,26 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(const uint32_t* in, 
     const uint32_t first = part * kScanPart + (uint32_t)tid * kScanItems;
     uint32_t v[kScanItems];
     uint32_t sum = 0;
+    if (perm == nullptr && first + kScanItems <= n) {
+        // Whole run in range: 16-B loads (first is a multiple of 16 items).
+        const uint4* in4 = reinterpret_cast<const uint4*>(in + first);
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        const uint32_t idx = first + k;
-        v[k] = idx < n ? in[perm ? perm[idx] : idx] : 0u;
-        sum += v[k];
+        for (int k = 0; k < kScanItems / 4; ++k) {
+            const uint4 q = in4[k];
+            v[4 * k + 0] = q.x;
+            v[4 * k + 1] = q.y;
+            v[4 * k + 2] = q.z;
+            v[4 * k + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            const uint32_t idx = first + k;
+            v[k] = idx < n ? in[perm ? perm[idx] : idx] : 0u;
+        }
     }
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) sum += v[k];
     const uint32_t incl = wave_inclusive_scan(sum, lane);
     if (lane == 63) s_wave[wave] = incl;
     __syncthreads();
@@ -90,32 +104,54 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(const uint32_t* in, 
         if (w < wave) wave_excl += t;
         block_total += t;
     }
-    if (tid == 0) {
+    if (wave == 0) {
+        // Wave-parallel decoupled look-back: lane l reads the status of
+        // partition q - l. The lanes before the first unpublished entry are
+        // summed up to and including the first inclusive prefix; if an entry
+        // is unpublished first, the ready lanes before it are consumed and
+        // the window moves there. Partition 0 always publishes a prefix, so
+        // the walk ends, and no lane whose partition index is negative is
+        // ever consumed.
         uint32_t excl = 0;
         if (part == 0) {
-            __hip_atomic_store(&status[0], pack_status(tag, kFlagPrefix, block_total),
-                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0)
+                __hip_atomic_store(&status[0], pack_status(tag, kFlagPrefix, block_total),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            __hip_atomic_store(&status[part], pack_status(tag, kFlagAgg, block_total),
-                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0)
+                __hip_atomic_store(&status[part], pack_status(tag, kFlagAgg, block_total),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t want = tag & 0x3FFFFFFFu;
             int64_t q = (int64_t)part - 1;
             while (true) {
-                const uint64_t s =
-                    __hip_atomic_load(&status[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const uint64_t flag = (s >> 32) & 3ull;
-                if ((uint32_t)(s >> 34) != (tag & 0x3FFFFFFFu) || flag == 0ull) {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                excl += (uint32_t)s;
-                if (flag == kFlagPrefix) break;
-                --q;
+                const int64_t idx = q - lane;
+                const uint64_t st = idx >= 0 ? __hip_atomic_load(&status[idx], __ATOMIC_RELAXED,
+                                                                 __HIP_MEMORY_SCOPE_AGENT)
+                                             : 0ull;
+                const uint64_t flag = (st >> 32) & 3ull;
+                const bool ready = (uint32_t)(st >> 34) == want && flag != 0ull;
+                const uint64_t not_ready = __ballot(!ready);
+                const uint64_t prefixes = __ballot(ready && flag == kFlagPrefix);
+                const int first_nr = not_ready ? __builtin_ctzll(not_ready) : 64;
+                const int first_pf = prefixes ? __builtin_ctzll(prefixes) : 64;
+                const bool done = first_pf < first_nr;
+                const int take = done ? first_pf + 1 : first_nr;
+                uint32_t val = lane < take ? (uint32_t)st : 0u;
+#pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) val += __shfl_xor(val, o);
+                excl += val;
+                if (done) break;
+                q -= take;
+                if (take < 64) __builtin_amdgcn_s_sleep(1);
             }
-            __hip_atomic_store(&status[part], pack_status(tag, kFlagPrefix, excl + block_total),
-                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0)
+                __hip_atomic_store(&status[part], pack_status(tag, kFlagPrefix, excl + block_total),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        s_prefix = excl;
-        if (part == nparts - 1) *total = excl + block_total;
+        if (lane == 0) {
+            s_prefix = excl;
+            if (part == nparts - 1) *total = excl + block_total;
+        }
     }
     __syncthreads();
     uint32_t run = s_prefix + wave_excl + (incl - sum);

@@ -9,9 +9,10 @@
 //     read of the keys (LDS histograms, one global atomic per bin per block);
 //   * each pass kernel takes a 4096-key partition (partition id from an
 //     atomic counter), ranks its keys stably with wave64 ballots (8 ballots
-//     per key at 8-bit digits), resolves the partition's global digit offsets
-//     by decoupled look-back over 64-bit tagged status words, permutes the
-//     partition through LDS and writes it out in digit runs.
+//     per key at 8-bit digits), publishes its digit counts, permutes the
+//     partition through LDS, resolves its global digit offsets by decoupled
+//     look-back over 64-bit tagged status words (16 predecessors per round
+//     trip) and writes it out in digit runs.
 // Digit width is 4 or 8 bits; only the bit range [begin_bit, end_bit) is
 // sorted, so a key whose high bits are known constant costs fewer passes.
 //
@@ -29,6 +30,7 @@ constexpr int kSortWaves = kSortThreads / 64;
 constexpr int kSortItems = 16;                       // keys per thread
 constexpr uint32_t kSortPart = kSortThreads * kSortItems;  // 4096 keys per partition
 constexpr int kMaxPasses = 16;
+constexpr int kLookback = 16;                        // predecessor statuses per look-back round
 
 struct SortScratch {
     uint32_t counters[kMaxPasses];   // partition counters, one per pass
@@ -174,38 +176,17 @@ __global__ __launch_bounds__(kSortThreads) void onesweep_kernel(
     const uint32_t hist_d = (uint32_t)tid < R ? sc->hist[pass * R + tid] : 0u;
     const uint32_t global_start = block_exclusive_scan_digits<R>(hist_d, s_tmp);
 
+    // Publish this partition's digit counts first (successors may sum them),
+    // then permute the partition through LDS into digit order (only block-
+    // local offsets are needed), which frees the key/value registers before
+    // the look-back.
+    uint64_t* my = status + (size_t)part * R + tid;
     if ((uint32_t)tid < R) {
         s_digit_start[tid] = digit_start;
-        // Decoupled look-back, one thread per digit.
-        uint64_t* my = status + (size_t)part * R + tid;
-        uint32_t excl = 0;
-        if (part == 0) {
-            __hip_atomic_store(my, pack_status(tag, kFlagPrefix, block_count), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            __hip_atomic_store(my, pack_status(tag, kFlagAgg, block_count), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-            int64_t q = (int64_t)part - 1;
-            while (true) {
-                const uint64_t s = __hip_atomic_load(status + (size_t)q * R + tid, __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
-                const uint64_t flag = (s >> 32) & 3ull;
-                if ((uint32_t)(s >> 34) != (tag & 0x3FFFFFFFu) || flag == 0ull) {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                excl += (uint32_t)s;
-                if (flag == kFlagPrefix) break;
-                --q;
-            }
-            __hip_atomic_store(my, pack_status(tag, kFlagPrefix, excl + block_count),
-                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        s_out_base[tid] = global_start + excl - digit_start;
+        __hip_atomic_store(my, pack_status(tag, part == 0 ? kFlagPrefix : kFlagAgg, block_count),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
-
-    // Permute the partition through LDS into digit order.
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
         const uint32_t idx = wbase + (uint32_t)r * 64u + (uint32_t)lane;
@@ -215,6 +196,47 @@ __global__ __launch_bounds__(kSortThreads) void onesweep_kernel(
             s_keys[pos] = key[r];
             s_vals[pos] = val[r];
         }
+    }
+    if ((uint32_t)tid < R) {
+        // Decoupled look-back, one thread per digit, kLookback predecessor
+        // statuses per round trip (nearest first). Aggregates are summed up
+        // to the first inclusive prefix; an entry not yet published ends the
+        // round and the next round resumes there. Partition 0 always
+        // publishes a prefix, so the walk ends; no load goes below it.
+        uint32_t excl = 0;
+        if (part != 0) {
+            int64_t q = (int64_t)part - 1;
+            const uint32_t want = tag & 0x3FFFFFFFu;
+            while (true) {
+                uint64_t st[kLookback];
+#pragma unroll
+                for (int k = 0; k < kLookback; ++k)
+                    st[k] = q - k >= 0 ? __hip_atomic_load(status + (size_t)(q - k) * R + tid,
+                                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                       : 0ull;
+                int used = 0;
+                bool found = false, stalled = false;
+#pragma unroll
+                for (int k = 0; k < kLookback; ++k) {
+                    if (found || stalled) continue;
+                    const uint64_t s = st[k];
+                    const uint64_t flag = (s >> 32) & 3ull;
+                    if ((uint32_t)(s >> 34) != want || flag == 0ull) {
+                        stalled = true;
+                        continue;
+                    }
+                    excl += (uint32_t)s;
+                    ++used;
+                    if (flag == kFlagPrefix) found = true;
+                }
+                if (found) break;
+                q -= used;
+                if (stalled) __builtin_amdgcn_s_sleep(1);
+            }
+            __hip_atomic_store(my, pack_status(tag, kFlagPrefix, excl + block_count),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_out_base[tid] = global_start + excl - digit_start;
     }
     __syncthreads();
     const uint32_t valid = min(kSortPart, n - part * kSortPart);
