@@ -58,6 +58,8 @@ def parse():
                    help="tile binning: presort with footprint-clipped rects (same images), "
                         "depth presort + 32-bit tile sort, or one 64-bit key sort")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-autotune", action="store_true",
+                   help="keep --digit-bits and the default tile queue instead of autotuning")
     p.add_argument("--cpu-crop", default="480x270")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run 5 device copies of 1 GiB (read 1 GiB + write 1 GiB each), the "
@@ -196,6 +198,12 @@ def main():
 
     # ---- batched + graph (the headline)
     tr = sa.Trainer(ctx, cfg, scene.as_init(), views)
+    schedule = {"digit_bits": a.digit_bits, "blocks_per_cu": None}
+    if not a.no_autotune:
+        # Untimed: real training steps under each candidate schedule.
+        t0 = time.perf_counter()
+        schedule = tr.autotune(ids_for(0), reps=2)
+        log(f"autotune picked {schedule} in {time.perf_counter() - t0:.1f}s")
     run(tr, tr.GRAPH, a.warmup, 0)
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -243,7 +251,7 @@ def main():
     T = V * ((W + 15) // 16) * ((H + 15) // 16)
     import math
     tile_bits = max(1, math.ceil(math.log2(T)))
-    passes = math.ceil((32 + tile_bits) / a.digit_bits)
+    passes = math.ceil((32 + tile_bits) / schedule["digit_bits"])
     sb = stage_bytes(n, V, entries_step, P, T, passes)
     dom = max(stage_ms, key=stage_ms.get)
     achieved = sb[dom] / (stage_ms[dom] * 1e-3) / 1e9
@@ -257,7 +265,8 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"C3: {n} surfels SH3 heavy-tailed scales, {V}x{W}x{H} views/step, "
                                f"L1+SSIM, Adam", "surfels": n, "views_per_step": V, "width": W,
-                   "height": H, "entries_per_step": entries_step, "digit_bits": a.digit_bits, "binning": a.binning,
+                   "height": H, "entries_per_step": entries_step, "digit_bits": schedule["digit_bits"],
+                   "raster_blocks_per_cu": schedule["blocks_per_cu"], "binning": a.binning,
                    "parallelism": "single-gpu"},
         "plain_steps_per_s": round(1.0 / t_plain, 4),
         "batched_nograph_steps_per_s": round(1.0 / t_batched, 4),

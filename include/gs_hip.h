@@ -294,7 +294,7 @@ typedef struct gs_train_config {
     float lr_pos, lr_scales, lr_quat, lr_opacity, lr_sh;   /* trainer.rs:471-476 */
     uint32_t max_views_per_step;
     uint32_t entries_per_view;  /* binning capacity per view (0 = 64 * n) */
-    uint32_t digit_bits;        /* 0 = autotune, else 4 or 8 */
+    uint32_t digit_bits;        /* 4 or 8; 0 = 8 until gs_trainer_autotune picks */
     uint32_t binning;           /* GS_BINNING_SORT64 (0), GS_BINNING_PRESORT or GS_BINNING_TIGHT */
 } gs_train_config;
 
@@ -310,12 +310,28 @@ gs_status gs_trainer_create(gs_ctx ctx, const gs_train_config* cfg, const gs_sce
 void gs_trainer_destroy(gs_trainer t);
 gs_status gs_trainer_step(gs_trainer t, void* stream, const uint32_t* view_ids, uint32_t n_ids,
                           uint32_t mode);
+/* Blocking. No counterpart in the reference (its digit width and dispatch
+ * shape are fixed): times `reps` batched steps on `view_ids` under each
+ * candidate schedule -- radix digit width {8, 4} x raster scheduling
+ * {persistent tile queue with 8 or 4 workgroups per CU, one workgroup per
+ * tile} -- after one settling step each, and keeps the fastest. These are
+ * real training steps (6 (reps + 1) of them); no candidate changes the
+ * step's result. chosen (optional, 2 entries) = digit bits, workgroups per
+ * CU (0 = one per tile). A captured step graph is recaptured afterwards. */
+gs_status gs_trainer_autotune(gs_trainer t, const uint32_t* view_ids, uint32_t n_ids, uint32_t reps,
+                              uint32_t* chosen);
+/* The schedule gs_trainer_autotune chooses among, set directly (no
+ * counterpart in the reference): radix digit bits 4 or 8; blocks_per_cu
+ * workgroups per CU for the persistent raster passes, 0 = one per tile. */
+gs_status gs_trainer_set_schedule(gs_trainer t, uint32_t digit_bits, uint32_t blocks_per_cu);
 /* Blocking: per-view (l1_mean, dssim_mean) of the last step, 2 per view. */
 gs_status gs_trainer_last_losses(gs_trainer t, double* out, uint32_t n_ids);
 /* Blocking: render view `view` with the current parameters into rgba (W*H*4). */
 gs_status gs_trainer_render(gs_trainer t, uint32_t view, float* rgba);
 /* Activated parameters (positions n*4, scales n*2, quats n*4, opacities n,
- * sh n*48) and parameter gradients of the last step. */
+ * sh n*48) and parameter gradients of the last step. The step's losses are
+ * colour-only, so its forward writes out_color, out_aux = (T_final, 0, 0, 0)
+ * and out_ncontrib, and leaves out_normal untouched. */
 gs_status gs_trainer_params(gs_trainer t, gs_raster_buffers* out);
 /* Blocking: entries of the last step (sum over its views); GS_ERR_OVERFLOW
  * if any view exceeded its binning capacity. */

@@ -449,6 +449,18 @@ class Trainer:
         ids = (C.c_uint32 * len(view_ids))(*[int(v) for v in view_ids])
         check(lib().gs_trainer_step(self.h, stream, ids, len(view_ids), mode))
 
+    def set_schedule(self, digit_bits: int, blocks_per_cu: int):
+        check(lib().gs_trainer_set_schedule(self.h, digit_bits, blocks_per_cu))
+
+    def autotune(self, view_ids, reps: int = 2) -> dict:
+        """Time the candidate schedules on real batched steps and keep the
+        fastest; returns {"digit_bits", "blocks_per_cu"} (0 = one workgroup
+        per tile)."""
+        ids = (C.c_uint32 * len(view_ids))(*[int(v) for v in view_ids])
+        chosen = (C.c_uint32 * 2)()
+        check(lib().gs_trainer_autotune(self.h, ids, len(view_ids), reps, chosen))
+        return {"digit_bits": int(chosen[0]), "blocks_per_cu": int(chosen[1])}
+
     def last_losses(self, k: int) -> np.ndarray:
         out = (C.c_double * (2 * k))()
         check(lib().gs_trainer_last_losses(self.h, out, k))

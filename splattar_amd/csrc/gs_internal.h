@@ -532,7 +532,8 @@ struct TileQueue {
 hipError_t launch_raster_fwd(hipStream_t st, const CamParams* cams_dev, const CamParams* cams_host,
                              uint32_t n_views, uint32_t total_tiles, const uint32_t* ranges,
                              const uint32_t* payloads, const Record* recs, FwdOut out,
-                             bool cull = true, TileQueue queue = {nullptr, nullptr, 0});
+                             bool cull = true, TileQueue queue = {nullptr, nullptr, 0},
+                             bool aux_outputs = true);
 
 // raster_bwd.hip
 struct BwdIn {

@@ -35,6 +35,9 @@ __device__ __forceinline__ uint32_t view_of_tile(const CamParams* cams, uint32_t
     return v;
 }
 
+// AUX = false (training with colour losses only) skips the depth and normal
+// accumulators: aux = (T_final, 0, 0, 0), normal is not written.
+template <bool AUX>
 __global__ __launch_bounds__(256) void raster_fwd_kernel(const CamParams* __restrict__ cams,
                                                          uint32_t n_views,
                                                          const uint32_t* __restrict__ ranges,
@@ -89,7 +92,7 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const CamParams* __rest
                 s_a[tid] = r.a_px;
                 s_b[tid] = r.b_py;
                 s_c[tid] = r.col_op;
-                s_n[tid] = r.nrm_dep;
+                if (AUX) s_n[tid] = r.nrm_dep;
                 s_lthr[tid] = record_lthr(r.col_op.w);
                 qmask = cull ? quadrant_mask(r, tx, ty) : 0xFu;
             }
@@ -120,7 +123,7 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const CamParams* __rest
                     r.a_px = s_a[j];
                     r.b_py = s_b[j];
                     r.col_op = s_c[j];
-                    r.nrm_dep = s_n[j];
+                    r.nrm_dep = AUX ? s_n[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                     const HitEval h = eval_hit(r, sx, sy, pxc, pyc, s_lthr[j]);
                     if (!done && h.alpha > 0.0f) {
                         const float w = T * h.alpha;
@@ -128,10 +131,12 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const CamParams* __rest
                         cg += r.col_op.y * w;
                         cb += r.col_op.z * w;
                         wsum += w;
-                        dsum += h.t * w;
-                        nx += r.nrm_dep.x * w;
-                        ny += r.nrm_dep.y * w;
-                        nz += r.nrm_dep.z * w;
+                        if (AUX) {
+                            dsum += h.t * w;
+                            nx += r.nrm_dep.x * w;
+                            ny += r.nrm_dep.y * w;
+                            nz += r.nrm_dep.z * w;
+                        }
                         T *= 1.0f - h.alpha;
                         last = base - start + j + 1u;
                         if (T < kTTerminate) done = true;
@@ -145,7 +150,7 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const CamParams* __rest
             const size_t pi = (size_t)cam.pix_base + (size_t)py * cam.width + (size_t)px;
             out.color[pi] = make_float4(cr, cg, cb, wsum);
             out.aux[pi] = make_float4(T, dsum, 0.0f, 0.0f);
-            out.normal[pi] = make_float4(nx, ny, nz, 0.0f);
+            if (AUX) out.normal[pi] = make_float4(nx, ny, nz, 0.0f);
             out.ncontrib[pi] = last;
         }
         __syncthreads();  // s_view and the LDS batch are reused by the next tile
@@ -157,12 +162,16 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const CamParams* __rest
 hipError_t launch_raster_fwd(hipStream_t st, const CamParams* cams_dev, const CamParams* /*cams_host*/,
                              uint32_t n_views, uint32_t total_tiles, const uint32_t* ranges,
                              const uint32_t* payloads, const Record* recs, FwdOut out, bool cull,
-                             TileQueue queue) {
+                             TileQueue queue, bool aux_outputs) {
     if (total_tiles == 0) return hipSuccess;
     const uint32_t grid = queue.cursor != nullptr ? std::min(total_tiles, std::max(queue.grid_blocks, 1u))
                                                   : total_tiles;
-    hipLaunchKernelGGL(raster_fwd_kernel, dim3(grid), dim3(256), 0, st, cams_dev, n_views, ranges, payloads,
-                       recs, out, cull, total_tiles, queue);
+    if (aux_outputs)
+        hipLaunchKernelGGL(raster_fwd_kernel<true>, dim3(grid), dim3(256), 0, st, cams_dev, n_views, ranges,
+                           payloads, recs, out, cull, total_tiles, queue);
+    else
+        hipLaunchKernelGGL(raster_fwd_kernel<false>, dim3(grid), dim3(256), 0, st, cams_dev, n_views, ranges,
+                           payloads, recs, out, cull, total_tiles, queue);
     return hipGetLastError();
 }
 

@@ -553,6 +553,7 @@ struct RasterCore {
     uint32_t* tkeys[2] = {nullptr, nullptr};  // 32-bit tile keys, stored inside keys[1]
     uint32_t binning = GS_BINNING_PRESORT;
     bool cull = true;                // per-quadrant footprint culling in the rasterizers
+    bool aux_outputs = true;         // forward writes depth (aux.y) and normals
     bool keys64_valid = false;       // keys[result_alt] holds 64-bit keys of the last forward
     uint32_t digit_bits = 8;
     int result_alt = 0;              // which ping-pong buffer holds the sorted entries
@@ -770,7 +771,7 @@ struct RasterCore {
             GS_CHECK(schedule_tiles(st, tiles));
             GS_CHECK(P.mark(st, GS_STAGE_RASTER_FWD));
             GS_HIP(launch_raster_fwd(st, cd, cams.data() + v0, nv, tiles, ranges, pays[result_alt], recs, o,
-                                     cull, queue(0)));
+                                     cull, queue(0), aux_outputs));
             return GS_OK;
         }
         // 64-bit (tile || depth) key path: one sort over every key bit that varies.
@@ -793,7 +794,7 @@ struct RasterCore {
         GS_CHECK(schedule_tiles(st, tiles));
         GS_CHECK(P.mark(st, GS_STAGE_RASTER_FWD));
         GS_HIP(launch_raster_fwd(st, cd, cams.data() + v0, nv, tiles, ranges, pays[result_alt], recs, o,
-                                 cull, queue(0)));
+                                 cull, queue(0), aux_outputs));
         return GS_OK;
     }
 
@@ -1433,6 +1434,9 @@ gs_status gs_trainer_create(gs_ctx ctx, const gs_train_config* cfg, const gs_sce
     GS_CHECK(t->core.init(ctx, n, V, (uint32_t)entries, max_tiles * V, max_px * V));
     t->core.digit_bits = cfg->digit_bits ? cfg->digit_bits : 8;
     t->core.binning = cfg->binning;
+    // The step's losses are colour-only (L1 + D-SSIM): the forward skips the
+    // depth and normal images (aux.y = 0; out_normal is left untouched).
+    t->core.aux_outputs = false;
     GS_CHECK(dalloc(&t->targets, toff, "targets"));
     {
         std::vector<float> buf;
@@ -1570,6 +1574,80 @@ gs_status gs_trainer_step(gs_trainer t, void* stream, const uint32_t* view_ids, 
     GS_CHECK(t->enqueue_step(st, view_ids, k, batched, t->prof.enabled));
     // Fold finished events so the event pool stays bounded.
     if (t->prof.enabled) GS_CHECK(t->prof.collect());
+    return GS_OK;
+}
+
+static void drop_graph(gs_trainer t) {
+    // A captured graph holds the previous schedule's launches: recapture.
+    if (t->graph_exec) {
+        (void)hipGraphExecDestroy(t->graph_exec);
+        t->graph_exec = nullptr;
+    }
+    if (t->graph) {
+        (void)hipGraphDestroy(t->graph);
+        t->graph = nullptr;
+    }
+    t->graph_shape.clear();
+}
+
+gs_status gs_trainer_set_schedule(gs_trainer t, uint32_t digit_bits, uint32_t blocks_per_cu) {
+    GS_REQUIRE(t != nullptr, "trainer is null");
+    GS_REQUIRE(digit_bits == 4 || digit_bits == 8, "digit_bits must be 4 or 8");
+    GS_REQUIRE(blocks_per_cu <= 64, "blocks_per_cu must be <= 64");
+    const uint32_t cus = (uint32_t)std::max(1, t->ctx->props.multiProcessorCount);
+    t->core.digit_bits = digit_bits;
+    t->core.tile_queue = blocks_per_cu != 0;
+    t->core.queue_blocks = std::max(1u, blocks_per_cu) * cus;
+    drop_graph(t);
+    return GS_OK;
+}
+
+gs_status gs_trainer_autotune(gs_trainer t, const uint32_t* view_ids, uint32_t n_ids, uint32_t reps,
+                              uint32_t* chosen) {
+    GS_REQUIRE(t && view_ids, "null argument");
+    GS_REQUIRE(reps >= 1 && reps <= 16, "reps must be in 1..16");
+    // Candidate schedules; none changes the step's result (the sorts are
+    // stable, scheduling only reorders float atomics).
+    struct Cand {
+        uint32_t digit_bits, blocks_per_cu;  // blocks_per_cu 0: one workgroup per tile
+    };
+    static const Cand cands[] = {{8, 8}, {8, 4}, {8, 0}, {4, 8}, {4, 4}, {4, 0}};
+    auto apply = [&](const Cand& c) { (void)gs_trainer_set_schedule(t, c.digit_bits, c.blocks_per_cu); };
+    hipStream_t st = t->ctx->stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    GS_HIP(hipEventCreate(&e0));
+    GS_HIP(hipEventCreate(&e1));
+    gs_status s = GS_OK;
+    float best = 0.0f;
+    size_t best_i = 0;
+    for (size_t i = 0; i < sizeof(cands) / sizeof(cands[0]) && s == GS_OK; ++i) {
+        apply(cands[i]);
+        // One settling step (first use of the schedule), then `reps` timed.
+        s = gs_trainer_step(t, st, view_ids, n_ids, GS_STEP_BATCHED);
+        if (s == GS_OK && hipEventRecord(e0, st) != hipSuccess) s = fail(GS_ERR_HIP, "hipEventRecord");
+        for (uint32_t r = 0; r < reps && s == GS_OK; ++r)
+            s = gs_trainer_step(t, st, view_ids, n_ids, GS_STEP_BATCHED);
+        if (s != GS_OK) break;
+        if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+            s = fail(GS_ERR_HIP, "autotune timing");
+            break;
+        }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (i == 0 || ms < best) {
+            best = ms;
+            best_i = i;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    GS_CHECK(s);
+    apply(cands[best_i]);
+    t->tuned = true;
+    if (chosen) {
+        chosen[0] = cands[best_i].digit_bits;
+        chosen[1] = cands[best_i].blocks_per_cu;
+    }
     return GS_OK;
 }
 

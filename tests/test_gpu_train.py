@@ -207,6 +207,41 @@ def test_batched_and_graph_steps_equal_plain(gpu):
             np.testing.assert_allclose(p[k], p0[k], rtol=1e-4, atol=2e-6, err_msg=f"mode {mode} {k}")
 
 
+SCHEDULES = [(8, 8), (8, 4), (8, 0), (4, 8), (4, 4), (4, 0)]
+
+
+def test_every_schedule_gives_the_same_step(gpu):
+    """The autotuner's candidates (digit width x raster scheduling) change
+    only launch shapes: three graph steps give the same parameters and
+    losses under each, and autotune picks one of them."""
+    deg = 2
+    sc, tgt, specs = _small_training_problem(deg=deg)
+    targets = _render_targets(gpu, tgt, specs, deg)
+    results = {}
+    for sched in SCHEDULES:
+        t = _trainer(gpu, sc, specs, targets, deg, 3)
+        t.set_schedule(*sched)
+        announce(f"schedule digit={sched[0]} blocks_per_cu={sched[1]}")
+        for it in range(3):
+            t.step([it % 3, (it + 1) % 3, (it + 2) % 3], mode=t.GRAPH)
+        results[sched] = (_params(t), t.last_losses(3), t.entry_count())
+        t.close()
+    p0, l0, e0 = results[SCHEDULES[0]]
+    for sched in SCHEDULES[1:]:
+        p, l, e = results[sched]
+        assert e == e0
+        np.testing.assert_allclose(l, l0, rtol=1e-5)
+        for k in p0:
+            np.testing.assert_allclose(p[k], p0[k], rtol=1e-4, atol=2e-6, err_msg=f"{sched} {k}")
+    t = _trainer(gpu, sc, specs, targets, deg, 3)
+    announce("autotune")
+    chosen = t.autotune([0, 1, 2], reps=1)
+    assert (chosen["digit_bits"], chosen["blocks_per_cu"]) in SCHEDULES
+    t.step([0, 1, 2], mode=t.GRAPH)
+    assert np.isfinite(t.last_losses(3)).all()
+    t.close()
+
+
 def test_training_reduces_loss(gpu):
     deg = 1
     sc, tgt, specs = _small_training_problem(n=2000, views=3, size=(64, 64), deg=deg, seed=11)
@@ -222,3 +257,47 @@ def test_training_reduces_loss(gpu):
     print(f"L1 {first:.5f} -> {last:.5f}")
     assert last < 0.7 * first
     t.close()
+
+
+def test_uneven_multiview_batch_matches_fp64(gpu):
+    """One captured batched step over 8 views of uneven resolution (one launch
+    per stage for all views, gradients summed per surfel) against the FP64
+    sum of per-view gradients; learning rates 0 leave the parameters at the
+    state the gradient is taken at. Per class normwise <= max(1e-4, 4x the
+    FP64 gradient's own unit-roundoff sensitivity)."""
+    import multiview as mv
+    import splattar_amd as sa
+    sc, tgt, specs = mv.problem(n=1200, seed=17)
+    tgts = mv.targets(tgt, specs)
+    s64 = mv.scene64(sc)
+    _, ref = mv.total_loss(s64, specs, tgts, grad=True)
+    rng = np.random.default_rng(0)
+    sens = {}
+    for _ in range(2):
+        sp = s64.copy()
+        for arr in (sp.pos, sp.scales, sp.quat, sp.opacity, sp.sh):
+            arr *= 1.0 + rng.choice([-1.0, 1.0], arr.shape) * 2.0 ** -24
+        _, g2 = mv.total_loss(sp, specs, tgts, grad=True)
+        for k in ("pos", "scales", "quat", "opacity", "sh"):
+            c = getattr(ref, k)
+            sens[k] = max(sens.get(k, 0.0), float(np.linalg.norm(getattr(g2, k) - c) / np.linalg.norm(c)))
+    cfg = sa.TrainConfig(lam=mv.LAM, dssim_scale=mv.DSSIM_SCALE, lr_pos=0.0, lr_scales=0.0,
+                         lr_quat=0.0, lr_opacity=0.0, lr_sh=0.0, max_views_per_step=mv.VIEWS)
+    views = []
+    for sp, t in zip(specs, tgts):
+        t4 = np.zeros((sp.height, sp.width, 4), np.float32)
+        t4[..., :3] = t
+        views.append((sa.Camera(sp.center, sp.quat, sp.focal, sc.sh_degree), t4))
+    tr = sa.Trainer(gpu, cfg, sc.as_init(), views)
+    announce(f"uneven 8-view batch: sizes {[(sp.width, sp.height) for sp in specs]}")
+    tr.step(list(range(mv.VIEWS)), mode=tr.GRAPH)
+    ncoef = (sc.sh_degree + 1) ** 2
+    got = dict(pos=tr.grad_pos.read()[:, :3], scales=tr.grad_scales.read(), quat=tr.grad_quat.read(),
+               opacity=tr.grad_opacity.read(), sh=tr.grad_sh.read()[:, :ncoef * 3])
+    tr.close()
+    for k in ("pos", "scales", "quat", "opacity", "sh"):
+        c = getattr(ref, k).reshape(got[k].shape)
+        nrm = float(np.linalg.norm(got[k] - c) / np.linalg.norm(c))
+        tol = max(1e-4, 4.0 * sens[k])
+        print(f"  {k}: normwise rel {nrm:.2e} (fp64 roundoff sensitivity {sens[k]:.2e}, bound {tol:.2e})")
+        assert nrm <= tol, k
