@@ -35,6 +35,36 @@ sys.path.insert(0, ROOT)
 
 METRIC = "train steps/s, 1M surfels @1080p, batched vs plain at equal loss; % HBM peak"
 HBM_PEAK_GBS = 8000.0
+# Non-packed VALU issue rate: 256 CUs x 4 SIMDs, one wave64 instruction per
+# SIMD every 4 cycles, 2.4 GHz (packed FP32 is not used).
+VALU_PEAK_WINST = 256 * 2.4e9
+COUNTERS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "c3_step_counters.json")
+STAGE_KERNEL = {"raster_bwd": "raster_bwd_kernel", "raster_fwd": "raster_fwd_kernel",
+                "sort": "onesweep_kernel", "project": "project_kernel", "l1_ssim": "ssim_fwd_kernel",
+                "adam": "adam_kernel", "geom_bwd": "geom_bwd_kernel"}
+
+
+def profile_figures(stage):
+    """HBM bytes and VALU rate of the stage's main kernel at its largest
+    launch, from the committed rocprofv3 summary (None if absent)."""
+    if not os.path.exists(COUNTERS) or stage not in STAGE_KERNEL:
+        return None
+    with open(COUNTERS) as f:
+        summ = json.load(f)
+    cands = [k for k in summ["kernels"] if STAGE_KERNEL[stage] in k["kernel"]]
+    if not cands:
+        return None
+    k = max(cands, key=lambda k: k["mean_ms"])
+    out = {"source": os.path.relpath(COUNTERS, os.path.dirname(COUNTERS) + "/.."), "kernel": k["kernel"],
+           "profiled_ms": round(k["mean_ms"], 4)}
+    if "hbm_read_bytes" in k and "hbm_write_bytes" in k:
+        out["traffic"] = int(k["hbm_read_bytes"] + k["hbm_write_bytes"])
+    cs = k.get("counters_per_dispatch", {})
+    if "SQ_INSTS_VALU" in cs:
+        rate = cs["SQ_INSTS_VALU"] / (k["mean_ms"] * 1e-3)
+        out["valu"] = {"achieved": round(rate / 1e9, 1), "peak": round(VALU_PEAK_WINST / 1e9, 1),
+                       "unit": "G wave-instr/s", "frac": round(rate / VALU_PEAK_WINST, 4)}
+    return out
 
 
 def log(msg):
@@ -60,6 +90,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-autotune", action="store_true",
                    help="keep --digit-bits and the default tile queue instead of autotuning")
+    p.add_argument("--schedule", default=None,
+                   help="DIGIT_BITS,BLOCKS_PER_CU: set this schedule instead of autotuning")
     p.add_argument("--cpu-crop", default="480x270")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run 5 device copies of 1 GiB (read 1 GiB + write 1 GiB each), the "
@@ -199,7 +231,11 @@ def main():
     # ---- batched + graph (the headline)
     tr = sa.Trainer(ctx, cfg, scene.as_init(), views)
     schedule = {"digit_bits": a.digit_bits, "blocks_per_cu": None}
-    if not a.no_autotune:
+    if a.schedule:
+        d, b = (int(x) for x in a.schedule.split(","))
+        tr.set_schedule(d, b)
+        schedule = {"digit_bits": d, "blocks_per_cu": b}
+    elif not a.no_autotune:
         # Untimed: real training steps under each candidate schedule.
         t0 = time.perf_counter()
         schedule = tr.autotune(ids_for(0), reps=2)
@@ -258,6 +294,10 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
                 "ms_per_launch": round(stage_ms[dom], 4), "algorithmic_bytes": int(sb[dom])}
+    prof = profile_figures(dom)
+    if prof is not None:
+        roofline["traffic"] = prof.pop("traffic", None)
+        roofline["profile"] = prof
     total_bytes = sum(sb.values())
     result = {
         "metric": METRIC, "value": round(value, 4), "unit": "steps/s", "n_gpus": 1, "steps": a.steps,

@@ -1,0 +1,116 @@
+"""Condense rocprofv3 output into the per-kernel summary bench.py reports.
+
+Inputs (CSV files written by rocprofv3 with `-f csv`):
+  --trace  <run_kernel_trace.csv>   from `--kernel-trace`
+  --pmc    <run_counter_collection.csv> (repeatable) from `--pmc ...` runs
+  --copy-fetch / --copy-write <run_counter_collection.csv> of
+           `bench.py --copy-calibration` under `--pmc FETCH_SIZE` /
+           `--pmc WRITE_SIZE`: a device copy of known size (1 GiB read,
+           1 GiB written per launch) that gives the scale of each counter.
+
+Output: JSON with, per (kernel, grid size): launches, mean duration (ms), the
+mean of every collected counter per dispatch, and for FETCH_SIZE /
+WRITE_SIZE the scaled HBM bytes per launch. Kernels are keyed by short name
+and total grid size, since one kernel runs at several shapes (one view per
+launch vs several).
+
+Usage:
+  python -m splattar_amd.rocprof_summary --trace T.csv --pmc A.csv --pmc B.csv \\
+      --copy-fetch CF.csv --copy-write CW.csv --command "..." --out summary.json
+"""
+
+from __future__ import annotations
+
+import argparse
+import collections
+import csv
+import json
+import re
+
+COPY_BYTES = 1 << 30  # bench.py --copy-calibration: 1 GiB read + 1 GiB written per launch
+
+
+def short_name(name: str) -> str:
+    name = name.replace("(anonymous namespace)::", "")
+    name = re.sub(r"^void ", "", name)
+    m = re.match(r"([\w:<>, ]+?)\(", name)
+    return (m.group(1) if m else name).strip()
+
+
+def _grid(row) -> int:
+    if "Grid_Size" in row:
+        return int(float(row["Grid_Size"]))
+    return int(row["Grid_Size_X"]) * int(row["Grid_Size_Y"]) * int(row["Grid_Size_Z"])
+
+
+def read_trace(path):
+    out = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        key = (short_name(r["Kernel_Name"]), _grid(r))
+        out[key].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6)
+    return out
+
+
+def read_pmc(path):
+    """(kernel, grid) -> counter -> list of per-dispatch values."""
+    per_dispatch = collections.defaultdict(dict)
+    keys = {}
+    for r in csv.DictReader(open(path)):
+        d = r["Dispatch_Id"]
+        keys[d] = (short_name(r["Kernel_Name"]), _grid(r))
+        per_dispatch[d][r["Counter_Name"]] = per_dispatch[d].get(r["Counter_Name"], 0.0) + float(
+            r["Counter_Value"])
+    out = collections.defaultdict(lambda: collections.defaultdict(list))
+    for d, counters in per_dispatch.items():
+        for c, v in counters.items():
+            out[keys[d]][c].append(v)
+    return out
+
+
+def copy_scale(path, counter):
+    """Bytes per counter unit: known copy bytes / measured counter (KiB)."""
+    vals = [v for (name, _), cs in read_pmc(path).items() if "copyBuffer" in name
+            for v in cs.get(counter, [])]
+    if not vals:
+        raise SystemExit(f"no copy dispatch with {counter} in {path}")
+    mean = sum(vals) / len(vals)
+    return COPY_BYTES / mean
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--trace", required=True)
+    ap.add_argument("--pmc", action="append", default=[])
+    ap.add_argument("--copy-fetch")
+    ap.add_argument("--copy-write")
+    ap.add_argument("--command", default="")
+    ap.add_argument("--out", required=True)
+    a = ap.parse_args(argv)
+    trace = read_trace(a.trace)
+    counters = collections.defaultdict(dict)
+    for p in a.pmc:
+        for key, cs in read_pmc(p).items():
+            for c, vals in cs.items():
+                counters[key][c] = sum(vals) / len(vals)
+    fetch_scale = copy_scale(a.copy_fetch, "FETCH_SIZE") if a.copy_fetch else None
+    write_scale = copy_scale(a.copy_write, "WRITE_SIZE") if a.copy_write else None
+    kernels = []
+    for (name, grid), durs in sorted(trace.items(), key=lambda kv: -sum(kv[1])):
+        k = {"kernel": name, "grid": grid, "launches": len(durs),
+             "mean_ms": sum(durs) / len(durs), "total_ms": sum(durs)}
+        cs = counters.get((name, grid), {})
+        if cs:
+            k["counters_per_dispatch"] = cs
+        if fetch_scale and "FETCH_SIZE" in cs:
+            k["hbm_read_bytes"] = cs["FETCH_SIZE"] * fetch_scale
+        if write_scale and "WRITE_SIZE" in cs:
+            k["hbm_write_bytes"] = cs["WRITE_SIZE"] * write_scale
+        kernels.append(k)
+    out = {"command": a.command, "fetch_bytes_per_unit": fetch_scale,
+           "write_bytes_per_unit": write_scale, "kernels": kernels}
+    with open(a.out, "w") as f:
+        json.dump(out, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()
