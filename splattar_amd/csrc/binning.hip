@@ -222,11 +222,8 @@ __global__ __launch_bounds__(256) void tile_order_keys_kernel(const uint32_t* __
                                                               uint32_t* __restrict__ count,
                                                               uint32_t* __restrict__ cursors) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0) {
-        *count = total_tiles;
-        cursors[0] = 0u;
-        cursors[1] = 0u;
-    }
+    if (t == 0) *count = total_tiles;
+    if (t < 2 * kTileQueues) cursors[t] = 0u;
     if (t >= total_tiles) return;
     const uint32_t len = ranges[2 * (size_t)t + 1] - ranges[2 * (size_t)t];
     keys[t] = 0xFFFFu - min(len, 0xFFFFu);

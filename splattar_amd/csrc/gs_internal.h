@@ -489,8 +489,9 @@ hipError_t launch_emit_tile_keys(hipStream_t st, uint32_t n_items, const uint32_
                                  uint32_t* sort_count, uint32_t* overflow_flag,
                                  const uint64_t* tile_masks = nullptr);
 // Longest-list-first tile order: keys[t] = 0xFFFF - min(list length, 0xFFFF),
-// vals[t] = t for the total_tiles tiles, *count = total_tiles and
-// cursors[0..1] = 0 (sort the pairs by 16 key bits, stably, to get the order).
+// vals[t] = t for the total_tiles tiles, *count = total_tiles and the
+// 2 kTileQueues cursors = 0 (sort the pairs by 16 key bits, stably, to get
+// the order).
 hipError_t launch_tile_order_keys(hipStream_t st, const uint32_t* ranges, uint32_t total_tiles,
                                   uint32_t* keys, uint32_t* vals, uint32_t* count, uint32_t* cursors);
 hipError_t launch_tile_ranges_u32(hipStream_t st, const uint32_t* keys, const uint32_t* count,
@@ -521,12 +522,16 @@ struct FwdOut {
     uint32_t* ncontrib;
 };
 // Tile scheduling for the raster passes: with a queue (cursor != nullptr),
-// grid_blocks resident workgroups pull tile positions k from *cursor (which
-// must be 0 at launch) and process tile order[k] (longest list first) until
-// k >= total_tiles; without one, workgroup b processes tile b.
+// grid_blocks resident workgroups pull tile positions from kTileQueues
+// cursors (which must be 0 at launch): workgroup b uses cursor
+// x = b % q, q = min(kTileQueues, grid), and claim j of it is position
+// k = j q + x; the workgroup processes tile order[k] (longest list first)
+// until k >= total_tiles. Several cursors spread the atomics over several
+// addresses. Without a queue, workgroup b processes tile b.
+constexpr uint32_t kTileQueues = 8;
 struct TileQueue {
     const uint32_t* order;
-    uint32_t* cursor;
+    uint32_t* cursor;  // kTileQueues counters
     uint32_t grid_blocks;
 };
 hipError_t launch_raster_fwd(hipStream_t st, const CamParams* cams_dev, const CamParams* cams_host,

@@ -113,7 +113,10 @@ __global__ __launch_bounds__(256) void raster_bwd_kernel(const CamParams* __rest
     for (uint32_t iter = 0;; ++iter) {
         uint32_t pos;
         if (queue.cursor != nullptr) {
-            if (tid == 0) s_next = atomicAdd(queue.cursor, 1u);
+            if (tid == 0) {
+                const uint32_t q = min(kTileQueues, gridDim.x), x = blockIdx.x % q;
+                s_next = atomicAdd(queue.cursor + x, 1u) * q + x;
+            }
             __syncthreads();
             pos = s_next;
             __syncthreads();  // every thread has pos before s_next is rewritten

@@ -641,7 +641,7 @@ struct RasterCore {
             GS_CHECK(dalloc(&order_vals[i], tiles ? tiles : 1, "tile order"));
         }
         GS_CHECK(dalloc(&order_count, 1, "tile order count"));
-        GS_CHECK(dalloc(&cursors, 2, "tile cursors"));
+        GS_CHECK(dalloc(&cursors, 2 * kTileQueues, "tile cursors"));
         {
             // Persistent raster launches: 8 workgroups per CU (more than can
             // be resident; the rest start as slots free up). GS_TILE_QUEUE=0
@@ -812,10 +812,10 @@ struct RasterCore {
         return GS_OK;
     }
 
-    // Queue of a raster pass (cursor 0 = forward, 1 = backward), or none.
+    // Queue of a raster pass (cursor set 0 = forward, 1 = backward), or none.
     TileQueue queue(int which) const {
         if (queue_tiles == 0) return TileQueue{nullptr, nullptr, 0};
-        return TileQueue{order_vals[0], cursors + which, queue_blocks};
+        return TileQueue{order_vals[0], cursors + which * kTileQueues, queue_blocks};
     }
 
     gs_status backward(hipStream_t st, uint32_t v0, uint32_t nv, uint32_t n, const float4* pos,
@@ -833,7 +833,7 @@ struct RasterCore {
         BwdIn in{dl_dcolor, dl_dnormal, out_color, out_aux, ncontrib};
         GS_CHECK(P.mark(st, GS_STAGE_RASTER_BWD));
         const TileQueue qb = queue_tiles == tiles ? queue(1) : TileQueue{nullptr, nullptr, 0};
-        if (qb.cursor != nullptr) GS_HIP(launch_fill_u32(st, qb.cursor, 0u, 1));
+        if (qb.cursor != nullptr) GS_HIP(launch_fill_u32(st, qb.cursor, 0u, kTileQueues));
         GS_HIP(launch_raster_bwd(st, cd, cams.data() + v0, nv, tiles, ranges, pays[result_alt], recs, in,
                                  grad_coef, aux, cull, qb));
         GS_CHECK(P.mark(st, GS_STAGE_GEOM_BWD));

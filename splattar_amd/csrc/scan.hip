@@ -63,11 +63,13 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(const uint32_t* in, 
     __shared__ uint32_t s_wave[kScanThreads / 64];
     __shared__ uint32_t s_prefix;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // Blocks beyond the partition count leave before the shared counter, so
+    // exactly nparts blocks claim ids 0..nparts-1 in arrival order.
+    const uint32_t nparts = (n + kScanPart - 1) / kScanPart;
+    if (blockIdx.x >= nparts) return;
     if (tid == 0) s_part = atomicAdd(&sc->counter, 1u);
     __syncthreads();
     const uint32_t part = s_part;
-    const uint32_t nparts = (n + kScanPart - 1) / kScanPart;
-    if (part >= nparts) return;
     const uint32_t tag = sc->tag;
     uint64_t* status = reinterpret_cast<uint64_t*>(sc + 1);
 

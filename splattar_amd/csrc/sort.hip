@@ -111,13 +111,17 @@ __global__ __launch_bounds__(kSortThreads) void onesweep_kernel(
     __shared__ uint32_t s_vals[kSortPart];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // The grid is sized for the capacity; blocks beyond the live partition
+    // count leave before touching the shared partition counter, so exactly
+    // nparts blocks claim the ids 0..nparts-1 (in arrival order, which keeps
+    // the look-back waiting only on blocks that are already running).
+    const uint32_t n = *count;
+    const uint32_t nparts = (n + kSortPart - 1) / kSortPart;
+    if (blockIdx.x >= nparts) return;
     if (tid == 0) s_part = atomicAdd(&sc->counters[pass], 1u);
     for (uint32_t k = tid; k < kSortWaves * R; k += kSortThreads) (&s_wave_hist[0][0])[k] = 0u;
     __syncthreads();
-    const uint32_t n = *count;
-    const uint32_t nparts = (n + kSortPart - 1) / kSortPart;
     const uint32_t part = s_part;
-    if (part >= nparts) return;
     const uint32_t tag = sc->sort_tag * (uint32_t)kMaxPasses + pass;
     uint64_t* status = reinterpret_cast<uint64_t*>(sc + 1);
 
