@@ -90,6 +90,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-autotune", action="store_true",
                    help="keep --digit-bits and the default tile queue instead of autotuning")
+    p.add_argument("--psnr-steps", type=int, default=500,
+                   help="steps of the batched-vs-plain equal-loss comparison (0 = skip)")
     p.add_argument("--schedule", default=None,
                    help="DIGIT_BITS,BLOCKS_PER_CU: set this schedule instead of autotuning")
     p.add_argument("--cpu-crop", default="480x270")
@@ -280,6 +282,33 @@ def main():
     tr.set_profiling(False)
     tr.close()
 
+    # ---- equal-loss check: the same K steps batched+graph and plain, from
+    # the same initial state, then mean PSNR over every view (untimed).
+    equal_loss = None
+    if a.psnr_steps > 0:
+        def train_and_score(mode):
+            t = sa.Trainer(ctx, cfg, scene.as_init(), views)
+            if schedule["blocks_per_cu"] is not None:
+                t.set_schedule(schedule["digit_bits"], schedule["blocks_per_cu"])
+            t0 = time.perf_counter()
+            run(t, mode, a.psnr_steps, 0)
+            ctx.synchronize()
+            secs = time.perf_counter() - t0
+            ps = []
+            for v, (_, tgt) in enumerate(views):
+                img = t.render(v)[..., :3].astype(np.float64)
+                mse = float(np.mean((img - tgt[..., :3].astype(np.float64)) ** 2))
+                ps.append(10.0 * np.log10(1.0 / max(mse, 1e-20)))
+            t.close()
+            return float(np.mean(ps)), a.psnr_steps / secs
+        pb, rb = train_and_score(sa.Trainer.GRAPH)
+        pp, rp = train_and_score(sa.Trainer.PLAIN)
+        equal_loss = {"steps": a.psnr_steps, "psnr_batched_graph_db": round(pb, 4),
+                      "psnr_plain_db": round(pp, 4), "psnr_diff_db": round(pb - pp, 4),
+                      "steps_per_s_batched_graph": round(rb, 2), "steps_per_s_plain": round(rp, 2),
+                      "ratio": round(rb / rp, 3)}
+        log(f"equal-loss check: {equal_loss}")
+
     ms_graph = 1000.0 * t_graph / a.steps
     value = a.steps / t_graph
     # Roofline of the dominant kernel.
@@ -311,6 +340,7 @@ def main():
         "plain_steps_per_s": round(1.0 / t_plain, 4),
         "batched_nograph_steps_per_s": round(1.0 / t_batched, 4),
         "batched_vs_plain": round(value * t_plain, 3),
+        "equal_loss": equal_loss,
         "step_hbm_frac": round(total_bytes / (ms_graph * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
         "mpix_per_s": round(P * value / 1e6, 2),
         "sorted_keys_per_s": round(entries_step / (stage_ms["sort"] * 1e-3), 1),
