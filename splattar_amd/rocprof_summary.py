@@ -25,6 +25,7 @@ import argparse
 import collections
 import csv
 import json
+import os
 import re
 
 COPY_BYTES = 1 << 30  # bench.py --copy-calibration: 1 GiB read + 1 GiB written per launch
@@ -89,6 +90,9 @@ def main(argv=None):
     trace = read_trace(a.trace)
     counters = collections.defaultdict(dict)
     for p in a.pmc:
+        if not os.path.exists(p):
+            print(f"rocprof_summary: no counter file {p}, skipped")
+            continue
         for key, cs in read_pmc(p).items():
             for c, vals in cs.items():
                 counters[key][c] = sum(vals) / len(vals)
