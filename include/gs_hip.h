@@ -311,13 +311,14 @@ void gs_trainer_destroy(gs_trainer t);
 gs_status gs_trainer_step(gs_trainer t, void* stream, const uint32_t* view_ids, uint32_t n_ids,
                           uint32_t mode);
 /* Blocking. No counterpart in the reference (its digit width and dispatch
- * shape are fixed): times `reps` batched steps on `view_ids` under each
- * candidate schedule -- radix digit width {8, 4} x raster scheduling
- * {persistent tile queue with 8 or 4 workgroups per CU, one workgroup per
- * tile} -- after one settling step each, and keeps the fastest. These are
- * real training steps (6 (reps + 1) of them); no candidate changes the
- * step's result. chosen (optional, 2 entries) = digit bits, workgroups per
- * CU (0 = one per tile). A captured step graph is recaptured afterwards. */
+ * shape are fixed): for each candidate schedule -- radix digit width {8, 4}
+ * x raster scheduling {persistent tile queue with 8 or 4 workgroups per CU,
+ * one workgroup per tile} -- runs one settling batched step on `view_ids`
+ * and times `reps` more one by one, over two interleaved rounds, and keeps
+ * the candidate with the fastest step. These are real training steps
+ * (12 (reps + 1) of them); no candidate changes the step's result. chosen
+ * (optional, 2 entries) = digit bits, workgroups per CU (0 = one per tile).
+ * A captured step graph is recaptured afterwards. */
 gs_status gs_trainer_autotune(gs_trainer t, const uint32_t* view_ids, uint32_t n_ids, uint32_t reps,
                               uint32_t* chosen);
 /* The schedule gs_trainer_autotune chooses among, set directly (no

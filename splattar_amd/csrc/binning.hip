@@ -135,11 +135,11 @@ __device__ __forceinline__ uint32_t select_set_bit(uint32_t lo, uint32_t hi, uin
 }
 
 __global__ __launch_bounds__(256) void emit_tile_keys_kernel(
-    uint32_t n_items, const uint32_t* __restrict__ order, const uint4* __restrict__ rects,
+    uint32_t n_items, const uint32_t* __restrict__ order, bool packed, const uint4* __restrict__ emit_info,
     const uint32_t* __restrict__ offsets_sorted, const CamParams* __restrict__ cams,
     uint32_t items_per_view, uint32_t* __restrict__ tkeys, uint32_t* __restrict__ payloads,
     uint32_t max_entries, const uint32_t* __restrict__ total, uint32_t* __restrict__ sort_count,
-    uint32_t* __restrict__ overflow_flag, const uint64_t* __restrict__ tile_masks) {
+    uint32_t* __restrict__ overflow_flag) {
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid == 0) {
         const uint32_t t = *total;
@@ -154,20 +154,21 @@ __global__ __launch_bounds__(256) void emit_tile_keys_kernel(
     uint64_t mask = 0;
     bool masked = false;
     if (gid < n_items) {
-        item = order[gid];
-        const uint4 r = rects[item];
-        if (r.z >= r.x && r.w >= r.y) {
-            w = r.z - r.x + 1u;
-            cnt = w * (r.w - r.y + 1u);
-            // Tight binning: only the rect tiles whose bit is set (rects of
-            // at most 64 tiles; project wrote the same count).
-            if (tile_masks != nullptr && cnt <= 64u) {
-                mask = tile_masks[item];
+        item = packed ? order[gid] >> 8 : order[gid];
+        const uint4 ei = emit_info[item];
+        const uint32_t rx0 = ei.x & 0xFFFFu, ry0 = ei.x >> 16, rx1 = ei.y & 0xFFFFu, ry1 = ei.y >> 16;
+        if (rx1 >= rx0 && ry1 >= ry0) {
+            w = rx1 - rx0 + 1u;
+            cnt = w * (ry1 - ry0 + 1u);
+            // Tight binning: only the rect tiles whose mask bit is set (an
+            // all-ones mask means every tile; project wrote the same count).
+            mask = (uint64_t)ei.z | ((uint64_t)ei.w << 32);
+            if (mask != ~0ull) {
                 masked = true;
                 cnt = (uint32_t)__popcll(mask);
             }
-            x0 = r.x;
-            y0 = r.y;
+            x0 = rx0;
+            y0 = ry0;
             base = offsets_sorted[gid];
             const CamParams& cam = cams[item / items_per_view];
             tiles_x = cam.tiles_x;
@@ -304,16 +305,15 @@ hipError_t launch_tile_ranges(hipStream_t st, const uint64_t* keys, const uint32
     return hipGetLastError();
 }
 
-hipError_t launch_emit_tile_keys(hipStream_t st, uint32_t n_items, const uint32_t* order,
-                                 const uint4* rects, const uint32_t* offsets_sorted,
+hipError_t launch_emit_tile_keys(hipStream_t st, uint32_t n_items, const uint32_t* order, bool packed,
+                                 const uint4* emit_info, const uint32_t* offsets_sorted,
                                  const CamParams* cams_dev, uint32_t items_per_view, uint32_t* tkeys,
                                  uint32_t* payloads, uint32_t max_entries, const uint32_t* total,
-                                 uint32_t* sort_count, uint32_t* overflow_flag,
-                                 const uint64_t* tile_masks) {
+                                 uint32_t* sort_count, uint32_t* overflow_flag) {
     uint32_t blocks = div_up(n_items > 0 ? n_items : 1, 256);
-    hipLaunchKernelGGL(emit_tile_keys_kernel, dim3(blocks), dim3(256), 0, st, n_items, order, rects,
-                       offsets_sorted, cams_dev, items_per_view > 0 ? items_per_view : 1, tkeys,
-                       payloads, max_entries, total, sort_count, overflow_flag, tile_masks);
+    hipLaunchKernelGGL(emit_tile_keys_kernel, dim3(blocks), dim3(256), 0, st, n_items, order, packed,
+                       emit_info, offsets_sorted, cams_dev, items_per_view > 0 ? items_per_view : 1, tkeys,
+                       payloads, max_entries, total, sort_count, overflow_flag);
     return hipGetLastError();
 }
 

@@ -80,6 +80,20 @@ struct __align__(16) CamParams {
     uint32_t pad1, pad2;
 };
 
+// ------------------------------------------------------------- launch shape
+
+// The dispatcher deals a grid's workgroups round-robin over the 8 XCDs, each
+// with its own L2: workgroup b runs on XCD b % 8. Blocks that share data
+// (neighbouring tiles, the views of one surfel range) should share an L2, so
+// kernels map the hardware index b to a logical block with this bijection of
+// [0, n): XCD x takes the contiguous logical range of the x-th 1/8 of n.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_swizzle(uint32_t b, uint32_t n) {
+    const uint32_t q = n / kXcds, r = n % kXcds;
+    const uint32_t x = b % kXcds, j = b / kXcds;
+    return x < r ? x * (q + 1) + j : r * (q + 1) + (x - r) * q + j;
+}
+
 // ------------------------------------------------------------- device math
 
 // The projection quantities that decide tile rects and depth keys are
@@ -464,10 +478,13 @@ hipError_t launch_project(hipStream_t st, const CamParams* cams_dev, uint32_t n_
                           uint32_t* depth_keys, uint32_t* counts, uint32_t* ranges_to_clear,
                           uint32_t num_ranges, uint32_t* presort_keys = nullptr,
                           uint32_t* presort_vals = nullptr, uint32_t* item_count = nullptr,
-                          bool clip_to_box = false, uint64_t* tile_masks = nullptr);
+                          bool clip_to_box = false, uint4* emit_info = nullptr,
+                          bool pack_counts = false);
 
 // scan.hip
 size_t scan_scratch_bytes(uint32_t capacity);
+// Exclusive scan of in[0..n) (in place allowed), of in[perm[i]] (gather),
+// or, with in == nullptr, of perm[i] & 0xFF (counts packed in the low byte).
 hipError_t launch_exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t n,
                                  uint32_t* total_out, void* scratch, const uint32_t* perm = nullptr);
 
@@ -482,12 +499,14 @@ hipError_t launch_tile_ranges(hipStream_t st, const uint64_t* keys, const uint32
                               uint32_t capacity, uint32_t* ranges);
 hipError_t launch_fill_u32(hipStream_t st, uint32_t* dst, uint32_t value, size_t count);
 hipError_t launch_count_tiles(hipStream_t st, uint32_t n, const uint4* rects, uint32_t* counts);
-hipError_t launch_emit_tile_keys(hipStream_t st, uint32_t n_items, const uint32_t* order,
-                                 const uint4* rects, const uint32_t* offsets_sorted,
+// Depth-ordered emission: order[k] is the k-th item in depth order (shifted
+// left by 8 over its tile count when packed), emit_info the per-item record
+// written by launch_project.
+hipError_t launch_emit_tile_keys(hipStream_t st, uint32_t n_items, const uint32_t* order, bool packed,
+                                 const uint4* emit_info, const uint32_t* offsets_sorted,
                                  const CamParams* cams_dev, uint32_t items_per_view, uint32_t* tkeys,
                                  uint32_t* payloads, uint32_t max_entries, const uint32_t* total,
-                                 uint32_t* sort_count, uint32_t* overflow_flag,
-                                 const uint64_t* tile_masks = nullptr);
+                                 uint32_t* sort_count, uint32_t* overflow_flag);
 // Longest-list-first tile order: keys[t] = 0xFFFF - min(list length, 0xFFFF),
 // vals[t] = t for the total_tiles tiles, *count = total_tiles and the
 // 2 kTileQueues cursors = 0 (sort the pairs by 16 key bits, stably, to get

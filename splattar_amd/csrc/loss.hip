@@ -49,7 +49,10 @@ __global__ __launch_bounds__(256) void ssim_fwd_kernel(uint32_t W, uint32_t H, f
     __shared__ float hb[5][kS][kPh];
     __shared__ double s_red[2][4];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    const int x0 = (int)blockIdx.x * kT - kH, y0 = (int)blockIdx.y * kT - kH;
+    // XCD-contiguous tile order: neighbouring tiles (shared halos) share an L2.
+    const uint32_t lb = xcd_swizzle(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int bx = (int)(lb % gridDim.x), by = (int)(lb / gridDim.x);
+    const int x0 = bx * kT - kH, y0 = by * kT - kH;
     for (int k = threadIdx.x; k < kS * kS; k += 256) {
         const int r = k / kS, c = k - r * kS;
         const int gx = x0 + c, gy = y0 + r;
@@ -66,7 +69,7 @@ __global__ __launch_bounds__(256) void ssim_fwd_kernel(uint32_t W, uint32_t H, f
         sy[2][r][c] = b.z;
     }
     __syncthreads();
-    const int gx = (int)blockIdx.x * kT + tx, gy = (int)blockIdx.y * kT + ty;
+    const int gx = bx * kT + tx, gy = by * kT + ty;
     const bool inside = gx < (int)W && gy < (int)H;
     const float inv_n3 = 1.0f / ((float)W * (float)H * 3.0f);
     const float dw = -lambda * dssim_scale * inv_n3;
@@ -133,7 +136,7 @@ __global__ __launch_bounds__(256) void ssim_fwd_kernel(uint32_t W, uint32_t H, f
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const size_t b = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+        const size_t b = lb;
         partials[2 * b] = ((s_red[0][0] + s_red[0][1]) + s_red[0][2]) + s_red[0][3];
         partials[2 * b + 1] = ((s_red[1][0] + s_red[1][1]) + s_red[1][2]) + s_red[1][3];
     }
@@ -152,9 +155,11 @@ __global__ __launch_bounds__(256) void ssim_bwd_kernel(uint32_t W, uint32_t H, f
     __shared__ float sc[3][kS][kPin];
     __shared__ float hb[3][kS][kPh];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    const int x0 = (int)blockIdx.x * kT - kH, y0 = (int)blockIdx.y * kT - kH;
+    const uint32_t lb = xcd_swizzle(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int bx = (int)(lb % gridDim.x), by = (int)(lb / gridDim.x);
+    const int x0 = bx * kT - kH, y0 = by * kT - kH;
     const size_t P = (size_t)W * H;
-    const int gx = (int)blockIdx.x * kT + tx, gy = (int)blockIdx.y * kT + ty;
+    const int gx = bx * kT + tx, gy = by * kT + ty;
     const bool inside = gx < (int)W && gy < (int)H;
     float b[3][3];  // [channel][map]
     for (int ch = 0; ch < 3; ++ch) {

@@ -22,13 +22,14 @@ __global__ __launch_bounds__(256) void project_kernel(
     uint4* __restrict__ rects, uint32_t* __restrict__ depth_keys, uint32_t* __restrict__ counts,
     uint32_t* __restrict__ ranges_to_clear, uint32_t num_ranges, uint32_t* __restrict__ presort_keys,
     uint32_t* __restrict__ presort_vals, uint32_t* __restrict__ item_count, bool clip_to_box,
-    uint64_t* __restrict__ tile_masks) {
-    // Views vary fastest over the blocks, so the V blocks that read the same
-    // 256 surfels run back to back and the later ones find the parameters in
-    // cache.
-    const uint32_t view = blockIdx.x % n_views;
+    uint4* __restrict__ emit_info, bool pack_counts) {
+    // Views vary fastest over the logical blocks and each XCD takes a
+    // contiguous range of them, so the V blocks that read the same 256
+    // surfels run on one XCD and the later ones find the parameters in its L2.
+    const uint32_t lb = xcd_swizzle(blockIdx.x, gridDim.x);
+    const uint32_t view = lb % n_views;
     const uint32_t sblocks = gridDim.x / n_views;
-    const uint32_t i = (blockIdx.x / n_views) * blockDim.x + threadIdx.x;
+    const uint32_t i = (lb / n_views) * blockDim.x + threadIdx.x;
     // The tile-range table of the following binning stage starts cleared
     // (binning.rs:228); the first view's blocks do it (stream order makes it
     // visible to tile_ranges).
@@ -39,7 +40,22 @@ __global__ __launch_bounds__(256) void project_kernel(
     if (i >= n) return;
     const CamParams cam = cams[view];
     const uint32_t g = view * n + i;
-    if (presort_vals) presort_vals[g] = g;
+    // Per-item binning outputs, written on every path. The presort payload
+    // carries the tile count in its low 8 bits when pack_counts (a rect spans
+    // at most 9 x 9 tiles), so the scan reads counts in depth order without a
+    // gather. emit_info = (x0 | y0 << 16, x1 | y1 << 16, mask lo, mask hi):
+    // the rect in 16-bit tile coordinates and the mask of its tiles the
+    // footprint reaches (all ones = every tile).
+    auto finish = [&](uint4 rect, uint32_t key, uint32_t count, uint64_t mask) {
+        rects[g] = rect;
+        depth_keys[g] = key;
+        counts[g] = count;
+        if (presort_keys) presort_keys[g] = key;
+        if (presort_vals) presort_vals[g] = pack_counts ? (g << 8) | count : g;
+        if (emit_info)
+            emit_info[g] = make_uint4(rect.x | (rect.y << 16), rect.z | (rect.w << 16), (uint32_t)mask,
+                                      (uint32_t)(mask >> 32));
+    };
 
     const float4 p = pos[i];
     const float2 s = scales[i];
@@ -48,10 +64,7 @@ __global__ __launch_bounds__(256) void project_kernel(
     const float depth = -f.cz_;
     uint4 rect = make_uint4(1u, 1u, 0u, 0u);
     if (!(depth >= kNearDepth)) {  // near cull (surfel_prep_fwd.wgsl:171-176); NaN culls too
-        rects[g] = rect;
-        depth_keys[g] = 0u;
-        counts[g] = 0u;
-        if (presort_keys) presort_keys[g] = 0u;
+        finish(rect, 0u, 0u, ~0ull);
         return;
     }
     const RectOut r = tile_rect(cam, f, depth);
@@ -71,6 +84,7 @@ __global__ __launch_bounds__(256) void project_kernel(
     rec.col_op.w = opac[i];
 
     uint32_t count = 0u;
+    uint64_t mask = ~0ull;
     if (r.on_screen) {
         rect = make_uint4(r.x0, r.y0, r.x1, r.y1);
         record_footprint(rec, cam);
@@ -80,30 +94,26 @@ __global__ __launch_bounds__(256) void project_kernel(
         if (rect.x <= rect.z && rect.y <= rect.w) {
             const uint32_t w = rect.z - rect.x + 1u;
             count = w * (rect.w - rect.y + 1u);
-            if (tile_masks != nullptr) {
+            if (clip_to_box && count <= 64u) {
                 // Per-tile footprint test (box, disk, ellipse) for rects of at
                 // most 64 tiles: bit t = row-major tile t of the rect. Larger
-                // rects keep every tile (mask all ones, not consulted).
-                uint64_t m = ~0ull;
-                if (count <= 64u) {
-                    m = 0ull;
-                    for (uint32_t t = 0; t < count; ++t) {
-                        const float tx = (float)((rect.x + t % w) * kTile);
-                        const float ty = (float)((rect.y + t / w) * kTile);
-                        if (footprint_hits(rec, tx + 0.5f, ty + 0.5f, tx + 15.5f, ty + 15.5f))
-                            m |= 1ull << t;
-                    }
-                    count = (uint32_t)__popcll(m);
+                // rects keep every tile.
+                mask = 0ull;
+                for (uint32_t t = 0; t < count; ++t) {
+                    const float tx = (float)((rect.x + t % w) * kTile);
+                    const float ty = (float)((rect.y + t / w) * kTile);
+                    if (footprint_hits(rec, tx + 0.5f, ty + 0.5f, tx + 15.5f, ty + 15.5f))
+                        mask |= 1ull << t;
                 }
-                tile_masks[g] = m;
+                count = (uint32_t)__popcll(mask);
             }
         }
-        if (count == 0u) rect = make_uint4(1u, 1u, 0u, 0u);
+        if (count == 0u) {
+            rect = make_uint4(1u, 1u, 0u, 0u);
+            mask = ~0ull;
+        }
     }
-    rects[g] = rect;
-    depth_keys[g] = __float_as_uint(depth);
-    if (presort_keys) presort_keys[g] = __float_as_uint(depth);
-    counts[g] = count;
+    finish(rect, __float_as_uint(depth), count, mask);
     if (count == 0u) return;  // no entry will ever read the record
 
     // View-dependent colour (surfel_prep_fwd.wgsl:113-147, 183-186).
@@ -151,14 +161,14 @@ hipError_t launch_project(hipStream_t st, const CamParams* cams_dev, uint32_t n_
                           const float* opac, const float* sh, Record* recs, uint4* rects,
                           uint32_t* depth_keys, uint32_t* counts, uint32_t* ranges_to_clear,
                           uint32_t num_ranges, uint32_t* presort_keys, uint32_t* presort_vals,
-                          uint32_t* item_count, bool clip_to_box, uint64_t* tile_masks) {
+                          uint32_t* item_count, bool clip_to_box, uint4* emit_info, bool pack_counts) {
     if (n == 0 && num_ranges == 0 && item_count == nullptr) return hipSuccess;
     if (n_views == 0) return hipSuccess;
     const uint32_t blocks = div_up(n > num_ranges ? (n ? n : 1) : num_ranges, 256);
     dim3 grid(blocks * n_views);
     hipLaunchKernelGGL(project_kernel, grid, dim3(256), 0, st, cams_dev, n_views, n, pos, scales, quats, opac,
                        sh, recs, rects, depth_keys, counts, ranges_to_clear, num_ranges, presort_keys,
-                       presort_vals, item_count, clip_to_box, tile_masks);
+                       presort_vals, item_count, clip_to_box, emit_info, pack_counts);
     return hipGetLastError();
 }
 

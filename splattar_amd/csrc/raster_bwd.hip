@@ -121,7 +121,7 @@ __global__ __launch_bounds__(256) void raster_bwd_kernel(const CamParams* __rest
             pos = s_next;
             __syncthreads();  // every thread has pos before s_next is rewritten
         } else {
-            pos = iter == 0 ? blockIdx.x : total_tiles;
+            pos = iter == 0 ? xcd_swizzle(blockIdx.x, gridDim.x) : total_tiles;
         }
         if (pos >= total_tiles) break;
         const uint32_t gtile = queue.order != nullptr ? queue.order[pos] : pos;

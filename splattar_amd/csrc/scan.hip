@@ -53,7 +53,8 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v, int lane) {
 // `in` and `out` may alias (in-place scan, as PrefixSum::encode): every block
 // reads its whole partition into registers before writing it back. With
 // `perm`, element i of the scanned sequence is in[perm[i]] (a gather; then
-// in and out must not alias).
+// in and out must not alias); with `perm` and in == nullptr it is
+// perm[i] & 0xFF (values packed into the low byte of a sorted payload).
 __global__ __launch_bounds__(kScanThreads) void scan_kernel(const uint32_t* in, uint32_t* out,
                                                             uint32_t n,
                                                             uint32_t* __restrict__ total,
@@ -76,22 +77,25 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(const uint32_t* in, 
     const uint32_t first = part * kScanPart + (uint32_t)tid * kScanItems;
     uint32_t v[kScanItems];
     uint32_t sum = 0;
-    if (perm == nullptr && first + kScanItems <= n) {
-        // Whole run in range: 16-B loads (first is a multiple of 16 items).
-        const uint4* in4 = reinterpret_cast<const uint4*>(in + first);
+    if ((perm == nullptr || in == nullptr) && first + kScanItems <= n) {
+        // Whole run in range, no gather: 16-B loads (first is a multiple of
+        // 16 items).
+        const uint32_t m = in != nullptr ? 0xFFFFFFFFu : 0xFFu;
+        const uint4* in4 = reinterpret_cast<const uint4*>((in != nullptr ? in : perm) + first);
 #pragma unroll
         for (int k = 0; k < kScanItems / 4; ++k) {
             const uint4 q = in4[k];
-            v[4 * k + 0] = q.x;
-            v[4 * k + 1] = q.y;
-            v[4 * k + 2] = q.z;
-            v[4 * k + 3] = q.w;
+            v[4 * k + 0] = q.x & m;
+            v[4 * k + 1] = q.y & m;
+            v[4 * k + 2] = q.z & m;
+            v[4 * k + 3] = q.w & m;
         }
     } else {
 #pragma unroll
         for (int k = 0; k < kScanItems; ++k) {
             const uint32_t idx = first + k;
-            v[k] = idx < n ? in[perm ? perm[idx] : idx] : 0u;
+            // in == nullptr: the values are the low 8 bits of perm (packed).
+            v[k] = idx >= n ? 0u : in == nullptr ? (perm[idx] & 0xFFu) : in[perm ? perm[idx] : idx];
         }
     }
 #pragma unroll
