@@ -10,11 +10,17 @@
 //     Nt, px, py, depth, colour, opacity, normal); the linear chain to
 //     camera-space (c, tu, tv) happens once per surfel in geom_bwd instead of
 //     four triple-product gradients per pixel-hit;
-//   * per record, each wave64 reduces its 64 pixels' contributions with a
+//   * a workgroup is two wave64s, each owning an 8x16 half of the 16x16
+//     tile with two pixels per lane (rows y and y + 8); records are staged
+//     128 at a time, split into per-wave lists that carry which of the
+//     wave's two 8x8 blocks the record's footprint reaches, so a record is
+//     evaluated only where it can contribute and its two pixels' terms are
+//     summed in registers before one reduction;
+//   * per record, each wave64 reduces its lanes' contributions with a
 //     transposed butterfly (17 lane exchanges for 16 values instead of 96,
 //     on permlane swaps and DPP rather than LDS permutes), adds them into an
-//     LDS slot (ds_add_f32), and the batch's slots are
-//     flushed with hardware global FP32 atomics (no CAS loops).
+//     LDS slot (ds_add_f32), and the batch's slots are flushed with hardware
+//     global FP32 atomics (no CAS loops).
 // AUX = true adds the depth / normal image losses and the in-walk depth
 // distortion loss (backward.rs:126-153).
 
@@ -24,7 +30,8 @@ namespace gs {
 
 namespace {
 
-constexpr int kBatch = 256;
+constexpr int kThreads = 128;  // two waves: the left and right 8x16 halves of a tile
+constexpr int kBatch = 128;    // records staged per batch, one per thread
 
 __device__ __forceinline__ uint32_t view_of_tile(const CamParams* cams, uint32_t n_views, uint32_t t) {
     uint32_t v = 0;
@@ -93,23 +100,152 @@ __device__ __forceinline__ float wave_reduce_transpose(float (&v)[NV], int lane)
     return transpose_reduce<NV, 32, NV>(v, lane);
 }
 
+// Per-pixel state of the backward walk.
+struct PixState {
+    float sx, sy, pxc, pyc;
+    float dlr, dlg, dlb, dld, dnx, dny, dnz;
+    float T, w_total, wt_total;
+    float suf_r, suf_g, suf_b, suf_d, suf_nx, suf_ny, suf_nz, suf_w, suf_cw;
+    uint32_t last;
+};
+
+// Loads a pixel's incoming gradients and forward state; last = 0 when the
+// pixel is outside the image or carries no loss gradient (nothing to walk).
 template <bool AUX>
-__global__ __launch_bounds__(256) void raster_bwd_kernel(const CamParams* __restrict__ cams,
-                                                         uint32_t n_views,
-                                                         const uint32_t* __restrict__ ranges,
-                                                         const uint32_t* __restrict__ payloads,
-                                                         const Record* __restrict__ recs, BwdIn in,
-                                                         float* __restrict__ grad_coef, bool cull,
-                                                         uint32_t total_tiles, TileQueue queue) {
+__device__ __forceinline__ void load_pixel(PixState& p, const CamParams& cam, int px, int py,
+                                           const BwdIn& in, float lambda) {
+    pixel_s(cam, px, py, p.sx, p.sy);
+    p.pxc = (float)px + 0.5f;
+    p.pyc = (float)py + 0.5f;
+    p.dlr = p.dlg = p.dlb = p.dld = p.dnx = p.dny = p.dnz = 0.f;
+    p.T = 1.0f;
+    p.w_total = p.wt_total = 0.f;
+    p.suf_r = p.suf_g = p.suf_b = p.suf_d = p.suf_nx = p.suf_ny = p.suf_nz = p.suf_w = p.suf_cw = 0.f;
+    p.last = 0;
+    const bool inside = px < (int)cam.width && py < (int)cam.height;
+    if (!inside) return;
+    const size_t pi = (size_t)cam.pix_base + (size_t)py * cam.width + (size_t)px;
+    const float4 dl = in.dl_dcolor[pi];
+    p.dlr = dl.x;
+    p.dlg = dl.y;
+    p.dlb = dl.z;
+    const float4 aux = in.out_aux[pi];
+    p.T = aux.x;
+    uint32_t last = in.ncontrib[pi];
+    bool has_loss = p.dlr != 0.f || p.dlg != 0.f || p.dlb != 0.f;
+    if (AUX) {
+        p.dld = dl.w;
+        if (in.dl_dnormal) {
+            const float4 dn = in.dl_dnormal[pi];
+            p.dnx = dn.x;
+            p.dny = dn.y;
+            p.dnz = dn.z;
+        }
+        p.w_total = in.out_color[pi].w;
+        p.wt_total = aux.y;
+        has_loss = has_loss || p.dld != 0.f || p.dnx != 0.f || p.dny != 0.f || p.dnz != 0.f || lambda != 0.f;
+    }
+    p.last = has_loss ? last : 0u;
+}
+
+template <bool AUX, bool ACC>
+__device__ __forceinline__ void put(float& dst, float v) {
+    if constexpr (ACC) dst += v;
+    else dst = v;
+}
+
+// Writes (ACC = false) or adds (ACC = true) one pixel-record hit's gradient
+// terms into g / g2 and steps the pixel's walk, given the hit evaluation h and
+// whether this lane contributes (act). Branch-free: a lane that does not
+// contribute gets w = 0 and 1/(1-alpha) = 1 exactly, so its running sums and
+// T are unchanged, and its gradient terms are selected to 0 (no NaN from
+// 0 * inf); a wave with no contributing lane need not call it at all.
+template <bool AUX, bool ACC>
+__device__ __forceinline__ void add_hit(PixState& p, const Record& r, const HitEval& h, bool act,
+                                        float lambda, float (&g)[16], float (&g2)[4]) {
+    const float alpha = act ? h.alpha : 0.0f;
+    const float inv_one_m = rcp_refined(1.0f - alpha);
+    const float t_before = p.T * inv_one_m;
+    const float w = t_before * alpha;
+    const float t_hit = h.t;
+    put<AUX, ACC>(g[kGCol + 0], p.dlr * w);
+    put<AUX, ACC>(g[kGCol + 1], p.dlg * w);
+    put<AUX, ACC>(g[kGCol + 2], p.dlb * w);
+    float dl_dt = 0.0f;
+    float dl_dalpha = (p.dlr * r.col_op.x + p.dlg * r.col_op.y + p.dlb * r.col_op.z) * t_before -
+                      (p.dlr * p.suf_r + p.dlg * p.suf_g + p.dlb * p.suf_b) * inv_one_m;
+    p.suf_r += r.col_op.x * w;
+    p.suf_g += r.col_op.y * w;
+    p.suf_b += r.col_op.z * w;
+    if (AUX) {
+        float dt_dist = 0.0f, coef = 0.0f;
+        if (lambda != 0.0f) {
+            const float a_pre = p.w_total - p.suf_w - w;
+            const float b_pre = p.wt_total - p.suf_d - w * t_hit;
+            dt_dist = lambda * 2.0f * w * (a_pre - p.suf_w);
+            coef = lambda * (2.0f * (t_hit * a_pre - b_pre) + 2.0f * (p.suf_d - t_hit * p.suf_w));
+        }
+        put<AUX, ACC>(g2[kGNrm - 16 + 0], p.dnx * w);
+        put<AUX, ACC>(g2[kGNrm - 16 + 1], p.dny * w);
+        put<AUX, ACC>(g2[kGNrm - 16 + 2], p.dnz * w);
+        dl_dt = p.dld * w + dt_dist;
+        dl_dalpha += p.dld * (t_hit * t_before - p.suf_d * inv_one_m) +
+                     (p.dnx * r.nrm_dep.x + p.dny * r.nrm_dep.y + p.dnz * r.nrm_dep.z) * t_before -
+                     (p.dnx * p.suf_nx + p.dny * p.suf_ny + p.dnz * p.suf_nz) * inv_one_m +
+                     coef * t_before - p.suf_cw * inv_one_m;
+        p.suf_d += t_hit * w;
+        p.suf_nx += r.nrm_dep.x * w;
+        p.suf_ny += r.nrm_dep.y * w;
+        p.suf_nz += r.nrm_dep.z * w;
+        p.suf_w += w;
+        p.suf_cw += coef * w;
+    }
+    p.T = t_before;
+    // Geometry terms exist only for contributing, unclamped hits.
+    const bool geo = act && !h.clamped;
+    put<AUX, ACC>(g[kGOp], geo ? dl_dalpha * h.ghat : 0.0f);
+    const float gd = geo ? h.ghat * (dl_dalpha * r.col_op.w) : 0.0f;  // ghat dL/dghat
+    const float dl_du = -h.u * gd;
+    const float dl_dv = -h.v * gd;
+    const float dnu = h.ray ? dl_du * h.inv_d : 0.0f;
+    const float dnv = h.ray ? dl_dv * h.inv_d : 0.0f;
+    const float dD = h.ray ? -(h.u * dl_du + h.v * dl_dv + t_hit * dl_dt) * h.inv_d : 0.0f;
+    put<AUX, ACC>(g[kGA + 0], dnu * p.sx);
+    put<AUX, ACC>(g[kGA + 1], dnu * p.sy);
+    put<AUX, ACC>(g[kGA + 2], dnu);
+    put<AUX, ACC>(g[kGB + 0], dnv * p.sx);
+    put<AUX, ACC>(g[kGB + 1], dnv * p.sy);
+    put<AUX, ACC>(g[kGB + 2], dnv);
+    put<AUX, ACC>(g[kGM + 0], dD * p.sx);
+    put<AUX, ACC>(g[kGM + 1], dD * p.sy);
+    put<AUX, ACC>(g[kGM + 2], dD);
+    const float gs = h.ray ? 0.0f : 2.0f * gd;
+    put<AUX, ACC>(g[kGPx], gs * (p.pxc - r.a_px.w));
+    put<AUX, ACC>(g[kGPy], gs * (p.pyc - r.b_py.w));
+    if (AUX) {
+        put<AUX, ACC>(g[kGNt], (geo && h.ray) ? dl_dt * h.inv_d : 0.0f);
+        put<AUX, ACC>(g2[kGDep - 16], (geo && !h.ray) ? dl_dt : 0.0f);
+    }
+}
+
+template <bool AUX>
+__global__ __launch_bounds__(kThreads) void raster_bwd_kernel(const CamParams* __restrict__ cams,
+                                                              uint32_t n_views,
+                                                              const uint32_t* __restrict__ ranges,
+                                                              const uint32_t* __restrict__ payloads,
+                                                              const Record* __restrict__ recs, BwdIn in,
+                                                              float* __restrict__ grad_coef, bool cull,
+                                                              uint32_t total_tiles, TileQueue queue) {
     constexpr int NS = AUX ? 20 : 16;  // LDS slots per record
     __shared__ float4 s_m[kBatch], s_a[kBatch], s_b[kBatch], s_c[kBatch], s_n[kBatch];
     __shared__ float s_lthr[kBatch];
-    __shared__ uint16_t s_list[4][kBatch];
-    __shared__ uint32_t s_qcnt[4][4];
+    __shared__ uint16_t s_list[2][kBatch];  // record | top block << 8 | bottom block << 9
+    __shared__ uint32_t s_qcnt[2][2];       // [staging wave][list wave]
     __shared__ float s_grad[kBatch * NS];
     __shared__ uint32_t s_view, s_maxlast, s_next;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (uint32_t iter = 0;; ++iter) {
         uint32_t pos;
         if (queue.cursor != nullptr) {
@@ -135,59 +271,27 @@ __global__ __launch_bounds__(256) void raster_bwd_kernel(const CamParams* __rest
         const CamParams& cam = cams[s_view];
         const uint32_t local = gtile - cam.tile_base;
         const uint32_t tx = local % cam.tiles_x, ty = local / cam.tiles_x;
-        const int px = (int)(tx * kTile) + (wave & 1) * 8 + (lane & 7);
-        const int py = (int)(ty * kTile) + (wave >> 1) * 8 + (lane >> 3);
-        const bool inside = px < (int)cam.width && py < (int)cam.height;
-        float sx, sy;
-        pixel_s(cam, px, py, sx, sy);
-        const float pxc = (float)px + 0.5f, pyc = (float)py + 0.5f;
-        const size_t pi = (size_t)cam.pix_base + (size_t)py * cam.width + (size_t)px;
-
-        float dlr = 0.f, dlg = 0.f, dlb = 0.f, dld = 0.f, dnx = 0.f, dny = 0.f, dnz = 0.f;
-        float T = 1.0f, w_total = 0.f, wt_total = 0.f;
-        uint32_t last = 0;
-        if (inside) {
-            const float4 dl = in.dl_dcolor[pi];
-            dlr = dl.x;
-            dlg = dl.y;
-            dlb = dl.z;
-            const float4 aux = in.out_aux[pi];
-            T = aux.x;
-            last = in.ncontrib[pi];
-            if (AUX) {
-                dld = dl.w;
-                if (in.dl_dnormal) {
-                    const float4 dn = in.dl_dnormal[pi];
-                    dnx = dn.x;
-                    dny = dn.y;
-                    dnz = dn.z;
-                }
-                w_total = in.out_color[pi].w;
-                wt_total = aux.y;
-            }
-        }
         const float lambda = AUX ? cam.lambda_dist : 0.0f;
-        bool has_loss = inside && (dlr != 0.f || dlg != 0.f || dlb != 0.f);
-        if (AUX) has_loss = inside && (has_loss || dld != 0.f || dnx != 0.f || dny != 0.f ||
-                                       dnz != 0.f || lambda != 0.f);
-        if (!has_loss) last = 0;
-        if (last) atomicMax(&s_maxlast, last);
+        // Wave w owns tile columns 8w..8w+7; a lane owns rows y and y + 8,
+        // i.e. one pixel in 8x8 block w (top) and one in block w + 2 (bottom).
+        const int px = (int)(tx * kTile) + wave * 8 + (lane & 7);
+        const int py = (int)(ty * kTile) + (lane >> 3);
+        PixState pa, pb;
+        load_pixel<AUX>(pa, cam, px, py, in, lambda);
+        load_pixel<AUX>(pb, cam, px, py + 8, in, lambda);
+        const uint32_t mylast = max(pa.last, pb.last);
+        if (mylast) atomicMax(&s_maxlast, mylast);
         __syncthreads();
         const uint32_t maxlast = s_maxlast;
         if (maxlast == 0) continue;
 
-        float suf_r = 0.f, suf_g = 0.f, suf_b = 0.f;
-        float suf_d = 0.f, suf_nx = 0.f, suf_ny = 0.f, suf_nz = 0.f, suf_w = 0.f, suf_cw = 0.f;
-
-        const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
         const int nbatch = (int)((maxlast + kBatch - 1) / kBatch);
         for (int b = nbatch - 1; b >= 0; --b) {
             const uint32_t base = start + (uint32_t)b * kBatch;
             const uint32_t cnt = min(min((uint32_t)kBatch, end - base), maxlast - (uint32_t)b * kBatch);
             uint32_t qmask = 0;
             if ((uint32_t)tid < cnt) {
-                const uint32_t pay = payloads[base + tid];
-                Record r = recs[pay];
+                Record r = recs[payloads[base + tid]];
                 s_m[tid] = r.m_nt;
                 s_a[tid] = r.a_px;
                 s_b[tid] = r.b_py;
@@ -196,110 +300,77 @@ __global__ __launch_bounds__(256) void raster_bwd_kernel(const CamParams* __rest
                 s_lthr[tid] = record_lthr(r.col_op.w);
                 qmask = cull ? quadrant_mask(r, tx, ty) : 0xFu;
             }
-            for (int k = tid; k < (int)cnt * NS; k += 256) s_grad[k] = 0.0f;
-            uint32_t qpos[4];
+            for (int k = tid; k < (int)cnt * NS; k += kThreads) s_grad[k] = 0.0f;
+            // Order-preserving per-wave lists: wave w takes the records that
+            // reach block w or block w + 2.
+            uint32_t qpos[2], memb[2];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint64_t bal = __ballot((qmask >> q) & 1u);
-                qpos[q] = (uint32_t)__popcll(bal & lt_mask);
-                if (lane == 0) s_qcnt[wave][q] = (uint32_t)__popcll(bal);
+            for (int w = 0; w < 2; ++w) {
+                const uint32_t top = (qmask >> w) & 1u, bot = (qmask >> (w + 2)) & 1u;
+                memb[w] = top | (bot << 1);
+                const uint64_t bal = __ballot(memb[w] != 0u);
+                qpos[w] = (uint32_t)__popcll(bal & lt_mask);
+                if (lane == 0) s_qcnt[wave][w] = (uint32_t)__popcll(bal);
             }
             __syncthreads();
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if ((qmask >> q) & 1u) {
-                    uint32_t off = 0;
-                    for (int w = 0; w < wave; ++w) off += s_qcnt[w][q];
-                    s_list[q][off + qpos[q]] = (uint16_t)tid;
+            for (int w = 0; w < 2; ++w) {
+                if (memb[w] != 0u) {
+                    const uint32_t off = wave == 1 ? s_qcnt[0][w] : 0u;
+                    s_list[w][off + qpos[w]] = (uint16_t)(tid | (memb[w] << 8));
                 }
             }
-            const int my_count = (int)(s_qcnt[0][wave] + s_qcnt[1][wave] + s_qcnt[2][wave] + s_qcnt[3][wave]);
+            const int my_count = (int)(s_qcnt[0][wave] + s_qcnt[1][wave]);
             __syncthreads();
 
             for (int kk = my_count - 1; kk >= 0; --kk) {
-                const int j = s_list[wave][kk];
-                const uint32_t rel = (uint32_t)b * kBatch + (uint32_t)j;
-                if (!__any(rel < last)) continue;
+                // The entry is the same for every lane: make it scalar, so the
+                // block branches below are uniform.
+                const uint32_t e = __builtin_amdgcn_readfirstlane((uint32_t)s_list[wave][kk]);
+                const uint32_t j = e & 0xFFu;
+                const bool top = (e >> 8) & 1u, bot = (e >> 9) & 1u;
+                const uint32_t rel = (uint32_t)b * kBatch + j;
+                // A block is walked only while some pixel of it is at or before
+                // its last contributor (a pixel past it changes nothing).
+                const bool need_a = top && __any(rel < pa.last);
+                const bool need_b = bot && __any(rel < pb.last);
+                if (!need_a && !need_b) continue;
                 Record r;
                 r.m_nt = s_m[j];
                 r.a_px = s_a[j];
                 r.b_py = s_b[j];
                 r.col_op = s_c[j];
                 r.nrm_dep = s_n[j];
-                // Branch-free body: a lane that does not contribute gets w = 0 and
-                // 1/(1-alpha) = 1 exactly, so its running sums and T are unchanged,
-                // and its gradient terms are selected to 0 (no NaN from 0 * inf).
-                const HitEval h = eval_hit(r, sx, sy, pxc, pyc, s_lthr[j]);
-                const bool act = rel < last && h.alpha > 0.0f;
-                const float alpha = act ? h.alpha : 0.0f;
-                const float one_m = 1.0f - alpha;
-                const float inv_one_m = rcp_refined(one_m);
-                const float t_before = T * inv_one_m;
-                const float w = t_before * alpha;
-                const float t_hit = h.t;
+                const float lthr = s_lthr[j];
+                // Evaluate first; the gradient terms and the reduction run only
+                // for blocks with a contributing lane (for the others the walk
+                // state would not change).
+                HitEval ha, hb;
+                bool act_a = false, act_b = false;
+                if (need_a) {
+                    ha = eval_hit(r, pa.sx, pa.sy, pa.pxc, pa.pyc, lthr);
+                    act_a = rel < pa.last && ha.alpha > 0.0f;
+                }
+                if (need_b) {
+                    hb = eval_hit(r, pb.sx, pb.sy, pb.pxc, pb.pyc, lthr);
+                    act_b = rel < pb.last && hb.alpha > 0.0f;
+                }
+                const bool go_a = need_a && __any(act_a);
+                const bool go_b = need_b && __any(act_b);
+                if (!go_a && !go_b) continue;
                 float g[16];
                 float g2[4];
-                g[kGCol + 0] = dlr * w;
-                g[kGCol + 1] = dlg * w;
-                g[kGCol + 2] = dlb * w;
-                float dl_dt = 0.0f;
-                float dl_dalpha = (dlr * r.col_op.x + dlg * r.col_op.y + dlb * r.col_op.z) * t_before -
-                                  (dlr * suf_r + dlg * suf_g + dlb * suf_b) * inv_one_m;
-                suf_r += r.col_op.x * w;
-                suf_g += r.col_op.y * w;
-                suf_b += r.col_op.z * w;
-                if (AUX) {
-                    float dt_dist = 0.0f, coef = 0.0f;
-                    if (lambda != 0.0f) {
-                        const float a_pre = w_total - suf_w - w;
-                        const float b_pre = wt_total - suf_d - w * t_hit;
-                        dt_dist = lambda * 2.0f * w * (a_pre - suf_w);
-                        coef = lambda * (2.0f * (t_hit * a_pre - b_pre) + 2.0f * (suf_d - t_hit * suf_w));
-                    }
-                    g2[kGNrm - 16 + 0] = dnx * w;
-                    g2[kGNrm - 16 + 1] = dny * w;
-                    g2[kGNrm - 16 + 2] = dnz * w;
-                    dl_dt = dld * w + dt_dist;
-                    dl_dalpha += dld * (t_hit * t_before - suf_d * inv_one_m) +
-                                 (dnx * r.nrm_dep.x + dny * r.nrm_dep.y + dnz * r.nrm_dep.z) * t_before -
-                                 (dnx * suf_nx + dny * suf_ny + dnz * suf_nz) * inv_one_m +
-                                 coef * t_before - suf_cw * inv_one_m;
-                    suf_d += t_hit * w;
-                    suf_nx += r.nrm_dep.x * w;
-                    suf_ny += r.nrm_dep.y * w;
-                    suf_nz += r.nrm_dep.z * w;
-                    suf_w += w;
-                    suf_cw += coef * w;
+                g[kGNt] = 0.0f;  // written only by the AUX terms
+#pragma unroll
+                for (int k = 0; k < 4; ++k) g2[k] = 0.0f;
+                if (go_a && go_b) {
+                    add_hit<AUX, false>(pa, r, ha, act_a, lambda, g, g2);
+                    add_hit<AUX, true>(pb, r, hb, act_b, lambda, g, g2);
+                } else if (go_a) {
+                    add_hit<AUX, false>(pa, r, ha, act_a, lambda, g, g2);
+                } else {
+                    add_hit<AUX, false>(pb, r, hb, act_b, lambda, g, g2);
                 }
-                T = t_before;
-                // Geometry terms exist only for contributing, unclamped hits.
-                const bool geo = act && !h.clamped;
-                g[kGOp] = geo ? dl_dalpha * h.ghat : 0.0f;
-                const float gd = geo ? h.ghat * (dl_dalpha * r.col_op.w) : 0.0f;  // ghat dL/dghat
-                const float dl_du = -h.u * gd;
-                const float dl_dv = -h.v * gd;
-                const float dnu = h.ray ? dl_du * h.inv_d : 0.0f;
-                const float dnv = h.ray ? dl_dv * h.inv_d : 0.0f;
-                const float dD = h.ray ? -(h.u * dl_du + h.v * dl_dv + t_hit * dl_dt) * h.inv_d : 0.0f;
-                g[kGA + 0] = dnu * sx;
-                g[kGA + 1] = dnu * sy;
-                g[kGA + 2] = dnu;
-                g[kGB + 0] = dnv * sx;
-                g[kGB + 1] = dnv * sy;
-                g[kGB + 2] = dnv;
-                g[kGM + 0] = dD * sx;
-                g[kGM + 1] = dD * sy;
-                g[kGM + 2] = dD;
-                const float gs = h.ray ? 0.0f : 2.0f * gd;
-                g[kGPx] = gs * (pxc - r.a_px.w);
-                g[kGPy] = gs * (pyc - r.b_py.w);
-                g[kGNt] = 0.0f;
-                if (AUX) {
-                    g[kGNt] = (geo && h.ray) ? dl_dt * h.inv_d : 0.0f;
-                    g2[kGDep - 16] = (geo && !h.ray) ? dl_dt : 0.0f;
-                }
-                const bool contrib = act;
-                if (!__any(contrib)) continue;
                 const float red = wave_reduce_transpose<16>(g, lane);
                 if ((lane & 3) == 0) {
                     if (red != 0.0f) atomicAdd(&s_grad[j * NS + (lane >> 2)], red);
@@ -310,7 +381,7 @@ __global__ __launch_bounds__(256) void raster_bwd_kernel(const CamParams* __rest
                 }
             }
             __syncthreads();
-            for (int k = tid; k < (int)cnt * NS; k += 256) {
+            for (int k = tid; k < (int)cnt * NS; k += kThreads) {
                 const float v = s_grad[k];
                 if (v != 0.0f) {
                     const int e = k / NS, slot = k - e * NS;
@@ -332,10 +403,10 @@ hipError_t launch_raster_bwd(hipStream_t st, const CamParams* cams_dev, const Ca
     const uint32_t grid = queue.cursor != nullptr ? std::min(total_tiles, std::max(queue.grid_blocks, 1u))
                                                   : total_tiles;
     if (aux)
-        hipLaunchKernelGGL(raster_bwd_kernel<true>, dim3(grid), dim3(256), 0, st, cams_dev, n_views, ranges,
+        hipLaunchKernelGGL(raster_bwd_kernel<true>, dim3(grid), dim3(kThreads), 0, st, cams_dev, n_views, ranges,
                            payloads, recs, in, grad_coef, cull, total_tiles, queue);
     else
-        hipLaunchKernelGGL(raster_bwd_kernel<false>, dim3(grid), dim3(256), 0, st, cams_dev, n_views, ranges,
+        hipLaunchKernelGGL(raster_bwd_kernel<false>, dim3(grid), dim3(kThreads), 0, st, cams_dev, n_views, ranges,
                            payloads, recs, in, grad_coef, cull, total_tiles, queue);
     return hipGetLastError();
 }
