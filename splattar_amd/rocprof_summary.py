@@ -10,9 +10,10 @@ Inputs (CSV files written by rocprofv3 with `-f csv`):
 
 Output: JSON with, per (kernel, grid size): launches, mean duration (ms), the
 mean of every collected counter per dispatch, and for FETCH_SIZE /
-WRITE_SIZE the scaled HBM bytes per launch. Kernels are keyed by short name
-and total grid size, since one kernel runs at several shapes (one view per
-launch vs several).
+WRITE_SIZE the scaled HBM bytes per launch; the same over the "large"
+launches only (duration >= 0.6 x the longest of the group), which separates
+the several-view launches of a persistent kernel from its one-view launches
+at the same grid size. Kernels are keyed by short name and total grid size.
 
 Usage:
   python -m splattar_amd.rocprof_summary --trace T.csv --pmc A.csv --pmc B.csv \\
@@ -52,17 +53,27 @@ def read_trace(path):
     return out
 
 
-def read_pmc(path):
+LARGE = 0.6  # a launch is "large" at >= this fraction of its group's longest
+
+
+def read_pmc(path, large_only=False):
     """(kernel, grid) -> counter -> list of per-dispatch values."""
     per_dispatch = collections.defaultdict(dict)
     keys = {}
+    dur = {}
     for r in csv.DictReader(open(path)):
         d = r["Dispatch_Id"]
         keys[d] = (short_name(r["Kernel_Name"]), _grid(r))
+        dur[d] = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6
         per_dispatch[d][r["Counter_Name"]] = per_dispatch[d].get(r["Counter_Name"], 0.0) + float(
             r["Counter_Value"])
+    longest = collections.defaultdict(float)
+    for d, k in keys.items():
+        longest[k] = max(longest[k], dur[d])
     out = collections.defaultdict(lambda: collections.defaultdict(list))
     for d, counters in per_dispatch.items():
+        if large_only and dur[d] < LARGE * longest[keys[d]]:
+            continue
         for c, v in counters.items():
             out[keys[d]][c].append(v)
     return out
@@ -89,13 +100,15 @@ def main(argv=None):
     a = ap.parse_args(argv)
     trace = read_trace(a.trace)
     counters = collections.defaultdict(dict)
+    counters_large = collections.defaultdict(dict)
     for p in a.pmc:
         if not os.path.exists(p):
             print(f"rocprof_summary: no counter file {p}, skipped")
             continue
-        for key, cs in read_pmc(p).items():
-            for c, vals in cs.items():
-                counters[key][c] = sum(vals) / len(vals)
+        for dst, large in ((counters, False), (counters_large, True)):
+            for key, cs in read_pmc(p, large).items():
+                for c, vals in cs.items():
+                    dst[key][c] = sum(vals) / len(vals)
     fetch_scale = copy_scale(a.copy_fetch, "FETCH_SIZE") if a.copy_fetch else None
     write_scale = copy_scale(a.copy_write, "WRITE_SIZE") if a.copy_write else None
     kernels = []
@@ -109,6 +122,16 @@ def main(argv=None):
             k["hbm_read_bytes"] = cs["FETCH_SIZE"] * fetch_scale
         if write_scale and "WRITE_SIZE" in cs:
             k["hbm_write_bytes"] = cs["WRITE_SIZE"] * write_scale
+        big = [d for d in durs if d >= LARGE * max(durs)]
+        lg = {"launches": len(big), "mean_ms": sum(big) / len(big)}
+        cl = counters_large.get((name, grid), {})
+        if cl:
+            lg["counters_per_dispatch"] = cl
+        if fetch_scale and "FETCH_SIZE" in cl:
+            lg["hbm_read_bytes"] = cl["FETCH_SIZE"] * fetch_scale
+        if write_scale and "WRITE_SIZE" in cl:
+            lg["hbm_write_bytes"] = cl["WRITE_SIZE"] * write_scale
+        k["large"] = lg
         kernels.append(k)
     out = {"command": a.command, "fetch_bytes_per_unit": fetch_scale,
            "write_bytes_per_unit": write_scale, "kernels": kernels}
