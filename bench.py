@@ -94,7 +94,8 @@ def parse():
     p.add_argument("--psnr-steps", type=int, default=500,
                    help="steps of the batched-vs-plain equal-loss comparison (0 = skip)")
     p.add_argument("--schedule", default=None,
-                   help="DIGIT_BITS,BLOCKS_PER_CU: set this schedule instead of autotuning")
+                   help="DIGIT_BITS,BLOCKS_PER_CU[,TILE,FWD_BATCH,BWD_BATCH]: set this schedule "
+                        "instead of autotuning")
     p.add_argument("--cpu-crop", default="480x270")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run 5 device copies of 1 GiB (read 1 GiB + write 1 GiB each), the "
@@ -233,11 +234,15 @@ def main():
 
     # ---- batched + graph (the headline)
     tr = sa.Trainer(ctx, cfg, scene.as_init(), views)
-    schedule = {"digit_bits": a.digit_bits, "blocks_per_cu": None}
+    schedule = {"digit_bits": a.digit_bits, "blocks_per_cu": None, "tile_size": 16, "fwd_batch": 256,
+                "bwd_batch": 128}
     if a.schedule:
-        d, b = (int(x) for x in a.schedule.split(","))
-        tr.set_schedule(d, b)
-        schedule = {"digit_bits": d, "blocks_per_cu": b}
+        f = [int(x) for x in a.schedule.split(",")]
+        tr.set_schedule(f[0], f[1])
+        schedule.update(digit_bits=f[0], blocks_per_cu=f[1])
+        if len(f) == 5:
+            tr.set_tiling(f[2], f[3], f[4])
+            schedule.update(tile_size=f[2], fwd_batch=f[3], bwd_batch=f[4])
     elif not a.no_autotune:
         # Untimed: real training steps under each candidate schedule.
         t0 = time.perf_counter()
@@ -291,6 +296,7 @@ def main():
             t = sa.Trainer(ctx, cfg, scene.as_init(), views)
             if schedule["blocks_per_cu"] is not None:
                 t.set_schedule(schedule["digit_bits"], schedule["blocks_per_cu"])
+            t.set_tiling(schedule["tile_size"], schedule["fwd_batch"], schedule["bwd_batch"])
             t0 = time.perf_counter()
             run(t, mode, a.psnr_steps, 0)
             ctx.synchronize()
@@ -314,7 +320,8 @@ def main():
     value = a.steps / t_graph
     # Roofline of the dominant kernel.
     P = V * W * H
-    T = V * ((W + 15) // 16) * ((H + 15) // 16)
+    ts = schedule["tile_size"]
+    T = V * ((W + ts - 1) // ts) * ((H + ts - 1) // ts)
     import math
     tile_bits = max(1, math.ceil(math.log2(T)))
     passes = math.ceil((32 + tile_bits) / schedule["digit_bits"])
@@ -336,7 +343,9 @@ def main():
         "config": {"workload": f"C3: {n} surfels SH3 heavy-tailed scales, {V}x{W}x{H} views/step, "
                                f"L1+SSIM, Adam", "surfels": n, "views_per_step": V, "width": W,
                    "height": H, "entries_per_step": entries_step, "digit_bits": schedule["digit_bits"],
-                   "raster_blocks_per_cu": schedule["blocks_per_cu"], "binning": a.binning,
+                   "raster_blocks_per_cu": schedule["blocks_per_cu"], "tile_size": schedule["tile_size"],
+                   "fwd_batch": schedule["fwd_batch"], "bwd_batch": schedule["bwd_batch"],
+                   "binning": a.binning,
                    "parallelism": "single-gpu"},
         "plain_steps_per_s": round(1.0 / t_plain, 4),
         "batched_nograph_steps_per_s": round(1.0 / t_batched, 4),

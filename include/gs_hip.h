@@ -286,6 +286,8 @@ typedef struct gs_train_view {
     uint32_t width, height;
     gs_raster_camera camera;
     const float* target_rgba;   /* host, width*height*4 (rgb used) */
+    uint32_t num_surfels;       /* surfels [0, num_surfels) render in this view (0 = all):
+                                   frames of a growing scene; no reference counterpart */
 } gs_train_view;
 
 typedef struct gs_train_config {
@@ -310,21 +312,37 @@ gs_status gs_trainer_create(gs_ctx ctx, const gs_train_config* cfg, const gs_sce
 void gs_trainer_destroy(gs_trainer t);
 gs_status gs_trainer_step(gs_trainer t, void* stream, const uint32_t* view_ids, uint32_t n_ids,
                           uint32_t mode);
-/* Blocking. No counterpart in the reference (its digit width and dispatch
- * shape are fixed): for each candidate schedule -- radix digit width {8, 4}
- * x raster scheduling {persistent tile queue with 8 or 4 workgroups per CU,
- * one workgroup per tile} -- runs one settling batched step on `view_ids`
- * and times `reps` more one by one, over two interleaved rounds, and keeps
- * the candidate with the fastest step. These are real training steps
- * (12 (reps + 1) of them); no candidate changes the step's result. chosen
- * (optional, 2 entries) = digit bits, workgroups per CU (0 = one per tile).
- * A captured step graph is recaptured afterwards. */
+/* Blocking. No counterpart in the reference (its tile size, LDS batch,
+ * digit width and dispatch shape are fixed). Stage 1 tries every work tiling
+ * (tile_size, fwd_batch, bwd_batch) in {(16, 256, 128), (16, 128, 64),
+ * (32, 256, 128), (32, 512, 256), (8, 128, 128), (8, 64, 64)} (16-px only
+ * unless binning is GS_BINNING_TIGHT) x raster scheduling {persistent tile
+ * queue with 8 or 4 workgroups per CU, one workgroup per tile} at 8-bit
+ * digits; stage 2 tries radix digit widths {8, 4} at the stage-1 winner.
+ * Each candidate runs one settling batched step on `view_ids` and `reps`
+ * more timed one by one, over two interleaved rounds; the fastest step wins.
+ * A candidate whose binning overflows the entry capacity is skipped
+ * (GS_ERR_OVERFLOW if all do). The steps are real training steps (no
+ * candidate changes a step's result beyond float-atomic order), but the
+ * parameters, Adam moments and step count are saved first and restored
+ * afterwards, so tuning leaves the training state as it found it (and
+ * gradients / losses of the last step are those of a tuning step). chosen (optional, 5
+ * entries) = digit bits, workgroups per CU (0 = one per tile), tile_size,
+ * fwd_batch, bwd_batch. A captured step graph is recaptured afterwards. */
 gs_status gs_trainer_autotune(gs_trainer t, const uint32_t* view_ids, uint32_t n_ids, uint32_t reps,
                               uint32_t* chosen);
 /* The schedule gs_trainer_autotune chooses among, set directly (no
  * counterpart in the reference): radix digit bits 4 or 8; blocks_per_cu
  * workgroups per CU for the persistent raster passes, 0 = one per tile. */
 gs_status gs_trainer_set_schedule(gs_trainer t, uint32_t digit_bits, uint32_t blocks_per_cu);
+/* Work tiling, set directly (no counterpart in the reference): the binning
+ * and raster passes work on tile_size x tile_size pixel tiles (8, 16 or 32;
+ * coverage stays the reference's 16-px rects, so images do not change) and
+ * stage fwd_batch / bwd_batch records per LDS batch. Supported:
+ * tile 8: fwd 64 | 128, bwd 64 | 128; tile 16: fwd 128 | 256, bwd 64 | 128;
+ * tile 32: fwd 256 | 512, bwd 128 | 256. Tiles other than 16 need
+ * GS_BINNING_TIGHT. Default (16, 256, 128). */
+gs_status gs_trainer_set_tiling(gs_trainer t, uint32_t tile_size, uint32_t fwd_batch, uint32_t bwd_batch);
 /* Blocking: per-view (l1_mean, dssim_mean) of the last step, 2 per view. */
 gs_status gs_trainer_last_losses(gs_trainer t, double* out, uint32_t n_ids);
 /* Blocking: render view `view` with the current parameters into rgba (W*H*4). */

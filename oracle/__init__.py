@@ -71,6 +71,9 @@ def lib():
         L.oref_distortion_loss.restype = C.c_double
         L.oref_gradients.argtypes = [C.POINTER(_Scene), C.POINTER(_Camera), C.c_int, _dp, _dp,
                                      _dp, C.c_double, C.POINTER(_Grads)]
+        L.oref_marginal_decisions.argtypes = [C.POINTER(_Scene), C.POINTER(_Camera), C.c_double,
+                                              C.POINTER(C.c_uint8)]
+        L.oref_marginal_decisions.restype = C.c_int64
         L.oref_prepare_order.argtypes = [C.POINTER(_Scene), C.POINTER(_Camera), _i64p]
         L.oref_prepare_order.restype = C.c_int64
         L.oref_prep_f32.argtypes = [C.c_int64, _fp, _fp, _fp, C.POINTER(_Cam32), _u32p, _u32p,
@@ -261,6 +264,23 @@ def gradients(scene: Scene, cam: Camera, dl_dcolor, dl_ddepth=None, dl_dnormal=N
     g.cam_quat = np.array(cg.cam_quat[:])
     g.focal = float(cg.focal)
     return g
+
+
+def marginal_decisions(scene: Scene, cam: Camera, rel: float = 1e-5):
+    """Test aid with no reference counterpart: (count, flags) of the forward
+    walk's discrete decisions within relative band `rel` of a threshold
+    (alpha skip / clamp, ray vs screen branch, near plane, transmittance stop,
+    16-px rect truncation, on-screen, near cull, f32 depth-key ties), flags[i]
+    = 1 for surfels involved. With count 0, an f32 evaluation whose relative
+    error stays below rel makes the same decisions as this FP64 one."""
+    s, keep = scene._c()
+    c = cam._c()
+    flags = np.zeros(scene.n, dtype=np.uint8)
+    k = lib().oref_marginal_decisions(C.byref(s), C.byref(c), float(rel),
+                                      flags.ctypes.data_as(C.POINTER(C.c_uint8)))
+    if k < 0:
+        raise RuntimeError("oref_marginal_decisions failed")
+    return int(k), flags
 
 
 def prepare_order(scene: Scene, cam: Camera) -> np.ndarray:

@@ -654,6 +654,136 @@ int oref_render(const oref_scene* s, const oref_camera* cam, int mode, double* c
     return 0;
 }
 
+// Decisions of the forward walk that sit within a relative band `rel` of a
+// threshold, where an evaluation in f32 (or any other rounding) may decide
+// the other way: alpha vs the 1/255 skip and the 0.995 clamp, the ray vs
+// screen branch (g_ray vs g_scr) and the ray's near plane, transmittance vs
+// the 1e-4 stop (flags the surfel that reached it and the next contributor),
+// the tile-rect truncation (px -+ r) / 16 near an integer, the on-screen test,
+// the near cull, and equal-to-two-ulp f32 depth keys of consecutive surfels
+// whose rects overlap (their order may swap). Not part of forward.rs: a
+// test aid that tells whether an f32 path and this FP64 evaluation must make
+// the same discrete decisions. flags (n, optional) gets 1 for every surfel in
+// a marginal decision; returns the number of marginal decisions.
+int64_t oref_marginal_decisions(const oref_scene* s, const oref_camera* cam, double rel, uint8_t* flags) {
+    if (!s || !cam || cam->width <= 0 || cam->height <= 0) return -1;
+    Scene sc{s, num_coeffs(s->sh_degree)};
+    const int64_t n = s->n;
+    std::vector<uint8_t> fl(n, 0);
+    int64_t count = 0;
+    auto near = [&](double v, double thr) { return std::fabs(v - thr) <= rel * std::max(std::fabs(thr), 1e-300); };
+    // Near cull (on every surfel, before prepare drops the culled ones).
+    {
+        M3 r_cam = quat_to_mat(cam->quat);
+        V3 C = cam_center(cam);
+        for (int64_t i = 0; i < n; ++i) {
+            const double depth = -r_cam.tmul(sc.pos(i) - C).z;
+            if (near(depth, NEAR_DEPTH)) {
+                fl[i] = 1;
+                ++count;
+            }
+        }
+    }
+    auto surfels = prepare(sc, cam);
+    // Rect truncation and the on-screen test (forward.rs:104-123).
+    for (const auto& sf : surfels) {
+        if (sf.depth < NEAR_DEPTH) continue;
+        const double r_raw = 3.0 * cam->focal * (length(sf.tu) + length(sf.tv)) / sf.depth + 3.0;
+        const double r_px = std::min(r_raw, 64.0);
+        bool m = near(r_raw, 64.0);
+        const double edges[4] = {sf.px - r_px, sf.px + r_px, sf.py - r_px, sf.py + r_px};
+        for (double e : edges) {
+            const double v = e / TILE;
+            const double k = std::round(v);
+            if (std::fabs(v - k) <= rel * std::max(std::fabs(v), 1.0)) m = true;
+        }
+        m = m || near(sf.px + r_px, 0.0) || near(sf.px - r_px, (double)cam->width) ||
+            near(sf.py + r_px, 0.0) || near(sf.py - r_px, (double)cam->height);
+        if (m) {
+            fl[sf.idx] = 1;
+            ++count;
+        }
+    }
+    // Depth-order ties.
+    for (size_t i = 1; i < surfels.size(); ++i) {
+        const SurfelCam &a = surfels[i - 1], &b = surfels[i];
+        float fa = (float)a.depth, fb = (float)b.depth;
+        uint32_t ka, kb;
+        std::memcpy(&ka, &fa, 4);
+        std::memcpy(&kb, &fb, 4);
+        const bool overlap = a.on_screen && b.on_screen && a.x0 <= b.x1 && b.x0 <= a.x1 && a.y0 <= b.y1 &&
+                             b.y0 <= a.y1;
+        if (overlap && kb - ka <= 2u) {
+            fl[a.idx] = fl[b.idx] = 1;
+            ++count;
+        }
+    }
+    // The per-pixel walk.
+    TileLists tl = build_tile_lists(surfels, cam);
+    const Window win = window_of(cam);
+    for (int yy = 0; yy < win.h; ++yy) {
+        for (int xx = 0; xx < win.w; ++xx) {
+            const int x = win.x0 + xx, y = win.y0 + yy;
+            V3 d = pixel_ray(cam, x, y);
+            const double pix_x = x + 0.5, pix_y = y + 0.5;
+            double transmittance = 1.0;
+            bool flag_next = false, stopped = false;
+            walk_pixel(surfels, &tl, x, y, [&](int64_t li) {
+                const SurfelCam& sf = surfels[li];
+                // evaluate() with the branch quantities kept.
+                V3 sv = -d;
+                const double det = triple(sf.tu, sf.tv, sv);
+                double g_ray = 0.0, t = 0.0;
+                bool m = false;
+                if (std::fabs(det) > 1e-12) {
+                    const double u = triple(-sf.c, sf.tv, sv) / det;
+                    const double v = triple(sf.tu, -sf.c, sv) / det;
+                    t = triple(sf.tu, sf.tv, -sf.c) / det;
+                    const double g = std::exp(-(u * u + v * v) * 0.5);
+                    if (t > NEAR_DEPTH) g_ray = g;
+                    m = m || (near(t, NEAR_DEPTH) && sf.opacity * g >= ALPHA_SKIP * (1.0 - rel));
+                }
+                const double dx = pix_x - sf.px, dy = pix_y - sf.py;
+                const double g_scr = std::exp(-(dx * dx + dy * dy) / (2.0 * LOWPASS_SIGMA2));
+                const double ghat = std::max(g_ray, g_scr);
+                const double raw = sf.opacity * ghat;
+                if (raw >= ALPHA_SKIP * (1.0 - rel))
+                    m = m || std::fabs(g_ray - g_scr) <= rel * ghat;  // branch matters only if it can contribute
+                m = m || near(raw, ALPHA_SKIP) || near(raw, ALPHA_CLAMP);
+                const double alpha = std::min(raw, ALPHA_CLAMP);
+                const bool contributes = alpha >= ALPHA_SKIP;
+                if (stopped) {  // past a marginal stop: the next contributor
+                    if (contributes) {
+                        fl[sf.idx] = 1;
+                        return false;
+                    }
+                    return true;
+                }
+                if (m) {
+                    fl[sf.idx] = 1;
+                    ++count;
+                }
+                if (!contributes) return true;
+                if (flag_next) {
+                    fl[sf.idx] = 1;
+                    flag_next = false;
+                }
+                transmittance *= 1.0 - alpha;
+                if (near(transmittance, T_TERMINATE)) {
+                    fl[sf.idx] = 1;
+                    ++count;
+                    flag_next = true;
+                    if (transmittance < T_TERMINATE) stopped = true;
+                    return true;
+                }
+                return !(transmittance < T_TERMINATE);
+            });
+        }
+    }
+    if (flags) std::memcpy(flags, fl.data(), (size_t)n);
+    return count;
+}
+
 // forward.rs:174-205
 double oref_distortion_loss(const oref_scene* s, const oref_camera* cam, int mode) {
     Scene sc{s, num_coeffs(s->sh_degree)};

@@ -100,6 +100,13 @@ int oref_gradients(const oref_scene* s, const oref_camera* cam, int mode,
                    const double* dl_dcolor, const double* dl_ddepth,
                    const double* dl_dnormal, double lambda_dist, oref_grads* g);
 
+/* Test aid (no reference counterpart): the forward walk's discrete
+ * decisions within relative band `rel` of a threshold (alpha skip / clamp,
+ * ray vs screen branch, near plane, transmittance stop, rect truncation,
+ * on-screen, near cull, f32 depth-key ties). flags (n, optional) marks the
+ * surfels involved; returns their count of decisions. */
+int64_t oref_marginal_decisions(const oref_scene* s, const oref_camera* cam, double rel, uint8_t* flags);
+
 /* Number of surfels that survive the near cull and their prepare() order
  * (indices sorted by (f32 depth bits, idx)). order may be NULL. */
 int64_t oref_prepare_order(const oref_scene* s, const oref_camera* cam, int64_t* order);

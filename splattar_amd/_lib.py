@@ -58,7 +58,7 @@ class RasterDebug(C.Structure):
 
 class TrainView(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("camera", RasterCamera),
-                ("target_rgba", C.c_void_p)]
+                ("target_rgba", C.c_void_p), ("num_surfels", C.c_uint32)]
 
 
 class TrainConfig(C.Structure):
@@ -139,6 +139,7 @@ _PROTOS = {
     "gs_trainer_step": ([_vp, _vp, C.POINTER(_u32), _u32, _u32], C.c_int),
     "gs_trainer_autotune": ([_vp, C.POINTER(_u32), _u32, _u32, C.POINTER(_u32)], C.c_int),
     "gs_trainer_set_schedule": ([_vp, _u32, _u32], C.c_int),
+    "gs_trainer_set_tiling": ([_vp, _u32, _u32, _u32], C.c_int),
     "gs_trainer_last_losses": ([_vp, C.POINTER(C.c_double), _u32], C.c_int),
     "gs_trainer_render": ([_vp, _u32, _vp], C.c_int),
     "gs_trainer_params": ([_vp, C.POINTER(RasterBuffers)], C.c_int),

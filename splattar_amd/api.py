@@ -406,7 +406,8 @@ class Trainer:
 
     def __init__(self, ctx: GpuContext, cfg: TrainConfig, init: dict, views: list):
         """init: activated arrays (positions, scales, quats, opacities, sh);
-        views: list of (Camera, target_rgba (H, W, 4))."""
+        views: list of (Camera, target_rgba (H, W, 4)) or (Camera, target,
+        num_surfels): a view renders surfels [0, num_surfels) only (0 = all)."""
         self.ctx = ctx
         keep: list = []
         si = scene_input(init["positions"], init["scales"], init["quats"], init["opacities"], init["sh"],
@@ -415,7 +416,9 @@ class Trainer:
         tv = (L.TrainView * len(views))()
         self._targets = []
         self.sizes = []
-        for k, (cam, tgt) in enumerate(views):
+        for k, view in enumerate(views):
+            cam, tgt = view[0], view[1]
+            tv[k].num_surfels = int(view[2]) if len(view) > 2 else 0
             t = np.ascontiguousarray(tgt, np.float32)
             H, W = t.shape[:2]
             self._targets.append(t)
@@ -452,14 +455,18 @@ class Trainer:
     def set_schedule(self, digit_bits: int, blocks_per_cu: int):
         check(lib().gs_trainer_set_schedule(self.h, digit_bits, blocks_per_cu))
 
+    def set_tiling(self, tile_size: int, fwd_batch: int, bwd_batch: int):
+        check(lib().gs_trainer_set_tiling(self.h, tile_size, fwd_batch, bwd_batch))
+
     def autotune(self, view_ids, reps: int = 2) -> dict:
         """Time the candidate schedules on real batched steps and keep the
-        fastest; returns {"digit_bits", "blocks_per_cu"} (0 = one workgroup
-        per tile)."""
+        fastest; returns {"digit_bits", "blocks_per_cu" (0 = one workgroup
+        per tile), "tile_size", "fwd_batch", "bwd_batch"}."""
         ids = (C.c_uint32 * len(view_ids))(*[int(v) for v in view_ids])
-        chosen = (C.c_uint32 * 2)()
+        chosen = (C.c_uint32 * 5)()
         check(lib().gs_trainer_autotune(self.h, ids, len(view_ids), reps, chosen))
-        return {"digit_bits": int(chosen[0]), "blocks_per_cu": int(chosen[1])}
+        keys = ("digit_bits", "blocks_per_cu", "tile_size", "fwd_batch", "bwd_batch")
+        return {k: int(v) for k, v in zip(keys, chosen)}
 
     def last_losses(self, k: int) -> np.ndarray:
         out = (C.c_double * (2 * k))()

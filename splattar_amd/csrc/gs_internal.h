@@ -21,6 +21,11 @@ constexpr float kAlphaClamp = 0.995f;
 constexpr float kTTerminate = 1e-4f;
 constexpr float kNearDepth = 0.05f;
 constexpr int kTile = 16;        // coverage granularity: part of the forward model
+// Work-tile sizes of the binning and raster passes (pixels). Independent of
+// kTile: a record's footprint box is clipped to the pixels of its kTile rect,
+// so lists of any work-tile size hold exactly the records the 16-px model
+// lets reach each pixel.
+constexpr bool valid_work_tile(uint32_t t) { return t == 8 || t == 16 || t == 32; }
 constexpr int kShFloats = 48;    // SH storage padded to degree 3 (rasterizer.rs:13-15)
 constexpr int kWave = 64;
 
@@ -70,14 +75,17 @@ struct __align__(16) CamParams {
     float focal;
     float cx, cy;          // principal point = (W/2, H/2)
     uint32_t width, height;
-    uint32_t tiles_x, tiles_y;
+    uint32_t tiles_x, tiles_y;  // work tiles (tile_size px) of this view
     uint32_t tile_base;    // first global tile id of this view (batched views)
     uint32_t pix_base;     // first pixel of this view in batched image buffers
     uint32_t rec_base;     // first record (= view * N) of this view
     uint32_t sh_degree;
     float lambda_dist;
     uint32_t target_base;  // first pixel of this view's loss target (trainer)
-    uint32_t pad1, pad2;
+    uint32_t tile_size;    // work-tile size in pixels (8, 16 or 32)
+    uint32_t rtiles_x, rtiles_y;  // kTile (16-px) tiles of the forward model's rects
+    uint32_t n_surfels;    // surfels [0, n_surfels) take part in this view
+    uint32_t pad[3];
 };
 
 // ------------------------------------------------------------- launch shape
@@ -170,7 +178,7 @@ __device__ __forceinline__ RectOut tile_rect(const CamParams& cam, const CamFram
     rad = rad < 64.0f ? rad : 64.0f;
     const float fw = (float)cam.width, fh = (float)cam.height;
     r.on_screen = r.px + rad >= 0.0f && r.px - rad < fw && r.py + rad >= 0.0f && r.py - rad < fh;
-    const float mx = (float)(cam.tiles_x - 1), my = (float)(cam.tiles_y - 1);
+    const float mx = (float)(cam.rtiles_x - 1), my = (float)(cam.rtiles_y - 1);
     auto cl = [](float v, float hi) {
         float c = v > 0.0f ? v : 0.0f;
         c = c < hi ? c : hi;
@@ -385,27 +393,43 @@ __device__ __forceinline__ bool footprint_hits(const Record& r, float x0, float 
     return m <= 1.0f;
 }
 
-// Bit q (q = 0..3, quadrant (q & 1, q >> 1) of the 16x16 tile at (tx, ty))
-// is set when the record's footprint reaches that 8x8 pixel block.
-__device__ __forceinline__ uint32_t quadrant_mask(const Record& r, uint32_t tx, uint32_t ty) {
+// Bit by * (TW / 8) + bx is set when the record's footprint reaches 8x8
+// pixel block (bx, by) of the TW x TW work tile whose top-left pixel is
+// (x0, y0).
+template <int TW>
+__device__ __forceinline__ uint32_t block_mask(const Record& r, uint32_t x0, uint32_t y0) {
+    constexpr int NB = TW / 8;
     uint32_t m = 0;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const float qx = (float)(tx * kTile + (q & 1) * 8), qy = (float)(ty * kTile + (q >> 1) * 8);
+    for (int q = 0; q < NB * NB; ++q) {
+        const float qx = (float)(x0 + (q % NB) * 8), qy = (float)(y0 + (q / NB) * 8);
         if (footprint_hits(r, qx + 0.5f, qy + 0.5f, qx + 7.5f, qy + 7.5f)) m |= 1u << q;
     }
     return m;
 }
 
-// The tiles of rect (x0, y0, x1, y1, inclusive) whose pixel centres
-// 16 t + 0.5 .. 16 t + 15.5 meet the box; x0 > x1 (or y0 > y1) when none do.
-__device__ __forceinline__ uint4 clip_rect_to_box(uint4 rect, const float4 b) {
+// Cuts a footprint box (pixel centres) to the pixels of the kTile rect
+// (x0, y0, x1, y1, inclusive): pixels outside the rect are not reached in the
+// forward model (forward.rs:104-123), whatever the footprint.
+__device__ __forceinline__ float4 clip_box_to_rect(float4 b, uint4 rect) {
+    b.x = fmaxf(b.x, (float)(rect.x * kTile) + 0.5f);
+    b.y = fmaxf(b.y, (float)(rect.y * kTile) + 0.5f);
+    b.z = fminf(b.z, (float)(rect.z * kTile) + (kTile - 0.5f));
+    b.w = fminf(b.w, (float)(rect.w * kTile) + (kTile - 0.5f));
+    return b;
+}
+
+// The work tiles (size ts px, tiles_x x tiles_y of them) whose pixel
+// centres ts t + 0.5 .. ts t + ts - 0.5 meet the box; x0 > x1 (or y0 > y1)
+// when none do.
+__device__ __forceinline__ uint4 box_tiles(const float4 b, uint32_t ts, uint32_t tiles_x, uint32_t tiles_y) {
     const float bx0 = fminf(fmaxf(b.x, -1e6f), 1e6f), bx1 = fminf(fmaxf(b.z, -1e6f), 1e6f);
     const float by0 = fminf(fmaxf(b.y, -1e6f), 1e6f), by1 = fminf(fmaxf(b.w, -1e6f), 1e6f);
-    const int tx0 = max((int)rect.x, (int)ceilf((bx0 - 15.5f) * (1.0f / kTile)));
-    const int tx1 = min((int)rect.z, (int)floorf((bx1 - 0.5f) * (1.0f / kTile)));
-    const int ty0 = max((int)rect.y, (int)ceilf((by0 - 15.5f) * (1.0f / kTile)));
-    const int ty1 = min((int)rect.w, (int)floorf((by1 - 0.5f) * (1.0f / kTile)));
+    const float inv = 1.0f / (float)ts, hi = (float)ts - 0.5f;
+    const int tx0 = max(0, (int)ceilf((bx0 - hi) * inv));
+    const int tx1 = min((int)tiles_x - 1, (int)floorf((bx1 - 0.5f) * inv));
+    const int ty0 = max(0, (int)ceilf((by0 - hi) * inv));
+    const int ty1 = min((int)tiles_y - 1, (int)floorf((by1 - 0.5f) * inv));
     if (tx0 > tx1 || ty0 > ty1) return make_uint4(1u, 1u, 0u, 0u);
     return make_uint4((uint32_t)tx0, (uint32_t)ty0, (uint32_t)tx1, (uint32_t)ty1);
 }
@@ -553,11 +577,23 @@ struct TileQueue {
     uint32_t* cursor;  // kTileQueues counters
     uint32_t grid_blocks;
 };
+// Work-tile size (px; must equal every view's CamParams::tile_size) and the
+// records staged through LDS per batch by the forward and backward kernels.
+// Instantiated: forward (tile, batch) in {(8, 64), (8, 128), (16, 128),
+// (16, 256), (32, 256), (32, 512)}, backward in {(8, 64), (8, 128),
+// (16, 64), (16, 128), (32, 128), (32, 256)}; the depth / normal (aux)
+// variants only at (16, 256) / (16, 128). Others return
+// hipErrorInvalidValue.
+struct RasterShape {
+    uint32_t tile = 16;
+    uint32_t fwd_batch = 256;
+    uint32_t bwd_batch = 128;
+};
 hipError_t launch_raster_fwd(hipStream_t st, const CamParams* cams_dev, const CamParams* cams_host,
                              uint32_t n_views, uint32_t total_tiles, const uint32_t* ranges,
                              const uint32_t* payloads, const Record* recs, FwdOut out,
                              bool cull = true, TileQueue queue = {nullptr, nullptr, 0},
-                             bool aux_outputs = true);
+                             bool aux_outputs = true, RasterShape shape = {});
 
 // raster_bwd.hip
 struct BwdIn {
@@ -571,7 +607,7 @@ hipError_t launch_raster_bwd(hipStream_t st, const CamParams* cams_dev, const Ca
                              uint32_t n_views, uint32_t total_tiles, const uint32_t* ranges,
                              const uint32_t* payloads, const Record* recs, BwdIn in,
                              float* grad_coef, bool aux, bool cull = true,
-                             TileQueue queue = {nullptr, nullptr, 0});
+                             TileQueue queue = {nullptr, nullptr, 0}, RasterShape shape = {});
 
 // geom_bwd.hip
 struct ParamGrads {

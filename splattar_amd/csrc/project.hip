@@ -6,8 +6,11 @@
 //
 // HBM traffic per (view, surfel): reads 16+8+16+4+12*(d+1)^2 B of parameters,
 // writes a 128-B record, a 16-B rect, a 4-B depth key and a 4-B count.
-// With clip_to_box the tile rect is cut down to the tiles the record's
-// conservative alpha >= 1/255 footprint box reaches (tight binning).
+// The record's footprint box is clipped to the pixels of its 16-px tile rect
+// (the forward model's coverage). With clip_to_box the emitted rect is the
+// set of work tiles (cam.tile_size px) that clipped box reaches (tight
+// binning); without it, the 16-px rect itself (work tiles must be 16 px).
+// Surfels i >= cam.n_surfels are culled in that view.
 
 #include "gs_internal.h"
 
@@ -41,11 +44,12 @@ __global__ __launch_bounds__(256) void project_kernel(
     const CamParams cam = cams[view];
     const uint32_t g = view * n + i;
     // Per-item binning outputs, written on every path. The presort payload
-    // carries the tile count in its low 8 bits when pack_counts (a rect spans
-    // at most 9 x 9 tiles), so the scan reads counts in depth order without a
-    // gather. emit_info = (x0 | y0 << 16, x1 | y1 << 16, mask lo, mask hi):
-    // the rect in 16-bit tile coordinates and the mask of its tiles the
-    // footprint reaches (all ones = every tile).
+    // carries the tile count in its low 8 bits when pack_counts (for work
+    // tiles of 16 or 32 px a rect spans at most 9 x 9 tiles), so the scan
+    // reads counts in depth order without a gather.
+    // emit_info = (x0 | y0 << 16, x1 | y1 << 16, mask lo, mask hi): the rect
+    // in 16-bit tile coordinates and the mask of its tiles the footprint
+    // reaches (all ones = every tile).
     auto finish = [&](uint4 rect, uint32_t key, uint32_t count, uint64_t mask) {
         rects[g] = rect;
         depth_keys[g] = key;
@@ -57,12 +61,16 @@ __global__ __launch_bounds__(256) void project_kernel(
                                       (uint32_t)(mask >> 32));
     };
 
+    uint4 rect = make_uint4(1u, 1u, 0u, 0u);
+    if (i >= cam.n_surfels) {  // not part of this view
+        finish(rect, 0u, 0u, ~0ull);
+        return;
+    }
     const float4 p = pos[i];
     const float2 s = scales[i];
     const float4 q = quats[i];
     const CamFrame f = cam_frame(cam, p, s, q);
     const float depth = -f.cz_;
-    uint4 rect = make_uint4(1u, 1u, 0u, 0u);
     if (!(depth >= kNearDepth)) {  // near cull (surfel_prep_fwd.wgsl:171-176); NaN culls too
         finish(rect, 0u, 0u, ~0ull);
         return;
@@ -88,9 +96,11 @@ __global__ __launch_bounds__(256) void project_kernel(
     if (r.on_screen) {
         rect = make_uint4(r.x0, r.y0, r.x1, r.y1);
         record_footprint(rec, cam);
-        // Tight binning keeps only the tiles the alpha >= 1/255 footprint
-        // reaches; the dropped entries would contribute nothing anywhere.
-        if (clip_to_box) rect = clip_rect_to_box(rect, rec.box);
+        rec.box = clip_box_to_rect(rec.box, rect);
+        // Tight binning keeps only the work tiles the alpha >= 1/255
+        // footprint reaches; the dropped entries would contribute nothing.
+        const uint32_t ts = cam.tile_size;
+        if (clip_to_box) rect = box_tiles(rec.box, ts, cam.tiles_x, cam.tiles_y);
         if (rect.x <= rect.z && rect.y <= rect.w) {
             const uint32_t w = rect.z - rect.x + 1u;
             count = w * (rect.w - rect.y + 1u);
@@ -99,11 +109,11 @@ __global__ __launch_bounds__(256) void project_kernel(
                 // most 64 tiles: bit t = row-major tile t of the rect. Larger
                 // rects keep every tile.
                 mask = 0ull;
+                const float span = (float)ts - 0.5f;
                 for (uint32_t t = 0; t < count; ++t) {
-                    const float tx = (float)((rect.x + t % w) * kTile);
-                    const float ty = (float)((rect.y + t / w) * kTile);
-                    if (footprint_hits(rec, tx + 0.5f, ty + 0.5f, tx + 15.5f, ty + 15.5f))
-                        mask |= 1ull << t;
+                    const float tx = (float)((rect.x + t % w) * ts);
+                    const float ty = (float)((rect.y + t / w) * ts);
+                    if (footprint_hits(rec, tx + 0.5f, ty + 0.5f, tx + span, ty + span)) mask |= 1ull << t;
                 }
                 count = (uint32_t)__popcll(mask);
             }

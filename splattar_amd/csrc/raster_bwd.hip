@@ -10,12 +10,13 @@
 //     Nt, px, py, depth, colour, opacity, normal); the linear chain to
 //     camera-space (c, tu, tv) happens once per surfel in geom_bwd instead of
 //     four triple-product gradients per pixel-hit;
-//   * a workgroup is two wave64s, each owning an 8x16 half of the 16x16
-//     tile with two pixels per lane (rows y and y + 8); records are staged
-//     128 at a time, split into per-wave lists that carry which of the
-//     wave's two 8x8 blocks the record's footprint reaches, so a record is
-//     evaluated only where it can contribute and its two pixels' terms are
-//     summed in registers before one reduction;
+//   * a TW x TW work tile (TW = 16 or 32) is TW^2 / 128 wave64s, each owning
+//     an 8-wide, 16-tall column of two 8x8 blocks with two pixels per lane
+//     (rows y and y + 8); an 8x8 work tile is one wave with one pixel per
+//     lane. Records are staged BATCH at a time, split into per-wave lists
+//     that carry which of the wave's blocks the record's footprint reaches,
+//     so a record is evaluated only where it can contribute and its two
+//     pixels' terms are summed in registers before one reduction;
 //   * per record, each wave64 reduces its lanes' contributions with a
 //     transposed butterfly (17 lane exchanges for 16 values instead of 96,
 //     on permlane swaps and DPP rather than LDS permutes), adds them into an
@@ -29,9 +30,6 @@
 namespace gs {
 
 namespace {
-
-constexpr int kThreads = 128;  // two waves: the left and right 8x16 halves of a tile
-constexpr int kBatch = 128;    // records staged per batch, one per thread
 
 __device__ __forceinline__ uint32_t view_of_tile(const CamParams* cams, uint32_t n_views, uint32_t t) {
     uint32_t v = 0;
@@ -228,24 +226,32 @@ __device__ __forceinline__ void add_hit(PixState& p, const Record& r, const HitE
     }
 }
 
-template <bool AUX>
-__global__ __launch_bounds__(kThreads) void raster_bwd_kernel(const CamParams* __restrict__ cams,
-                                                              uint32_t n_views,
-                                                              const uint32_t* __restrict__ ranges,
-                                                              const uint32_t* __restrict__ payloads,
-                                                              const Record* __restrict__ recs, BwdIn in,
-                                                              float* __restrict__ grad_coef, bool cull,
-                                                              uint32_t total_tiles, TileQueue queue) {
-    constexpr int NS = AUX ? 20 : 16;  // LDS slots per record
-    __shared__ float4 s_m[kBatch], s_a[kBatch], s_b[kBatch], s_c[kBatch], s_n[kBatch];
-    __shared__ float s_lthr[kBatch];
-    __shared__ uint16_t s_list[2][kBatch];  // record | top block << 8 | bottom block << 9
-    __shared__ uint32_t s_qcnt[2][2];       // [staging wave][list wave]
-    __shared__ float s_grad[kBatch * NS];
+template <bool AUX, int TW, int BATCH>
+__global__ __launch_bounds__(TW == 8 ? 64 : TW * TW / 2) void raster_bwd_kernel(
+    const CamParams* __restrict__ cams, uint32_t n_views, const uint32_t* __restrict__ ranges,
+    const uint32_t* __restrict__ payloads, const Record* __restrict__ recs, BwdIn in,
+    float* __restrict__ grad_coef, bool cull, uint32_t total_tiles, TileQueue queue) {
+    constexpr int NS = AUX ? 20 : 16;        // LDS slots per record
+    constexpr bool TWO = TW >= 16;           // two pixels per lane (an 8x16 column per wave)
+    constexpr int NC = TW / 8;               // 8x8 blocks per tile row
+    constexpr int NW = TWO ? TW * TW / 128 : 1;  // waves
+    constexpr int NT = NW * 64;
+    constexpr int SW = BATCH / 64;           // staging waves per batch
+    constexpr int SPT = (BATCH + NT - 1) / NT;
+    constexpr int MB = 10;                   // list entry: slot | top << MB | bottom << (MB + 1)
+    static_assert(BATCH % 64 == 0 && BATCH <= (1 << MB), "batch of whole waves");
+    __shared__ float4 s_m[BATCH], s_a[BATCH], s_b[BATCH], s_c[BATCH], s_n[BATCH];
+    __shared__ float s_lthr[BATCH];
+    __shared__ uint16_t s_list[NW][BATCH];
+    __shared__ uint32_t s_qcnt[SW][NW];      // [staging wave][list wave]
+    __shared__ float s_grad[BATCH * NS];
     __shared__ uint32_t s_view, s_maxlast, s_next;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    // Wave w owns block column c = w % NC, rows 16 (w / NC) .. + 15: blocks
+    // top = (c, 2 (w / NC)) and bottom = (c, 2 (w / NC) + 1).
+    const int col = wave % NC, hrow = wave / NC;
     for (uint32_t iter = 0;; ++iter) {
         uint32_t pos;
         if (queue.cursor != nullptr) {
@@ -271,65 +277,82 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_kernel(const CamParams* _
         const CamParams& cam = cams[s_view];
         const uint32_t local = gtile - cam.tile_base;
         const uint32_t tx = local % cam.tiles_x, ty = local / cam.tiles_x;
+        const uint32_t x0 = tx * TW, y0 = ty * TW;
         const float lambda = AUX ? cam.lambda_dist : 0.0f;
-        // Wave w owns tile columns 8w..8w+7; a lane owns rows y and y + 8,
-        // i.e. one pixel in 8x8 block w (top) and one in block w + 2 (bottom).
-        const int px = (int)(tx * kTile) + wave * 8 + (lane & 7);
-        const int py = (int)(ty * kTile) + (lane >> 3);
+        const int px = (int)x0 + col * 8 + (lane & 7);
+        const int py = (int)y0 + hrow * 16 + (lane >> 3);
         PixState pa, pb;
         load_pixel<AUX>(pa, cam, px, py, in, lambda);
-        load_pixel<AUX>(pb, cam, px, py + 8, in, lambda);
+        if constexpr (TWO) load_pixel<AUX>(pb, cam, px, py + 8, in, lambda);
+        else pb.last = 0u;
         const uint32_t mylast = max(pa.last, pb.last);
         if (mylast) atomicMax(&s_maxlast, mylast);
         __syncthreads();
         const uint32_t maxlast = s_maxlast;
         if (maxlast == 0) continue;
 
-        const int nbatch = (int)((maxlast + kBatch - 1) / kBatch);
+        const int nbatch = (int)((maxlast + BATCH - 1) / BATCH);
         for (int b = nbatch - 1; b >= 0; --b) {
-            const uint32_t base = start + (uint32_t)b * kBatch;
-            const uint32_t cnt = min(min((uint32_t)kBatch, end - base), maxlast - (uint32_t)b * kBatch);
-            uint32_t qmask = 0;
-            if ((uint32_t)tid < cnt) {
-                Record r = recs[payloads[base + tid]];
-                s_m[tid] = r.m_nt;
-                s_a[tid] = r.a_px;
-                s_b[tid] = r.b_py;
-                s_c[tid] = r.col_op;
-                s_n[tid] = r.nrm_dep;
-                s_lthr[tid] = record_lthr(r.col_op.w);
-                qmask = cull ? quadrant_mask(r, tx, ty) : 0xFu;
-            }
-            for (int k = tid; k < (int)cnt * NS; k += kThreads) s_grad[k] = 0.0f;
-            // Order-preserving per-wave lists: wave w takes the records that
-            // reach block w or block w + 2.
-            uint32_t qpos[2], memb[2];
+            const uint32_t base = start + (uint32_t)b * BATCH;
+            const uint32_t cnt = min(min((uint32_t)BATCH, end - base), maxlast - (uint32_t)b * BATCH);
+            uint32_t memb[SPT][NW], qpos[SPT][NW];
 #pragma unroll
-            for (int w = 0; w < 2; ++w) {
-                const uint32_t top = (qmask >> w) & 1u, bot = (qmask >> (w + 2)) & 1u;
-                memb[w] = top | (bot << 1);
-                const uint64_t bal = __ballot(memb[w] != 0u);
-                qpos[w] = (uint32_t)__popcll(bal & lt_mask);
-                if (lane == 0) s_qcnt[wave][w] = (uint32_t)__popcll(bal);
-            }
-            __syncthreads();
+            for (int m = 0; m < SPT; ++m) {
+                const int slot = tid + m * NT;
+                if (slot >= BATCH) continue;  // whole waves (NT, BATCH multiples of 64)
+                uint32_t qmask = 0;
+                if ((uint32_t)slot < cnt) {
+                    Record r = recs[payloads[base + slot]];
+                    s_m[slot] = r.m_nt;
+                    s_a[slot] = r.a_px;
+                    s_b[slot] = r.b_py;
+                    s_c[slot] = r.col_op;
+                    s_n[slot] = r.nrm_dep;
+                    s_lthr[slot] = record_lthr(r.col_op.w);
+                    qmask = cull ? block_mask<TW>(r, x0, y0) : ~0u;
+                }
+                // Order-preserving per-wave lists: wave w takes the records
+                // that reach its top or bottom block.
+                const int sw = slot >> 6;
 #pragma unroll
-            for (int w = 0; w < 2; ++w) {
-                if (memb[w] != 0u) {
-                    const uint32_t off = wave == 1 ? s_qcnt[0][w] : 0u;
-                    s_list[w][off + qpos[w]] = (uint16_t)(tid | (memb[w] << 8));
+                for (int w = 0; w < NW; ++w) {
+                    const int bt = TWO ? (2 * (w / NC)) * NC + w % NC : 0;
+                    const uint32_t top = (qmask >> bt) & 1u;
+                    const uint32_t bot = TWO ? (qmask >> (bt + NC)) & 1u : 0u;
+                    memb[m][w] = top | (bot << 1);
+                    const uint64_t bal = __ballot(memb[m][w] != 0u);
+                    qpos[m][w] = (uint32_t)__popcll(bal & lt_mask);
+                    if (lane == 0) s_qcnt[sw][w] = (uint32_t)__popcll(bal);
                 }
             }
-            const int my_count = (int)(s_qcnt[0][wave] + s_qcnt[1][wave]);
+            for (int k = tid; k < (int)cnt * NS; k += NT) s_grad[k] = 0.0f;
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < SPT; ++m) {
+                const int slot = tid + m * NT;
+                if (slot >= BATCH) continue;
+                const int sw = slot >> 6;
+#pragma unroll
+                for (int w = 0; w < NW; ++w) {
+                    if (memb[m][w] != 0u) {
+                        uint32_t off = 0;
+                        for (int v = 0; v < sw; ++v) off += s_qcnt[v][w];
+                        s_list[w][off + qpos[m][w]] = (uint16_t)(slot | (memb[m][w] << MB));
+                    }
+                }
+            }
+            int my_count = 0;
+#pragma unroll
+            for (int v = 0; v < SW; ++v) my_count += (int)s_qcnt[v][wave];
             __syncthreads();
 
             for (int kk = my_count - 1; kk >= 0; --kk) {
                 // The entry is the same for every lane: make it scalar, so the
                 // block branches below are uniform.
                 const uint32_t e = __builtin_amdgcn_readfirstlane((uint32_t)s_list[wave][kk]);
-                const uint32_t j = e & 0xFFu;
-                const bool top = (e >> 8) & 1u, bot = (e >> 9) & 1u;
-                const uint32_t rel = (uint32_t)b * kBatch + j;
+                const uint32_t j = e & ((1u << MB) - 1u);
+                const bool top = (e >> MB) & 1u, bot = TWO && ((e >> (MB + 1)) & 1u);
+                const uint32_t rel = (uint32_t)b * BATCH + j;
                 // A block is walked only while some pixel of it is at or before
                 // its last contributor (a pixel past it changes nothing).
                 const bool need_a = top && __any(rel < pa.last);
@@ -381,7 +404,7 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_kernel(const CamParams* _
                 }
             }
             __syncthreads();
-            for (int k = tid; k < (int)cnt * NS; k += kThreads) {
+            for (int k = tid; k < (int)cnt * NS; k += NT) {
                 const float v = s_grad[k];
                 if (v != 0.0f) {
                     const int e = k / NS, slot = k - e * NS;
@@ -393,21 +416,46 @@ __global__ __launch_bounds__(kThreads) void raster_bwd_kernel(const CamParams* _
     }
 }
 
+template <bool AUX, int TW, int BATCH>
+void launch_bwd(hipStream_t st, uint32_t grid, const CamParams* cams_dev, uint32_t n_views,
+                uint32_t total_tiles, const uint32_t* ranges, const uint32_t* payloads, const Record* recs,
+                BwdIn in, float* grad_coef, bool cull, TileQueue queue) {
+    constexpr int NT = TW == 8 ? 64 : TW * TW / 2;
+    hipLaunchKernelGGL((raster_bwd_kernel<AUX, TW, BATCH>), dim3(grid), dim3(NT), 0, st, cams_dev, n_views,
+                       ranges, payloads, recs, in, grad_coef, cull, total_tiles, queue);
+}
+
 }  // namespace
 
 hipError_t launch_raster_bwd(hipStream_t st, const CamParams* cams_dev, const CamParams* /*cams_host*/,
                              uint32_t n_views, uint32_t total_tiles, const uint32_t* ranges,
                              const uint32_t* payloads, const Record* recs, BwdIn in,
-                             float* grad_coef, bool aux, bool cull, TileQueue queue) {
+                             float* grad_coef, bool aux, bool cull, TileQueue queue, RasterShape shape) {
     if (total_tiles == 0) return hipSuccess;
     const uint32_t grid = queue.cursor != nullptr ? std::min(total_tiles, std::max(queue.grid_blocks, 1u))
                                                   : total_tiles;
-    if (aux)
-        hipLaunchKernelGGL(raster_bwd_kernel<true>, dim3(grid), dim3(kThreads), 0, st, cams_dev, n_views, ranges,
-                           payloads, recs, in, grad_coef, cull, total_tiles, queue);
-    else
-        hipLaunchKernelGGL(raster_bwd_kernel<false>, dim3(grid), dim3(kThreads), 0, st, cams_dev, n_views, ranges,
-                           payloads, recs, in, grad_coef, cull, total_tiles, queue);
+    const uint32_t t = shape.tile, b = shape.bwd_batch;
+    auto go = [&](auto fn) {
+        fn(st, grid, cams_dev, n_views, total_tiles, ranges, payloads, recs, in, grad_coef, cull, queue);
+    };
+    if (aux) {
+        if (t != 16 || b != 128) return hipErrorInvalidValue;  // depth / normal losses: 16-px tiles
+        go(launch_bwd<true, 16, 128>);
+    } else if (t == 8 && b == 64) {
+        go(launch_bwd<false, 8, 64>);
+    } else if (t == 8 && b == 128) {
+        go(launch_bwd<false, 8, 128>);
+    } else if (t == 16 && b == 64) {
+        go(launch_bwd<false, 16, 64>);
+    } else if (t == 16 && b == 128) {
+        go(launch_bwd<false, 16, 128>);
+    } else if (t == 32 && b == 128) {
+        go(launch_bwd<false, 32, 128>);
+    } else if (t == 32 && b == 256) {
+        go(launch_bwd<false, 32, 256>);
+    } else {
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

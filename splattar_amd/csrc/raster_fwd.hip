@@ -1,18 +1,18 @@
-// Tile rasterizer forward (A8): one 256-thread workgroup per 16x16 tile
-// (of any view in a batch), front-to-back alpha compositing.
+// Tile rasterizer forward (A8): one workgroup per work tile (of any view in
+// a batch), front-to-back alpha compositing.
 //
 // Restates crates/gs-kernels/src/shaders/rasterize_fwd.wgsl:20-85 and
 // evaluate_hit of raster_common.wgsl:69-95 (the oracle's forward.rs:207-246).
-// MI355X layout: the four wave64s of a workgroup each own an 8x8 quadrant of
-// the tile. Surfel records are staged through LDS in batches of 256 (one
-// record per thread, gathered through the sorted payloads); while staging,
-// each thread tests its record's alpha >= 1/255 footprint (box, screen disk
-// and ellipse, computed at projection) against the four quadrants, and the
-// batch is split into four order-preserving per-quadrant lists, so a wave
-// only walks the records that can touch its 64 pixels. Per pixel, an
-// exponent test rejects non-contributing records before exp(). A wave leaves
-// the batch once its pixels are saturated; the workgroup stops when every
-// pixel's transmittance is below 1e-4.
+// MI355X layout: a TW x TW work tile (TW = 8, 16 or 32) is (TW/8)^2 wave64s,
+// each owning one 8x8 pixel block. Surfel records are staged through LDS in
+// batches of BATCH (gathered through the sorted payloads); while staging,
+// each record's alpha >= 1/255 footprint (box clipped to its 16-px rect,
+// screen disk and ellipse, computed at projection) is tested against the
+// tile's blocks, and the batch is split into order-preserving per-block
+// lists, so a wave only walks the records that can touch its 64 pixels. Per
+// pixel, an exponent test rejects non-contributing records before exp(). A
+// wave leaves the batch once its pixels are saturated; the workgroup stops
+// when every pixel's transmittance is below 1e-4.
 //
 // Outputs per pixel (same buffers as the reference Rasterizer):
 //   color  = (sum w c, sum w),  aux = (T_final, sum w t, 0, 0),
@@ -25,8 +25,6 @@ namespace gs {
 
 namespace {
 
-constexpr int kBatch = 256;
-
 // Locate the view owning a global tile id (views are few; tile_base ascends).
 __device__ __forceinline__ uint32_t view_of_tile(const CamParams* cams, uint32_t n_views, uint32_t t) {
     uint32_t v = 0;
@@ -37,21 +35,25 @@ __device__ __forceinline__ uint32_t view_of_tile(const CamParams* cams, uint32_t
 
 // AUX = false (training with colour losses only) skips the depth and normal
 // accumulators: aux = (T_final, 0, 0, 0), normal is not written.
-template <bool AUX>
-__global__ __launch_bounds__(256) void raster_fwd_kernel(const CamParams* __restrict__ cams,
-                                                         uint32_t n_views,
-                                                         const uint32_t* __restrict__ ranges,
-                                                         const uint32_t* __restrict__ payloads,
-                                                         const Record* __restrict__ recs,
-                                                         FwdOut out, bool cull, uint32_t total_tiles,
-                                                         TileQueue queue) {
-    __shared__ float4 s_m[kBatch], s_a[kBatch], s_b[kBatch], s_c[kBatch], s_n[kBatch];
-    __shared__ float s_lthr[kBatch];
-    __shared__ uint16_t s_list[4][kBatch];
-    __shared__ uint32_t s_qcnt[4][4];
+template <bool AUX, int TW, int BATCH>
+__global__ __launch_bounds__((TW / 8) * (TW / 8) * 64) void raster_fwd_kernel(
+    const CamParams* __restrict__ cams, uint32_t n_views, const uint32_t* __restrict__ ranges,
+    const uint32_t* __restrict__ payloads, const Record* __restrict__ recs, FwdOut out, bool cull,
+    uint32_t total_tiles, TileQueue queue) {
+    constexpr int NC = TW / 8;             // blocks per tile row
+    constexpr int NB = NC * NC;            // blocks (= waves) per tile
+    constexpr int NT = NB * 64;            // threads
+    constexpr int SW = BATCH / 64;         // staging waves per batch
+    constexpr int SPT = (BATCH + NT - 1) / NT;  // batch slots per thread
+    static_assert(BATCH % 64 == 0 && BATCH <= 1024, "batch of whole waves");
+    __shared__ float4 s_m[BATCH], s_a[BATCH], s_b[BATCH], s_c[BATCH], s_n[AUX ? BATCH : 1];
+    __shared__ float s_lthr[BATCH];
+    __shared__ uint16_t s_list[NB][BATCH];
+    __shared__ uint32_t s_qcnt[SW][NB];
     __shared__ uint32_t s_view, s_next;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t all_blocks = NB == 32 ? ~0u : (1u << NB) - 1u;
     for (uint32_t iter = 0;; ++iter) {
         uint32_t pos;
         if (queue.cursor != nullptr) {
@@ -72,8 +74,9 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const CamParams* __rest
         const CamParams& cam = cams[s_view];
         const uint32_t local = gtile - cam.tile_base;
         const uint32_t tx = local % cam.tiles_x, ty = local / cam.tiles_x;
-        const int px = (int)(tx * kTile) + (wave & 1) * 8 + (lane & 7);
-        const int py = (int)(ty * kTile) + (wave >> 1) * 8 + (lane >> 3);
+        const uint32_t x0 = tx * TW, y0 = ty * TW;
+        const int px = (int)x0 + (wave % NC) * 8 + (lane & 7);
+        const int py = (int)y0 + (wave / NC) * 8 + (lane >> 3);
         const bool inside = px < (int)cam.width && py < (int)cam.height;
         float sx, sy;
         pixel_s(cam, px, py, sx, sy);
@@ -86,37 +89,52 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const CamParams* __rest
         uint32_t last = 0;
         bool done = !inside;
 
-        for (uint32_t base = start; base < end; base += kBatch) {
-            const uint32_t cnt = min((uint32_t)kBatch, end - base);
-            uint32_t qmask = 0;
-            if ((uint32_t)tid < cnt) {
-                Record r = recs[payloads[base + tid]];
-                s_m[tid] = r.m_nt;
-                s_a[tid] = r.a_px;
-                s_b[tid] = r.b_py;
-                s_c[tid] = r.col_op;
-                if (AUX) s_n[tid] = r.nrm_dep;
-                s_lthr[tid] = record_lthr(r.col_op.w);
-                qmask = cull ? quadrant_mask(r, tx, ty) : 0xFu;
-            }
-            // Order-preserving split of the batch into per-quadrant lists.
-            uint32_t qpos[4];
+        for (uint32_t base = start; base < end; base += BATCH) {
+            const uint32_t cnt = min((uint32_t)BATCH, end - base);
+            uint32_t qm[SPT];
+            uint32_t qpos[SPT][NB];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint64_t bal = __ballot((qmask >> q) & 1u);
-                qpos[q] = (uint32_t)__popcll(bal & lt_mask);
-                if (lane == 0) s_qcnt[wave][q] = (uint32_t)__popcll(bal);
+            for (int m = 0; m < SPT; ++m) {
+                const int slot = tid + m * NT;
+                qm[m] = 0;
+                if (slot >= BATCH) continue;  // whole waves (NT, BATCH multiples of 64)
+                if ((uint32_t)slot < cnt) {
+                    Record r = recs[payloads[base + slot]];
+                    s_m[slot] = r.m_nt;
+                    s_a[slot] = r.a_px;
+                    s_b[slot] = r.b_py;
+                    s_c[slot] = r.col_op;
+                    if (AUX) s_n[slot] = r.nrm_dep;
+                    s_lthr[slot] = record_lthr(r.col_op.w);
+                    qm[m] = cull ? block_mask<TW>(r, x0, y0) : all_blocks;
+                }
+                // Order-preserving split of the batch into per-block lists.
+                const int sw = slot >> 6;
+#pragma unroll
+                for (int q = 0; q < NB; ++q) {
+                    const uint64_t bal = __ballot((qm[m] >> q) & 1u);
+                    qpos[m][q] = (uint32_t)__popcll(bal & lt_mask);
+                    if (lane == 0) s_qcnt[sw][q] = (uint32_t)__popcll(bal);
+                }
             }
             __syncthreads();
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if ((qmask >> q) & 1u) {
-                    uint32_t off = 0;
-                    for (int w = 0; w < wave; ++w) off += s_qcnt[w][q];
-                    s_list[q][off + qpos[q]] = (uint16_t)tid;
+            for (int m = 0; m < SPT; ++m) {
+                const int slot = tid + m * NT;
+                if (slot >= BATCH) continue;
+                const int sw = slot >> 6;
+#pragma unroll
+                for (int q = 0; q < NB; ++q) {
+                    if ((qm[m] >> q) & 1u) {
+                        uint32_t off = 0;
+                        for (int w = 0; w < sw; ++w) off += s_qcnt[w][q];
+                        s_list[q][off + qpos[m][q]] = (uint16_t)slot;
+                    }
                 }
             }
-            const uint32_t my_count = s_qcnt[0][wave] + s_qcnt[1][wave] + s_qcnt[2][wave] + s_qcnt[3][wave];
+            uint32_t my_count = 0;
+#pragma unroll
+            for (int w = 0; w < SW; ++w) my_count += s_qcnt[w][wave];
             __syncthreads();
             if (__any(!done)) {
                 for (uint32_t k = 0; k < my_count; ++k) {
@@ -126,7 +144,7 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const CamParams* __rest
                     r.a_px = s_a[j];
                     r.b_py = s_b[j];
                     r.col_op = s_c[j];
-                    r.nrm_dep = AUX ? s_n[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    r.nrm_dep = AUX ? s_n[AUX ? j : 0] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                     const HitEval h = eval_hit(r, sx, sy, pxc, pyc, s_lthr[j]);
                     if (!done && h.alpha > 0.0f) {
                         const float w = T * h.alpha;
@@ -160,21 +178,44 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(const CamParams* __rest
     }
 }
 
+template <bool AUX, int TW, int BATCH>
+void launch_fwd(hipStream_t st, uint32_t grid, const CamParams* cams_dev, uint32_t n_views,
+                uint32_t total_tiles, const uint32_t* ranges, const uint32_t* payloads, const Record* recs,
+                FwdOut out, bool cull, TileQueue queue) {
+    constexpr int NT = (TW / 8) * (TW / 8) * 64;
+    hipLaunchKernelGGL((raster_fwd_kernel<AUX, TW, BATCH>), dim3(grid), dim3(NT), 0, st, cams_dev, n_views,
+                       ranges, payloads, recs, out, cull, total_tiles, queue);
+}
+
 }  // namespace
 
 hipError_t launch_raster_fwd(hipStream_t st, const CamParams* cams_dev, const CamParams* /*cams_host*/,
                              uint32_t n_views, uint32_t total_tiles, const uint32_t* ranges,
                              const uint32_t* payloads, const Record* recs, FwdOut out, bool cull,
-                             TileQueue queue, bool aux_outputs) {
+                             TileQueue queue, bool aux_outputs, RasterShape shape) {
     if (total_tiles == 0) return hipSuccess;
     const uint32_t grid = queue.cursor != nullptr ? std::min(total_tiles, std::max(queue.grid_blocks, 1u))
                                                   : total_tiles;
-    if (aux_outputs)
-        hipLaunchKernelGGL(raster_fwd_kernel<true>, dim3(grid), dim3(256), 0, st, cams_dev, n_views, ranges,
-                           payloads, recs, out, cull, total_tiles, queue);
-    else
-        hipLaunchKernelGGL(raster_fwd_kernel<false>, dim3(grid), dim3(256), 0, st, cams_dev, n_views, ranges,
-                           payloads, recs, out, cull, total_tiles, queue);
+    const uint32_t t = shape.tile, b = shape.fwd_batch;
+    auto go = [&](auto fn) { fn(st, grid, cams_dev, n_views, total_tiles, ranges, payloads, recs, out, cull, queue); };
+    if (aux_outputs) {
+        if (t != 16 || b != 256) return hipErrorInvalidValue;  // depth / normal outputs: 16-px tiles
+        go(launch_fwd<true, 16, 256>);
+    } else if (t == 8 && b == 64) {
+        go(launch_fwd<false, 8, 64>);
+    } else if (t == 8 && b == 128) {
+        go(launch_fwd<false, 8, 128>);
+    } else if (t == 16 && b == 128) {
+        go(launch_fwd<false, 16, 128>);
+    } else if (t == 16 && b == 256) {
+        go(launch_fwd<false, 16, 256>);
+    } else if (t == 32 && b == 256) {
+        go(launch_fwd<false, 32, 256>);
+    } else if (t == 32 && b == 512) {
+        go(launch_fwd<false, 32, 512>);
+    } else {
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

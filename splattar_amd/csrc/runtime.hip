@@ -90,7 +90,7 @@ static void quat_to_rot(const float q[4], float rot[9]) {
         for (int j = 0; j < 3; ++j) rot[3 * k + j] = (float)c[k][j];
 }
 
-static CamParams make_cam(const gs_raster_camera& c, uint32_t w, uint32_t h) {
+static CamParams make_cam(const gs_raster_camera& c, uint32_t w, uint32_t h, uint32_t tile = kTile) {
     CamParams p;
     std::memset(&p, 0, sizeof(p));
     quat_to_rot(c.quat, p.rot);
@@ -102,8 +102,12 @@ static CamParams make_cam(const gs_raster_camera& c, uint32_t w, uint32_t h) {
     p.cy = (float)h * 0.5f;
     p.width = w;
     p.height = h;
-    p.tiles_x = div_up(w, kTile);
-    p.tiles_y = div_up(h, kTile);
+    p.tiles_x = div_up(w, tile);
+    p.tiles_y = div_up(h, tile);
+    p.tile_size = tile;
+    p.rtiles_x = div_up(w, kTile);
+    p.rtiles_y = div_up(h, kTile);
+    p.n_surfels = 0xFFFFFFFFu;
     p.sh_degree = c.sh_degree;
     return p;
 }
@@ -427,8 +431,10 @@ gs_status gs_binner_encode(gs_binner b, void* stream, uint32_t num_items, uint32
     if (b->cam_tiles_x != tiles_x) {
         CamParams c;
         std::memset(&c, 0, sizeof(c));
-        c.tiles_x = tiles_x;
-        c.tiles_y = b->num_tiles / tiles_x;
+        c.tiles_x = c.rtiles_x = tiles_x;
+        c.tiles_y = c.rtiles_y = b->num_tiles / tiles_x;
+        c.tile_size = kTile;
+        c.n_surfels = 0xFFFFFFFFu;
         GS_HIP(hipMemcpyAsync(b->cam, &c, sizeof(c), hipMemcpyHostToDevice, st));
         GS_HIP(hipStreamSynchronize(st));
         b->cam_tiles_x = tiles_x;
@@ -552,7 +558,8 @@ struct RasterCore {
     uint32_t queue_tiles = 0;        // tiles covered by the current order
     uint32_t* tkeys[2] = {nullptr, nullptr};  // 32-bit tile keys, stored inside keys[1]
     uint32_t binning = GS_BINNING_PRESORT;
-    bool cull = true;                // per-quadrant footprint culling in the rasterizers
+    bool cull = true;                // per-block footprint culling in the rasterizers
+    RasterShape shape;               // work-tile size and LDS batches (tile != 16 needs TIGHT binning)
     bool aux_outputs = true;         // forward writes depth (aux.y) and normals
     bool keys64_valid = false;       // keys[result_alt] holds 64-bit keys of the last forward
     uint32_t digit_bits = 8;
@@ -731,6 +738,8 @@ struct RasterCore {
                       const float2* sc, const float4* q, const float* op, const float* sh,
                       Profiler* prof = nullptr) {
         if (n > cap_n) return fail(GS_ERR_CAPACITY, "num_surfels exceeds capacity");
+        if (shape.tile != (uint32_t)kTile && binning != GS_BINNING_TIGHT)
+            return fail(GS_ERR_INVALID, "work tiles other than 16 px need GS_BINNING_TIGHT");
         Profiler dummy;
         Profiler& P = prof ? *prof : dummy;
         const CamParams* cd = cams_dev + v0;
@@ -746,7 +755,8 @@ struct RasterCore {
             // a stable sort on the tile bits only (binning.hip).
             // Up to 2^24 items the presort payload also carries the tile
             // count (low 8 bits), so the scan needs no gather.
-            const bool packed = items < (1u << 24);
+            // 8-px work tiles can give a rect more than 255 tiles: no packing.
+            const bool packed = items < (1u << 24) && shape.tile >= 16;
             GS_CHECK(P.mark(st, GS_STAGE_PROJECT));
             GS_HIP(launch_project(st, cd, nv, n, pos, sc, q, op, sh, recs, rects, depth_keys, offsets,
                                   ranges, 2 * max_tile_id, pre_keys[0], pre_vals[0], item_count,
@@ -774,7 +784,7 @@ struct RasterCore {
             GS_CHECK(schedule_tiles(st, tiles));
             GS_CHECK(P.mark(st, GS_STAGE_RASTER_FWD));
             GS_HIP(launch_raster_fwd(st, cd, cams.data() + v0, nv, tiles, ranges, pays[result_alt], recs, o,
-                                     cull, queue(0), aux_outputs));
+                                     cull, queue(0), aux_outputs, shape));
             return GS_OK;
         }
         // 64-bit (tile || depth) key path: one sort over every key bit that varies.
@@ -797,7 +807,7 @@ struct RasterCore {
         GS_CHECK(schedule_tiles(st, tiles));
         GS_CHECK(P.mark(st, GS_STAGE_RASTER_FWD));
         GS_HIP(launch_raster_fwd(st, cd, cams.data() + v0, nv, tiles, ranges, pays[result_alt], recs, o,
-                                 cull, queue(0), aux_outputs));
+                                 cull, queue(0), aux_outputs, shape));
         return GS_OK;
     }
 
@@ -838,7 +848,7 @@ struct RasterCore {
         const TileQueue qb = queue_tiles == tiles ? queue(1) : TileQueue{nullptr, nullptr, 0};
         if (qb.cursor != nullptr) GS_HIP(launch_fill_u32(st, qb.cursor, 0u, kTileQueues));
         GS_HIP(launch_raster_bwd(st, cd, cams.data() + v0, nv, tiles, ranges, pays[result_alt], recs, in,
-                                 grad_coef, aux, cull, qb));
+                                 grad_coef, aux, cull, qb, shape));
         GS_CHECK(P.mark(st, GS_STAGE_GEOM_BWD));
         GS_HIP(launch_geom_bwd(st, cd, nv, n, pos, sc, q, sh, recs, grad_coef, g, accumulate,
                                cam_partials));
@@ -1299,6 +1309,7 @@ struct TrainViewDev {
     uint32_t width, height;
     gs_raster_camera camera;
     size_t target_offset;   // in float4 units
+    uint32_t num_surfels;   // surfels [0, num_surfels) are part of this view
 };
 
 struct gs_trainer_s {
@@ -1336,6 +1347,7 @@ struct gs_trainer_s {
     gs::Profiler prof;
 
     ~gs_trainer_s() {
+        dfree(saved);
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (optim) gs_optim_destroy(optim);
@@ -1357,6 +1369,47 @@ struct gs_trainer_s {
 
     gs_status enqueue_step(hipStream_t st, const uint32_t* ids, uint32_t k, bool batched,
                            bool profile = false);
+
+    // Training-state snapshot (raw, m, v of every optimizer class and the
+    // Adam step state) in one device buffer; restore also re-activates.
+    float* saved = nullptr;
+    size_t saved_floats = 0;
+    gs_status save_state() {
+        hipStream_t st = ctx->stream;
+        size_t total = 0;
+        for (const auto& c : optim->classes) total += 3 * (size_t)c.n;
+        if (total != saved_floats) {
+            dfree(saved);
+            saved = nullptr;
+            GS_CHECK(dalloc(&saved, total + sizeof(gs::AdamState) / 4 + 1, "training-state snapshot"));
+            saved_floats = total;
+        }
+        size_t off = 0;
+        for (const auto& c : optim->classes) {
+            for (const float* src : {c.raw, c.m, c.v}) {
+                GS_HIP(hipMemcpyAsync(saved + off, src, 4 * (size_t)c.n, hipMemcpyDeviceToDevice, st));
+                off += c.n;
+            }
+        }
+        GS_HIP(hipMemcpyAsync(saved + off, optim->state, sizeof(gs::AdamState), hipMemcpyDeviceToDevice, st));
+        GS_HIP(hipStreamSynchronize(st));
+        return GS_OK;
+    }
+    gs_status restore_state() {
+        if (saved == nullptr) return fail(GS_ERR_INVALID, "no saved training state");
+        hipStream_t st = ctx->stream;
+        size_t off = 0;
+        for (const auto& c : optim->classes) {
+            for (float* dst : {c.raw, c.m, c.v}) {
+                GS_HIP(hipMemcpyAsync(dst, saved + off, 4 * (size_t)c.n, hipMemcpyDeviceToDevice, st));
+                off += c.n;
+            }
+        }
+        GS_HIP(hipMemcpyAsync(optim->state, saved + off, sizeof(gs::AdamState), hipMemcpyDeviceToDevice, st));
+        GS_CHECK(gs_optim_encode_activate(optim, st));
+        GS_HIP(hipStreamSynchronize(st));
+        return GS_OK;
+    }
 };
 
 gs_status gs_trainer_s::enqueue_step(hipStream_t st, const uint32_t* ids, uint32_t k, bool batched,
@@ -1423,9 +1476,12 @@ gs_status gs_trainer_create(gs_ctx ctx, const gs_train_config* cfg, const gs_sce
         GS_REQUIRE(tv.width > 0 && tv.height > 0 && tv.target_rgba != nullptr, "bad view");
         GS_REQUIRE(toff + (size_t)tv.width * tv.height < (1ull << 32), "targets exceed 2^32 pixels");
         GS_CHECK(check_camera(&tv.camera));
-        t->views.push_back(TrainViewDev{tv.width, tv.height, tv.camera, toff});
+        GS_REQUIRE(tv.num_surfels <= init->n, "view num_surfels exceeds the scene");
+        t->views.push_back(TrainViewDev{tv.width, tv.height, tv.camera, toff,
+                                        tv.num_surfels ? tv.num_surfels : init->n});
         toff += (size_t)tv.width * tv.height;
-        max_tiles = std::max(max_tiles, div_up(tv.width, kTile) * div_up(tv.height, kTile));
+        // Room for the smallest work tile (8 px).
+        max_tiles = std::max(max_tiles, div_up(tv.width, 8) * div_up(tv.height, 8));
         max_px = std::max(max_px, tv.width * tv.height);
         t->max_w = std::max(t->max_w, tv.width);
         t->max_h = std::max(t->max_h, tv.height);
@@ -1533,8 +1589,9 @@ gs_status gs_trainer_step(gs_trainer t, void* stream, const uint32_t* view_ids, 
         const TrainViewDev& tv = t->views[view_ids[j]];
         gs_raster_camera c = tv.camera;
         c.sh_degree = t->sh_degree;
-        cams.push_back(make_cam(c, tv.width, tv.height));
+        cams.push_back(make_cam(c, tv.width, tv.height, t->core.shape.tile));
         cams.back().target_base = (uint32_t)tv.target_offset;
+        cams.back().n_surfels = tv.num_surfels;
     }
     t->last_ids.assign(view_ids, view_ids + k);
     t->last_batched = batched;
@@ -1605,60 +1662,140 @@ gs_status gs_trainer_set_schedule(gs_trainer t, uint32_t digit_bits, uint32_t bl
     return GS_OK;
 }
 
+static bool tiling_supported(uint32_t tile, uint32_t fb, uint32_t bb) {
+    const bool f = (tile == 8 && (fb == 64 || fb == 128)) || (tile == 16 && (fb == 128 || fb == 256)) ||
+                   (tile == 32 && (fb == 256 || fb == 512));
+    const bool b = (tile == 8 && (bb == 64 || bb == 128)) || (tile == 16 && (bb == 64 || bb == 128)) ||
+                   (tile == 32 && (bb == 128 || bb == 256));
+    return f && b;
+}
+
+gs_status gs_trainer_set_tiling(gs_trainer t, uint32_t tile_size, uint32_t fwd_batch, uint32_t bwd_batch) {
+    GS_REQUIRE(t != nullptr, "trainer is null");
+    GS_REQUIRE(tiling_supported(tile_size, fwd_batch, bwd_batch),
+               "unsupported (tile_size, fwd_batch, bwd_batch); see gs_hip.h");
+    GS_REQUIRE(tile_size == (uint32_t)kTile || t->core.binning == GS_BINNING_TIGHT,
+               "work tiles other than 16 px need GS_BINNING_TIGHT");
+    t->core.shape = gs::RasterShape{tile_size, fwd_batch, bwd_batch};
+    drop_graph(t);
+    return GS_OK;
+}
+
 gs_status gs_trainer_autotune(gs_trainer t, const uint32_t* view_ids, uint32_t n_ids, uint32_t reps,
                               uint32_t* chosen) {
     GS_REQUIRE(t && view_ids, "null argument");
     GS_REQUIRE(reps >= 1 && reps <= 16, "reps must be in 1..16");
     // Candidate schedules; none changes the step's result (the sorts are
-    // stable, scheduling only reorders float atomics).
+    // stable, exact culling makes the per-pixel compositing sequence the same
+    // for every work-tile size, and scheduling only reorders float atomics).
     struct Cand {
         uint32_t digit_bits, blocks_per_cu;  // blocks_per_cu 0: one workgroup per tile
+        uint32_t tile, fwd_batch, bwd_batch;
     };
-    static const Cand cands[] = {{8, 8}, {8, 4}, {8, 0}, {4, 8}, {4, 4}, {4, 0}};
-    auto apply = [&](const Cand& c) { (void)gs_trainer_set_schedule(t, c.digit_bits, c.blocks_per_cu); };
+    std::vector<Cand> cands;
+    const bool tight = t->core.binning == GS_BINNING_TIGHT;
+    static const uint32_t tilings[][3] = {{16, 256, 128}, {16, 128, 64}, {32, 256, 128},
+                                          {32, 512, 256}, {8, 128, 128}, {8, 64, 64}};
+    static const uint32_t queues[] = {8, 4, 0};
+    // Stage 1: work tiling x raster scheduling at 8-bit digits; stage 2: the
+    // digit width at the stage-1 winner.
+    for (const auto& tl : tilings) {
+        if (tl[0] != (uint32_t)kTile && !tight) continue;
+        for (uint32_t q : queues) cands.push_back(Cand{8, q, tl[0], tl[1], tl[2]});
+    }
+    const gs::RasterShape shape0 = t->core.shape;
+    const uint32_t digit0 = t->core.digit_bits, queue0 = t->core.tile_queue ? t->core.queue_blocks : 0;
+    // The candidates run real steps; the training state (raw parameters,
+    // Adam moments and step count) is saved first and restored afterwards,
+    // so tuning does not advance training.
+    GS_CHECK(t->save_state());
+    auto apply = [&](const Cand& c) {
+        (void)gs_trainer_set_schedule(t, c.digit_bits, c.blocks_per_cu);
+        (void)gs_trainer_set_tiling(t, c.tile, c.fwd_batch, c.bwd_batch);
+    };
     hipStream_t st = t->ctx->stream;
-    constexpr size_t kCands = sizeof(cands) / sizeof(cands[0]);
     constexpr uint32_t kRounds = 2;
     std::vector<hipEvent_t> ev(reps + 1, nullptr);
     for (auto& e : ev) GS_HIP(hipEventCreate(&e));
     gs_status s = GS_OK;
-    // Candidates interleaved over two rounds (drift hits all alike); each
-    // step timed on its own; a candidate's score is its fastest step.
-    float best_ms[kCands];
-    for (size_t i = 0; i < kCands; ++i) best_ms[i] = 3.4e38f;
-    for (uint32_t round = 0; round < kRounds && s == GS_OK; ++round) {
-        for (size_t i = 0; i < kCands && s == GS_OK; ++i) {
-            apply(cands[i]);
-            // One settling step (first use of the schedule), then `reps` timed.
-            s = gs_trainer_step(t, st, view_ids, n_ids, GS_STEP_BATCHED);
-            if (s == GS_OK && hipEventRecord(ev[0], st) != hipSuccess) s = fail(GS_ERR_HIP, "hipEventRecord");
-            for (uint32_t r = 0; r < reps && s == GS_OK; ++r) {
+    // Candidates of a stage interleaved over two rounds (drift hits all
+    // alike); each step timed on its own; a candidate's score is its fastest
+    // step. A candidate whose binning overflows the entry capacity is out.
+    auto run_stage = [&](const std::vector<Cand>& cs, size_t* best) {
+        std::vector<float> best_ms(cs.size(), 3.4e38f);
+        std::vector<bool> overflowed(cs.size(), false);
+        for (uint32_t round = 0; round < kRounds && s == GS_OK; ++round) {
+            for (size_t i = 0; i < cs.size() && s == GS_OK; ++i) {
+                apply(cs[i]);
+                // One settling step (first use of the schedule), then `reps` timed.
                 s = gs_trainer_step(t, st, view_ids, n_ids, GS_STEP_BATCHED);
-                if (s == GS_OK && hipEventRecord(ev[r + 1], st) != hipSuccess)
-                    s = fail(GS_ERR_HIP, "hipEventRecord");
-            }
-            if (s != GS_OK) break;
-            if (hipEventSynchronize(ev[reps]) != hipSuccess) {
-                s = fail(GS_ERR_HIP, "autotune timing");
-                break;
-            }
-            for (uint32_t r = 0; r < reps; ++r) {
-                float ms = 0.0f;
-                (void)hipEventElapsedTime(&ms, ev[r], ev[r + 1]);
-                best_ms[i] = std::min(best_ms[i], ms);
+                if (s == GS_OK && hipEventRecord(ev[0], st) != hipSuccess) s = fail(GS_ERR_HIP, "hipEventRecord");
+                for (uint32_t r = 0; r < reps && s == GS_OK; ++r) {
+                    s = gs_trainer_step(t, st, view_ids, n_ids, GS_STEP_BATCHED);
+                    if (s == GS_OK && hipEventRecord(ev[r + 1], st) != hipSuccess)
+                        s = fail(GS_ERR_HIP, "hipEventRecord");
+                }
+                if (s != GS_OK) break;
+                if (hipEventSynchronize(ev[reps]) != hipSuccess) {
+                    s = fail(GS_ERR_HIP, "autotune timing");
+                    break;
+                }
+                uint32_t of = 0;
+                if (hipMemcpy(&of, t->core.overflow, 4, hipMemcpyDeviceToHost) != hipSuccess) {
+                    s = fail(GS_ERR_HIP, "autotune overflow read");
+                    break;
+                }
+                if (of) {
+                    overflowed[i] = true;
+                    const uint32_t zero = 0;
+                    if (hipMemcpy(t->core.overflow, &zero, 4, hipMemcpyHostToDevice) != hipSuccess) {
+                        s = fail(GS_ERR_HIP, "autotune overflow reset");
+                        break;
+                    }
+                }
+                for (uint32_t r = 0; r < reps; ++r) {
+                    float ms = 0.0f;
+                    (void)hipEventElapsedTime(&ms, ev[r], ev[r + 1]);
+                    best_ms[i] = std::min(best_ms[i], ms);
+                }
             }
         }
+        size_t b = cs.size();
+        for (size_t i = 0; i < cs.size(); ++i)
+            if (!overflowed[i] && (b == cs.size() || best_ms[i] < best_ms[b])) b = i;
+        *best = b;
+    };
+    size_t b1 = 0, b2 = 0;
+    run_stage(cands, &b1);
+    std::vector<Cand> stage2;
+    if (s == GS_OK && b1 < cands.size()) {
+        for (uint32_t d : {8u, 4u}) {
+            Cand c = cands[b1];
+            c.digit_bits = d;
+            stage2.push_back(c);
+        }
+        run_stage(stage2, &b2);
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
-    GS_CHECK(s);
-    size_t best_i = 0;
-    for (size_t i = 1; i < kCands; ++i)
-        if (best_ms[i] < best_ms[best_i]) best_i = i;
-    apply(cands[best_i]);
+    const gs_status rs = t->restore_state();
+    if (s == GS_OK) s = rs;
+    if (s != GS_OK || b1 >= cands.size() || b2 >= stage2.size()) {
+        // Failure, or every candidate overflowed: back to the schedule on entry.
+        const uint32_t cus = (uint32_t)std::max(1, t->ctx->props.multiProcessorCount);
+        (void)gs_trainer_set_schedule(t, digit0, queue0 / cus);
+        (void)gs_trainer_set_tiling(t, shape0.tile, shape0.fwd_batch, shape0.bwd_batch);
+        GS_CHECK(s);
+        return fail(GS_ERR_OVERFLOW, "autotune: every candidate overflowed the entry capacity");
+    }
+    const Cand& w = stage2[b2];
+    apply(w);
     t->tuned = true;
     if (chosen) {
-        chosen[0] = cands[best_i].digit_bits;
-        chosen[1] = cands[best_i].blocks_per_cu;
+        chosen[0] = w.digit_bits;
+        chosen[1] = w.blocks_per_cu;
+        chosen[2] = w.tile;
+        chosen[3] = w.fwd_batch;
+        chosen[4] = w.bwd_batch;
     }
     return GS_OK;
 }
@@ -1683,7 +1820,9 @@ gs_status gs_trainer_render(gs_trainer t, uint32_t view, float* rgba) {
     hipStream_t st = t->ctx->stream;
     gs_raster_camera c = tv.camera;
     c.sh_degree = t->sh_degree;
-    GS_CHECK(t->core.set_views(st, {make_cam(c, tv.width, tv.height)}, true));
+    CamParams cp = make_cam(c, tv.width, tv.height, t->core.shape.tile);
+    cp.n_surfels = tv.num_surfels;
+    GS_CHECK(t->core.set_views(st, {cp}, true));
     GS_CHECK(t->core.forward(st, 0, 1, t->n, t->positions, t->scales, t->quats, t->opacities, t->sh));
     GS_HIP(hipStreamSynchronize(st));
     GS_HIP(hipMemcpy(rgba, t->core.out_color, (size_t)tv.width * tv.height * 16, hipMemcpyDeviceToHost));

@@ -141,6 +141,50 @@ def sphere_views(count: int, seed: int, sizes, radius: float = 3.0):
     return out
 
 
+def _quat_mul(a, b):
+    """Hamilton product of xyzw quaternions."""
+    ax, ay, az, aw = a
+    bx, by, bz, bw = b
+    return np.array([aw * bx + ax * bw + ay * bz - az * by,
+                     aw * by - ax * bz + ay * bw + az * bx,
+                     aw * bz + ax * by - ay * bx + az * bw,
+                     aw * bw - ax * bx - ay * by - az * bz])
+
+
+def walk_views(count: int, seed: int, size, step_m: float = 0.01, step_deg: float = 0.5,
+               radius: float = 3.0):
+    """A smoothly moving camera: a start pose on the sphere of `radius`
+    looking at the origin, then an SE(3) random walk, each frame moving
+    `step_m` along a random direction and turning `step_deg` about a random
+    camera-frame axis (the streaming-frame configuration: 1 cm and 0.5 deg
+    per frame). size = (w, h) of every frame."""
+    rng = np.random.default_rng(seed)
+    yaw = rng.uniform(0.0, 2 * np.pi)
+    pitch = rng.uniform(-np.pi / 6, np.pi / 6)
+    eye = radius * np.array([np.cos(pitch) * np.sin(yaw), np.sin(pitch), np.cos(pitch) * np.cos(yaw)])
+    q = look_at(eye)
+    w, h = size
+    out = []
+    for _ in range(count):
+        out.append(ViewSpec(w, h, eye.copy(), q.copy(), focal_for(h)))
+        d = rng.normal(size=3)
+        eye = eye + step_m * d / np.linalg.norm(d)
+        axis = rng.normal(size=3)
+        axis /= np.linalg.norm(axis)
+        half = 0.5 * np.deg2rad(step_deg)
+        q = _quat_mul(q, np.concatenate([np.sin(half) * axis, [np.cos(half)]]))
+        q /= np.linalg.norm(q)
+    return out
+
+
+def frame_surfel_counts(count: int, base: int, growth: float = 0.1):
+    """Surfels seen by frame k of a growing scene: base plus
+    round(growth * base) newly densified surfels per frame (frame k renders
+    surfels [0, base + k round(growth base)))."""
+    step = int(round(growth * base))
+    return [base + k * step for k in range(count)]
+
+
 def uneven_sizes(count: int, seed: int):
     """16:9 sizes between 640x360 and 1920x1080."""
     rng = np.random.default_rng(seed)
@@ -161,5 +205,7 @@ def config_scene(index: int, n: int | None = None) -> SceneArrays:
     if index == 3:
         return make_scene(n or 1_000_000, 3, -5.0, 1.0, seed)
     if index == 4:
-        return make_scene(n or 500_000, 3, -5.0, 0.5, seed)
+        # The base scene plus 15 frames of 10 % densification (the scene at
+        # the last of 16 frames); frame k renders a prefix of it.
+        return make_scene(n or frame_surfel_counts(16, 500_000)[-1], 3, -5.0, 0.5, seed)
     raise ValueError(index)

@@ -1,8 +1,14 @@
-"""A multi-view training problem of uneven resolutions (the shape of the
-8-view, 360p..1080p sphere-camera configuration, scaled down 10x so the FP64
-oracle evaluates it in seconds), and its FP64 loss and gradient: the sum over
-views of L1 + D-SSIM (each view's mean over its own pixels), as the batched
-trainer step sums per-view gradients (trainer.rs:1139-1332).
+"""Multi-view training problems and their FP64 loss and gradient: the sum
+over views of L1 + D-SSIM (each view's mean over its own pixels), as the
+batched trainer step sums per-view gradients (trainer.rs:1139-1332).
+
+problem(): 8 views of uneven resolutions (the shape of the 8-view,
+360p..1080p sphere-camera configuration, scaled down 10x so the FP64 oracle
+evaluates it in seconds).
+frames_problem(): 16 frames of a moving camera (SE(3) walk, 1 cm / 0.5 deg
+per frame) over a growing scene, frame k rendering the first
+base + k * 10 % base surfels (the streaming-frame configuration, 720p and
+500k base surfels, scaled down 10x in size and ~800x in surfels).
 """
 
 import numpy as np
@@ -25,6 +31,21 @@ def problem(n=1200, deg=3, seed=17):
     return sc, tgt, specs
 
 
+def frames_problem(base=500, frames=16, size=(128, 72), seed=29):
+    from splattar_amd import scenes
+    counts = scenes.frame_surfel_counts(frames, base)
+    sc = scenes.make_scene(counts[-1], 3, -3.4, 0.4, seed)
+    tgt = scenes.jittered(sc, 0.01, seed ^ 1)
+    specs = scenes.walk_views(frames, seed + 3, size)
+    return sc, tgt, specs, counts
+
+
+def subset(s64, m):
+    """The first m surfels of an FP64 scene."""
+    return oracle.Scene(s64.pos[:m], s64.scales[:m], s64.quat[:m], s64.opacity[:m], s64.sh[:m],
+                        s64.sh_degree)
+
+
 def scene64(sc):
     n = sc.n
     ncoef = (sc.sh_degree + 1) ** 2
@@ -39,25 +60,64 @@ def camera64(sp):
                          float(np.float32(sp.focal)), sp.width, sp.height)
 
 
-def targets(tgt, specs):
-    """FP64 renders of the target scene, rounded to f32 (what the GPU holds)."""
+def targets(tgt, specs, counts=None, offset=0.0):
+    """FP64 renders of the target scene (view k: its first counts[k]
+    surfels) plus `offset` per channel, rounded to f32 (what the GPU holds).
+
+    An offset above every rendered value (2.0; colours here stay below 1.6)
+    keeps sign(render - target) of the L1 term at -1 for every pixel, so
+    f32 and FP64 cannot take opposite signs where the two agree to ~1e-7,
+    while the target's structure (what SSIM sees) is unchanged."""
     s = scene64(tgt)
-    return [oracle.render(s, camera64(sp))["color"].astype(np.float32) for sp in specs]
+    counts = counts or [s.pos.shape[0]] * len(specs)
+    return [(oracle.render(subset(s, m), camera64(sp))["color"] + offset).astype(np.float32)
+            for sp, m in zip(specs, counts)]
 
 
-def total_loss(s64, specs, tgts, grad=False):
+def l1_sign_margin(s64, specs, tgts, counts=None):
+    """Largest render - target over every pixel and channel (FP64): below
+    zero by a margin means the L1 sign is the same for any rounding."""
+    n = s64.pos.shape[0]
+    counts = counts or [n] * len(specs)
+    return max(float((oracle.render(subset(s64, m), camera64(sp))["color"] - t).max())
+               for sp, t, m in zip(specs, tgts, counts))
+
+
+def marginal_mask(s64, specs, counts=None, rel=1e-4):
+    """Surfels in a forward decision within relative band rel of a threshold
+    in some view (oracle.marginal_decisions), and the number of such
+    decisions. 1e-4 is several times the relative error of the f32 path's
+    alpha (its screen-branch exponent carries the f32 rounding of the
+    projected centre)."""
+    n = s64.pos.shape[0]
+    counts = counts or [n] * len(specs)
+    mask = np.zeros(n, bool)
+    total = 0
+    for sp, m in zip(specs, counts):
+        k, f = oracle.marginal_decisions(subset(s64, m), camera64(sp), rel)
+        total += k
+        mask[:m] |= f.astype(bool)
+    return mask, total
+
+
+def total_loss(s64, specs, tgts, grad=False, counts=None):
+    """Summed loss and FP64 gradient; view k sees the first counts[k]
+    surfels (the others get no gradient from it)."""
     total = 0.0
     acc = None
-    for sp, t in zip(specs, tgts):
+    n = s64.pos.shape[0]
+    counts = counts or [n] * len(specs)
+    for sp, t, m in zip(specs, tgts, counts):
         cam = camera64(sp)
-        img = oracle.render(s64, cam)["color"]
+        sv = s64 if m == n else subset(s64, m)
+        img = oracle.render(sv, cam)["color"]
         loss, dl, _, _ = oracle.l1_ssim(img, t.astype(np.float64), LAM, DSSIM_SCALE, grad)
         total += loss
         if grad:
-            g = oracle.gradients(s64, cam, dl.reshape(-1, 3))
+            g = oracle.gradients(sv, cam, dl.reshape(-1, 3))
             if acc is None:
-                acc = g
-            else:
-                for f in ("pos", "scales", "quat", "opacity", "sh"):
-                    setattr(acc, f, getattr(acc, f) + getattr(g, f))
+                acc = oracle.Gradients(np.zeros((n, 3)), np.zeros((n, 2)), np.zeros((n, 4)), np.zeros(n),
+                                       np.zeros(s64.sh.shape))
+            for f in ("pos", "scales", "quat", "opacity", "sh"):
+                getattr(acc, f)[:m] += getattr(g, f)
     return total, acc

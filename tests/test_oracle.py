@@ -276,6 +276,33 @@ def test_weights_sum_to_one_minus_transmittance():
     assert not np.any(g.pos) and not np.any(g.opacity) and g.focal == 0.0
 
 
+def test_marginal_decisions_finds_planted_threshold_crossings():
+    """oracle.marginal_decisions (a test aid): a surfel whose alpha at one
+    pixel is planted 3e-6 above 1/255 is reported at rel 1e-5 and not at
+    1e-6; the same surfel with 1 % more opacity has no marginal decision;
+    two overlapping surfels at the same depth (equal f32 keys) are both
+    reported."""
+    W = H = 32
+    f = 40.0
+    cam = oracle.Camera(np.zeros(3), np.array([0, 0, 0, 1.0]), f, W, H)
+    base = _single_scene([[0.05, -0.02, -3.0]], [[0.2, 0.1]], [0.7], [[0.4, -0.3, 1.0]])
+    alpha = oracle.render(base, cam, mode=0)["alpha"]
+    ys, xs = np.nonzero((alpha > 0.02) & (alpha < 0.2))
+    a_p = alpha[ys[0], xs[0]]
+    op = 0.7 / 255.0 / a_p * (1 + 3e-6)
+    planted = _single_scene([[0.05, -0.02, -3.0]], [[0.2, 0.1]], [op], [[0.4, -0.3, 1.0]])
+    k, flags = oracle.marginal_decisions(planted, cam, 1e-5)
+    assert k >= 1 and flags[0] == 1
+    k6, _ = oracle.marginal_decisions(planted, cam, 1e-6)
+    assert k6 == 0
+    clear = _single_scene([[0.05, -0.02, -3.0]], [[0.2, 0.1]], [op * 1.01], [[0.4, -0.3, 1.0]])
+    assert oracle.marginal_decisions(clear, cam, 1e-5)[0] == 0
+    tie = _single_scene([[0.0, 0.0, -3.0], [0.1, 0.0, -3.0], [0.0, 0.0, -5.0]], [[0.2, 0.2]] * 3,
+                        [0.5] * 3, [[0.1, 0.2, 0.3]] * 3)
+    k, flags = oracle.marginal_decisions(tie, cam, 1e-9)
+    assert k >= 1 and list(flags) == [1, 1, 0]
+
+
 def test_near_culled_scene_is_empty():
     cam = oracle.Camera(np.zeros(3), np.array([0, 0, 0, 1.0]), 20.0, 16, 16)
     scene = _single_scene([[0, 0, 1.0], [0, 0, -0.01]], [[0.3, 0.3]] * 2, [0.9, 0.9], [[0, 0, 0]] * 2)
