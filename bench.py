@@ -6,6 +6,12 @@ Workload (default): 1M surfels, SH degree 3, heavy-tailed log-normal scales
 sphere of radius 3, loss 0.8 L1 + 0.2 (1 - SSIM) against targets rendered
 from a jittered copy of the scene, one Adam step per training step.
 
+--workload c4: the streaming-frame batch instead -- 16 frames of 1280x720
+from a camera on an SE(3) random walk (1 cm / 0.5 deg per frame) over a scene
+of 500k surfels that grows 10 % per frame (frame k renders its first
+500k + 50k k surfels; 1.25M in all), every frame in one graph launch per
+step, targets rendered from a jittered copy.
+
 A "step" = project, scan, emit keys, 64-bit radix sort, tile ranges,
 rasterize forward, L1 + D-SSIM, rasterize backward, geometry backward and
 fused Adam over the 4 views. `value` is steps/s of the batched step captured
@@ -34,6 +40,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "train steps/s, 1M surfels @1080p, batched vs plain at equal loss; % HBM peak"
+METRIC_C4 = "train steps/s, 16-frame 720p streaming batch over a growing 500k..1.25M-surfel scene"
 HBM_PEAK_GBS = 8000.0
 # Non-packed VALU issue rate: 256 CUs x 4 SIMDs, one wave64 instruction per
 # SIMD every 4 cycles, 2.4 GHz (packed FP32 is not used).
@@ -75,6 +82,9 @@ def log(msg):
 def parse():
     p = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--workload", choices=["c3", "c4"], default="c3",
+                   help="c3: 1M surfels, 4 x 1080p views per step (default); c4: 16-frame 720p "
+                        "streaming batch over a growing 500k..1.25M-surfel scene")
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--n", type=int, default=1_000_000, help="surfels")
@@ -115,32 +125,35 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 5}))
 
 
-def stage_bytes(n, V, E, P, T, sort_passes, S=232):
+def stage_bytes(n, V, E, P, T, sort_passes, S=232, nv=None):
     """Algorithmic HBM bytes per stage for one step over V views (E, P, T
-    summed over the views): compulsory traffic only."""
+    summed over the views; nv = sum over the views of the surfels each sees,
+    V n by default): compulsory traffic only."""
+    nv = V * n if nv is None else nv
     return {
-        "project": V * n * (S + 60),
-        "scan": 8 * V * n,
-        "emit_keys": 24 * V * n + 12 * E,
+        "project": nv * (S + 60),
+        "scan": 8 * nv,
+        "emit_keys": 24 * nv + 12 * E,
         "sort": (8 + 24 * sort_passes) * E,
         "tile_ranges": 8 * E + 8 * T,
         "raster_fwd": 56 * E + 20 * P,
         "l1_ssim": 36 * P,
-        "grad_clear": 80 * V * n,
-        "raster_bwd": 56 * E + 20 * P + 52 * V * n,
-        "geom_bwd": V * n * (2 * S + 52),
+        "grad_clear": 80 * nv,
+        "raster_bwd": 56 * E + 20 * P + 52 * nv,
+        "geom_bwd": nv * (2 * S + 52),
         "adam": 7 * S * n,
     }
 
 
-def cpu_baseline(scene, spec, crop, views_per_step):
+def cpu_baseline(scene, spec, crop, views_per_step, n_view=None):
+    """n_view: surfels the sampled view renders (a prefix; all by default)."""
     import oracle
-    n = scene.n
+    n = scene.n if n_view is None else n_view
     deg = scene.sh_degree
     ncoef = (deg + 1) ** 2
-    s64 = oracle.Scene(scene.positions.astype(np.float64), scene.scales.astype(np.float64),
-                       scene.quats.astype(np.float64), scene.opacities.astype(np.float64),
-                       scene.sh.astype(np.float64).reshape(n, ncoef, 3), deg)
+    s64 = oracle.Scene(scene.positions[:n].astype(np.float64), scene.scales[:n].astype(np.float64),
+                       scene.quats[:n].astype(np.float64), scene.opacities[:n].astype(np.float64),
+                       scene.sh[:n].astype(np.float64).reshape(n, ncoef, 3), deg)
     W, H = spec.width, spec.height
     cw, ch = crop
     x0, y0 = (W - cw) // 2, (H - ch) // 2
@@ -160,8 +173,8 @@ def cpu_baseline(scene, spec, crop, views_per_step):
     _, dl, _, _ = oracle.l1_ssim(img, tgt, 0.2, 1.0)
     oracle.gradients(s64, cam((x0, y0, cw, ch)), dl.reshape(-1, 3))
     t_crop = time.perf_counter() - t0
-    raw = np.zeros(n * 58, np.float32)
-    g = np.full(n * 58, 1e-3, np.float32)
+    raw = np.zeros(scene.n * 58, np.float32)
+    g = np.full(scene.n * 58, 1e-3, np.float32)
     m = np.zeros_like(raw)
     v = np.zeros_like(raw)
     t0 = time.perf_counter()
@@ -173,7 +186,7 @@ def cpu_baseline(scene, spec, crop, views_per_step):
     return {"value": 1.0 / t_step, "unit": "steps/s", "cores": oracle.num_threads(), "kind": "port",
             "sample": (f"FP64 oracle (tile-binned, OpenMP) on view 0: fixed per-view cost {t_fixed:.2f}s + "
                        f"centred {cw}x{ch} crop fwd+L1/SSIM+bwd {t_crop:.2f}s scaled x{scale:.1f} to "
-                       f"{W}x{H}; x{views_per_step} views + f32 Adam over {n}x58 params "
+                       f"{W}x{H}; x{views_per_step} views + f32 Adam over {scene.n}x58 params "
                        f"{t_adam:.2f}s = {t_step:.1f}s/step"),
             "sample_seconds": round(t_fixed + t_crop + t_adam, 2)}
 
@@ -194,10 +207,20 @@ def main():
     ctx = sa.GpuContext(0)
     name, cus = ctx.device_info()
     log(f"device {name}, {cus} CUs")
-    n, W, H, V = a.n, a.width, a.height, a.views_per_step
-    scene = scenes.config_scene(3, n)
-    target_scene = scenes.jittered(scene, 0.01, (scenes.CONFIG_SEED + 3) ^ 1)
-    specs = scenes.sphere_views(a.num_views, scenes.CONFIG_SEED + 3, [(W, H)])
+    if a.workload == "c4":
+        W, H, V = 1280, 720, 16
+        a.num_views = 16
+        counts = scenes.frame_surfel_counts(16, 500_000)
+        n = counts[-1]
+        scene = scenes.config_scene(4, n)
+        target_scene = scenes.jittered(scene, 0.01, (scenes.CONFIG_SEED + 4) ^ 1)
+        specs = scenes.walk_views(16, scenes.CONFIG_SEED + 4, (W, H))
+    else:
+        n, W, H, V = a.n, a.width, a.height, a.views_per_step
+        scene = scenes.config_scene(3, n)
+        target_scene = scenes.jittered(scene, 0.01, (scenes.CONFIG_SEED + 3) ^ 1)
+        specs = scenes.sphere_views(a.num_views, scenes.CONFIG_SEED + 3, [(W, H)])
+        counts = [n] * a.num_views
     deg = scene.sh_degree
 
     # Targets rendered on the GPU from the jittered scene (setup, untimed).
@@ -207,8 +230,8 @@ def main():
     r.upload_scene(target_scene.positions, target_scene.scales, target_scene.quats,
                    target_scene.opacities, target_scene.sh)
     targets, ents = [], []
-    for sp in specs:
-        r.forward(sa.Camera(sp.center, sp.quat, sp.focal, deg), n)
+    for sp, m in zip(specs, counts):
+        r.forward(sa.Camera(sp.center, sp.quat, sp.focal, deg), m)
         ents.append(r.entry_count())
         targets.append(r.out_color.read().copy())
     r.close()
@@ -222,7 +245,7 @@ def main():
                          digit_bits=a.digit_bits,
                          binning={"tight": GS_BINNING_TIGHT, "presort": GS_BINNING_PRESORT,
                                   "sort64": GS_BINNING_SORT64}[a.binning])
-    views = [(sa.Camera(sp.center, sp.quat, sp.focal, deg), t) for sp, t in zip(specs, targets)]
+    views = [(sa.Camera(sp.center, sp.quat, sp.focal, deg), t, m) for sp, t, m in zip(specs, targets, counts)]
     del targets
 
     def ids_for(step):
@@ -302,7 +325,7 @@ def main():
             ctx.synchronize()
             secs = time.perf_counter() - t0
             ps = []
-            for v, (_, tgt) in enumerate(views):
+            for v, (_, tgt, _) in enumerate(views):
                 img = t.render(v)[..., :3].astype(np.float64)
                 mse = float(np.mean((img - tgt[..., :3].astype(np.float64)) ** 2))
                 ps.append(10.0 * np.log10(1.0 / max(mse, 1e-20)))
@@ -325,23 +348,29 @@ def main():
     import math
     tile_bits = max(1, math.ceil(math.log2(T)))
     passes = math.ceil((32 + tile_bits) / schedule["digit_bits"])
-    sb = stage_bytes(n, V, entries_step, P, T, passes)
+    nv = sum(counts[j] for j in ids_for(0))
+    sb = stage_bytes(n, V, entries_step, P, T, passes, nv=nv)
     dom = max(stage_ms, key=stage_ms.get)
     achieved = sb[dom] / (stage_ms[dom] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
                 "ms_per_launch": round(stage_ms[dom], 4), "algorithmic_bytes": int(sb[dom])}
-    prof = profile_figures(dom)
+    # The committed counter summary is of the default (C3) workload.
+    prof = profile_figures(dom) if a.workload == "c3" else None
     if prof is not None:
         roofline["traffic"] = prof.pop("traffic", None)
         roofline["profile"] = prof
     total_bytes = sum(sb.values())
     result = {
-        "metric": METRIC, "value": round(value, 4), "unit": "steps/s", "n_gpus": 1, "steps": a.steps,
+        "metric": METRIC if a.workload == "c3" else METRIC_C4, "value": round(value, 4), "unit": "steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(ms_graph, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"C3: {n} surfels SH3 heavy-tailed scales, {V}x{W}x{H} views/step, "
-                               f"L1+SSIM, Adam", "surfels": n, "views_per_step": V, "width": W,
+        "config": {"workload": (f"C3: {n} surfels SH3 heavy-tailed scales, {V}x{W}x{H} views/step, "
+                                f"L1+SSIM, Adam") if a.workload == "c3" else
+                               (f"C4: {V} frames {W}x{H}, SE(3) walk 1 cm / 0.5 deg per frame, "
+                                f"{counts[0]}..{counts[-1]} surfels per frame (+10 %/frame), one graph "
+                                f"launch per step, L1+SSIM, Adam"),
+                   "surfels": n, "views_per_step": V, "width": W,
                    "height": H, "entries_per_step": entries_step, "digit_bits": schedule["digit_bits"],
                    "raster_blocks_per_cu": schedule["blocks_per_cu"], "tile_size": schedule["tile_size"],
                    "fwd_batch": schedule["fwd_batch"], "bwd_batch": schedule["bwd_batch"],
@@ -362,7 +391,7 @@ def main():
     if not a.no_cpu_baseline:
         cw, chh = (int(x) for x in a.cpu_crop.split("x"))
         log("timing CPU oracle baseline ...")
-        result["cpu_baseline"] = cpu_baseline(scene, specs[0], (cw, chh), V)
+        result["cpu_baseline"] = cpu_baseline(scene, specs[0], (cw, chh), V, counts[0])
     else:
         result["cpu_baseline"] = None
     ctx.close()

@@ -375,6 +375,57 @@ def test_growing_scene_frame_batch_matches_fp64(gpu, tiling):
     tr.close()
 
 
+def test_streaming_frame_batch_full_size(gpu):
+    """The streaming-frame configuration at full size: 16 frames of
+    1280x720 (SE(3) walk), 500k surfels growing 10 % per frame (1.25M), all
+    frames in one graph launch with the autotuned schedule. With learning
+    rates 0 (the parameters stay put), the graph step's gradients equal the
+    plain per-frame step's (same math, per-frame launches, default
+    schedule) per class normwise to 1e-5 (float-atomic order), losses to
+    1e-6; no frame overflows its entry capacity."""
+    import splattar_amd as sa
+    from splattar_amd import scenes
+    counts = scenes.frame_surfel_counts(16, 500_000)
+    n = counts[-1]
+    sc = scenes.config_scene(4, n)
+    tgt = scenes.jittered(sc, 0.01, (scenes.CONFIG_SEED + 4) ^ 1)
+    specs = scenes.walk_views(16, scenes.CONFIG_SEED + 4, (1280, 720))
+    announce("streaming batch full size: targets")
+    r = sa.Rasterizer(gpu, n, 1280, 720, 40_000_000)
+    r.upload_scene(tgt.positions, tgt.scales, tgt.quats, tgt.opacities, tgt.sh)
+    targets = []
+    for sp, m in zip(specs, counts):
+        r.forward(sa.Camera(sp.center, sp.quat, sp.focal, 3), m)
+        targets.append(r.out_color.read().copy())
+    r.close()
+    cfg = sa.TrainConfig(lam=0.2, dssim_scale=1.0, lr_pos=0.0, lr_scales=0.0, lr_quat=0.0, lr_opacity=0.0,
+                         lr_sh=0.0, max_views_per_step=16, entries_per_view=12_000_000)
+    views = [(sa.Camera(sp.center, sp.quat, sp.focal, 3), t, m) for sp, t, m in zip(specs, targets, counts)]
+    ids = list(range(16))
+    results = {}
+    for mode in (sa.Trainer.GRAPH, sa.Trainer.PLAIN):
+        t = sa.Trainer(gpu, cfg, sc.as_init(), views)
+        if mode == sa.Trainer.GRAPH:
+            announce("streaming batch full size: autotune")
+            chosen = t.autotune(ids, reps=1)
+            print(f"  autotune chose {chosen}")
+        announce(f"streaming batch full size: mode {mode}")
+        t.step(ids, mode=mode)
+        ents = t.entry_count()  # raises on overflow
+        grads = dict(pos=t.grad_pos.read()[:, :3].copy(), scales=t.grad_scales.read().copy(),
+                     quat=t.grad_quat.read().copy(), opacity=t.grad_opacity.read().copy(),
+                     sh=t.grad_sh.read().copy())
+        results[mode] = (grads, t.last_losses(16), ents)
+        t.close()
+    (gg, lg, eg), (gp, lp, ep) = results[sa.Trainer.GRAPH], results[sa.Trainer.PLAIN]
+    print(f"  entries per step: graph {eg}, plain {ep}")
+    np.testing.assert_allclose(lg, lp, rtol=1e-6)
+    for k in gg:
+        nrm = float(np.linalg.norm(gg[k] - gp[k]) / np.linalg.norm(gp[k]))
+        print(f"  {k}: graph vs plain normwise {nrm:.2e}")
+        assert nrm <= 1e-5, k
+
+
 def test_uneven_multiview_batch_matches_fp64(gpu):
     """One captured batched step over 8 views of uneven resolution (one launch
     per stage for all views, gradients summed per surfel) against the FP64
