@@ -74,6 +74,8 @@ def lib():
         L.oref_marginal_decisions.argtypes = [C.POINTER(_Scene), C.POINTER(_Camera), C.c_double,
                                               C.POINTER(C.c_uint8)]
         L.oref_marginal_decisions.restype = C.c_int64
+        L.oref_pixel_contributors.argtypes = [C.POINTER(_Scene), C.POINTER(_Camera), C.c_int64,
+                                              C.POINTER(C.c_int32), C.POINTER(C.c_uint8)]
         L.oref_prepare_order.argtypes = [C.POINTER(_Scene), C.POINTER(_Camera), _i64p]
         L.oref_prepare_order.restype = C.c_int64
         L.oref_prep_f32.argtypes = [C.c_int64, _fp, _fp, _fp, C.POINTER(_Cam32), _u32p, _u32p,
@@ -281,6 +283,21 @@ def marginal_decisions(scene: Scene, cam: Camera, rel: float = 1e-5):
     if k < 0:
         raise RuntimeError("oref_marginal_decisions failed")
     return int(k), flags
+
+
+def pixel_contributors(scene: Scene, cam: Camera, pixels) -> np.ndarray:
+    """Test aid: flags[i] = 1 for surfels that composite at any of the
+    (x, y) pixels (alpha >= 1/255 before the transmittance stop)."""
+    xy = np.ascontiguousarray(np.asarray(pixels, dtype=np.int32).reshape(-1, 2))
+    s, keep = scene._c()
+    c = cam._c()
+    flags = np.zeros(scene.n, dtype=np.uint8)
+    rc = lib().oref_pixel_contributors(C.byref(s), C.byref(c), xy.shape[0],
+                                       xy.ctypes.data_as(C.POINTER(C.c_int32)),
+                                       flags.ctypes.data_as(C.POINTER(C.c_uint8)))
+    if rc != 0:
+        raise RuntimeError("oref_pixel_contributors failed")
+    return flags
 
 
 def prepare_order(scene: Scene, cam: Camera) -> np.ndarray:

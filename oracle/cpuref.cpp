@@ -718,9 +718,15 @@ int64_t oref_marginal_decisions(const oref_scene* s, const oref_camera* cam, dou
             ++count;
         }
     }
-    // The per-pixel walk.
+    // The per-pixel walk (rows in parallel; a flag is only ever set to 1).
     TileLists tl = build_tile_lists(surfels, cam);
     const Window win = window_of(cam);
+    uint8_t* flp = fl.data();
+    auto mark = [flp](int64_t i) {
+#pragma omp atomic write
+        flp[i] = 1;
+    };
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : count)
     for (int yy = 0; yy < win.h; ++yy) {
         for (int xx = 0; xx < win.w; ++xx) {
             const int x = win.x0 + xx, y = win.y0 + yy;
@@ -754,23 +760,23 @@ int64_t oref_marginal_decisions(const oref_scene* s, const oref_camera* cam, dou
                 const bool contributes = alpha >= ALPHA_SKIP;
                 if (stopped) {  // past a marginal stop: the next contributor
                     if (contributes) {
-                        fl[sf.idx] = 1;
+                        mark(sf.idx);
                         return false;
                     }
                     return true;
                 }
                 if (m) {
-                    fl[sf.idx] = 1;
+                    mark(sf.idx);
                     ++count;
                 }
                 if (!contributes) return true;
                 if (flag_next) {
-                    fl[sf.idx] = 1;
+                    mark(sf.idx);
                     flag_next = false;
                 }
                 transmittance *= 1.0 - alpha;
                 if (near(transmittance, T_TERMINATE)) {
-                    fl[sf.idx] = 1;
+                    mark(sf.idx);
                     ++count;
                     flag_next = true;
                     if (transmittance < T_TERMINATE) stopped = true;
@@ -782,6 +788,36 @@ int64_t oref_marginal_decisions(const oref_scene* s, const oref_camera* cam, dou
     }
     if (flags) std::memcpy(flags, fl.data(), (size_t)n);
     return count;
+}
+
+// The surfels that composite (alpha >= 1/255, before the transmittance stop)
+// at each of the npix pixels xy[2k], xy[2k + 1]: flags[i] = 1. A test aid (no
+// forward.rs counterpart) for masking surfels whose gradient depends on a
+// per-pixel decision made elsewhere (the L1 sign where render and target
+// nearly agree).
+int oref_pixel_contributors(const oref_scene* s, const oref_camera* cam, int64_t npix, const int32_t* xy,
+                            uint8_t* flags) {
+    if (!s || !cam || !flags || cam->width <= 0 || cam->height <= 0) return -1;
+    Scene sc{s, num_coeffs(s->sh_degree)};
+    auto surfels = prepare(sc, cam);
+    TileLists tl = build_tile_lists(surfels, cam);
+    std::memset(flags, 0, (size_t)s->n);
+    for (int64_t k = 0; k < npix; ++k) {
+        const int x = xy[2 * k], y = xy[2 * k + 1];
+        if (x < 0 || y < 0 || x >= cam->width || y >= cam->height) continue;
+        V3 d = pixel_ray(cam, x, y);
+        double transmittance = 1.0;
+        walk_pixel(surfels, &tl, x, y, [&](int64_t li) {
+            const SurfelCam& sf = surfels[li];
+            Hit hit = evaluate(sf, d, x + 0.5, y + 0.5);
+            const double a = std::min(sf.opacity * hit.ghat, ALPHA_CLAMP);
+            if (a < ALPHA_SKIP) return true;
+            flags[sf.idx] = 1;
+            transmittance *= 1.0 - a;
+            return !(transmittance < T_TERMINATE);
+        });
+    }
+    return 0;
 }
 
 // forward.rs:174-205

@@ -107,6 +107,11 @@ int oref_gradients(const oref_scene* s, const oref_camera* cam, int mode,
  * surfels involved; returns their count of decisions. */
 int64_t oref_marginal_decisions(const oref_scene* s, const oref_camera* cam, double rel, uint8_t* flags);
 
+/* Test aid (no reference counterpart): flags[i] = 1 for the surfels that
+ * composite at any of the npix pixels (x, y) = xy[2k], xy[2k+1]. */
+int oref_pixel_contributors(const oref_scene* s, const oref_camera* cam, int64_t npix, const int32_t* xy,
+                            uint8_t* flags);
+
 /* Number of surfels that survive the near cull and their prepare() order
  * (indices sorted by (f32 depth bits, idx)). order may be NULL. */
 int64_t oref_prepare_order(const oref_scene* s, const oref_camera* cam, int64_t* order);

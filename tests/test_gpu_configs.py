@@ -1,0 +1,236 @@
+"""The benchmark configurations at their own sizes (the multi-view and
+streaming ones at full size are in test_gpu_train.py, with their scaled-down
+FP64 checks).
+
+C0: plain single-view forward + backward of the 10k-surfel SH0 scene at
+256x256 (camera at z = -3 looking at the origin, 60-degree FOV), loss
+0.8 L1 + 0.2 (1 - SSIM) against the render of the centre-jittered copy.
+C1: 100k surfels, SH3, 1920x1080: projection rects, depth keys, sorted
+(tile << 32 | depth) keys and tile ranges bit-exact against the fp32 mirror,
+then one forward + backward + Adam step (lr 1e-3, betas 0.9 / 0.999).
+C2: the 8-view uneven batch (640x360 .. 1920x1080) on the 100k scene in one
+launch per stage: gradients equal the plain per-view path's.
+
+FP64 comparisons take the discrete decisions out first, as the multi-view
+tests do: surfels in a forward decision within 1e-4 relative of a
+threshold (oracle.marginal_decisions) and surfels compositing at a pixel
+where render and target agree to 1e-5 in some channel (the L1 sign) are
+masked and counted; the rest are held to per class normwise
+<= max(1e-4, 4x the FP64 gradient's unit-roundoff sensitivity).
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import announce
+
+pytestmark = pytest.mark.gpu
+
+LAM, DSSIM_SCALE = 0.2, 1.0  # 0.8 L1 + 0.2 (1 - SSIM)
+
+
+def _scene64(sc):
+    import multiview as mv
+    return mv.scene64(sc)
+
+
+def _camera64(sp):
+    import multiview as mv
+    return mv.camera64(sp)
+
+
+def _fp64_step(s64, cam, target):
+    out = oracle.render(s64, cam)
+    loss, dl, l1, ss = oracle.l1_ssim(out["color"], target.astype(np.float64), LAM, DSSIM_SCALE)
+    return out, l1, ss, oracle.gradients(s64, cam, dl.reshape(-1, 3))
+
+
+def _decision_mask(s64, cam, img, target):
+    k, walk = oracle.marginal_decisions(s64, cam, 1e-4)
+    d = img - target.astype(np.float64)
+    ys, xs, _ = np.nonzero((np.abs(d) < 1e-5) & ((img != 0) | (target != 0)))
+    l1 = oracle.pixel_contributors(s64, cam, np.stack([xs, ys], 1))
+    mask = walk.astype(bool) | l1.astype(bool)
+    print(f"  marginal walk decisions {k}, L1-marginal pixel channels {len(ys)}, "
+          f"surfels masked {int(mask.sum())} of {mask.size}")
+    assert mask.mean() < 0.1
+    return mask, k
+
+
+def _sensitivity(s64, cam, target, ref, mask, draws):
+    rng = np.random.default_rng(0)
+    sens = {}
+    for _ in range(draws):
+        sp = s64.copy()
+        for arr in (sp.pos, sp.scales, sp.quat, sp.opacity, sp.sh):
+            arr *= 1.0 + rng.choice([-1.0, 1.0], arr.shape) * 2.0 ** -24
+        _, _, _, g2 = _fp64_step(sp, cam, target)
+        for k in ("pos", "scales", "quat", "opacity", "sh"):
+            c = getattr(ref, k)[~mask]
+            sens[k] = max(sens.get(k, 0.0),
+                          float(np.linalg.norm(getattr(g2, k)[~mask] - c) / np.linalg.norm(c)))
+    return sens
+
+
+def _trainer_grads(tr, n, ncoef):
+    return dict(pos=tr.grad_pos.read()[:, :3].copy(), scales=tr.grad_scales.read().copy(),
+                quat=tr.grad_quat.read().copy(), opacity=tr.grad_opacity.read().copy(),
+                sh=tr.grad_sh.read()[:, :ncoef * 3].copy())
+
+
+def _check(got, ref, mask, sens, n):
+    for k in ("pos", "scales", "quat", "opacity", "sh"):
+        c = getattr(ref, k).reshape(n, -1)[~mask]
+        g = got[k].reshape(n, -1)[~mask]
+        nrm = float(np.linalg.norm(g - c) / np.linalg.norm(c))
+        tol = max(1e-4, 4.0 * sens[k])
+        print(f"  {k}: normwise rel {nrm:.2e} (fp64 sensitivity {sens[k]:.2e}, bound {tol:.2e})")
+        assert nrm <= tol, k
+
+
+def _trainer(gpu, sc, sp, target, lr, entries=0):
+    import splattar_amd as sa
+    cfg = sa.TrainConfig(lam=LAM, dssim_scale=DSSIM_SCALE, lr_pos=lr, lr_scales=lr, lr_quat=lr,
+                         lr_opacity=lr, lr_sh=lr, max_views_per_step=1, entries_per_view=entries)
+    t4 = np.zeros((sp.height, sp.width, 4), np.float32)
+    t4[..., :3] = target
+    cam = sa.Camera(np.asarray(sp.center, np.float32), np.asarray(sp.quat, np.float32),
+                    float(np.float32(sp.focal)), sc.sh_degree)
+    return sa.Trainer(gpu, cfg, sc.as_init(), [(cam, t4)])
+
+
+def test_config0_step_matches_fp64(gpu):
+    from splattar_amd import scenes
+    sc = scenes.config_scene(0)
+    sp = scenes.front_view(256, 256)
+    tgt = scenes.jittered(sc, 0.01, scenes.CONFIG_SEED ^ 1)
+    cam = _camera64(sp)
+    target = oracle.render(_scene64(tgt), cam)["color"].astype(np.float32)
+    s64 = _scene64(sc)
+    out, l1, ss, ref = _fp64_step(s64, cam, target)
+    mask, k = _decision_mask(s64, cam, out["color"], target)
+    sens = _sensitivity(s64, cam, target, ref, mask, 2)
+    announce("config 0: 10k surfels, 256x256, one plain step")
+    tr = _trainer(gpu, sc, sp, target, 0.0)
+    tr.step([0], mode=tr.PLAIN)
+    losses = tr.last_losses(1)[0]
+    got = _trainer_grads(tr, sc.n, 1)
+    img = tr.render(0)
+    tr.close()
+    err = np.abs(img[..., :3] - out["color"]).max(axis=2)
+    print(f"  image: max |d| {err.max():.2e}, pixels above 1e-5: {int((err > 1e-5).sum())}; "
+          f"L1 {losses[0]:.8f} vs {l1:.8f}; D-SSIM {losses[1]:.8f} vs {DSSIM_SCALE * (1 - ss):.8f}")
+    assert int((err > 1e-5).sum()) <= k and float(np.median(err)) < 1e-6
+    assert abs(losses[0] - l1) <= 1e-6 * l1
+    assert abs(losses[1] - DSSIM_SCALE * (1 - ss)) <= 1e-5 * DSSIM_SCALE * (1 - ss)
+    _check(got, ref, mask, sens, sc.n)
+
+
+def test_config1_keys_bit_exact_and_step(gpu):
+    import splattar_amd as sa
+    from splattar_amd import scenes
+    sc = scenes.config_scene(1)
+    sp = scenes.front_view(1920, 1080)
+    cam32 = sa.Camera(np.asarray(sp.center, np.float32), np.asarray(sp.quat, np.float32),
+                      float(np.float32(sp.focal)), 3)
+    # Keys: both exact binnings against the mirror.
+    rot = oracle.quat_to_mat(np.asarray(sp.quat, np.float32).astype(np.float64))
+    c32 = oracle.cam32(rot, np.asarray(sp.center, np.float32), float(np.float32(sp.focal)), 1920, 1080)
+    pos4 = np.concatenate([sc.positions, np.ones((sc.n, 1), np.float32)], 1)
+    rects, keys, _ = oracle.prep_f32(pos4, sc.scales, sc.quats, c32)
+    k, p, e = oracle.emit_keys(rects, keys, c32.tiles_x)
+    k, p = oracle.stable_sort(k, p)
+    ranges = oracle.tile_ranges(k, c32.tiles_x * c32.tiles_y)
+    for mode in (0, 1):
+        announce(f"config 1: keys, binning {mode}")
+        r = sa.Rasterizer(gpu, sc.n, 1920, 1080, sc.n * 64)
+        r.upload_scene(sc.positions, sc.scales, sc.quats, sc.opacities, sc.sh)
+        r.set_binning(mode)
+        r.forward(cam32, sc.n)
+        d = r.debug()
+        r.close()
+        assert np.array_equal(d["rects"][:sc.n], rects) and np.array_equal(d["depth_keys"][:sc.n], keys)
+        assert d["entries"] == e
+        assert np.array_equal(d["sorted_keys"], k) and np.array_equal(d["sorted_payloads"], p)
+        assert np.array_equal(d["ranges"], ranges)
+    print(f"  {e} entries bit-exact under both binnings")
+    # One step: forward, L1 + D-SSIM, backward, Adam (lr 1e-3).
+    tgt = scenes.jittered(sc, 0.01, (scenes.CONFIG_SEED + 1) ^ 1)
+    cam = _camera64(sp)
+    target = oracle.render(_scene64(tgt), cam)["color"].astype(np.float32)
+    s64 = _scene64(sc)
+    out, l1, ss, ref = _fp64_step(s64, cam, target)
+    mask, _ = _decision_mask(s64, cam, out["color"], target)
+    sens = _sensitivity(s64, cam, target, ref, mask, 1)
+    announce("config 1: one step")
+    tr = _trainer(gpu, sc, sp, target, 1e-3)
+    tr.step([0], mode=tr.PLAIN)
+    losses = tr.last_losses(1)[0]
+    got = _trainer_grads(tr, sc.n, 16)
+    params = dict(pos=tr.positions.read()[:, :3].copy(), scales=tr.scales.read().copy(),
+                  quat=tr.quats.read().copy(), opacity=tr.opacities.read().copy(),
+                  sh=tr.sh.read().copy())
+    tr.close()
+    assert abs(losses[0] - l1) <= 1e-6 * l1
+    _check(got, ref, mask, sens, sc.n)
+    # Adam, t = 1, from the FP64 gradient (f32 loop of adam.wgsl:44-74).
+    n = sc.n
+
+    def adam(raw, grad, mode):
+        raw = raw.astype(np.float32).ravel().copy()
+        m = np.zeros_like(raw)
+        v = np.zeros_like(raw)
+        return oracle.adam_step_f32(raw, grad.astype(np.float32).ravel(), m, v, 1e-3, mode, 1)
+
+    o = np.clip(sc.opacities, 1e-4, 1 - 1e-4)
+    want = dict(pos=adam(sc.positions, ref.pos, 0).reshape(n, 3),
+                scales=adam(np.log(np.maximum(sc.scales, 1e-6)), ref.scales, 1).reshape(n, 2),
+                quat=adam(sc.quats, ref.quat, 0).reshape(n, 4),
+                opacity=adam(np.log(o / (1 - o)), ref.opacity, 2),
+                sh=adam(sc.sh, ref.sh.reshape(n, -1), 0).reshape(n, -1))
+    for key, w in want.items():
+        g = params[key][:, :w.shape[1]] if w.ndim == 2 else params[key]
+        sel = ~mask if w.ndim == 1 else np.repeat(~mask[:, None], w.shape[1], 1)
+        off = (np.abs(g - w) > 2e-6 + 1e-5 * np.abs(w)) & sel
+        # Adam's first step moves a parameter by ~lr sign(g): only gradients
+        # at roundoff level can take the other sign.
+        print(f"  Adam {key}: {off.mean():.5f} off")
+        assert off.mean() < 0.01, key
+
+
+def test_config2_uneven_batch_full_size(gpu):
+    """8 views of 640x360 .. 1920x1080 on the 100k-surfel SH3 scene, one
+    launch per stage (graph): gradients equal the plain per-view step's
+    (learning rates 0) per class normwise to 1e-5, losses to 1e-6."""
+    import splattar_amd as sa
+    from splattar_amd import scenes
+    sc = scenes.config_scene(2)
+    tgt = scenes.jittered(sc, 0.01, (scenes.CONFIG_SEED + 2) ^ 1)
+    sizes = scenes.uneven_sizes(8, scenes.CONFIG_SEED + 2)
+    specs = scenes.sphere_views(8, scenes.CONFIG_SEED + 2, sizes)
+    announce(f"config 2: targets {sizes}")
+    targets = []
+    for sp in specs:
+        r = sa.Rasterizer(gpu, sc.n, sp.width, sp.height, sc.n * 64)
+        r.upload_scene(tgt.positions, tgt.scales, tgt.quats, tgt.opacities, tgt.sh)
+        r.forward(sa.Camera(sp.center, sp.quat, sp.focal, 3), sc.n)
+        targets.append(r.out_color.read().copy())
+        r.close()
+    cfg = sa.TrainConfig(lam=LAM, dssim_scale=DSSIM_SCALE, lr_pos=0.0, lr_scales=0.0, lr_quat=0.0,
+                         lr_opacity=0.0, lr_sh=0.0, max_views_per_step=8)
+    views = [(sa.Camera(sp.center, sp.quat, sp.focal, 3), t) for sp, t in zip(specs, targets)]
+    res = {}
+    for mode in (sa.Trainer.GRAPH, sa.Trainer.PLAIN):
+        announce(f"config 2: mode {mode}")
+        t = sa.Trainer(gpu, cfg, sc.as_init(), views)
+        t.step(list(range(8)), mode=mode)
+        t.entry_count()  # raises on overflow
+        res[mode] = (_trainer_grads(t, sc.n, 16), t.last_losses(8))
+        t.close()
+    (gg, lg), (gp, lp) = res[sa.Trainer.GRAPH], res[sa.Trainer.PLAIN]
+    np.testing.assert_allclose(lg, lp, rtol=1e-6)
+    for k in gg:
+        nrm = float(np.linalg.norm(gg[k] - gp[k]) / np.linalg.norm(gp[k]))
+        print(f"  {k}: graph vs plain normwise {nrm:.2e}")
+        assert nrm <= 1e-5, k

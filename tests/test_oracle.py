@@ -303,6 +303,20 @@ def test_marginal_decisions_finds_planted_threshold_crossings():
     assert k >= 1 and list(flags) == [1, 1, 0]
 
 
+def test_pixel_contributors_follow_the_walk():
+    """oracle.pixel_contributors (a test aid): three stacked wide opaque
+    surfels over the centre pixel -- the two nearest composite there (alpha
+    clamped to 0.995), the third lies past the transmittance stop
+    (0.005^2 < 1e-4); a fourth surfel off to the side reaches only pixels
+    near it."""
+    cam = oracle.Camera(np.zeros(3), np.array([0, 0, 0, 1.0]), 20.0, 16, 16)
+    scene = _single_scene([[0, 0, -2.0], [0, 0, -3.0], [0, 0, -4.0], [0.7, 0.0, -2.0]],
+                          [[2.0, 2.0]] * 3 + [[0.02, 0.02]], [0.999] * 4, [[0.1, 0.2, 0.3]] * 4)
+    assert list(oracle.pixel_contributors(scene, cam, [(8, 8)])) == [1, 1, 0, 0]
+    assert oracle.pixel_contributors(scene, cam, [(15, 7)])[3] == 1
+    assert not oracle.pixel_contributors(scene, cam, np.zeros((0, 2))).any()
+
+
 def test_near_culled_scene_is_empty():
     cam = oracle.Camera(np.zeros(3), np.array([0, 0, 0, 1.0]), 20.0, 16, 16)
     scene = _single_scene([[0, 0, 1.0], [0, 0, -0.01]], [[0.3, 0.3]] * 2, [0.9, 0.9], [[0, 0, 0]] * 2)
