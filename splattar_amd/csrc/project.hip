@@ -40,7 +40,9 @@ __global__ __launch_bounds__(256) void project_kernel(
         for (uint32_t k = i; k < num_ranges; k += sblocks * blockDim.x) ranges_to_clear[k] = 0u;
     }
     if (view == 0 && i == 0 && item_count != nullptr) *item_count = n * n_views;
-    if (i >= n) return;
+    // Every thread of the block runs to the cooperative tile-mask step below
+    // (it uses the whole wave); items past n only take part with no tiles.
+    const bool in_range = i < n;
     const CamParams cam = cams[view];
     const uint32_t g = view * n + i;
     // Per-item binning outputs, written on every path. The presort payload
@@ -62,66 +64,122 @@ __global__ __launch_bounds__(256) void project_kernel(
     };
 
     uint4 rect = make_uint4(1u, 1u, 0u, 0u);
-    if (i >= cam.n_surfels) {  // not part of this view
-        finish(rect, 0u, 0u, ~0ull);
-        return;
+    bool live = in_range && i < cam.n_surfels;  // surfels past n_surfels are not part of this view
+    CamFrame f;
+    float depth = 0.0f;
+    if (live) {
+        f = cam_frame(cam, pos[i], scales[i], quats[i]);
+        depth = -f.cz_;
+        // Near cull (surfel_prep_fwd.wgsl:171-176); NaN culls too.
+        live = depth >= kNearDepth;
     }
-    const float4 p = pos[i];
-    const float2 s = scales[i];
-    const float4 q = quats[i];
-    const CamFrame f = cam_frame(cam, p, s, q);
-    const float depth = -f.cz_;
-    if (!(depth >= kNearDepth)) {  // near cull (surfel_prep_fwd.wgsl:171-176); NaN culls too
-        finish(rect, 0u, 0u, ~0ull);
-        return;
-    }
-    const RectOut r = tile_rect(cam, f, depth);
-
-    // Ray-splat setup.
-    const float3 c = make_float3(f.cx_, f.cy_, f.cz_);
-    const float3 tu = make_float3(f.tux, f.tuy, f.tuz);
-    const float3 tv = make_float3(f.tvx, f.tvy, f.tvz);
-    const float3 M = cross3(tu, tv);
-    const float3 A = cross3(tv, c);
-    const float3 B = cross3(c, tu);
-    const float Nt = -dot3(M, c);
     Record rec;
-    rec.m_nt = make_float4(M.x, M.y, M.z, Nt);
-    rec.a_px = make_float4(A.x, A.y, A.z, r.px);
-    rec.b_py = make_float4(B.x, B.y, B.z, r.py);
-    rec.col_op.w = opac[i];
-
     uint32_t count = 0u;
     uint64_t mask = ~0ull;
-    if (r.on_screen) {
-        rect = make_uint4(r.x0, r.y0, r.x1, r.y1);
-        record_footprint(rec, cam);
-        rec.box = clip_box_to_rect(rec.box, rect);
-        // Tight binning keeps only the work tiles the alpha >= 1/255
-        // footprint reaches; the dropped entries would contribute nothing.
-        const uint32_t ts = cam.tile_size;
-        if (clip_to_box) rect = box_tiles(rec.box, ts, cam.tiles_x, cam.tiles_y);
-        if (rect.x <= rect.z && rect.y <= rect.w) {
-            const uint32_t w = rect.z - rect.x + 1u;
-            count = w * (rect.w - rect.y + 1u);
-            if (clip_to_box && count <= 64u) {
-                // Per-tile footprint test (box, disk, ellipse) for rects of at
-                // most 64 tiles: bit t = row-major tile t of the rect. Larger
-                // rects keep every tile.
-                mask = 0ull;
-                const float span = (float)ts - 0.5f;
-                for (uint32_t t = 0; t < count; ++t) {
-                    const float tx = (float)((rect.x + t % w) * ts);
-                    const float ty = (float)((rect.y + t / w) * ts);
-                    if (footprint_hits(rec, tx + 0.5f, ty + 0.5f, tx + span, ty + span)) mask |= 1ull << t;
-                }
-                count = (uint32_t)__popcll(mask);
+    bool masked = false;  // rect of at most 64 tiles: per-tile footprint test below
+    float3 c = make_float3(0.0f, 0.0f, 0.0f), M = c;
+    if (live) {
+        const RectOut r = tile_rect(cam, f, depth);
+        // Ray-splat setup.
+        c = make_float3(f.cx_, f.cy_, f.cz_);
+        const float3 tu = make_float3(f.tux, f.tuy, f.tuz);
+        const float3 tv = make_float3(f.tvx, f.tvy, f.tvz);
+        M = cross3(tu, tv);
+        const float3 A = cross3(tv, c);
+        const float3 B = cross3(c, tu);
+        const float Nt = -dot3(M, c);
+        rec.m_nt = make_float4(M.x, M.y, M.z, Nt);
+        rec.a_px = make_float4(A.x, A.y, A.z, r.px);
+        rec.b_py = make_float4(B.x, B.y, B.z, r.py);
+        rec.col_op.w = opac[i];
+        if (r.on_screen) {
+            rect = make_uint4(r.x0, r.y0, r.x1, r.y1);
+            record_footprint(rec, cam);
+            rec.box = clip_box_to_rect(rec.box, rect);
+            // Tight binning keeps only the work tiles the alpha >= 1/255
+            // footprint reaches; the dropped entries would contribute nothing.
+            if (clip_to_box) rect = box_tiles(rec.box, cam.tile_size, cam.tiles_x, cam.tiles_y);
+            if (rect.x <= rect.z && rect.y <= rect.w) {
+                count = (rect.z - rect.x + 1u) * (rect.w - rect.y + 1u);
+                // Rects of at most 64 tiles get a per-tile footprint test (box,
+                // disk, ellipse); larger ones keep every tile.
+                masked = clip_to_box && count <= 64u;
             }
         }
-        if (count == 0u) {
-            rect = make_uint4(1u, 1u, 0u, 0u);
-            mask = ~0ull;
+    }
+
+    // Per-tile footprint masks, wave-cooperatively: the (item, tile) pairs of
+    // the wave's masked items are dealt 64 at a time over the lanes (owner by
+    // binary search over the wave's exclusive scan of tile counts), each lane
+    // tests one tile against its owner's footprint read from LDS and sets the
+    // owner's bit with an LDS atomic OR. Bit t = row-major tile t of the rect.
+    // The cost follows the wave's total tile count, not 64 x its widest item.
+    __shared__ float4 s_fp[256][4];     // box, ell0, ell1, (px, py, rect width, 0)
+    __shared__ uint32_t s_bits[256][2];
+    {
+        const int tid = threadIdx.x, lane = tid & 63, wbase = tid & ~63;
+        const uint32_t mine = masked ? count : 0u;
+        const uint32_t rw = rect.z - rect.x + 1u;
+        if (masked) {
+            s_fp[tid][0] = rec.box;
+            s_fp[tid][1] = rec.ell0;
+            s_fp[tid][2] = rec.ell1;
+            s_fp[tid][3] = make_float4(rec.a_px.w, rec.b_py.w, (float)rw, 0.0f);
         }
+        s_bits[tid][0] = 0u;
+        s_bits[tid][1] = 0u;
+        uint32_t incl = mine;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
+        }
+        const uint32_t total = __shfl(incl, 63);
+        const uint32_t excl = incl - mine;
+        __syncthreads();
+        const uint32_t ts = cam.tile_size;
+        const float span = (float)ts - 0.5f;
+        for (uint32_t k0 = 0; k0 < total; k0 += 64) {
+            const uint32_t k = k0 + (uint32_t)lane;
+            int owner = 0;
+            for (int step = 32; step >= 1; step >>= 1) {
+                const int cand = owner + step;
+                const uint32_t ex = __shfl(excl, cand & 63);
+                if (cand < 64 && ex <= k) owner = cand;
+            }
+            const uint32_t o_excl = __shfl(excl, owner);
+            const uint32_t o_x0 = __shfl(rect.x, owner), o_y0 = __shfl(rect.y, owner);
+            if (k < total) {
+                const uint32_t t = k - o_excl;
+                const float4 q3 = s_fp[wbase + owner][3];
+                const uint32_t w = (uint32_t)q3.z;
+                // t < 64 and w <= 64: the float quotient's floor is exact.
+                const uint32_t ty = (uint32_t)(((float)t + 0.5f) / q3.z);
+                const uint32_t tx = t - ty * w;
+                Record fr;
+                fr.box = s_fp[wbase + owner][0];
+                fr.ell0 = s_fp[wbase + owner][1];
+                fr.ell1 = s_fp[wbase + owner][2];
+                fr.a_px.w = q3.x;
+                fr.b_py.w = q3.y;
+                const float px0 = (float)((o_x0 + tx) * ts), py0 = (float)((o_y0 + ty) * ts);
+                if (footprint_hits(fr, px0 + 0.5f, py0 + 0.5f, px0 + span, py0 + span))
+                    atomicOr(&s_bits[wbase + owner][t >> 5], 1u << (t & 31u));
+            }
+        }
+        __syncthreads();
+        if (masked) {
+            mask = (uint64_t)s_bits[tid][0] | ((uint64_t)s_bits[tid][1] << 32);
+            count = (uint32_t)__popcll(mask);
+        }
+    }
+    if (!in_range) return;
+    if (!live) {
+        finish(make_uint4(1u, 1u, 0u, 0u), 0u, 0u, ~0ull);
+        return;
+    }
+    if (count == 0u) {
+        rect = make_uint4(1u, 1u, 0u, 0u);
+        mask = ~0ull;
     }
     finish(rect, __float_as_uint(depth), count, mask);
     if (count == 0u) return;  // no entry will ever read the record
