@@ -282,6 +282,13 @@ __global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t* __restrict__ ds
         dst[i] = value;
 }
 
+__global__ __launch_bounds__(256) void copy_u32_kernel(uint32_t* __restrict__ dst,
+                                                       const uint32_t* __restrict__ src, size_t count) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 }  // namespace
 
 hipError_t launch_emit_keys(hipStream_t st, uint32_t n_items, const uint4* rects,
@@ -346,6 +353,14 @@ hipError_t launch_compose_keys(hipStream_t st, const uint32_t* tkeys, const uint
 hipError_t launch_count_tiles(hipStream_t st, uint32_t n, const uint4* rects, uint32_t* counts) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(count_tiles_kernel, dim3(div_up(n, 256)), dim3(256), 0, st, n, rects, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_u32(hipStream_t st, uint32_t* dst, const uint32_t* src, size_t count) {
+    if (count == 0) return hipSuccess;
+    uint64_t blocks = (count + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(copy_u32_kernel, dim3((uint32_t)blocks), dim3(256), 0, st, dst, src, count);
     return hipGetLastError();
 }
 

@@ -495,7 +495,8 @@ inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) /
 // project.hip
 // presort_keys / presort_vals (optional) receive a copy of the depth key and
 // the item index for the depth presort; item_count (optional) receives
-// n * n_views.
+// n * n_views; rects / depth_keys / counts may be null; write_normal false
+// leaves Record::nrm_dep unwritten (colour-only consumers).
 hipError_t launch_project(hipStream_t st, const CamParams* cams_dev, uint32_t n_views, uint32_t n,
                           const float4* pos, const float2* scales, const float4* quats,
                           const float* opac, const float* sh, Record* recs, uint4* rects,
@@ -503,7 +504,7 @@ hipError_t launch_project(hipStream_t st, const CamParams* cams_dev, uint32_t n_
                           uint32_t num_ranges, uint32_t* presort_keys = nullptr,
                           uint32_t* presort_vals = nullptr, uint32_t* item_count = nullptr,
                           bool clip_to_box = false, uint4* emit_info = nullptr,
-                          bool pack_counts = false);
+                          bool pack_counts = false, bool write_normal = true);
 
 // scan.hip
 size_t scan_scratch_bytes(uint32_t capacity);
@@ -522,6 +523,8 @@ hipError_t launch_emit_keys(hipStream_t st, uint32_t n_items, const uint4* rects
 hipError_t launch_tile_ranges(hipStream_t st, const uint64_t* keys, const uint32_t* count,
                               uint32_t capacity, uint32_t* ranges);
 hipError_t launch_fill_u32(hipStream_t st, uint32_t* dst, uint32_t value, size_t count);
+// dst[0..count) = src[0..count) as a kernel on the stream (device to device).
+hipError_t launch_copy_u32(hipStream_t st, uint32_t* dst, const uint32_t* src, size_t count);
 hipError_t launch_count_tiles(hipStream_t st, uint32_t n, const uint4* rects, uint32_t* counts);
 // Depth-ordered emission: order[k] is the k-th item in depth order (shifted
 // left by 8 over its tile count when packed), emit_info the per-item record

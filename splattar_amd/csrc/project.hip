@@ -10,7 +10,10 @@
 // (the forward model's coverage). With clip_to_box the emitted rect is the
 // set of work tiles (cam.tile_size px) that clipped box reaches (tight
 // binning); without it, the 16-px rect itself (work tiles must be 16 px).
-// Surfels i >= cam.n_surfels are culled in that view.
+// Surfels i >= cam.n_surfels are culled in that view. rects, depth_keys and
+// counts may be null (callers that read only the presort and emit_info
+// outputs); with write_normal false the record's normal / depth float4 is
+// not written (colour-only consumers never read it).
 
 #include "gs_internal.h"
 
@@ -25,7 +28,7 @@ __global__ __launch_bounds__(256) void project_kernel(
     uint4* __restrict__ rects, uint32_t* __restrict__ depth_keys, uint32_t* __restrict__ counts,
     uint32_t* __restrict__ ranges_to_clear, uint32_t num_ranges, uint32_t* __restrict__ presort_keys,
     uint32_t* __restrict__ presort_vals, uint32_t* __restrict__ item_count, bool clip_to_box,
-    uint4* __restrict__ emit_info, bool pack_counts) {
+    uint4* __restrict__ emit_info, bool pack_counts, bool write_normal) {
     // Views vary fastest over the logical blocks and each XCD takes a
     // contiguous range of them, so the V blocks that read the same 256
     // surfels run on one XCD and the later ones find the parameters in its L2.
@@ -53,9 +56,9 @@ __global__ __launch_bounds__(256) void project_kernel(
     // in 16-bit tile coordinates and the mask of its tiles the footprint
     // reaches (all ones = every tile).
     auto finish = [&](uint4 rect, uint32_t key, uint32_t count, uint64_t mask) {
-        rects[g] = rect;
-        depth_keys[g] = key;
-        counts[g] = count;
+        if (rects) rects[g] = rect;
+        if (depth_keys) depth_keys[g] = key;
+        if (counts) counts[g] = count;
         if (presort_keys) presort_keys[g] = key;
         if (presort_vals) presort_vals[g] = pack_counts ? (g << 8) | count : g;
         if (emit_info)
@@ -219,7 +222,15 @@ __global__ __launch_bounds__(256) void project_kernel(
     }
     rec.col_op = make_float4(fmaxf(cr, 0.0f), fmaxf(cg, 0.0f), fmaxf(cb, 0.0f), opac[i]);
     rec.nrm_dep = make_float4(nrm.x, nrm.y, nrm.z, depth);
-    recs[g] = rec;
+    Record& out = recs[g];
+    out.m_nt = rec.m_nt;
+    out.a_px = rec.a_px;
+    out.b_py = rec.b_py;
+    out.col_op = rec.col_op;
+    if (write_normal) out.nrm_dep = rec.nrm_dep;
+    out.box = rec.box;
+    out.ell0 = rec.ell0;
+    out.ell1 = rec.ell1;
 }
 
 }  // namespace
@@ -229,14 +240,15 @@ hipError_t launch_project(hipStream_t st, const CamParams* cams_dev, uint32_t n_
                           const float* opac, const float* sh, Record* recs, uint4* rects,
                           uint32_t* depth_keys, uint32_t* counts, uint32_t* ranges_to_clear,
                           uint32_t num_ranges, uint32_t* presort_keys, uint32_t* presort_vals,
-                          uint32_t* item_count, bool clip_to_box, uint4* emit_info, bool pack_counts) {
+                          uint32_t* item_count, bool clip_to_box, uint4* emit_info, bool pack_counts,
+                          bool write_normal) {
     if (n == 0 && num_ranges == 0 && item_count == nullptr) return hipSuccess;
     if (n_views == 0) return hipSuccess;
     const uint32_t blocks = div_up(n > num_ranges ? (n ? n : 1) : num_ranges, 256);
     dim3 grid(blocks * n_views);
     hipLaunchKernelGGL(project_kernel, grid, dim3(256), 0, st, cams_dev, n_views, n, pos, scales, quats, opac,
                        sh, recs, rects, depth_keys, counts, ranges_to_clear, num_ranges, presort_keys,
-                       presort_vals, item_count, clip_to_box, emit_info, pack_counts);
+                       presort_vals, item_count, clip_to_box, emit_info, pack_counts, write_normal);
     return hipGetLastError();
 }
 

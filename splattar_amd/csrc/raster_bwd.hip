@@ -207,7 +207,10 @@ __device__ __forceinline__ void add_hit(PixState& p, const Record& r, const HitE
     const float dl_dv = -h.v * gd;
     const float dnu = h.ray ? dl_du * h.inv_d : 0.0f;
     const float dnv = h.ray ? dl_dv * h.inv_d : 0.0f;
-    const float dD = h.ray ? -(h.u * dl_du + h.v * dl_dv + t_hit * dl_dt) * h.inv_d : 0.0f;
+    // Colour-only (AUX = false): dl_dt = 0, and the record's depth is not
+    // staged, so t_hit must not enter (it may hold anything).
+    const float dD_num = AUX ? h.u * dl_du + h.v * dl_dv + t_hit * dl_dt : h.u * dl_du + h.v * dl_dv;
+    const float dD = h.ray ? -dD_num * h.inv_d : 0.0f;
     put<AUX, ACC>(g[kGA + 0], dnu * p.sx);
     put<AUX, ACC>(g[kGA + 1], dnu * p.sy);
     put<AUX, ACC>(g[kGA + 2], dnu);
@@ -240,7 +243,7 @@ __global__ __launch_bounds__(TW == 8 ? 64 : TW * TW / 2) void raster_bwd_kernel(
     constexpr int SPT = (BATCH + NT - 1) / NT;
     constexpr int MB = 10;                   // list entry: slot | top << MB | bottom << (MB + 1)
     static_assert(BATCH % 64 == 0 && BATCH <= (1 << MB), "batch of whole waves");
-    __shared__ float4 s_m[BATCH], s_a[BATCH], s_b[BATCH], s_c[BATCH], s_n[BATCH];
+    __shared__ float4 s_m[BATCH], s_a[BATCH], s_b[BATCH], s_c[BATCH], s_n[AUX ? BATCH : 1];
     __shared__ float s_lthr[BATCH];
     __shared__ uint16_t s_list[NW][BATCH];
     __shared__ uint32_t s_qcnt[SW][NW];      // [staging wave][list wave]
@@ -307,7 +310,7 @@ __global__ __launch_bounds__(TW == 8 ? 64 : TW * TW / 2) void raster_bwd_kernel(
                     s_a[slot] = r.a_px;
                     s_b[slot] = r.b_py;
                     s_c[slot] = r.col_op;
-                    s_n[slot] = r.nrm_dep;
+                    if (AUX) s_n[slot] = r.nrm_dep;
                     s_lthr[slot] = record_lthr(r.col_op.w);
                     qmask = cull ? block_mask<TW>(r, x0, y0) : ~0u;
                 }
@@ -363,7 +366,7 @@ __global__ __launch_bounds__(TW == 8 ? 64 : TW * TW / 2) void raster_bwd_kernel(
                 r.a_px = s_a[j];
                 r.b_py = s_b[j];
                 r.col_op = s_c[j];
-                r.nrm_dep = s_n[j];
+                r.nrm_dep = AUX ? s_n[AUX ? j : 0] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 const float lthr = s_lthr[j];
                 // Evaluate first; the gradient terms and the reduction run only
                 // for blocks with a contributing lane (for the others the walk

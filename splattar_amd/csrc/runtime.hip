@@ -57,7 +57,11 @@ static gs_status dalloc(T** out, size_t count, const char* what) {
     void* p = nullptr;
     hipError_t e = hipMalloc(&p, count * sizeof(T));
     if (e != hipSuccess) return hip_fail(e, what);
+    // The zero fill runs on the null stream, which is not ordered with the
+    // context's non-blocking stream: wait for it, or a kernel enqueued right
+    // after the allocation can be overwritten by the late fill.
     e = hipMemset(p, 0, count * sizeof(T));
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) {
         (void)hipFree(p);
         return hip_fail(e, what);
@@ -561,6 +565,7 @@ struct RasterCore {
     bool cull = true;                // per-block footprint culling in the rasterizers
     RasterShape shape;               // work-tile size and LDS batches (tile != 16 needs TIGHT binning)
     bool aux_outputs = true;         // forward writes depth (aux.y) and normals
+    bool lean = false;               // skip rects / depth keys / counts the packed presort path never reads
     bool keys64_valid = false;       // keys[result_alt] holds 64-bit keys of the last forward
     uint32_t digit_bits = 8;
     int result_alt = 0;              // which ping-pong buffer holds the sorted entries
@@ -717,10 +722,12 @@ struct RasterCore {
     }
 
     const uint32_t* last_sort_count = nullptr;
+    bool keys_lean = false;          // the last forward left depth_keys unwritten
 
     // 64-bit (tile << 32 | depth) keys of the last forward (composed on
     // demand for the presorted path) and its sorted payloads.
     gs_status sorted_keys(hipStream_t st, const uint64_t** out) {
+        if (keys_lean) return fail(GS_ERR_INVALID, "the last forward did not keep its depth keys");
         if (!keys64_valid && last_sort_count) {
             GS_HIP(launch_compose_keys(st, tkeys[result_alt], pays[result_alt], depth_keys, last_sort_count,
                                        cap_entries, keys[0]));
@@ -758,9 +765,15 @@ struct RasterCore {
             // 8-px work tiles can give a rect more than 255 tiles: no packing.
             const bool packed = items < (1u << 24) && shape.tile >= 16;
             GS_CHECK(P.mark(st, GS_STAGE_PROJECT));
-            GS_HIP(launch_project(st, cd, nv, n, pos, sc, q, op, sh, recs, rects, depth_keys, offsets,
-                                  ranges, 2 * max_tile_id, pre_keys[0], pre_vals[0], item_count,
-                                  binning == GS_BINNING_TIGHT, emit_info, packed));
+            // Lean (trainer): with packed counts nothing downstream reads the
+            // rects, depth keys or counts, and colour-only passes never read
+            // the records' normal / depth.
+            const bool skip = lean && packed;
+            GS_HIP(launch_project(st, cd, nv, n, pos, sc, q, op, sh, recs, skip ? nullptr : rects,
+                                  skip ? nullptr : depth_keys, skip ? nullptr : offsets, ranges, 2 * max_tile_id,
+                                  pre_keys[0], pre_vals[0], item_count, binning == GS_BINNING_TIGHT, emit_info,
+                                  packed, aux_outputs));
+            keys_lean = skip;
             GS_CHECK(P.mark(st, GS_STAGE_SORT));
             int alt = 0;
             GS_HIP(launch_radix_sort_u32(st, pre_keys[0], pre_vals[0], pre_keys[1], pre_vals[1],
@@ -803,6 +816,7 @@ struct RasterCore {
         GS_CHECK(P.mark(st, GS_STAGE_RANGES));
         GS_HIP(launch_tile_ranges(st, keys[result_alt], sort_count + v0, cap_entries, ranges));
         keys64_valid = true;
+        keys_lean = false;
         last_sort_count = sort_count + v0;
         GS_CHECK(schedule_tiles(st, tiles));
         GS_CHECK(P.mark(st, GS_STAGE_RASTER_FWD));
@@ -1371,7 +1385,8 @@ struct gs_trainer_s {
                            bool profile = false);
 
     // Training-state snapshot (raw, m, v of every optimizer class and the
-    // Adam step state) in one device buffer; restore also re-activates.
+    // Adam step state) in one device buffer; restore also re-activates. The
+    // copies are kernels on the trainer's stream.
     float* saved = nullptr;
     size_t saved_floats = 0;
     gs_status save_state() {
@@ -1387,11 +1402,13 @@ struct gs_trainer_s {
         size_t off = 0;
         for (const auto& c : optim->classes) {
             for (const float* src : {c.raw, c.m, c.v}) {
-                GS_HIP(hipMemcpyAsync(saved + off, src, 4 * (size_t)c.n, hipMemcpyDeviceToDevice, st));
+                GS_HIP(launch_copy_u32(st, reinterpret_cast<uint32_t*>(saved + off),
+                                       reinterpret_cast<const uint32_t*>(src), c.n));
                 off += c.n;
             }
         }
-        GS_HIP(hipMemcpyAsync(saved + off, optim->state, sizeof(gs::AdamState), hipMemcpyDeviceToDevice, st));
+        GS_HIP(launch_copy_u32(st, reinterpret_cast<uint32_t*>(saved + off),
+                               reinterpret_cast<const uint32_t*>(optim->state), sizeof(gs::AdamState) / 4));
         GS_HIP(hipStreamSynchronize(st));
         return GS_OK;
     }
@@ -1401,11 +1418,13 @@ struct gs_trainer_s {
         size_t off = 0;
         for (const auto& c : optim->classes) {
             for (float* dst : {c.raw, c.m, c.v}) {
-                GS_HIP(hipMemcpyAsync(dst, saved + off, 4 * (size_t)c.n, hipMemcpyDeviceToDevice, st));
+                GS_HIP(launch_copy_u32(st, reinterpret_cast<uint32_t*>(dst),
+                                       reinterpret_cast<const uint32_t*>(saved + off), c.n));
                 off += c.n;
             }
         }
-        GS_HIP(hipMemcpyAsync(optim->state, saved + off, sizeof(gs::AdamState), hipMemcpyDeviceToDevice, st));
+        GS_HIP(launch_copy_u32(st, reinterpret_cast<uint32_t*>(optim->state),
+                               reinterpret_cast<const uint32_t*>(saved + off), sizeof(gs::AdamState) / 4));
         GS_CHECK(gs_optim_encode_activate(optim, st));
         GS_HIP(hipStreamSynchronize(st));
         return GS_OK;
@@ -1496,6 +1515,7 @@ gs_status gs_trainer_create(gs_ctx ctx, const gs_train_config* cfg, const gs_sce
     // The step's losses are colour-only (L1 + D-SSIM): the forward skips the
     // depth and normal images (aux.y = 0; out_normal is left untouched).
     t->core.aux_outputs = false;
+    t->core.lean = true;
     GS_CHECK(dalloc(&t->targets, toff, "targets"));
     {
         std::vector<float> buf;
