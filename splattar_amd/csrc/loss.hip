@@ -5,13 +5,17 @@
 // sigma 1.5 separable Gaussian with zero padding (the reference's f32 taps),
 // C1 = 1e-4, C2 = 9e-4, means over pixels x 3 channels; s = dssim_scale is
 // 0.5 in the reference ((1 - SSIM)/2) and selectable here.
-// The reference runs 19 dispatches over 16 P-sized intermediates; here:
-//   ssim_fwd: per 16x16 tile (+5 px halo in LDS) compute the five blurred
-//             moments, SSIM, and per channel the three coefficient maps
-//             C_mu = dw dS/dmu_x, C_x2 = dw dS/dE[x^2], C_xy = dw dS/dE[xy],
-//             plus per-block L1 / SSIM partial sums (deterministic reduction);
-//   ssim_bwd: per tile (+halo), one channel at a time, blur the coefficient
-//             maps (the zero-padded symmetric blur is self-adjoint) and combine
+// The reference runs 19 dispatches over 16 P-sized intermediates; here one
+// workgroup per (16x16 tile, channel) -- the channels are independent, and a
+// channel's working set (x, y staged with a 5 px halo; five blurred rows)
+// fits in ~14 KB of LDS, so many workgroups share a CU:
+//   ssim_fwd: stage x, y; horizontal pass in column pairs (one 12-tap window
+//             shared by two outputs, products formed once per input);
+//             vertical pass; SSIM and the three coefficient maps
+//             C_mu = dw dS/dmu_x, C_x2 = dw dS/dE[x^2], C_xy = dw dS/dE[xy];
+//             per-(channel, tile) L1 / SSIM partial sums (deterministic);
+//   ssim_bwd: stage the channel's coefficient maps, blur them (the
+//             zero-padded symmetric blur is self-adjoint) and write
 //             dL/dx = (1-l) sign(x-y)/3P + B(C_mu) + 2x B(C_x2) + y B(C_xy).
 
 #include "gs_internal.h"
@@ -23,18 +27,17 @@ namespace {
 constexpr int kT = 16;          // output tile
 constexpr int kH = 5;           // halo (11-tap window)
 constexpr int kS = kT + 2 * kH; // staged side = 26
-// LDS pitches chosen for conflict-free access: in both blur passes a 32-lane
-// half-wave covers two rows of 16 consecutive columns, so row pitches are
-// congruent to 16 modulo the 32 banks (staged inputs: 48; blurred rows: 16).
-constexpr int kPin = 48;
-constexpr int kPh = 16;
+constexpr int kPairs = kS * (kT / 2);  // horizontal tasks: staged rows x output column pairs = 208
 __constant__ float c_w[6] = {0.26601818f, 0.21300237f, 0.10936069f,
                              0.03599398f, 0.00759866f, 0.00102838f};
 
 constexpr float kC1 = 0.0001f, kC2 = 0.0009f;
 
+__device__ __forceinline__ float tap(int d) { return c_w[d < 0 ? -d : d]; }
+
 // Coefficient maps between the two kernels: per channel ch and pixel p one
-// float4 (C_mu, C_x2, C_xy, 0) at coef[ch * P + p].
+// float4 (C_mu, C_x2, C_xy, 0) at coef[ch * P + p]. Partial sums: per
+// channel ch and tile t the pair (sum |x - y|, sum SSIM) at partials[2 (ch T + t)].
 
 __global__ __launch_bounds__(256) void ssim_fwd_kernel(uint32_t W, uint32_t H, float lambda,
                                                        float dssim_scale,
@@ -44,88 +47,88 @@ __global__ __launch_bounds__(256) void ssim_fwd_kernel(uint32_t W, uint32_t H, f
                                                        double* __restrict__ partials,
                                                        const CamParams* __restrict__ view) {
     if (view) tgt += view->target_base;
-    __shared__ float sx[3][kS][kPin];
-    __shared__ float sy[3][kS][kPin];
-    __shared__ float hb[5][kS][kPh];
+    __shared__ float2 sxy[kS][kS + 2];   // (x, y) of this channel
+    __shared__ float4 hb4[kS][kT];       // horizontally blurred x, y, x^2, y^2
+    __shared__ float hb1[kS][kT];        // horizontally blurred xy
     __shared__ double s_red[2][4];
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const int ch = (int)blockIdx.z;
     // XCD-contiguous tile order: neighbouring tiles (shared halos) share an L2.
-    const uint32_t lb = xcd_swizzle(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const uint32_t tiles = gridDim.x * gridDim.y;
+    const uint32_t lb = xcd_swizzle(blockIdx.y * gridDim.x + blockIdx.x, tiles);
     const int bx = (int)(lb % gridDim.x), by = (int)(lb / gridDim.x);
     const int x0 = bx * kT - kH, y0 = by * kT - kH;
-    for (int k = threadIdx.x; k < kS * kS; k += 256) {
-        const int r = k / kS, c = k - r * kS;
-        const int gx = x0 + c, gy = y0 + r;
-        float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
-        if (gx >= 0 && gy >= 0 && gx < (int)W && gy < (int)H) {
-            a = img[(size_t)gy * W + gx];
-            b = tgt[(size_t)gy * W + gx];
+    const float* imgf = reinterpret_cast<const float*>(img) + ch;
+    const float* tgtf = reinterpret_cast<const float*>(tgt) + ch;
+    for (int r = ty; r < kS; r += 16) {
+        for (int c = tx; c < kS; c += 16) {
+            const int gx = x0 + c, gy = y0 + r;
+            float2 v = make_float2(0.f, 0.f);
+            if (gx >= 0 && gy >= 0 && gx < (int)W && gy < (int)H) {
+                const size_t q = 4 * ((size_t)gy * W + gx);
+                v = make_float2(imgf[q], tgtf[q]);
+            }
+            sxy[r][c] = v;
         }
-        sx[0][r][c] = a.x;
-        sx[1][r][c] = a.y;
-        sx[2][r][c] = a.z;
-        sy[0][r][c] = b.x;
-        sy[1][r][c] = b.y;
-        sy[2][r][c] = b.z;
     }
     __syncthreads();
+    if (tid < kPairs) {
+        const int r = tid >> 3, c0 = (tid & 7) * 2;
+        float a0[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, a1[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int j = 0; j < 2 * kH + 2; ++j) {
+            const float2 v = sxy[r][c0 + j];
+            const float m[5] = {v.x, v.y, v.x * v.x, v.y * v.y, v.x * v.y};
+            if (j <= 2 * kH) {
+                const float w = tap(j - kH);
+#pragma unroll
+                for (int k = 0; k < 5; ++k) a0[k] += w * m[k];
+            }
+            if (j >= 1) {
+                const float w = tap(j - 1 - kH);
+#pragma unroll
+                for (int k = 0; k < 5; ++k) a1[k] += w * m[k];
+            }
+        }
+        hb4[r][c0] = make_float4(a0[0], a0[1], a0[2], a0[3]);
+        hb1[r][c0] = a0[4];
+        hb4[r][c0 + 1] = make_float4(a1[0], a1[1], a1[2], a1[3]);
+        hb1[r][c0 + 1] = a1[4];
+    }
+    __syncthreads();
+    float mu_x = 0.f, mu_y = 0.f, bx2 = 0.f, by2 = 0.f, bxy = 0.f;
+#pragma unroll
+    for (int d = -kH; d <= kH; ++d) {
+        const float w = tap(d);
+        const float4 h = hb4[ty + kH + d][tx];
+        mu_x += w * h.x;
+        mu_y += w * h.y;
+        bx2 += w * h.z;
+        by2 += w * h.w;
+        bxy += w * hb1[ty + kH + d][tx];
+    }
     const int gx = bx * kT + tx, gy = by * kT + ty;
     const bool inside = gx < (int)W && gy < (int)H;
     const float inv_n3 = 1.0f / ((float)W * (float)H * 3.0f);
     const float dw = -lambda * dssim_scale * inv_n3;
     const size_t P = (size_t)W * H;
     double l1 = 0.0, ss = 0.0;
-    for (int ch = 0; ch < 3; ++ch) {
-        // Horizontal pass over all staged rows for the 16 output columns.
-        for (int k = threadIdx.x; k < kS * kT; k += 256) {
-            const int r = k / kT, c = k - r * kT;
-            float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f;
-#pragma unroll
-            for (int d = -kH; d <= kH; ++d) {
-                const float wgt = c_w[d < 0 ? -d : d];
-                const float xv = sx[ch][r][c + kH + d], yv = sy[ch][r][c + kH + d];
-                m0 += wgt * xv;
-                m1 += wgt * yv;
-                m2 += wgt * xv * xv;
-                m3 += wgt * yv * yv;
-                m4 += wgt * xv * yv;
-            }
-            hb[0][r][c] = m0;
-            hb[1][r][c] = m1;
-            hb[2][r][c] = m2;
-            hb[3][r][c] = m3;
-            hb[4][r][c] = m4;
-        }
-        __syncthreads();
-        float mu_x = 0.f, mu_y = 0.f, bx2 = 0.f, by2 = 0.f, bxy = 0.f;
-#pragma unroll
-        for (int d = -kH; d <= kH; ++d) {
-            const float wgt = c_w[d < 0 ? -d : d];
-            const int r = ty + kH + d;
-            mu_x += wgt * hb[0][r][tx];
-            mu_y += wgt * hb[1][r][tx];
-            bx2 += wgt * hb[2][r][tx];
-            by2 += wgt * hb[3][r][tx];
-            bxy += wgt * hb[4][r][tx];
-        }
-        __syncthreads();
-        // ssim.wgsl:95-126 (partials).
-        const float sx2 = bx2 - mu_x * mu_x, sy2 = by2 - mu_y * mu_y, sxy = bxy - mu_x * mu_y;
-        const float p = mu_x * mu_x + mu_y * mu_y + kC1;
-        const float q = sx2 + sy2 + kC2;
-        const float l = (2.0f * mu_x * mu_y + kC1) / p;
-        const float cs = (2.0f * sxy + kC2) / q;
-        const float s = l * cs;
-        const float ds_dmux = cs * (2.0f * mu_y - 2.0f * mu_x * l) / p + l * (-2.0f * mu_y + 2.0f * mu_x * cs) / q;
-        if (inside) {
-            coef[ch * P + (size_t)gy * W + gx] =
-                make_float4(ds_dmux * dw, (-l * cs / q) * dw, (2.0f * l / q) * dw, 0.0f);
-            ss += (double)s;
-            l1 += (double)fabsf(sx[ch][ty + kH][tx + kH] - sy[ch][ty + kH][tx + kH]);
-        }
+    // ssim.wgsl:95-126 (partials).
+    const float sx2 = bx2 - mu_x * mu_x, sy2 = by2 - mu_y * mu_y, sxy_ = bxy - mu_x * mu_y;
+    const float p = mu_x * mu_x + mu_y * mu_y + kC1;
+    const float q = sx2 + sy2 + kC2;
+    const float l = (2.0f * mu_x * mu_y + kC1) / p;
+    const float cs = (2.0f * sxy_ + kC2) / q;
+    const float s = l * cs;
+    const float ds_dmux = cs * (2.0f * mu_y - 2.0f * mu_x * l) / p + l * (-2.0f * mu_y + 2.0f * mu_x * cs) / q;
+    if (inside) {
+        coef[ch * P + (size_t)gy * W + gx] = make_float4(ds_dmux * dw, (-l * cs / q) * dw, (2.0f * l / q) * dw, 0.0f);
+        ss = (double)s;
+        const float2 c = sxy[ty + kH][tx + kH];
+        l1 = (double)fabsf(c.x - c.y);
     }
     // Deterministic block reduction of the loss sums.
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = tid & 63, wave = tid >> 6;
     for (int o = 32; o >= 1; o >>= 1) {
         l1 += __shfl_xor(l1, o);
         ss += __shfl_xor(ss, o);
@@ -135,16 +138,17 @@ __global__ __launch_bounds__(256) void ssim_fwd_kernel(uint32_t W, uint32_t H, f
         s_red[1][wave] = ss;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const size_t b = lb;
+    if (tid == 0) {
+        const size_t b = (size_t)ch * tiles + lb;
         partials[2 * b] = ((s_red[0][0] + s_red[0][1]) + s_red[0][2]) + s_red[0][3];
         partials[2 * b + 1] = ((s_red[1][0] + s_red[1][1]) + s_red[1][2]) + s_red[1][3];
     }
 }
 
-// One channel at a time: stage its three coefficient maps (one float4 per
-// pixel), blur them (the zero-padded symmetric blur is self-adjoint) and
-// combine dL/dx = (1-l) sign(x-y)/3P + B(C_mu) + 2x B(C_x2) + y B(C_xy).
+// One channel per workgroup: stage its three coefficient maps (one float4
+// per pixel), blur them (column pairs, then rows) and write that channel of
+// dL/dx = (1-l) sign(x-y)/3P + B(C_mu) + 2x B(C_x2) + y B(C_xy); the
+// channel-0 workgroup also writes w = 0.
 __global__ __launch_bounds__(256) void ssim_bwd_kernel(uint32_t W, uint32_t H, float lambda,
                                                        const float4* __restrict__ img,
                                                        const float4* __restrict__ tgt,
@@ -152,72 +156,78 @@ __global__ __launch_bounds__(256) void ssim_bwd_kernel(uint32_t W, uint32_t H, f
                                                        float4* __restrict__ dl_out,
                                                        const CamParams* __restrict__ view) {
     if (view) tgt += view->target_base;
-    __shared__ float sc[3][kS][kPin];
-    __shared__ float hb[3][kS][kPh];
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    __shared__ float4 sc[kS][kS + 1];
+    __shared__ float4 hb[kS][kT];
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const int ch = (int)blockIdx.z;
     const uint32_t lb = xcd_swizzle(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
     const int bx = (int)(lb % gridDim.x), by = (int)(lb / gridDim.x);
     const int x0 = bx * kT - kH, y0 = by * kT - kH;
     const size_t P = (size_t)W * H;
-    const int gx = bx * kT + tx, gy = by * kT + ty;
-    const bool inside = gx < (int)W && gy < (int)H;
-    float b[3][3];  // [channel][map]
-    for (int ch = 0; ch < 3; ++ch) {
-        for (int k = threadIdx.x; k < kS * kS; k += 256) {
-            const int r = k / kS, c = k - r * kS;
-            const int sxp = x0 + c, syp = y0 + r;
+    const float4* cch = coef + ch * P;
+    for (int r = ty; r < kS; r += 16) {
+        for (int c = tx; c < kS; c += 16) {
+            const int gx = x0 + c, gy = y0 + r;
             float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (sxp >= 0 && syp >= 0 && sxp < (int)W && syp < (int)H)
-                a = coef[ch * P + (size_t)syp * W + sxp];
-            sc[0][r][c] = a.x;
-            sc[1][r][c] = a.y;
-            sc[2][r][c] = a.z;
+            if (gx >= 0 && gy >= 0 && gx < (int)W && gy < (int)H) a = cch[(size_t)gy * W + gx];
+            sc[r][c] = a;
         }
-        __syncthreads();
-        for (int k = threadIdx.x; k < kS * kT; k += 256) {
-            const int r = k / kT, c = k - r * kT;
+    }
+    __syncthreads();
+    if (tid < kPairs) {
+        const int r = tid >> 3, c0 = (tid & 7) * 2;
+        float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = a0;
 #pragma unroll
-            for (int m = 0; m < 3; ++m) {
-                float acc = 0.f;
-#pragma unroll
-                for (int d = -kH; d <= kH; ++d) acc += c_w[d < 0 ? -d : d] * sc[m][r][c + kH + d];
-                hb[m][r][c] = acc;
+        for (int j = 0; j < 2 * kH + 2; ++j) {
+            const float4 v = sc[r][c0 + j];
+            if (j <= 2 * kH) {
+                const float w = tap(j - kH);
+                a0.x += w * v.x;
+                a0.y += w * v.y;
+                a0.z += w * v.z;
+            }
+            if (j >= 1) {
+                const float w = tap(j - 1 - kH);
+                a1.x += w * v.x;
+                a1.y += w * v.y;
+                a1.z += w * v.z;
             }
         }
-        __syncthreads();
-#pragma unroll
-        for (int m = 0; m < 3; ++m) {
-            float acc = 0.f;
-#pragma unroll
-            for (int d = -kH; d <= kH; ++d) acc += c_w[d < 0 ? -d : d] * hb[m][ty + kH + d][tx];
-            b[ch][m] = acc;
-        }
-        __syncthreads();  // sc / hb are reused by the next channel
+        hb[r][c0] = a0;
+        hb[r][c0 + 1] = a1;
     }
-    if (!inside) return;
+    __syncthreads();
+    float b0 = 0.f, b1 = 0.f, b2 = 0.f;
+#pragma unroll
+    for (int d = -kH; d <= kH; ++d) {
+        const float w = tap(d);
+        const float4 h = hb[ty + kH + d][tx];
+        b0 += w * h.x;
+        b1 += w * h.y;
+        b2 += w * h.z;
+    }
+    const int gx = bx * kT + tx, gy = by * kT + ty;
+    if (gx >= (int)W || gy >= (int)H) return;
     const size_t pix = (size_t)gy * W + gx;
-    const float4 x = img[pix], y = tgt[pix];
-    const float inv_n3 = 1.0f / ((float)W * (float)H * 3.0f);
-    const float l1w = (1.0f - lambda) * inv_n3;
-    auto sgn = [](float v) { return v > 0.f ? 1.f : (v < 0.f ? -1.f : 0.f); };
-    float4 o;
-    o.x = sgn(x.x - y.x) * l1w + b[0][0] + 2.0f * x.x * b[0][1] + y.x * b[0][2];
-    o.y = sgn(x.y - y.y) * l1w + b[1][0] + 2.0f * x.y * b[1][1] + y.y * b[1][2];
-    o.z = sgn(x.z - y.z) * l1w + b[2][0] + 2.0f * x.z * b[2][1] + y.z * b[2][2];
-    o.w = 0.0f;
-    dl_out[pix] = o;
+    const float x = reinterpret_cast<const float*>(img)[4 * pix + ch];
+    const float y = reinterpret_cast<const float*>(tgt)[4 * pix + ch];
+    const float l1w = (1.0f - lambda) / ((float)W * (float)H * 3.0f);
+    const float sg = x - y > 0.f ? 1.f : (x - y < 0.f ? -1.f : 0.f);
+    float* o = reinterpret_cast<float*>(dl_out) + 4 * pix;
+    o[ch] = sg * l1w + b0 + 2.0f * x * b1 + y * b2;
+    if (ch == 0) o[3] = 0.0f;
 }
 
 }  // namespace
 
-uint32_t l1_ssim_partials(uint32_t w, uint32_t h) { return div_up(w, kT) * div_up(h, kT); }
+uint32_t l1_ssim_partials(uint32_t w, uint32_t h) { return 3u * div_up(w, kT) * div_up(h, kT); }
 
 hipError_t launch_l1_ssim(hipStream_t st, uint32_t w, uint32_t h, float lambda, float dssim_scale,
                           const float4* img, const float4* target, float4* dl_out,
                           float* coef_scratch, double* partial_sums, uint32_t /*n_partials*/,
                           const CamParams* view) {
     if (w == 0 || h == 0) return hipSuccess;
-    dim3 grid(div_up(w, kT), div_up(h, kT));
+    dim3 grid(div_up(w, kT), div_up(h, kT), 3);
     float4* coef = reinterpret_cast<float4*>(coef_scratch);
     hipLaunchKernelGGL(ssim_fwd_kernel, grid, dim3(256), 0, st, w, h, lambda, dssim_scale, img, target,
                        coef, partial_sums, view);

@@ -1537,7 +1537,7 @@ gs_status gs_trainer_create(gs_ctx ctx, const gs_train_config* cfg, const gs_sce
     GS_CHECK(dalloc(&t->grad_sh, (size_t)n * kShFloats, "grad_sh"));
     GS_CHECK(dalloc(&t->dl_dcolor, (size_t)max_px * V, "dl_dcolor"));
     GS_CHECK(dalloc(&t->loss_coef, (size_t)max_px * 12, "loss coefficient maps"));
-    t->partials_per_view = div_up(t->max_w, 16) * div_up(t->max_h, 16);
+    t->partials_per_view = l1_ssim_partials(t->max_w, t->max_h);
     GS_CHECK(dalloc(&t->loss_partials, (size_t)t->partials_per_view * 2 * V, "loss partials"));
     GS_CHECK(upload_scene_into(ctx, init, n, t->positions, t->scales, t->quats, t->opacities, t->sh));
     t->sh_degree = 0;
@@ -1825,7 +1825,7 @@ gs_status gs_trainer_last_losses(gs_trainer t, double* out, uint32_t n_ids) {
     GS_REQUIRE(n_ids <= t->last_ids.size(), "more views requested than the last step had");
     for (uint32_t j = 0; j < n_ids; ++j) {
         const TrainViewDev& tv = t->views[t->last_ids[j]];
-        const uint32_t np = div_up(tv.width, 16) * div_up(tv.height, 16);
+        const uint32_t np = l1_ssim_partials(tv.width, tv.height);
         GS_CHECK(read_loss_partials(t->ctx->stream, t->loss_partials + (size_t)j * t->partials_per_view * 2,
                                     np, tv.width, tv.height, t->cfg.dssim_scale, &out[2 * j],
                                     &out[2 * j + 1]));
