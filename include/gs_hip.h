@@ -343,6 +343,15 @@ gs_status gs_trainer_set_schedule(gs_trainer t, uint32_t digit_bits, uint32_t bl
  * tile 32: fwd 256 | 512, bwd 128 | 256. Tiles other than 16 need
  * GS_BINNING_TIGHT. Default (16, 256, 128). */
 gs_status gs_trainer_set_tiling(gs_trainer t, uint32_t tile_size, uint32_t fwd_batch, uint32_t bwd_batch);
+/* Overlapped batched steps (default off; no counterpart in the reference):
+ * with two or more views the step splits them in halves and runs each
+ * half's L1 + D-SSIM on a second stream beside the other half's raster pass
+ * (forward of the second half, backward of the first). Same result as the
+ * in-order step up to float-atomic order; profiled steps always run in
+ * order so their stage times stay separable. On MI355X the persistent raster
+ * passes already fill every CU, so the split costs more than it hides on the
+ * 1M-surfel 4 x 1080p step and gains nothing on the 16-frame 720p batch. */
+gs_status gs_trainer_set_overlap(gs_trainer t, int enable);
 /* Blocking: per-view (l1_mean, dssim_mean) of the last step, 2 per view. */
 gs_status gs_trainer_last_losses(gs_trainer t, double* out, uint32_t n_ids);
 /* Blocking: render view `view` with the current parameters into rgba (W*H*4). */

@@ -458,6 +458,9 @@ class Trainer:
     def set_tiling(self, tile_size: int, fwd_batch: int, bwd_batch: int):
         check(lib().gs_trainer_set_tiling(self.h, tile_size, fwd_batch, bwd_batch))
 
+    def set_overlap(self, enable: bool):
+        check(lib().gs_trainer_set_overlap(self.h, 1 if enable else 0))
+
     def autotune(self, view_ids, reps: int = 2) -> dict:
         """Time the candidate schedules on real batched steps and keep the
         fastest; returns {"digit_bits", "blocks_per_cu" (0 = one workgroup

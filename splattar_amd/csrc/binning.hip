@@ -217,7 +217,7 @@ __global__ __launch_bounds__(256) void emit_tile_keys_kernel(
 }
 
 __global__ __launch_bounds__(256) void tile_order_keys_kernel(const uint32_t* __restrict__ ranges,
-                                                              uint32_t total_tiles,
+                                                              uint32_t tile_begin, uint32_t total_tiles,
                                                               uint32_t* __restrict__ keys,
                                                               uint32_t* __restrict__ vals,
                                                               uint32_t* __restrict__ count,
@@ -226,9 +226,10 @@ __global__ __launch_bounds__(256) void tile_order_keys_kernel(const uint32_t* __
     if (t == 0) *count = total_tiles;
     if (t < 2 * kTileQueues) cursors[t] = 0u;
     if (t >= total_tiles) return;
-    const uint32_t len = ranges[2 * (size_t)t + 1] - ranges[2 * (size_t)t];
+    const size_t g = (size_t)tile_begin + t;
+    const uint32_t len = ranges[2 * g + 1] - ranges[2 * g];
     keys[t] = 0xFFFFu - min(len, 0xFFFFu);
-    vals[t] = t;
+    vals[t] = (uint32_t)g;
 }
 
 __global__ __launch_bounds__(256) void tile_ranges_u32_kernel(const uint32_t* __restrict__ keys,
@@ -324,11 +325,12 @@ hipError_t launch_emit_tile_keys(hipStream_t st, uint32_t n_items, const uint32_
     return hipGetLastError();
 }
 
-hipError_t launch_tile_order_keys(hipStream_t st, const uint32_t* ranges, uint32_t total_tiles,
-                                  uint32_t* keys, uint32_t* vals, uint32_t* count, uint32_t* cursors) {
+hipError_t launch_tile_order_keys(hipStream_t st, const uint32_t* ranges, uint32_t tile_begin,
+                                  uint32_t total_tiles, uint32_t* keys, uint32_t* vals, uint32_t* count,
+                                  uint32_t* cursors) {
     const uint32_t blocks = div_up(total_tiles > 0 ? total_tiles : 1, 256);
-    hipLaunchKernelGGL(tile_order_keys_kernel, dim3(blocks), dim3(256), 0, st, ranges, total_tiles, keys,
-                       vals, count, cursors);
+    hipLaunchKernelGGL(tile_order_keys_kernel, dim3(blocks), dim3(256), 0, st, ranges, tile_begin, total_tiles,
+                       keys, vals, count, cursors);
     return hipGetLastError();
 }
 

@@ -534,12 +534,13 @@ hipError_t launch_emit_tile_keys(hipStream_t st, uint32_t n_items, const uint32_
                                  const CamParams* cams_dev, uint32_t items_per_view, uint32_t* tkeys,
                                  uint32_t* payloads, uint32_t max_entries, const uint32_t* total,
                                  uint32_t* sort_count, uint32_t* overflow_flag);
-// Longest-list-first tile order: keys[t] = 0xFFFF - min(list length, 0xFFFF),
-// vals[t] = t for the total_tiles tiles, *count = total_tiles and the
-// 2 kTileQueues cursors = 0 (sort the pairs by 16 key bits, stably, to get
-// the order).
-hipError_t launch_tile_order_keys(hipStream_t st, const uint32_t* ranges, uint32_t total_tiles,
-                                  uint32_t* keys, uint32_t* vals, uint32_t* count, uint32_t* cursors);
+// Longest-list-first order of the tiles [tile_begin, tile_begin + total_tiles):
+// keys[t] = 0xFFFF - min(list length, 0xFFFF), vals[t] = tile_begin + t,
+// *count = total_tiles and the 2 kTileQueues cursors = 0 (sort the pairs by
+// 16 key bits, stably, to get the order).
+hipError_t launch_tile_order_keys(hipStream_t st, const uint32_t* ranges, uint32_t tile_begin,
+                                  uint32_t total_tiles, uint32_t* keys, uint32_t* vals, uint32_t* count,
+                                  uint32_t* cursors);
 hipError_t launch_tile_ranges_u32(hipStream_t st, const uint32_t* keys, const uint32_t* count,
                                   uint32_t capacity, uint32_t* ranges);
 hipError_t launch_compose_keys(hipStream_t st, const uint32_t* tkeys, const uint32_t* payloads,
@@ -592,11 +593,13 @@ struct RasterShape {
     uint32_t fwd_batch = 256;
     uint32_t bwd_batch = 128;
 };
+// The pass covers total_tiles tiles: queue.order[0 .. total_tiles) with a
+// queue, else the global tiles [tile_begin, tile_begin + total_tiles).
 hipError_t launch_raster_fwd(hipStream_t st, const CamParams* cams_dev, const CamParams* cams_host,
                              uint32_t n_views, uint32_t total_tiles, const uint32_t* ranges,
                              const uint32_t* payloads, const Record* recs, FwdOut out,
                              bool cull = true, TileQueue queue = {nullptr, nullptr, 0},
-                             bool aux_outputs = true, RasterShape shape = {});
+                             bool aux_outputs = true, RasterShape shape = {}, uint32_t tile_begin = 0);
 
 // raster_bwd.hip
 struct BwdIn {
@@ -610,7 +613,8 @@ hipError_t launch_raster_bwd(hipStream_t st, const CamParams* cams_dev, const Ca
                              uint32_t n_views, uint32_t total_tiles, const uint32_t* ranges,
                              const uint32_t* payloads, const Record* recs, BwdIn in,
                              float* grad_coef, bool aux, bool cull = true,
-                             TileQueue queue = {nullptr, nullptr, 0}, RasterShape shape = {});
+                             TileQueue queue = {nullptr, nullptr, 0}, RasterShape shape = {},
+                             uint32_t tile_begin = 0);
 
 // geom_bwd.hip
 struct ParamGrads {

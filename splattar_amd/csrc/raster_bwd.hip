@@ -233,7 +233,7 @@ template <bool AUX, int TW, int BATCH>
 __global__ __launch_bounds__(TW == 8 ? 64 : TW * TW / 2) void raster_bwd_kernel(
     const CamParams* __restrict__ cams, uint32_t n_views, const uint32_t* __restrict__ ranges,
     const uint32_t* __restrict__ payloads, const Record* __restrict__ recs, BwdIn in,
-    float* __restrict__ grad_coef, bool cull, uint32_t total_tiles, TileQueue queue) {
+    float* __restrict__ grad_coef, bool cull, uint32_t total_tiles, TileQueue queue, uint32_t tile_begin) {
     constexpr int NS = AUX ? 20 : 16;        // LDS slots per record
     constexpr bool TWO = TW >= 16;           // two pixels per lane (an 8x16 column per wave)
     constexpr int NC = TW / 8;               // 8x8 blocks per tile row
@@ -269,7 +269,7 @@ __global__ __launch_bounds__(TW == 8 ? 64 : TW * TW / 2) void raster_bwd_kernel(
             pos = iter == 0 ? xcd_swizzle(blockIdx.x, gridDim.x) : total_tiles;
         }
         if (pos >= total_tiles) break;
-        const uint32_t gtile = queue.order != nullptr ? queue.order[pos] : pos;
+        const uint32_t gtile = queue.order != nullptr ? queue.order[pos] : tile_begin + pos;
         const uint32_t start = ranges[2 * (size_t)gtile], end = ranges[2 * (size_t)gtile + 1];
         if (end <= start) continue;
         if (tid == 0) {
@@ -422,10 +422,10 @@ __global__ __launch_bounds__(TW == 8 ? 64 : TW * TW / 2) void raster_bwd_kernel(
 template <bool AUX, int TW, int BATCH>
 void launch_bwd(hipStream_t st, uint32_t grid, const CamParams* cams_dev, uint32_t n_views,
                 uint32_t total_tiles, const uint32_t* ranges, const uint32_t* payloads, const Record* recs,
-                BwdIn in, float* grad_coef, bool cull, TileQueue queue) {
+                BwdIn in, float* grad_coef, bool cull, TileQueue queue, uint32_t tile_begin) {
     constexpr int NT = TW == 8 ? 64 : TW * TW / 2;
     hipLaunchKernelGGL((raster_bwd_kernel<AUX, TW, BATCH>), dim3(grid), dim3(NT), 0, st, cams_dev, n_views,
-                       ranges, payloads, recs, in, grad_coef, cull, total_tiles, queue);
+                       ranges, payloads, recs, in, grad_coef, cull, total_tiles, queue, tile_begin);
 }
 
 }  // namespace
@@ -433,13 +433,15 @@ void launch_bwd(hipStream_t st, uint32_t grid, const CamParams* cams_dev, uint32
 hipError_t launch_raster_bwd(hipStream_t st, const CamParams* cams_dev, const CamParams* /*cams_host*/,
                              uint32_t n_views, uint32_t total_tiles, const uint32_t* ranges,
                              const uint32_t* payloads, const Record* recs, BwdIn in,
-                             float* grad_coef, bool aux, bool cull, TileQueue queue, RasterShape shape) {
+                             float* grad_coef, bool aux, bool cull, TileQueue queue, RasterShape shape,
+                             uint32_t tile_begin) {
     if (total_tiles == 0) return hipSuccess;
     const uint32_t grid = queue.cursor != nullptr ? std::min(total_tiles, std::max(queue.grid_blocks, 1u))
                                                   : total_tiles;
     const uint32_t t = shape.tile, b = shape.bwd_batch;
     auto go = [&](auto fn) {
-        fn(st, grid, cams_dev, n_views, total_tiles, ranges, payloads, recs, in, grad_coef, cull, queue);
+        fn(st, grid, cams_dev, n_views, total_tiles, ranges, payloads, recs, in, grad_coef, cull, queue,
+           tile_begin);
     };
     if (aux) {
         if (t != 16 || b != 128) return hipErrorInvalidValue;  // depth / normal losses: 16-px tiles

@@ -39,7 +39,7 @@ template <bool AUX, int TW, int BATCH>
 __global__ __launch_bounds__((TW / 8) * (TW / 8) * 64) void raster_fwd_kernel(
     const CamParams* __restrict__ cams, uint32_t n_views, const uint32_t* __restrict__ ranges,
     const uint32_t* __restrict__ payloads, const Record* __restrict__ recs, FwdOut out, bool cull,
-    uint32_t total_tiles, TileQueue queue) {
+    uint32_t total_tiles, TileQueue queue, uint32_t tile_begin) {
     constexpr int NC = TW / 8;             // blocks per tile row
     constexpr int NB = NC * NC;            // blocks (= waves) per tile
     constexpr int NT = NB * 64;            // threads
@@ -68,7 +68,7 @@ __global__ __launch_bounds__((TW / 8) * (TW / 8) * 64) void raster_fwd_kernel(
             pos = iter == 0 ? xcd_swizzle(blockIdx.x, gridDim.x) : total_tiles;
         }
         if (pos >= total_tiles) break;
-        const uint32_t gtile = queue.order != nullptr ? queue.order[pos] : pos;
+        const uint32_t gtile = queue.order != nullptr ? queue.order[pos] : tile_begin + pos;
         if (tid == 0) s_view = view_of_tile(cams, n_views, gtile);
         __syncthreads();
         const CamParams& cam = cams[s_view];
@@ -181,10 +181,10 @@ __global__ __launch_bounds__((TW / 8) * (TW / 8) * 64) void raster_fwd_kernel(
 template <bool AUX, int TW, int BATCH>
 void launch_fwd(hipStream_t st, uint32_t grid, const CamParams* cams_dev, uint32_t n_views,
                 uint32_t total_tiles, const uint32_t* ranges, const uint32_t* payloads, const Record* recs,
-                FwdOut out, bool cull, TileQueue queue) {
+                FwdOut out, bool cull, TileQueue queue, uint32_t tile_begin) {
     constexpr int NT = (TW / 8) * (TW / 8) * 64;
     hipLaunchKernelGGL((raster_fwd_kernel<AUX, TW, BATCH>), dim3(grid), dim3(NT), 0, st, cams_dev, n_views,
-                       ranges, payloads, recs, out, cull, total_tiles, queue);
+                       ranges, payloads, recs, out, cull, total_tiles, queue, tile_begin);
 }
 
 }  // namespace
@@ -192,12 +192,14 @@ void launch_fwd(hipStream_t st, uint32_t grid, const CamParams* cams_dev, uint32
 hipError_t launch_raster_fwd(hipStream_t st, const CamParams* cams_dev, const CamParams* /*cams_host*/,
                              uint32_t n_views, uint32_t total_tiles, const uint32_t* ranges,
                              const uint32_t* payloads, const Record* recs, FwdOut out, bool cull,
-                             TileQueue queue, bool aux_outputs, RasterShape shape) {
+                             TileQueue queue, bool aux_outputs, RasterShape shape, uint32_t tile_begin) {
     if (total_tiles == 0) return hipSuccess;
     const uint32_t grid = queue.cursor != nullptr ? std::min(total_tiles, std::max(queue.grid_blocks, 1u))
                                                   : total_tiles;
     const uint32_t t = shape.tile, b = shape.fwd_batch;
-    auto go = [&](auto fn) { fn(st, grid, cams_dev, n_views, total_tiles, ranges, payloads, recs, out, cull, queue); };
+    auto go = [&](auto fn) {
+        fn(st, grid, cams_dev, n_views, total_tiles, ranges, payloads, recs, out, cull, queue, tile_begin);
+    };
     if (aux_outputs) {
         if (t != 16 || b != 256) return hipErrorInvalidValue;  // depth / normal outputs: 16-px tiles
         go(launch_fwd<true, 16, 256>);

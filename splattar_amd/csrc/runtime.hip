@@ -555,11 +555,14 @@ struct RasterCore {
     // the two work cursors (forward, backward) of the persistent raster passes.
     uint32_t* order_keys[2] = {nullptr, nullptr};
     uint32_t* order_vals[2] = {nullptr, nullptr};
-    uint32_t* order_count = nullptr;
-    uint32_t* cursors = nullptr;
+    uint32_t* order_count = nullptr;  // one per tile range
+    uint32_t* cursors = nullptr;      // per tile range: forward, backward cursor sets
     bool tile_queue = true;
     uint32_t queue_blocks = 0;       // resident workgroups of a persistent raster launch
-    uint32_t queue_tiles = 0;        // tiles covered by the current order
+    // Tile ranges scheduled by the last forward: range s covers global tiles
+    // [q_begin[s], q_begin[s] + q_tiles[s]) (q_tiles 0: no queue).
+    static constexpr int kRanges = 2;
+    uint32_t q_begin[kRanges] = {0, 0}, q_tiles[kRanges] = {0, 0};
     uint32_t* tkeys[2] = {nullptr, nullptr};  // 32-bit tile keys, stored inside keys[1]
     uint32_t binning = GS_BINNING_PRESORT;
     bool cull = true;                // per-block footprint culling in the rasterizers
@@ -652,8 +655,8 @@ struct RasterCore {
             GS_CHECK(dalloc(&order_keys[i], tiles ? tiles : 1, "tile order keys"));
             GS_CHECK(dalloc(&order_vals[i], tiles ? tiles : 1, "tile order"));
         }
-        GS_CHECK(dalloc(&order_count, 1, "tile order count"));
-        GS_CHECK(dalloc(&cursors, 2 * kTileQueues, "tile cursors"));
+        GS_CHECK(dalloc(&order_count, kRanges, "tile order count"));
+        GS_CHECK(dalloc(&cursors, 2 * kRanges * kTileQueues, "tile cursors"));
         {
             // Persistent raster launches: 8 workgroups per CU (more than can
             // be resident; the rest start as slots free up). GS_TILE_QUEUE=0
@@ -741,9 +744,11 @@ struct RasterCore {
 
     // Forward over views [v0, v0 + nv) of the current set (nv > 1 only when
     // the set was laid out batched). Parameters are the activated buffers.
+    // With raster = false it stops after the tile ranges (the raster passes
+    // then run per tile range: raster_fwd_range / raster_bwd_range).
     gs_status forward(hipStream_t st, uint32_t v0, uint32_t nv, uint32_t n, const float4* pos,
                       const float2* sc, const float4* q, const float* op, const float* sh,
-                      Profiler* prof = nullptr) {
+                      Profiler* prof = nullptr, bool raster = true) {
         if (n > cap_n) return fail(GS_ERR_CAPACITY, "num_surfels exceeds capacity");
         if (shape.tile != (uint32_t)kTile && binning != GS_BINNING_TIGHT)
             return fail(GS_ERR_INVALID, "work tiles other than 16 px need GS_BINNING_TIGHT");
@@ -794,10 +799,11 @@ struct RasterCore {
             GS_HIP(launch_tile_ranges_u32(st, tkeys[result_alt], sort_count + v0, cap_entries, ranges));
             keys64_valid = false;
             last_sort_count = sort_count + v0;
-            GS_CHECK(schedule_tiles(st, tiles));
+            if (!raster) return GS_OK;
+            GS_CHECK(schedule_range(st, 0, 0, tiles));
             GS_CHECK(P.mark(st, GS_STAGE_RASTER_FWD));
             GS_HIP(launch_raster_fwd(st, cd, cams.data() + v0, nv, tiles, ranges, pays[result_alt], recs, o,
-                                     cull, queue(0), aux_outputs, shape));
+                                     cull, queue(0, 0), aux_outputs, shape));
             return GS_OK;
         }
         // 64-bit (tile || depth) key path: one sort over every key bit that varies.
@@ -818,31 +824,58 @@ struct RasterCore {
         keys64_valid = true;
         keys_lean = false;
         last_sort_count = sort_count + v0;
-        GS_CHECK(schedule_tiles(st, tiles));
+        if (!raster) return GS_OK;
+        GS_CHECK(schedule_range(st, 0, 0, tiles));
         GS_CHECK(P.mark(st, GS_STAGE_RASTER_FWD));
         GS_HIP(launch_raster_fwd(st, cd, cams.data() + v0, nv, tiles, ranges, pays[result_alt], recs, o,
-                                 cull, queue(0), aux_outputs, shape));
+                                 cull, queue(0, 0), aux_outputs, shape));
         return GS_OK;
     }
 
-    // Longest-list-first order of the tiles just binned (16-bit keys, two
-    // stable 8-bit passes) and both work cursors reset to 0.
-    gs_status schedule_tiles(hipStream_t st, uint32_t tiles) {
-        queue_tiles = 0;
-        if (!tile_queue || tiles == 0) return GS_OK;
-        GS_HIP(launch_tile_order_keys(st, ranges, tiles, order_keys[0], order_vals[0], order_count, cursors));
+    // Longest-list-first order of the binned tiles [t0, t0 + nt) (16-bit
+    // keys, two stable 8-bit passes) as tile range `slot`, kept in the
+    // segment [t0, t0 + nt) of the order buffers, and its two work cursor
+    // sets reset to 0.
+    gs_status schedule_range(hipStream_t st, int slot, uint32_t t0, uint32_t nt) {
+        q_begin[slot] = t0;
+        q_tiles[slot] = 0;
+        if (!tile_queue || nt == 0) return GS_OK;
+        GS_HIP(launch_tile_order_keys(st, ranges, t0, nt, order_keys[0] + t0, order_vals[0] + t0,
+                                      order_count + slot, cursors + 2 * slot * kTileQueues));
         int alt = 0;
-        GS_HIP(launch_radix_sort_u32(st, order_keys[0], order_vals[0], order_keys[1], order_vals[1],
-                                     order_count, tiles, 0, 16, 8, sort_scratch, &alt));
+        GS_HIP(launch_radix_sort_u32(st, order_keys[0] + t0, order_vals[0] + t0, order_keys[1] + t0,
+                                     order_vals[1] + t0, order_count + slot, nt, 0, 16, 8, sort_scratch, &alt));
         if (alt != 0) return fail(GS_ERR_INVALID, "tile order sort ended in the alternate buffer");
-        queue_tiles = tiles;
+        q_tiles[slot] = nt;
         return GS_OK;
     }
 
-    // Queue of a raster pass (cursor set 0 = forward, 1 = backward), or none.
-    TileQueue queue(int which) const {
-        if (queue_tiles == 0) return TileQueue{nullptr, nullptr, 0};
-        return TileQueue{order_vals[0], cursors + which * kTileQueues, queue_blocks};
+    // Queue of a raster pass over tile range `slot` (which: 0 = forward,
+    // 1 = backward cursor set), or none.
+    TileQueue queue(int slot, int which) const {
+        if (q_tiles[slot] == 0) return TileQueue{nullptr, nullptr, 0};
+        return TileQueue{order_vals[0] + q_begin[slot], cursors + (2 * slot + which) * kTileQueues, queue_blocks};
+    }
+
+    // Raster passes over the binned tiles [t0, t0 + nt) of views
+    // [v0, v0 + nv) (tile ids are global to the view set), as tile range
+    // `slot`; the forward schedules the range's queue first.
+    gs_status raster_fwd_range(hipStream_t st, int slot, uint32_t v0, uint32_t nv, uint32_t t0, uint32_t nt) {
+        FwdOut o{out_color, out_aux, out_normal, ncontrib};
+        GS_CHECK(schedule_range(st, slot, t0, nt));
+        GS_HIP(launch_raster_fwd(st, cams_dev + v0, cams.data() + v0, nv, nt, ranges, pays[result_alt], recs, o,
+                                 cull, queue(slot, 0), aux_outputs, shape, t0));
+        return GS_OK;
+    }
+    gs_status raster_bwd_range(hipStream_t st, int slot, uint32_t v0, uint32_t nv, uint32_t t0, uint32_t nt,
+                               const float4* dl_dcolor) {
+        BwdIn in{dl_dcolor, nullptr, out_color, out_aux, ncontrib};
+        const TileQueue qb = (q_tiles[slot] == nt && q_begin[slot] == t0) ? queue(slot, 1)
+                                                                            : TileQueue{nullptr, nullptr, 0};
+        if (qb.cursor != nullptr) GS_HIP(launch_fill_u32(st, qb.cursor, 0u, kTileQueues));
+        GS_HIP(launch_raster_bwd(st, cams_dev + v0, cams.data() + v0, nv, nt, ranges, pays[result_alt], recs, in,
+                                 grad_coef, false, cull, qb, shape, t0));
+        return GS_OK;
     }
 
     gs_status backward(hipStream_t st, uint32_t v0, uint32_t nv, uint32_t n, const float4* pos,
@@ -859,7 +892,7 @@ struct RasterCore {
                                (size_t)n * nv * kGradStride));
         BwdIn in{dl_dcolor, dl_dnormal, out_color, out_aux, ncontrib};
         GS_CHECK(P.mark(st, GS_STAGE_RASTER_BWD));
-        const TileQueue qb = queue_tiles == tiles ? queue(1) : TileQueue{nullptr, nullptr, 0};
+        const TileQueue qb = (q_tiles[0] == tiles && q_begin[0] == 0) ? queue(0, 1) : TileQueue{nullptr, nullptr, 0};
         if (qb.cursor != nullptr) GS_HIP(launch_fill_u32(st, qb.cursor, 0u, kTileQueues));
         GS_HIP(launch_raster_bwd(st, cd, cams.data() + v0, nv, tiles, ranges, pays[result_alt], recs, in,
                                  grad_coef, aux, cull, qb, shape));
@@ -1359,8 +1392,16 @@ struct gs_trainer_s {
     std::vector<uint32_t> graph_shape;   // (w, h) per view of the captured step
     bool tuned = false;
     gs::Profiler prof;
+    // Pipelined batched step (opt-in): the loss of one half of the views runs
+    // on a second stream while the raster passes of the other half run.
+    bool pipeline = false;
+    hipStream_t side = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
     ~gs_trainer_s() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (side) (void)hipStreamDestroy(side);
         dfree(saved);
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -1383,6 +1424,18 @@ struct gs_trainer_s {
 
     gs_status enqueue_step(hipStream_t st, const uint32_t* ids, uint32_t k, bool batched,
                            bool profile = false);
+    gs_status enqueue_pipelined(hipStream_t st, const uint32_t* ids, uint32_t k);
+    // L1 + D-SSIM of view slot j (the coefficient scratch is shared: the
+    // losses of one step run in stream order on one stream).
+    gs_status loss_view(hipStream_t s, const uint32_t* ids, uint32_t j) {
+        const TrainViewDev& tv = views[ids[j]];
+        const CamParams& c = core.cams[j];
+        GS_HIP(launch_l1_ssim(s, tv.width, tv.height, cfg.lambda, cfg.dssim_scale, core.out_color + c.pix_base,
+                              targets, dl_dcolor + c.pix_base, loss_coef,
+                              loss_partials + (size_t)j * partials_per_view * 2, partials_per_view,
+                              core.cams_dev + j));
+        return GS_OK;
+    }
 
     // Training-state snapshot (raw, m, v of every optimizer class and the
     // Adam step state) in one device buffer; restore also re-activates. The
@@ -1435,17 +1488,11 @@ gs_status gs_trainer_s::enqueue_step(hipStream_t st, const uint32_t* ids, uint32
                                      bool profile) {
     ParamGrads g{grad_pos, grad_scales, grad_quat, grad_opacity, grad_sh, nullptr};
     gs::Profiler* P = profile ? &prof : nullptr;
+    if (batched && P == nullptr && pipeline && k >= 2) return enqueue_pipelined(st, ids, k);
     if (batched) {
         GS_CHECK(core.forward(st, 0, k, n, positions, scales, quats, opacities, sh, P));
         if (P) GS_CHECK(P->mark(st, GS_STAGE_LOSS));
-        for (uint32_t j = 0; j < k; ++j) {
-            const TrainViewDev& tv = views[ids[j]];
-            const CamParams& c = core.cams[j];
-            GS_HIP(launch_l1_ssim(st, tv.width, tv.height, cfg.lambda, cfg.dssim_scale,
-                                  core.out_color + c.pix_base, targets, dl_dcolor + c.pix_base, loss_coef,
-                                  loss_partials + (size_t)j * partials_per_view * 2, partials_per_view,
-                                  core.cams_dev + j));
-        }
+        for (uint32_t j = 0; j < k; ++j) GS_CHECK(loss_view(st, ids, j));
         GS_CHECK(core.backward(st, 0, k, n, positions, scales, quats, sh, dl_dcolor, nullptr, g, false,
                                false, P));
     } else {
@@ -1467,6 +1514,43 @@ gs_status gs_trainer_s::enqueue_step(hipStream_t st, const uint32_t* ids, uint32
         GS_CHECK(P->mark(st, -1));
         P->steps += 1;
     }
+    return GS_OK;
+}
+
+// Batched step with the views split in halves A = [0, k/2), B = [k/2, k):
+//   stream st:   bin all views | fwd(A) | fwd(B) | .. bwd(A) | .. bwd(B) | geom | Adam
+//   side stream:                   loss(A) after fwd(A) | loss(B) after fwd(B)
+// so each half's memory-bound loss runs beside a VALU-bound raster pass of
+// the other half; bwd(A) waits for loss(A), bwd(B) for loss(B). Events fork
+// and join the side stream, so the step captures into one graph as well.
+gs_status gs_trainer_s::enqueue_pipelined(hipStream_t st, const uint32_t* ids, uint32_t k) {
+    if (side == nullptr) {
+        GS_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        for (auto& e : ev) GS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    ParamGrads g{grad_pos, grad_scales, grad_quat, grad_opacity, grad_sh, nullptr};
+    GS_CHECK(core.forward(st, 0, k, n, positions, scales, quats, opacities, sh, nullptr, false));
+    const uint32_t ka = k / 2;
+    const uint32_t ta = core.cams[ka].tile_base;
+    const uint32_t tt = core.total_tiles;
+    GS_HIP(launch_fill_u32(st, reinterpret_cast<uint32_t*>(core.grad_coef), 0u, (size_t)n * k * kGradStride));
+    GS_CHECK(core.raster_fwd_range(st, 0, 0, k, 0, ta));
+    GS_HIP(hipEventRecord(ev[0], st));
+    GS_CHECK(core.raster_fwd_range(st, 1, 0, k, ta, tt - ta));
+    GS_HIP(hipEventRecord(ev[1], st));
+    GS_HIP(hipStreamWaitEvent(side, ev[0], 0));
+    for (uint32_t j = 0; j < ka; ++j) GS_CHECK(loss_view(side, ids, j));
+    GS_HIP(hipEventRecord(ev[2], side));
+    GS_HIP(hipStreamWaitEvent(side, ev[1], 0));
+    for (uint32_t j = ka; j < k; ++j) GS_CHECK(loss_view(side, ids, j));
+    GS_HIP(hipEventRecord(ev[3], side));
+    GS_HIP(hipStreamWaitEvent(st, ev[2], 0));
+    GS_CHECK(core.raster_bwd_range(st, 0, 0, k, 0, ta, dl_dcolor));
+    GS_HIP(hipStreamWaitEvent(st, ev[3], 0));
+    GS_CHECK(core.raster_bwd_range(st, 1, 0, k, ta, tt - ta, dl_dcolor));
+    GS_HIP(launch_geom_bwd(st, core.cams_dev, k, n, positions, scales, quats, sh, core.recs, core.grad_coef, g,
+                           false, core.cam_partials));
+    GS_CHECK(gs_optim_encode_step(optim, st));
     return GS_OK;
 }
 
@@ -1516,6 +1600,10 @@ gs_status gs_trainer_create(gs_ctx ctx, const gs_train_config* cfg, const gs_sce
     // depth and normal images (aux.y = 0; out_normal is left untouched).
     t->core.aux_outputs = false;
     t->core.lean = true;
+    {
+        const char* env = std::getenv("GS_PIPELINE");
+        t->pipeline = env != nullptr && env[0] == '1';
+    }
     GS_CHECK(dalloc(&t->targets, toff, "targets"));
     {
         std::vector<float> buf;
@@ -1678,6 +1766,13 @@ gs_status gs_trainer_set_schedule(gs_trainer t, uint32_t digit_bits, uint32_t bl
     t->core.digit_bits = digit_bits;
     t->core.tile_queue = blocks_per_cu != 0;
     t->core.queue_blocks = std::max(1u, blocks_per_cu) * cus;
+    drop_graph(t);
+    return GS_OK;
+}
+
+gs_status gs_trainer_set_overlap(gs_trainer t, int enable) {
+    GS_REQUIRE(t != nullptr, "trainer is null");
+    t->pipeline = enable != 0;
     drop_graph(t);
     return GS_OK;
 }

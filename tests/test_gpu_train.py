@@ -252,6 +252,38 @@ def test_every_schedule_gives_the_same_step(gpu):
 TILINGS = [(16, 256, 128), (16, 128, 64), (32, 256, 128), (32, 512, 256), (8, 128, 128), (8, 64, 64)]
 
 
+def test_overlapped_step_equals_in_order_step(gpu):
+    """The overlapped batched step (each half's loss on a second stream beside
+    the other half's raster pass) against the same step in stream order:
+    three graph steps and three plain batched steps over 4 views of two
+    sizes give the same parameters and losses up to float-atomic order, and
+    every graph replay agrees with the eager overlapped step."""
+    import splattar_amd as sa
+    deg = 2
+    sc, tgt, specs = _small_training_problem(n=3000, views=4, size=(96, 64), deg=deg, seed=5)
+    specs[1].width, specs[1].height = 80, 48
+    specs[3].width, specs[3].height = 80, 48
+    targets = _render_targets(gpu, tgt, specs, deg)
+    results = {}
+    for overlap in (True, False):
+        for mode in (sa.Trainer.GRAPH, sa.Trainer.BATCHED):
+            t = _trainer(gpu, sc, specs, targets, deg, 4)
+            t.set_overlap(overlap)
+            announce(f"overlap={overlap} mode={mode}")
+            per_step = []
+            for it in range(3):
+                t.step([(it + j) % 4 for j in range(4)], mode=mode)
+                per_step.append((_params(t), t.last_losses(4)))
+            results[(overlap, mode)] = per_step
+            t.close()
+    ref = results[(False, sa.Trainer.BATCHED)]
+    for key, steps in results.items():
+        for it, ((p, l), (p0, l0)) in enumerate(zip(steps, ref)):
+            np.testing.assert_allclose(l, l0, rtol=1e-5, err_msg=f"{key} step {it}")
+            for k in p0:
+                np.testing.assert_allclose(p[k], p0[k], rtol=1e-4, atol=2e-6, err_msg=f"{key} step {it} {k}")
+
+
 def test_every_work_tiling_gives_the_same_images_and_step(gpu):
     """Work tiles of 8, 16 and 32 px and every LDS batch size: the per-pixel
     compositing sequence is the same (coverage stays the 16-px rects of the
