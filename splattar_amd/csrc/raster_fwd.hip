@@ -137,15 +137,16 @@ __global__ __launch_bounds__((TW / 8) * (TW / 8) * 64) void raster_fwd_kernel(
             for (int w = 0; w < SW; ++w) my_count += s_qcnt[w][wave];
             __syncthreads();
             if (__any(!done)) {
-                for (uint32_t k = 0; k < my_count; ++k) {
-                    const uint32_t j = s_list[wave][k];
-                    Record r;
+                auto load = [&](uint32_t j, Record& r, float& lthr) {
                     r.m_nt = s_m[j];
                     r.a_px = s_a[j];
                     r.b_py = s_b[j];
                     r.col_op = s_c[j];
                     r.nrm_dep = AUX ? s_n[AUX ? j : 0] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                    const HitEval h = eval_hit(r, sx, sy, pxc, pyc, s_lthr[j]);
+                    lthr = s_lthr[j];
+                };
+                auto composite = [&](uint32_t j, const Record& r, float lthr) {
+                    const HitEval h = eval_hit(r, sx, sy, pxc, pyc, lthr);
                     if (!done && h.alpha > 0.0f) {
                         const float w = T * h.alpha;
                         cr += r.col_op.x * w;
@@ -162,6 +163,23 @@ __global__ __launch_bounds__((TW / 8) * (TW / 8) * 64) void raster_fwd_kernel(
                         last = base - start + j + 1u;
                         if (T < kTTerminate) done = true;
                     }
+                };
+                // Two records per round: both list entries and both records
+                // are read from LDS together (one wait instead of two
+                // dependent round trips each), then composited in list order
+                // with the same per-record early exit.
+                for (uint32_t k = 0; k < my_count; k += 2) {
+                    const bool two = k + 1 < my_count;
+                    const uint32_t j0 = s_list[wave][k];
+                    const uint32_t j1 = two ? (uint32_t)s_list[wave][k + 1] : j0;
+                    Record r0, r1;
+                    float l0, l1;
+                    load(j0, r0, l0);
+                    load(j1, r1, l1);
+                    composite(j0, r0, l0);
+                    if (!__any(!done)) break;
+                    if (!two) break;
+                    composite(j1, r1, l1);
                     if (!__any(!done)) break;
                 }
             }
