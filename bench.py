@@ -269,7 +269,7 @@ def main():
     elif not a.no_autotune:
         # Untimed: real training steps under each candidate schedule.
         t0 = time.perf_counter()
-        schedule = tr.autotune(ids_for(0), reps=2)
+        schedule = tr.autotune(ids_for(0), reps=4)
         log(f"autotune picked {schedule} in {time.perf_counter() - t0:.1f}s")
     run(tr, tr.GRAPH, a.warmup, 0)
     ctx.synchronize()

@@ -352,6 +352,13 @@ gs_status gs_trainer_set_tiling(gs_trainer t, uint32_t tile_size, uint32_t fwd_b
  * passes already fill every CU, so the split costs more than it hides on the
  * 1M-surfel 4 x 1080p step and gains nothing on the 16-frame 720p batch. */
 gs_status gs_trainer_set_overlap(gs_trainer t, int enable);
+/* Parameter gradients of batched steps (no counterpart in the reference,
+ * whose trainer keeps them in internal buffers): a batched or graph step runs
+ * the geometry backward and the Adam update as one kernel and, by default,
+ * never stores the summed parameter gradients. With enable != 0 it also
+ * writes them to the grad_* buffers of gs_trainer_buffers, as plain steps
+ * always do (callers that inspect or check gradients). Default 0. */
+gs_status gs_trainer_set_keep_grads(gs_trainer t, int enable);
 /* Blocking: per-view (l1_mean, dssim_mean) of the last step, 2 per view. */
 gs_status gs_trainer_last_losses(gs_trainer t, double* out, uint32_t n_ids);
 /* Blocking: render view `view` with the current parameters into rgba (W*H*4). */

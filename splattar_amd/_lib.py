@@ -141,6 +141,7 @@ _PROTOS = {
     "gs_trainer_set_schedule": ([_vp, _u32, _u32], C.c_int),
     "gs_trainer_set_tiling": ([_vp, _u32, _u32, _u32], C.c_int),
     "gs_trainer_set_overlap": ([_vp, C.c_int], C.c_int),
+    "gs_trainer_set_keep_grads": ([_vp, C.c_int], C.c_int),
     "gs_trainer_last_losses": ([_vp, C.POINTER(C.c_double), _u32], C.c_int),
     "gs_trainer_render": ([_vp, _u32, _vp], C.c_int),
     "gs_trainer_params": ([_vp, C.POINTER(RasterBuffers)], C.c_int),

@@ -461,6 +461,11 @@ class Trainer:
     def set_overlap(self, enable: bool):
         check(lib().gs_trainer_set_overlap(self.h, 1 if enable else 0))
 
+    def set_keep_grads(self, enable: bool):
+        """Batched steps also store the summed parameter gradients in
+        grad_* (plain steps always do); off by default."""
+        check(lib().gs_trainer_set_keep_grads(self.h, 1 if enable else 0))
+
     def autotune(self, view_ids, reps: int = 2) -> dict:
         """Time the candidate schedules on real batched steps and keep the
         fastest; returns {"digit_bits", "blocks_per_cu" (0 = one workgroup

@@ -33,21 +33,6 @@ __global__ void adam_tick_kernel(AdamState* st) {
     *st = s;
 }
 
-__device__ __forceinline__ float activate(int act, float x) {
-    if (act == 1) return __expf(x);
-    if (act == 2) return 1.0f / (1.0f + __expf(-x));
-    return x;
-}
-
-__device__ __forceinline__ float dact(int act, float x) {
-    if (act == 1) return __expf(x);
-    if (act == 2) {
-        const float s = 1.0f / (1.0f + __expf(-x));
-        return s * (1.0f - s);
-    }
-    return 1.0f;
-}
-
 __global__ __launch_bounds__(kAdamThreads) void adam_kernel(const AdamTable table,
                                                             const AdamState* __restrict__ state,
                                                             int step) {
@@ -58,7 +43,6 @@ __global__ __launch_bounds__(kAdamThreads) void adam_kernel(const AdamTable tabl
     const AdamClass c = table.c[ci];
     const uint32_t first = (blockIdx.x - c.block_begin) * (kAdamThreads * kAdamItems) + threadIdx.x * kAdamItems;
     if (first >= c.n) return;
-    const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
     const float bc1 = state->bc1_inv, bc2 = state->bc2_inv;
     const uintptr_t align = (uintptr_t)c.raw | (uintptr_t)c.grad | (uintptr_t)c.m | (uintptr_t)c.v |
                             (uintptr_t)c.activated;
@@ -88,15 +72,8 @@ __global__ __launch_bounds__(kAdamThreads) void adam_kernel(const AdamTable tabl
     float act[kAdamItems];
 #pragma unroll
     for (int k = 0; k < kAdamItems; ++k) {
-        if (step) {
-            const float g = (gr[k] + c.reg) * dact(c.activation, raw[k]);
-            m[k] = beta1 * m[k] + (1.0f - beta1) * g;
-            v[k] = beta2 * v[k] + (1.0f - beta2) * g * g;
-            const float mhat = m[k] * bc1;
-            const float vhat = v[k] * bc2;
-            raw[k] -= c.lr * mhat / (sqrtf(vhat) + eps);
-        }
-        act[k] = activate(c.activation, raw[k]);
+        if (step) adam_element(c.activation, c.lr, c.reg, bc1, bc2, raw[k], gr[k], m[k], v[k]);
+        act[k] = adam_activate(c.activation, raw[k]);
     }
     if (vec) {
         if (step) {

@@ -628,8 +628,8 @@ struct ParamGrads {
 size_t geom_cam_partials_bytes(uint32_t n, uint32_t n_views);
 hipError_t launch_geom_bwd(hipStream_t st, const CamParams* cams_dev, uint32_t n_views, uint32_t n,
                            const float4* pos, const float2* scales, const float4* quats,
-                           const float* sh, const Record* recs, const float* grad_coef,
-                           ParamGrads g, bool accumulate, float* cam_partials);
+                           const float* sh, const float* grad_coef, ParamGrads g, bool accumulate,
+                           float* cam_partials);
 
 // loss.hip
 // With `view` set, the target is read at target + view->target_base (the
@@ -664,11 +664,49 @@ struct AdamTable {
     AdamClass c[kMaxAdamClasses];
     uint32_t n_classes;
 };
+
+// Activation of a raw parameter (0 identity, 1 exp, 2 sigmoid) and its
+// derivative, as adam.wgsl's activate / the chain factor of adam_step.
+__device__ __forceinline__ float adam_activate(int act, float x) {
+    if (act == 1) return __expf(x);
+    if (act == 2) return 1.0f / (1.0f + __expf(-x));
+    return x;
+}
+__device__ __forceinline__ float adam_dact(int act, float x) {
+    if (act == 1) return __expf(x);
+    if (act == 2) {
+        const float s = 1.0f / (1.0f + __expf(-x));
+        return s * (1.0f - s);
+    }
+    return 1.0f;
+}
+// One Adam element update (adam.wgsl:44-74 with the host bias corrections of
+// optim.rs:179-197); shared by the class-table kernel and the fused geometry
+// backward so both give the same bits.
+__device__ __forceinline__ void adam_element(int act, float lr, float reg, float bc1, float bc2, float& raw,
+                                             float grad, float& m, float& v) {
+    const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+    const float g = (grad + reg) * adam_dact(act, raw);
+    m = beta1 * m + (1.0f - beta1) * g;
+    v = beta2 * v + (1.0f - beta2) * g * g;
+    const float mhat = m * bc1;
+    const float vhat = v * bc2;
+    raw -= lr * mhat / (sqrtf(vhat) + eps);
+}
+
 uint32_t adam_blocks_for(uint32_t n);
 hipError_t launch_adam_tick(hipStream_t st, AdamState* state);
 // The class table travels by value in the kernel arguments (a captured graph
 // keeps the learning rates it was captured with).
 hipError_t launch_adam(hipStream_t st, const AdamTable& table, uint32_t total_blocks,
                        const AdamState* state, bool step);
+// Geometry backward of a batch fused with the Adam step (after adam_tick) of
+// the trainer's five classes (pos float4, scales float2 exp, quat float4,
+// opacity sigmoid, SH 48): reads raw parameters, writes raw, m, v and the
+// activated copies; keep.pos..sh (all or none) also receive the summed
+// parameter gradients. hipErrorInvalidValue for any other class table.
+hipError_t launch_geom_adam(hipStream_t st, const CamParams* cams_dev, uint32_t n_views, uint32_t n,
+                            const float* grad_coef, const AdamTable& table, const AdamState* state,
+                            ParamGrads keep);
 
 }  // namespace gs

@@ -411,6 +411,14 @@ __global__ __launch_bounds__(TW == 8 ? 64 : TW * TW / 2) void raster_bwd_kernel(
                 const float v = s_grad[k];
                 if (v != 0.0f) {
                     const int e = k / NS, slot = k - e * NS;
+                    // A colour channel clamped at 0 in the forward passes no
+                    // gradient to SH (surfel_prep_bwd.wgsl's clamp mask; the
+                    // record stores such a channel as exactly 0).
+                    if (slot >= kGCol && slot < kGCol + 3) {
+                        const float4 c = s_c[e];
+                        const float cc = slot == kGCol ? c.x : (slot == kGCol + 1 ? c.y : c.z);
+                        if (cc == 0.0f) continue;
+                    }
                     atomicAdd(&grad_coef[(size_t)payloads[base + e] * kGradStride + slot], v);
                 }
             }

@@ -881,7 +881,7 @@ struct RasterCore {
     gs_status backward(hipStream_t st, uint32_t v0, uint32_t nv, uint32_t n, const float4* pos,
                        const float2* sc, const float4* q, const float* sh, const float4* dl_dcolor,
                        const float4* dl_dnormal, ParamGrads g, bool accumulate, bool aux,
-                       Profiler* prof = nullptr) {
+                       Profiler* prof = nullptr, bool geom = true) {
         Profiler dummy;
         Profiler& P = prof ? *prof : dummy;
         const CamParams* cd = cams_dev + v0;
@@ -896,8 +896,9 @@ struct RasterCore {
         if (qb.cursor != nullptr) GS_HIP(launch_fill_u32(st, qb.cursor, 0u, kTileQueues));
         GS_HIP(launch_raster_bwd(st, cd, cams.data() + v0, nv, tiles, ranges, pays[result_alt], recs, in,
                                  grad_coef, aux, cull, qb, shape));
+        if (!geom) return GS_OK;
         GS_CHECK(P.mark(st, GS_STAGE_GEOM_BWD));
-        GS_HIP(launch_geom_bwd(st, cd, nv, n, pos, sc, q, sh, recs, grad_coef, g, accumulate,
+        GS_HIP(launch_geom_bwd(st, cd, nv, n, pos, sc, q, sh, grad_coef, g, accumulate,
                                cam_partials));
         return GS_OK;
     }
@@ -1397,6 +1398,9 @@ struct gs_trainer_s {
     bool pipeline = false;
     hipStream_t side = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // Batched steps run the geometry backward and Adam as one kernel; the
+    // summed parameter gradients reach grad_* only when asked for.
+    bool keep_grads = false;
 
     ~gs_trainer_s() {
         for (auto& e : ev)
@@ -1425,6 +1429,17 @@ struct gs_trainer_s {
     gs_status enqueue_step(hipStream_t st, const uint32_t* ids, uint32_t k, bool batched,
                            bool profile = false);
     gs_status enqueue_pipelined(hipStream_t st, const uint32_t* ids, uint32_t k);
+    // Geometry backward of the k views of a batched step fused with the Adam
+    // step (bias-correction tick first).
+    gs_status geom_adam(hipStream_t st, uint32_t k) {
+        ParamGrads keep{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+        if (keep_grads) keep = ParamGrads{grad_pos, grad_scales, grad_quat, grad_opacity, grad_sh, nullptr};
+        uint32_t blocks = 0;
+        const AdamTable tab = optim->table(&blocks);
+        GS_HIP(launch_adam_tick(st, optim->state));
+        GS_HIP(launch_geom_adam(st, core.cams_dev, k, n, core.grad_coef, tab, optim->state, keep));
+        return GS_OK;
+    }
     // L1 + D-SSIM of view slot j (the coefficient scratch is shared: the
     // losses of one step run in stream order on one stream).
     gs_status loss_view(hipStream_t s, const uint32_t* ids, uint32_t j) {
@@ -1494,7 +1509,9 @@ gs_status gs_trainer_s::enqueue_step(hipStream_t st, const uint32_t* ids, uint32
         if (P) GS_CHECK(P->mark(st, GS_STAGE_LOSS));
         for (uint32_t j = 0; j < k; ++j) GS_CHECK(loss_view(st, ids, j));
         GS_CHECK(core.backward(st, 0, k, n, positions, scales, quats, sh, dl_dcolor, nullptr, g, false,
-                               false, P));
+                               false, P, false));
+        if (P) GS_CHECK(P->mark(st, GS_STAGE_GEOM_BWD));
+        GS_CHECK(geom_adam(st, k));
     } else {
         for (uint32_t j = 0; j < k; ++j) {
             const TrainViewDev& tv = views[ids[j]];
@@ -1507,9 +1524,9 @@ gs_status gs_trainer_s::enqueue_step(hipStream_t st, const uint32_t* ids, uint32
             GS_CHECK(core.backward(st, j, 1, n, positions, scales, quats, sh, dl_dcolor, nullptr, g,
                                    j > 0, false, P));
         }
+        if (P) GS_CHECK(P->mark(st, GS_STAGE_ADAM));
+        GS_CHECK(gs_optim_encode_step(optim, st));
     }
-    if (P) GS_CHECK(P->mark(st, GS_STAGE_ADAM));
-    GS_CHECK(gs_optim_encode_step(optim, st));
     if (P) {
         GS_CHECK(P->mark(st, -1));
         P->steps += 1;
@@ -1528,7 +1545,6 @@ gs_status gs_trainer_s::enqueue_pipelined(hipStream_t st, const uint32_t* ids, u
         GS_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
         for (auto& e : ev) GS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
-    ParamGrads g{grad_pos, grad_scales, grad_quat, grad_opacity, grad_sh, nullptr};
     GS_CHECK(core.forward(st, 0, k, n, positions, scales, quats, opacities, sh, nullptr, false));
     const uint32_t ka = k / 2;
     const uint32_t ta = core.cams[ka].tile_base;
@@ -1548,10 +1564,7 @@ gs_status gs_trainer_s::enqueue_pipelined(hipStream_t st, const uint32_t* ids, u
     GS_CHECK(core.raster_bwd_range(st, 0, 0, k, 0, ta, dl_dcolor));
     GS_HIP(hipStreamWaitEvent(st, ev[3], 0));
     GS_CHECK(core.raster_bwd_range(st, 1, 0, k, ta, tt - ta, dl_dcolor));
-    GS_HIP(launch_geom_bwd(st, core.cams_dev, k, n, positions, scales, quats, sh, core.recs, core.grad_coef, g,
-                           false, core.cam_partials));
-    GS_CHECK(gs_optim_encode_step(optim, st));
-    return GS_OK;
+    return geom_adam(st, k);
 }
 
 extern "C" {
@@ -1773,6 +1786,13 @@ gs_status gs_trainer_set_schedule(gs_trainer t, uint32_t digit_bits, uint32_t bl
 gs_status gs_trainer_set_overlap(gs_trainer t, int enable) {
     GS_REQUIRE(t != nullptr, "trainer is null");
     t->pipeline = enable != 0;
+    drop_graph(t);
+    return GS_OK;
+}
+
+gs_status gs_trainer_set_keep_grads(gs_trainer t, int enable) {
+    GS_REQUIRE(t != nullptr, "trainer is null");
+    t->keep_grads = enable != 0;
     drop_graph(t);
     return GS_OK;
 }

@@ -224,6 +224,7 @@ def test_config2_uneven_batch_full_size(gpu):
     for mode in (sa.Trainer.GRAPH, sa.Trainer.PLAIN):
         announce(f"config 2: mode {mode}")
         t = sa.Trainer(gpu, cfg, sc.as_init(), views)
+        t.set_keep_grads(True)
         t.step(list(range(8)), mode=mode)
         t.entry_count()  # raises on overflow
         res[mode] = (_trainer_grads(t, sc.n, 16), t.last_losses(8))

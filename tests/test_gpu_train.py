@@ -125,15 +125,17 @@ def _params(t):
                 quats=t.quats.read().copy(), opac=t.opacities.read().copy(), sh=t.sh.read().copy())
 
 
-def test_train_step_matches_oracle_composition(gpu):
-    """One plain step over 2 views == oracle render/loss/gradients/Adam."""
+@pytest.mark.parametrize("mode", [0, 2])
+def test_train_step_matches_oracle_composition(gpu, mode):
+    """One step over 2 views == oracle render/loss/gradients/Adam, plain
+    (geometry backward and Adam as separate kernels) and graph (fused)."""
     deg = 2
     sc, tgt, specs = _small_training_problem(n=1500, views=2, size=(64, 48), deg=deg)
     announce("train step vs oracle: targets")
     targets = _render_targets(gpu, tgt, specs, deg)
     t = _trainer(gpu, sc, specs, targets, deg, 2)
-    announce("train step vs oracle: plain step")
-    t.step([0, 1], mode=t.PLAIN)
+    announce(f"train step vs oracle: mode {mode}")
+    t.step([0, 1], mode=mode)
     got = _params(t)
     losses = t.last_losses(2)
     t.close()
@@ -205,6 +207,63 @@ def test_batched_and_graph_steps_equal_plain(gpu):
         for k in p0:
             # Same math; float atomics make the order of gradient sums differ.
             np.testing.assert_allclose(p[k], p0[k], rtol=1e-4, atol=2e-6, err_msg=f"mode {mode} {k}")
+
+
+def test_fused_geometry_adam_applies_adam_to_the_summed_gradients(gpu):
+    """Batched steps run the geometry backward and Adam as one kernel. With
+    the gradients kept, two graph steps equal the oracle's f32 Adam
+    (adam.wgsl:44-74 with the optim.rs bias corrections) applied to the kept
+    gradients on the identity classes (position, quaternion, SH); log-scales
+    and opacity logits move by the Adam step of their chained gradients
+    (dact = s, o(1 - o)); keeping the gradients changes nothing else."""
+    deg = 2
+    lr = 1e-3
+    sc, tgt, specs = _small_training_problem(deg=deg)
+    targets = _render_targets(gpu, tgt, specs, deg)
+    t = _trainer(gpu, sc, specs, targets, deg, 3)
+    t.set_keep_grads(True)
+    p0 = _params(t)
+    raw = dict(pos=t.positions.read().copy(), quats=t.quats.read().copy(), sh=t.sh.read().copy())
+    m = {k: np.zeros_like(v) for k, v in raw.items()}
+    v = {k: np.zeros_like(x) for k, x in raw.items()}
+    for it in range(2):
+        announce(f"fused geometry + adam: step {it}")
+        t.step([0, 1, 2], mode=t.GRAPH)
+        g = dict(pos=t.grad_pos.read().copy(), quats=t.grad_quat.read().copy(), sh=t.grad_sh.read().copy())
+        assert np.all(g["pos"][:, 3] == 0.0)
+        for k in raw:
+            r, gg = raw[k].reshape(-1), g[k].reshape(-1)
+            mm, vv = m[k].reshape(-1), v[k].reshape(-1)
+            oracle.adam_step_f32(r, gg, mm, vv, lr, 0, it + 1)
+        if it == 0:
+            gs, go = t.grad_scales.read().copy(), t.grad_opacity.read().copy()
+            p1 = _params(t)
+    got = dict(pos=t.positions.read(), quats=t.quats.read(), sh=t.sh.read())
+    for k in raw:
+        err = np.abs(got[k] - raw[k]).max()
+        print(f"  {k}: max |fused - host Adam| {err:.2e}")
+        np.testing.assert_allclose(got[k], raw[k], rtol=0, atol=4e-7, err_msg=k)
+    # First step (t = 1): raw moves by -lr * g / (|g| + eps) up to f32 rounding.
+    gr = gs.astype(np.float64) * p0["scales"]
+    want = -lr * gr / (np.abs(gr) + 1e-8)
+    np.testing.assert_allclose(np.log(p1["scales"] / p0["scales"].astype(np.float64)), want, rtol=0,
+                               atol=1e-6)
+    o0, o1 = p0["opac"].astype(np.float64), p1["opac"].astype(np.float64)
+    mid = (o0 > 0.01) & (o0 < 0.99)
+    gr = go.astype(np.float64) * o0 * (1 - o0)
+    want = -lr * gr / (np.abs(gr) + 1e-8)
+    logit = lambda o: np.log(o / (1 - o))
+    np.testing.assert_allclose((logit(o1) - logit(o0))[mid], want[mid], rtol=0, atol=1e-5)
+    final = _params(t)
+    t.close()
+    # The same two steps without kept gradients (float atomics aside).
+    t2 = _trainer(gpu, sc, specs, targets, deg, 3)
+    for it in range(2):
+        t2.step([0, 1, 2], mode=t2.GRAPH)
+    q = _params(t2)
+    for k in q:
+        np.testing.assert_allclose(q[k], final[k], rtol=1e-4, atol=2e-6, err_msg=k)
+    t2.close()
 
 
 SCHEDULES = [(8, 8), (8, 4), (8, 0), (4, 8), (4, 4), (4, 0)]
@@ -400,6 +459,7 @@ def test_growing_scene_frame_batch_matches_fp64(gpu, tiling):
         t4[..., :3] = t
         views.append((sa.Camera(sp.center, sp.quat, sp.focal, sc.sh_degree), t4, m))
     tr = sa.Trainer(gpu, cfg, sc.as_init(), views)
+    tr.set_keep_grads(True)
     tr.set_tiling(*tiling)
     announce(f"growing-scene 16-frame batch, tiling {tiling}, counts {counts[0]}..{counts[-1]}")
     tr.step(list(range(len(specs))), mode=tr.GRAPH)
@@ -437,6 +497,7 @@ def test_streaming_frame_batch_full_size(gpu):
     results = {}
     for mode in (sa.Trainer.GRAPH, sa.Trainer.PLAIN):
         t = sa.Trainer(gpu, cfg, sc.as_init(), views)
+        t.set_keep_grads(True)
         if mode == sa.Trainer.GRAPH:
             announce("streaming batch full size: autotune")
             chosen = t.autotune(ids, reps=1)
@@ -479,6 +540,7 @@ def test_uneven_multiview_batch_matches_fp64(gpu):
         t4[..., :3] = t
         views.append((sa.Camera(sp.center, sp.quat, sp.focal, sc.sh_degree), t4))
     tr = sa.Trainer(gpu, cfg, sc.as_init(), views)
+    tr.set_keep_grads(True)
     announce(f"uneven 8-view batch: sizes {[(sp.width, sp.height) for sp in specs]}")
     tr.step(list(range(mv.VIEWS)), mode=tr.GRAPH)
     _check_step_grads(tr, sc, ref, mask, sens)
