@@ -232,14 +232,37 @@ __global__ __launch_bounds__(256) void tile_order_keys_kernel(const uint32_t* __
     vals[t] = (uint32_t)g;
 }
 
+// Four sorted keys per thread (one 16-B load, keys 16-B aligned) and their
+// two neighbours: a key that differs from its predecessor opens its tile's
+// range, one that differs from its successor closes it.
 __global__ __launch_bounds__(256) void tile_ranges_u32_kernel(const uint32_t* __restrict__ keys,
                                                               const uint32_t* __restrict__ count,
                                                               uint32_t* __restrict__ ranges) {
     const uint32_t n = *count;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t tile = keys[i];
-        if (i == 0 || keys[i - 1] != tile) ranges[2 * (size_t)tile] = i;
-        if (i == n - 1 || keys[i + 1] != tile) ranges[2 * (size_t)tile + 1] = i + 1;
+    const uint32_t nq = (n + 3u) / 4u;
+    for (uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x; qi < nq; qi += gridDim.x * blockDim.x) {
+        const uint32_t i0 = qi * 4u;
+        uint32_t k[6];
+        if (i0 + 4u <= n) {
+            const uint4 v = reinterpret_cast<const uint4*>(keys)[qi];
+            k[1] = v.x;
+            k[2] = v.y;
+            k[3] = v.z;
+            k[4] = v.w;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) k[1 + j] = i0 + j < n ? keys[i0 + j] : 0u;
+        }
+        k[0] = i0 > 0u ? keys[i0 - 1u] : 0u;
+        k[5] = i0 + 4u < n ? keys[i0 + 4u] : 0u;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t i = i0 + (uint32_t)j;
+            if (i >= n) break;
+            const uint32_t tile = k[1 + j];
+            if (i == 0u || k[j] != tile) ranges[2 * (size_t)tile] = i;
+            if (i == n - 1u || k[2 + j] != tile) ranges[2 * (size_t)tile + 1] = i + 1u;
+        }
     }
 }
 
@@ -336,8 +359,9 @@ hipError_t launch_tile_order_keys(hipStream_t st, const uint32_t* ranges, uint32
 
 hipError_t launch_tile_ranges_u32(hipStream_t st, const uint32_t* keys, const uint32_t* count,
                                   uint32_t capacity, uint32_t* ranges) {
-    uint32_t blocks = div_up(capacity > 0 ? capacity : 1, 256);
-    if (blocks > 8192) blocks = 8192;
+    if (reinterpret_cast<uintptr_t>(keys) & 15u) return hipErrorInvalidValue;
+    uint32_t blocks = div_up(capacity > 0 ? capacity : 1, 4 * 256);
+    if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(tile_ranges_u32_kernel, dim3(blocks), dim3(256), 0, st, keys, count, ranges);
     return hipGetLastError();
 }

@@ -39,8 +39,7 @@ __device__ __forceinline__ float tap(int d) { return c_w[d < 0 ? -d : d]; }
 // float4 (C_mu, C_x2, C_xy, 0) at coef[ch * P + p]. Partial sums: per
 // channel ch and tile t the pair (sum |x - y|, sum SSIM) at partials[2 (ch T + t)].
 
-__global__ __launch_bounds__(256) void ssim_fwd_kernel(uint32_t W, uint32_t H, float lambda,
-                                                       float dssim_scale,
+__global__ __launch_bounds__(256) void ssim_fwd_kernel(uint32_t W, uint32_t H, float dw,
                                                        const float4* __restrict__ img,
                                                        const float4* __restrict__ tgt,
                                                        float4* __restrict__ coef,
@@ -109,20 +108,21 @@ __global__ __launch_bounds__(256) void ssim_fwd_kernel(uint32_t W, uint32_t H, f
     }
     const int gx = bx * kT + tx, gy = by * kT + ty;
     const bool inside = gx < (int)W && gy < (int)H;
-    const float inv_n3 = 1.0f / ((float)W * (float)H * 3.0f);
-    const float dw = -lambda * dssim_scale * inv_n3;
     const size_t P = (size_t)W * H;
     double l1 = 0.0, ss = 0.0;
     // ssim.wgsl:95-126 (partials).
     const float sx2 = bx2 - mu_x * mu_x, sy2 = by2 - mu_y * mu_y, sxy_ = bxy - mu_x * mu_y;
     const float p = mu_x * mu_x + mu_y * mu_y + kC1;
     const float q = sx2 + sy2 + kC2;
-    const float l = (2.0f * mu_x * mu_y + kC1) / p;
-    const float cs = (2.0f * sxy_ + kC2) / q;
+    // p, q >= C1, C2 > 0: two refined reciprocals (<= 1 ulp) instead of five
+    // divisions.
+    const float rp = rcp_refined(p), rq = rcp_refined(q);
+    const float l = (2.0f * mu_x * mu_y + kC1) * rp;
+    const float cs = (2.0f * sxy_ + kC2) * rq;
     const float s = l * cs;
-    const float ds_dmux = cs * (2.0f * mu_y - 2.0f * mu_x * l) / p + l * (-2.0f * mu_y + 2.0f * mu_x * cs) / q;
+    const float ds_dmux = cs * (2.0f * mu_y - 2.0f * mu_x * l) * rp + l * (-2.0f * mu_y + 2.0f * mu_x * cs) * rq;
     if (inside) {
-        coef[ch * P + (size_t)gy * W + gx] = make_float4(ds_dmux * dw, (-l * cs / q) * dw, (2.0f * l / q) * dw, 0.0f);
+        coef[ch * P + (size_t)gy * W + gx] = make_float4(ds_dmux * dw, (-l * cs * rq) * dw, (2.0f * l * rq) * dw, 0.0f);
         ss = (double)s;
         const float2 c = sxy[ty + kH][tx + kH];
         l1 = (double)fabsf(c.x - c.y);
@@ -149,7 +149,7 @@ __global__ __launch_bounds__(256) void ssim_fwd_kernel(uint32_t W, uint32_t H, f
 // per pixel), blur them (column pairs, then rows) and write that channel of
 // dL/dx = (1-l) sign(x-y)/3P + B(C_mu) + 2x B(C_x2) + y B(C_xy); the
 // channel-0 workgroup also writes w = 0.
-__global__ __launch_bounds__(256) void ssim_bwd_kernel(uint32_t W, uint32_t H, float lambda,
+__global__ __launch_bounds__(256) void ssim_bwd_kernel(uint32_t W, uint32_t H, float l1w,
                                                        const float4* __restrict__ img,
                                                        const float4* __restrict__ tgt,
                                                        const float4* __restrict__ coef,
@@ -211,7 +211,6 @@ __global__ __launch_bounds__(256) void ssim_bwd_kernel(uint32_t W, uint32_t H, f
     const size_t pix = (size_t)gy * W + gx;
     const float x = reinterpret_cast<const float*>(img)[4 * pix + ch];
     const float y = reinterpret_cast<const float*>(tgt)[4 * pix + ch];
-    const float l1w = (1.0f - lambda) / ((float)W * (float)H * 3.0f);
     const float sg = x - y > 0.f ? 1.f : (x - y < 0.f ? -1.f : 0.f);
     float* o = reinterpret_cast<float*>(dl_out) + 4 * pix;
     o[ch] = sg * l1w + b0 + 2.0f * x * b1 + y * b2;
@@ -229,12 +228,16 @@ hipError_t launch_l1_ssim(hipStream_t st, uint32_t w, uint32_t h, float lambda, 
     if (w == 0 || h == 0) return hipSuccess;
     dim3 grid(div_up(w, kT), div_up(h, kT), 3);
     float4* coef = reinterpret_cast<float4*>(coef_scratch);
-    hipLaunchKernelGGL(ssim_fwd_kernel, grid, dim3(256), 0, st, w, h, lambda, dssim_scale, img, target,
-                       coef, partial_sums, view);
+    // Uniform loss weights, computed once: dw = -l s / 3P (D-SSIM term), the
+    // L1 term's (1 - l) / 3P.
+    const float n3 = (float)w * (float)h * 3.0f;
+    const float dw = -lambda * dssim_scale * (1.0f / n3);
+    const float l1w = (1.0f - lambda) / n3;
+    hipLaunchKernelGGL(ssim_fwd_kernel, grid, dim3(256), 0, st, w, h, dw, img, target, coef, partial_sums,
+                       view);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || dl_out == nullptr) return e;
-    hipLaunchKernelGGL(ssim_bwd_kernel, grid, dim3(256), 0, st, w, h, lambda, img, target, coef, dl_out,
-                       view);
+    hipLaunchKernelGGL(ssim_bwd_kernel, grid, dim3(256), 0, st, w, h, l1w, img, target, coef, dl_out, view);
     return hipGetLastError();
 }
 
