@@ -154,7 +154,7 @@ __global__ __launch_bounds__(256) void emit_tile_keys_kernel(
     uint64_t mask = 0;
     bool masked = false;
     if (gid < n_items) {
-        item = packed ? order[gid] >> 8 : order[gid];
+        item = packed ? order[gid] >> kPackBits : order[gid];
         const uint4 ei = emit_info[item];
         const uint32_t rx0 = ei.x & 0xFFFFu, ry0 = ei.x >> 16, rx1 = ei.y & 0xFFFFu, ry1 = ei.y >> 16;
         if (rx1 >= rx0 && ry1 >= ry0) {

@@ -27,6 +27,12 @@ constexpr int kTile = 16;        // coverage granularity: part of the forward mo
 // lets reach each pixel.
 constexpr bool valid_work_tile(uint32_t t) { return t == 8 || t == 16 || t == 32; }
 constexpr int kShFloats = 48;    // SH storage padded to degree 3 (rasterizer.rs:13-15)
+// Depth-presort payload with packed tile counts: item << kPackBits | count.
+// A record's 16-px rect has radius <= 64 px, so it spans at most 9 x 9 work
+// tiles of 16 px (5 x 5 of 32 px): counts stay <= 81 < 2^kPackBits, and
+// packing needs work tiles >= 16 px and fewer than 2^(32 - kPackBits) items.
+constexpr uint32_t kPackBits = 7;
+constexpr uint32_t kPackMask = (1u << kPackBits) - 1u;
 constexpr int kWave = 64;
 
 // Per-surfel, per-view raster record written by the project kernel: the
@@ -509,7 +515,7 @@ hipError_t launch_project(hipStream_t st, const CamParams* cams_dev, uint32_t n_
 // scan.hip
 size_t scan_scratch_bytes(uint32_t capacity);
 // Exclusive scan of in[0..n) (in place allowed), of in[perm[i]] (gather),
-// or, with in == nullptr, of perm[i] & 0xFF (counts packed in the low byte).
+// or, with in == nullptr, of perm[i] & kPackMask (packed tile counts).
 hipError_t launch_exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t n,
                                  uint32_t* total_out, void* scratch, const uint32_t* perm = nullptr);
 
@@ -527,7 +533,7 @@ hipError_t launch_fill_u32(hipStream_t st, uint32_t* dst, uint32_t value, size_t
 hipError_t launch_copy_u32(hipStream_t st, uint32_t* dst, const uint32_t* src, size_t count);
 hipError_t launch_count_tiles(hipStream_t st, uint32_t n, const uint4* rects, uint32_t* counts);
 // Depth-ordered emission: order[k] is the k-th item in depth order (shifted
-// left by 8 over its tile count when packed), emit_info the per-item record
+// left by kPackBits over its tile count when packed), emit_info the per-item record
 // written by launch_project.
 hipError_t launch_emit_tile_keys(hipStream_t st, uint32_t n_items, const uint32_t* order, bool packed,
                                  const uint4* emit_info, const uint32_t* offsets_sorted,

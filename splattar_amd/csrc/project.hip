@@ -49,9 +49,9 @@ __global__ __launch_bounds__(256) void project_kernel(
     const CamParams cam = cams[view];
     const uint32_t g = view * n + i;
     // Per-item binning outputs, written on every path. The presort payload
-    // carries the tile count in its low 8 bits when pack_counts (for work
-    // tiles of 16 or 32 px a rect spans at most 9 x 9 tiles), so the scan
-    // reads counts in depth order without a gather.
+    // carries the tile count in its low kPackBits bits when pack_counts (for
+    // work tiles of 16 or 32 px a rect spans at most 9 x 9 tiles), so the
+    // scan reads counts in depth order without a gather.
     // emit_info = (x0 | y0 << 16, x1 | y1 << 16, mask lo, mask hi): the rect
     // in 16-bit tile coordinates and the mask of its tiles the footprint
     // reaches (all ones = every tile).
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(256) void project_kernel(
         if (depth_keys) depth_keys[g] = key;
         if (counts) counts[g] = count;
         if (presort_keys) presort_keys[g] = key;
-        if (presort_vals) presort_vals[g] = pack_counts ? (g << 8) | count : g;
+        if (presort_vals) presort_vals[g] = pack_counts ? (g << kPackBits) | count : g;
         if (emit_info)
             emit_info[g] = make_uint4(rect.x | (rect.y << 16), rect.z | (rect.w << 16), (uint32_t)mask,
                                       (uint32_t)(mask >> 32));

@@ -768,7 +768,7 @@ struct RasterCore {
             // Up to 2^24 items the presort payload also carries the tile
             // count (low 8 bits), so the scan needs no gather.
             // 8-px work tiles can give a rect more than 255 tiles: no packing.
-            const bool packed = items < (1u << 24) && shape.tile >= 16;
+            const bool packed = items < (1u << (32 - kPackBits)) && shape.tile >= 16;
             GS_CHECK(P.mark(st, GS_STAGE_PROJECT));
             // Lean (trainer): with packed counts nothing downstream reads the
             // rects, depth keys or counts, and colour-only passes never read
