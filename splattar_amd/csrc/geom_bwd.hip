@@ -76,13 +76,17 @@ __device__ __forceinline__ void zero_acc(SurfelAcc& a) {
     for (int k = 0; k < kShFloats; ++k) a.sh[k] = 0.f;
 }
 
-// Loads one view's coefficient gradients of surfel i; false when all are 0
-// (the surfel did not reach a pixel of the view).
+// Loads one view's coefficient gradients of surfel i (GS slots stored per
+// (view, surfel); the rest are 0); false when all are 0 (the surfel did not
+// reach a pixel of the view).
+template <int GS>
 __device__ __forceinline__ bool load_coef_grads(const float* __restrict__ grad_coef, uint32_t v, uint32_t n,
                                                 uint32_t i, float gc[kGradStride]) {
-    const float4* src = reinterpret_cast<const float4*>(grad_coef + ((size_t)v * n + i) * kGradStride);
+    const float4* src = reinterpret_cast<const float4*>(grad_coef + ((size_t)v * n + i) * GS);
 #pragma unroll
-    for (int k = 0; k < kGradStride / 4; ++k) {
+    for (int k = GS; k < kGradStride; ++k) gc[k] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < GS / 4; ++k) {
         const float4 t = src[k];
         gc[4 * k + 0] = t.x;
         gc[4 * k + 1] = t.y;
@@ -91,7 +95,7 @@ __device__ __forceinline__ bool load_coef_grads(const float* __restrict__ grad_c
     }
     bool any = false;
 #pragma unroll
-    for (int k = 0; k < kGradStride; ++k) any |= gc[k] != 0.0f;
+    for (int k = 0; k < GS; ++k) any |= gc[k] != 0.0f;
     return any;
 }
 
@@ -166,20 +170,20 @@ __device__ __forceinline__ void add_view(const CamParams& cam, const float gc[kG
         const float len = sqrtf(f.xx * f.xx + f.xy * f.xy + f.xz * f.xz);
         const float il = 1.0f / len;
         const V3f dir = v3(f.xx * il, f.xy * il, f.xz * il);
-        float basis[16], bgrad[48];
+        float basis[16], sdot[16];
         const int nb = sh_basis((int)cam.sh_degree, dir.x, dir.y, dir.z, basis);
-        sh_basis_grad((int)cam.sh_degree, dir.x, dir.y, dir.z, bgrad);
-        V3f ddir = v3(0.f, 0.f, 0.f);
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
+            sdot[k] = 0.f;
             if (k < nb) {
                 a.sh[3 * k + 0] += dcol.x * basis[k];
                 a.sh[3 * k + 1] += dcol.y * basis[k];
                 a.sh[3 * k + 2] += dcol.z * basis[k];
-                const float sdot = shp[3 * k] * dcol.x + shp[3 * k + 1] * dcol.y + shp[3 * k + 2] * dcol.z;
-                ddir = ddir + v3(bgrad[3 * k], bgrad[3 * k + 1], bgrad[3 * k + 2]) * sdot;
+                sdot[k] = shp[3 * k] * dcol.x + shp[3 * k + 1] * dcol.y + shp[3 * k + 2] * dcol.z;
             }
         }
+        const float3 dd = sh_dir_grad((int)cam.sh_degree, dir.x, dir.y, dir.z, sdot);
+        const V3f ddir = v3(dd.x, dd.y, dd.z);
         const V3f dv = (ddir - dir * dt(dir, ddir)) * il;
         dpos = dpos + dv;
         dcenter = dcenter - dv;
@@ -209,6 +213,7 @@ __device__ __forceinline__ void add_view(const CamParams& cam, const float gc[kG
     }
 }
 
+template <int GS>
 __global__ __launch_bounds__(kGeomThreads) void geom_bwd_kernel(
     const CamParams* __restrict__ cams, uint32_t n_views, uint32_t n, const float4* __restrict__ pos,
     const float2* __restrict__ scales, const float4* __restrict__ quats, const float* __restrict__ sh,
@@ -235,7 +240,7 @@ __global__ __launch_bounds__(kGeomThreads) void geom_bwd_kernel(
 #pragma unroll
         for (int k = 0; k < 16; ++k) cg[k] = 0.f;
         float gc[kGradStride];
-        const bool any = valid && load_coef_grads(grad_coef, v, n, i, gc);
+        const bool any = valid && load_coef_grads<GS>(grad_coef, v, n, i, gc);
         if (any) add_view(cams[v], gc, p, s, q, sh + (size_t)i * kShFloats, a, want_cam, cg);
         if (want_cam) {
             // Block reduction of the 16 camera slots (every thread takes part).
@@ -338,7 +343,7 @@ __global__ __launch_bounds__(kFuseThreads, 3) void geom_adam_kernel(
     zero_acc(a);
     for (uint32_t v = 0; v < n_views; ++v) {
         float gc[kGradStride];
-        const bool any = valid && load_coef_grads(grad_coef, v, n, i, gc);
+        const bool any = valid && load_coef_grads<grad_stride(false)>(grad_coef, v, n, i, gc);
         if (any) add_view(cams[v], gc, p, s, q, ch.raw + (size_t)i * kShFloats, a, false, nullptr);
     }
     const float bc1 = state->bc1_inv, bc2 = state->bc2_inv;
@@ -452,13 +457,17 @@ size_t geom_cam_partials_bytes(uint32_t n, uint32_t n_views) {
 
 hipError_t launch_geom_bwd(hipStream_t st, const CamParams* cams_dev, uint32_t n_views, uint32_t n,
                            const float4* pos, const float2* scales, const float4* quats,
-                           const float* sh, const float* grad_coef,
+                           const float* sh, const float* grad_coef, bool aux,
                            ParamGrads g, bool accumulate, float* cam_partials) {
     if (n == 0) return hipSuccess;
     const uint32_t blocks = div_up(n, kGeomThreads);
     float* parts = g.cam ? cam_partials : nullptr;
-    hipLaunchKernelGGL(geom_bwd_kernel, dim3(blocks), dim3(kGeomThreads), 0, st, cams_dev, n_views, n,
-                       pos, scales, quats, sh, grad_coef, g, accumulate ? 1 : 0, parts);
+    if (aux)
+        hipLaunchKernelGGL(geom_bwd_kernel<grad_stride(true)>, dim3(blocks), dim3(kGeomThreads), 0, st, cams_dev,
+                           n_views, n, pos, scales, quats, sh, grad_coef, g, accumulate ? 1 : 0, parts);
+    else
+        hipLaunchKernelGGL(geom_bwd_kernel<grad_stride(false)>, dim3(blocks), dim3(kGeomThreads), 0, st, cams_dev,
+                           n_views, n, pos, scales, quats, sh, grad_coef, g, accumulate ? 1 : 0, parts);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !parts) return e;
     hipLaunchKernelGGL(cam_reduce_kernel, dim3(n_views), dim3(256), 0, st, parts, blocks, g.cam,

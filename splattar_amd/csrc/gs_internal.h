@@ -72,6 +72,9 @@ enum GradSlot : int {
     kGNrm = 17,     // dL/dnormal (3) (normal loss only)
     kGradStride = 20,
 };
+// Slots per (view, surfel) in memory: colour-only backward passes (no depth,
+// distortion or normal terms) keep the first 16.
+constexpr int grad_stride(bool aux) { return aux ? (int)kGradStride : 16; }
 
 // Camera as the kernels see it (crates/gs-kernels/src/shaders/
 // raster_common.wgsl:5-20 RasterCamera), one per view.
@@ -231,31 +234,37 @@ __device__ __forceinline__ int sh_basis(int degree, float x, float y, float z, f
     return 16;
 }
 
-// d b_k / d dir (math.rs:159-191), 3 floats per basis function.
-__device__ __forceinline__ void sh_basis_grad(int degree, float x, float y, float z, float* g) {
-    g[0] = g[1] = g[2] = 0.0f;
-    if (degree < 1) return;
-    g[3] = 0.0f; g[4] = -kShC1; g[5] = 0.0f;
-    g[6] = 0.0f; g[7] = 0.0f; g[8] = kShC1;
-    g[9] = -kShC1; g[10] = 0.0f; g[11] = 0.0f;
-    if (degree < 2) return;
+// sum_k w_k d b_k / d dir (math.rs:159-191) for the basis functions of the
+// degree, accumulated term by term in k order (no 48-float table of
+// gradients held at once).
+__device__ __forceinline__ float3 sh_dir_grad(int degree, float x, float y, float z, const float* w) {
+    float3 d = make_float3(0.0f, 0.0f, 0.0f);
+    auto add = [&](float gx, float gy, float gz, float s) {
+        d.x += gx * s;
+        d.y += gy * s;
+        d.z += gz * s;
+    };
+    if (degree < 1) return d;
+    add(0.0f, -kShC1, 0.0f, w[1]);
+    add(0.0f, 0.0f, kShC1, w[2]);
+    add(-kShC1, 0.0f, 0.0f, w[3]);
+    if (degree < 2) return d;
     const float xx = x * x, yy = y * y, zz = z * z;
-    g[12] = kShC2_0 * y; g[13] = kShC2_0 * x; g[14] = 0.0f;
-    g[15] = 0.0f; g[16] = kShC2_1 * z; g[17] = kShC2_1 * y;
-    g[18] = kShC2_2 * (-2.0f * x); g[19] = kShC2_2 * (-2.0f * y); g[20] = kShC2_2 * (4.0f * z);
-    g[21] = kShC2_3 * z; g[22] = 0.0f; g[23] = kShC2_3 * x;
-    g[24] = kShC2_4 * (2.0f * x); g[25] = kShC2_4 * (-2.0f * y); g[26] = 0.0f;
-    if (degree < 3) return;
-    g[27] = kShC3_0 * (6.0f * x * y); g[28] = kShC3_0 * (3.0f * xx - 3.0f * yy); g[29] = 0.0f;
-    g[30] = kShC3_1 * (y * z); g[31] = kShC3_1 * (x * z); g[32] = kShC3_1 * (x * y);
-    g[33] = kShC3_2 * (-2.0f * x * y); g[34] = kShC3_2 * (4.0f * zz - xx - 3.0f * yy);
-    g[35] = kShC3_2 * (8.0f * y * z);
-    g[36] = kShC3_3 * (-6.0f * x * z); g[37] = kShC3_3 * (-6.0f * y * z);
-    g[38] = kShC3_3 * (6.0f * zz - 3.0f * xx - 3.0f * yy);
-    g[39] = kShC3_4 * (4.0f * zz - 3.0f * xx - yy); g[40] = kShC3_4 * (-2.0f * x * y);
-    g[41] = kShC3_4 * (8.0f * x * z);
-    g[42] = kShC3_5 * (2.0f * x * z); g[43] = kShC3_5 * (-2.0f * y * z); g[44] = kShC3_5 * (xx - yy);
-    g[45] = kShC3_6 * (3.0f * xx - 3.0f * yy); g[46] = kShC3_6 * (-6.0f * x * y); g[47] = 0.0f;
+    add(kShC2_0 * y, kShC2_0 * x, 0.0f, w[4]);
+    add(0.0f, kShC2_1 * z, kShC2_1 * y, w[5]);
+    add(kShC2_2 * (-2.0f * x), kShC2_2 * (-2.0f * y), kShC2_2 * (4.0f * z), w[6]);
+    add(kShC2_3 * z, 0.0f, kShC2_3 * x, w[7]);
+    add(kShC2_4 * (2.0f * x), kShC2_4 * (-2.0f * y), 0.0f, w[8]);
+    if (degree < 3) return d;
+    add(kShC3_0 * (6.0f * x * y), kShC3_0 * (3.0f * xx - 3.0f * yy), 0.0f, w[9]);
+    add(kShC3_1 * (y * z), kShC3_1 * (x * z), kShC3_1 * (x * y), w[10]);
+    add(kShC3_2 * (-2.0f * x * y), kShC3_2 * (4.0f * zz - xx - 3.0f * yy), kShC3_2 * (8.0f * y * z), w[11]);
+    add(kShC3_3 * (-6.0f * x * z), kShC3_3 * (-6.0f * y * z), kShC3_3 * (6.0f * zz - 3.0f * xx - 3.0f * yy),
+        w[12]);
+    add(kShC3_4 * (4.0f * zz - 3.0f * xx - yy), kShC3_4 * (-2.0f * x * y), kShC3_4 * (8.0f * x * z), w[13]);
+    add(kShC3_5 * (2.0f * x * z), kShC3_5 * (-2.0f * y * z), kShC3_5 * (xx - yy), w[14]);
+    add(kShC3_6 * (3.0f * xx - 3.0f * yy), kShC3_6 * (-6.0f * x * y), 0.0f, w[15]);
+    return d;
 }
 
 __device__ __forceinline__ float3 cross3(float3 a, float3 b) {
@@ -634,7 +643,7 @@ struct ParamGrads {
 size_t geom_cam_partials_bytes(uint32_t n, uint32_t n_views);
 hipError_t launch_geom_bwd(hipStream_t st, const CamParams* cams_dev, uint32_t n_views, uint32_t n,
                            const float4* pos, const float2* scales, const float4* quats,
-                           const float* sh, const float* grad_coef, ParamGrads g, bool accumulate,
+                           const float* sh, const float* grad_coef, bool aux, ParamGrads g, bool accumulate,
                            float* cam_partials);
 
 // loss.hip
@@ -710,7 +719,9 @@ hipError_t launch_adam(hipStream_t st, const AdamTable& table, uint32_t total_bl
 // the trainer's five classes (pos float4, scales float2 exp, quat float4,
 // opacity sigmoid, SH 48): reads raw parameters, writes raw, m, v and the
 // activated copies; keep.pos..sh (all or none) also receive the summed
-// parameter gradients. hipErrorInvalidValue for any other class table.
+// parameter gradients. grad_coef holds grad_stride(false) slots per (view,
+// surfel) (colour-only backward). hipErrorInvalidValue for any other class
+// table.
 hipError_t launch_geom_adam(hipStream_t st, const CamParams* cams_dev, uint32_t n_views, uint32_t n,
                             const float* grad_coef, const AdamTable& table, const AdamState* state,
                             ParamGrads keep);

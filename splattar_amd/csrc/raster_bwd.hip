@@ -234,7 +234,7 @@ __global__ __launch_bounds__(TW == 8 ? 64 : TW * TW / 2) void raster_bwd_kernel(
     const CamParams* __restrict__ cams, uint32_t n_views, const uint32_t* __restrict__ ranges,
     const uint32_t* __restrict__ payloads, const Record* __restrict__ recs, BwdIn in,
     float* __restrict__ grad_coef, bool cull, uint32_t total_tiles, TileQueue queue, uint32_t tile_begin) {
-    constexpr int NS = AUX ? 20 : 16;        // LDS slots per record
+    constexpr int NS = grad_stride(AUX);     // slots per record (LDS and grad_coef)
     constexpr bool TWO = TW >= 16;           // two pixels per lane (an 8x16 column per wave)
     constexpr int NC = TW / 8;               // 8x8 blocks per tile row
     constexpr int NW = TWO ? TW * TW / 128 : 1;  // waves
@@ -419,7 +419,7 @@ __global__ __launch_bounds__(TW == 8 ? 64 : TW * TW / 2) void raster_bwd_kernel(
                         const float cc = slot == kGCol ? c.x : (slot == kGCol + 1 ? c.y : c.z);
                         if (cc == 0.0f) continue;
                     }
-                    atomicAdd(&grad_coef[(size_t)payloads[base + e] * kGradStride + slot], v);
+                    atomicAdd(&grad_coef[(size_t)payloads[base + e] * NS + slot], v);
                 }
             }
             __syncthreads();

@@ -889,7 +889,7 @@ struct RasterCore {
         for (uint32_t v = v0; v < v0 + nv; ++v) tiles += cams[v].tiles_x * cams[v].tiles_y;
         GS_CHECK(P.mark(st, GS_STAGE_GRAD_CLEAR));
         GS_HIP(launch_fill_u32(st, reinterpret_cast<uint32_t*>(grad_coef), 0u,
-                               (size_t)n * nv * kGradStride));
+                               (size_t)n * nv * grad_stride(aux)));
         BwdIn in{dl_dcolor, dl_dnormal, out_color, out_aux, ncontrib};
         GS_CHECK(P.mark(st, GS_STAGE_RASTER_BWD));
         const TileQueue qb = (q_tiles[0] == tiles && q_begin[0] == 0) ? queue(0, 1) : TileQueue{nullptr, nullptr, 0};
@@ -898,7 +898,7 @@ struct RasterCore {
                                  grad_coef, aux, cull, qb, shape));
         if (!geom) return GS_OK;
         GS_CHECK(P.mark(st, GS_STAGE_GEOM_BWD));
-        GS_HIP(launch_geom_bwd(st, cd, nv, n, pos, sc, q, sh, grad_coef, g, accumulate,
+        GS_HIP(launch_geom_bwd(st, cd, nv, n, pos, sc, q, sh, grad_coef, aux, g, accumulate,
                                cam_partials));
         return GS_OK;
     }
@@ -1549,7 +1549,8 @@ gs_status gs_trainer_s::enqueue_pipelined(hipStream_t st, const uint32_t* ids, u
     const uint32_t ka = k / 2;
     const uint32_t ta = core.cams[ka].tile_base;
     const uint32_t tt = core.total_tiles;
-    GS_HIP(launch_fill_u32(st, reinterpret_cast<uint32_t*>(core.grad_coef), 0u, (size_t)n * k * kGradStride));
+    GS_HIP(launch_fill_u32(st, reinterpret_cast<uint32_t*>(core.grad_coef), 0u,
+                           (size_t)n * k * grad_stride(false)));
     GS_CHECK(core.raster_fwd_range(st, 0, 0, k, 0, ta));
     GS_HIP(hipEventRecord(ev[0], st));
     GS_CHECK(core.raster_fwd_range(st, 1, 0, k, ta, tt - ta));
