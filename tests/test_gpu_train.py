@@ -153,6 +153,9 @@ def test_train_step_matches_oracle_composition(gpu, mode):
         img = oracle.render(s64, cam)["color"]
         loss, dl, l1, ss = oracle.l1_ssim(img, targets[k][..., :3].astype(np.float64), 0.2, 0.5)
         assert abs(losses[k, 0] - l1) < 1e-5 * max(l1, 1e-3)
+        # D-SSIM from the trainer's precomputed target statistics.
+        dssim = 0.5 * (1.0 - ss)
+        assert abs(losses[k, 1] - dssim) < 1e-5 * max(dssim, 1e-3)
         g = oracle.gradients(s64, cam, dl.reshape(-1, 3))
         if acc is None:
             acc = g
