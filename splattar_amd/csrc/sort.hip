@@ -5,8 +5,8 @@
 // (tile || depth) key replaces the reference's two stable 32-bit sorts
 // (binning.rs:5-8, 209-227). Each digit pass is reduce-then-scan over
 // 4096-key partitions, three kernels:
-//   * upsweep: per-partition digit counts (wave64 ballot groups, one LDS add
-//     per digit group), written digit-major;
+//   * upsweep: per-partition digit counts (per-wave LDS histograms), written
+//     digit-major;
 //   * scan: one block per digit turns its row of partition counts into
 //     exclusive offsets and writes the digit's total;
 //   * downsweep: ranks the partition's keys stably with wave64 ballots (8
@@ -76,19 +76,21 @@ __device__ __forceinline__ uint32_t block_exclusive_scan_digits(uint32_t v, uint
     return add + incl - v;
 }
 
-// Partition `part`'s digit counts of the pass at `shift`.
+// Partition `part`'s digit counts of the pass at `shift`: per-wave LDS
+// histograms with plain atomics (a round whose live lanes share one digit
+// adds its lane count once), summed per digit at the end.
 template <typename K, int RB>
 __global__ __launch_bounds__(kSortThreads) void upsweep_kernel(const K* __restrict__ kin,
                                                                const uint32_t* __restrict__ count,
                                                                uint32_t cap, uint32_t shift, uint32_t stride,
                                                                SortScratch* __restrict__ sc) {
     constexpr uint32_t R = 1u << RB;
-    __shared__ uint32_t s_h[R];
-    const int tid = threadIdx.x, lane = tid & 63;
+    __shared__ uint32_t s_h[kSortWaves][R];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t n = min(*count, cap);
     const uint32_t part = blockIdx.x;
     if (part >= (n + kSortPart - 1) / kSortPart) return;  // grid sized for the capacity
-    for (uint32_t k = tid; k < R; k += kSortThreads) s_h[k] = 0u;
+    for (uint32_t k = tid; k < kSortWaves * R; k += kSortThreads) (&s_h[0][0])[k] = 0u;
     const uint32_t base = part * kSortPart;
     K key[kSortItems];
 #pragma unroll
@@ -97,18 +99,27 @@ __global__ __launch_bounds__(kSortThreads) void upsweep_kernel(const K* __restri
         key[r] = idx < n ? kin[idx] : (K)0;
     }
     __syncthreads();
-    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
         const uint32_t idx = base + (uint32_t)r * kSortThreads + (uint32_t)tid;
         const bool ok = idx < n;
         const uint32_t d = (uint32_t)(key[r] >> shift) & (R - 1u);
-        const uint64_t match = digit_match<RB>(d, ok);
-        if (ok && (match & lt_mask) == 0ull) atomicAdd(&s_h[d], (uint32_t)__popcll(match));
+        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+        const uint64_t live = __ballot(ok);
+        if (__ballot(ok && d != d0) == 0ull) {
+            if (lane == 0 && live != 0ull) atomicAdd(&s_h[wave][d0], (uint32_t)__popcll(live));
+        } else if (ok) {
+            atomicAdd(&s_h[wave][d], 1u);
+        }
     }
     __syncthreads();
     uint32_t* off = part_offsets(sc);
-    for (uint32_t k = tid; k < R; k += kSortThreads) off[(size_t)k * stride + part] = s_h[k];
+    for (uint32_t k = tid; k < R; k += kSortThreads) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int w = 0; w < kSortWaves; ++w) c += s_h[w][k];
+        off[(size_t)k * stride + part] = c;
+    }
 }
 
 // One block per digit: row of partition counts -> exclusive offsets; the
