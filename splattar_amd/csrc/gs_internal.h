@@ -464,7 +464,10 @@ __device__ __forceinline__ HitEval eval_hit(const Record& r, float sx, float sy,
     const float Nu = fmaf(r.a_px.x, sx, fmaf(r.a_px.y, sy, r.a_px.z));
     const float Nv = fmaf(r.b_py.x, sx, fmaf(r.b_py.y, sy, r.b_py.z));
     // Branch-free: |D| <= 1e-12 gives inv = 0, hence t = 0 and no ray hit.
-    const float inv = fabsf(D) > 1e-12f ? rcp_refined(D) : 0.0f;
+    // v_rcp_f32 (<= 1 ulp): the error stays in this hit (the backward's
+    // running 1 / (1 - alpha) keeps the Newton step: it compounds along a
+    // list).
+    const float inv = fabsf(D) > 1e-12f ? __builtin_amdgcn_rcpf(D) : 0.0f;
     h.u = Nu * inv;
     h.v = Nv * inv;
     h.t = r.m_nt.w * inv;
