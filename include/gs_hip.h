@@ -93,7 +93,8 @@ void* gs_sorter_keys(gs_sorter s);          /* RadixSorter::keys     (sort.rs:10
 uint32_t* gs_sorter_payloads(gs_sorter s);  /* RadixSorter::payloads (sort.rs:106-109) */
 uint32_t* gs_sorter_counts(gs_sorter s);    /* RadixSorter::counts   (sort.rs:111-114) */
 /* RadixSorter::encode (sort.rs:117-119): sorts key bits [begin_bit,
- * end_bit) with 4- or 8-bit digits; the result is back in keys()/payloads(). */
+ * end_bit) with 4- or 8-bit digits; the result is back in keys()/payloads().
+ * A device count above the capacity sorts the first `capacity` pairs. */
 gs_status gs_sorter_encode(gs_sorter s, void* stream, uint32_t begin_bit, uint32_t end_bit,
                            uint32_t digit_bits);
 

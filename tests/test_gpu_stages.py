@@ -55,6 +55,27 @@ def test_sort32_reference_cases(gpu, digit_bits):
     s.close()
 
 
+def test_sort32_count_above_capacity_sorts_the_capacity(gpu):
+    """A device count larger than the sorter's capacity sorts the first
+    `capacity` pairs (the kernels clamp the count; nothing past the buffers
+    is read or written), with an odd pass count (3 passes of 8 bits) so the
+    result is copied back."""
+    import splattar_amd as sa
+    cap = 10_007
+    s = sa.RadixSorter(gpu, cap, key_bits=32)
+    keys = oracle.xorshift32(0xC0FFEE, cap, 1 << 24)
+    pay = np.arange(cap, dtype=np.uint32)
+    announce("sort32 count above capacity")
+    s.keys().write(keys)
+    s.payloads().write(pay)
+    s.counts().write(np.array([cap + 5000], np.uint32))
+    s.encode(end_bit=24)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(s.keys().read(cap), keys[order])
+    assert np.array_equal(s.payloads().read(cap), pay[order])
+    s.close()
+
+
 def test_sort32_full_range(gpu):
     # sort.rs:82-113: full 32-bit keys at n = 65537.
     import splattar_amd as sa
