@@ -117,8 +117,14 @@ __global__ __launch_bounds__(256) void project_kernel(
     // tests one tile against its owner's footprint read from LDS and sets the
     // owner's bit with an LDS atomic OR. Bit t = row-major tile t of the rect.
     // The cost follows the wave's total tile count, not 64 x its widest item.
-    __shared__ float4 s_fp[256][4];     // box, ell0, ell1, (px, py, rect width, 0)
+    // s_stage holds the footprints for the masks (s_fp), then the block's
+    // records for one contiguous store (rows of kRecRow float4, one spare
+    // against LDS bank conflicts).
+    constexpr int kRecRow = 9;
+    __shared__ float4 s_stage[256 * kRecRow];
     __shared__ uint32_t s_bits[256][2];
+    __shared__ uint32_t s_write[256];
+    float4 (*s_fp)[4] = reinterpret_cast<float4 (*)[4]>(s_stage);  // box, ell0, ell1, (px, py, rect width, 0)
     {
         const int tid = threadIdx.x, lane = tid & 63, wbase = tid & ~63;
         const uint32_t mine = masked ? count : 0u;
@@ -175,62 +181,85 @@ __global__ __launch_bounds__(256) void project_kernel(
             count = (uint32_t)__popcll(mask);
         }
     }
-    if (!in_range) return;
-    if (!live) {
-        finish(make_uint4(1u, 1u, 0u, 0u), 0u, 0u, ~0ull);
-        return;
-    }
-    if (count == 0u) {
-        rect = make_uint4(1u, 1u, 0u, 0u);
-        mask = ~0ull;
-    }
-    finish(rect, __float_as_uint(depth), count, mask);
-    if (count == 0u) return;  // no entry will ever read the record
-
-    // View-dependent colour (surfel_prep_fwd.wgsl:113-147, 183-186).
-    const float inv_len = 1.0f / sqrtf(f.xx * f.xx + f.xy * f.xy + f.xz * f.xz);
-    float basis[16];
-    const int nb = sh_basis((int)cam.sh_degree, f.xx * inv_len, f.xy * inv_len, f.xz * inv_len, basis);
-    // 16-B loads of the (48-float, 192-B aligned) coefficient row, only as
-    // far as the degree needs.
-    const float4* sh4 = reinterpret_cast<const float4*>(sh + (size_t)i * kShFloats);
-    float shv[kShFloats];
-#pragma unroll
-    for (int q = 0; q < kShFloats / 4; ++q) {
-        const float4 v = 4 * q < 3 * nb ? sh4[q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        shv[4 * q + 0] = v.x;
-        shv[4 * q + 1] = v.y;
-        shv[4 * q + 2] = v.z;
-        shv[4 * q + 3] = v.w;
-    }
-    float cr = 0.5f, cg = 0.5f, cb = 0.5f;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        if (k < nb) {
-            cr += basis[k] * shv[3 * k + 0];
-            cg += basis[k] * shv[3 * k + 1];
-            cb += basis[k] * shv[3 * k + 2];
+    // Binning outputs; a record only for items that reach a tile (no entry
+    // ever reads the others).
+    bool write_rec = false;
+    if (in_range) {
+        if (!live) {
+            finish(make_uint4(1u, 1u, 0u, 0u), 0u, 0u, ~0ull);
+        } else {
+            if (count == 0u) {
+                rect = make_uint4(1u, 1u, 0u, 0u);
+                mask = ~0ull;
+            }
+            finish(rect, __float_as_uint(depth), count, mask);
+            write_rec = count != 0u;
         }
     }
-    // Camera-facing unit normal (surfel_prep_fwd.wgsl:188-194).
-    float3 nrm = M;
-    const float mm = dot3(M, M);
-    if (mm > 0.0f) {
-        const float inv = 1.0f / sqrtf(mm);
-        nrm = make_float3(M.x * inv, M.y * inv, M.z * inv);
-        if (dot3(nrm, c) > 0.0f) nrm = make_float3(-nrm.x, -nrm.y, -nrm.z);
+    if (write_rec) {
+        // View-dependent colour (surfel_prep_fwd.wgsl:113-147, 183-186).
+        const float inv_len = 1.0f / sqrtf(f.xx * f.xx + f.xy * f.xy + f.xz * f.xz);
+        float basis[16];
+        const int nb = sh_basis((int)cam.sh_degree, f.xx * inv_len, f.xy * inv_len, f.xz * inv_len, basis);
+        // 16-B loads of the (48-float, 192-B aligned) coefficient row, only as
+        // far as the degree needs.
+        const float4* sh4 = reinterpret_cast<const float4*>(sh + (size_t)i * kShFloats);
+        float shv[kShFloats];
+#pragma unroll
+        for (int q = 0; q < kShFloats / 4; ++q) {
+            const float4 v = 4 * q < 3 * nb ? sh4[q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            shv[4 * q + 0] = v.x;
+            shv[4 * q + 1] = v.y;
+            shv[4 * q + 2] = v.z;
+            shv[4 * q + 3] = v.w;
+        }
+        float cr = 0.5f, cg = 0.5f, cb = 0.5f;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if (k < nb) {
+                cr += basis[k] * shv[3 * k + 0];
+                cg += basis[k] * shv[3 * k + 1];
+                cb += basis[k] * shv[3 * k + 2];
+            }
+        }
+        // Camera-facing unit normal (surfel_prep_fwd.wgsl:188-194).
+        float3 nrm = M;
+        const float mm = dot3(M, M);
+        if (mm > 0.0f) {
+            const float inv = 1.0f / sqrtf(mm);
+            nrm = make_float3(M.x * inv, M.y * inv, M.z * inv);
+            if (dot3(nrm, c) > 0.0f) nrm = make_float3(-nrm.x, -nrm.y, -nrm.z);
+        }
+        rec.col_op = make_float4(fmaxf(cr, 0.0f), fmaxf(cg, 0.0f), fmaxf(cb, 0.0f), opac[i]);
+        rec.nrm_dep = make_float4(nrm.x, nrm.y, nrm.z, depth);
     }
-    rec.col_op = make_float4(fmaxf(cr, 0.0f), fmaxf(cg, 0.0f), fmaxf(cb, 0.0f), opac[i]);
-    rec.nrm_dep = make_float4(nrm.x, nrm.y, nrm.z, depth);
-    Record& out = recs[g];
-    out.m_nt = rec.m_nt;
-    out.a_px = rec.a_px;
-    out.b_py = rec.b_py;
-    out.col_op = rec.col_op;
-    if (write_normal) out.nrm_dep = rec.nrm_dep;
-    out.box = rec.box;
-    out.ell0 = rec.ell0;
-    out.ell1 = rec.ell1;
+    // The block's items are consecutive (one view, 256 consecutive surfels):
+    // stage the records in LDS and store them as one run, lane-consecutive
+    // 16-B stores instead of eight 16-B stores at a 128-B stride per lane.
+    {
+        const int tid = threadIdx.x;
+        s_write[tid] = write_rec ? 1u : 0u;
+        if (write_rec) {
+            float4* row = s_stage + tid * kRecRow;
+            row[0] = rec.m_nt;
+            row[1] = rec.a_px;
+            row[2] = rec.b_py;
+            row[3] = rec.col_op;
+            row[4] = rec.nrm_dep;
+            row[5] = rec.box;
+            row[6] = rec.ell0;
+            row[7] = rec.ell1;
+        }
+        __syncthreads();
+        const uint32_t i0 = i - (uint32_t)tid;
+        const uint32_t nrec = i0 < n ? min(256u, n - i0) : 0u;
+        float4* dst = reinterpret_cast<float4*>(recs + (size_t)view * n + i0);
+        for (uint32_t q = (uint32_t)tid; q < nrec * 8u; q += 256u) {
+            const uint32_t r = q >> 3, fld = q & 7u;
+            if (s_write[r] == 0u || (fld == 4u && !write_normal)) continue;
+            dst[q] = s_stage[r * kRecRow + fld];
+        }
+    }
 }
 
 }  // namespace
