@@ -271,6 +271,18 @@ gs_status gs_optim_set_reg(gs_optim o, const char* name, float reg);  /* optim.r
 gs_status gs_optim_encode_step(gs_optim o, void* stream);
 gs_status gs_optim_encode_activate(gs_optim o, void* stream);  /* optim.rs:230-241 */
 gs_status gs_optim_step_count(gs_optim o, uint32_t* t);        /* blocking */
+/* Optimizer::snapshot_layout (optim.rs:148-153): the classes in add order,
+ * each with its name and raw-parameter size in bytes (n * 4). */
+gs_status gs_optim_class_count(gs_optim o, uint32_t* count);
+gs_status gs_optim_class_info(gs_optim o, uint32_t index, const char** name, uint64_t* bytes);
+/* Optimizer::encode_snapshot / encode_restore (optim.rs:155-165): copy every
+ * class's raw parameters into bufs[k] (device buffers of at least the class's
+ * bytes, add order; count = number of classes), or back from them, as
+ * kernels on the stream (graph-capturable). Restore leaves the activated
+ * copies untouched: follow it with gs_optim_encode_activate, as the
+ * reference's trainer does (trainer.rs:1891-1893). */
+gs_status gs_optim_encode_snapshot(gs_optim o, void* stream, float* const* bufs, uint32_t count);
+gs_status gs_optim_encode_restore(gs_optim o, void* stream, const float* const* bufs, uint32_t count);
 
 /* ------------------------------------------------------------------ Trainer
  * Trainer::step (first_attempt/crates/gs-train/src/trainer.rs:1139-1332):

@@ -133,6 +133,10 @@ _PROTOS = {
     "gs_optim_encode_step": ([_vp, _vp], C.c_int),
     "gs_optim_encode_activate": ([_vp, _vp], C.c_int),
     "gs_optim_step_count": ([_vp, C.POINTER(_u32)], C.c_int),
+    "gs_optim_class_count": ([_vp, C.POINTER(_u32)], C.c_int),
+    "gs_optim_class_info": ([_vp, _u32, C.POINTER(C.c_char_p), C.POINTER(C.c_uint64)], C.c_int),
+    "gs_optim_encode_snapshot": ([_vp, _vp, C.POINTER(_vp), _u32], C.c_int),
+    "gs_optim_encode_restore": ([_vp, _vp, C.POINTER(_vp), _u32], C.c_int),
     "gs_trainer_create": ([_vp, C.POINTER(TrainConfig), C.POINTER(SceneInput), _u32,
                            C.POINTER(TrainView), C.POINTER(_vp)], C.c_int),
     "gs_trainer_destroy": ([_vp], None),

@@ -377,6 +377,28 @@ class Optimizer:
         check(lib().gs_optim_step_count(self.h, C.byref(t)))
         return int(t.value)
 
+    def snapshot_layout(self) -> list[tuple[str, int]]:
+        """[(class name, raw bytes)] in add order (optim.rs:148-153)."""
+        n = C.c_uint32()
+        check(lib().gs_optim_class_count(self.h, C.byref(n)))
+        out = []
+        for k in range(n.value):
+            name, nbytes = C.c_char_p(), C.c_uint64()
+            check(lib().gs_optim_class_info(self.h, k, C.byref(name), C.byref(nbytes)))
+            out.append((name.value.decode(), int(nbytes.value)))
+        return out
+
+    def encode_snapshot(self, dst: list, stream=None):
+        """Copy every class's raw parameters into dst[k] (optim.rs:155-159)."""
+        arr = (C.c_void_p * len(dst))(*[b.ptr for b in dst])
+        check(lib().gs_optim_encode_snapshot(self.h, stream, arr, len(dst)))
+
+    def encode_restore(self, src: list, stream=None):
+        """Copy every class's raw parameters back from src[k] (optim.rs:161-165);
+        the activated copies change only with encode_activate."""
+        arr = (C.c_void_p * len(src))(*[b.ptr for b in src])
+        check(lib().gs_optim_encode_restore(self.h, stream, arr, len(src)))
+
     def close(self):
         if self.h:
             lib().gs_optim_destroy(self.h)

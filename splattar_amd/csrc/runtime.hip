@@ -1349,6 +1349,42 @@ gs_status gs_optim_step_count(gs_optim o, uint32_t* t) {
     return GS_OK;
 }
 
+gs_status gs_optim_class_count(gs_optim o, uint32_t* count) {
+    GS_REQUIRE(o && count, "null argument");
+    *count = (uint32_t)o->classes.size();
+    return GS_OK;
+}
+
+gs_status gs_optim_class_info(gs_optim o, uint32_t index, const char** name, uint64_t* bytes) {
+    GS_REQUIRE(o && name && bytes, "null argument");
+    GS_REQUIRE(index < o->classes.size(), "class index out of range");
+    *name = o->classes[index].name.c_str();
+    *bytes = (uint64_t)o->classes[index].n * sizeof(float);
+    return GS_OK;
+}
+
+gs_status gs_optim_encode_snapshot(gs_optim o, void* stream, float* const* bufs, uint32_t count) {
+    GS_REQUIRE(o && bufs, "null argument");
+    GS_REQUIRE(count == o->classes.size(), "one buffer per optimizer class");
+    for (uint32_t k = 0; k < count; ++k) GS_REQUIRE(bufs[k] != nullptr, "null snapshot buffer");
+    hipStream_t st = as_stream(stream, o->ctx->stream);
+    for (uint32_t k = 0; k < count; ++k)
+        GS_HIP(launch_copy_u32(st, reinterpret_cast<uint32_t*>(bufs[k]),
+                               reinterpret_cast<const uint32_t*>(o->classes[k].raw), o->classes[k].n));
+    return GS_OK;
+}
+
+gs_status gs_optim_encode_restore(gs_optim o, void* stream, const float* const* bufs, uint32_t count) {
+    GS_REQUIRE(o && bufs, "null argument");
+    GS_REQUIRE(count == o->classes.size(), "one buffer per optimizer class");
+    for (uint32_t k = 0; k < count; ++k) GS_REQUIRE(bufs[k] != nullptr, "null snapshot buffer");
+    hipStream_t st = as_stream(stream, o->ctx->stream);
+    for (uint32_t k = 0; k < count; ++k)
+        GS_HIP(launch_copy_u32(st, reinterpret_cast<uint32_t*>(o->classes[k].raw),
+                               reinterpret_cast<const uint32_t*>(bufs[k]), o->classes[k].n));
+    return GS_OK;
+}
+
 }  // extern "C"
 
 // ================================================================== Trainer

@@ -90,6 +90,46 @@ def test_adam_matches_reference(gpu, mode):
     act.free()
 
 
+def test_optimizer_snapshot_and_restore(gpu):
+    """Optimizer::snapshot_layout / encode_snapshot / encode_restore
+    (optim.rs:148-165): every class's raw parameters copied out and back in
+    add order; restore leaves the activated copies until encode_activate (the
+    reference trainer's adaptive-stop restore, trainer.rs:1891-1893)."""
+    import splattar_amd as sa
+    N = 1000
+    opt = sa.Optimizer(gpu)
+    g1, a1 = gpu.alloc(np.float32, 3 * N), gpu.alloc(np.float32, 3 * N)
+    g2, a2 = gpu.alloc(np.float32, N), gpu.alloc(np.float32, N)
+    opt.add_class("pos", 3 * N, opt.IDENTITY, 0.01, g1, a1)
+    opt.add_class("opacity", N, opt.SIGMOID, 0.01, g2, a2)
+    assert opt.snapshot_layout() == [("pos", 12 * N), ("opacity", 4 * N)]
+    rng = np.random.default_rng(5)
+    r1 = rng.normal(size=3 * N).astype(np.float32)
+    r2 = rng.normal(size=N).astype(np.float32)
+    opt.raw("pos").write(r1)
+    opt.raw("opacity").write(r2)
+    snap = [gpu.alloc(np.float32, 3 * N), gpu.alloc(np.float32, N)]
+    announce("optimizer snapshot / restore")
+    opt.encode_snapshot(snap)
+    g1.write(rng.normal(size=3 * N).astype(np.float32))
+    g2.write(rng.normal(size=N).astype(np.float32))
+    opt.encode_step()
+    stepped = a2.read().copy()
+    assert not np.array_equal(opt.raw("pos").read(), r1)
+    opt.encode_restore(snap)
+    assert np.array_equal(opt.raw("pos").read(), r1)
+    assert np.array_equal(opt.raw("opacity").read(), r2)
+    assert np.array_equal(a2.read(), stepped)  # restore touches raw only
+    opt.encode_activate()
+    assert np.array_equal(a1.read(), r1)
+    np.testing.assert_allclose(a2.read(), 1.0 / (1.0 + np.exp(-r2.astype(np.float64))), rtol=2e-6)
+    with pytest.raises(sa.GsError):
+        opt.encode_snapshot(snap[:1])
+    opt.close()
+    for b in (g1, a1, g2, a2, *snap):
+        b.free()
+
+
 # ---------------------------------------------------------------- full step
 
 def _small_training_problem(n=3000, views=3, size=(96, 64), deg=2, seed=3):
