@@ -194,6 +194,29 @@ gs_status gs_raster_backward(gs_raster r, void* stream, uint32_t num_surfels);
 #define GS_BWD_COLOR_ONLY 2u  /* only dL/d(rgb) is non-zero (faster kernel) */
 #define GS_BWD_NO_CAMERA 4u   /* skip grad_cam */
 gs_status gs_raster_backward_ex(gs_raster r, void* stream, uint32_t num_surfels, uint32_t flags);
+/* Batched multi-view extension (no counterpart in the reference, whose
+ * Rasterizer renders one view per forward; for callers that render or
+ * differentiate several views of one scene at once: frame batches, multi-view
+ * training loops). gs_raster_create_batch makes a rasterizer with room for
+ * max_views views of at most width x height pixels and max_entries tile
+ * entries each; its per-pixel buffers (out_*, dl_dcolor, dl_dnormal) hold a
+ * batch's views back to back in view order (view j starts at pixel
+ * sum_{k<j} w_k h_k) and grad_cam holds 16 floats per view. The single-view
+ * calls work on it as on a gs_raster_create(width, height) rasterizer, which
+ * is gs_raster_create_batch with max_views = 1. */
+typedef struct gs_view {
+    gs_raster_camera camera;
+    uint32_t width, height;  /* <= the rasterizer's width x height */
+    uint32_t num_surfels;    /* the view renders surfels [0, num_surfels) */
+} gs_view;
+gs_status gs_raster_create_batch(gs_ctx ctx, uint32_t capacity, uint32_t width, uint32_t height,
+                                 uint32_t max_entries, uint32_t max_views, gs_raster* out);
+/* Forward of n_views views: one launch per stage for the whole batch. */
+gs_status gs_raster_forward_batch(gs_raster r, void* stream, const gs_view* views, uint32_t n_views);
+/* Backward of the last gs_raster_forward_batch (flags as
+ * gs_raster_backward_ex): grad_* receive the gradients summed over its views,
+ * grad_cam[16 j ..] view j's camera gradients. */
+gs_status gs_raster_backward_batch(gs_raster r, void* stream, uint32_t flags);
 /* Blocking: entries produced by the last forward; GS_ERR_OVERFLOW if it
  * exceeded max_entries (the reference does not check, binning.rs:186). */
 gs_status gs_raster_entry_count(gs_raster r, uint32_t* entries);

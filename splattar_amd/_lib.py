@@ -37,6 +37,11 @@ class RasterCamera(C.Structure):
                 ("sh_degree", C.c_uint32)]
 
 
+class View(C.Structure):
+    _fields_ = [("camera", RasterCamera), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("num_surfels", C.c_uint32)]
+
+
 class SceneInput(C.Structure):
     _fields_ = [("positions", C.c_void_p), ("scales", C.c_void_p), ("quats", C.c_void_p),
                 ("opacities", C.c_void_p), ("sh", C.c_void_p), ("sh_coeffs", C.c_uint32),
@@ -116,6 +121,9 @@ _PROTOS = {
     "gs_raster_forward": ([_vp, _vp, C.POINTER(RasterCamera), _u32], C.c_int),
     "gs_raster_backward": ([_vp, _vp, _u32], C.c_int),
     "gs_raster_backward_ex": ([_vp, _vp, _u32, _u32], C.c_int),
+    "gs_raster_create_batch": ([_vp, _u32, _u32, _u32, _u32, _u32, C.POINTER(_vp)], C.c_int),
+    "gs_raster_forward_batch": ([_vp, _vp, C.POINTER(View), _u32], C.c_int),
+    "gs_raster_backward_batch": ([_vp, _vp, _u32], C.c_int),
     "gs_raster_entry_count": ([_vp, C.POINTER(_u32)], C.c_int),
     "gs_raster_debug_buffers": ([_vp, C.POINTER(RasterDebug)], C.c_int),
     "gs_loss_create": ([_vp, _u32, _u32, C.c_float, _vp, _vp, C.POINTER(_vp)], C.c_int),

@@ -230,11 +230,17 @@ def scene_input(positions, scales, quats, opacities, sh, keep: list) -> L.SceneI
 class Rasterizer:
     """Rasterizer (rasterizer.rs:56-448)."""
 
-    def __init__(self, ctx: GpuContext, capacity: int, width: int, height: int, max_entries: int):
+    def __init__(self, ctx: GpuContext, capacity: int, width: int, height: int, max_entries: int,
+                 max_views: int = 1):
+        """max_views > 1 makes a batched rasterizer (gs_raster_create_batch):
+        per-pixel buffers then hold max_views images of width x height (a
+        batch's views back to back) and grad_cam 16 floats per view."""
         self.ctx = ctx
         self.capacity, self.width, self.height, self.max_entries = capacity, width, height, max_entries
+        self.max_views = max_views
         h = C.c_void_p()
-        check(lib().gs_raster_create(ctx.h, capacity, width, height, max_entries, C.byref(h)))
+        check(lib().gs_raster_create_batch(ctx.h, capacity, width, height, max_entries, max_views,
+                                           C.byref(h)))
         self.h = h.value
         b = L.RasterBuffers()
         check(lib().gs_raster_get_buffers(self.h, C.byref(b)))
@@ -245,18 +251,28 @@ class Rasterizer:
         self.quats = DeviceBuffer(ctx, b.quats, f, (capacity, 4))
         self.opacities = DeviceBuffer(ctx, b.opacities, f, capacity)
         self.sh = DeviceBuffer(ctx, b.sh, f, (capacity, 48))
-        self.out_color = DeviceBuffer(ctx, b.out_color, f, (height, width, 4))
-        self.out_aux = DeviceBuffer(ctx, b.out_aux, f, (height, width, 4))
-        self.out_normal = DeviceBuffer(ctx, b.out_normal, f, (height, width, 4))
-        self.out_ncontrib = DeviceBuffer(ctx, b.out_ncontrib, u, (height, width))
-        self.dl_dcolor = DeviceBuffer(ctx, b.dl_dcolor, f, (height, width, 4))
-        self.dl_dnormal = DeviceBuffer(ctx, b.dl_dnormal, f, (height, width, 4))
+        if max_views > 1:
+            # Flat per-pixel views (a batch packs its views back to back).
+            Pb = P * max_views
+            self.out_color = DeviceBuffer(ctx, b.out_color, f, (Pb, 4))
+            self.out_aux = DeviceBuffer(ctx, b.out_aux, f, (Pb, 4))
+            self.out_normal = DeviceBuffer(ctx, b.out_normal, f, (Pb, 4))
+            self.out_ncontrib = DeviceBuffer(ctx, b.out_ncontrib, u, Pb)
+            self.dl_dcolor = DeviceBuffer(ctx, b.dl_dcolor, f, (Pb, 4))
+            self.dl_dnormal = DeviceBuffer(ctx, b.dl_dnormal, f, (Pb, 4))
+        else:
+            self.out_color = DeviceBuffer(ctx, b.out_color, f, (height, width, 4))
+            self.out_aux = DeviceBuffer(ctx, b.out_aux, f, (height, width, 4))
+            self.out_normal = DeviceBuffer(ctx, b.out_normal, f, (height, width, 4))
+            self.out_ncontrib = DeviceBuffer(ctx, b.out_ncontrib, u, (height, width))
+            self.dl_dcolor = DeviceBuffer(ctx, b.dl_dcolor, f, (height, width, 4))
+            self.dl_dnormal = DeviceBuffer(ctx, b.dl_dnormal, f, (height, width, 4))
         self.grad_pos = DeviceBuffer(ctx, b.grad_pos, f, (capacity, 4))
         self.grad_scales = DeviceBuffer(ctx, b.grad_scales, f, (capacity, 2))
         self.grad_quat = DeviceBuffer(ctx, b.grad_quat, f, (capacity, 4))
         self.grad_opacity = DeviceBuffer(ctx, b.grad_opacity, f, capacity)
         self.grad_sh = DeviceBuffer(ctx, b.grad_sh, f, (capacity, 48))
-        self.grad_cam = DeviceBuffer(ctx, b.grad_cam, f, 16)
+        self.grad_cam = DeviceBuffer(ctx, b.grad_cam, f, 16 if max_views == 1 else (max_views, 16))
         self._P = P
 
     def upload_scene(self, positions, scales, quats, opacities, sh):
@@ -284,6 +300,18 @@ class Rasterizer:
 
     def backward(self, num_surfels: int, stream=None, flags: int = 0):
         check(lib().gs_raster_backward_ex(self.h, stream, num_surfels, flags))
+
+    def forward_batch(self, views, stream=None):
+        """views: [(Camera, width, height, num_surfels)]; view j's pixels follow
+        view j-1's in the per-pixel buffers."""
+        arr = (L.View * len(views))()
+        for k, (cam, w, h, m) in enumerate(views):
+            arr[k].camera = cam.c()
+            arr[k].width, arr[k].height, arr[k].num_surfels = w, h, m
+        check(lib().gs_raster_forward_batch(self.h, stream, arr, len(views)))
+
+    def backward_batch(self, stream=None, flags: int = 0):
+        check(lib().gs_raster_backward_batch(self.h, stream, flags))
 
     def entry_count(self) -> int:
         e = C.c_uint32()

@@ -928,6 +928,9 @@ struct gs_raster_s {
     float* grad_cam = nullptr;
     uint32_t last_n = 0;
     bool have_forward = false;
+    uint32_t max_views = 1;
+    uint32_t last_views = 0;   // views of the last forward (0: none)
+    bool last_batch = false;   // the last forward was gs_raster_forward_batch
     ~gs_raster_s() {
         dfree(positions);
         dfree(scales);
@@ -973,11 +976,13 @@ static gs_status upload_scene_into(gs_ctx ctx, const gs_scene_input* s, uint32_t
 
 extern "C" {
 
-gs_status gs_raster_create(gs_ctx ctx, uint32_t capacity, uint32_t width, uint32_t height,
-                           uint32_t max_entries, gs_raster* out) {
+gs_status gs_raster_create_batch(gs_ctx ctx, uint32_t capacity, uint32_t width, uint32_t height,
+                                 uint32_t max_entries, uint32_t max_views, gs_raster* out) {
     GS_REQUIRE(ctx && out, "null argument");
     GS_REQUIRE(width > 0 && height > 0, "image size must be > 0");
-    GS_REQUIRE((uint64_t)width * height < (1ull << 31), "image too large");
+    GS_REQUIRE(max_views >= 1 && max_views <= 64, "max_views must be in 1..64");
+    GS_REQUIRE((uint64_t)width * height * max_views < (1ull << 31), "image too large");
+    GS_REQUIRE((uint64_t)max_entries * max_views < (1ull << 30), "max_entries * max_views must be < 2^30");
     *out = nullptr;
     auto r = std::make_unique<gs_raster_s>();
     r->ctx = ctx;
@@ -985,9 +990,10 @@ gs_status gs_raster_create(gs_ctx ctx, uint32_t capacity, uint32_t width, uint32
     r->width = width;
     r->height = height;
     r->max_entries = max_entries;
+    r->max_views = max_views;
     const uint32_t tiles = div_up(width, kTile) * div_up(height, kTile);
-    const uint32_t px = width * height;
-    GS_CHECK(r->core.init(ctx, capacity, 1, max_entries, tiles, px));
+    const uint32_t px = width * height * max_views;
+    GS_CHECK(r->core.init(ctx, capacity, max_views, max_entries * max_views, tiles * max_views, px));
     GS_CHECK(dalloc(&r->positions, capacity, "positions"));
     GS_CHECK(dalloc(&r->scales, capacity, "scales"));
     GS_CHECK(dalloc(&r->quats, capacity, "quats"));
@@ -1000,9 +1006,14 @@ gs_status gs_raster_create(gs_ctx ctx, uint32_t capacity, uint32_t width, uint32
     GS_CHECK(dalloc(&r->grad_quat, capacity, "grad_quat"));
     GS_CHECK(dalloc(&r->grad_opacity, capacity, "grad_opacity"));
     GS_CHECK(dalloc(&r->grad_sh, (size_t)capacity * kShFloats, "grad_sh"));
-    GS_CHECK(dalloc(&r->grad_cam, 16, "grad_cam"));
+    GS_CHECK(dalloc(&r->grad_cam, (size_t)16 * max_views, "grad_cam"));
     *out = r.release();
     return GS_OK;
+}
+
+gs_status gs_raster_create(gs_ctx ctx, uint32_t capacity, uint32_t width, uint32_t height,
+                           uint32_t max_entries, gs_raster* out) {
+    return gs_raster_create_batch(ctx, capacity, width, height, max_entries, 1, out);
 }
 
 void gs_raster_destroy(gs_raster r) {
@@ -1077,13 +1088,55 @@ gs_status gs_raster_forward(gs_raster r, void* stream, const gs_raster_camera* c
     GS_CHECK(r->core.set_views(st, {c}, true));
     GS_CHECK(r->core.forward(st, 0, 1, num_surfels, r->positions, r->scales, r->quats, r->opacities, r->sh));
     r->last_n = num_surfels;
+    r->last_views = 1;
+    r->last_batch = false;
     r->have_forward = true;
+    return GS_OK;
+}
+
+gs_status gs_raster_forward_batch(gs_raster r, void* stream, const gs_view* views, uint32_t n_views) {
+    GS_REQUIRE(r && views, "null argument");
+    GS_REQUIRE(n_views >= 1 && n_views <= r->max_views, "view count out of range for this rasterizer");
+    std::vector<CamParams> cams;
+    uint32_t n = 0;
+    for (uint32_t j = 0; j < n_views; ++j) {
+        const gs_view& v = views[j];
+        GS_CHECK(check_camera(&v.camera));
+        GS_REQUIRE(v.width > 0 && v.height > 0 && v.width <= r->width && v.height <= r->height,
+                   "view size must be within the rasterizer's max width x height");
+        if (v.num_surfels > r->capacity) return fail(GS_ERR_CAPACITY, "num_surfels exceeds capacity");
+        CamParams c = make_cam(v.camera, v.width, v.height);
+        c.lambda_dist = r->lambda_dist;
+        c.n_surfels = v.num_surfels;
+        cams.push_back(c);
+        n = std::max(n, v.num_surfels);
+    }
+    hipStream_t st = as_stream(stream, r->ctx->stream);
+    GS_CHECK(r->core.set_views(st, cams, true));
+    GS_CHECK(r->core.forward(st, 0, n_views, n, r->positions, r->scales, r->quats, r->opacities, r->sh));
+    r->last_n = n;
+    r->last_views = n_views;
+    r->last_batch = true;
+    r->have_forward = true;
+    return GS_OK;
+}
+
+gs_status gs_raster_backward_batch(gs_raster r, void* stream, uint32_t flags) {
+    GS_REQUIRE(r != nullptr, "raster is null");
+    GS_REQUIRE(r->have_forward && r->last_batch, "backward_batch needs a forward_batch on this rasterizer first");
+    hipStream_t st = as_stream(stream, r->ctx->stream);
+    const bool accumulate = (flags & GS_BWD_ACCUMULATE) != 0;
+    const bool aux = (flags & GS_BWD_COLOR_ONLY) == 0;
+    ParamGrads g{r->grad_pos, r->grad_scales, r->grad_quat, r->grad_opacity, r->grad_sh,
+                 (flags & GS_BWD_NO_CAMERA) ? nullptr : r->grad_cam};
+    GS_CHECK(r->core.backward(st, 0, r->last_views, r->last_n, r->positions, r->scales, r->quats, r->sh,
+                              r->dl_dcolor, r->dl_dnormal, g, accumulate, aux));
     return GS_OK;
 }
 
 gs_status gs_raster_backward_ex(gs_raster r, void* stream, uint32_t num_surfels, uint32_t flags) {
     GS_REQUIRE(r != nullptr, "raster is null");
-    GS_REQUIRE(r->have_forward, "backward needs a forward on this rasterizer first");
+    GS_REQUIRE(r->have_forward && !r->last_batch, "backward needs a forward on this rasterizer first");
     GS_REQUIRE(num_surfels == r->last_n, "backward num_surfels must match the forward");
     hipStream_t st = as_stream(stream, r->ctx->stream);
     const bool accumulate = (flags & GS_BWD_ACCUMULATE) != 0;
