@@ -51,11 +51,15 @@ __global__ __launch_bounds__(256) void ssim_fwd_kernel(uint32_t W, uint32_t H, f
     __shared__ float hb1[kS][kT];        // horizontally blurred xy
     __shared__ double s_red[2][4];
     const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-    const int ch = (int)blockIdx.z;
-    // XCD-contiguous tile order: neighbouring tiles (shared halos) share an L2.
-    const uint32_t tiles = gridDim.x * gridDim.y;
-    const uint32_t lb = xcd_swizzle(blockIdx.y * gridDim.x + blockIdx.x, tiles);
-    const int bx = (int)(lb % gridDim.x), by = (int)(lb / gridDim.x);
+    // Logical block = 3 tile + channel, XCD-contiguous: a tile's three
+    // channel workgroups and its neighbours (shared halos) share an L2, so
+    // the float4 pixels each channel reads one component of come from L2.
+    const uint32_t tiles_x = (W + kT - 1) / kT;
+    const uint32_t tiles = tiles_x * ((H + kT - 1) / kT);
+    const uint32_t lg = xcd_swizzle(blockIdx.x, gridDim.x);
+    const uint32_t lb = lg / 3u;
+    const int ch = (int)(lg - 3u * lb);
+    const int bx = (int)(lb % tiles_x), by = (int)(lb / tiles_x);
     const int x0 = bx * kT - kH, y0 = by * kT - kH;
     const float* imgf = reinterpret_cast<const float*>(img) + ch;
     const float* tgtf = reinterpret_cast<const float*>(tgt) + ch;
@@ -159,9 +163,11 @@ __global__ __launch_bounds__(256) void ssim_bwd_kernel(uint32_t W, uint32_t H, f
     __shared__ float4 sc[kS][kS + 1];
     __shared__ float4 hb[kS][kT];
     const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-    const int ch = (int)blockIdx.z;
-    const uint32_t lb = xcd_swizzle(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
-    const int bx = (int)(lb % gridDim.x), by = (int)(lb / gridDim.x);
+    const uint32_t tiles_x = (W + kT - 1) / kT;
+    const uint32_t lg = xcd_swizzle(blockIdx.x, gridDim.x);  // 3 tile + channel, as ssim_fwd
+    const uint32_t lb = lg / 3u;
+    const int ch = (int)(lg - 3u * lb);
+    const int bx = (int)(lb % tiles_x), by = (int)(lb / tiles_x);
     const int x0 = bx * kT - kH, y0 = by * kT - kH;
     const size_t P = (size_t)W * H;
     const float4* cch = coef + ch * P;
@@ -226,7 +232,7 @@ hipError_t launch_l1_ssim(hipStream_t st, uint32_t w, uint32_t h, float lambda, 
                           float* coef_scratch, double* partial_sums, uint32_t /*n_partials*/,
                           const CamParams* view) {
     if (w == 0 || h == 0) return hipSuccess;
-    dim3 grid(div_up(w, kT), div_up(h, kT), 3);
+    dim3 grid(div_up(w, kT) * div_up(h, kT) * 3);
     float4* coef = reinterpret_cast<float4*>(coef_scratch);
     // Uniform loss weights, computed once: dw = -l s / 3P (D-SSIM term), the
     // L1 term's (1 - l) / 3P.
