@@ -257,7 +257,9 @@ __global__ __launch_bounds__(256) void project_kernel(
         for (uint32_t q = (uint32_t)tid; q < nrec * 8u; q += 256u) {
             const uint32_t r = q >> 3, fld = q & 7u;
             if (s_write[r] == 0u || (fld == 4u && !write_normal)) continue;
-            dst[q] = s_stage[r * kRecRow + fld];
+            const float4 v = s_stage[r * kRecRow + fld];
+            typedef float f4v __attribute__((ext_vector_type(4)));
+            __builtin_nontemporal_store(f4v{v.x, v.y, v.z, v.w}, reinterpret_cast<f4v*>(dst + q));
         }
     }
 }
