@@ -296,6 +296,14 @@ constexpr int kFuseThreads = 128;
 constexpr int kShVec = kShFloats / 4;  // float4 per SH row
 constexpr int kShRow = kShVec + 1;     // LDS row stride in float4 (one vector of padding)
 
+// The optimizer state and parameters written here are next read a whole
+// step later: non-temporal stores keep them from displacing the step's
+// working set in L2 / MALL.
+typedef float f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void store_nt(float4* p, float4 v) {
+    __builtin_nontemporal_store(f4v{v.x, v.y, v.z, v.w}, reinterpret_cast<f4v*>(p));
+}
+
 // Adam over 4 consecutive floats of class c at float4 index j (raw given).
 __device__ __forceinline__ void adam4(const AdamClass& c, float bc1, float bc2, uint32_t j, float4 raw, float4 g) {
     float4 m = reinterpret_cast<const float4*>(c.m)[j];
@@ -304,12 +312,12 @@ __device__ __forceinline__ void adam4(const AdamClass& c, float bc1, float bc2, 
     adam_element(c.activation, c.lr, c.reg, bc1, bc2, raw.y, g.y, m.y, v.y);
     adam_element(c.activation, c.lr, c.reg, bc1, bc2, raw.z, g.z, m.z, v.z);
     adam_element(c.activation, c.lr, c.reg, bc1, bc2, raw.w, g.w, m.w, v.w);
-    reinterpret_cast<float4*>(c.raw)[j] = raw;
-    reinterpret_cast<float4*>(c.m)[j] = m;
-    reinterpret_cast<float4*>(c.v)[j] = v;
-    reinterpret_cast<float4*>(c.activated)[j] =
-        make_float4(adam_activate(c.activation, raw.x), adam_activate(c.activation, raw.y),
-                    adam_activate(c.activation, raw.z), adam_activate(c.activation, raw.w));
+    store_nt(reinterpret_cast<float4*>(c.raw) + j, raw);
+    store_nt(reinterpret_cast<float4*>(c.m) + j, m);
+    store_nt(reinterpret_cast<float4*>(c.v) + j, v);
+    store_nt(reinterpret_cast<float4*>(c.activated) + j,
+             make_float4(adam_activate(c.activation, raw.x), adam_activate(c.activation, raw.y),
+                         adam_activate(c.activation, raw.z), adam_activate(c.activation, raw.w)));
 }
 
 // Table classes: 0 position (float4 rows), 1 scales (float2), 2 quaternion
