@@ -9,8 +9,8 @@
 // workgroup per (16x16 tile, channel) -- the channels are independent, and a
 // channel's working set (x, y staged with a 5 px halo; five blurred rows)
 // fits in ~14 KB of LDS, so many workgroups share a CU:
-//   ssim_fwd: stage x, y; horizontal pass in column pairs (one 12-tap window
-//             shared by two outputs, products formed once per input);
+//   ssim_fwd: stage x, y; horizontal pass in column quads (one 14-tap window
+//             shared by four outputs, products formed once per input);
 //             vertical pass; SSIM and the three coefficient maps
 //             C_mu = dw dS/dmu_x, C_x2 = dw dS/dE[x^2], C_xy = dw dS/dE[xy];
 //             per-(channel, tile) L1 / SSIM partial sums (deterministic);
@@ -27,7 +27,7 @@ namespace {
 constexpr int kT = 16;          // output tile
 constexpr int kH = 5;           // halo (11-tap window)
 constexpr int kS = kT + 2 * kH; // staged side = 26
-constexpr int kPairs = kS * (kT / 2);  // horizontal tasks: staged rows x output column pairs = 208
+constexpr int kQuads = kS * (kT / 4);  // horizontal tasks: staged rows x output column quads = 104
 __constant__ float c_w[6] = {0.26601818f, 0.21300237f, 0.10936069f,
                              0.03599398f, 0.00759866f, 0.00102838f};
 
@@ -75,28 +75,34 @@ __global__ __launch_bounds__(256) void ssim_fwd_kernel(uint32_t W, uint32_t H, f
         }
     }
     __syncthreads();
-    if (tid < kPairs) {
-        const int r = tid >> 3, c0 = (tid & 7) * 2;
-        float a0[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, a1[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    if (tid < kQuads) {
+        // Four outputs per task: a 14-tap window shared by four columns, the
+        // products formed once per input; each output sums its 11 taps in
+        // window order.
+        const int r = tid >> 2, c0 = (tid & 3) * 4;
+        float a[4][5];
 #pragma unroll
-        for (int j = 0; j < 2 * kH + 2; ++j) {
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int k = 0; k < 5; ++k) a[q][k] = 0.f;
+#pragma unroll
+        for (int j = 0; j < 2 * kH + 4; ++j) {
             const float2 v = sxy[r][c0 + j];
             const float m[5] = {v.x, v.y, v.x * v.x, v.y * v.y, v.x * v.y};
-            if (j <= 2 * kH) {
-                const float w = tap(j - kH);
 #pragma unroll
-                for (int k = 0; k < 5; ++k) a0[k] += w * m[k];
-            }
-            if (j >= 1) {
-                const float w = tap(j - 1 - kH);
+            for (int q = 0; q < 4; ++q) {
+                if (j >= q && j - q <= 2 * kH) {
+                    const float w = tap(j - q - kH);
 #pragma unroll
-                for (int k = 0; k < 5; ++k) a1[k] += w * m[k];
+                    for (int k = 0; k < 5; ++k) a[q][k] += w * m[k];
+                }
             }
         }
-        hb4[r][c0] = make_float4(a0[0], a0[1], a0[2], a0[3]);
-        hb1[r][c0] = a0[4];
-        hb4[r][c0 + 1] = make_float4(a1[0], a1[1], a1[2], a1[3]);
-        hb1[r][c0 + 1] = a1[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            hb4[r][c0 + q] = make_float4(a[q][0], a[q][1], a[q][2], a[q][3]);
+            hb1[r][c0 + q] = a[q][4];
+        }
     }
     __syncthreads();
     float mu_x = 0.f, mu_y = 0.f, bx2 = 0.f, by2 = 0.f, bxy = 0.f;
@@ -152,7 +158,7 @@ __global__ __launch_bounds__(256) void ssim_fwd_kernel(uint32_t W, uint32_t H, f
 }
 
 // One channel per workgroup: stage its three coefficient maps (one float4
-// per pixel), blur them (column pairs, then rows) and write that channel of
+// per pixel), blur them (column quads, then rows) and write that channel of
 // dL/dx = (1-l) sign(x-y)/3P + B(C_mu) + 2x B(C_x2) + y B(C_xy); the
 // channel-0 workgroup also writes w = 0.
 __global__ __launch_bounds__(256) void ssim_bwd_kernel(uint32_t W, uint32_t H, float l1w,
@@ -182,27 +188,26 @@ __global__ __launch_bounds__(256) void ssim_bwd_kernel(uint32_t W, uint32_t H, f
         }
     }
     __syncthreads();
-    if (tid < kPairs) {
-        const int r = tid >> 3, c0 = (tid & 7) * 2;
-        float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = a0;
+    if (tid < kQuads) {
+        const int r = tid >> 2, c0 = (tid & 3) * 4;  // four columns per task, as ssim_fwd
+        float4 a[4];
 #pragma unroll
-        for (int j = 0; j < 2 * kH + 2; ++j) {
+        for (int q = 0; q < 4; ++q) a[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int j = 0; j < 2 * kH + 4; ++j) {
             const float4 v = sc[r][c0 + j];
-            if (j <= 2 * kH) {
-                const float w = tap(j - kH);
-                a0.x += w * v.x;
-                a0.y += w * v.y;
-                a0.z += w * v.z;
-            }
-            if (j >= 1) {
-                const float w = tap(j - 1 - kH);
-                a1.x += w * v.x;
-                a1.y += w * v.y;
-                a1.z += w * v.z;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (j >= q && j - q <= 2 * kH) {
+                    const float w = tap(j - q - kH);
+                    a[q].x += w * v.x;
+                    a[q].y += w * v.y;
+                    a[q].z += w * v.z;
+                }
             }
         }
-        hb[r][c0] = a0;
-        hb[r][c0 + 1] = a1;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) hb[r][c0 + q] = a[q];
     }
     __syncthreads();
     float b0 = 0.f, b1 = 0.f, b2 = 0.f;
