@@ -197,8 +197,9 @@ def main():
         copy_calibration()
         return
     if a.gpus != 1:
-        print(f"bench.py runs on exactly one GPU (asked for --gpus {a.gpus}); multi-GPU sharding is "
-              "described in DESIGN.md and not built.", file=sys.stderr)
+        print(f"bench.py runs on exactly one GPU (asked for --gpus {a.gpus}); the multi-GPU "
+              "sharding of the step (by view, one gradient all-reduce) is designed, not built.",
+              file=sys.stderr)
         sys.exit(2)
     import splattar_amd as sa
     from splattar_amd._lib import GS_BINNING_PRESORT, GS_BINNING_SORT64, GS_BINNING_TIGHT
