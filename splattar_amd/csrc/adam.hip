@@ -1,4 +1,4 @@
-// Fused Adam over every parameter class in one launch (A12).
+// Fused Adam over every parameter class in one launch.
 //
 // Restates first_attempt/crates/gs-train/src/shaders/adam.wgsl:44-74 (adam_step
 // then activate) with the host-side bias corrections of optim.rs:179-197:

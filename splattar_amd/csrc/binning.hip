@@ -1,4 +1,4 @@
-// Tile binning: key emission (A5) and tile ranges (A7).
+// Tile binning: key emission and tile ranges.
 //
 // emit_keys restates `expand` of crates/gs-kernels/src/shaders/
 // tile_binning.wgsl:46-66, with one 64-bit key per entry instead of the

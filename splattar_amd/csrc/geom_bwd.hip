@@ -1,4 +1,4 @@
-// Geometry backward (A11): one thread per surfel, all views of a batch.
+// Geometry backward: one thread per surfel, all views of a batch.
 //
 // Restates crates/gs-kernels/src/shaders/surfel_prep_bwd.wgsl:129-284 (the
 // oracle's backward.rs:176-245 and quat_grad of math.rs:38-88). The

@@ -1,4 +1,4 @@
-// Fused L1 + D-SSIM loss and its image gradient (A9), two LDS-tiled kernels.
+// Fused L1 + D-SSIM loss and its image gradient, two LDS-tiled kernels.
 //
 // Restates first_attempt/crates/gs-train/src/shaders/ssim.wgsl:41-143 and
 // ssim.rs:1-4,84: loss = (1-l) mean|x-y| + l * s * mean(1 - SSIM), 11-tap

@@ -1,4 +1,4 @@
-// Surfel projection (A2) with tile counting (A3) fused: one thread per
+// Surfel projection with tile counting fused: one thread per
 // (view, surfel). Restates crates/gs-kernels/src/shaders/surfel_prep_fwd.wgsl:
 // 64-132 and count_tiles of tile_binning.wgsl:30-43, and additionally emits
 // the per-surfel ray-splat setup (M, A, B, Nt) so the rasterizer's per-pixel

@@ -1,4 +1,4 @@
-// Tile rasterizer backward (A10): the hand-written adjoint of raster_fwd.
+// Tile rasterizer backward: the hand-written adjoint of raster_fwd.
 //
 // Restates crates/gs-kernels/src/shaders/rasterize_bwd.wgsl:113-312 (the
 // oracle's backward.rs:67-173): per pixel, walk the tile list back to front

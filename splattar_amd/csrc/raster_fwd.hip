@@ -1,4 +1,4 @@
-// Tile rasterizer forward (A8): one workgroup per work tile (of any view in
+// Tile rasterizer forward: one workgroup per work tile (of any view in
 // a batch), front-to-back alpha compositing.
 //
 // Restates crates/gs-kernels/src/shaders/rasterize_fwd.wgsl:20-85 and

@@ -1,4 +1,4 @@
-// Exclusive prefix sum over u32 (A4): reduce-then-scan over 4096-item
+// Exclusive prefix sum over u32: reduce-then-scan over 4096-item
 // partitions.
 //
 // Semantics of crates/gs-wgpu/src/prefix_sum.rs:19-164 / prefix_sum.wgsl:

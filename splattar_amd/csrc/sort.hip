@@ -1,4 +1,4 @@
-// Stable LSD radix sort of (key, u32 payload) pairs, 64- or 32-bit keys (A6).
+// Stable LSD radix sort of (key, u32 payload) pairs, 64- or 32-bit keys.
 //
 // Semantics of crates/gs-wgpu/src/sort.rs:17-327 (stable, element count read
 // from a device buffer so it never round-trips to the host) but one 64-bit

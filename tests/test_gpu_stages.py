@@ -1,4 +1,4 @@
-"""GPU parity of the integer stages (A2-A7), bit-exact, through the C ABI.
+"""GPU parity of the integer stages (projection to tile ranges), bit-exact, through the C ABI.
 
 Case lists restate the reference's own GPU tests:
   scan    crates/gs-wgpu/tests/prefix_sum.rs:57-73

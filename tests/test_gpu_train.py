@@ -1,5 +1,5 @@
-"""GPU parity of the training-side stages: L1 + D-SSIM loss (A9), fused Adam
-(A12) and the whole training step (A13) in its plain, batched and graph forms.
+"""GPU parity of the training-side stages: L1 + D-SSIM loss, fused Adam and
+the whole training step in its plain, batched and graph forms.
 
 Loss: against the FP64 oracle restatement of ssim.wgsl (gradient relative
 error <= 1e-4 normwise; loss values to 1e-6 relative).
