@@ -467,7 +467,14 @@ hipError_t launch_geom_bwd(hipStream_t st, const CamParams* cams_dev, uint32_t n
                            const float4* pos, const float2* scales, const float4* quats,
                            const float* sh, const float* grad_coef, bool aux,
                            ParamGrads g, bool accumulate, float* cam_partials) {
-    if (n == 0) return hipSuccess;
+    if (n_views == 0) return hipSuccess;
+    if (n == 0) {
+        // No surfel: a pass that does not accumulate still sets the camera
+        // gradients to 0 (the reduction over no partials reads nothing).
+        if (g.cam == nullptr || accumulate) return hipSuccess;
+        hipLaunchKernelGGL(cam_reduce_kernel, dim3(n_views), dim3(256), 0, st, cam_partials, 0u, g.cam, 0);
+        return hipGetLastError();
+    }
     const uint32_t blocks = div_up(n, kGeomThreads);
     float* parts = g.cam ? cam_partials : nullptr;
     if (aux)

@@ -522,3 +522,33 @@ def test_batched_views_equal_single_view_renders(gpu):
         scale = max(float(np.abs(cam_g[k]).max()), 1e-12)
         assert float(np.abs(gc[k] - cam_g[k]).max()) <= 1e-4 * scale, f"view {k} camera gradient"
     b.close()
+
+
+@pytest.mark.parametrize("case", ["behind_camera", "zero_surfels"])
+def test_empty_frame_renders_background_and_zero_gradients(gpu, case):
+    """A frame with no visible surfel (all behind the camera, or a forward
+    over 0 surfels): zero entries, every pixel black with T = 1 and no
+    contributor (the oracle's empty render, forward.rs:207-246), and a
+    backward that leaves every gradient exactly 0."""
+    n, size, deg = 500, 72, 1
+    scene, cam = _dense_scene(n, size, deg, 21)
+    scene.pos[:, 2] = -3.5  # behind the eye at z = -3
+    s32, c32 = scene_from_f32(scene), camera_from_f32(cam)
+    ref = oracle.render(s32, c32)
+    assert not ref["color"].any()
+    m = n if case == "behind_camera" else 0
+    announce(f"empty frame: {case}")
+    r = _rasterizer(gpu, scene, cam)
+    dl = np.zeros((size, size, 4), np.float32)
+    dl[..., :3] = np.random.default_rng(3).uniform(-1, 1, (size, size, 3))
+    r.dl_dcolor.write(dl)
+    r.grad_cam.write(np.full(16, 7.0, np.float32))  # must be overwritten
+    r.forward(_camera(cam, deg), m)
+    assert r.entry_count() == 0
+    assert not r.out_color.read().any()
+    assert np.all(r.out_aux.read()[..., 0] == 1.0)
+    assert not r.out_ncontrib.read().any()
+    r.backward(m, flags=0)
+    for g in (r.grad_pos, r.grad_scales, r.grad_quat, r.grad_opacity, r.grad_sh, r.grad_cam):
+        assert not g.read().any()
+    r.close()
