@@ -1,8 +1,8 @@
 // C-ABI runtime for the MI355X surfel training step (include/gs_hip.h).
 //
 // Host-side orchestration only: every handle allocates its device buffers
-// once at capacity (crates/gs-wgpu CLAUDE-style "no mid-training
-// reallocation"), validates shapes before any launch, and enqueues the
+// once at capacity (as crates/gs-kernels/src/rasterizer.rs:1-4: no
+// reallocation mid-training), validates shapes before any launch, and enqueues the
 // gfx950 kernels of the sibling .hip files on the caller's stream.
 
 #include <hip/hip_runtime.h>
