@@ -347,12 +347,26 @@ __global__ __launch_bounds__(kFuseThreads, 3) void geom_adam_kernel(
     const float2 s = make_float2(adam_activate(cs.activation, sraw.x), adam_activate(cs.activation, sraw.y));
     const float4 q = make_float4(adam_activate(cq.activation, qraw.x), adam_activate(cq.activation, qraw.y),
                                  adam_activate(cq.activation, qraw.z), adam_activate(cq.activation, qraw.w));
+    // The block's SH rows staged in LDS with lane-consecutive float4 loads:
+    // every view reads a surfel's 48 coefficients (colour -> direction
+    // chain), which from global memory would be 48 scattered loads per view.
+    const uint32_t base = blockIdx.x * kFuseThreads * kShVec;
+    const uint32_t total = n * kShVec;
+#pragma unroll 4
+    for (int r = 0; r < kShVec; ++r) {
+        const uint32_t e = (uint32_t)r * kFuseThreads + threadIdx.x;
+        if (base + e >= total) break;
+        const uint32_t j = e / kShVec, k = e - j * kShVec;
+        s_sh[j * kShRow + k] = reinterpret_cast<const float4*>(ch.raw)[base + e];
+    }
+    __syncthreads();
+    float4* row = s_sh + threadIdx.x * kShRow;
     SurfelAcc a;
     zero_acc(a);
     for (uint32_t v = 0; v < n_views; ++v) {
         float gc[kGradStride];
         const bool any = valid && load_coef_grads<grad_stride(false)>(grad_coef, v, n, i, gc);
-        if (any) add_view(cams[v], gc, p, s, q, ch.raw + (size_t)i * kShFloats, a, false, nullptr);
+        if (any) add_view(cams[v], gc, p, s, q, reinterpret_cast<const float*>(row), a, false, nullptr);
     }
     const float bc1 = state->bc1_inv, bc2 = state->bc2_inv;
     if (valid) {
@@ -386,14 +400,12 @@ __global__ __launch_bounds__(kFuseThreads, 3) void geom_adam_kernel(
             co.activated[i] = adam_activate(co.activation, r);
         }
     }
-    // SH: gradients through LDS, then the block's rows updated as one
+    // SH: gradients through LDS (each thread overwrites only its own row,
+    // which no other thread read), then the block's rows updated as one
     // contiguous range (lane-consecutive float4).
-    float4* row = s_sh + threadIdx.x * kShRow;
 #pragma unroll
     for (int k = 0; k < kShVec; ++k) row[k] = make_float4(a.sh[4 * k], a.sh[4 * k + 1], a.sh[4 * k + 2], a.sh[4 * k + 3]);
     __syncthreads();
-    const uint32_t base = blockIdx.x * kFuseThreads * kShVec;
-    const uint32_t total = n * kShVec;
 #pragma unroll 4
     for (int r = 0; r < kShVec; ++r) {
         const uint32_t e = (uint32_t)r * kFuseThreads + threadIdx.x;
