@@ -28,6 +28,29 @@ namespace gs {
 namespace {
 
 constexpr int kGeomThreads = 256;
+constexpr int kShVec = kShFloats / 4;  // float4 per SH row
+constexpr int kShRow = kShVec + 1;     // LDS row stride in float4 (one vector of padding)
+
+// Stages the SH rows of the block's surfels [blockIdx.x * T, + T) in LDS
+// (row stride kShRow float4) with lane-consecutive float4 loads; every
+// view's colour -> direction chain then reads a surfel's 48 coefficients
+// from LDS instead of with 48 scattered global loads. Ends with a barrier.
+// (Worth it over several views: for the one-view launches of geom_bwd_kernel
+// the staging cost more than the scattered loads, 0.54 -> 0.65 ms per
+// step at 1M surfels, so that kernel reads SH from global memory.)
+template <int T>
+__device__ __forceinline__ void stage_sh_rows(const float* __restrict__ sh, uint32_t n, float4* s_sh) {
+    const uint32_t base = blockIdx.x * T * kShVec;
+    const uint32_t total = n * kShVec;
+#pragma unroll 4
+    for (int r = 0; r < kShVec; ++r) {
+        const uint32_t e = (uint32_t)r * T + threadIdx.x;
+        if (base + e >= total) break;
+        const uint32_t j = e / kShVec, k = e - j * kShVec;
+        s_sh[j * kShRow + k] = reinterpret_cast<const float4*>(sh)[base + e];
+    }
+    __syncthreads();
+}
 
 struct V3f {
     float x, y, z;
@@ -293,8 +316,6 @@ __global__ __launch_bounds__(kGeomThreads) void geom_bwd_kernel(
 }
 
 constexpr int kFuseThreads = 128;
-constexpr int kShVec = kShFloats / 4;  // float4 per SH row
-constexpr int kShRow = kShVec + 1;     // LDS row stride in float4 (one vector of padding)
 
 // The optimizer state and parameters written here are next read a whole
 // step later: non-temporal stores keep them from displacing the step's
@@ -347,19 +368,9 @@ __global__ __launch_bounds__(kFuseThreads, 3) void geom_adam_kernel(
     const float2 s = make_float2(adam_activate(cs.activation, sraw.x), adam_activate(cs.activation, sraw.y));
     const float4 q = make_float4(adam_activate(cq.activation, qraw.x), adam_activate(cq.activation, qraw.y),
                                  adam_activate(cq.activation, qraw.z), adam_activate(cq.activation, qraw.w));
-    // The block's SH rows staged in LDS with lane-consecutive float4 loads:
-    // every view reads a surfel's 48 coefficients (colour -> direction
-    // chain), which from global memory would be 48 scattered loads per view.
+    stage_sh_rows<kFuseThreads>(ch.raw, n, s_sh);
     const uint32_t base = blockIdx.x * kFuseThreads * kShVec;
     const uint32_t total = n * kShVec;
-#pragma unroll 4
-    for (int r = 0; r < kShVec; ++r) {
-        const uint32_t e = (uint32_t)r * kFuseThreads + threadIdx.x;
-        if (base + e >= total) break;
-        const uint32_t j = e / kShVec, k = e - j * kShVec;
-        s_sh[j * kShRow + k] = reinterpret_cast<const float4*>(ch.raw)[base + e];
-    }
-    __syncthreads();
     float4* row = s_sh + threadIdx.x * kShRow;
     SurfelAcc a;
     zero_acc(a);
