@@ -245,6 +245,7 @@ __global__ __launch_bounds__(TW == 8 ? 64 : TW * TW / 2) void raster_bwd_kernel(
     static_assert(BATCH % 64 == 0 && BATCH <= (1 << MB), "batch of whole waves");
     __shared__ float4 s_m[BATCH], s_a[BATCH], s_b[BATCH], s_c[BATCH], s_n[AUX ? BATCH : 1];
     __shared__ float s_lthr[BATCH];
+    __shared__ uint32_t s_pay[BATCH];        // the batch's surfel ids, for the flush
     __shared__ uint16_t s_list[NW][BATCH];
     __shared__ uint32_t s_qcnt[SW][NW];      // [staging wave][list wave]
     __shared__ float s_grad[BATCH * NS];
@@ -305,7 +306,9 @@ __global__ __launch_bounds__(TW == 8 ? 64 : TW * TW / 2) void raster_bwd_kernel(
                 if (slot >= BATCH) continue;  // whole waves (NT, BATCH multiples of 64)
                 uint32_t qmask = 0;
                 if ((uint32_t)slot < cnt) {
-                    Record r = recs[payloads[base + slot]];
+                    const uint32_t pay = payloads[base + slot];
+                    Record r = recs[pay];
+                    s_pay[slot] = pay;
                     s_m[slot] = r.m_nt;
                     s_a[slot] = r.a_px;
                     s_b[slot] = r.b_py;
@@ -419,7 +422,7 @@ __global__ __launch_bounds__(TW == 8 ? 64 : TW * TW / 2) void raster_bwd_kernel(
                         const float cc = slot == kGCol ? c.x : (slot == kGCol + 1 ? c.y : c.z);
                         if (cc == 0.0f) continue;
                     }
-                    atomicAdd(&grad_coef[(size_t)payloads[base + e] * NS + slot], v);
+                    atomicAdd(&grad_coef[(size_t)s_pay[e] * NS + slot], v);
                 }
             }
             __syncthreads();
