@@ -45,3 +45,13 @@ def test_header_is_plain_c():
     text = re.sub(r"/\*.*?\*/", "", open(L.HEADER_PATH).read(), flags=re.S)
     for banned in ("torch", "hip_runtime", "std::", "class ", "template"):
         assert banned not in text
+
+
+def test_rust_binding_declares_every_entry_point():
+    """The `-sys` crate a maintainer adds beside gs-wgpu declares one
+    `pub fn` per function of the header, so the documented binding and the
+    library cannot drift apart."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    text = open(os.path.join(root, "INTEGRATION.md")).read()
+    missing = [s for s in sa.header_symbols() if f"pub fn {s}(" not in text]
+    assert not missing, missing
