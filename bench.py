@@ -15,15 +15,22 @@ step, targets rendered from a jittered copy.
 A "step" = project, scan, emit keys, 64-bit radix sort, tile ranges,
 rasterize forward, L1 + D-SSIM, rasterize backward, geometry backward and
 fused Adam over the 4 views. `value` is steps/s of the batched step captured
-in a hipGraph; the plain per-view step (every kernel launched per view, no
-graph) is timed beside it. Inputs are resident in HBM before timing starts.
+in a hipGraph: 1000 / the median per-step time over steps 10..500 of a
+500-step training run from the initial state (a HIP event on the step stream
+before every step). The plain per-view step (every kernel launched per
+view, no graph) runs the same 500 steps from the same state and is timed the
+same way; both runs end with the PSNR over all views (equal loss). Inputs
+are resident in HBM before timing starts.
 
 Roofline: the dominant stage's algorithmic bytes (per-stage formulas of the
 design notes) divided by its duration measured with HIP events on the step
 stream (profiled batched steps after the timed region).
-CPU baseline: the FP64 oracle (oracle/, OpenMP tile-binned mode) on a
-bounded sample -- one view's forward + loss + backward over a centred crop,
-scaled to a full step -- timed on this host.
+CPU baseline: the FP64 oracle (oracle/, OpenMP tile-binned mode, compiled
+for this host with -O3 -march=native) on a bounded sample -- one view's
+forward + loss + backward over a centred crop, scaled to a full step, median
+of 3 -- timed on this host, plus the literal single-threaded walk on the
+10k-surfel 256x256 scene. The same crop's FP64 image and gradients give the
+GPU's error against FP64 at the benchmark size (fp64_crop_error).
 """
 
 from __future__ import annotations
@@ -46,9 +53,12 @@ HBM_PEAK_GBS = 8000.0
 # SIMD every 4 cycles, 2.4 GHz (packed FP32 is not used).
 VALU_PEAK_WINST = 256 * 2.4e9
 COUNTERS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "c3_step_counters.json")
+# The main kernel of each stage (its largest launch in the batched step).
 STAGE_KERNEL = {"raster_bwd": "raster_bwd_kernel", "raster_fwd": "raster_fwd_kernel",
-                "sort": "onesweep_kernel", "project": "project_kernel", "l1_ssim": "ssim_fwd_kernel",
-                "adam": "adam_kernel", "geom_bwd": "geom_bwd_kernel"}
+                "sort": "downsweep_kernel", "project": "project_kernel", "l1_ssim": "ssim_bwd_kernel",
+                "adam": "adam_kernel", "geom_bwd": "geom_adam_kernel", "emit_keys": "emit_tile_keys_kernel",
+                "scan": "scan_down_kernel", "tile_ranges": "tile_ranges_u32_kernel",
+                "grad_clear": "fill_u32_kernel"}
 
 
 def profile_figures(stage):
@@ -86,13 +96,12 @@ def parse():
                    help="c3: 1M surfels, 4 x 1080p views per step (default); c4: 16-frame 720p "
                         "streaming batch over a growing 500k..1.25M-surfel scene")
     p.add_argument("--steps", type=int, default=20)
-    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=10)
     p.add_argument("--n", type=int, default=1_000_000, help="surfels")
     p.add_argument("--width", type=int, default=1920)
     p.add_argument("--height", type=int, default=1080)
     p.add_argument("--views-per-step", type=int, default=4)
     p.add_argument("--num-views", type=int, default=16)
-    p.add_argument("--plain-steps", type=int, default=None)
     p.add_argument("--profile-steps", type=int, default=3)
     p.add_argument("--digit-bits", type=int, default=8)
     p.add_argument("--binning", choices=["tight", "presort", "sort64"], default="tight",
@@ -102,7 +111,8 @@ def parse():
     p.add_argument("--no-autotune", action="store_true",
                    help="keep --digit-bits and the default tile queue instead of autotuning")
     p.add_argument("--psnr-steps", type=int, default=500,
-                   help="steps of the batched-vs-plain equal-loss comparison (0 = skip)")
+                   help="steps of the headline run and of the plain run it is compared with at equal "
+                        "loss (value = median step over steps 10..psnr-steps)")
     p.add_argument("--schedule", default=None,
                    help="DIGIT_BITS,BLOCKS_PER_CU[,TILE,FWD_BATCH,BWD_BATCH]: set this schedule "
                         "instead of autotuning")
@@ -145,50 +155,177 @@ def stage_bytes(n, V, E, P, T, sort_passes, S=232, nv=None):
     }
 
 
-def cpu_baseline(scene, spec, crop, views_per_step, n_view=None):
-    """n_view: surfels the sampled view renders (a prefix; all by default)."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_scene(scene, n):
     import oracle
-    n = scene.n if n_view is None else n_view
     deg = scene.sh_degree
     ncoef = (deg + 1) ** 2
-    s64 = oracle.Scene(scene.positions[:n].astype(np.float64), scene.scales[:n].astype(np.float64),
-                       scene.quats[:n].astype(np.float64), scene.opacities[:n].astype(np.float64),
-                       scene.sh[:n].astype(np.float64).reshape(n, ncoef, 3), deg)
+    return oracle.Scene(scene.positions[:n].astype(np.float64), scene.scales[:n].astype(np.float64),
+                        scene.quats[:n].astype(np.float64), scene.opacities[:n].astype(np.float64),
+                        scene.sh[:n].astype(np.float64).reshape(n, ncoef, 3), deg)
+
+
+def cpu_baseline(scene, spec, crop, views_per_step, n_view=None, repeats=3):
+    """The FP64 oracle (OpenMP tile-binned mode) on a bounded sample: one
+    view's fixed per-view cost plus a centred crop's forward + L1/SSIM +
+    backward, scaled to the full view and to the step's views, median of
+    `repeats`; plus the reference's own shape -- the literal single-threaded
+    walk -- on the 10k-surfel 256x256 scene. Returns the baseline and the
+    crop's FP64 image, loss gradient and parameter gradients (the FP64 side
+    of the crop parity check)."""
+    import oracle
+    from splattar_amd import scenes
+    n = scene.n if n_view is None else n_view
+    s64 = _oracle_scene(scene, n)
     W, H = spec.width, spec.height
     cw, ch = crop
     x0, y0 = (W - cw) // 2, (H - ch) // 2
 
     def cam(c):
-        return oracle.Camera(np.asarray(spec.center, np.float64), np.asarray(spec.quat, np.float64),
-                             float(spec.focal), W, H, c)
+        return oracle.Camera(np.asarray(spec.center, np.float32).astype(np.float64),
+                             np.asarray(spec.quat, np.float32).astype(np.float64),
+                             float(np.float32(spec.focal)), W, H, c)
 
-    # Fixed per-view cost (prepare: projection + SH + depth sort; tile lists).
-    t0 = time.perf_counter()
-    oracle.render(s64, cam((x0, y0, 1, 1)))
-    oracle.gradients(s64, cam((x0, y0, 1, 1)), np.zeros((1, 3)))
-    t_fixed = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    img = oracle.render(s64, cam((x0, y0, cw, ch)))["color"]
-    tgt = np.clip(img + 0.01, 0, 1)
-    _, dl, _, _ = oracle.l1_ssim(img, tgt, 0.2, 1.0)
-    oracle.gradients(s64, cam((x0, y0, cw, ch)), dl.reshape(-1, 3))
-    t_crop = time.perf_counter() - t0
+    fixed, crop_t = [], []
+    out = None
+    for _ in range(repeats):
+        # Fixed per-view cost (prepare: projection + SH + depth sort; tile lists).
+        t0 = time.perf_counter()
+        oracle.render(s64, cam((x0, y0, 1, 1)))
+        oracle.gradients(s64, cam((x0, y0, 1, 1)), np.zeros((1, 3)))
+        fixed.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        img = oracle.render(s64, cam((x0, y0, cw, ch)))["color"]
+        tgt = np.clip(img + 0.01, 0, 1)
+        _, dl, _, _ = oracle.l1_ssim(img, tgt, 0.2, 1.0)
+        g = oracle.gradients(s64, cam((x0, y0, cw, ch)), dl.reshape(-1, 3))
+        crop_t.append(time.perf_counter() - t0)
+        out = (img, dl, g)
+    t_fixed, t_crop = float(np.median(fixed)), float(np.median(crop_t))
     raw = np.zeros(scene.n * 58, np.float32)
-    g = np.full(scene.n * 58, 1e-3, np.float32)
+    gr = np.full(scene.n * 58, 1e-3, np.float32)
     m = np.zeros_like(raw)
     v = np.zeros_like(raw)
     t0 = time.perf_counter()
-    oracle.adam_step_f32(raw, g, m, v, 1e-3, 0, 1)
+    oracle.adam_step_f32(raw, gr, m, v, 1e-3, 0, 1)
     t_adam = time.perf_counter() - t0
     scale = (W * H) / float(cw * ch)
     t_view = t_fixed + max(t_crop - t_fixed, 0.0) * scale
     t_step = views_per_step * t_view + t_adam
-    return {"value": 1.0 / t_step, "unit": "steps/s", "cores": oracle.num_threads(), "kind": "port",
-            "sample": (f"FP64 oracle (tile-binned, OpenMP) on view 0: fixed per-view cost {t_fixed:.2f}s + "
-                       f"centred {cw}x{ch} crop fwd+L1/SSIM+bwd {t_crop:.2f}s scaled x{scale:.1f} to "
-                       f"{W}x{H}; x{views_per_step} views + f32 Adam over {scene.n}x58 params "
-                       f"{t_adam:.2f}s = {t_step:.1f}s/step"),
-            "sample_seconds": round(t_fixed + t_crop + t_adam, 2)}
+    # The reference's shape: literal O(pixels x surfels) walk, one thread,
+    # on the 10k-surfel SH0 256x256 configuration (forward + L1/SSIM +
+    # backward of one view), median of `repeats`.
+    c0 = scenes.config_scene(0)
+    sp0 = scenes.front_view(256, 256)
+    c0_64 = _oracle_scene(c0, c0.n)
+    cam0 = oracle.Camera(np.asarray(sp0.center, np.float32).astype(np.float64),
+                         np.asarray(sp0.quat, np.float32).astype(np.float64), float(np.float32(sp0.focal)),
+                         sp0.width, sp0.height)
+    lit = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        img0 = oracle.render(c0_64, cam0, mode=0)["color"]
+        _, dl0, _, _ = oracle.l1_ssim(img0, np.clip(img0 + 0.01, 0, 1), 0.2, 1.0)
+        oracle.gradients(c0_64, cam0, dl0.reshape(-1, 3), mode=0)
+        lit.append(time.perf_counter() - t0)
+    base = {"value": 1.0 / t_step, "unit": "steps/s", "cores": oracle.num_threads(), "kind": "port",
+            "cpu_model": cpu_model(), "build": oracle.build_info(),
+            "sample": (f"FP64 oracle (tile-binned, OpenMP, {oracle.num_threads()} threads) on view 0, "
+                       f"median of {repeats}: fixed per-view cost {t_fixed:.2f}s + centred {cw}x{ch} crop "
+                       f"fwd+L1/SSIM+bwd {t_crop:.2f}s scaled x{scale:.1f} to {W}x{H}; x{views_per_step} "
+                       f"views + f32 Adam over {scene.n}x58 params {t_adam:.2f}s = {t_step:.1f}s/step"),
+            "sample_seconds": round(sum(fixed) + sum(crop_t) + t_adam, 2),
+            "literal_c0": {"seconds_per_view": round(float(np.median(lit)), 3), "threads": 1,
+                           "sample": (f"literal single-threaded walk (the reference's O(P x N) shape), "
+                                      f"{c0.n} surfels SH0 {sp0.width}x{sp0.height}: fwd + L1/SSIM + bwd, "
+                                      f"median of {repeats}")}}
+    return base, (x0, y0, cw, ch), out
+
+
+def fp64_crop_error(ctx, scene, spec, window, fp64, n_view=None):
+    """GPU image and gradients of view 0 against the FP64 oracle on the crop
+    the CPU baseline rendered: the GPU forward of the whole view, dL/dimage
+    = the oracle's crop loss gradient inside the window and 0 outside, the
+    GPU backward, per-class normwise error against the oracle's crop
+    gradients (no decision masking: discrete f32-vs-FP64 decisions stay in)."""
+    import splattar_amd as sa
+    img64, dl64, g64 = fp64
+    n = scene.n if n_view is None else n_view
+    x0, y0, cw, ch = window
+    W, H = spec.width, spec.height
+    r = sa.Rasterizer(ctx, n, W, H, min(64 * n, (1 << 30) // 2))
+    r.upload_scene(scene.positions[:n], scene.scales[:n], scene.quats[:n], scene.opacities[:n], scene.sh[:n])
+    r.forward(sa.Camera(spec.center, spec.quat, spec.focal, scene.sh_degree), n)
+    img = r.out_color.read()[y0:y0 + ch, x0:x0 + cw, :3].astype(np.float64)
+    dl = np.zeros((H, W, 4), np.float32)
+    dl[y0:y0 + ch, x0:x0 + cw, :3] = dl64
+    r.dl_dcolor.write(dl)
+    r.backward(n, flags=sa._lib.GS_BWD_COLOR_ONLY)
+    got = {"pos": r.grad_pos.read()[:n, :3], "scales": r.grad_scales.read()[:n],
+           "quat": r.grad_quat.read()[:n], "opacity": r.grad_opacity.read()[:n],
+           "sh": r.grad_sh.read()[:n, :g64.sh.shape[1] * 3]}
+    r.close()
+    err = np.abs(img - img64)
+    out = {"window": [x0, y0, cw, ch], "image_max_abs": float(err.max()),
+           "image_mean_abs": float(err.mean()), "pixels_above_1e-5": int((err.max(axis=2) > 1e-5).sum())}
+    for k, want in (("pos", g64.pos), ("scales", g64.scales), ("quat", g64.quat),
+                    ("opacity", g64.opacity), ("sh", g64.sh.reshape(n, -1))):
+        den = np.linalg.norm(want)
+        out[f"grad_{k}_normwise"] = float(np.linalg.norm(got[k].astype(np.float64) - want) / den) if den else 0.0
+    return out
+
+
+class HipEvents:
+    """HIP events recorded on a given stream (the trainer's), for per-step
+    GPU timing; torch.cuda.Event would only see torch's current stream."""
+
+    def __init__(self, count):
+        import ctypes as C
+        self.C = C
+        self.L = C.CDLL("libamdhip64.so")
+        self.L.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.L.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.L.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.L.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.L.hipEventDestroy.argtypes = [C.c_void_p]
+        self.ev = []
+        for _ in range(count):
+            e = C.c_void_p()
+            if self.L.hipEventCreate(C.byref(e)) != 0:
+                raise RuntimeError("hipEventCreate failed")
+            self.ev.append(e.value)
+
+    def record(self, i, stream):
+        if self.L.hipEventRecord(self.ev[i], stream) != 0:
+            raise RuntimeError("hipEventRecord failed")
+
+    def intervals(self, count):
+        """ms between consecutive events 0..count."""
+        C = self.C
+        if self.L.hipEventSynchronize(self.ev[count]) != 0:
+            raise RuntimeError("hipEventSynchronize failed")
+        out = []
+        for i in range(count):
+            ms = C.c_float()
+            if self.L.hipEventElapsedTime(C.byref(ms), self.ev[i], self.ev[i + 1]) != 0:
+                raise RuntimeError("hipEventElapsedTime failed")
+            out.append(ms.value)
+        return np.array(out)
+
+    def close(self):
+        for e in self.ev:
+            self.L.hipEventDestroy(e)
+        self.ev = []
 
 
 def main():
@@ -248,100 +385,111 @@ def main():
                                   "sort64": GS_BINNING_SORT64}[a.binning])
     views = [(sa.Camera(sp.center, sp.quat, sp.focal, deg), t, m) for sp, t, m in zip(specs, targets, counts)]
     del targets
+    stream = ctx.stream
 
     def ids_for(step):
         return [(step * V + j) % a.num_views for j in range(V)]
 
-    def run(trainer, mode, steps, start):
+    def run(trainer, mode, steps, start, ev=None, ev_base=0):
         for s in range(steps):
+            if ev is not None:
+                ev.record(ev_base + s, stream)
             trainer.step(ids_for(start + s), mode=mode)
+        if ev is not None:
+            ev.record(ev_base + steps, stream)
 
-    # ---- batched + graph (the headline)
+    def psnr(trainer):
+        ps = []
+        for v, (_, tgt, _) in enumerate(views):
+            img = trainer.render(v)[..., :3].astype(np.float64)
+            mse = float(np.mean((img - tgt[..., :3].astype(np.float64)) ** 2))
+            ps.append(10.0 * np.log10(1.0 / max(mse, 1e-20)))
+        return float(np.mean(ps))
+
+    # ---- schedule: autotuned on real batched steps (the training state is
+    # saved before and restored after, so it does not advance the run).
     tr = sa.Trainer(ctx, cfg, scene.as_init(), views)
     schedule = {"digit_bits": a.digit_bits, "blocks_per_cu": None, "tile_size": 16, "fwd_batch": 256,
                 "bwd_batch": 128}
     if a.schedule:
         f = [int(x) for x in a.schedule.split(",")]
-        tr.set_schedule(f[0], f[1])
         schedule.update(digit_bits=f[0], blocks_per_cu=f[1])
         if len(f) == 5:
-            tr.set_tiling(f[2], f[3], f[4])
             schedule.update(tile_size=f[2], fwd_batch=f[3], bwd_batch=f[4])
     elif not a.no_autotune:
-        # Untimed: real training steps under each candidate schedule.
         t0 = time.perf_counter()
         schedule = tr.autotune(ids_for(0), reps=4)
         log(f"autotune picked {schedule} in {time.perf_counter() - t0:.1f}s")
-    run(tr, tr.GRAPH, a.warmup, 0)
+
+    def apply_schedule(t):
+        if schedule["blocks_per_cu"] is not None:
+            t.set_schedule(schedule["digit_bits"], schedule["blocks_per_cu"])
+        t.set_tiling(schedule["tile_size"], schedule["fwd_batch"], schedule["bwd_batch"])
+
+    apply_schedule(tr)
+    # ---- the headline run: batched + hipGraph from the initial state, one
+    # HIP event per step on the step stream. The driver's K timed steps
+    # (bracketed by synchronisations, wall clock -> ms_per_step) follow W
+    # warm-up steps; the run then continues to `psnr_steps` steps, and
+    # `value` is 1000 / the median per-step ms over steps 10..psnr_steps.
+    total = max(a.psnr_steps, a.warmup + a.steps)
+    ev = HipEvents(total + 1)
+    run(tr, tr.GRAPH, a.warmup, 0, ev, 0)
     ctx.synchronize()
     t0 = time.perf_counter()
-    run(tr, tr.GRAPH, a.steps, a.warmup)
+    run(tr, tr.GRAPH, a.steps, a.warmup, ev, a.warmup)
     ctx.synchronize()
     t_graph = time.perf_counter() - t0
     graph_losses = tr.last_losses(V)
     entries_step = tr.entry_count()
-    log(f"graph: {a.steps} steps in {t_graph:.3f}s -> {a.steps / t_graph:.2f} steps/s; "
-        f"entries/step {entries_step}")
+    rest = total - a.warmup - a.steps
+    run(tr, tr.GRAPH, rest, a.warmup + a.steps, ev, a.warmup + a.steps)
+    ctx.synchronize()
+    graph_ms = ev.intervals(total)
+    log(f"graph: {a.steps} timed steps in {t_graph:.3f}s -> {a.steps / t_graph:.2f} steps/s; "
+        f"entries/step {entries_step}; per-step ms over {total} steps: "
+        f"median(10..) {np.median(graph_ms[10:]):.3f}, steps 0-9 {np.round(graph_ms[:10], 3).tolist()}, "
+        f"by 100-step block {[round(float(np.median(graph_ms[b:b + 100])), 3) for b in range(0, total, 100)]}")
+    equal_loss = {"steps": total, "psnr_batched_graph_db": round(psnr(tr), 4)}
+    entries_late = tr.entry_count()
+    tr.close()
 
-    # ---- batched without graph, then profiled batched steps (stage times)
-    run(tr, tr.BATCHED, 1, 0)
+    # ---- plain per-view steps from the same initial state: the same views,
+    # the same math, every kernel launched per view, no graph.
+    tp = sa.Trainer(ctx, cfg, scene.as_init(), views)
+    apply_schedule(tp)
+    run(tp, tp.PLAIN, total, 0, ev, 0)
     ctx.synchronize()
-    kb = max(2, a.steps // 4)
-    t0 = time.perf_counter()
-    run(tr, tr.BATCHED, kb, 1)
+    plain_ms = ev.intervals(total)
+    equal_loss["psnr_plain_db"] = round(psnr(tp), 4)
+    equal_loss["psnr_diff_db"] = round(equal_loss["psnr_batched_graph_db"] - equal_loss["psnr_plain_db"], 4)
+    tp.set_profiling(True)
+    run(tp, tp.PLAIN, 1, total)
+    plain_stages, _ = tp.stage_times()
+    tp.close()
+    log(f"plain: per-step ms median(10..) {np.median(plain_ms[10:]):.3f}; equal loss {equal_loss}")
+
+    # ---- batched without a graph, then profiled batched steps (stage times)
+    tb = sa.Trainer(ctx, cfg, scene.as_init(), views)
+    apply_schedule(tb)
+    nb = min(total, 60)
+    run(tb, tb.BATCHED, nb, 0, ev, 0)
     ctx.synchronize()
-    t_batched = (time.perf_counter() - t0) / kb
-    tr.set_profiling(True)
-    run(tr, tr.BATCHED, a.profile_steps, 3)
-    stages, psteps = tr.stage_times()
-    tr.set_profiling(False)
+    batched_ms = ev.intervals(nb)
+    tb.set_profiling(True)
+    run(tb, tb.BATCHED, a.profile_steps, nb)
+    stages, psteps = tb.stage_times()
+    tb.set_profiling(False)
+    tb.close()
+    ev.close()
     stage_ms = {k: v / max(psteps, 1) for k, v in stages.items()}
     log("stage ms/step (batched): " + ", ".join(f"{k} {v:.3f}" for k, v in stage_ms.items()))
 
-    # ---- plain per-view steps (same views, same math, no graph)
-    kp = a.plain_steps if a.plain_steps is not None else max(2, a.steps // 4)
-    run(tr, tr.PLAIN, 1, 0)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    run(tr, tr.PLAIN, kp, 1)
-    ctx.synchronize()
-    t_plain = (time.perf_counter() - t0) / kp
-    tr.set_profiling(True)
-    run(tr, tr.PLAIN, 1, 2)
-    plain_stages, _ = tr.stage_times()
-    tr.set_profiling(False)
-    tr.close()
-
-    # ---- equal-loss check: the same K steps batched+graph and plain, from
-    # the same initial state, then mean PSNR over every view (untimed).
-    equal_loss = None
-    if a.psnr_steps > 0:
-        def train_and_score(mode):
-            t = sa.Trainer(ctx, cfg, scene.as_init(), views)
-            if schedule["blocks_per_cu"] is not None:
-                t.set_schedule(schedule["digit_bits"], schedule["blocks_per_cu"])
-            t.set_tiling(schedule["tile_size"], schedule["fwd_batch"], schedule["bwd_batch"])
-            t0 = time.perf_counter()
-            run(t, mode, a.psnr_steps, 0)
-            ctx.synchronize()
-            secs = time.perf_counter() - t0
-            ps = []
-            for v, (_, tgt, _) in enumerate(views):
-                img = t.render(v)[..., :3].astype(np.float64)
-                mse = float(np.mean((img - tgt[..., :3].astype(np.float64)) ** 2))
-                ps.append(10.0 * np.log10(1.0 / max(mse, 1e-20)))
-            t.close()
-            return float(np.mean(ps)), a.psnr_steps / secs
-        pb, rb = train_and_score(sa.Trainer.GRAPH)
-        pp, rp = train_and_score(sa.Trainer.PLAIN)
-        equal_loss = {"steps": a.psnr_steps, "psnr_batched_graph_db": round(pb, 4),
-                      "psnr_plain_db": round(pp, 4), "psnr_diff_db": round(pb - pp, 4),
-                      "steps_per_s_batched_graph": round(rb, 2), "steps_per_s_plain": round(rp, 2),
-                      "ratio": round(rb / rp, 3)}
-        log(f"equal-loss check: {equal_loss}")
-
-    ms_graph = 1000.0 * t_graph / a.steps
-    value = a.steps / t_graph
+    med_graph = float(np.median(graph_ms[10:]))
+    med_plain = float(np.median(plain_ms[10:]))
+    med_batched = float(np.median(batched_ms[10:]))
+    value = 1000.0 / med_graph
+    ms_burst = 1000.0 * t_graph / a.steps
     # Roofline of the dominant kernel.
     P = V * W * H
     ts = schedule["tile_size"]
@@ -362,26 +510,35 @@ def main():
         roofline["traffic"] = prof.pop("traffic", None)
         roofline["profile"] = prof
     total_bytes = sum(sb.values())
+    equal_loss.update(steps_per_s_batched_graph=round(value, 2), steps_per_s_plain=round(1000.0 / med_plain, 2),
+                      ratio=round(med_plain / med_graph, 3))
     result = {
-        "metric": METRIC if a.workload == "c3" else METRIC_C4, "value": round(value, 4), "unit": "steps/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": round(ms_graph, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "metric": METRIC if a.workload == "c3" else METRIC_C4, "value": round(value, 4), "unit": "steps/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_burst, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "value_definition": (f"1000 / median per-step ms over steps 10..{total} of the batched + hipGraph "
+                             f"run from the initial state (HIP events on the step stream); ms_per_step is "
+                             f"the wall clock of the {a.steps} synchronised timed steps after {a.warmup} "
+                             f"warm-up steps of the same run"),
         "config": {"workload": (f"C3: {n} surfels SH3 heavy-tailed scales, {V}x{W}x{H} views/step, "
                                 f"L1+SSIM, Adam") if a.workload == "c3" else
                                (f"C4: {V} frames {W}x{H}, SE(3) walk 1 cm / 0.5 deg per frame, "
                                 f"{counts[0]}..{counts[-1]} surfels per frame (+10 %/frame), one graph "
                                 f"launch per step, L1+SSIM, Adam"),
                    "surfels": n, "views_per_step": V, "width": W,
-                   "height": H, "entries_per_step": entries_step, "digit_bits": schedule["digit_bits"],
+                   "height": H, "entries_per_step": entries_step, "entries_per_step_at_end": entries_late,
+                   "digit_bits": schedule["digit_bits"],
                    "raster_blocks_per_cu": schedule["blocks_per_cu"], "tile_size": schedule["tile_size"],
                    "fwd_batch": schedule["fwd_batch"], "bwd_batch": schedule["bwd_batch"],
-                   "binning": a.binning,
-                   "parallelism": "single-gpu"},
-        "plain_steps_per_s": round(1.0 / t_plain, 4),
-        "batched_nograph_steps_per_s": round(1.0 / t_batched, 4),
-        "batched_vs_plain": round(value * t_plain, 3),
+                   "binning": a.binning, "parallelism": "single-gpu"},
+        "burst_steps_per_s": round(a.steps / t_graph, 4),
+        "plain_steps_per_s": round(1000.0 / med_plain, 4),
+        "batched_nograph_steps_per_s": round(1000.0 / med_batched, 4),
+        "batched_vs_plain": round(med_plain / med_graph, 3),
+        "step_ms_quartiles": {"graph": np.round(np.percentile(graph_ms[10:], [25, 50, 75]), 4).tolist(),
+                              "plain": np.round(np.percentile(plain_ms[10:], [25, 50, 75]), 4).tolist()},
         "equal_loss": equal_loss,
-        "step_hbm_frac": round(total_bytes / (ms_graph * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+        "step_hbm_frac": round(total_bytes / (med_graph * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
         "mpix_per_s": round(P * value / 1e6, 2),
         "sorted_keys_per_s": round(entries_step / (stage_ms["sort"] * 1e-3), 1),
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
@@ -390,13 +547,35 @@ def main():
         "roofline": roofline,
     }
     if not a.no_cpu_baseline:
+        import oracle
+        native = native_oracle()
+        if native:
+            oracle.use_library(native)
         cw, chh = (int(x) for x in a.cpu_crop.split("x"))
-        log("timing CPU oracle baseline ...")
-        result["cpu_baseline"] = cpu_baseline(scene, specs[0], (cw, chh), V, counts[0])
+        log(f"timing CPU oracle baseline ({native or 'prebuilt x86-64-v3 oracle'}) ...")
+        base, window, fp64 = cpu_baseline(scene, specs[0], (cw, chh), V, counts[0])
+        result["cpu_baseline"] = base
+        log("FP64 crop parity of the GPU image and gradients ...")
+        result["fp64_crop_error"] = fp64_crop_error(ctx, scene, specs[0], window, fp64, counts[0])
     else:
         result["cpu_baseline"] = None
     ctx.close()
     print(json.dumps(result), flush=True)
+
+
+def native_oracle():
+    """Compiles the oracle for this host's CPU (-O3 -march=native) into a
+    temporary directory; None if that fails (the prebuilt one is used)."""
+    import subprocess
+    import tempfile
+    out = os.path.join(tempfile.mkdtemp(prefix="oracle_native_"), "liboracle_native.so")
+    try:
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native", f"NATIVE_OUT={out}"],
+                       check=True, timeout=120, capture_output=True)
+    except (OSError, subprocess.SubprocessError) as e:
+        log(f"native oracle build failed ({e}); using the prebuilt one")
+        return None
+    return out
 
 
 if __name__ == "__main__":

@@ -287,8 +287,22 @@ void gs_optim_destroy(gs_optim o);
 gs_status gs_optim_add_class(gs_optim o, const char* name, uint32_t n, int activation, float lr,
                              const float* grads, float* activated);
 float* gs_optim_class_raw(gs_optim o, const char* name);  /* class(name).raw (optim.rs:41) */
+/* Optimizer::set_lr / set_reg (optim.rs:171-177): take effect at the next
+ * encode, whose stream carries the upload into device memory (the rates,
+ * schedules and regularisers live there, so a captured step graph replays
+ * whatever was set before its launch; changing them while the encoding
+ * stream is being captured is GS_ERR_INVALID). set_lr clears a schedule. */
 gs_status gs_optim_set_lr(gs_optim o, const char* name, float lr);    /* optim.rs:171 */
 gs_status gs_optim_set_reg(gs_optim o, const char* name, float reg);  /* optim.rs:175 */
+/* The exponential decay Trainer::step applies to the position rate
+ * (trainer.rs:1149-1151, schedule_t trainer.rs:407-416): the step with
+ * 0-based index iter (the optimizer's step count before it) uses
+ * lr = base * final_factor^(iter / iters), computed on the device by the
+ * step itself. The reference sets it from the host every iteration. */
+gs_status gs_optim_set_lr_schedule(gs_optim o, const char* name, float base, float final_factor,
+                                   uint32_t iters);
+/* Blocking: the learning rate the last step used for the class. */
+gs_status gs_optim_current_lr(gs_optim o, const char* name, float* lr);
 /* Optimizer::encode_step (optim.rs:200-227): one Adam step on every class,
  * then the activated copies. One fused launch. */
 gs_status gs_optim_encode_step(gs_optim o, void* stream);
@@ -306,6 +320,16 @@ gs_status gs_optim_class_info(gs_optim o, uint32_t index, const char** name, uin
  * reference's trainer does (trainer.rs:1891-1893). */
 gs_status gs_optim_encode_snapshot(gs_optim o, void* stream, float* const* bufs, uint32_t count);
 gs_status gs_optim_encode_restore(gs_optim o, void* stream, const float* const* bufs, uint32_t count);
+/* Exact resume (extends optim.rs:148-165, which snapshots raw parameters
+ * only): the whole optimizer state -- class descriptors, step count, bias
+ * products, learning rates, schedules, regularisers and every class's raw,
+ * m and v -- as one host blob of gs_optim_state_bytes bytes. Blocking.
+ * Import checks the class names, sizes and activations, restores the state
+ * bit for bit and re-activates, so training continues exactly where the
+ * export left it. */
+gs_status gs_optim_state_bytes(gs_optim o, uint64_t* bytes);
+gs_status gs_optim_export_state(gs_optim o, void* dst, uint64_t bytes);
+gs_status gs_optim_import_state(gs_optim o, const void* src, uint64_t bytes);
 
 /* ------------------------------------------------------------------ Trainer
  * Trainer::step (first_attempt/crates/gs-train/src/trainer.rs:1139-1332):
@@ -334,6 +358,9 @@ typedef struct gs_train_config {
     uint32_t entries_per_view;  /* binning capacity per view (0 = 64 * n) */
     uint32_t digit_bits;        /* 4 or 8; 0 = 8 until gs_trainer_autotune picks */
     uint32_t binning;           /* GS_BINNING_SORT64 (0), GS_BINNING_PRESORT or GS_BINNING_TIGHT */
+    float pos_lr_final_factor;  /* trainer.rs:477 (0.01); with iters > 0 the position rate */
+    uint32_t iters;             /* decays as lr_pos * factor^(step / iters) (trainer.rs:1149-1151); */
+                                /* iters = 0 keeps it constant */
 } gs_train_config;
 
 #define GS_STEP_PLAIN 0u
@@ -395,6 +422,18 @@ gs_status gs_trainer_set_overlap(gs_trainer t, int enable);
  * writes them to the grad_* buffers of gs_trainer_buffers, as plain steps
  * always do (callers that inspect or check gradients). Default 0. */
 gs_status gs_trainer_set_keep_grads(gs_trainer t, int enable);
+/* The trainer's optimizer (Trainer::optim, trainer.rs): its five classes
+ * "pos", "scales", "quat", "opacity", "sh" in that order. Use it to change
+ * rates and schedules (they take effect at the next step, graph replays
+ * included) or read the current rate. Owned by the trainer; a step
+ * enqueued on a caller's stream must be complete before the optimizer's
+ * blocking calls read it (the gs_trainer_*_state calls below wait for it). */
+gs_status gs_trainer_optim(gs_trainer t, gs_optim* out);
+/* Exact resume of a trainer: gs_optim_export_state / import_state of its
+ * optimizer after the last step (on any stream) has finished. */
+gs_status gs_trainer_state_bytes(gs_trainer t, uint64_t* bytes);
+gs_status gs_trainer_export_state(gs_trainer t, void* dst, uint64_t bytes);
+gs_status gs_trainer_import_state(gs_trainer t, const void* src, uint64_t bytes);
 /* Blocking: per-view (l1_mean, dssim_mean) of the last step, 2 per view. */
 gs_status gs_trainer_last_losses(gs_trainer t, double* out, uint32_t n_ids);
 /* Blocking: render view `view` with the current parameters into rgba (W*H*4). */

@@ -54,6 +54,20 @@ class _Cam32(C.Structure):
                 ("tiles_y", C.c_uint32)]
 
 
+def use_library(path: str):
+    """Load the oracle from `path` (a build of the same cpuref.cpp, e.g. one
+    compiled for the host's own CPU) instead of oracle/liboracle.so; only
+    before the first call into the oracle."""
+    global _LIB_PATH
+    if _lib is not None:
+        raise RuntimeError("oracle library already loaded")
+    _LIB_PATH = path
+
+
+def build_info() -> str:
+    return lib().oref_build_info().decode()
+
+
 def lib():
     """Load (once) and return the oracle shared library."""
     global _lib
@@ -94,6 +108,7 @@ def lib():
         L.oref_adam_step_f32.argtypes = [C.c_int64, _fp, _fp, _fp, _fp, _fp, C.c_float, C.c_float,
                                          C.c_int, C.c_int]
         L.oref_num_threads.restype = C.c_int
+        L.oref_build_info.restype = C.c_char_p
         L.oref_xorshift32_fill.argtypes = [C.c_uint32, C.c_int64, C.c_uint32, _u32p]
         _lib = L
     return _lib

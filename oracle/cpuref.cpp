@@ -1252,4 +1252,24 @@ void oref_xorshift32_fill(uint32_t seed, int64_t n, uint32_t modulus, uint32_t* 
 
 int oref_num_threads(void) { return threads_available(); }
 
+const char* oref_build_info(void) {
+    return "g++ " __VERSION__
+#ifdef __OPTIMIZE__
+           " optimized"
+#endif
+#ifdef __AVX512F__
+           " avx512f"
+#endif
+#ifdef __AVX2__
+           " avx2"
+#endif
+#ifdef __FMA__
+           " fma"
+#endif
+#ifdef _OPENMP
+           " openmp"
+#endif
+        ;
+}
+
 }  // extern "C"

@@ -168,6 +168,8 @@ void oref_xorshift32_fill(uint32_t seed, int64_t n, uint32_t modulus, uint32_t* 
 
 /* ---- timing helper for the CPU baseline ---- */
 int oref_num_threads(void);
+/* Compiler and target features the library was built with. */
+const char* oref_build_info(void);
 
 #ifdef __cplusplus
 }

@@ -70,7 +70,8 @@ class TrainConfig(C.Structure):
     _fields_ = [("lambda_", C.c_float), ("dssim_scale", C.c_float), ("lr_pos", C.c_float),
                 ("lr_scales", C.c_float), ("lr_quat", C.c_float), ("lr_opacity", C.c_float),
                 ("lr_sh", C.c_float), ("max_views_per_step", C.c_uint32),
-                ("entries_per_view", C.c_uint32), ("digit_bits", C.c_uint32), ("binning", C.c_uint32)]
+                ("entries_per_view", C.c_uint32), ("digit_bits", C.c_uint32), ("binning", C.c_uint32),
+                ("pos_lr_final_factor", C.c_float), ("iters", C.c_uint32)]
 
 
 _vp = C.c_void_p
@@ -138,6 +139,11 @@ _PROTOS = {
     "gs_optim_class_raw": ([_vp, C.c_char_p], _vp),
     "gs_optim_set_lr": ([_vp, C.c_char_p, C.c_float], C.c_int),
     "gs_optim_set_reg": ([_vp, C.c_char_p, C.c_float], C.c_int),
+    "gs_optim_set_lr_schedule": ([_vp, C.c_char_p, C.c_float, C.c_float, _u32], C.c_int),
+    "gs_optim_current_lr": ([_vp, C.c_char_p, C.POINTER(C.c_float)], C.c_int),
+    "gs_optim_state_bytes": ([_vp, C.POINTER(C.c_uint64)], C.c_int),
+    "gs_optim_export_state": ([_vp, _vp, C.c_uint64], C.c_int),
+    "gs_optim_import_state": ([_vp, _vp, C.c_uint64], C.c_int),
     "gs_optim_encode_step": ([_vp, _vp], C.c_int),
     "gs_optim_encode_activate": ([_vp, _vp], C.c_int),
     "gs_optim_step_count": ([_vp, C.POINTER(_u32)], C.c_int),
@@ -154,6 +160,10 @@ _PROTOS = {
     "gs_trainer_set_tiling": ([_vp, _u32, _u32, _u32], C.c_int),
     "gs_trainer_set_overlap": ([_vp, C.c_int], C.c_int),
     "gs_trainer_set_keep_grads": ([_vp, C.c_int], C.c_int),
+    "gs_trainer_optim": ([_vp, C.POINTER(_vp)], C.c_int),
+    "gs_trainer_state_bytes": ([_vp, C.POINTER(C.c_uint64)], C.c_int),
+    "gs_trainer_export_state": ([_vp, _vp, C.c_uint64], C.c_int),
+    "gs_trainer_import_state": ([_vp, _vp, C.c_uint64], C.c_int),
     "gs_trainer_last_losses": ([_vp, C.POINTER(C.c_double), _u32], C.c_int),
     "gs_trainer_render": ([_vp, _u32, _vp], C.c_int),
     "gs_trainer_params": ([_vp, C.POINTER(RasterBuffers)], C.c_int),

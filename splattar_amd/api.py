@@ -375,6 +375,7 @@ class Optimizer:
         h = C.c_void_p()
         check(lib().gs_optim_create(ctx.h, C.byref(h)))
         self.h = h.value
+        self._owned = True
         self.sizes: dict[str, int] = {}
 
     def add_class(self, name: str, n: int, activation: int, lr: float, grads: DeviceBuffer,
@@ -393,6 +394,24 @@ class Optimizer:
 
     def set_reg(self, name: str, reg: float):
         check(lib().gs_optim_set_reg(self.h, name.encode(), reg))
+
+    def set_lr_schedule(self, name: str, base: float, final_factor: float, iters: int):
+        """lr of the step with 0-based index i = base * final_factor**(i / iters)
+        (trainer.rs:1149-1151), evaluated on the device by the step."""
+        check(lib().gs_optim_set_lr_schedule(self.h, name.encode(), base, final_factor, iters))
+
+    def current_lr(self, name: str) -> float:
+        lr = C.c_float()
+        check(lib().gs_optim_current_lr(self.h, name.encode(), C.byref(lr)))
+        return float(lr.value)
+
+    def export_state(self) -> bytes:
+        """The exact optimizer state (raw, m, v, step count, rates) as bytes."""
+        return _export_state(lib().gs_optim_state_bytes, lib().gs_optim_export_state, self.h)
+
+    def import_state(self, blob: bytes):
+        buf = C.create_string_buffer(bytes(blob), len(blob))
+        check(lib().gs_optim_import_state(self.h, buf, len(blob)))
 
     def encode_step(self, stream=None):
         check(lib().gs_optim_encode_step(self.h, stream))
@@ -428,9 +447,17 @@ class Optimizer:
         check(lib().gs_optim_encode_restore(self.h, stream, arr, len(src)))
 
     def close(self):
-        if self.h:
+        if self.h and self._owned:
             lib().gs_optim_destroy(self.h)
-            self.h = None
+        self.h = None
+
+
+def _export_state(size_fn, export_fn, h) -> bytes:
+    n = C.c_uint64()
+    check(size_fn(h, C.byref(n)))
+    buf = C.create_string_buffer(int(n.value))
+    check(export_fn(h, buf, n.value))
+    return buf.raw
 
 
 @dataclass
@@ -447,6 +474,8 @@ class TrainConfig:
     entries_per_view: int = 0
     digit_bits: int = 8
     binning: int = L.GS_BINNING_TIGHT
+    pos_lr_final_factor: float = 0.0   # trainer.rs:477 default 0.01; 0 = no decay
+    iters: int = 0                     # decay horizon (trainer.rs:460); 0 = no decay
 
 
 class Trainer:
@@ -478,7 +507,7 @@ class Trainer:
             tv[k].target_rgba = t.ctypes.data
         c = L.TrainConfig(cfg.lam, cfg.dssim_scale, cfg.lr_pos, cfg.lr_scales, cfg.lr_quat, cfg.lr_opacity,
                           cfg.lr_sh, cfg.max_views_per_step, cfg.entries_per_view, cfg.digit_bits,
-                          cfg.binning)
+                          cfg.binning, cfg.pos_lr_final_factor, cfg.iters)
         h = C.c_void_p()
         check(lib().gs_trainer_create(ctx.h, C.byref(c), C.byref(si), len(views), tv, C.byref(h)))
         self.h = h.value
@@ -504,6 +533,24 @@ class Trainer:
 
     def set_schedule(self, digit_bits: int, blocks_per_cu: int):
         check(lib().gs_trainer_set_schedule(self.h, digit_bits, blocks_per_cu))
+
+    def optim(self) -> "Optimizer":
+        """The trainer's optimizer (classes pos, scales, quat, opacity, sh), not owned."""
+        h = C.c_void_p()
+        check(lib().gs_trainer_optim(self.h, C.byref(h)))
+        o = Optimizer.__new__(Optimizer)
+        o.ctx, o.h, o._owned = self.ctx, h.value, False
+        o.sizes = {"pos": 4 * self.n, "scales": 2 * self.n, "quat": 4 * self.n, "opacity": self.n,
+                   "sh": 48 * self.n}
+        return o
+
+    def export_state(self) -> bytes:
+        """Exact training state after the last step (gs_trainer_export_state)."""
+        return _export_state(lib().gs_trainer_state_bytes, lib().gs_trainer_export_state, self.h)
+
+    def import_state(self, blob: bytes):
+        buf = C.create_string_buffer(bytes(blob), len(blob))
+        check(lib().gs_trainer_import_state(self.h, buf, len(blob)))
 
     def set_tiling(self, tile_size: int, fwd_batch: int, bwd_batch: int):
         check(lib().gs_trainer_set_tiling(self.h, tile_size, fwd_batch, bwd_batch))

@@ -8,8 +8,10 @@
 // The reference launches 2 dispatches per class (10 per step) and re-reads
 // raw for the activation; here one grid covers all classes (block ranges per
 // class) and the activated copy is written from registers in the same pass.
-// The step counter and the running f32 products b1^t, b2^t live in device
-// memory (adam_tick), so a captured graph replays correct bias corrections.
+// The step counter, the running f32 products b1^t, b2^t and each class's
+// learning rate for the step (from its schedule) live in device memory
+// (adam_tick), so a captured graph replays correct bias corrections and a
+// decaying learning rate.
 //
 // HBM traffic: 28 B per parameter float (read raw, g, m, v; write raw, m, v,
 // activated).
@@ -24,13 +26,23 @@ constexpr int kAdamThreads = 256;
 constexpr int kAdamItems = 4;   // floats per thread (one float4)
 
 __global__ void adam_tick_kernel(AdamState* st) {
-    AdamState s = *st;
-    s.t += 1u;
-    s.p1 *= 0.9f;
-    s.p2 *= 0.999f;
-    s.bc1_inv = 1.0f / (1.0f - s.p1);
-    s.bc2_inv = 1.0f / (1.0f - s.p2);
-    *st = s;
+    // One wave: lane k < kMaxAdamClasses computes class k's learning rate.
+    const uint32_t k = threadIdx.x;
+    const uint32_t t = st->t + 1u;
+    if (k < (uint32_t)kMaxAdamClasses) {
+        const AdamSchedule sc = st->hyper.sched[k];
+        float lr = sc.base;
+        if (sc.iters > 0u) lr = sc.base * powf(sc.final_factor, (float)(t - 1u) / (float)sc.iters);
+        st->lr[k] = lr;
+    }
+    if (k == 0) {
+        const float p1 = st->p1 * 0.9f, p2 = st->p2 * 0.999f;
+        st->t = t;
+        st->p1 = p1;
+        st->p2 = p2;
+        st->bc1_inv = 1.0f / (1.0f - p1);
+        st->bc2_inv = 1.0f / (1.0f - p2);
+    }
 }
 
 __global__ __launch_bounds__(kAdamThreads) void adam_kernel(const AdamTable table,
@@ -44,6 +56,7 @@ __global__ __launch_bounds__(kAdamThreads) void adam_kernel(const AdamTable tabl
     const uint32_t first = (blockIdx.x - c.block_begin) * (kAdamThreads * kAdamItems) + threadIdx.x * kAdamItems;
     if (first >= c.n) return;
     const float bc1 = state->bc1_inv, bc2 = state->bc2_inv;
+    const float lr = state->lr[ci], reg = state->hyper.reg[ci];
     const uintptr_t align = (uintptr_t)c.raw | (uintptr_t)c.grad | (uintptr_t)c.m | (uintptr_t)c.v |
                             (uintptr_t)c.activated;
     const bool vec = (first + kAdamItems <= c.n) && ((align & 15u) == 0);
@@ -72,7 +85,7 @@ __global__ __launch_bounds__(kAdamThreads) void adam_kernel(const AdamTable tabl
     float act[kAdamItems];
 #pragma unroll
     for (int k = 0; k < kAdamItems; ++k) {
-        if (step) adam_element(c.activation, c.lr, c.reg, bc1, bc2, raw[k], gr[k], m[k], v[k]);
+        if (step) adam_element(c.activation, lr, reg, bc1, bc2, raw[k], gr[k], m[k], v[k]);
         act[k] = adam_activate(c.activation, raw[k]);
     }
     if (vec) {
@@ -102,7 +115,7 @@ __global__ __launch_bounds__(kAdamThreads) void adam_kernel(const AdamTable tabl
 uint32_t adam_blocks_for(uint32_t n) { return div_up(n ? n : 1, kAdamThreads * kAdamItems); }
 
 hipError_t launch_adam_tick(hipStream_t st, AdamState* state) {
-    hipLaunchKernelGGL(adam_tick_kernel, dim3(1), dim3(1), 0, st, state);
+    hipLaunchKernelGGL(adam_tick_kernel, dim3(1), dim3(kWave), 0, st, state);
     return hipGetLastError();
 }
 

@@ -326,13 +326,14 @@ __device__ __forceinline__ void store_nt(float4* p, float4 v) {
 }
 
 // Adam over 4 consecutive floats of class c at float4 index j (raw given).
-__device__ __forceinline__ void adam4(const AdamClass& c, float bc1, float bc2, uint32_t j, float4 raw, float4 g) {
+__device__ __forceinline__ void adam4(const AdamClass& c, float lr, float reg, float bc1, float bc2, uint32_t j,
+                                      float4 raw, float4 g) {
     float4 m = reinterpret_cast<const float4*>(c.m)[j];
     float4 v = reinterpret_cast<const float4*>(c.v)[j];
-    adam_element(c.activation, c.lr, c.reg, bc1, bc2, raw.x, g.x, m.x, v.x);
-    adam_element(c.activation, c.lr, c.reg, bc1, bc2, raw.y, g.y, m.y, v.y);
-    adam_element(c.activation, c.lr, c.reg, bc1, bc2, raw.z, g.z, m.z, v.z);
-    adam_element(c.activation, c.lr, c.reg, bc1, bc2, raw.w, g.w, m.w, v.w);
+    adam_element(c.activation, lr, reg, bc1, bc2, raw.x, g.x, m.x, v.x);
+    adam_element(c.activation, lr, reg, bc1, bc2, raw.y, g.y, m.y, v.y);
+    adam_element(c.activation, lr, reg, bc1, bc2, raw.z, g.z, m.z, v.z);
+    adam_element(c.activation, lr, reg, bc1, bc2, raw.w, g.w, m.w, v.w);
     store_nt(reinterpret_cast<float4*>(c.raw) + j, raw);
     store_nt(reinterpret_cast<float4*>(c.m) + j, m);
     store_nt(reinterpret_cast<float4*>(c.v) + j, v);
@@ -380,6 +381,8 @@ __global__ __launch_bounds__(kFuseThreads, 3) void geom_adam_kernel(
         if (any) add_view(cams[v], gc, p, s, q, reinterpret_cast<const float*>(row), a, false, nullptr);
     }
     const float bc1 = state->bc1_inv, bc2 = state->bc2_inv;
+    const float* lr = state->lr;
+    const float* reg = state->hyper.reg;
     if (valid) {
         const float4 gp = make_float4(a.pos[0], a.pos[1], a.pos[2], 0.f);
         const float4 gq = make_float4(a.q[0], a.q[1], a.q[2], a.q[3]);
@@ -389,13 +392,13 @@ __global__ __launch_bounds__(kFuseThreads, 3) void geom_adam_kernel(
             keep.quat[i] = gq;
             keep.opacity[i] = a.op;
         }
-        adam4(cp, bc1, bc2, i, praw, gp);
-        adam4(cq, bc1, bc2, i, qraw, gq);
+        adam4(cp, lr[0], reg[0], bc1, bc2, i, praw, gp);
+        adam4(cq, lr[2], reg[2], bc1, bc2, i, qraw, gq);
         {
             float2 m = reinterpret_cast<const float2*>(cs.m)[i];
             float2 vv = reinterpret_cast<const float2*>(cs.v)[i];
-            adam_element(cs.activation, cs.lr, cs.reg, bc1, bc2, sraw.x, a.sc[0], m.x, vv.x);
-            adam_element(cs.activation, cs.lr, cs.reg, bc1, bc2, sraw.y, a.sc[1], m.y, vv.y);
+            adam_element(cs.activation, lr[1], reg[1], bc1, bc2, sraw.x, a.sc[0], m.x, vv.x);
+            adam_element(cs.activation, lr[1], reg[1], bc1, bc2, sraw.y, a.sc[1], m.y, vv.y);
             reinterpret_cast<float2*>(cs.raw)[i] = sraw;
             reinterpret_cast<float2*>(cs.m)[i] = m;
             reinterpret_cast<float2*>(cs.v)[i] = vv;
@@ -404,7 +407,7 @@ __global__ __launch_bounds__(kFuseThreads, 3) void geom_adam_kernel(
         }
         {
             float r = co.raw[i], m = co.m[i], vv = co.v[i];
-            adam_element(co.activation, co.lr, co.reg, bc1, bc2, r, a.op, m, vv);
+            adam_element(co.activation, lr[3], reg[3], bc1, bc2, r, a.op, m, vv);
             co.raw[i] = r;
             co.m[i] = m;
             co.v[i] = vv;
@@ -424,7 +427,7 @@ __global__ __launch_bounds__(kFuseThreads, 3) void geom_adam_kernel(
         const uint32_t j = e / kShVec, k = e - j * kShVec;
         const float4 g4 = s_sh[j * kShRow + k];
         if (keep.sh != nullptr) reinterpret_cast<float4*>(keep.sh)[base + e] = g4;
-        adam4(ch, bc1, bc2, base + e, reinterpret_cast<const float4*>(ch.raw)[base + e], g4);
+        adam4(ch, lr[4], reg[4], bc1, bc2, base + e, reinterpret_cast<const float4*>(ch.raw)[base + e], g4);
     }
 }
 

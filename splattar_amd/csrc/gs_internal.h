@@ -668,16 +668,35 @@ struct AdamClass {
     float* activated;
     uint32_t n;
     int activation;   // 0 identity, 1 exp, 2 sigmoid
-    float lr;
-    float reg;
     uint32_t block_begin;  // first block of this class in the fused grid
 };
+constexpr int kMaxAdamClasses = 8;
+// Learning-rate schedule of one class: lr = base while iters == 0, else
+// lr = base * final_factor^(iter / iters) with iter the 0-based index of the
+// step being taken (the exponential decay of trainer.rs:1149-1151 with
+// schedule_t = iter / iters, trainer.rs:407-416).
+struct AdamSchedule {
+    float base;
+    float final_factor;
+    uint32_t iters;
+    uint32_t pad;
+};
+// Host-written hyperparameters (uploaded on the step's stream when changed).
+struct AdamHyper {
+    AdamSchedule sched[kMaxAdamClasses];
+    float reg[kMaxAdamClasses];
+};
+// Device-resident optimizer state: the step counter, the running f32
+// products beta1^t, beta2^t and their bias corrections, and this step's
+// learning rates -- all advanced by adam_tick on the device, so a captured
+// graph replays the bias corrections and the learning-rate schedule.
 struct AdamState {
     uint32_t t;
     float p1, p2;          // beta1^t, beta2^t (running f32 products)
     float bc1_inv, bc2_inv;
+    float lr[kMaxAdamClasses];  // learning rate of the current step, per class
+    AdamHyper hyper;
 };
-constexpr int kMaxAdamClasses = 8;
 struct AdamTable {
     AdamClass c[kMaxAdamClasses];
     uint32_t n_classes;
@@ -714,8 +733,8 @@ __device__ __forceinline__ void adam_element(int act, float lr, float reg, float
 
 uint32_t adam_blocks_for(uint32_t n);
 hipError_t launch_adam_tick(hipStream_t st, AdamState* state);
-// The class table travels by value in the kernel arguments (a captured graph
-// keeps the learning rates it was captured with).
+// The class table (buffers and sizes) travels by value in the kernel
+// arguments; learning rates and regularisers are read from the state.
 hipError_t launch_adam(hipStream_t st, const AdamTable& table, uint32_t total_blocks,
                        const AdamState* state, bool step);
 // Geometry backward of a batch fused with the Adam step (after adam_tick) of

@@ -1276,7 +1276,6 @@ struct OptimClass {
     std::string name;
     uint32_t n = 0;
     int activation = 0;
-    float lr = 0.0f, reg = 0.0f;
     float* raw = nullptr;
     float* m = nullptr;
     float* v = nullptr;
@@ -1288,6 +1287,15 @@ struct gs_optim_s {
     gs_ctx ctx = nullptr;
     std::vector<OptimClass> classes;
     AdamState* state = nullptr;
+    // Learning rates, schedules and regularisers as set on the host; uploaded
+    // into state->hyper on the stream of the next encode that follows a
+    // change (the reference writes its uniforms at encode time too,
+    // optim.rs:179-197), through a pinned staging copy.
+    AdamHyper hyper{};
+    bool hyper_dirty = true;
+    AdamHyper* hyper_pinned = nullptr;
+    hipEvent_t hyper_done = nullptr;
+    bool hyper_pending = false;
     ~gs_optim_s() {
         for (auto& c : classes) {
             dfree(c.raw);
@@ -1295,6 +1303,8 @@ struct gs_optim_s {
             dfree(c.v);
         }
         dfree(state);
+        if (hyper_done) (void)hipEventDestroy(hyper_done);
+        if (hyper_pinned) (void)hipHostFree(hyper_pinned);
     }
     AdamTable table(uint32_t* total_blocks) const {
         AdamTable t;
@@ -1302,7 +1312,7 @@ struct gs_optim_s {
         uint32_t b = 0;
         for (size_t k = 0; k < classes.size(); ++k) {
             const OptimClass& c = classes[k];
-            t.c[k] = AdamClass{c.raw, c.grads, c.m, c.v, c.activated, c.n, c.activation, c.lr, c.reg, b};
+            t.c[k] = AdamClass{c.raw, c.grads, c.m, c.v, c.activated, c.n, c.activation, b};
             b += adam_blocks_for(c.n);
         }
         t.n_classes = (uint32_t)classes.size();
@@ -1315,7 +1325,52 @@ struct gs_optim_s {
             if (c.name == name) return &c;
         return nullptr;
     }
+    int index_of(const char* name) const {
+        if (!name) return -1;
+        for (size_t k = 0; k < classes.size(); ++k)
+            if (classes[k].name == name) return (int)k;
+        return -1;
+    }
+    // Enqueues the upload of changed hyperparameters on `st` (not while st
+    // is being captured: a captured step replays the device copy as it was
+    // uploaded last, so callers flush before capturing).
+    gs_status sync_hyper(hipStream_t st) {
+        if (!hyper_dirty) return GS_OK;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        GS_HIP(hipStreamIsCapturing(st, &cs));
+        if (cs != hipStreamCaptureStatusNone)
+            return fail(GS_ERR_INVALID, "optimizer learning rates changed during stream capture; "
+                                        "encode an uncaptured step (or gs_optim_encode_activate) first");
+        if (hyper_pending) GS_HIP(hipEventSynchronize(hyper_done));
+        *hyper_pinned = hyper;
+        GS_HIP(hipMemcpyAsync(&state->hyper, hyper_pinned, sizeof(AdamHyper), hipMemcpyHostToDevice, st));
+        GS_HIP(hipEventRecord(hyper_done, st));
+        hyper_pending = true;
+        hyper_dirty = false;
+        return GS_OK;
+    }
 };
+
+// Serialized optimizer state (gs_optim_export_state): a header, the class
+// descriptors, the device AdamState and each class's raw, m and v.
+namespace {
+constexpr uint32_t kStateMagic = 0x534F5347u;  // "GSOS"
+constexpr uint32_t kStateVersion = 1u;
+struct StateHeader {
+    uint32_t magic, version, n_classes, state_bytes;
+};
+struct StateClass {
+    char name[32];
+    uint32_t n;
+    int32_t activation;
+};
+}  // namespace
+
+static uint64_t optim_state_bytes(const gs_optim_s* o) {
+    uint64_t b = sizeof(StateHeader) + o->classes.size() * sizeof(StateClass) + sizeof(AdamState);
+    for (const auto& c : o->classes) b += 3ull * c.n * sizeof(float);
+    return b;
+}
 
 extern "C" {
 
@@ -1325,13 +1380,20 @@ gs_status gs_optim_create(gs_ctx ctx, gs_optim* out) {
     auto o = std::make_unique<gs_optim_s>();
     o->ctx = ctx;
     GS_CHECK(dalloc(&o->state, 1, "adam state"));
-    AdamState s0{0u, 1.0f, 1.0f, 1.0f, 1.0f};
+    GS_HIP(hipHostMalloc((void**)&o->hyper_pinned, sizeof(AdamHyper), hipHostMallocDefault));
+    GS_HIP(hipEventCreateWithFlags(&o->hyper_done, hipEventDisableTiming));
+    AdamState s0;
+    std::memset(&s0, 0, sizeof(s0));
+    s0.p1 = s0.p2 = s0.bc1_inv = s0.bc2_inv = 1.0f;
     GS_HIP(hipMemcpy(o->state, &s0, sizeof(s0), hipMemcpyHostToDevice));
     *out = o.release();
     return GS_OK;
 }
 
-void gs_optim_destroy(gs_optim o) { delete o; }
+void gs_optim_destroy(gs_optim o) {
+    if (o && o->hyper_pending) (void)hipEventSynchronize(o->hyper_done);
+    delete o;
+}
 
 gs_status gs_optim_add_class(gs_optim o, const char* name, uint32_t n, int activation, float lr,
                              const float* grads, float* activated) {
@@ -1339,16 +1401,21 @@ gs_status gs_optim_add_class(gs_optim o, const char* name, uint32_t n, int activ
     GS_REQUIRE(activation >= 0 && activation <= 2, "activation must be 0, 1 or 2");
     GS_REQUIRE(o->classes.size() < (size_t)kMaxAdamClasses, "too many optimizer classes");
     GS_REQUIRE(o->find(name) == nullptr, "duplicate optimizer class name");
+    GS_REQUIRE(std::strlen(name) < sizeof(StateClass::name), "optimizer class name longer than 31 bytes");
+    GS_REQUIRE(std::isfinite(lr), "learning rate must be finite");
     OptimClass c;
     c.name = name;
     c.n = n;
     c.activation = activation;
-    c.lr = lr;
     c.grads = grads;
     c.activated = activated;
     GS_CHECK(dalloc(&c.raw, n, "adam raw"));
     GS_CHECK(dalloc(&c.m, n, "adam m"));
     GS_CHECK(dalloc(&c.v, n, "adam v"));
+    const size_t k = o->classes.size();
+    o->hyper.sched[k] = AdamSchedule{lr, 1.0f, 0u, 0u};
+    o->hyper.reg[k] = 0.0f;
+    o->hyper_dirty = true;
     o->classes.push_back(c);
     return GS_OK;
 }
@@ -1361,23 +1428,40 @@ float* gs_optim_class_raw(gs_optim o, const char* name) {
 
 gs_status gs_optim_set_lr(gs_optim o, const char* name, float lr) {
     GS_REQUIRE(o != nullptr, "optim is null");
-    OptimClass* c = o->find(name);
-    GS_REQUIRE(c != nullptr, "unknown optimizer class");
-    c->lr = lr;
+    const int k = o->index_of(name);
+    GS_REQUIRE(k >= 0, "unknown optimizer class");
+    GS_REQUIRE(std::isfinite(lr), "learning rate must be finite");
+    o->hyper.sched[k] = AdamSchedule{lr, 1.0f, 0u, 0u};
+    o->hyper_dirty = true;
+    return GS_OK;
+}
+
+gs_status gs_optim_set_lr_schedule(gs_optim o, const char* name, float base, float final_factor,
+                                   uint32_t iters) {
+    GS_REQUIRE(o != nullptr, "optim is null");
+    const int k = o->index_of(name);
+    GS_REQUIRE(k >= 0, "unknown optimizer class");
+    GS_REQUIRE(std::isfinite(base) && std::isfinite(final_factor) && final_factor > 0.0f,
+               "schedule base must be finite and final_factor finite and > 0");
+    o->hyper.sched[k] = AdamSchedule{base, final_factor, iters, 0u};
+    o->hyper_dirty = true;
     return GS_OK;
 }
 
 gs_status gs_optim_set_reg(gs_optim o, const char* name, float reg) {
     GS_REQUIRE(o != nullptr, "optim is null");
-    OptimClass* c = o->find(name);
-    GS_REQUIRE(c != nullptr, "unknown optimizer class");
-    c->reg = reg;
+    const int k = o->index_of(name);
+    GS_REQUIRE(k >= 0, "unknown optimizer class");
+    GS_REQUIRE(std::isfinite(reg), "regulariser must be finite");
+    o->hyper.reg[k] = reg;
+    o->hyper_dirty = true;
     return GS_OK;
 }
 
 gs_status gs_optim_encode_step(gs_optim o, void* stream) {
     GS_REQUIRE(o != nullptr, "optim is null");
     hipStream_t st = as_stream(stream, o->ctx->stream);
+    GS_CHECK(o->sync_hyper(st));
     uint32_t blocks = 0;
     AdamTable t = o->table(&blocks);
     GS_HIP(launch_adam_tick(st, o->state));
@@ -1387,9 +1471,11 @@ gs_status gs_optim_encode_step(gs_optim o, void* stream) {
 
 gs_status gs_optim_encode_activate(gs_optim o, void* stream) {
     GS_REQUIRE(o != nullptr, "optim is null");
+    hipStream_t st = as_stream(stream, o->ctx->stream);
+    GS_CHECK(o->sync_hyper(st));
     uint32_t blocks = 0;
     AdamTable t = o->table(&blocks);
-    GS_HIP(launch_adam(as_stream(stream, o->ctx->stream), t, blocks, o->state, false));
+    GS_HIP(launch_adam(st, t, blocks, o->state, false));
     return GS_OK;
 }
 
@@ -1399,6 +1485,95 @@ gs_status gs_optim_step_count(gs_optim o, uint32_t* t) {
     GS_HIP(hipStreamSynchronize(o->ctx->stream));
     GS_HIP(hipMemcpy(&s, o->state, sizeof(s), hipMemcpyDeviceToHost));
     *t = s.t;
+    return GS_OK;
+}
+
+gs_status gs_optim_current_lr(gs_optim o, const char* name, float* lr) {
+    GS_REQUIRE(o && lr, "null argument");
+    const int k = o->index_of(name);
+    GS_REQUIRE(k >= 0, "unknown optimizer class");
+    AdamState s;
+    GS_HIP(hipStreamSynchronize(o->ctx->stream));
+    GS_HIP(hipMemcpy(&s, o->state, sizeof(s), hipMemcpyDeviceToHost));
+    *lr = s.lr[k];
+    return GS_OK;
+}
+
+gs_status gs_optim_state_bytes(gs_optim o, uint64_t* bytes) {
+    GS_REQUIRE(o && bytes, "null argument");
+    *bytes = optim_state_bytes(o);
+    return GS_OK;
+}
+
+gs_status gs_optim_export_state(gs_optim o, void* dst, uint64_t bytes) {
+    GS_REQUIRE(o && dst, "null argument");
+    GS_REQUIRE(bytes >= optim_state_bytes(o), "buffer smaller than gs_optim_state_bytes");
+    hipStream_t st = o->ctx->stream;
+    GS_CHECK(o->sync_hyper(st));
+    GS_HIP(hipStreamSynchronize(st));
+    char* p = static_cast<char*>(dst);
+    StateHeader h{kStateMagic, kStateVersion, (uint32_t)o->classes.size(), (uint32_t)sizeof(AdamState)};
+    std::memcpy(p, &h, sizeof(h));
+    p += sizeof(h);
+    for (const auto& c : o->classes) {
+        StateClass sc;
+        std::memset(&sc, 0, sizeof(sc));
+        std::memcpy(sc.name, c.name.data(), c.name.size());
+        sc.n = c.n;
+        sc.activation = c.activation;
+        std::memcpy(p, &sc, sizeof(sc));
+        p += sizeof(sc);
+    }
+    GS_HIP(hipMemcpy(p, o->state, sizeof(AdamState), hipMemcpyDeviceToHost));
+    p += sizeof(AdamState);
+    for (const auto& c : o->classes) {
+        for (const float* src : {(const float*)c.raw, (const float*)c.m, (const float*)c.v}) {
+            GS_HIP(hipMemcpy(p, src, (size_t)c.n * sizeof(float), hipMemcpyDeviceToHost));
+            p += (size_t)c.n * sizeof(float);
+        }
+    }
+    return GS_OK;
+}
+
+gs_status gs_optim_import_state(gs_optim o, const void* src, uint64_t bytes) {
+    GS_REQUIRE(o && src, "null argument");
+    GS_REQUIRE(bytes >= sizeof(StateHeader), "state blob too small");
+    const char* p = static_cast<const char*>(src);
+    StateHeader h;
+    std::memcpy(&h, p, sizeof(h));
+    GS_REQUIRE(h.magic == kStateMagic, "not an optimizer state blob (bad magic)");
+    GS_REQUIRE(h.version == kStateVersion, "unsupported optimizer state version");
+    GS_REQUIRE(h.state_bytes == sizeof(AdamState), "optimizer state layout differs from this build");
+    GS_REQUIRE(h.n_classes == o->classes.size(), "optimizer state has a different number of classes");
+    GS_REQUIRE(bytes >= optim_state_bytes(o), "state blob truncated");
+    p += sizeof(h);
+    for (const auto& c : o->classes) {
+        StateClass sc;
+        std::memcpy(&sc, p, sizeof(sc));
+        p += sizeof(sc);
+        sc.name[sizeof(sc.name) - 1] = 0;
+        GS_REQUIRE(c.name == sc.name && c.n == sc.n && c.activation == sc.activation,
+                   "optimizer state classes (name, size, activation) differ from this optimizer's");
+    }
+    AdamState s;
+    std::memcpy(&s, p, sizeof(s));
+    p += sizeof(s);
+    hipStream_t st = o->ctx->stream;
+    if (o->hyper_pending) GS_HIP(hipEventSynchronize(o->hyper_done));
+    GS_HIP(hipStreamSynchronize(st));
+    GS_HIP(hipMemcpy(o->state, &s, sizeof(s), hipMemcpyHostToDevice));
+    o->hyper = s.hyper;
+    o->hyper_dirty = false;
+    for (const auto& c : o->classes) {
+        for (float* dst : {c.raw, c.m, c.v}) {
+            GS_HIP(hipMemcpy(dst, p, (size_t)c.n * sizeof(float), hipMemcpyHostToDevice));
+            p += (size_t)c.n * sizeof(float);
+        }
+    }
+    uint32_t blocks = 0;
+    AdamTable t = o->table(&blocks);
+    GS_HIP(launch_adam(st, t, blocks, o->state, false));
+    GS_HIP(hipStreamSynchronize(st));
     return GS_OK;
 }
 
@@ -1490,8 +1665,28 @@ struct gs_trainer_s {
     // Batched steps run the geometry backward and Adam as one kernel; the
     // summed parameter gradients reach grad_* only when asked for.
     bool keep_grads = false;
+    // Recorded on the stream of every step, so blocking reads and renders
+    // on the trainer's own stream are ordered after a step enqueued on a
+    // caller's stream.
+    hipEvent_t step_done = nullptr;
+    bool step_pending = false;
+
+    // Blocks until the last step and everything on the trainer's stream
+    // have finished.
+    gs_status wait_idle() {
+        if (step_pending) GS_HIP(hipEventSynchronize(step_done));
+        GS_HIP(hipStreamSynchronize(ctx->stream));
+        return GS_OK;
+    }
+    // Orders later work on the trainer's stream after the last step.
+    gs_status order_after_step() {
+        if (step_pending) GS_HIP(hipStreamWaitEvent(ctx->stream, step_done, 0));
+        return GS_OK;
+    }
 
     ~gs_trainer_s() {
+        if (step_pending) (void)hipEventSynchronize(step_done);
+        if (step_done) (void)hipEventDestroy(step_done);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         if (side) (void)hipStreamDestroy(side);
@@ -1548,6 +1743,7 @@ struct gs_trainer_s {
     size_t saved_floats = 0;
     gs_status save_state() {
         hipStream_t st = ctx->stream;
+        GS_CHECK(order_after_step());
         size_t total = 0;
         for (const auto& c : optim->classes) total += 3 * (size_t)c.n;
         if (total != saved_floats) {
@@ -1572,6 +1768,7 @@ struct gs_trainer_s {
     gs_status restore_state() {
         if (saved == nullptr) return fail(GS_ERR_INVALID, "no saved training state");
         hipStream_t st = ctx->stream;
+        GS_CHECK(order_after_step());
         size_t off = 0;
         for (const auto& c : optim->classes) {
             for (float* dst : {c.raw, c.m, c.v}) {
@@ -1751,6 +1948,11 @@ gs_status gs_trainer_create(gs_ctx ctx, const gs_train_config* cfg, const gs_sce
         {"sh", n * kShFloats, GS_ACT_IDENTITY, cfg->lr_sh, t->grad_sh, t->sh},
     };
     for (auto& c : cls) GS_CHECK(gs_optim_add_class(t->optim, c.name, c.count, c.act, c.lr, c.grads, c.activated));
+    // Exponential position-LR decay (trainer.rs:1149-1151), evaluated per
+    // step on the device from the step counter.
+    if (cfg->iters > 0 && cfg->pos_lr_final_factor > 0.0f)
+        GS_CHECK(gs_optim_set_lr_schedule(t->optim, "pos", cfg->lr_pos, cfg->pos_lr_final_factor, cfg->iters));
+    GS_HIP(hipEventCreateWithFlags(&t->step_done, hipEventDisableTiming));
     {
         // raw init on the host (log / logit with the reference's clamps).
         std::vector<float> pos4(n * 4), sc(n * 2), q(init->quats, init->quats + n * 4), op(n),
@@ -1806,6 +2008,9 @@ gs_status gs_trainer_step(gs_trainer t, void* stream, const uint32_t* view_ids, 
     }
     t->last_ids.assign(view_ids, view_ids + k);
     t->last_batched = batched;
+    // Learning-rate changes reach the device state before the step (and
+    // before a capture: the graph reads them from device memory).
+    GS_CHECK(t->optim->sync_hyper(st));
     if (mode == GS_STEP_GRAPH) {
         // The graph bakes grid sizes, so it is valid for one sequence of view
         // sizes; the camera table is refreshed in device memory before each
@@ -1839,10 +2044,14 @@ gs_status gs_trainer_step(gs_trainer t, void* stream, const uint32_t* view_ids, 
             t->graph_shape = shape;
         }
         GS_HIP(hipGraphLaunch(t->graph_exec, st));
+        GS_HIP(hipEventRecord(t->step_done, st));
+        t->step_pending = true;
         return GS_OK;
     }
     GS_CHECK(t->core.set_views(st, cams, batched));
     GS_CHECK(t->enqueue_step(st, view_ids, k, batched, t->prof.enabled));
+    GS_HIP(hipEventRecord(t->step_done, st));
+    t->step_pending = true;
     // Fold finished events so the event pool stays bounded.
     if (t->prof.enabled) GS_CHECK(t->prof.collect());
     return GS_OK;
@@ -2025,9 +2234,33 @@ gs_status gs_trainer_autotune(gs_trainer t, const uint32_t* view_ids, uint32_t n
     return GS_OK;
 }
 
+gs_status gs_trainer_optim(gs_trainer t, gs_optim* out) {
+    GS_REQUIRE(t && out, "null argument");
+    *out = t->optim;
+    return GS_OK;
+}
+
+gs_status gs_trainer_state_bytes(gs_trainer t, uint64_t* bytes) {
+    GS_REQUIRE(t && bytes, "null argument");
+    return gs_optim_state_bytes(t->optim, bytes);
+}
+
+gs_status gs_trainer_export_state(gs_trainer t, void* dst, uint64_t bytes) {
+    GS_REQUIRE(t && dst, "null argument");
+    GS_CHECK(t->wait_idle());
+    return gs_optim_export_state(t->optim, dst, bytes);
+}
+
+gs_status gs_trainer_import_state(gs_trainer t, const void* src, uint64_t bytes) {
+    GS_REQUIRE(t && src, "null argument");
+    GS_CHECK(t->wait_idle());
+    return gs_optim_import_state(t->optim, src, bytes);
+}
+
 gs_status gs_trainer_last_losses(gs_trainer t, double* out, uint32_t n_ids) {
     GS_REQUIRE(t && out, "null argument");
     GS_REQUIRE(n_ids <= t->last_ids.size(), "more views requested than the last step had");
+    GS_CHECK(t->order_after_step());
     for (uint32_t j = 0; j < n_ids; ++j) {
         const TrainViewDev& tv = t->views[t->last_ids[j]];
         const uint32_t np = l1_ssim_partials(tv.width, tv.height);
@@ -2047,6 +2280,9 @@ gs_status gs_trainer_render(gs_trainer t, uint32_t view, float* rgba) {
     c.sh_degree = t->sh_degree;
     CamParams cp = make_cam(c, tv.width, tv.height, t->core.shape.tile);
     cp.n_surfels = tv.num_surfels;
+    // The render overwrites the step's camera table, records and images:
+    // it must not start before a step enqueued on another stream ends.
+    GS_CHECK(t->order_after_step());
     GS_CHECK(t->core.set_views(st, {cp}, true));
     GS_CHECK(t->core.forward(st, 0, 1, t->n, t->positions, t->scales, t->quats, t->opacities, t->sh));
     GS_HIP(hipStreamSynchronize(st));
@@ -2077,7 +2313,7 @@ gs_status gs_trainer_params(gs_trainer t, gs_raster_buffers* o) {
 
 gs_status gs_trainer_entry_count(gs_trainer t, uint32_t* entries) {
     GS_REQUIRE(t && entries, "null argument");
-    GS_HIP(hipStreamSynchronize(t->ctx->stream));
+    GS_CHECK(t->wait_idle());
     std::vector<uint32_t> tot(t->core.cap_views);
     uint32_t of = 0;
     GS_HIP(hipMemcpy(tot.data(), t->core.total, 4 * tot.size(), hipMemcpyDeviceToHost));
