@@ -82,6 +82,19 @@ uint32_t* gs_scan_total(gs_scan scan);  /* PrefixSum::total */
 /* PrefixSum::encode(ctx, encoder, n) (prefix_sum.rs:134-164); n <= capacity. */
 gs_status gs_scan_encode(gs_scan scan, void* stream, uint32_t n);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the exclusive scan of gs_scan_encode (same kernels, same
+ * results) over the caller's own buffers -- data[0..n) in place, the grand
+ * total into *total (both device pointers on ctx's device; stream null = the
+ * context stream). Calls no allocating routine: the partition sums live in a
+ * fixed module-scope device array of ~256 KiB per device, held until the
+ * process ends. n > GS_SCAN_NOALLOC_MAX_ITEMS is GS_ERR_CAPACITY. Every call
+ * on one device shares that array, so calls that overlap on one device (two
+ * streams, two threads) race; serialize them. */
+#define GS_SCAN_NOALLOC_MAX_ITEMS (1u << 28)
+gs_status gs_scan_encode_noalloc(gs_ctx ctx, void* stream, uint32_t* data, uint32_t n, uint32_t* total);
+
 /* -------------------------------------------------------------- RadixSorter
  * crates/gs-wgpu/src/sort.rs:17-327: stable LSD sort of the first counts[0]
  * (key, u32 payload) pairs in place; the count lives in device memory.

@@ -95,6 +95,7 @@ _PROTOS = {
     "gs_scan_data": ([_vp], _vp),
     "gs_scan_total": ([_vp], _vp),
     "gs_scan_encode": ([_vp, _vp, _u32], C.c_int),
+    "gs_scan_encode_noalloc": ([_vp, _vp, _vp, _u32, _vp], C.c_int),
     "gs_sorter_create": ([_vp, _u32, _u32, C.POINTER(_vp)], C.c_int),
     "gs_sorter_destroy": ([_vp], None),
     "gs_sorter_keys": ([_vp], _vp),

@@ -525,6 +525,12 @@ hipError_t launch_project(hipStream_t st, const CamParams* cams_dev, uint32_t n_
                           bool pack_counts = false, bool write_normal = true);
 
 // scan.hip
+// Items per scan partition; the scratch holds a 256-B header and one u32 sum
+// per partition plus one.
+constexpr uint32_t kScanPartItems = 4096;
+constexpr size_t scan_scratch_size(uint32_t capacity) {
+    return 256 + ((size_t)(capacity + kScanPartItems - 1) / kScanPartItems + 1) * sizeof(uint32_t);
+}
 size_t scan_scratch_bytes(uint32_t capacity);
 // Exclusive scan of in[0..n) (in place allowed), of in[perm[i]] (gather),
 // or, with in == nullptr, of perm[i] & kPackMask (packed tile counts).

@@ -14,6 +14,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -136,7 +137,41 @@ struct gs_context {
     int device = 0;
     hipStream_t stream = nullptr;
     hipDeviceProp_t props{};
+    // Device address of this device's `scratch` array (the no-allocation
+    // scan's partition sums).
+    uint32_t* noalloc_scratch = nullptr;
 };
+
+namespace gs {
+
+// Scratch of gs_scan_encode_noalloc: the partition sums of a scan of up to
+// GS_SCAN_NOALLOC_MAX_ITEMS items, in a module-scope device array so that the
+// entry point allocates nothing. Each device holds its own copy until the
+// process ends; every call on a device shares it.
+constexpr size_t kNoallocScratchWords = (scan_scratch_size(GS_SCAN_NOALLOC_MAX_ITEMS) + 255) / 256 * 64;
+static __device__ __align__(256) uint32_t scratch[kNoallocScratchWords];
+
+// The host-side name `scratch` is not a device address: the address is
+// looked up once per device (with that device current) and only that
+// pointer reaches kernels.
+static std::mutex g_noalloc_mu;
+static std::map<int, uint32_t*> g_noalloc_ptr;
+
+static gs_status noalloc_scratch_for_current_device(int device, uint32_t** out) {
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    auto it = g_noalloc_ptr.find(device);
+    if (it == g_noalloc_ptr.end()) {
+        uint32_t* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || ptr == nullptr)
+            return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+        it = g_noalloc_ptr.emplace(device, ptr).first;
+    }
+    *out = it->second;
+    return GS_OK;
+}
+
+}  // namespace gs
 
 extern "C" {
 
@@ -155,6 +190,7 @@ gs_status gs_ctx_create(int device_index, gs_ctx* out) {
     auto ctx = std::make_unique<gs_context>();
     ctx->device = device_index;
     GS_HIP(hipGetDeviceProperties(&ctx->props, device_index));
+    GS_CHECK(noalloc_scratch_for_current_device(device_index, &ctx->noalloc_scratch));
     GS_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     *out = ctx.release();
     return GS_OK;
@@ -267,6 +303,21 @@ gs_status gs_scan_encode(gs_scan s, void* stream, uint32_t n) {
     if (n > s->capacity) return fail(GS_ERR_CAPACITY, "scan: n exceeds capacity");  // prefix_sum.rs:135
     GS_HIP(launch_exclusive_scan(as_stream(stream, s->ctx->stream), s->data, s->data, n, s->total,
                                  s->scratch));
+    return GS_OK;
+}
+
+gs_status gs_scan_encode_noalloc(gs_ctx ctx, void* stream, uint32_t* data, uint32_t n, uint32_t* total) {
+    GS_REQUIRE(ctx != nullptr, "ctx is null");
+    GS_REQUIRE(total != nullptr && (data != nullptr || n == 0), "null argument");
+    if (n > GS_SCAN_NOALLOC_MAX_ITEMS)
+        return fail(GS_ERR_CAPACITY, "gs_scan_encode_noalloc: n = " + std::to_string(n) +
+                                         " exceeds the no-allocation limit of " +
+                                         std::to_string(GS_SCAN_NOALLOC_MAX_ITEMS) + " items");
+    GS_REQUIRE(ctx->noalloc_scratch != nullptr, "no-allocation scratch not set up for this context");
+    // The same kernels as gs_scan_encode; the partition sums live in this
+    // device's `scratch` array. The reduce kernel writes every sum the later
+    // kernels read, so nothing in it needs clearing between calls.
+    GS_HIP(launch_exclusive_scan(as_stream(stream, ctx->stream), data, data, n, total, ctx->noalloc_scratch));
     return GS_OK;
 }
 

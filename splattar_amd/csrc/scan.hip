@@ -24,6 +24,7 @@ struct ScanScratch {
     uint32_t pad[64];
     // followed by uint32_t sums[max_parts]
 };
+static_assert(kScanPart == kScanPartItems && sizeof(ScanScratch) == 256, "scan scratch layout");
 
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v, int lane) {
     for (int o = 1; o < 64; o <<= 1) {
@@ -165,10 +166,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_down_kernel(const uint32_t*
 
 }  // namespace
 
-size_t scan_scratch_bytes(uint32_t capacity) {
-    const uint32_t parts = (capacity + kScanPart - 1) / kScanPart + 1;
-    return sizeof(ScanScratch) + (size_t)parts * sizeof(uint32_t);
-}
+size_t scan_scratch_bytes(uint32_t capacity) { return scan_scratch_size(capacity); }
 
 hipError_t launch_exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t n,
                                  uint32_t* total_out, void* scratch, const uint32_t* perm) {
