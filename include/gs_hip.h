@@ -271,6 +271,20 @@ gs_status gs_raster_set_binning(gs_raster r, uint32_t mode);
  * gradients are the same with it off. */
 gs_status gs_raster_set_culling(gs_raster r, int enable);
 
+/* Decision trace of the next single-view forwards (no counterpart in the
+ * reference: a verification aid, so that an FP64 evaluation can replay the
+ * f32 path's discrete decisions). For each pixel of the window [x0, x0 + w) x
+ * [y0, y0 + h), in row-major window order, the forward records its
+ * contributions in compositing order: lists[p * cap + k] = surfel index
+ * (bits 0..29) | ray branch taken (bit 30) | alpha clamped at 0.995 (bit 31),
+ * and counts[p] = the number of contributions (those past cap are counted,
+ * not stored). Needs the default 16-px tiles; the trace forward is a
+ * separate kernel instantiation, so untraced forwards are unaffected.
+ * w * h * cap = 0 turns the trace off. Allocates the trace buffers. */
+gs_status gs_raster_set_trace(gs_raster r, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t cap);
+/* Blocking: device pointers of the trace (null when off). */
+gs_status gs_raster_trace_buffers(gs_raster r, const uint32_t** lists, const uint32_t** counts);
+
 /* ----------------------------------------------------------------- SsimLoss
  * first_attempt/crates/gs-train/src/ssim.rs:25-298: fused L1 + D-SSIM,
  * writes dL/d(image) rgb (w = 0) into dl_out. Loss =

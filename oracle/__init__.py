@@ -90,6 +90,9 @@ def lib():
         L.oref_marginal_decisions.restype = C.c_int64
         L.oref_pixel_contributors.argtypes = [C.POINTER(_Scene), C.POINTER(_Camera), C.c_int64,
                                               C.POINTER(C.c_int32), C.POINTER(C.c_uint8)]
+        L.oref_trace.argtypes = [C.POINTER(_Scene), C.POINTER(_Camera), C.c_int64, _u32p, _u32p]
+        L.oref_replay.argtypes = [C.POINTER(_Scene), C.POINTER(_Camera), _u32p, _u32p, C.c_int64, _dp,
+                                  _dp, _dp, C.POINTER(_Grads)]
         L.oref_prepare_order.argtypes = [C.POINTER(_Scene), C.POINTER(_Camera), _i64p]
         L.oref_prepare_order.restype = C.c_int64
         L.oref_prep_f32.argtypes = [C.c_int64, _fp, _fp, _fp, C.POINTER(_Cam32), _u32p, _u32p,
@@ -313,6 +316,55 @@ def pixel_contributors(scene: Scene, cam: Camera, pixels) -> np.ndarray:
     if rc != 0:
         raise RuntimeError("oref_pixel_contributors failed")
     return flags
+
+
+def trace(scene: Scene, cam: Camera, cap: int = 1024):
+    """The faithful FP64 walk's contribution lists over the camera's window,
+    in replay()'s format: (lists (P, cap) u32, counts (P,) u32)."""
+    W, H = cam.out_size
+    lists = np.zeros((W * H, cap), np.uint32)
+    counts = np.zeros(W * H, np.uint32)
+    s, keep = scene._c()
+    c = cam._c()
+    if lib().oref_trace(C.byref(s), C.byref(c), cap, _ptr(lists, _u32p), _ptr(counts, _u32p)) != 0:
+        raise RuntimeError("oref_trace failed")
+    return lists, counts
+
+
+def replay(scene: Scene, cam: Camera, lists, counts, dl_dcolor=None):
+    """Decision replay (test aid, no forward.rs counterpart): the FP64
+    forward, and with dl_dcolor the backward, along another evaluation's
+    per-pixel contribution lists over the camera's window (lists (P, cap) of
+    surfel | ray << 30 | clamped << 31, counts (P,)). Returns (color (H, W, 3),
+    alpha (H, W), Gradients or None)."""
+    W, H = cam.out_size
+    P = W * H
+    lists = np.ascontiguousarray(lists, dtype=np.uint32).reshape(P, -1)
+    counts = np.ascontiguousarray(counts, dtype=np.uint32).reshape(P)
+    color = np.zeros(P * 3)
+    alpha = np.zeros(P)
+    s, keep = scene._c()
+    c = cam._c()
+    g, cg, dlc = None, None, None
+    if dl_dcolor is not None:
+        n = scene.n
+        ncoef = (scene.sh_degree + 1) ** 2
+        g = Gradients(np.zeros((n, 3)), np.zeros((n, 2)), np.zeros((n, 4)), np.zeros(n), np.zeros((n, ncoef, 3)))
+        cg = _Grads(_ptr(g.pos, _dp), _ptr(g.scales, _dp), _ptr(g.quat, _dp), _ptr(g.opacity, _dp),
+                    _ptr(g.sh, _dp))
+        dlc = _d(dl_dcolor).reshape(-1)
+        if dlc.size != 3 * P:
+            raise ValueError("dl_dcolor must be (H, W, 3) over the window")
+    rc = lib().oref_replay(C.byref(s), C.byref(c), _ptr(lists, _u32p), _ptr(counts, _u32p), lists.shape[1],
+                           _ptr(color, _dp), _ptr(alpha, _dp), _ptr(dlc, _dp),
+                           C.byref(cg) if cg is not None else None)
+    if rc != 0:
+        raise RuntimeError(f"oref_replay failed ({rc}: a count above cap or a surfel index out of range)")
+    if g is not None:
+        g.cam_center = np.array(cg.cam_center[:])
+        g.cam_quat = np.array(cg.cam_quat[:])
+        g.focal = float(cg.focal)
+    return color.reshape(H, W, 3), alpha.reshape(H, W), g
 
 
 def prepare_order(scene: Scene, cam: Camera) -> np.ndarray:

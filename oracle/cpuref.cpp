@@ -243,49 +243,56 @@ inline bool covers_pixel(const SurfelCam& sc, int x, int y) {
     return tx >= sc.x0 && tx <= sc.x1 && ty >= sc.y0 && ty <= sc.y1;
 }
 
+// forward.rs:50-92 for surfel idx (no near cull; `prepare` applies it).
+SurfelCam make_surfel_cam(const Scene& sc, const oref_camera* cam, const M3& r_cam, int64_t idx) {
+    double cx = cam->width * 0.5, cy = cam->height * 0.5;
+    V3 C = cam_center(cam);
+    double basis[16];
+    V3 p = sc.pos(idx);
+    V3 x = p - C;
+    V3 c = r_cam.tmul(x);
+    double depth = -c.z;
+    M3 rs = quat_to_mat(sc.s->quat + 4 * idx);
+    V3 tu = r_cam.tmul(rs.col[0] * sc.s->scales[2 * idx]);
+    V3 tv = r_cam.tmul(rs.col[1] * sc.s->scales[2 * idx + 1]);
+    V3 dir = normalize(p - C);
+    int nb = sh_basis(sc.s->sh_degree, dir, basis);
+    V3 raw(0.5, 0.5, 0.5);
+    for (int k = 0; k < nb; ++k) raw += basis[k] * sc.sh(idx, k);
+    SurfelCam s;
+    s.idx = idx;
+    s.c = c;
+    s.tu = tu;
+    s.tv = tv;
+    s.clamped[0] = raw.x < 0.0;
+    s.clamped[1] = raw.y < 0.0;
+    s.clamped[2] = raw.z < 0.0;
+    s.color = V3(std::max(raw.x, 0.0), std::max(raw.y, 0.0), std::max(raw.z, 0.0));
+    V3 normal = cross(tu, tv);
+    if (dot(normal, normal) > 0.0) {
+        normal = normalize(normal);
+        if (dot(normal, c) > 0.0) normal = -normal;
+    }
+    s.normal = normal;
+    s.opacity = sc.s->opacity[idx];
+    s.depth = depth;
+    double w = -c.z;
+    s.px = cx + cam->focal * c.x / w;
+    s.py = cy - cam->focal * c.y / w;
+    compute_rect(s, cam);
+    return s;
+}
+
 // forward.rs:44-97
 std::vector<SurfelCam> prepare(const Scene& sc, const oref_camera* cam) {
     M3 r_cam = quat_to_mat(cam->quat);
-    double cx = cam->width * 0.5, cy = cam->height * 0.5;
     V3 C = cam_center(cam);
     std::vector<SurfelCam> out;
     out.reserve(sc.s->n);
-    double basis[16];
     for (int64_t idx = 0; idx < sc.s->n; ++idx) {
-        V3 p = sc.pos(idx);
-        V3 x = p - C;
-        V3 c = r_cam.tmul(x);
-        double depth = -c.z;
-        if (depth < NEAR_DEPTH) continue;
-        M3 rs = quat_to_mat(sc.s->quat + 4 * idx);
-        V3 tu = r_cam.tmul(rs.col[0] * sc.s->scales[2 * idx]);
-        V3 tv = r_cam.tmul(rs.col[1] * sc.s->scales[2 * idx + 1]);
-        V3 dir = normalize(p - C);
-        int nb = sh_basis(sc.s->sh_degree, dir, basis);
-        V3 raw(0.5, 0.5, 0.5);
-        for (int k = 0; k < nb; ++k) raw += basis[k] * sc.sh(idx, k);
-        SurfelCam s;
-        s.idx = idx;
-        s.c = c;
-        s.tu = tu;
-        s.tv = tv;
-        s.clamped[0] = raw.x < 0.0;
-        s.clamped[1] = raw.y < 0.0;
-        s.clamped[2] = raw.z < 0.0;
-        s.color = V3(std::max(raw.x, 0.0), std::max(raw.y, 0.0), std::max(raw.z, 0.0));
-        V3 normal = cross(tu, tv);
-        if (dot(normal, normal) > 0.0) {
-            normal = normalize(normal);
-            if (dot(normal, c) > 0.0) normal = -normal;
-        }
-        s.normal = normal;
-        s.opacity = sc.s->opacity[idx];
-        s.depth = depth;
-        double w = -c.z;
-        s.px = cx + cam->focal * c.x / w;
-        s.py = cy - cam->focal * c.y / w;
-        compute_rect(s, cam);
-        out.push_back(s);
+        V3 c = r_cam.tmul(sc.pos(idx) - C);
+        if (-c.z < NEAR_DEPTH) continue;  // forward.rs:56-58
+        out.push_back(make_surfel_cam(sc, cam, r_cam, idx));
     }
     // forward.rs:95: ascending f32 depth-key bits, stable by original index.
     auto key = [](const SurfelCam& s) {
@@ -467,6 +474,10 @@ struct Workspace {
     std::vector<Contribution> contribs;
 };
 
+void reverse_walk(const std::vector<SurfelCam>& surfels, const oref_camera* cam, V3 d, double pix_x,
+                  double pix_y, double w_total, double wt_total, V3 dl_dc_pix, double dl_dd_pix,
+                  V3 dl_dn_pix, double lambda_dist, Workspace& ws);
+
 // backward.rs:67-173 for one pixel.
 void backward_pixel(const std::vector<SurfelCam>& surfels, const TileLists* tl,
                     const oref_camera* cam, int x, int y, V3 dl_dc_pix, double dl_dd_pix,
@@ -491,7 +502,15 @@ void backward_pixel(const std::vector<SurfelCam>& surfels, const TileLists* tl,
         transmittance *= 1.0 - alpha;
         return !(transmittance < T_TERMINATE);
     });
-    // Reverse walk (backward.rs:112-172).
+    reverse_walk(surfels, cam, d, pix_x, pix_y, w_total, wt_total, dl_dc_pix, dl_dd_pix, dl_dn_pix,
+                 lambda_dist, ws);
+}
+
+// backward.rs:112-172: the reverse walk over the pixel's contributions
+// (ws.contribs, front to back), accumulating into ws.acc[list_idx].
+void reverse_walk(const std::vector<SurfelCam>& surfels, const oref_camera* cam, V3 d, double pix_x,
+                  double pix_y, double w_total, double wt_total, V3 dl_dc_pix, double dl_dd_pix,
+                  V3 dl_dn_pix, double lambda_dist, Workspace& ws) {
     V3 suffix_color, suffix_normal;
     double suffix_depth = 0.0, suffix_w = 0.0, suffix_cw = 0.0;
     for (auto it = ws.contribs.rbegin(); it != ws.contribs.rend(); ++it) {
@@ -853,47 +872,12 @@ double oref_distortion_loss(const oref_scene* s, const oref_camera* cam, int mod
     return total;
 }
 
-// backward.rs:57-249
-int oref_gradients(const oref_scene* s, const oref_camera* cam, int mode,
-                   const double* dl_dcolor, const double* dl_ddepth, const double* dl_dnormal,
-                   double lambda_dist, oref_grads* g) {
-    if (!s || !cam || !dl_dcolor || !g) return -1;
-    Scene sc{s, num_coeffs(s->sh_degree)};
+// backward.rs:176-245: camera-space accumulators acc[li] of surfels[li] ->
+// parameter and camera gradients (the focal term is added by the caller).
+void chain_to_params(const std::vector<SurfelCam>& surfels, const std::vector<CamSpaceAcc>& acc,
+                     const Scene& sc, const oref_camera* cam, oref_grads* g) {
+    const oref_scene* s = sc.s;
     const int64_t n = s->n;
-    auto surfels = prepare(sc, cam);
-    TileLists tl;
-    const TileLists* tlp = nullptr;
-    if (mode == 1) {
-        tl = build_tile_lists(surfels, cam);
-        tlp = &tl;
-    }
-    const Window win = window_of(cam);
-    // Per-thread accumulators, reduced in thread order (literal mode is
-    // single-threaded and so accumulates in exactly the reference's order).
-    int nthreads = mode == 1 ? threads_available() : 1;
-    std::vector<Workspace> ws(nthreads);
-    for (auto& w : ws) w.acc.assign(surfels.size(), CamSpaceAcc());
-#pragma omp parallel for schedule(static, 1) num_threads(nthreads) if (mode == 1)
-    for (int yy = 0; yy < win.h; ++yy) {
-        int tid = 0;
-#ifdef _OPENMP
-        tid = omp_get_thread_num();
-#endif
-        for (int xx = 0; xx < win.w; ++xx) {
-            const int x = win.x0 + xx, y = win.y0 + yy;
-            int64_t pi = (int64_t)yy * win.w + xx;
-            V3 dlc(dl_dcolor[3 * pi], dl_dcolor[3 * pi + 1], dl_dcolor[3 * pi + 2]);
-            double dld = dl_ddepth ? dl_ddepth[pi] : 0.0;
-            V3 dln = dl_dnormal ? V3(dl_dnormal[3 * pi], dl_dnormal[3 * pi + 1], dl_dnormal[3 * pi + 2])
-                                : V3();
-            backward_pixel(surfels, tlp, cam, x, y, dlc, dld, dln, lambda_dist, ws[tid]);
-        }
-    }
-    for (int t = 1; t < nthreads; ++t) {
-        for (size_t i = 0; i < surfels.size(); ++i) add_acc(ws[0].acc[i], ws[t].acc[i]);
-        ws[0].dfocal += ws[t].dfocal;
-    }
-    const std::vector<CamSpaceAcc>& acc = ws[0].acc;
 
     const int ncoef = sc.ncoef;
     std::fill(g->pos, g->pos + 3 * n, 0.0);
@@ -976,6 +960,188 @@ int oref_gradients(const oref_scene* s, const oref_camera* cam, int mode,
     g->cam_center[1] = gcenter.y;
     g->cam_center[2] = gcenter.z;
     quat_grad(cam->quat, dl_dr_cam, g->cam_quat);
+}
+
+// backward.rs:57-249
+int oref_gradients(const oref_scene* s, const oref_camera* cam, int mode,
+                   const double* dl_dcolor, const double* dl_ddepth, const double* dl_dnormal,
+                   double lambda_dist, oref_grads* g) {
+    if (!s || !cam || !dl_dcolor || !g) return -1;
+    Scene sc{s, num_coeffs(s->sh_degree)};
+    auto surfels = prepare(sc, cam);
+    TileLists tl;
+    const TileLists* tlp = nullptr;
+    if (mode == 1) {
+        tl = build_tile_lists(surfels, cam);
+        tlp = &tl;
+    }
+    const Window win = window_of(cam);
+    // Per-thread accumulators, reduced in thread order (literal mode is
+    // single-threaded and so accumulates in exactly the reference's order).
+    int nthreads = mode == 1 ? threads_available() : 1;
+    std::vector<Workspace> ws(nthreads);
+    for (auto& w : ws) w.acc.assign(surfels.size(), CamSpaceAcc());
+#pragma omp parallel for schedule(static, 1) num_threads(nthreads) if (mode == 1)
+    for (int yy = 0; yy < win.h; ++yy) {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        for (int xx = 0; xx < win.w; ++xx) {
+            const int x = win.x0 + xx, y = win.y0 + yy;
+            int64_t pi = (int64_t)yy * win.w + xx;
+            V3 dlc(dl_dcolor[3 * pi], dl_dcolor[3 * pi + 1], dl_dcolor[3 * pi + 2]);
+            double dld = dl_ddepth ? dl_ddepth[pi] : 0.0;
+            V3 dln = dl_dnormal ? V3(dl_dnormal[3 * pi], dl_dnormal[3 * pi + 1], dl_dnormal[3 * pi + 2])
+                                : V3();
+            backward_pixel(surfels, tlp, cam, x, y, dlc, dld, dln, lambda_dist, ws[tid]);
+        }
+    }
+    for (int t = 1; t < nthreads; ++t) {
+        for (size_t i = 0; i < surfels.size(); ++i) add_acc(ws[0].acc[i], ws[t].acc[i]);
+        ws[0].dfocal += ws[t].dfocal;
+    }
+    chain_to_params(surfels, ws[0].acc, sc, cam, g);
+    g->focal = ws[0].dfocal;
+    return 0;
+}
+
+// The contribution lists of the faithful FP64 walk (forward.rs:207-246),
+// in oref_replay's format, over the camera's window: what an evaluation
+// with exactly FP64's decisions would record. counts[p] may exceed cap
+// (entries past cap are counted, not stored). A test aid.
+int oref_trace(const oref_scene* s, const oref_camera* cam, int64_t cap, uint32_t* lists, uint32_t* counts) {
+    if (!s || !cam || !lists || !counts || cap <= 0) return -1;
+    Scene sc{s, num_coeffs(s->sh_degree)};
+    auto surfels = prepare(sc, cam);
+    TileLists tl = build_tile_lists(surfels, cam);
+    const Window win = window_of(cam);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int yy = 0; yy < win.h; ++yy) {
+        for (int xx = 0; xx < win.w; ++xx) {
+            const int x = win.x0 + xx, y = win.y0 + yy;
+            const int64_t pi = (int64_t)yy * win.w + xx;
+            V3 d = pixel_ray(cam, x, y);
+            double pix_x = x + 0.5, pix_y = y + 0.5, transmittance = 1.0;
+            uint32_t cnt = 0;
+            walk_pixel(surfels, &tl, x, y, [&](int64_t li) {
+                const SurfelCam& sf = surfels[li];
+                Hit hit = evaluate(sf, d, pix_x, pix_y);
+                double raw = sf.opacity * hit.ghat;
+                double a = std::min(raw, ALPHA_CLAMP);
+                if (a < ALPHA_SKIP) return true;
+                if ((int64_t)cnt < cap)
+                    lists[pi * cap + cnt] = (uint32_t)sf.idx | (hit.ray_branch ? 1u << 30 : 0u) |
+                                            (raw > ALPHA_CLAMP ? 1u << 31 : 0u);
+                ++cnt;
+                transmittance *= 1.0 - a;
+                return !(transmittance < T_TERMINATE);
+            });
+            counts[pi] = cnt;
+        }
+    }
+    return 0;
+}
+
+// Decision replay (a test aid with no forward.rs counterpart): the
+// FP64 forward and backward along the discrete decisions another evaluation
+// took. For window pixel p (row-major over the camera's crop window),
+// lists[p * cap + k], k < counts[p], is its k-th contribution in compositing
+// order: surfel index (bits 0..29) | ray branch (bit 30) | alpha clamped at
+// 0.995 (bit 31). Each listed hit is evaluated in FP64 with the given branch
+// (no near-plane or det test: the other evaluation took the branch) and the
+// given clamp (alpha = 0.995 if clamped, else opacity * ghat with no skip
+// test); the walk ends with the list (the transmittance stop is replayed
+// too). The surfel's colour clamp stays FP64's. Gradients follow the
+// backward.rs:112-245 chain along the replayed walk, so they are the exact
+// derivative of the function the decisions define. color (P x 3) and alpha
+// (P) optional; dl_dcolor + g optional (P x 3). Returns -2 if a count
+// exceeds cap or a surfel index is out of range.
+int oref_replay(const oref_scene* s, const oref_camera* cam, const uint32_t* lists, const uint32_t* counts,
+                int64_t cap, double* color, double* alpha, const double* dl_dcolor, oref_grads* g) {
+    if (!s || !cam || !lists || !counts || cap <= 0) return -1;
+    Scene sc{s, num_coeffs(s->sh_degree)};
+    const int64_t n = s->n;
+    M3 r_cam = quat_to_mat(cam->quat);
+    std::vector<SurfelCam> all(n);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) all[i] = make_surfel_cam(sc, cam, r_cam, i);
+    const Window win = window_of(cam);
+    const int64_t P = (int64_t)win.w * win.h;
+    for (int64_t p = 0; p < P; ++p) {
+        if ((int64_t)counts[p] > cap) return -2;
+        for (uint32_t k = 0; k < counts[p]; ++k)
+            if ((int64_t)(lists[p * cap + k] & 0x3FFFFFFFu) >= n) return -2;
+    }
+    const bool grads = dl_dcolor != nullptr && g != nullptr;
+    int nthreads = threads_available();
+    std::vector<Workspace> ws(nthreads);
+    if (grads)
+        for (auto& w : ws) w.acc.assign(n, CamSpaceAcc());
+#pragma omp parallel for schedule(static, 1) num_threads(nthreads)
+    for (int yy = 0; yy < win.h; ++yy) {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        Workspace& w = ws[tid];
+        for (int xx = 0; xx < win.w; ++xx) {
+            const int x = win.x0 + xx, y = win.y0 + yy;
+            const int64_t pi = (int64_t)yy * win.w + xx;
+            V3 d = pixel_ray(cam, x, y);
+            V3 sv = -d;
+            double pix_x = x + 0.5, pix_y = y + 0.5;
+            w.contribs.clear();
+            double transmittance = 1.0, w_total = 0.0, wt_total = 0.0;
+            V3 c_acc;
+            for (uint32_t k = 0; k < counts[pi]; ++k) {
+                const uint32_t e = lists[pi * cap + k];
+                const int64_t id = (int64_t)(e & 0x3FFFFFFFu);
+                const bool ray = (e >> 30) & 1u, clamped = (e >> 31) & 1u;
+                const SurfelCam& sf = all[id];
+                // forward.rs:136-169 with the branch given.
+                Hit hit;
+                if (ray) {
+                    double det = triple(sf.tu, sf.tv, sv);
+                    double u = 0.0, v = 0.0, t = 0.0, gr = 0.0;
+                    if (det != 0.0) {
+                        u = triple(-sf.c, sf.tv, sv) / det;
+                        v = triple(sf.tu, -sf.c, sv) / det;
+                        t = triple(sf.tu, sf.tv, -sf.c) / det;
+                        gr = std::exp(-(u * u + v * v) * 0.5);
+                    }
+                    hit = {gr, true, u, v, t};
+                } else {
+                    double dx = pix_x - sf.px, dy = pix_y - sf.py;
+                    hit = {std::exp(-(dx * dx + dy * dy) / (2.0 * LOWPASS_SIGMA2)), false, 0.0, 0.0, sf.depth};
+                }
+                const double a = clamped ? ALPHA_CLAMP : sf.opacity * hit.ghat;
+                const double wgt = transmittance * a;
+                c_acc += sf.color * wgt;
+                w_total += wgt;
+                wt_total += wgt * hit.t;
+                w.contribs.push_back({id, a, clamped, transmittance, hit});
+                transmittance *= 1.0 - a;
+            }
+            if (color) {
+                color[3 * pi] = c_acc.x;
+                color[3 * pi + 1] = c_acc.y;
+                color[3 * pi + 2] = c_acc.z;
+            }
+            if (alpha) alpha[pi] = w_total;
+            if (grads) {
+                V3 dlc(dl_dcolor[3 * pi], dl_dcolor[3 * pi + 1], dl_dcolor[3 * pi + 2]);
+                if (!is_zero(dlc))
+                    reverse_walk(all, cam, d, pix_x, pix_y, w_total, wt_total, dlc, 0.0, V3(), 0.0, w);
+            }
+        }
+    }
+    if (!grads) return 0;
+    for (int t = 1; t < nthreads; ++t) {
+        for (int64_t i = 0; i < n; ++i) add_acc(ws[0].acc[i], ws[t].acc[i]);
+        ws[0].dfocal += ws[t].dfocal;
+    }
+    chain_to_params(all, ws[0].acc, sc, cam, g);
     g->focal = ws[0].dfocal;
     return 0;
 }

@@ -120,6 +120,8 @@ _PROTOS = {
     "gs_raster_set_digit_bits": ([_vp, _u32], C.c_int),
     "gs_raster_set_binning": ([_vp, _u32], C.c_int),
     "gs_raster_set_culling": ([_vp, C.c_int], C.c_int),
+    "gs_raster_set_trace": ([_vp, _u32, _u32, _u32, _u32, _u32], C.c_int),
+    "gs_raster_trace_buffers": ([_vp, C.POINTER(_vp), C.POINTER(_vp)], C.c_int),
     "gs_raster_forward": ([_vp, _vp, C.POINTER(RasterCamera), _u32], C.c_int),
     "gs_raster_backward": ([_vp, _vp, _u32], C.c_int),
     "gs_raster_backward_ex": ([_vp, _vp, _u32, _u32], C.c_int),

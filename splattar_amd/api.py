@@ -294,6 +294,21 @@ class Rasterizer:
         """Per-quadrant footprint culling in the rasterizer kernels (default on)."""
         check(lib().gs_raster_set_culling(self.h, 1 if enable else 0))
 
+    def set_trace(self, x0: int, y0: int, w: int, h: int, cap: int):
+        """Record the next forwards' contributions over a pixel window
+        (verification aid; cap 0 turns it off)."""
+        check(lib().gs_raster_set_trace(self.h, x0, y0, w, h, cap))
+        self._trace = (w, h, cap) if w * h * cap else None
+
+    def trace(self):
+        """(lists (h*w, cap) u32, counts (h*w,) u32) of the last traced forward."""
+        w, h, cap = self._trace
+        lp, cp = C.c_void_p(), C.c_void_p()
+        check(lib().gs_raster_trace_buffers(self.h, C.byref(lp), C.byref(cp)))
+        lists = DeviceBuffer(self.ctx, lp.value, np.uint32, (h * w, cap)).read()
+        counts = DeviceBuffer(self.ctx, cp.value, np.uint32, h * w).read()
+        return lists, counts
+
     def forward(self, camera: Camera, num_surfels: int, stream=None):
         rc = camera.c()
         check(lib().gs_raster_forward(self.h, stream, C.byref(rc), num_surfels))

@@ -336,35 +336,52 @@ __device__ inline void record_footprint(Record& r, const CamParams& cam) {
     const double f = cam.focal;
     const double det2 = Q[0][0] * Q[1][1] - Q[0][1] * Q[0][1];
     const double mag = Q[0][0] * Q[1][1] + Q[0][1] * Q[0][1];
-    bool bounded = Q[0][0] > 0.0 && det2 > 1e-9 * mag;
-    double ex = kFar, ey = kFar, h00 = 1.0, h01 = 0.0, h11 = 1.0;  // empty ellipse by default
+    // The per-pixel test runs in f32 on f32 coefficients, so a conic whose
+    // shape or scale sits at f32 rounding level is not the region the pixels
+    // see. Shape: det2 / mag is ~ the axis ratio squared; at 1e-6 the long
+    // axis already moves by a pixel under relative errors of ~1e-7. Scale:
+    // the conic's value at the centre, q_c, against the magnitude of its
+    // terms (|A_i A_j| + |B_i B_j| + 2L |M_i M_j| at (|csx|, |csy|, 1)); below
+    // 1e-7 of it q_c is rounding noise. Either way the ray part is treated as
+    // unbounded (box = the rect).
+    bool bounded = Q[0][0] > 0.0 && det2 > 1e-6 * mag;
+    double ex = kFar, ey = kFar, h00 = 1.0, h01 = 0.0, h11 = 1.0;
     if (bounded) {
-        const double a00 = Q[1][1] * Q[2][2] - Q[1][2] * Q[1][2];
-        const double a11 = Q[0][0] * Q[2][2] - Q[0][2] * Q[0][2];
-        const double a02 = Q[0][1] * Q[1][2] - Q[0][2] * Q[1][1];
-        const double a12 = Q[0][1] * Q[0][2] - Q[0][0] * Q[1][2];
-        const double dx = a02 * a02 - a00 * det2, dy = a12 * a12 - a11 * det2;
-        bounded = dx >= 0.0 && dy >= 0.0;
-        if (bounded) {
-            const double sdx = sqrt(dx), sdy = sqrt(dy);
-            const double sx_lo = (a02 - sdx) / det2, sx_hi = (a02 + sdx) / det2;
-            const double sy_lo = (a12 - sdy) / det2, sy_hi = (a12 + sdy) / det2;
-            // X = cx - f sx (pixel centre), Y = cy + f sy.
-            x0 = fmin(x0, cam.cx - f * sx_hi - 1.0);
-            x1 = fmax(x1, cam.cx - f * sx_lo + 1.0);
-            y0 = fmin(y0, cam.cy + f * sy_lo - 1.0);
-            y1 = fmax(y1, cam.cy + f * sy_hi + 1.0);
-        }
         const double csx = -(Q[1][1] * Q[0][2] - Q[0][1] * Q[1][2]) / det2;
         const double csy = -(Q[0][0] * Q[1][2] - Q[0][1] * Q[0][2]) / det2;
         const double qc = Q[2][2] + Q[0][2] * csx + Q[1][2] * csy;
-        if (bounded && qc < 0.0) {
-            const double sc = 1.0 / (-qc * f * f);
-            ex = cam.cx - f * csx;
-            ey = cam.cy + f * csy;
-            h00 = Q[0][0] * sc;
-            h01 = -Q[0][1] * sc;
-            h11 = Q[1][1] * sc;
+        const double sc3[3] = {fabs(csx), fabs(csy), 1.0};
+        double qabs = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                qabs += sc3[i] * sc3[j] *
+                        (fabs(A[i] * A[j]) + fabs(B[i] * B[j]) + k * fabs(M[i] * M[j]));
+        bounded = qc < -1e-7 * qabs;
+        if (bounded) {
+            const double a00 = Q[1][1] * Q[2][2] - Q[1][2] * Q[1][2];
+            const double a11 = Q[0][0] * Q[2][2] - Q[0][2] * Q[0][2];
+            const double a02 = Q[0][1] * Q[1][2] - Q[0][2] * Q[1][1];
+            const double a12 = Q[0][1] * Q[0][2] - Q[0][0] * Q[1][2];
+            const double dx = a02 * a02 - a00 * det2, dy = a12 * a12 - a11 * det2;
+            bounded = dx >= 0.0 && dy >= 0.0;
+            if (bounded) {
+                const double sdx = sqrt(dx), sdy = sqrt(dy);
+                const double sx_lo = (a02 - sdx) / det2, sx_hi = (a02 + sdx) / det2;
+                const double sy_lo = (a12 - sdy) / det2, sy_hi = (a12 + sdy) / det2;
+                // X = cx - f sx (pixel centre), Y = cy + f sy.
+                x0 = fmin(x0, cam.cx - f * sx_hi - 1.0);
+                x1 = fmax(x1, cam.cx - f * sx_lo + 1.0);
+                y0 = fmin(y0, cam.cy + f * sy_lo - 1.0);
+                y1 = fmax(y1, cam.cy + f * sy_hi + 1.0);
+                const double sc = 1.0 / (-qc * f * f);
+                ex = cam.cx - f * csx;
+                ey = cam.cy + f * csy;
+                h00 = Q[0][0] * sc;
+                h01 = -Q[0][1] * sc;
+                h11 = Q[1][1] * sc;
+            }
         }
     }
     const bool finite = x0 == x0 && y0 == y0 && x1 == x1 && y1 == y1 && ex == ex && ey == ey &&
@@ -592,6 +609,18 @@ struct FwdOut {
     float4* normal;    // sum w n
     uint32_t* ncontrib;
 };
+// Decision trace of the forward (a verification aid): for each pixel of the
+// window [x0, x0 + w) x [y0, y0 + h) of view `view`, the contributing
+// records in compositing order, lists[p * cap + k] = record index (bits
+// 0..29) | ray branch (bit 30) | alpha clamped (bit 31), and counts[p] = the
+// number of contributions (entries past cap are counted, not stored).
+// lists == nullptr: no trace.
+struct FwdTrace {
+    uint32_t* lists = nullptr;
+    uint32_t* counts = nullptr;
+    uint32_t x0 = 0, y0 = 0, w = 0, h = 0, cap = 0, view = 0;
+};
+constexpr uint32_t kTraceRecordMask = (1u << 30) - 1u;
 // Tile scheduling for the raster passes: with a queue (cursor != nullptr),
 // grid_blocks resident workgroups pull tile positions from kTileQueues
 // cursors (which must be 0 at launch): workgroup b uses cursor
@@ -623,7 +652,8 @@ hipError_t launch_raster_fwd(hipStream_t st, const CamParams* cams_dev, const Ca
                              uint32_t n_views, uint32_t total_tiles, const uint32_t* ranges,
                              const uint32_t* payloads, const Record* recs, FwdOut out,
                              bool cull = true, TileQueue queue = {nullptr, nullptr, 0},
-                             bool aux_outputs = true, RasterShape shape = {}, uint32_t tile_begin = 0);
+                             bool aux_outputs = true, RasterShape shape = {}, uint32_t tile_begin = 0,
+                             FwdTrace trace = {});
 
 // raster_bwd.hip
 struct BwdIn {

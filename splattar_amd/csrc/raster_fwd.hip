@@ -34,12 +34,14 @@ __device__ __forceinline__ uint32_t view_of_tile(const CamParams* cams, uint32_t
 }
 
 // AUX = false (training with colour losses only) skips the depth and normal
-// accumulators: aux = (T_final, 0, 0, 0), normal is not written.
-template <bool AUX, int TW, int BATCH>
+// accumulators: aux = (T_final, 0, 0, 0), normal is not written. TRACE
+// additionally records every contribution of the trace window's pixels
+// (FwdTrace; a verification variant, never used by training steps).
+template <bool AUX, int TW, int BATCH, bool TRACE = false>
 __global__ __launch_bounds__((TW / 8) * (TW / 8) * 64) void raster_fwd_kernel(
     const CamParams* __restrict__ cams, uint32_t n_views, const uint32_t* __restrict__ ranges,
     const uint32_t* __restrict__ payloads, const Record* __restrict__ recs, FwdOut out, bool cull,
-    uint32_t total_tiles, TileQueue queue, uint32_t tile_begin) {
+    uint32_t total_tiles, TileQueue queue, uint32_t tile_begin, FwdTrace trace) {
     constexpr int NC = TW / 8;             // blocks per tile row
     constexpr int NB = NC * NC;            // blocks (= waves) per tile
     constexpr int NT = NB * 64;            // threads
@@ -88,6 +90,12 @@ __global__ __launch_bounds__((TW / 8) * (TW / 8) * 64) void raster_fwd_kernel(
         float cr = 0.f, cg = 0.f, cb = 0.f, wsum = 0.f, dsum = 0.f, nx = 0.f, ny = 0.f, nz = 0.f;
         uint32_t last = 0;
         bool done = !inside;
+        // Trace window slot of this pixel (TRACE only).
+        const bool traced = TRACE && inside && s_view == trace.view && (uint32_t)px >= trace.x0 &&
+                            (uint32_t)px < trace.x0 + trace.w && (uint32_t)py >= trace.y0 &&
+                            (uint32_t)py < trace.y0 + trace.h;
+        const size_t tslot = traced ? (size_t)(py - trace.y0) * trace.w + (size_t)(px - trace.x0) : 0;
+        uint32_t tcount = 0;
 
         for (uint32_t base = start; base < end; base += BATCH) {
             const uint32_t cnt = min((uint32_t)BATCH, end - base);
@@ -162,6 +170,13 @@ __global__ __launch_bounds__((TW / 8) * (TW / 8) * 64) void raster_fwd_kernel(
                         T *= 1.0f - h.alpha;
                         last = base - start + j + 1u;
                         if (T < kTTerminate) done = true;
+                        if (TRACE && traced) {
+                            if (tcount < trace.cap)
+                                trace.lists[tslot * trace.cap + tcount] =
+                                    (payloads[base + j] & kTraceRecordMask) | (h.ray ? 1u << 30 : 0u) |
+                                    (h.clamped ? 1u << 31 : 0u);
+                            ++tcount;
+                        }
                     }
                 };
                 // Two records per round: both list entries and both records
@@ -191,18 +206,19 @@ __global__ __launch_bounds__((TW / 8) * (TW / 8) * 64) void raster_fwd_kernel(
             out.aux[pi] = make_float4(T, dsum, 0.0f, 0.0f);
             if (AUX) out.normal[pi] = make_float4(nx, ny, nz, 0.0f);
             out.ncontrib[pi] = last;
+            if (TRACE && traced) trace.counts[tslot] = tcount;
         }
         __syncthreads();  // s_view and the LDS batch are reused by the next tile
     }
 }
 
-template <bool AUX, int TW, int BATCH>
+template <bool AUX, int TW, int BATCH, bool TRACE = false>
 void launch_fwd(hipStream_t st, uint32_t grid, const CamParams* cams_dev, uint32_t n_views,
                 uint32_t total_tiles, const uint32_t* ranges, const uint32_t* payloads, const Record* recs,
-                FwdOut out, bool cull, TileQueue queue, uint32_t tile_begin) {
+                FwdOut out, bool cull, TileQueue queue, uint32_t tile_begin, FwdTrace trace) {
     constexpr int NT = (TW / 8) * (TW / 8) * 64;
-    hipLaunchKernelGGL((raster_fwd_kernel<AUX, TW, BATCH>), dim3(grid), dim3(NT), 0, st, cams_dev, n_views,
-                       ranges, payloads, recs, out, cull, total_tiles, queue, tile_begin);
+    hipLaunchKernelGGL((raster_fwd_kernel<AUX, TW, BATCH, TRACE>), dim3(grid), dim3(NT), 0, st, cams_dev,
+                       n_views, ranges, payloads, recs, out, cull, total_tiles, queue, tile_begin, trace);
 }
 
 }  // namespace
@@ -210,15 +226,20 @@ void launch_fwd(hipStream_t st, uint32_t grid, const CamParams* cams_dev, uint32
 hipError_t launch_raster_fwd(hipStream_t st, const CamParams* cams_dev, const CamParams* /*cams_host*/,
                              uint32_t n_views, uint32_t total_tiles, const uint32_t* ranges,
                              const uint32_t* payloads, const Record* recs, FwdOut out, bool cull,
-                             TileQueue queue, bool aux_outputs, RasterShape shape, uint32_t tile_begin) {
+                             TileQueue queue, bool aux_outputs, RasterShape shape, uint32_t tile_begin,
+                             FwdTrace trace) {
     if (total_tiles == 0) return hipSuccess;
     const uint32_t grid = queue.cursor != nullptr ? std::min(total_tiles, std::max(queue.grid_blocks, 1u))
                                                   : total_tiles;
     const uint32_t t = shape.tile, b = shape.fwd_batch;
     auto go = [&](auto fn) {
-        fn(st, grid, cams_dev, n_views, total_tiles, ranges, payloads, recs, out, cull, queue, tile_begin);
+        fn(st, grid, cams_dev, n_views, total_tiles, ranges, payloads, recs, out, cull, queue, tile_begin, trace);
     };
-    if (aux_outputs) {
+    if (trace.lists != nullptr) {
+        // The trace variant: depth / normal outputs, 16-px tiles, 256-record batches.
+        if (!aux_outputs || t != 16 || b != 256 || trace.counts == nullptr) return hipErrorInvalidValue;
+        go(launch_fwd<true, 16, 256, true>);
+    } else if (aux_outputs) {
         if (t != 16 || b != 256) return hipErrorInvalidValue;  // depth / normal outputs: 16-px tiles
         go(launch_fwd<true, 16, 256>);
     } else if (t == 8 && b == 64) {

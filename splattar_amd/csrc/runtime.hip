@@ -596,6 +596,8 @@ struct RasterCore {
     uint32_t* ncontrib = nullptr;
     float* grad_coef = nullptr;
     float* cam_partials = nullptr;
+    // Decision trace of the forward (verification aid; lists == nullptr: off).
+    FwdTrace trace;
     // Depth-presort path (binning = GS_BINNING_PRESORT or GS_BINNING_TIGHT).
     uint32_t* pre_keys[2] = {nullptr, nullptr};
     uint32_t* pre_vals[2] = {nullptr, nullptr};
@@ -854,7 +856,7 @@ struct RasterCore {
             GS_CHECK(schedule_range(st, 0, 0, tiles));
             GS_CHECK(P.mark(st, GS_STAGE_RASTER_FWD));
             GS_HIP(launch_raster_fwd(st, cd, cams.data() + v0, nv, tiles, ranges, pays[result_alt], recs, o,
-                                     cull, queue(0, 0), aux_outputs, shape));
+                                     cull, queue(0, 0), aux_outputs, shape, 0, trace));
             return GS_OK;
         }
         // 64-bit (tile || depth) key path: one sort over every key bit that varies.
@@ -879,7 +881,7 @@ struct RasterCore {
         GS_CHECK(schedule_range(st, 0, 0, tiles));
         GS_CHECK(P.mark(st, GS_STAGE_RASTER_FWD));
         GS_HIP(launch_raster_fwd(st, cd, cams.data() + v0, nv, tiles, ranges, pays[result_alt], recs, o,
-                                 cull, queue(0, 0), aux_outputs, shape));
+                                 cull, queue(0, 0), aux_outputs, shape, 0, trace));
         return GS_OK;
     }
 
@@ -982,7 +984,11 @@ struct gs_raster_s {
     uint32_t max_views = 1;
     uint32_t last_views = 0;   // views of the last forward (0: none)
     bool last_batch = false;   // the last forward was gs_raster_forward_batch
+    uint32_t* trace_lists = nullptr;
+    uint32_t* trace_counts = nullptr;
     ~gs_raster_s() {
+        dfree(trace_lists);
+        dfree(trace_counts);
         dfree(positions);
         dfree(scales);
         dfree(quats);
@@ -1118,6 +1124,38 @@ gs_status gs_raster_set_digit_bits(gs_raster r, uint32_t digit_bits) {
 gs_status gs_raster_set_culling(gs_raster r, int enable) {
     GS_REQUIRE(r != nullptr, "raster is null");
     r->core.cull = enable != 0;
+    return GS_OK;
+}
+
+gs_status gs_raster_set_trace(gs_raster r, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t cap) {
+    GS_REQUIRE(r != nullptr, "raster is null");
+    dfree(r->trace_lists);
+    dfree(r->trace_counts);
+    r->core.trace = FwdTrace{};
+    if ((uint64_t)w * h * cap == 0) return GS_OK;
+    GS_REQUIRE((uint64_t)x0 + w <= r->width && (uint64_t)y0 + h <= r->height,
+               "trace window must lie inside the image");
+    GS_REQUIRE((uint64_t)w * h * cap <= (1ull << 30), "trace window x cap must be <= 2^30 entries");
+    GS_CHECK(dalloc(&r->trace_lists, (size_t)w * h * cap, "trace lists"));
+    GS_CHECK(dalloc(&r->trace_counts, (size_t)w * h, "trace counts"));
+    FwdTrace t;
+    t.lists = r->trace_lists;
+    t.counts = r->trace_counts;
+    t.x0 = x0;
+    t.y0 = y0;
+    t.w = w;
+    t.h = h;
+    t.cap = cap;
+    t.view = 0;
+    r->core.trace = t;
+    return GS_OK;
+}
+
+gs_status gs_raster_trace_buffers(gs_raster r, const uint32_t** lists, const uint32_t** counts) {
+    GS_REQUIRE(r && lists && counts, "null argument");
+    GS_HIP(hipStreamSynchronize(r->ctx->stream));
+    *lists = r->trace_lists;
+    *counts = r->trace_counts;
     return GS_OK;
 }
 

@@ -121,3 +121,30 @@ def total_loss(s64, specs, tgts, grad=False, counts=None):
             for f in ("pos", "scales", "quat", "opacity", "sh"):
                 getattr(acc, f)[:m] += getattr(g, f)
     return total, acc
+
+
+def replay_conditioning(s64, cam, lists, counts, dl, g_ref, img_ref, draws=2, rel=1e-5, seed=0):
+    """Conditioning of the replayed FP64 evaluation: every input is scaled by
+    1 +- 2^-24 (random signs) and replayed along the same decision lists
+    (oracle.replay), `draws` times. Returns (ill, pixel_sens):
+    - ill[i]: some class of surfel i's gradient moved by more than rel x
+      max(its own norm, 1e-3 x the class's largest surfel norm);
+    - pixel_sens (H, W): the largest change of the pixel's colour.
+    Neither is determined better than this by f32 inputs, whatever
+    evaluates them."""
+    n = s64.n
+    rng = np.random.default_rng(seed)
+    ill = np.zeros(n, bool)
+    sens = np.zeros(img_ref.shape[:2])
+    for _ in range(draws):
+        sp = s64.copy()
+        for arr in (sp.pos, sp.scales, sp.quat, sp.opacity, sp.sh):
+            arr *= 1.0 + rng.choice([-1.0, 1.0], arr.shape) * 2.0 ** -24
+        img2, _, g2 = oracle.replay(sp, cam, lists, counts, dl)
+        sens = np.maximum(sens, np.abs(img2 - img_ref).max(axis=2))
+        for k in ("pos", "scales", "quat", "opacity", "sh"):
+            a = getattr(g_ref, k).reshape(n, -1)
+            b = getattr(g2, k).reshape(n, -1)
+            own = np.linalg.norm(a, axis=1)
+            ill |= np.linalg.norm(b - a, axis=1) > rel * np.maximum(own, 1e-3 * own.max())
+    return ill, sens

@@ -235,3 +235,190 @@ def test_config2_uneven_batch_full_size(gpu):
         nrm = float(np.linalg.norm(gg[k] - gp[k]) / np.linalg.norm(gp[k]))
         print(f"  {k}: graph vs plain normwise {nrm:.2e}")
         assert nrm <= 1e-5, k
+
+
+def _c3_problem():
+    """The headline configuration: 1M surfels, SH3, scales exp(N(-5, 1)), the
+    first four of the 16 sphere views at 1920x1080 (the views of the first
+    training step), targets from the centre-jittered copy."""
+    from splattar_amd import scenes
+    sc = scenes.config_scene(3)
+    specs = scenes.sphere_views(16, scenes.CONFIG_SEED + 3, [(1920, 1080)])[:4]
+    tgt = scenes.jittered(sc, 0.01, (scenes.CONFIG_SEED + 3) ^ 1)
+    return sc, specs, tgt
+
+
+def _cam32(sp, deg):
+    import splattar_amd as sa
+    return sa.Camera(np.asarray(sp.center, np.float32), np.asarray(sp.quat, np.float32),
+                     float(np.float32(sp.focal)), deg)
+
+
+def test_config3_view0_keys_bit_exact(gpu):
+    """C3 view 0: rects, depth keys, the sorted 64-bit (tile << 32 | depth)
+    keys with their payloads and the tile ranges equal the fp32 mirror's bit
+    for bit under both exact binnings (SORT64 and PRESORT)."""
+    import splattar_amd as sa
+    sc, specs, _ = _c3_problem()
+    sp = specs[0]
+    rot = oracle.quat_to_mat(np.asarray(sp.quat, np.float32).astype(np.float64))
+    c32 = oracle.cam32(rot, np.asarray(sp.center, np.float32), float(np.float32(sp.focal)), sp.width,
+                       sp.height)
+    pos4 = np.concatenate([sc.positions, np.ones((sc.n, 1), np.float32)], 1)
+    rects, keys, _ = oracle.prep_f32(pos4, sc.scales, sc.quats, c32)
+    k, p, e = oracle.emit_keys(rects, keys, c32.tiles_x)
+    k, p = oracle.stable_sort(k, p)
+    ranges = oracle.tile_ranges(k, c32.tiles_x * c32.tiles_y)
+    print(f"  mirror: {e} entries over {c32.tiles_x * c32.tiles_y} tiles")
+    for mode in (sa._lib.GS_BINNING_SORT64, sa._lib.GS_BINNING_PRESORT):
+        announce(f"config 3: view 0 keys, binning {mode}")
+        r = sa.Rasterizer(gpu, sc.n, sp.width, sp.height, sc.n * 32)
+        r.upload_scene(sc.positions, sc.scales, sc.quats, sc.opacities, sc.sh)
+        r.set_binning(mode)
+        r.forward(_cam32(sp, 3), sc.n)
+        d = r.debug()
+        r.close()
+        assert np.array_equal(d["rects"][:sc.n], rects) and np.array_equal(d["depth_keys"][:sc.n], keys)
+        assert d["entries"] == e
+        assert np.array_equal(d["sorted_keys"], k) and np.array_equal(d["sorted_payloads"], p)
+        assert np.array_equal(d["ranges"], ranges)
+
+
+def test_config3_tight_binning_and_culling_are_exact(gpu):
+    """C3, all four views: the training binning (TIGHT, footprint culling on)
+    renders bit-identical images to the reference binning (PRESORT) and to
+    TIGHT with culling off; ncontrib is identical with culling on and off."""
+    import splattar_amd as sa
+    sc, specs, _ = _c3_problem()
+    for v, sp in enumerate(specs):
+        announce(f"config 3: view {v} tight / presort / culling off")
+        out = {}
+        for name, mode, cull in (("presort", sa._lib.GS_BINNING_PRESORT, True),
+                                 ("tight", sa._lib.GS_BINNING_TIGHT, True),
+                                 ("tight_nocull", sa._lib.GS_BINNING_TIGHT, False)):
+            r = sa.Rasterizer(gpu, sc.n, sp.width, sp.height, sc.n * 32)
+            r.upload_scene(sc.positions, sc.scales, sc.quats, sc.opacities, sc.sh)
+            r.set_binning(mode)
+            r.set_culling(cull)
+            r.forward(_cam32(sp, 3), sc.n)
+            out[name] = (r.out_color.read().copy(), r.out_aux.read().copy(), r.out_ncontrib.read().copy(),
+                         r.entry_count())
+            r.close()
+        print(f"  view {v}: entries presort {out['presort'][3]} tight {out['tight'][3]}")
+        assert out["tight"][3] < out["presort"][3]
+        for j in (0, 1):
+            assert np.array_equal(out["tight"][j], out["presort"][j])
+            assert np.array_equal(out["tight"][j], out["tight_nocull"][j])
+        assert np.array_equal(out["tight"][2], out["tight_nocull"][2])
+
+
+def test_config3_crop_matches_fp64(gpu):
+    """C3 view 0 against FP64 on the centred 480x270 crop. The GPU renders the
+    whole 1080p view with the training binning (TIGHT, culling on) and records
+    its decisions over the crop (gs_raster_set_trace); the backward gets
+    dL/dimage = the FP64 crop loss gradient inside the window, 0 outside.
+    - Image against FP64 replaying the GPU's decisions (same hits, branches
+      and clamps): max |d| <= 1e-5 on every pixel.
+    - Image against faithful FP64: a pixel above 1e-5 must be one whose
+      decision list differs between the two (counted and reported).
+    - Gradients against the replayed FP64 gradients, per class normwise
+      <= 1e-4 over the surfels whose replayed FP64 gradient is well
+      conditioned (moves by <= 1e-5 relative under a one-ulp perturbation of
+      every input with the decisions held); the excluded count is reported."""
+    import splattar_amd as sa
+    import multiview as mv
+    sc, specs, _ = _c3_problem()
+    sp = specs[0]
+    W, H, cw, ch = sp.width, sp.height, 480, 270
+    x0, y0 = (W - cw) // 2, (H - ch) // 2
+    s64 = _scene64(sc)
+    cam = _camera64(sp)
+    cam.crop = (x0, y0, cw, ch)
+    announce("config 3: FP64 crop render and loss gradient")
+    img64 = oracle.render(s64, cam)["color"]
+    # Target: the render + 0.01 (no clamp), so sign(render - target) = -1.
+    _, dl64, _, _ = oracle.l1_ssim(img64, img64 + 0.01, LAM, DSSIM_SCALE)
+    announce("config 3: GPU forward (traced) + backward, tight binning")
+    cap = 1024
+    r = sa.Rasterizer(gpu, sc.n, W, H, sc.n * 32)
+    r.upload_scene(sc.positions, sc.scales, sc.quats, sc.opacities, sc.sh)
+    r.set_binning(sa._lib.GS_BINNING_TIGHT)
+    r.set_trace(x0, y0, cw, ch, cap)
+    r.forward(_cam32(sp, 3), sc.n)
+    lists, counts = r.trace()
+    img = r.out_color.read()[y0:y0 + ch, x0:x0 + cw, :3].astype(np.float64)
+    dl = np.zeros((H, W, 4), np.float32)
+    dl[y0:y0 + ch, x0:x0 + cw, :3] = dl64
+    r.dl_dcolor.write(dl)
+    r.backward(sc.n, flags=sa._lib.GS_BWD_COLOR_ONLY)
+    got = dict(pos=r.grad_pos.read()[:, :3].copy(), scales=r.grad_scales.read().copy(),
+               quat=r.grad_quat.read().copy(), opacity=r.grad_opacity.read().copy(),
+               sh=r.grad_sh.read()[:, :48].copy())
+    r.close()
+    print(f"  contributions per pixel: mean {counts.mean():.1f}, max {counts.max()} (cap {cap})")
+    assert counts.max() <= cap
+    announce("config 3: FP64 replay of the GPU decisions and its conditioning")
+    dl_win = dl[y0:y0 + ch, x0:x0 + cw, :3].astype(np.float64)
+    img_r, _, g_r = oracle.replay(s64, cam, lists, counts, dl_win)
+    ill, sens = mv.replay_conditioning(s64, cam, lists, counts, dl_win, g_r, img_r)
+    err_r = np.abs(img - img_r).max(axis=2)
+    cond = sens <= 1e-6
+    print(f"  image vs replayed FP64: max {err_r.max():.2e}, median {np.median(err_r):.2e}; pixels whose "
+          f"replayed value moves > 1e-6 under a one-ulp input perturbation: {int((~cond).sum())} of "
+          f"{cond.size}; max over the rest {err_r[cond].max():.2e}, max over those {err_r[~cond].max():.2e}")
+    # Faithful FP64: differences above 1e-5 only where the decisions differ
+    # (or at an ill-conditioned pixel).
+    fl, fc = oracle.trace(s64, cam, cap)
+    same = (fc == counts) & np.all(np.where(np.arange(cap)[None, :] < np.minimum(counts, cap)[:, None],
+                                            fl == lists, True), axis=1)
+    flipped = ~same.reshape(ch, cw)
+    err_f = np.abs(img - img64).max(axis=2)
+    unexplained = (err_f > 1e-5) & ~flipped & cond
+    print(f"  image vs faithful FP64: max {err_f.max():.2e}; pixels above 1e-5 {int((err_f > 1e-5).sum())}, "
+          f"pixels whose decisions differ {int(flipped.sum())} of {flipped.size}, above 1e-5 with the same "
+          f"decisions at a well-conditioned pixel {int(unexplained.sum())}")
+    print(f"  surfels excluded as ill-conditioned: {int(ill.sum())} of {sc.n}")
+    nrms = {}
+    for key in ("pos", "scales", "quat", "opacity", "sh"):
+        want = getattr(g_r, key).reshape(sc.n, -1)[~ill]
+        have = got[key].reshape(sc.n, -1).astype(np.float64)[~ill]
+        nrms[key] = float(np.linalg.norm(have - want) / np.linalg.norm(want))
+        print(f"  {key}: normwise rel {nrms[key]:.2e} against the replayed FP64")
+    assert err_r[cond].max() <= 1e-5 and cond.mean() > 0.999
+    assert not np.any(unexplained)
+    assert ill.mean() < 0.01
+    for key, v in nrms.items():
+        assert v <= 1e-4, key
+
+
+def test_config3_graph_step_equals_plain(gpu):
+    """C3 one training step over its four views (learning rates 0, gradients
+    kept): the batched graph step's summed parameter gradients equal the plain
+    per-view step's to 1e-5 normwise per class, losses to 1e-6."""
+    import splattar_amd as sa
+    sc, specs, tgt = _c3_problem()
+    targets = []
+    for sp in specs:
+        r = sa.Rasterizer(gpu, sc.n, sp.width, sp.height, sc.n * 32)
+        r.upload_scene(tgt.positions, tgt.scales, tgt.quats, tgt.opacities, tgt.sh)
+        r.forward(_cam32(sp, 3), sc.n)
+        targets.append(r.out_color.read().copy())
+        r.close()
+    cfg = sa.TrainConfig(lam=LAM, dssim_scale=DSSIM_SCALE, lr_pos=0.0, lr_scales=0.0, lr_quat=0.0,
+                         lr_opacity=0.0, lr_sh=0.0, max_views_per_step=4, entries_per_view=sc.n * 24)
+    views = [(_cam32(sp, 3), t) for sp, t in zip(specs, targets)]
+    res = {}
+    for mode in (sa.Trainer.GRAPH, sa.Trainer.PLAIN):
+        announce(f"config 3: one step, mode {mode}")
+        t = sa.Trainer(gpu, cfg, sc.as_init(), views)
+        t.set_keep_grads(True)
+        t.step([0, 1, 2, 3], mode=mode)
+        t.entry_count()  # raises on overflow
+        res[mode] = (_trainer_grads(t, sc.n, 16), t.last_losses(4))
+        t.close()
+    (gg, lg), (gp, lp) = res[sa.Trainer.GRAPH], res[sa.Trainer.PLAIN]
+    np.testing.assert_allclose(lg, lp, rtol=1e-6)
+    for key in gg:
+        nrm = float(np.linalg.norm(gg[key] - gp[key]) / np.linalg.norm(gp[key]))
+        print(f"  {key}: graph vs plain normwise {nrm:.2e}")
+        assert nrm <= 1e-5, key

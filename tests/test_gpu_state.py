@@ -215,8 +215,9 @@ def test_trainer_resume_continues_the_run(gpu):
 def test_reads_after_a_step_on_a_caller_stream(gpu):
     """Steps enqueued on a caller's stream: the blocking loss read, the entry
     count and a render wait for them (the trainer records an event on the
-    step's stream)."""
-    import torch
+    step's stream). The stream comes from the HIP runtime the library itself
+    links, default priority, non-blocking."""
+    import ctypes
 
     import splattar_amd as sa
     deg = 2
@@ -227,18 +228,21 @@ def test_reads_after_a_step_on_a_caller_stream(gpu):
     views = [(sa.Camera(sp.center, sp.quat, sp.focal, deg), t) for sp, t in zip(specs, targets)]
     ref = sa.Trainer(gpu, cfg, sc.as_init(), views)
     t = sa.Trainer(gpu, cfg, sc.as_init(), views)
-    side = torch.cuda.Stream(device=0)
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    side = ctypes.c_void_p()
+    assert hip.hipStreamCreateWithFlags(ctypes.byref(side), 1) == 0
     announce("steps on a caller stream, then reads")
     for it in range(4):
         ids = [it % 3, (it + 1) % 3, (it + 2) % 3]
         ref.step(ids, mode=ref.GRAPH)
-        t.step(ids, mode=t.GRAPH, stream=side.cuda_stream)
+        t.step(ids, mode=t.GRAPH, stream=side.value)
     # No synchronisation of `side` by the caller before the reads.
     l_t, e_t, img_t = t.last_losses(3), t.entry_count(), t.render(1)
     l_r, e_r, img_r = ref.last_losses(3), ref.entry_count(), ref.render(1)
     np.testing.assert_allclose(l_t, l_r, rtol=1e-4)
     assert e_t == e_r
     assert np.abs(img_t - img_r).max() < 1e-4
-    side.synchronize()
+    assert hip.hipStreamSynchronize(side) == 0
     t.close()
     ref.close()
+    assert hip.hipStreamDestroy(side) == 0

@@ -369,3 +369,79 @@ def test_multiview_loss_gradient_matches_central_differences():
         assert err <= 1e-7 + 1e-3 * denom, f"{name}[{i},{c}]: analytic {an:.9e} fd {fd:.9e}"
     print(f"worst relative FD disagreement {worst:.2e}; jumps skipped {len(jumps)}: {jumps}")
     assert len(jumps) <= 8
+
+
+@pytest.mark.parametrize("seed,n,deg,size", [(101, 20, 0, 64), (303, 12, 3, 48), (404, 50, 1, 100)])
+def test_replay_of_the_faithful_decisions_is_the_faithful_evaluation(seed, n, deg, size):
+    """Decision replay fed the FP64 walk's own decisions (oracle.trace)
+    reproduces the faithful render and every parameter gradient: it takes the
+    same hits in the same order with the same branch and clamp."""
+    scene, cam = make_scene(seed, n, deg, size, flavor="parity")
+    lists, counts = oracle.trace(scene, cam, 256)
+    assert counts.max() <= 256 and counts.sum() > 0
+    ref = oracle.render(scene, cam)
+    w = make_weights(seed ^ 0xFEED, size * size).reshape(size, size, 3)
+    g = oracle.gradients(scene, cam, w)
+    col, alpha, gr = oracle.replay(scene, cam, lists, counts, w)
+    assert np.abs(col - ref["color"]).max() <= 1e-14
+    assert np.abs(alpha - ref["alpha"]).max() <= 1e-14
+    for k in ("pos", "scales", "quat", "opacity", "sh"):
+        np.testing.assert_allclose(getattr(gr, k), getattr(g, k), rtol=1e-12, atol=1e-15, err_msg=k)
+    np.testing.assert_allclose(gr.cam_center, g.cam_center, rtol=1e-10, atol=1e-13)
+
+
+def test_replay_follows_the_given_decisions():
+    """Replay uses the lists, not its own decisions: flipping one hit's branch
+    bit, setting its clamp bit or dropping the tail of a list changes the
+    replayed image at that pixel and only there."""
+    scene, cam = make_scene(303, 12, 3, 48, flavor="parity")
+    lists, counts = oracle.trace(scene, cam, 256)
+    base, _, _ = oracle.replay(scene, cam, lists, counts)
+    p = int(np.argmax(counts))
+    assert counts[p] >= 2
+    y, x = divmod(p, 48)
+    for what in ("branch", "clamp", "truncate"):
+        l2, c2 = lists.copy(), counts.copy()
+        if what == "branch":
+            l2[p, 0] ^= 1 << 30
+        elif what == "clamp":
+            l2[p, 0] |= np.uint32(1 << 31)
+        else:
+            c2[p] -= 1
+        img, _, _ = oracle.replay(scene, cam, l2, c2)
+        diff = np.abs(img - base).max(axis=2)
+        assert diff[y, x] > 1e-6, what
+        diff[y, x] = 0.0
+        assert diff.max() == 0.0, what
+
+
+@pytest.mark.parametrize("seed,n,deg,size", [(111, 15, 1, 48), (222, 15, 3, 48)])
+def test_replay_gradients_match_central_differences_with_the_decisions_fixed(seed, n, deg, size):
+    """With the decisions fixed the replayed loss is smooth in every
+    parameter, so central differences of it (h = 1e-6, gradcheck.rs's
+    1e-7 + 1e-3 denom) pin the replay's analytic gradients without the
+    discontinuities a faithful FD must avoid."""
+    scene, cam = make_scene(seed, n, deg, size, flavor="parity")
+    lists, counts = oracle.trace(scene, cam, 256)
+    w = make_weights(seed ^ 0xFEED, size * size).reshape(size, size, 3)
+    _, _, g = oracle.replay(scene, cam, lists, counts, w)
+
+    def loss(s):
+        return float((oracle.replay(s, cam, lists, counts)[0] * w).sum())
+
+    rng = np.random.default_rng(seed)
+    h = 1e-6
+    checked = 0
+    for cls, arr_g in (("pos", g.pos), ("scales", g.scales), ("quat", g.quat), ("opacity", g.opacity),
+                       ("sh", g.sh)):
+        base = getattr(scene, cls)
+        for _ in range(6):
+            idx = tuple(int(rng.integers(0, d)) for d in base.shape)
+            sp, sm = scene.copy(), scene.copy()
+            getattr(sp, cls)[idx] += h
+            getattr(sm, cls)[idx] -= h
+            fd = (loss(sp) - loss(sm)) / (2 * h)
+            an = float(arr_g[idx])
+            assert abs(fd - an) <= 1e-7 + 1e-3 * max(abs(fd), abs(an)), (cls, idx, fd, an)
+            checked += 1
+    assert checked == 30
