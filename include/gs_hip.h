@@ -302,6 +302,33 @@ gs_status gs_loss_encode(gs_loss l, void* stream);   /* SsimLoss::encode (ssim.r
  * dssim_mean = dssim_scale * mean(1 - SSIM). */
 gs_status gs_loss_read(gs_loss l, double* l1_mean, double* dssim_mean);
 
+/* --------------------------------------------------------------- NormalLoss
+ * first_attempt/crates/gs-train/src/normal_loss.rs + shaders/normal_loss.wgsl:
+ * 52-145: 2DGS normal consistency, L = lambda/(W H) * sum_q alpha_q
+ * (1 - nhat_q . N_d(q)) with N_d from forward differences of the mean-depth
+ * points (alpha and the orientation sign detached). Reads the rasterizer's
+ * out_color (.w = alpha), out_aux (.y = depth_acc) and out_normal; writes
+ * dl_dnormal and dl_dcolor.w (dL/d depth_acc; rgb untouched). Encode after
+ * the colour loss and before the rasterizer backward (without
+ * GS_BWD_COLOR_ONLY). */
+typedef struct gs_normal_loss_s* gs_normal_loss;
+/* NormalLoss::new(ctx, w, h, out_color, out_aux, out_normal, dl_dnormal,
+ * dl_dcolor) (normal_loss.rs:40-120); all float4 per pixel. lambda starts 0. */
+gs_status gs_normal_loss_create(gs_ctx ctx, uint32_t width, uint32_t height, const float* out_color,
+                                const float* out_aux, const float* out_normal, float* dl_dnormal,
+                                float* dl_dcolor, gs_normal_loss* out);
+void gs_normal_loss_destroy(gs_normal_loss nl);
+/* pub loss_map (normal_loss.rs:33): P float4, x = alpha_q (1 - nhat . N_d). */
+float* gs_normal_loss_loss_map(gs_normal_loss nl);
+/* NormalLoss::set_lambda(ctx, lambda, focal) (normal_loss.rs:123-135): the
+ * kernel weight is lambda / (W H); alpha_min 0.2. */
+gs_status gs_normal_loss_set_lambda(gs_normal_loss nl, float lambda, float focal);
+gs_status gs_normal_loss_encode(gs_normal_loss nl, void* stream);  /* NormalLoss::encode */
+/* Blocking: the loss lambda/(W H) * sum of the loss map, summed in double
+ * over per-tile f32 partials (no reference counterpart: the reference
+ * leaves the loss map to the caller). */
+gs_status gs_normal_loss_read(gs_normal_loss nl, double* loss);
+
 /* ---------------------------------------------------------------- Optimizer
  * first_attempt/crates/gs-train/src/optim.rs:50-241. */
 typedef struct gs_optim_s* gs_optim;

@@ -90,9 +90,11 @@ def lib():
         L.oref_marginal_decisions.restype = C.c_int64
         L.oref_pixel_contributors.argtypes = [C.POINTER(_Scene), C.POINTER(_Camera), C.c_int64,
                                               C.POINTER(C.c_int32), C.POINTER(C.c_uint8)]
+        L.oref_normal_loss.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp,
+                                       _dp, _dp, _dp]
         L.oref_trace.argtypes = [C.POINTER(_Scene), C.POINTER(_Camera), C.c_int64, _u32p, _u32p]
         L.oref_replay.argtypes = [C.POINTER(_Scene), C.POINTER(_Camera), _u32p, _u32p, C.c_int64, _dp,
-                                  _dp, _dp, C.POINTER(_Grads)]
+                                  _dp, _dp, _dp, _dp, C.POINTER(_Grads)]
         L.oref_prepare_order.argtypes = [C.POINTER(_Scene), C.POINTER(_Camera), _i64p]
         L.oref_prepare_order.restype = C.c_int64
         L.oref_prep_f32.argtypes = [C.c_int64, _fp, _fp, _fp, C.POINTER(_Cam32), _u32p, _u32p,
@@ -331,7 +333,7 @@ def trace(scene: Scene, cam: Camera, cap: int = 1024):
     return lists, counts
 
 
-def replay(scene: Scene, cam: Camera, lists, counts, dl_dcolor=None):
+def replay(scene: Scene, cam: Camera, lists, counts, dl_dcolor=None, dl_ddepth=None, dl_dnormal=None):
     """Decision replay (test aid, no forward.rs counterpart): the FP64
     forward, and with dl_dcolor the backward, along another evaluation's
     per-pixel contribution lists over the camera's window (lists (P, cap) of
@@ -355,8 +357,10 @@ def replay(scene: Scene, cam: Camera, lists, counts, dl_dcolor=None):
         dlc = _d(dl_dcolor).reshape(-1)
         if dlc.size != 3 * P:
             raise ValueError("dl_dcolor must be (H, W, 3) over the window")
+    dld = _d(dl_ddepth).reshape(-1) if dl_ddepth is not None else None
+    dln = _d(dl_dnormal).reshape(-1) if dl_dnormal is not None else None
     rc = lib().oref_replay(C.byref(s), C.byref(c), _ptr(lists, _u32p), _ptr(counts, _u32p), lists.shape[1],
-                           _ptr(color, _dp), _ptr(alpha, _dp), _ptr(dlc, _dp),
+                           _ptr(color, _dp), _ptr(alpha, _dp), _ptr(dlc, _dp), _ptr(dld, _dp), _ptr(dln, _dp),
                            C.byref(cg) if cg is not None else None)
     if rc != 0:
         raise RuntimeError(f"oref_replay failed ({rc}: a count above cap or a surfel index out of range)")
@@ -456,6 +460,20 @@ def l1_ssim(img, target, lam: float = 0.2, dssim_scale: float = 0.5, grad: bool 
     loss = lib().oref_l1_ssim(W, H, _ptr(img.reshape(-1), _dp), _ptr(target.reshape(-1), _dp),
                               lam, dssim_scale, _ptr(dl, _dp), _ptr(l1, _dp), _ptr(ss, _dp))
     return (float(loss), dl.reshape(H, W, 3) if grad else None, float(l1[0]), float(ss[0]))
+
+
+def normal_loss(color4, aux4, normal4, focal: float, lam: float, alpha_min: float = 0.2):
+    """normal_loss.wgsl:52-145 in FP64. Inputs (H, W, 4); lam is the kernel
+    weight (raw lambda / (W H)). Returns (loss_map (H, W), dl_dnormal
+    (H, W, 3), dl_ddepth (H, W)); the loss is lam * loss_map.sum()."""
+    H, W = np.shape(color4)[:2]
+    c4, a4, n4 = _d(color4), _d(aux4), _d(normal4)
+    lm = np.zeros(H * W)
+    dn = np.zeros(H * W * 3)
+    dd = np.zeros(H * W)
+    lib().oref_normal_loss(W, H, float(focal), float(lam), float(alpha_min), _ptr(c4, _dp), _ptr(a4, _dp),
+                           _ptr(n4, _dp), _ptr(lm, _dp), _ptr(dn, _dp), _ptr(dd, _dp))
+    return lm.reshape(H, W), dn.reshape(H, W, 3), dd.reshape(H, W)
 
 
 def blur(channel: np.ndarray) -> np.ndarray:

@@ -1052,13 +1052,15 @@ int oref_trace(const oref_scene* s, const oref_camera* cam, int64_t cap, uint32_
 // (no near-plane or det test: the other evaluation took the branch) and the
 // given clamp (alpha = 0.995 if clamped, else opacity * ghat with no skip
 // test); the walk ends with the list (the transmittance stop is replayed
-// too). The surfel's colour clamp stays FP64's. Gradients follow the
+// too). The surfel's colour clamp stays FP64's. dl_ddepth (P) and dl_dnormal
+// (3P) optional, as in oref_gradients. Gradients follow the
 // backward.rs:112-245 chain along the replayed walk, so they are the exact
 // derivative of the function the decisions define. color (P x 3) and alpha
 // (P) optional; dl_dcolor + g optional (P x 3). Returns -2 if a count
 // exceeds cap or a surfel index is out of range.
 int oref_replay(const oref_scene* s, const oref_camera* cam, const uint32_t* lists, const uint32_t* counts,
-                int64_t cap, double* color, double* alpha, const double* dl_dcolor, oref_grads* g) {
+                int64_t cap, double* color, double* alpha, const double* dl_dcolor, const double* dl_ddepth,
+                const double* dl_dnormal, oref_grads* g) {
     if (!s || !cam || !lists || !counts || cap <= 0) return -1;
     Scene sc{s, num_coeffs(s->sh_degree)};
     const int64_t n = s->n;
@@ -1131,8 +1133,10 @@ int oref_replay(const oref_scene* s, const oref_camera* cam, const uint32_t* lis
             if (alpha) alpha[pi] = w_total;
             if (grads) {
                 V3 dlc(dl_dcolor[3 * pi], dl_dcolor[3 * pi + 1], dl_dcolor[3 * pi + 2]);
-                if (!is_zero(dlc))
-                    reverse_walk(all, cam, d, pix_x, pix_y, w_total, wt_total, dlc, 0.0, V3(), 0.0, w);
+                double dld = dl_ddepth ? dl_ddepth[pi] : 0.0;
+                V3 dln = dl_dnormal ? V3(dl_dnormal[3 * pi], dl_dnormal[3 * pi + 1], dl_dnormal[3 * pi + 2]) : V3();
+                if (!is_zero(dlc) || dld != 0.0 || !is_zero(dln))
+                    reverse_walk(all, cam, d, pix_x, pix_y, w_total, wt_total, dlc, dld, dln, 0.0, w);
             }
         }
     }
@@ -1144,6 +1148,67 @@ int oref_replay(const oref_scene* s, const oref_camera* cam, const uint32_t* lis
     chain_to_params(all, ws[0].acc, sc, cam, g);
     g->focal = ws[0].dfocal;
     return 0;
+}
+
+// first_attempt/crates/gs-train/src/shaders/normal_loss.wgsl:52-145 (pass1 +
+// pass2) in FP64: inputs per pixel color4 (alpha in [3]), aux4 (depth_acc in
+// [1]), normal4 (rows of 4, row-major W x H); lambda is the kernel weight
+// (the reference's set_lambda divides by W H before it reaches the kernel).
+// Outputs: loss_map[P] = alpha_q (1 - nhat . N_d), dl_dnormal[3P],
+// dl_ddepth[P] = dL/d(depth_acc) (the value pass2 writes to dl_dcolor.w).
+void oref_normal_loss(int w, int h, double focal, double lambda, double alpha_min, const double* color4,
+                      const double* aux4, const double* normal4, double* loss_map, double* dl_dnormal,
+                      double* dl_ddepth) {
+    const int64_t P = (int64_t)w * h;
+    std::vector<V3> cpx(P), cpy(P);
+    const double cx = w * 0.5, cy = h * 0.5;
+    auto ray = [&](int x, int y) { return V3((x + 0.5 - cx) / focal, -(y + 0.5 - cy) / focal, -1.0); };
+    auto alpha = [&](int64_t i) { return color4[4 * i + 3]; };
+    auto tbar = [&](int64_t i) { return alpha(i) < alpha_min ? 0.0 : aux4[4 * i + 1] / alpha(i); };
+    for (int64_t i = 0; i < P; ++i) {
+        loss_map[i] = 0.0;
+        dl_dnormal[3 * i] = dl_dnormal[3 * i + 1] = dl_dnormal[3 * i + 2] = 0.0;
+        if (lambda == 0.0) continue;
+        const int x = (int)(i % w), y = (int)(i / w);
+        if (x + 1 >= w || y + 1 >= h) continue;
+        const int64_t ix = i + 1, iy = i + w;
+        const double a_q = alpha(i);
+        if (a_q < alpha_min || alpha(ix) < alpha_min || alpha(iy) < alpha_min) continue;
+        V3 p = ray(x, y) * tbar(i);
+        V3 dpx = ray(x + 1, y) * tbar(ix) - p;
+        V3 dpy = ray(x, y + 1) * tbar(iy) - p;
+        V3 m = cross(dpx, dpy);
+        double len2 = dot(m, m);
+        if (len2 < 1e-18) continue;
+        double len = std::sqrt(len2);
+        V3 mhat = m / len;
+        double sign = dot(mhat, p) > 0.0 ? -1.0 : 1.0;
+        V3 nd = mhat * sign;
+        V3 av(normal4[4 * i], normal4[4 * i + 1], normal4[4 * i + 2]);
+        double la2 = dot(av, av);
+        if (la2 < 1e-18) continue;
+        double la = std::sqrt(la2);
+        V3 nhat = av / la;
+        loss_map[i] = (1.0 - dot(nhat, nd)) * a_q;
+        V3 dl_dnd = nhat * (-lambda * a_q);
+        V3 dl_dnhat = nd * (-lambda * a_q);
+        V3 dl_da = (dl_dnhat - nhat * dot(nhat, dl_dnhat)) / la;
+        dl_dnormal[3 * i] = dl_da.x;
+        dl_dnormal[3 * i + 1] = dl_da.y;
+        dl_dnormal[3 * i + 2] = dl_da.z;
+        V3 dl_dm = (dl_dnd - mhat * dot(mhat, dl_dnd)) * (sign / len);
+        cpx[i] = cross(dpy, dl_dm);
+        cpy[i] = cross(dl_dm, dpx);
+    }
+    for (int64_t i = 0; i < P; ++i) {
+        dl_ddepth[i] = 0.0;
+        if (lambda == 0.0) continue;
+        const int x = (int)(i % w), y = (int)(i / w);
+        V3 acc = -cpx[i] - cpy[i];
+        if (x > 0) acc += cpx[i - 1];
+        if (y > 0) acc += cpy[i - w];
+        if (alpha(i) >= alpha_min) dl_ddepth[i] = dot(ray(x, y), acc) / alpha(i);
+    }
 }
 
 // ------------------------------------------------ fp32 mirror: integer stages

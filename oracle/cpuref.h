@@ -114,12 +114,17 @@ int oref_pixel_contributors(const oref_scene* s, const oref_camera* cam, int64_t
 
 /* Number of surfels that survive the near cull and their prepare() order
  * (indices sorted by (f32 depth bits, idx)). order may be NULL. */
+/* normal_loss.wgsl:52-145 (2DGS normal consistency) in FP64. */
+void oref_normal_loss(int w, int h, double focal, double lambda, double alpha_min, const double* color4,
+                      const double* aux4, const double* normal4, double* loss_map, double* dl_dnormal,
+                      double* dl_ddepth);
 /* The faithful FP64 walk's contribution lists in oref_replay's format. */
 int oref_trace(const oref_scene* s, const oref_camera* cam, int64_t cap, uint32_t* lists, uint32_t* counts);
 /* Decision replay: FP64 forward (+ backward) along given per-pixel
  * contribution lists (surfel | ray << 30 | clamped << 31), see cpuref.cpp. */
 int oref_replay(const oref_scene* s, const oref_camera* cam, const uint32_t* lists, const uint32_t* counts,
-                int64_t cap, double* color, double* alpha, const double* dl_dcolor, oref_grads* g);
+                int64_t cap, double* color, double* alpha, const double* dl_dcolor, const double* dl_ddepth,
+                const double* dl_dnormal, oref_grads* g);
 
 int64_t oref_prepare_order(const oref_scene* s, const oref_camera* cam, int64_t* order);
 

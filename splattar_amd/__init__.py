@@ -7,9 +7,9 @@ Importing the API without the built library raises: there is no CPU path.
 """
 
 from ._lib import GsError, LIB_PATH, header_symbols, lib  # noqa: F401
-from .api import (Camera, DeviceBuffer, GpuContext, Optimizer, PrefixSum,  # noqa: F401
+from .api import (Camera, DeviceBuffer, GpuContext, NormalLoss, Optimizer, PrefixSum,  # noqa: F401
                   RadixSorter, Rasterizer, SsimLoss, TileBinner, TrainConfig, Trainer)
 
 __all__ = ["GpuContext", "DeviceBuffer", "PrefixSum", "RadixSorter", "TileBinner", "Rasterizer",
-           "SsimLoss", "Optimizer", "Trainer", "TrainConfig", "Camera", "GsError", "lib",
+           "SsimLoss", "NormalLoss", "Optimizer", "Trainer", "TrainConfig", "Camera", "GsError", "lib",
            "header_symbols", "LIB_PATH"]

@@ -136,6 +136,12 @@ _PROTOS = {
     "gs_loss_set_dssim_scale": ([_vp, C.c_float], C.c_int),
     "gs_loss_encode": ([_vp, _vp], C.c_int),
     "gs_loss_read": ([_vp, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
+    "gs_normal_loss_create": ([_vp, _u32, _u32, _vp, _vp, _vp, _vp, _vp, C.POINTER(_vp)], C.c_int),
+    "gs_normal_loss_destroy": ([_vp], None),
+    "gs_normal_loss_loss_map": ([_vp], _vp),
+    "gs_normal_loss_set_lambda": ([_vp, C.c_float, C.c_float], C.c_int),
+    "gs_normal_loss_encode": ([_vp, _vp], C.c_int),
+    "gs_normal_loss_read": ([_vp, C.POINTER(C.c_double)], C.c_int),
     "gs_optim_create": ([_vp, C.POINTER(_vp)], C.c_int),
     "gs_optim_destroy": ([_vp], None),
     "gs_optim_add_class": ([_vp, C.c_char_p, _u32, C.c_int, C.c_float, _vp, _vp], C.c_int),

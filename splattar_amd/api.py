@@ -380,6 +380,37 @@ class SsimLoss:
             self.h = None
 
 
+class NormalLoss:
+    """NormalLoss (normal_loss.rs) bound to a rasterizer's forward outputs and
+    backward inputs (out_color, out_aux, out_normal -> dl_dnormal, dl_dcolor.w)."""
+
+    def __init__(self, ctx: GpuContext, width: int, height: int, out_color: DeviceBuffer,
+                 out_aux: DeviceBuffer, out_normal: DeviceBuffer, dl_dnormal: DeviceBuffer,
+                 dl_dcolor: DeviceBuffer):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(lib().gs_normal_loss_create(ctx.h, width, height, out_color.ptr, out_aux.ptr, out_normal.ptr,
+                                          dl_dnormal.ptr, dl_dcolor.ptr, C.byref(h)))
+        self.h = h.value
+        self.loss_map = DeviceBuffer(ctx, lib().gs_normal_loss_loss_map(self.h), np.float32, (height, width, 4))
+
+    def set_lambda(self, lam: float, focal: float):
+        check(lib().gs_normal_loss_set_lambda(self.h, lam, focal))
+
+    def encode(self, stream=None):
+        check(lib().gs_normal_loss_encode(self.h, stream))
+
+    def read_loss(self) -> float:
+        v = C.c_double()
+        check(lib().gs_normal_loss_read(self.h, C.byref(v)))
+        return v.value
+
+    def close(self):
+        if self.h:
+            lib().gs_normal_loss_destroy(self.h)
+            self.h = None
+
+
 class Optimizer:
     """Optimizer (optim.rs:50-241)."""
 

@@ -695,6 +695,22 @@ hipError_t launch_l1_ssim(hipStream_t st, uint32_t w, uint32_t h, float lambda, 
                           const CamParams* view = nullptr);
 uint32_t l1_ssim_partials(uint32_t w, uint32_t h);
 
+// normal_loss.hip: normal-consistency loss over one W x H view (the
+// reference's NormalLoss, normal_loss.wgsl:52-145). lambda carries the 1/N.
+struct NormalLossArgs {
+    uint32_t width, height;
+    float focal, lambda, alpha_min;
+    const float4* color;   // .w = alpha
+    const float4* aux;     // .y = depth_acc
+    const float4* normal;  // sum w n
+    float4* dl_dnormal;
+    float* dl_dcolor_w;    // &dl_dcolor[0].w, stride 4 floats
+    float4* loss_map;      // may be null
+    float* partials;       // one per 16x16 tile
+};
+hipError_t launch_normal_loss(hipStream_t st, const NormalLossArgs& a);
+uint32_t normal_loss_partials(uint32_t width, uint32_t height);
+
 // adam.hip
 struct AdamClass {
     float* raw;

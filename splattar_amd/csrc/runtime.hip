@@ -1359,6 +1359,95 @@ gs_status gs_loss_read(gs_loss l, double* l1_mean, double* dssim_mean) {
 
 }  // extern "C"
 
+// ============================================================== NormalLoss
+
+struct gs_normal_loss_s {
+    gs_ctx ctx = nullptr;
+    uint32_t width = 0, height = 0;
+    float lambda = 0.0f, focal = 1.0f, raw_lambda = 0.0f;
+    const float4* color = nullptr;
+    const float4* aux = nullptr;
+    const float4* normal = nullptr;
+    float4* dl_dnormal = nullptr;
+    float4* dl_dcolor = nullptr;
+    float4* loss_map = nullptr;
+    float* partials = nullptr;
+    uint32_t n_partials = 0;
+    ~gs_normal_loss_s() {
+        dfree(loss_map);
+        dfree(partials);
+    }
+};
+
+extern "C" {
+
+gs_status gs_normal_loss_create(gs_ctx ctx, uint32_t width, uint32_t height, const float* out_color,
+                                const float* out_aux, const float* out_normal, float* dl_dnormal,
+                                float* dl_dcolor, gs_normal_loss* out) {
+    GS_REQUIRE(ctx && out && out_color && out_aux && out_normal && dl_dnormal && dl_dcolor, "null argument");
+    GS_REQUIRE(width > 0 && height > 0, "image size must be > 0");
+    *out = nullptr;
+    auto nl = std::make_unique<gs_normal_loss_s>();
+    nl->ctx = ctx;
+    nl->width = width;
+    nl->height = height;
+    nl->color = reinterpret_cast<const float4*>(out_color);
+    nl->aux = reinterpret_cast<const float4*>(out_aux);
+    nl->normal = reinterpret_cast<const float4*>(out_normal);
+    nl->dl_dnormal = reinterpret_cast<float4*>(dl_dnormal);
+    nl->dl_dcolor = reinterpret_cast<float4*>(dl_dcolor);
+    GS_CHECK(dalloc(&nl->loss_map, (size_t)width * height, "normal loss map"));
+    nl->n_partials = normal_loss_partials(width, height);
+    GS_CHECK(dalloc(&nl->partials, nl->n_partials, "normal loss partials"));
+    *out = nl.release();
+    return GS_OK;
+}
+
+void gs_normal_loss_destroy(gs_normal_loss nl) { delete nl; }
+float* gs_normal_loss_loss_map(gs_normal_loss nl) { return nl ? reinterpret_cast<float*>(nl->loss_map) : nullptr; }
+
+gs_status gs_normal_loss_set_lambda(gs_normal_loss nl, float lambda, float focal) {
+    GS_REQUIRE(nl != nullptr, "normal loss is null");
+    GS_REQUIRE(std::isfinite(lambda) && lambda >= 0.0f, "lambda must be finite and >= 0");
+    GS_REQUIRE(std::isfinite(focal) && focal > 0.0f, "focal must be finite and > 0");
+    nl->raw_lambda = lambda;
+    nl->lambda = lambda / (float)((double)nl->width * nl->height);  // normal_loss.rs:127
+    nl->focal = focal;
+    return GS_OK;
+}
+
+gs_status gs_normal_loss_encode(gs_normal_loss nl, void* stream) {
+    GS_REQUIRE(nl != nullptr, "normal loss is null");
+    NormalLossArgs a;
+    a.width = nl->width;
+    a.height = nl->height;
+    a.focal = nl->focal;
+    a.lambda = nl->lambda;
+    a.alpha_min = 0.2f;  // normal_loss.rs:129
+    a.color = nl->color;
+    a.aux = nl->aux;
+    a.normal = nl->normal;
+    a.dl_dnormal = nl->dl_dnormal;
+    a.dl_dcolor_w = reinterpret_cast<float*>(nl->dl_dcolor) + 3;
+    a.loss_map = nl->loss_map;
+    a.partials = nl->partials;
+    GS_HIP(launch_normal_loss(as_stream(stream, nl->ctx->stream), a));
+    return GS_OK;
+}
+
+gs_status gs_normal_loss_read(gs_normal_loss nl, double* loss) {
+    GS_REQUIRE(nl && loss, "null argument");
+    std::vector<float> p(nl->n_partials);
+    GS_HIP(hipStreamSynchronize(nl->ctx->stream));
+    GS_HIP(hipMemcpy(p.data(), nl->partials, p.size() * sizeof(float), hipMemcpyDeviceToHost));
+    double sum = 0.0;
+    for (float v : p) sum += v;
+    *loss = (double)nl->lambda * sum;
+    return GS_OK;
+}
+
+}  // extern "C"
+
 // ================================================================ Optimizer
 
 struct OptimClass {
