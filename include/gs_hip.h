@@ -38,7 +38,8 @@ typedef enum gs_status {
     GS_ERR_OOM = 3,       /* device allocation failed */
     GS_ERR_HIP = 4,       /* HIP runtime error (message names the call) */
     GS_ERR_OVERFLOW = 5,  /* tile entries exceeded max_entries (reference: unchecked) */
-    GS_ERR_NO_DEVICE = 6  /* no HIP device (reference: GpuError::NoAdapter, gs-wgpu/src/lib.rs:19-27) */
+    GS_ERR_NO_DEVICE = 6, /* no HIP device (reference: GpuError::NoAdapter, gs-wgpu/src/lib.rs:19-27) */
+    GS_ERR_IO = 7         /* file i/o failed (reference: std::io::Error, gs-io/src/ply/writer.rs:17) */
 } gs_status;
 
 /* Message of the last failing call on this thread ("" if none). */
@@ -302,6 +303,21 @@ gs_status gs_loss_encode(gs_loss l, void* stream);   /* SsimLoss::encode (ssim.r
  * dssim_mean = dssim_scale * mean(1 - SSIM). */
 gs_status gs_loss_read(gs_loss l, double* l1_mean, double* dssim_mean);
 
+/* -------------------------------------------------------- 3DGS .ply export
+ * first_attempt/crates/gs-io/src/ply/writer.rs:17-79 write_3dgs_ply(path,
+ * count, positions, scales, quats, opacities, sh) over device buffers (the
+ * activated layout: positions float4, scales float2, quats xyzw, opacity,
+ * SH 48 floats coefficient-major): header, then per surfel 59 floats
+ * x y z | f_dc rgb | f_rest channel-major | logit(clamp(o, 1e-6, 1-1e-6)) |
+ * ln(max(s, 1e-9)) x2, -10 | w x y z. The rows are built on the device;
+ * blocking; GS_ERR_IO when the file cannot be written. */
+gs_status gs_write_3dgs_ply(gs_ctx ctx, const char* path, uint32_t count, const float* positions,
+                            const float* scales, const float* quats, const float* opacities, const float* sh);
+/* The rows alone: count x 59 floats into device memory `rows`, enqueued on
+ * the stream (no reference counterpart: callers streaming the export). */
+gs_status gs_ply_pack_rows(gs_ctx ctx, void* stream, uint32_t count, const float* positions, const float* scales,
+                           const float* quats, const float* opacities, const float* sh, float* rows);
+
 /* --------------------------------------------------------------- NormalLoss
  * first_attempt/crates/gs-train/src/normal_loss.rs + shaders/normal_loss.wgsl:
  * 52-145: 2DGS normal consistency, L = lambda/(W H) * sum_q alpha_q
@@ -488,6 +504,9 @@ gs_status gs_trainer_optim(gs_trainer t, gs_optim* out);
 gs_status gs_trainer_state_bytes(gs_trainer t, uint64_t* bytes);
 gs_status gs_trainer_export_state(gs_trainer t, void* dst, uint64_t bytes);
 gs_status gs_trainer_import_state(gs_trainer t, const void* src, uint64_t bytes);
+/* Trainer::read_scene + gs_io::write_3dgs_ply (trainer.rs:1938-1950,
+ * writer.rs:17-79): the baked .ply of the current activated parameters. */
+gs_status gs_trainer_write_ply(gs_trainer t, const char* path);
 /* Blocking: per-view (l1_mean, dssim_mean) of the last step, 2 per view. */
 gs_status gs_trainer_last_losses(gs_trainer t, double* out, uint32_t n_ids);
 /* Blocking: render view `view` with the current parameters into rgba (W*H*4). */

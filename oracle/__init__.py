@@ -476,6 +476,60 @@ def normal_loss(color4, aux4, normal4, focal: float, lam: float, alpha_min: floa
     return lm.reshape(H, W), dn.reshape(H, W, 3), dd.reshape(H, W)
 
 
+# ------------------------------------------------------------ 3DGS .ply
+
+def ply_header(count: int) -> bytes:
+    """first_attempt/crates/gs-io/src/ply/writer.rs:31-51."""
+    lines = ["ply", "format binary_little_endian 1.0", "comment splattar baked export (surfels flattened to 3DGS)",
+             f"element vertex {count}"]
+    names = (["x", "y", "z"] + [f"f_dc_{i}" for i in range(3)] + [f"f_rest_{i}" for i in range(45)] + ["opacity"]
+             + [f"scale_{i}" for i in range(3)] + [f"rot_{i}" for i in range(4)])
+    lines += [f"property float {nm}" for nm in names] + ["end_header"]
+    return ("\n".join(lines) + "\n").encode()
+
+
+def ply_rows(positions4, scales2, quats4, opacities, sh48) -> np.ndarray:
+    """writer.rs:53-77 in f32: (n, 59) rows. The three logarithms are the
+    correctly rounded f32 log of the f32 argument (the FP64 log rounded once;
+    the reference's f32::ln is libm's logf)."""
+    p = np.asarray(positions4, np.float32).reshape(-1, 4)
+    n = p.shape[0]
+    sh = np.asarray(sh48, np.float32).reshape(n, 16, 3)
+    rows = np.zeros((n, 59), np.float32)
+    rows[:, 0:3] = p[:, :3]
+    rows[:, 3:6] = sh[:, 0, :]
+    rows[:, 6:51] = sh[:, 1:, :].transpose(0, 2, 1).reshape(n, 45)  # channel-major over coeffs 1..15
+    o = np.clip(np.asarray(opacities, np.float32).reshape(n), np.float32(1e-6), np.float32(1.0 - 1e-6))
+    rows[:, 51] = np.log((o / (np.float32(1.0) - o)).astype(np.float64)).astype(np.float32)
+    s = np.asarray(scales2, np.float32).reshape(n, 2)
+    rows[:, 52:54] = np.log(np.maximum(s, np.float32(1e-9)).astype(np.float64)).astype(np.float32)
+    rows[:, 54] = np.float32(-10.0)
+    q = np.asarray(quats4, np.float32).reshape(n, 4)
+    rows[:, 55] = q[:, 3]
+    rows[:, 56:59] = q[:, :3]
+    return rows
+
+
+def ply_read_canonical(data: bytes):
+    """gs-io/src/ply/reader.rs:77-147 (push_row_fast) on a canonical INRIA
+    degree-3 file: (positions (n, 3), sh (n, 16, 3) coefficient-major,
+    opacity (sigmoid), scales (n, 3) (exp), quats xyzw normalized)."""
+    end = data.index(b"end_header\n") + len(b"end_header\n")
+    head = data[:end].decode().split("\n")
+    n = int([h for h in head if h.startswith("element vertex")][0].split()[-1])
+    rows = np.frombuffer(data[end:], dtype="<f4").reshape(n, 59).astype(np.float32)
+    pos = rows[:, 0:3]
+    sh = np.zeros((n, 16, 3), np.float32)
+    sh[:, 0, :] = rows[:, 3:6]
+    sh[:, 1:, :] = rows[:, 6:51].reshape(n, 3, 15).transpose(0, 2, 1)
+    opac = (np.float32(1.0) / (np.float32(1.0) + np.exp(-rows[:, 51]))).astype(np.float32)
+    scales = np.exp(rows[:, 52:55])
+    q = np.stack([rows[:, 56], rows[:, 57], rows[:, 58], rows[:, 55]], 1)
+    nrm = np.sqrt((q.astype(np.float64) ** 2).sum(1, keepdims=True))
+    q = np.where(nrm ** 2 > 1e-12, q / np.maximum(nrm, 1e-30), np.array([0, 0, 0, 1.0]))
+    return pos, sh, opac, scales, q
+
+
 def blur(channel: np.ndarray) -> np.ndarray:
     a = _d(channel)
     H, W = a.shape

@@ -16,7 +16,7 @@ HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "gs_hip.h")
 
 GS_OK = 0
 STATUS_NAMES = {0: "GS_OK", 1: "GS_ERR_INVALID", 2: "GS_ERR_CAPACITY", 3: "GS_ERR_OOM",
-                4: "GS_ERR_HIP", 5: "GS_ERR_OVERFLOW", 6: "GS_ERR_NO_DEVICE"}
+                4: "GS_ERR_HIP", 5: "GS_ERR_OVERFLOW", 6: "GS_ERR_NO_DEVICE", 7: "GS_ERR_IO"}
 
 GS_BWD_ACCUMULATE = 1
 GS_BWD_COLOR_ONLY = 2
@@ -136,6 +136,9 @@ _PROTOS = {
     "gs_loss_set_dssim_scale": ([_vp, C.c_float], C.c_int),
     "gs_loss_encode": ([_vp, _vp], C.c_int),
     "gs_loss_read": ([_vp, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
+    "gs_write_3dgs_ply": ([_vp, C.c_char_p, _u32, _vp, _vp, _vp, _vp, _vp], C.c_int),
+    "gs_ply_pack_rows": ([_vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp], C.c_int),
+    "gs_trainer_write_ply": ([_vp, C.c_char_p], C.c_int),
     "gs_normal_loss_create": ([_vp, _u32, _u32, _vp, _vp, _vp, _vp, _vp, C.POINTER(_vp)], C.c_int),
     "gs_normal_loss_destroy": ([_vp], None),
     "gs_normal_loss_loss_map": ([_vp], _vp),

@@ -20,6 +20,7 @@ write() (queue.write_buffer) and read() (buffers::readback).
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -380,6 +381,20 @@ class SsimLoss:
             self.h = None
 
 
+def write_3dgs_ply(ctx: "GpuContext", path: str, count: int, positions: DeviceBuffer, scales: DeviceBuffer,
+                   quats: DeviceBuffer, opacities: DeviceBuffer, sh: DeviceBuffer):
+    """gs_io::write_3dgs_ply (writer.rs:17-79) over device buffers."""
+    check(lib().gs_write_3dgs_ply(ctx.h, os.fsencode(path), count, positions.ptr, scales.ptr, quats.ptr,
+                                  opacities.ptr, sh.ptr))
+
+
+def ply_pack_rows(ctx: "GpuContext", count: int, positions: DeviceBuffer, scales: DeviceBuffer,
+                  quats: DeviceBuffer, opacities: DeviceBuffer, sh: DeviceBuffer, rows: DeviceBuffer, stream=None):
+    """The export's rows (count x 59 floats) into device memory."""
+    check(lib().gs_ply_pack_rows(ctx.h, stream, count, positions.ptr, scales.ptr, quats.ptr, opacities.ptr,
+                                 sh.ptr, rows.ptr))
+
+
 class NormalLoss:
     """NormalLoss (normal_loss.rs) bound to a rasterizer's forward outputs and
     backward inputs (out_color, out_aux, out_normal -> dl_dnormal, dl_dcolor.w)."""
@@ -597,6 +612,11 @@ class Trainer:
     def import_state(self, blob: bytes):
         buf = C.create_string_buffer(bytes(blob), len(blob))
         check(lib().gs_trainer_import_state(self.h, buf, len(blob)))
+
+    def write_ply(self, path: str):
+        """Baked 3DGS .ply of the current parameters (Trainer::read_scene +
+        write_3dgs_ply)."""
+        check(lib().gs_trainer_write_ply(self.h, os.fsencode(path)))
 
     def set_tiling(self, tile_size: int, fwd_batch: int, bwd_batch: int):
         check(lib().gs_trainer_set_tiling(self.h, tile_size, fwd_batch, bwd_batch))

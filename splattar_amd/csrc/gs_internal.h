@@ -333,55 +333,60 @@ __device__ inline void record_footprint(Record& r, const CamParams& cam) {
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) Q[i][j] = A[i] * A[j] + B[i] * B[j] - k * M[i] * M[j];
+    // The per-pixel test evaluates D = M.s, Nu = A.s, Nv = B.s in f32 (two
+    // FMAs each, on f32 s), so the conic value it sees, Nu^2 + Nv^2 - 2L D^2,
+    // is off by up to a few f32 roundings of its terms: |err| <= eps
+    // |s|^T Qa |s| with Qa_ij = |A_i A_j| + |B_i B_j| + 2L |M_i M_j|. A
+    // Laplacian argument gives |s|^T Qa |s| <= s^T diag(rowsum Qa) s, so the
+    // conic Q' = Q - eps diag(rowsum Qa) contains every pixel the f32 test
+    // can accept. For well-conditioned records this grows the ellipse by a
+    // fraction of a percent; a near-degenerate one (thin surfel seen edge-on)
+    // becomes a larger ellipse or unbounded instead of a region that rounding
+    // has made arbitrary.
+    constexpr double kConicEps = 2.5e-7;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        double row = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) row += fabs(A[i] * A[j]) + fabs(B[i] * B[j]) + k * fabs(M[i] * M[j]);
+        Q[i][i] -= kConicEps * row;
+    }
     const double f = cam.focal;
     const double det2 = Q[0][0] * Q[1][1] - Q[0][1] * Q[0][1];
     const double mag = Q[0][0] * Q[1][1] + Q[0][1] * Q[0][1];
-    // The per-pixel test runs in f32 on f32 coefficients, so a conic whose
-    // shape or scale sits at f32 rounding level is not the region the pixels
-    // see. Shape: det2 / mag is ~ the axis ratio squared; at 1e-6 the long
-    // axis already moves by a pixel under relative errors of ~1e-7. Scale:
-    // the conic's value at the centre, q_c, against the magnitude of its
-    // terms (|A_i A_j| + |B_i B_j| + 2L |M_i M_j| at (|csx|, |csy|, 1)); below
-    // 1e-7 of it q_c is rounding noise. Either way the ray part is treated as
-    // unbounded (box = the rect).
-    bool bounded = Q[0][0] > 0.0 && det2 > 1e-6 * mag;
-    double ex = kFar, ey = kFar, h00 = 1.0, h01 = 0.0, h11 = 1.0;
+    // det2 / mag ~ 1 / cond(H): the ellipse test reads H in f32, which keeps
+    // the long axis (the small eigenvalue) only while cond(H) stays well
+    // below 1 / (f32 unit roundoff); past 1e5 the ray part is treated as
+    // unbounded.
+    bool bounded = Q[0][0] > 0.0 && det2 > 1e-5 * mag;
+    double ex = kFar, ey = kFar, h00 = 1.0, h01 = 0.0, h11 = 1.0;  // empty ellipse by default
     if (bounded) {
+        const double a00 = Q[1][1] * Q[2][2] - Q[1][2] * Q[1][2];
+        const double a11 = Q[0][0] * Q[2][2] - Q[0][2] * Q[0][2];
+        const double a02 = Q[0][1] * Q[1][2] - Q[0][2] * Q[1][1];
+        const double a12 = Q[0][1] * Q[0][2] - Q[0][0] * Q[1][2];
+        const double dx = a02 * a02 - a00 * det2, dy = a12 * a12 - a11 * det2;
+        bounded = dx >= 0.0 && dy >= 0.0;
+        if (bounded) {
+            const double sdx = sqrt(dx), sdy = sqrt(dy);
+            const double sx_lo = (a02 - sdx) / det2, sx_hi = (a02 + sdx) / det2;
+            const double sy_lo = (a12 - sdy) / det2, sy_hi = (a12 + sdy) / det2;
+            // X = cx - f sx (pixel centre), Y = cy + f sy.
+            x0 = fmin(x0, cam.cx - f * sx_hi - 1.0);
+            x1 = fmax(x1, cam.cx - f * sx_lo + 1.0);
+            y0 = fmin(y0, cam.cy + f * sy_lo - 1.0);
+            y1 = fmax(y1, cam.cy + f * sy_hi + 1.0);
+        }
         const double csx = -(Q[1][1] * Q[0][2] - Q[0][1] * Q[1][2]) / det2;
         const double csy = -(Q[0][0] * Q[1][2] - Q[0][1] * Q[0][2]) / det2;
         const double qc = Q[2][2] + Q[0][2] * csx + Q[1][2] * csy;
-        const double sc3[3] = {fabs(csx), fabs(csy), 1.0};
-        double qabs = 0.0;
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-                qabs += sc3[i] * sc3[j] *
-                        (fabs(A[i] * A[j]) + fabs(B[i] * B[j]) + k * fabs(M[i] * M[j]));
-        bounded = qc < -1e-7 * qabs;
-        if (bounded) {
-            const double a00 = Q[1][1] * Q[2][2] - Q[1][2] * Q[1][2];
-            const double a11 = Q[0][0] * Q[2][2] - Q[0][2] * Q[0][2];
-            const double a02 = Q[0][1] * Q[1][2] - Q[0][2] * Q[1][1];
-            const double a12 = Q[0][1] * Q[0][2] - Q[0][0] * Q[1][2];
-            const double dx = a02 * a02 - a00 * det2, dy = a12 * a12 - a11 * det2;
-            bounded = dx >= 0.0 && dy >= 0.0;
-            if (bounded) {
-                const double sdx = sqrt(dx), sdy = sqrt(dy);
-                const double sx_lo = (a02 - sdx) / det2, sx_hi = (a02 + sdx) / det2;
-                const double sy_lo = (a12 - sdy) / det2, sy_hi = (a12 + sdy) / det2;
-                // X = cx - f sx (pixel centre), Y = cy + f sy.
-                x0 = fmin(x0, cam.cx - f * sx_hi - 1.0);
-                x1 = fmax(x1, cam.cx - f * sx_lo + 1.0);
-                y0 = fmin(y0, cam.cy + f * sy_lo - 1.0);
-                y1 = fmax(y1, cam.cy + f * sy_hi + 1.0);
-                const double sc = 1.0 / (-qc * f * f);
-                ex = cam.cx - f * csx;
-                ey = cam.cy + f * csy;
-                h00 = Q[0][0] * sc;
-                h01 = -Q[0][1] * sc;
-                h11 = Q[1][1] * sc;
-            }
+        if (bounded && qc < 0.0) {
+            const double sc = 1.0 / (-qc * f * f);
+            ex = cam.cx - f * csx;
+            ey = cam.cy + f * csy;
+            h00 = Q[0][0] * sc;
+            h01 = -Q[0][1] * sc;
+            h11 = Q[1][1] * sc;
         }
     }
     const bool finite = x0 == x0 && y0 == y0 && x1 == x1 && y1 == y1 && ex == ex && ey == ey &&
@@ -694,6 +699,12 @@ hipError_t launch_l1_ssim(hipStream_t st, uint32_t w, uint32_t h, float lambda, 
                           float* coef_scratch, double* partial_sums, uint32_t n_partials,
                           const CamParams* view = nullptr);
 uint32_t l1_ssim_partials(uint32_t w, uint32_t h);
+
+// ply.hip: rows of the baked 3DGS .ply export (writer.rs:17-79), 59 floats
+// per surfel, for surfels [0, n) of the activated SoA.
+constexpr int kPlyRowFloats = 59;
+hipError_t launch_ply_rows(hipStream_t st, uint32_t n, const float4* pos, const float2* scales,
+                           const float4* quats, const float* opac, const float* sh, float* rows);
 
 // normal_loss.hip: normal-consistency loss over one W x H view (the
 // reference's NormalLoss, normal_loss.wgsl:52-145). lambda carries the 1/N.

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -1359,6 +1360,87 @@ gs_status gs_loss_read(gs_loss l, double* l1_mean, double* dssim_mean) {
 
 }  // extern "C"
 
+// ======================================================== 3DGS .ply export
+
+namespace gs {
+
+// writer.rs:31-51: the header of the baked export.
+static std::string ply_header(uint32_t count) {
+    std::string h = "ply\nformat binary_little_endian 1.0\n";
+    h += "comment splattar baked export (surfels flattened to 3DGS)\n";
+    h += "element vertex " + std::to_string(count) + "\n";
+    for (const char* nm : {"x", "y", "z"}) h += std::string("property float ") + nm + "\n";
+    for (int i = 0; i < 3; ++i) h += "property float f_dc_" + std::to_string(i) + "\n";
+    for (int i = 0; i < 45; ++i) h += "property float f_rest_" + std::to_string(i) + "\n";
+    h += "property float opacity\n";
+    for (int i = 0; i < 3; ++i) h += "property float scale_" + std::to_string(i) + "\n";
+    for (int i = 0; i < 4; ++i) h += "property float rot_" + std::to_string(i) + "\n";
+    h += "end_header\n";
+    return h;
+}
+
+static gs_status write_ply_from_device(gs_ctx ctx, const char* path, uint32_t count, const float4* pos,
+                                       const float2* sc, const float4* q, const float* op, const float* sh) {
+    // Rows are packed on the device in chunks of up to 2^18 surfels (62 MB),
+    // copied to pinned host memory and appended to the file.
+    const uint32_t chunk = std::min<uint32_t>(std::max<uint32_t>(count, 1u), 1u << 18);
+    const size_t chunk_bytes = (size_t)chunk * kPlyRowFloats * sizeof(float);
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(GS_ERR_IO, std::string("cannot open ") + path + " for writing");
+    float* rows = nullptr;
+    float* host = nullptr;
+    gs_status st = GS_OK;
+    const std::string header = ply_header(count);
+    if (std::fwrite(header.data(), 1, header.size(), f) != header.size())
+        st = fail(GS_ERR_IO, std::string("write failed: ") + path);
+    hipError_t e = hipSuccess;
+    if (st == GS_OK && count > 0) {
+        e = hipMalloc((void**)&rows, chunk_bytes);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&host, chunk_bytes, hipHostMallocDefault);
+        for (uint32_t first = 0; e == hipSuccess && st == GS_OK && first < count; first += chunk) {
+            const uint32_t m = std::min(chunk, count - first);
+            e = launch_ply_rows(ctx->stream, m, pos + first, sc + first, q + first, op + first,
+                                sh + (size_t)first * kShFloats, rows);
+            const size_t bytes = (size_t)m * kPlyRowFloats * sizeof(float);
+            if (e == hipSuccess) e = hipMemcpyAsync(host, rows, bytes, hipMemcpyDeviceToHost, ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            if (e == hipSuccess && std::fwrite(host, 1, bytes, f) != bytes)
+                st = fail(GS_ERR_IO, std::string("write failed: ") + path);
+        }
+    }
+    if (e != hipSuccess) st = hip_fail(e, "ply export");
+    if (host) (void)hipHostFree(host);
+    if (rows) (void)hipFree(rows);
+    if (std::fclose(f) != 0 && st == GS_OK) st = fail(GS_ERR_IO, std::string("close failed: ") + path);
+    return st;
+}
+
+}  // namespace gs
+
+extern "C" {
+
+gs_status gs_ply_pack_rows(gs_ctx ctx, void* stream, uint32_t count, const float* positions, const float* scales,
+                           const float* quats, const float* opacities, const float* sh, float* rows) {
+    GS_REQUIRE(ctx != nullptr, "ctx is null");
+    GS_REQUIRE(count == 0 || (positions && scales && quats && opacities && sh && rows), "null argument");
+    GS_HIP(launch_ply_rows(as_stream(stream, ctx->stream), count, reinterpret_cast<const float4*>(positions),
+                           reinterpret_cast<const float2*>(scales), reinterpret_cast<const float4*>(quats),
+                           opacities, sh, rows));
+    return GS_OK;
+}
+
+gs_status gs_write_3dgs_ply(gs_ctx ctx, const char* path, uint32_t count, const float* positions,
+                            const float* scales, const float* quats, const float* opacities, const float* sh) {
+    GS_REQUIRE(ctx != nullptr && path != nullptr, "null argument");
+    GS_REQUIRE(count == 0 || (positions && scales && quats && opacities && sh), "null argument");
+    GS_HIP(hipStreamSynchronize(ctx->stream));
+    return write_ply_from_device(ctx, path, count, reinterpret_cast<const float4*>(positions),
+                                 reinterpret_cast<const float2*>(scales), reinterpret_cast<const float4*>(quats),
+                                 opacities, sh);
+}
+
+}  // extern "C"
+
 // ============================================================== NormalLoss
 
 struct gs_normal_loss_s {
@@ -2433,6 +2515,12 @@ gs_status gs_trainer_import_state(gs_trainer t, const void* src, uint64_t bytes)
     GS_REQUIRE(t && src, "null argument");
     GS_CHECK(t->wait_idle());
     return gs_optim_import_state(t->optim, src, bytes);
+}
+
+gs_status gs_trainer_write_ply(gs_trainer t, const char* path) {
+    GS_REQUIRE(t && path, "null argument");
+    GS_CHECK(t->wait_idle());
+    return write_ply_from_device(t->ctx, path, t->n, t->positions, t->scales, t->quats, t->opacities, t->sh);
 }
 
 gs_status gs_trainer_last_losses(gs_trainer t, double* out, uint32_t n_ids) {
