@@ -313,6 +313,23 @@ gs_status gs_loss_read(gs_loss l, double* l1_mean, double* dssim_mean);
  * blocking; GS_ERR_IO when the file cannot be written. */
 gs_status gs_write_3dgs_ply(gs_ctx ctx, const char* path, uint32_t count, const float* positions,
                             const float* scales, const float* quats, const float* opacities, const float* sh);
+/* -------------------------------------------------------------------- MCMC
+ * shaders/mcmc.wgsl:39-58 add_noise over device buffers: pos_raw float4
+ * (raw positions), scales_act float2, opac_act (activated); amp = sigma *
+ * mean scale * exp(-opacity_k * opacity). */
+gs_status gs_mcmc_add_noise(gs_ctx ctx, void* stream, uint32_t n, uint32_t seed, float sigma, float opacity_k,
+                           float* pos_raw, const float* scales_act, const float* opac_act);
+/* Host part of Trainer::mcmc_relocate (trainer.rs:1558-1650) on host arrays:
+ * dead = activated opacity < dead_threshold; one xorshift64 draw per dead
+ * surfel picks an alive target in proportion to opacity, without
+ * replacement (a colliding pick is skipped); then, when split_ratio > 0 and
+ * at least 16 surfels are alive, alive surfels larger than split_ratio x the
+ * median alive size (max of the two scales) take the remaining dead slots,
+ * largest first (mode 1). pairs receives (dead, alive, mode, 0) x n_pairs
+ * (room for 4 n). No device work. */
+gs_status gs_mcmc_sample_pairs(const float* opacity, const float* scales, uint32_t n, float dead_threshold,
+                               float split_ratio, uint64_t* rng_state, uint32_t* pairs, uint32_t* n_pairs);
+
 /* The rows alone: count x 59 floats into device memory `rows`, enqueued on
  * the stream (no reference counterpart: callers streaming the export). */
 gs_status gs_ply_pack_rows(gs_ctx ctx, void* stream, uint32_t count, const float* positions, const float* scales,
@@ -504,6 +521,24 @@ gs_status gs_trainer_optim(gs_trainer t, gs_optim* out);
 gs_status gs_trainer_state_bytes(gs_trainer t, uint64_t* bytes);
 gs_status gs_trainer_export_state(gs_trainer t, void* dst, uint64_t bytes);
 gs_status gs_trainer_import_state(gs_trainer t, const void* src, uint64_t bytes);
+/* MCMC exploration noise (TrainConfig::mcmc_noise, trainer.rs:58, 1256-1272;
+ * shaders/mcmc.wgsl): after every step's optimizer update the raw positions
+ * receive noise of amplitude noise * the step's position rate * mean scale *
+ * exp(-5 opacity), seeded by the step index; graph replays included. 0 =
+ * off (the default). */
+gs_status gs_trainer_set_mcmc_noise(gs_trainer t, float noise);
+/* Trainer::mcmc_relocate (trainer.rs:1558-1690): one blocking call -- reads
+ * the activated opacities (and scales when split_ratio > 0), samples the
+ * (dead, alive, mode) pairs as gs_mcmc_sample_pairs, then moves the data on
+ * the device (shaders/relocate.wgsl: parameters, split opacity, scale
+ * shrink, moments zeroed) and re-activates. rng_state is the reference's
+ * xorshift64 state (Trainer::rng), advanced in place. */
+gs_status gs_trainer_mcmc_relocate(gs_trainer t, float dead_threshold, float split_ratio, uint64_t* rng_state,
+                                   uint32_t* n_pairs);
+/* The device half alone for host pairs (dead, alive, mode, 0) x n_pairs (no
+ * reference counterpart as a separate call); GS_ERR_INVALID unless every
+ * index is in range and used at most once. Blocking. */
+gs_status gs_trainer_relocate_pairs(gs_trainer t, const uint32_t* pairs, uint32_t n_pairs);
 /* Trainer::read_scene + gs_io::write_3dgs_ply (trainer.rs:1938-1950,
  * writer.rs:17-79): the baked .ply of the current activated parameters. */
 gs_status gs_trainer_write_ply(gs_trainer t, const char* path);

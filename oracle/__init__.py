@@ -476,6 +476,132 @@ def normal_loss(color4, aux4, normal4, focal: float, lam: float, alpha_min: floa
     return lm.reshape(H, W), dn.reshape(H, W, 3), dd.reshape(H, W)
 
 
+# ------------------------------------------------------------ MCMC
+
+def _xorshift64(x: int) -> int:
+    m = (1 << 64) - 1
+    x ^= (x << 13) & m
+    x ^= x >> 7
+    x ^= (x << 17) & m
+    return x
+
+
+def _total_key(f) -> int:
+    """Rust f32::total_cmp as an integer key."""
+    b = int(np.float32(f).view(np.int32))
+    return b ^ (((b >> 31) & 0xFFFFFFFF) >> 1)
+
+
+def mcmc_sample_pairs(opacity, scales, dead: float, split_ratio: float, rng: int):
+    """first_attempt/crates/gs-train/src/trainer.rs:1558-1650 (the host half
+    of Trainer::mcmc_relocate), step by step in Python: (pairs [(dead,
+    alive, mode, 0)], rng)."""
+    op = np.asarray(opacity, np.float32).ravel()
+    n = op.size
+    dead_t = np.float32(dead)
+    dead_idx = [i for i in range(n) if op[i] < dead_t]
+    if not dead_idx:
+        return [], rng
+    cum, total = [], 0.0
+    for o in op:
+        if o >= dead_t:
+            total += float(o)
+        cum.append(total)
+    if total <= 0.0:
+        return [], rng
+    used, spent, pairs = [False] * n, [False] * len(dead_idx), []
+    for di, d in enumerate(dead_idx):
+        rng = _xorshift64(rng)
+        r = float(rng >> 11) / float(1 << 53) * total
+        a = min(int(np.searchsorted(np.asarray(cum), r, side="left")), n - 1)
+        if a == d or used[a]:
+            continue
+        used[a] = True
+        spent[di] = True
+        pairs.append((d, a, 0, 0))
+    if split_ratio > 0.0:
+        sc = np.asarray(scales, np.float32).reshape(n, 2)
+        size = [max(sc[i, 0], sc[i, 1]) for i in range(n)]
+        alive = sorted((size[i] for i in range(n) if op[i] >= dead_t), key=_total_key)
+        if len(alive) >= 16:
+            cap = np.float32(split_ratio) * np.float32(alive[len(alive) // 2])
+            big = [i for i in range(n) if op[i] >= dead_t and size[i] > cap and not used[i]]
+            big.sort(key=lambda i: -_total_key(size[i]))  # stable, largest first
+            left = [d for di, d in enumerate(dead_idx) if not spent[di]]
+            for b, slot in zip(big, left):
+                used[b] = True
+                pairs.append((slot, b, 1, 0))
+    return pairs, rng
+
+
+def mcmc_relocate_arrays(pairs, opac_act, raw, m, v):
+    """relocate.wgsl:39-125 in f32 on host copies: raw / m / v are dicts of the
+    five classes (pos (n,4), scales (n,2), quat (n,4), opacity (n,), sh
+    (n,48)), changed in place. sqrt / log / exp and the normalization are
+    f32 operations here (numpy), correctly rounded or within an ulp."""
+    f = np.float32
+    for d, a, mode, _ in pairs:
+        alpha_a = np.clip(f(opac_act[a]), f(1e-4), f(0.999))
+        alpha_new = np.clip(f(1.0) - np.sqrt(f(1.0) - alpha_a), f(1e-4), f(0.999))
+        logit = np.float32(np.log(np.float64(alpha_new / (f(1.0) - alpha_new))))
+        raw["opacity"][d] = raw["opacity"][a] = logit
+        q = raw["quat"][a].copy()
+        raw["quat"][d] = q
+        sa = raw["scales"][a].copy()
+        if mode == 1:
+            qn = np.sqrt((q * q).sum(dtype=np.float32))
+            x, y, z, w = q / qn
+            col0 = np.array([1 - 2 * (y * y + z * z), 2 * (x * y + w * z), 2 * (x * z - w * y)], np.float32)
+            col1 = np.array([2 * (x * y - w * z), 1 - 2 * (x * x + z * z), 2 * (y * z + w * x)], np.float32)
+            sw = np.exp(sa.astype(np.float64)).astype(np.float32)
+            axis = col1 * sw[1] if sw[1] > sw[0] else col0 * sw[0]
+            c = raw["pos"][a].copy()
+            off = f(0.6) * axis
+            raw["pos"][d] = np.append(c[:3] + off, c[3])
+            raw["pos"][a] = np.append(c[:3] - off, c[3])
+            s = sa - f(0.51)
+        else:
+            raw["pos"][d] = raw["pos"][a]
+            s = sa - f(0.16)
+        raw["scales"][a] = s
+        raw["scales"][d] = s
+        raw["sh"][d] = raw["sh"][a]
+        for k in raw:
+            m[k][d] = 0.0
+            m[k][a] = 0.0
+            v[k][d] = 0.0
+            v[k][a] = 0.0
+
+
+def _pcg(v):
+    v = np.asarray(v, np.uint64)
+    state = (v * np.uint64(747796405) + np.uint64(2891336453)) & np.uint64(0xFFFFFFFF)
+    word = (((state >> ((state >> np.uint64(28)) + np.uint64(4))) ^ state) * np.uint64(277803737)) & np.uint64(0xFFFFFFFF)
+    return ((word >> np.uint64(22)) ^ word) & np.uint64(0xFFFFFFFF)
+
+
+def mcmc_noise(pos_raw, scales_act, opac_act, seed: int, sigma: float, opacity_k: float = 5.0):
+    """shaders/mcmc.wgsl:25-58 (add_noise) in f32: the new raw positions (n, 4).
+    The PCG stream is integer-exact; exp is the correctly rounded f32 value."""
+    p = np.asarray(pos_raw, np.float32).reshape(-1, 4).copy()
+    n = p.shape[0]
+    s = np.asarray(scales_act, np.float32).reshape(n, 2)
+    mean_scale = np.float32(0.5) * (s[:, 0] + s[:, 1])
+    gate = np.exp((np.float32(-opacity_k) * np.asarray(opac_act, np.float32)).astype(np.float64)).astype(np.float32)
+    amp = (np.float32(sigma) * mean_scale * gate).astype(np.float32)
+    base = _pcg(np.arange(n, dtype=np.uint64) ^ np.uint64(seed & 0xFFFFFFFF))
+    for c, salt in enumerate((0x68bc21eb, 0x2c1b3c6d, 0x1f123bb5)):
+        st = base ^ np.uint64(salt)
+        acc = np.zeros(n, np.float32)
+        for _ in range(4):
+            st = _pcg(st)
+            acc = acc + (st & np.uint64(0x00ffffff)).astype(np.float32) / np.float32(16777216.0)
+        nz = (acc - np.float32(2.0)).astype(np.float32)
+        upd = amp > 0
+        p[upd, c] = p[upd, c] + amp[upd] * nz[upd]
+    return p
+
+
 # ------------------------------------------------------------ 3DGS .ply
 
 def ply_header(count: int) -> bytes:

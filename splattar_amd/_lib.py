@@ -136,6 +136,12 @@ _PROTOS = {
     "gs_loss_set_dssim_scale": ([_vp, C.c_float], C.c_int),
     "gs_loss_encode": ([_vp, _vp], C.c_int),
     "gs_loss_read": ([_vp, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
+    "gs_mcmc_add_noise": ([_vp, _vp, _u32, _u32, C.c_float, C.c_float, _vp, _vp, _vp], C.c_int),
+    "gs_mcmc_sample_pairs": ([_vp, _vp, _u32, C.c_float, C.c_float, C.POINTER(C.c_uint64), _vp,
+                              C.POINTER(_u32)], C.c_int),
+    "gs_trainer_set_mcmc_noise": ([_vp, C.c_float], C.c_int),
+    "gs_trainer_mcmc_relocate": ([_vp, C.c_float, C.c_float, C.POINTER(C.c_uint64), C.POINTER(_u32)], C.c_int),
+    "gs_trainer_relocate_pairs": ([_vp, _vp, _u32], C.c_int),
     "gs_write_3dgs_ply": ([_vp, C.c_char_p, _u32, _vp, _vp, _vp, _vp, _vp], C.c_int),
     "gs_ply_pack_rows": ([_vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp], C.c_int),
     "gs_trainer_write_ply": ([_vp, C.c_char_p], C.c_int),

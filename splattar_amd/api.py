@@ -388,6 +388,27 @@ def write_3dgs_ply(ctx: "GpuContext", path: str, count: int, positions: DeviceBu
                                   opacities.ptr, sh.ptr))
 
 
+def mcmc_sample_pairs(opacity, scales, dead: float, split_ratio: float, rng: int):
+    """The host sampling of Trainer::mcmc_relocate: ((n_pairs, 4) u32 pairs,
+    advanced rng state). Runs on the host only."""
+    op = np.ascontiguousarray(opacity, dtype=np.float32).ravel()
+    sc = np.ascontiguousarray(scales, dtype=np.float32).ravel() if scales is not None else None
+    n = op.size
+    pairs = np.zeros((max(n, 1), 4), np.uint32)
+    r = C.c_uint64(rng)
+    npairs = C.c_uint32()
+    check(lib().gs_mcmc_sample_pairs(op.ctypes.data, sc.ctypes.data if sc is not None else None, n, dead,
+                                     split_ratio, C.byref(r), pairs.ctypes.data, C.byref(npairs)))
+    return pairs[:npairs.value].copy(), int(r.value)
+
+
+def mcmc_add_noise(ctx: "GpuContext", n: int, seed: int, sigma: float, opacity_k: float, pos_raw: DeviceBuffer,
+                   scales_act: DeviceBuffer, opac_act: DeviceBuffer, stream=None):
+    """mcmc.wgsl add_noise over device buffers."""
+    check(lib().gs_mcmc_add_noise(ctx.h, stream, n, seed, sigma, opacity_k, pos_raw.ptr, scales_act.ptr,
+                                  opac_act.ptr))
+
+
 def ply_pack_rows(ctx: "GpuContext", count: int, positions: DeviceBuffer, scales: DeviceBuffer,
                   quats: DeviceBuffer, opacities: DeviceBuffer, sh: DeviceBuffer, rows: DeviceBuffer, stream=None):
     """The export's rows (count x 59 floats) into device memory."""
@@ -612,6 +633,21 @@ class Trainer:
     def import_state(self, blob: bytes):
         buf = C.create_string_buffer(bytes(blob), len(blob))
         check(lib().gs_trainer_import_state(self.h, buf, len(blob)))
+
+    def set_mcmc_noise(self, noise: float):
+        """MCMC exploration noise after every step (TrainConfig::mcmc_noise); 0 = off."""
+        check(lib().gs_trainer_set_mcmc_noise(self.h, noise))
+
+    def mcmc_relocate(self, dead: float = 0.005, split_ratio: float = 8.0, rng: int = 0x9E3779B97F4A7C15):
+        """Trainer::mcmc_relocate; returns (pairs moved, advanced rng state)."""
+        r = C.c_uint64(rng)
+        npairs = C.c_uint32()
+        check(lib().gs_trainer_mcmc_relocate(self.h, dead, split_ratio, C.byref(r), C.byref(npairs)))
+        return int(npairs.value), int(r.value)
+
+    def relocate_pairs(self, pairs):
+        p = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 4)
+        check(lib().gs_trainer_relocate_pairs(self.h, p.ctypes.data, p.shape[0]))
 
     def write_ply(self, path: str):
         """Baked 3DGS .ply of the current parameters (Trainer::read_scene +

@@ -700,6 +700,40 @@ hipError_t launch_l1_ssim(hipStream_t st, uint32_t w, uint32_t h, float lambda, 
                           const CamParams* view = nullptr);
 uint32_t l1_ssim_partials(uint32_t w, uint32_t h);
 
+// mcmc.hip: exploration noise on raw positions (mcmc.wgsl) and the relocation
+// of dead surfels (relocate.wgsl). With state set, the noise takes its seed
+// ((t - 1) * 0x9e3779b9) and sigma (noise_scale * lr[lr_class]) from the
+// device-resident Adam state of the step just taken; else seed and sigma.
+struct AdamState;
+struct NoiseArgs {
+    uint32_t n = 0, seed = 0;
+    float sigma = 0.0f, opacity_k = 5.0f;
+    const AdamState* state = nullptr;
+    float noise_scale = 0.0f;
+    uint32_t lr_class = 0;
+    float4* pos_raw = nullptr;
+    const float2* scales_act = nullptr;
+    const float* opac_act = nullptr;
+};
+hipError_t launch_add_noise(hipStream_t st, const NoiseArgs& a);
+// pairs[i] = (dead, alive, mode, 0): mode 0 relocation, 1 split. Class c of
+// the five (pos, scales, quat, opacity, SH) holds class_width[c] floats per
+// surfel (4, 2, 4, 1, 48) in m[c] / v[c].
+struct RelocateArgs {
+    const uint4* pairs = nullptr;
+    uint32_t n_pairs = 0;
+    const float* opac_act = nullptr;
+    float4* pos_raw = nullptr;
+    float2* scales_raw = nullptr;
+    float4* quat_raw = nullptr;
+    float* opac_raw = nullptr;
+    float4* sh_raw = nullptr;
+    float* m[5] = {};
+    float* v[5] = {};
+    uint32_t class_width[5] = {4, 2, 4, 1, 48};
+};
+hipError_t launch_relocate(hipStream_t st, const RelocateArgs& a);
+
 // ply.hip: rows of the baked 3DGS .ply export (writer.rs:17-79), 59 floats
 // per surfel, for surfels [0, n) of the activated SoA.
 constexpr int kPlyRowFloats = 59;

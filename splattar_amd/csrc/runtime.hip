@@ -1419,6 +1419,99 @@ static gs_status write_ply_from_device(gs_ctx ctx, const char* path, uint32_t co
 
 extern "C" {
 
+gs_status gs_mcmc_add_noise(gs_ctx ctx, void* stream, uint32_t n, uint32_t seed, float sigma, float opacity_k,
+                           float* pos_raw, const float* scales_act, const float* opac_act) {
+    GS_REQUIRE(ctx != nullptr, "ctx is null");
+    GS_REQUIRE(n == 0 || (pos_raw && scales_act && opac_act), "null argument");
+    NoiseArgs a;
+    a.n = n;
+    a.seed = seed;
+    a.sigma = sigma;
+    a.opacity_k = opacity_k;
+    a.pos_raw = reinterpret_cast<float4*>(pos_raw);
+    a.scales_act = reinterpret_cast<const float2*>(scales_act);
+    a.opac_act = opac_act;
+    GS_HIP(launch_add_noise(as_stream(stream, ctx->stream), a));
+    return GS_OK;
+}
+
+// trainer.rs:1558-1650 (the host part of Trainer::mcmc_relocate).
+gs_status gs_mcmc_sample_pairs(const float* opacity, const float* scales, uint32_t n, float dead_threshold,
+                               float split_ratio, uint64_t* rng_state, uint32_t* pairs, uint32_t* n_pairs) {
+    GS_REQUIRE(opacity && rng_state && pairs && n_pairs, "null argument");
+    GS_REQUIRE(split_ratio <= 0.0f || scales != nullptr, "splits need the activated scales");
+    *n_pairs = 0;
+    std::vector<uint32_t> dead;
+    for (uint32_t i = 0; i < n; ++i)
+        if (opacity[i] < dead_threshold) dead.push_back(i);
+    if (dead.empty()) return GS_OK;
+    // Opacity-proportional alive sampling via a cumulative table.
+    std::vector<double> cum(n);
+    double total = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (opacity[i] >= dead_threshold) total += (double)opacity[i];
+        cum[i] = total;
+    }
+    if (total <= 0.0) return GS_OK;
+    std::vector<uint8_t> used(n, 0), spent(dead.size(), 0);
+    uint32_t np = 0;
+    uint64_t x = *rng_state;
+    for (size_t di = 0; di < dead.size(); ++di) {
+        const uint32_t d = dead[di];
+        x ^= x << 13;
+        x ^= x >> 7;
+        x ^= x << 17;
+        const double r = (double)(x >> 11) / (double)(1ull << 53) * total;
+        size_t a = (size_t)(std::lower_bound(cum.begin(), cum.end(), r) - cum.begin());
+        a = std::min<size_t>(a, n - 1);
+        if (a == d || used[a]) continue;
+        used[a] = 1;
+        spent[di] = 1;
+        uint32_t* p = pairs + 4 * np++;
+        p[0] = d;
+        p[1] = (uint32_t)a;
+        p[2] = 0;
+        p[3] = 0;
+    }
+    *rng_state = x;
+    // Oversized-surfel splits (trainer.rs:1609-1650).
+    if (split_ratio > 0.0f) {
+        auto size = [&](uint32_t i) { return std::max(scales[2 * (size_t)i], scales[2 * (size_t)i + 1]); };
+        // f32 total order (Rust f32::total_cmp).
+        auto key = [](float f) {
+            int32_t b;
+            std::memcpy(&b, &f, 4);
+            return b ^ (int32_t)((uint32_t)(b >> 31) >> 1);
+        };
+        std::vector<float> alive;
+        for (uint32_t i = 0; i < n; ++i)
+            if (opacity[i] >= dead_threshold) alive.push_back(size(i));
+        if (alive.size() >= 16) {
+            std::sort(alive.begin(), alive.end(), [&](float a, float b) { return key(a) < key(b); });
+            const float med = alive[alive.size() / 2];
+            const float cap = split_ratio * med;
+            std::vector<uint32_t> big;
+            for (uint32_t i = 0; i < n; ++i)
+                if (opacity[i] >= dead_threshold && size(i) > cap && !used[i]) big.push_back(i);
+            std::stable_sort(big.begin(), big.end(),
+                             [&](uint32_t a, uint32_t b) { return key(size(b)) < key(size(a)); });
+            size_t li = 0;
+            for (uint32_t b : big) {
+                while (li < dead.size() && spent[li]) ++li;
+                if (li >= dead.size()) break;
+                used[b] = 1;
+                uint32_t* p = pairs + 4 * np++;
+                p[0] = dead[li++];
+                p[1] = b;
+                p[2] = 1;
+                p[3] = 0;
+            }
+        }
+    }
+    *n_pairs = np;
+    return GS_OK;
+}
+
 gs_status gs_ply_pack_rows(gs_ctx ctx, void* stream, uint32_t count, const float* positions, const float* scales,
                            const float* quats, const float* opacities, const float* sh, float* rows) {
     GS_REQUIRE(ctx != nullptr, "ctx is null");
@@ -1968,10 +2061,30 @@ struct gs_trainer_s {
         dfree(dl_dcolor);
         dfree(loss_coef);
         dfree(loss_partials);
+        dfree(reloc_pairs);
     }
 
     gs_status enqueue_step(hipStream_t st, const uint32_t* ids, uint32_t k, bool batched,
                            bool profile = false);
+    // MCMC exploration noise after the optimizer step (mcmc.wgsl, trainer.rs:
+    // 1256-1272): sigma = mcmc_noise * the step's position rate, seed from
+    // the step count, both read on the device.
+    float mcmc_noise = 0.0f;
+    uint32_t* reloc_pairs = nullptr;  // device (dead, alive, mode, 0) pairs, n of them at most
+    gs_status noise_step(hipStream_t st) {
+        if (!(mcmc_noise > 0.0f)) return GS_OK;
+        NoiseArgs a;
+        a.n = n;
+        a.opacity_k = 5.0f;  // trainer.rs:1262
+        a.state = optim->state;
+        a.noise_scale = mcmc_noise;
+        a.lr_class = 0;
+        a.pos_raw = reinterpret_cast<float4*>(optim->classes[0].raw);
+        a.scales_act = scales;
+        a.opac_act = opacities;
+        GS_HIP(launch_add_noise(st, a));
+        return GS_OK;
+    }
     gs_status enqueue_pipelined(hipStream_t st, const uint32_t* ids, uint32_t k);
     // Geometry backward of the k views of a batched step fused with the Adam
     // step (bias-correction tick first).
@@ -2073,6 +2186,7 @@ gs_status gs_trainer_s::enqueue_step(hipStream_t st, const uint32_t* ids, uint32
         if (P) GS_CHECK(P->mark(st, GS_STAGE_ADAM));
         GS_CHECK(gs_optim_encode_step(optim, st));
     }
+    GS_CHECK(noise_step(st));
     if (P) {
         GS_CHECK(P->mark(st, -1));
         P->steps += 1;
@@ -2111,7 +2225,8 @@ gs_status gs_trainer_s::enqueue_pipelined(hipStream_t st, const uint32_t* ids, u
     GS_CHECK(core.raster_bwd_range(st, 0, 0, k, 0, ta, dl_dcolor));
     GS_HIP(hipStreamWaitEvent(st, ev[3], 0));
     GS_CHECK(core.raster_bwd_range(st, 1, 0, k, ta, tt - ta, dl_dcolor));
-    return geom_adam(st, k);
+    GS_CHECK(geom_adam(st, k));
+    return noise_step(st);
 }
 
 extern "C" {
@@ -2515,6 +2630,64 @@ gs_status gs_trainer_import_state(gs_trainer t, const void* src, uint64_t bytes)
     GS_REQUIRE(t && src, "null argument");
     GS_CHECK(t->wait_idle());
     return gs_optim_import_state(t->optim, src, bytes);
+}
+
+gs_status gs_trainer_set_mcmc_noise(gs_trainer t, float noise) {
+    GS_REQUIRE(t != nullptr, "trainer is null");
+    GS_REQUIRE(std::isfinite(noise) && noise >= 0.0f, "mcmc noise must be finite and >= 0");
+    if ((noise > 0.0f) != (t->mcmc_noise > 0.0f)) drop_graph(t);  // the step gains or loses a launch
+    t->mcmc_noise = noise;
+    return GS_OK;
+}
+
+gs_status gs_trainer_relocate_pairs(gs_trainer t, const uint32_t* pairs, uint32_t n_pairs) {
+    GS_REQUIRE(t != nullptr && (pairs != nullptr || n_pairs == 0), "null argument");
+    GS_REQUIRE(n_pairs <= t->n, "more pairs than surfels");
+    // Pairs must not race: dead and alive indices in range and all distinct.
+    std::vector<uint8_t> seen(t->n, 0);
+    for (uint32_t i = 0; i < n_pairs; ++i) {
+        const uint32_t d = pairs[4 * i], a = pairs[4 * i + 1], mode = pairs[4 * i + 2];
+        GS_REQUIRE(d < t->n && a < t->n && mode <= 1u, "pair index or mode out of range");
+        GS_REQUIRE(!seen[d] && !seen[a] && d != a, "pairs must use every index at most once");
+        seen[d] = seen[a] = 1;
+    }
+    GS_CHECK(t->wait_idle());
+    if (n_pairs == 0) return GS_OK;
+    hipStream_t st = t->ctx->stream;
+    if (t->reloc_pairs == nullptr) GS_CHECK(dalloc(&t->reloc_pairs, (size_t)t->n * 4, "relocation pairs"));
+    GS_HIP(hipMemcpyAsync(t->reloc_pairs, pairs, (size_t)n_pairs * 16, hipMemcpyHostToDevice, st));
+    RelocateArgs a;
+    a.pairs = reinterpret_cast<const uint4*>(t->reloc_pairs);
+    a.n_pairs = n_pairs;
+    a.opac_act = t->opacities;
+    auto& c = t->optim->classes;
+    a.pos_raw = reinterpret_cast<float4*>(c[0].raw);
+    a.scales_raw = reinterpret_cast<float2*>(c[1].raw);
+    a.quat_raw = reinterpret_cast<float4*>(c[2].raw);
+    a.opac_raw = c[3].raw;
+    a.sh_raw = reinterpret_cast<float4*>(c[4].raw);
+    for (int k = 0; k < 5; ++k) {
+        a.m[k] = c[k].m;
+        a.v[k] = c[k].v;
+        a.class_width[k] = c[k].n / t->n;
+    }
+    GS_HIP(launch_relocate(st, a));
+    GS_CHECK(gs_optim_encode_activate(t->optim, st));  // trainer.rs:1682
+    GS_HIP(hipStreamSynchronize(st));
+    return GS_OK;
+}
+
+gs_status gs_trainer_mcmc_relocate(gs_trainer t, float dead_threshold, float split_ratio, uint64_t* rng_state,
+                                   uint32_t* n_pairs) {
+    GS_REQUIRE(t && rng_state && n_pairs, "null argument");
+    GS_CHECK(t->wait_idle());
+    std::vector<float> op(t->n), sc((size_t)t->n * 2);
+    GS_HIP(hipMemcpy(op.data(), t->opacities, op.size() * 4, hipMemcpyDeviceToHost));
+    GS_HIP(hipMemcpy(sc.data(), t->scales, sc.size() * 4, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> pairs((size_t)t->n * 4);
+    GS_CHECK(gs_mcmc_sample_pairs(op.data(), sc.data(), t->n, dead_threshold, split_ratio, rng_state, pairs.data(),
+                                  n_pairs));
+    return gs_trainer_relocate_pairs(t, pairs.data(), *n_pairs);
 }
 
 gs_status gs_trainer_write_ply(gs_trainer t, const char* path) {
