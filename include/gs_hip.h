@@ -335,6 +335,52 @@ gs_status gs_mcmc_sample_pairs(const float* opacity, const float* scales, uint32
 gs_status gs_ply_pack_rows(gs_ctx ctx, void* stream, uint32_t count, const float* positions, const float* scales,
                            const float* quats, const float* opacities, const float* sh, float* rows);
 
+/* ------------------------------------------------------------------ Tracker
+ * crates/gs-track/src/lib.rs:89-337 + shaders/photometric.wgsl: photometric
+ * camera tracking against a frozen map held in a rasterizer. Per iteration:
+ * forward at the current pose, softened L1 against the target (dL/d(render) =
+ * clamp(d / 1e-4, -1, 1) / 3P), backward, then the Riemannian gradient
+ * (antisymmetric part of R^T dL/dR; dL/dC), Adam over the six (lr_rot,
+ * lr_center * depth; betas 0.9 / 0.999, eps 1e-10) and the right
+ * perturbation R' = R exp([-step]x), C' = C - step. The whole loop runs on
+ * the device (no per-iteration readback); the pose returned is the best one
+ * seen. Uses the rasterizer's images and gradient buffers. */
+typedef struct gs_tracker_s* gs_tracker;
+/* TrackConfig (lib.rs:46-87); defaults 60, 2e-3, 2e-3, 1e-4, 10. */
+typedef struct gs_track_config {
+    uint32_t iterations;
+    float lr_rot;
+    float lr_center;
+    float min_improvement;
+    uint32_t patience;
+} gs_track_config;
+/* TrackReport (lib.rs:17-44): iterations run, coverage of the last
+ * iteration, best residual; plus the pose of the last evaluated iteration
+ * (not part of the reference report). */
+typedef struct gs_track_report {
+    uint32_t iterations;
+    float coverage;
+    float best;
+    float last_quat[4];
+    float last_center[3];
+} gs_track_report;
+/* Tracker::new(ctx, raster, width, height); width x height = the raster's. */
+gs_status gs_tracker_create(gs_ctx ctx, gs_raster raster, uint32_t width, uint32_t height, gs_tracker* out);
+void gs_tracker_destroy(gs_tracker t);
+/* Tracker::set_target (lib.rs:159-165): host RGBA frame, W x H x 4 floats. */
+gs_status gs_tracker_set_target(gs_tracker t, const float* frame_rgba);
+/* Device pointer of the target (for frames already in device memory). */
+float* gs_tracker_target(gs_tracker t);
+/* Tracker::track (lib.rs:167-222): camera in/out (returns the best pose);
+ * residuals (optional, cfg.iterations floats) receives the trace. Blocking. */
+gs_status gs_tracker_track(gs_tracker t, uint32_t surfels, gs_raster_camera* camera, float depth,
+                           const gs_track_config* cfg, float* residuals, gs_track_report* report);
+/* The photometric pass alone over the rasterizer's current render
+ * (photometric.wgsl:47-73 with gain / bias on the target): writes the
+ * rasterizer's dl_dcolor; residual / coverage (optional) make it blocking. */
+gs_status gs_tracker_photometric(gs_tracker t, void* stream, float gain, float bias, double* residual,
+                                 double* coverage);
+
 /* --------------------------------------------------------------- NormalLoss
  * first_attempt/crates/gs-train/src/normal_loss.rs + shaders/normal_loss.wgsl:
  * 52-145: 2DGS normal consistency, L = lambda/(W H) * sum_q alpha_q

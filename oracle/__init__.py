@@ -602,6 +602,77 @@ def mcmc_noise(pos_raw, scales_act, opac_act, seed: int, sigma: float, opacity_k
     return p
 
 
+# ------------------------------------------------------------ tracking
+
+def photometric(render4, frame4, gain: float = 1.0, bias: float = 0.0):
+    """crates/gs-track/src/shaders/photometric.wgsl:47-73 in f32 (sums in
+    float64): (dl_dcolor (P, 4), residual, coverage)."""
+    r = np.asarray(render4, np.float32).reshape(-1, 4)
+    t = np.asarray(frame4, np.float32).reshape(-1, 4)
+    P = r.shape[0]
+    d = r[:, :3] - (np.float32(gain) * t[:, :3] + np.float32(bias))
+    dl = np.zeros((P, 4), np.float32)
+    dl[:, :3] = np.clip(d / np.float32(1e-4), -1.0, 1.0) * (np.float32(1.0) / np.float32(3.0 * P))
+    residual = float(np.abs(d).astype(np.float64).sum() / (3.0 * P))
+    coverage = float((r[:, 3] > 0.5).sum() / P)
+    return dl, residual, coverage
+
+
+def _mat3_from_quat32(q):
+    x, y, z, w = (np.float32(v) for v in q)
+    x2, y2, z2 = x + x, y + y, z + z
+    xx, xy, xz, yy, yz, zz = x * x2, x * y2, x * z2, y * y2, y * z2, z * z2
+    wx, wy, wz = w * x2, w * y2, w * z2
+    one = np.float32(1.0)
+    # Rows of R (R[i][j]); glam's columns are (1-(yy+zz), xy+wz, xz-wy), ...
+    return np.array([[one - (yy + zz), xy - wz, xz + wy],
+                     [xy + wz, one - (xx + zz), yz - wx],
+                     [xz - wy, yz + wx, one - (xx + yy)]], np.float32)
+
+
+def camera_step_grad(grad_cam, quat):
+    """crates/gs-track/src/lib.rs:276-297: [rotation triple (antisymmetric part
+    of R^T dL/dR), dL/dC] in f32; None if non-finite."""
+    g = np.asarray(grad_cam, np.float32)[:12]
+    if not np.all(np.isfinite(g)):
+        return None
+    A = g[:9].reshape(3, 3).T  # columns g[0..2], g[3..5], g[6..8]
+    B = (_mat3_from_quat32(quat).T @ A).astype(np.float32)
+    return np.array([B[2, 1] - B[1, 2], B[0, 2] - B[2, 0], B[1, 0] - B[0, 1], g[9], g[10], g[11]], np.float32)
+
+
+def adam6(m, v, t: int, grad, lr_rot: float, lr_center: float):
+    """lib.rs:300-319 in f32 (m, v updated in place); returns the step."""
+    f = np.float32
+    bc1 = f(1.0) - f(0.9) ** f(t)
+    bc2 = f(1.0) - f(0.999) ** f(t)
+    step = np.zeros(6, np.float32)
+    for k in range(6):
+        m[k] = f(0.9) * m[k] + f(0.1) * grad[k]
+        v[k] = f(0.999) * v[k] + f(0.001) * grad[k] * grad[k]
+        lr = f(lr_rot) if k < 3 else f(lr_center)
+        step[k] = lr * (m[k] / bc1) / (np.sqrt(v[k] / bc2) + f(1e-10))
+    return step
+
+
+def apply_camera_step(quat, center, step):
+    """lib.rs:321-329: quat' = normalize(quat * exp(-step[0:3] / 2)), centre' =
+    centre - step[3:6], in f32."""
+    d = -np.asarray(step[:3], np.float32)
+    ang = np.float32(np.sqrt((d.astype(np.float64) ** 2).sum()))
+    if ang > 0:
+        e = np.append(d / ang * np.float32(np.sin(ang * 0.5)), np.float32(np.cos(ang * 0.5))).astype(np.float32)
+    else:
+        e = np.array([0, 0, 0, 1], np.float32)
+    q = np.asarray(quat, np.float32)
+    p = np.array([q[3] * e[0] + q[0] * e[3] + q[1] * e[2] - q[2] * e[1],
+                  q[3] * e[1] - q[0] * e[2] + q[1] * e[3] + q[2] * e[0],
+                  q[3] * e[2] + q[0] * e[1] - q[1] * e[0] + q[2] * e[3],
+                  q[3] * e[3] - q[0] * e[0] - q[1] * e[1] - q[2] * e[2]], np.float32)
+    p = p / np.float32(np.sqrt((p.astype(np.float64) ** 2).sum()))
+    return p.astype(np.float32), (np.asarray(center, np.float32) - np.asarray(step[3:], np.float32)).astype(np.float32)
+
+
 # ------------------------------------------------------------ 3DGS .ply
 
 def ply_header(count: int) -> bytes:

@@ -8,9 +8,10 @@ Importing the API without the built library raises: there is no CPU path.
 
 from ._lib import GsError, LIB_PATH, header_symbols, lib  # noqa: F401
 from .api import (Camera, DeviceBuffer, GpuContext, NormalLoss, Optimizer, PrefixSum,  # noqa: F401
-                  RadixSorter, Rasterizer, SsimLoss, TileBinner, TrainConfig, Trainer, mcmc_add_noise,
+                  RadixSorter, Rasterizer, SsimLoss, TileBinner, TrackConfig, Tracker, TrainConfig, Trainer,
+                  mcmc_add_noise,
                   mcmc_sample_pairs, ply_pack_rows, write_3dgs_ply)
 
 __all__ = ["GpuContext", "DeviceBuffer", "PrefixSum", "RadixSorter", "TileBinner", "Rasterizer",
            "SsimLoss", "NormalLoss", "Optimizer", "Trainer", "TrainConfig", "Camera", "GsError", "lib",
-           "header_symbols", "LIB_PATH", "write_3dgs_ply", "ply_pack_rows", "mcmc_add_noise", "mcmc_sample_pairs"]
+           "header_symbols", "LIB_PATH", "write_3dgs_ply", "ply_pack_rows", "mcmc_add_noise", "mcmc_sample_pairs", "Tracker", "TrackConfig"]

@@ -32,6 +32,16 @@ class GsError(RuntimeError):
         self.status = status
 
 
+class TrackConfig(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("lr_rot", C.c_float), ("lr_center", C.c_float),
+                ("min_improvement", C.c_float), ("patience", C.c_uint32)]
+
+
+class TrackReport(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("coverage", C.c_float), ("best", C.c_float),
+                ("last_quat", C.c_float * 4), ("last_center", C.c_float * 3)]
+
+
 class RasterCamera(C.Structure):
     _fields_ = [("center", C.c_float * 3), ("quat", C.c_float * 4), ("focal", C.c_float),
                 ("sh_degree", C.c_uint32)]
@@ -145,6 +155,14 @@ _PROTOS = {
     "gs_write_3dgs_ply": ([_vp, C.c_char_p, _u32, _vp, _vp, _vp, _vp, _vp], C.c_int),
     "gs_ply_pack_rows": ([_vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp], C.c_int),
     "gs_trainer_write_ply": ([_vp, C.c_char_p], C.c_int),
+    "gs_tracker_create": ([_vp, _vp, _u32, _u32, C.POINTER(_vp)], C.c_int),
+    "gs_tracker_destroy": ([_vp], None),
+    "gs_tracker_target": ([_vp], _vp),
+    "gs_tracker_set_target": ([_vp, _vp], C.c_int),
+    "gs_tracker_track": ([_vp, _u32, C.POINTER(RasterCamera), C.c_float, C.POINTER(TrackConfig), _vp,
+                          C.POINTER(TrackReport)], C.c_int),
+    "gs_tracker_photometric": ([_vp, _vp, C.c_float, C.c_float, C.POINTER(C.c_double), C.POINTER(C.c_double)],
+                               C.c_int),
     "gs_normal_loss_create": ([_vp, _u32, _u32, _vp, _vp, _vp, _vp, _vp, C.POINTER(_vp)], C.c_int),
     "gs_normal_loss_destroy": ([_vp], None),
     "gs_normal_loss_loss_map": ([_vp], _vp),

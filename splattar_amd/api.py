@@ -416,6 +416,75 @@ def ply_pack_rows(ctx: "GpuContext", count: int, positions: DeviceBuffer, scales
                                  sh.ptr, rows.ptr))
 
 
+@dataclass
+class TrackConfig:
+    """TrackConfig (crates/gs-track/src/lib.rs:46-87) with its defaults."""
+    iterations: int = 60
+    lr_rot: float = 2e-3
+    lr_center: float = 2e-3
+    min_improvement: float = 1e-4
+    patience: int = 10
+
+
+@dataclass
+class TrackReport:
+    """TrackReport (lib.rs:17-44) plus the last evaluated pose."""
+    residuals: list
+    coverage: float
+    iterations: int
+    best: float
+    last_quat: np.ndarray
+    last_center: np.ndarray
+
+    def final_residual(self) -> float:
+        return self.best
+
+    def improved(self) -> bool:
+        return bool(self.residuals) and self.best < self.residuals[0]
+
+
+class Tracker:
+    """Tracker (crates/gs-track/src/lib.rs:89-337) over a Rasterizer holding the map."""
+
+    def __init__(self, ctx: GpuContext, raster: "Rasterizer", width: int, height: int):
+        self.ctx = ctx
+        self.raster = raster
+        self.width, self.height = width, height
+        h = C.c_void_p()
+        check(lib().gs_tracker_create(ctx.h, raster.h, width, height, C.byref(h)))
+        self.h = h.value
+        self.target = DeviceBuffer(ctx, lib().gs_tracker_target(self.h), np.float32, (height, width, 4))
+
+    def set_target(self, frame):
+        f = np.ascontiguousarray(frame, np.float32).reshape(self.height, self.width, 4)
+        check(lib().gs_tracker_set_target(self.h, f.ctypes.data))
+
+    def track(self, surfels: int, camera: Camera, depth: float, cfg: TrackConfig | None = None):
+        """Recover `camera` (updated in place to the best pose); returns the report."""
+        cfg = cfg or TrackConfig()
+        rc = camera.c()
+        tc = L.TrackConfig(cfg.iterations, cfg.lr_rot, cfg.lr_center, cfg.min_improvement, cfg.patience)
+        res = np.zeros(cfg.iterations, np.float32)
+        rep = L.TrackReport()
+        check(lib().gs_tracker_track(self.h, surfels, C.byref(rc), depth, C.byref(tc), res.ctypes.data,
+                                     C.byref(rep)))
+        camera.center = np.array(rc.center[:], np.float32)
+        camera.quat = np.array(rc.quat[:], np.float32)
+        return TrackReport(res[:rep.iterations].tolist(), rep.coverage, rep.iterations, rep.best,
+                           np.array(rep.last_quat[:], np.float32), np.array(rep.last_center[:], np.float32))
+
+    def photometric(self, gain: float = 1.0, bias: float = 0.0, stream=None):
+        """The photometric pass on the rasterizer's current render: (residual, coverage)."""
+        e, c = C.c_double(), C.c_double()
+        check(lib().gs_tracker_photometric(self.h, stream, gain, bias, C.byref(e), C.byref(c)))
+        return e.value, c.value
+
+    def close(self):
+        if self.h:
+            lib().gs_tracker_destroy(self.h)
+            self.h = None
+
+
 class NormalLoss:
     """NormalLoss (normal_loss.rs) bound to a rasterizer's forward outputs and
     backward inputs (out_color, out_aux, out_normal -> dl_dnormal, dl_dcolor.w)."""

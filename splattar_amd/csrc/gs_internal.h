@@ -734,6 +734,31 @@ struct RelocateArgs {
 };
 hipError_t launch_relocate(hipStream_t st, const RelocateArgs& a);
 
+// track.hip: photometric tracking (crates/gs-track). Device state of one
+// track call: Adam moments of the six pose parameters, the residual
+// bookkeeping and the current / best pose (quat xyzw, centre).
+struct TrackState {
+    float m[6], v[6];
+    float best, coverage;
+    uint32_t stale, stopped, iters, pad;
+    float cur_q[4], cur_c[3];
+    float best_q[4], best_c[3];
+};
+struct TrackArgs {
+    TrackState* state;
+    const float2* partials;
+    uint32_t n_partials, npix;
+    const float* grad_cam;  // [dL/dR col-major 9, dL/dC 3, dL/df, pad]
+    CamParams* cam;         // the camera the next forward reads
+    float* residuals;       // one per iteration
+    uint32_t iterations, patience;
+    float lr_rot, lr_center, min_improvement;
+};
+uint32_t photometric_partials(uint32_t npix);
+hipError_t launch_photometric(hipStream_t st, uint32_t npix, float gain, float bias, const float4* render,
+                              const float4* frame, float4* dl_dcolor, float2* partials);
+hipError_t launch_track_update(hipStream_t st, const TrackArgs& a, uint32_t t);
+
 // ply.hip: rows of the baked 3DGS .ply export (writer.rs:17-79), 59 floats
 // per surfel, for surfels [0, n) of the activated SoA.
 constexpr int kPlyRowFloats = 59;

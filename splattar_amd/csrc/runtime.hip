@@ -1360,6 +1360,161 @@ gs_status gs_loss_read(gs_loss l, double* l1_mean, double* dssim_mean) {
 
 }  // extern "C"
 
+// ================================================================= Tracker
+
+struct gs_tracker_s {
+    gs_ctx ctx = nullptr;
+    gs_raster raster = nullptr;
+    uint32_t width = 0, height = 0;
+    float4* target = nullptr;
+    float2* partials = nullptr;
+    uint32_t n_partials = 0;
+    TrackState* state = nullptr;
+    TrackState* state_host = nullptr;  // pinned
+    float* residuals = nullptr;
+    uint32_t residual_cap = 0;
+    ~gs_tracker_s() {
+        dfree(target);
+        dfree(partials);
+        dfree(state);
+        dfree(residuals);
+        if (state_host) (void)hipHostFree(state_host);
+    }
+};
+
+extern "C" {
+
+gs_status gs_tracker_create(gs_ctx ctx, gs_raster raster, uint32_t width, uint32_t height, gs_tracker* out) {
+    GS_REQUIRE(ctx && raster && out, "null argument");
+    GS_REQUIRE(width == raster->width && height == raster->height,
+               "tracker size must equal the rasterizer's image size");
+    *out = nullptr;
+    auto t = std::make_unique<gs_tracker_s>();
+    t->ctx = ctx;
+    t->raster = raster;
+    t->width = width;
+    t->height = height;
+    const uint32_t P = width * height;
+    GS_CHECK(dalloc(&t->target, P, "tracker target"));
+    t->n_partials = photometric_partials(P);
+    GS_CHECK(dalloc(&t->partials, t->n_partials, "photometric partials"));
+    GS_CHECK(dalloc(&t->state, 1, "track state"));
+    GS_HIP(hipHostMalloc((void**)&t->state_host, sizeof(TrackState), hipHostMallocDefault));
+    *out = t.release();
+    return GS_OK;
+}
+
+void gs_tracker_destroy(gs_tracker t) { delete t; }
+float* gs_tracker_target(gs_tracker t) { return t ? reinterpret_cast<float*>(t->target) : nullptr; }
+
+gs_status gs_tracker_set_target(gs_tracker t, const float* frame_rgba) {
+    GS_REQUIRE(t && frame_rgba, "null argument");
+    GS_HIP(hipMemcpyAsync(t->target, frame_rgba, (size_t)t->width * t->height * sizeof(float4),
+                          hipMemcpyHostToDevice, t->ctx->stream));
+    GS_HIP(hipStreamSynchronize(t->ctx->stream));
+    return GS_OK;
+}
+
+gs_status gs_tracker_track(gs_tracker tr, uint32_t surfels, gs_raster_camera* camera, float depth,
+                           const gs_track_config* cfg, float* residuals, gs_track_report* report) {
+    GS_REQUIRE(tr && camera && cfg && report, "null argument");
+    GS_CHECK(check_camera(camera));
+    GS_REQUIRE(cfg->iterations >= 1, "iterations must be >= 1");
+    GS_REQUIRE(std::isfinite(depth) && std::isfinite(cfg->lr_rot) && std::isfinite(cfg->lr_center),
+               "rates and depth must be finite");
+    gs_raster r = tr->raster;
+    if (surfels > r->capacity) return fail(GS_ERR_CAPACITY, "surfels exceeds the rasterizer's capacity");
+    hipStream_t st = tr->ctx->stream;
+    if (tr->residual_cap < cfg->iterations) {
+        dfree(tr->residuals);
+        GS_CHECK(dalloc(&tr->residuals, cfg->iterations, "track residuals"));
+        tr->residual_cap = cfg->iterations;
+    }
+    // Initial state: moments 0, best = +inf, current = best = the camera.
+    TrackState s0;
+    std::memset(&s0, 0, sizeof(s0));
+    s0.best = INFINITY;
+    for (int k = 0; k < 4; ++k) s0.cur_q[k] = s0.best_q[k] = camera->quat[k];
+    for (int k = 0; k < 3; ++k) s0.cur_c[k] = s0.best_c[k] = camera->center[k];
+    *tr->state_host = s0;
+    GS_HIP(hipMemcpyAsync(tr->state, tr->state_host, sizeof(TrackState), hipMemcpyHostToDevice, st));
+    CamParams c = make_cam(*camera, r->width, r->height);
+    c.lambda_dist = 0.0f;
+    GS_CHECK(r->core.set_views(st, {c}, true));
+    TrackArgs a;
+    a.state = tr->state;
+    a.partials = tr->partials;
+    a.n_partials = tr->n_partials;
+    a.npix = r->width * r->height;
+    a.grad_cam = r->grad_cam;
+    a.cam = r->core.cams_dev;
+    a.residuals = tr->residuals;
+    a.iterations = cfg->iterations;
+    a.patience = cfg->patience;
+    a.lr_rot = cfg->lr_rot;
+    a.lr_center = cfg->lr_center * depth;  // lib.rs:217
+    a.min_improvement = cfg->min_improvement;
+    ParamGrads g{r->grad_pos, r->grad_scales, r->grad_quat, r->grad_opacity, r->grad_sh, r->grad_cam};
+    // Iterations are enqueued in groups; the stop flag is read between
+    // groups, so a stopped call runs at most a group past its last step
+    // (those iterations record nothing and leave the pose alone).
+    constexpr uint32_t kGroup = 8;
+    for (uint32_t t = 1; t <= cfg->iterations; ++t) {
+        GS_CHECK(r->core.forward(st, 0, 1, surfels, r->positions, r->scales, r->quats, r->opacities, r->sh));
+        GS_HIP(launch_photometric(st, a.npix, 1.0f, 0.0f, r->core.out_color, tr->target, r->dl_dcolor,
+                                  tr->partials));
+        GS_CHECK(r->core.backward(st, 0, 1, surfels, r->positions, r->scales, r->quats, r->sh, r->dl_dcolor,
+                                  nullptr, g, false, false));
+        GS_HIP(launch_track_update(st, a, t));
+        if (t % kGroup == 0 && t < cfg->iterations) {
+            GS_HIP(hipMemcpyAsync(tr->state_host, tr->state, sizeof(TrackState), hipMemcpyDeviceToHost, st));
+            GS_HIP(hipStreamSynchronize(st));
+            if (tr->state_host->stopped) break;
+        }
+    }
+    GS_HIP(hipMemcpyAsync(tr->state_host, tr->state, sizeof(TrackState), hipMemcpyDeviceToHost, st));
+    GS_HIP(hipStreamSynchronize(st));
+    const TrackState& s = *tr->state_host;
+    if (residuals && s.iters) GS_HIP(hipMemcpy(residuals, tr->residuals, s.iters * sizeof(float), hipMemcpyDeviceToHost));
+    report->iterations = s.iters;
+    report->coverage = s.coverage;
+    report->best = s.best;
+    for (int k = 0; k < 4; ++k) report->last_quat[k] = s.cur_q[k];
+    for (int k = 0; k < 3; ++k) report->last_center[k] = s.cur_c[k];
+    // lib.rs:219-221: the camera returned is the best pose seen.
+    for (int k = 0; k < 4; ++k) camera->quat[k] = s.best_q[k];
+    for (int k = 0; k < 3; ++k) camera->center[k] = s.best_c[k];
+    r->last_n = surfels;
+    r->last_views = 1;
+    r->last_batch = false;
+    r->have_forward = true;
+    return GS_OK;
+}
+
+gs_status gs_tracker_photometric(gs_tracker tr, void* stream, float gain, float bias, double* residual,
+                                 double* coverage) {
+    GS_REQUIRE(tr != nullptr, "tracker is null");
+    gs_raster r = tr->raster;
+    hipStream_t st = as_stream(stream, tr->ctx->stream);
+    const uint32_t P = r->width * r->height;
+    GS_HIP(launch_photometric(st, P, gain, bias, r->core.out_color, tr->target, r->dl_dcolor, tr->partials));
+    if (residual || coverage) {
+        std::vector<float2> p(tr->n_partials);
+        GS_HIP(hipStreamSynchronize(st));
+        GS_HIP(hipMemcpy(p.data(), tr->partials, p.size() * sizeof(float2), hipMemcpyDeviceToHost));
+        double e = 0.0, c = 0.0;
+        for (const auto& v : p) {
+            e += v.x;
+            c += v.y;
+        }
+        if (residual) *residual = e / (3.0 * P);
+        if (coverage) *coverage = c / P;
+    }
+    return GS_OK;
+}
+
+}  // extern "C"
+
 // ======================================================== 3DGS .ply export
 
 namespace gs {

@@ -209,3 +209,70 @@ def config_scene(index: int, n: int | None = None) -> SceneArrays:
         # the last of 16 frames); frame k renders a prefix of it.
         return make_scene(n or frame_surfel_counts(16, 500_000)[-1], 3, -5.0, 0.5, seed)
     raise ValueError(index)
+
+
+def _swatch(face: int, i: int, j: int):
+    """A stable pseudo-random mid-bright colour per (face, i, j)
+    (crates/gs-map/src/synthetic.rs:68-82)."""
+    h = ((face * 0x9E3779B9) ^ (i * 0x85EBCA6B) ^ (j * 0xC2B2AE35)) & 0xFFFFFFFF
+    out = []
+    for _ in range(3):
+        h ^= h >> 15
+        h = (h * 0x2545F491) & 0xFFFFFFFF
+        h ^= h >> 13
+        out.append(np.float32(0.25) + np.float32(0.6) * np.float32((h >> 8) / float(1 << 24)))
+    return out
+
+
+def _quat_arc_from_z(n):
+    """Unit quaternion (xyzw) rotating +z onto the unit vector n (glam
+    Quat::from_rotation_arc); for n = -z, a half turn about x."""
+    z = np.array([0.0, 0.0, 1.0])
+    d = float(np.dot(z, n))
+    if d < -1.0 + 1e-6:
+        return np.array([1.0, 0.0, 0.0, 0.0])
+    c = np.cross(z, n)
+    q = np.array([c[0], c[1], c[2], 1.0 + d])
+    return q / np.linalg.norm(q)
+
+
+def room(per_side: int = 24, size: float = 3.0) -> SceneArrays:
+    """The tracking tests' synthetic room (crates/gs-map/src/synthetic.rs:22-66):
+    six walls of per_side x per_side round surfels (radius 0.75 x the spacing,
+    opacity 0.95, normals facing inward, hashed colours), SH degree 0."""
+    n = max(per_side, 2)
+    step = 2.0 * size / (n - 1)
+    radius = step * 0.75
+    pos, quats, cols = [], [], []
+    faces = [((1, 0, 0), 1.0), ((1, 0, 0), -1.0), ((0, 1, 0), 1.0), ((0, 1, 0), -1.0), ((0, 0, 1), 1.0),
+             ((0, 0, 1), -1.0)]
+    for fi, (axis, sign) in enumerate(faces):
+        axis = np.array(axis, np.float64)
+        u = np.array([0.0, 1.0, 0.0]) if axis[0] != 0.0 else np.array([1.0, 0.0, 0.0])
+        v = np.cross(axis, u)
+        v /= np.linalg.norm(v)
+        q = _quat_arc_from_z(-axis * sign)
+        for i in range(n):
+            for j in range(n):
+                a = -size + i * step
+                b = -size + j * step
+                pos.append(axis * sign * size + u * a + v * b)
+                quats.append(q)
+                cols.append(_swatch(fi, i, j))
+    m = len(pos)
+    rgb = np.array(cols, np.float64)
+    return SceneArrays(positions=np.array(pos, np.float32), scales=np.full((m, 2), radius, np.float32),
+                       quats=np.array(quats, np.float32), opacities=np.full(m, 0.95, np.float32),
+                       sh=((rgb - 0.5) / SH_C0).astype(np.float32), sh_degree=0)
+
+
+def quat_from_rotation_y(angle: float) -> np.ndarray:
+    return np.array([0.0, np.sin(angle / 2), 0.0, np.cos(angle / 2)])
+
+
+def quat_from_rotation_x(angle: float) -> np.ndarray:
+    return np.array([np.sin(angle / 2), 0.0, 0.0, np.cos(angle / 2)])
+
+
+def quat_mul(a, b) -> np.ndarray:
+    return _quat_mul(a, b)
