@@ -90,6 +90,8 @@ def lib():
         L.oref_marginal_decisions.restype = C.c_int64
         L.oref_pixel_contributors.argtypes = [C.POINTER(_Scene), C.POINTER(_Camera), C.c_int64,
                                               C.POINTER(C.c_int32), C.POINTER(C.c_uint8)]
+        L.oref_prepare_rects.argtypes = [C.POINTER(_Scene), C.POINTER(_Camera), C.POINTER(C.c_int32),
+                                         C.POINTER(C.c_uint8), C.POINTER(C.c_uint8)]
         L.oref_normal_loss.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp,
                                        _dp, _dp, _dp]
         L.oref_trace.argtypes = [C.POINTER(_Scene), C.POINTER(_Camera), C.c_int64, _u32p, _u32p]
@@ -369,6 +371,22 @@ def replay(scene: Scene, cam: Camera, lists, counts, dl_dcolor=None, dl_ddepth=N
         g.cam_quat = np.array(cg.cam_quat[:])
         g.focal = float(cg.focal)
     return color.reshape(H, W, 3), alpha.reshape(H, W), g
+
+
+def prepare_rects(scene: Scene, cam: Camera):
+    """forward.rs:104-123 in FP64: (rects (n, 4) int32 tile x0 y0 x1 y1,
+    on_screen (n,) bool, culled (n,) bool: behind the near plane)."""
+    n = scene.n
+    rects = np.zeros((n, 4), np.int32)
+    on = np.zeros(n, np.uint8)
+    cul = np.zeros(n, np.uint8)
+    s, keep = scene._c()
+    c = cam._c()
+    if lib().oref_prepare_rects(C.byref(s), C.byref(c), rects.ctypes.data_as(C.POINTER(C.c_int32)),
+                                on.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                cul.ctypes.data_as(C.POINTER(C.c_uint8))) != 0:
+        raise RuntimeError("oref_prepare_rects failed")
+    return rects, on.astype(bool), cul.astype(bool)
 
 
 def prepare_order(scene: Scene, cam: Camera) -> np.ndarray:

@@ -618,6 +618,31 @@ int64_t oref_prepare_order(const oref_scene* s, const oref_camera* cam, int64_t*
     return (int64_t)surfels.size();
 }
 
+// forward.rs:104-123 per surfel (FP64): rects[4 i] = (x0, y0, x1, y1) tile
+// rect and on_screen[i] for every surfel in front of the near plane;
+// culled[i] = 1 for those behind it (rect left zero). A test aid.
+int oref_prepare_rects(const oref_scene* s, const oref_camera* cam, int32_t* rects, uint8_t* on_screen,
+                       uint8_t* culled) {
+    if (!s || !cam || !rects || !on_screen || !culled) return -1;
+    Scene sc{s, num_coeffs(s->sh_degree)};
+    M3 r_cam = quat_to_mat(cam->quat);
+    V3 C = cam_center(cam);
+    for (int64_t i = 0; i < s->n; ++i) {
+        V3 c = r_cam.tmul(sc.pos(i) - C);
+        culled[i] = -c.z < NEAR_DEPTH;
+        on_screen[i] = 0;
+        rects[4 * i] = rects[4 * i + 1] = rects[4 * i + 2] = rects[4 * i + 3] = 0;
+        if (culled[i]) continue;
+        SurfelCam sf = make_surfel_cam(sc, cam, r_cam, i);
+        on_screen[i] = sf.on_screen;
+        rects[4 * i] = sf.x0;
+        rects[4 * i + 1] = sf.y0;
+        rects[4 * i + 2] = sf.x1;
+        rects[4 * i + 3] = sf.y1;
+    }
+    return 0;
+}
+
 // forward.rs:207-246
 int oref_render(const oref_scene* s, const oref_camera* cam, int mode, double* color,
                 double* alpha, double* depth, double* normal, int64_t* n_hits) {

@@ -114,6 +114,9 @@ int oref_pixel_contributors(const oref_scene* s, const oref_camera* cam, int64_t
 
 /* Number of surfels that survive the near cull and their prepare() order
  * (indices sorted by (f32 depth bits, idx)). order may be NULL. */
+/* forward.rs:104-123 tile rects (FP64) of every surfel; a test aid. */
+int oref_prepare_rects(const oref_scene* s, const oref_camera* cam, int32_t* rects, uint8_t* on_screen,
+                       uint8_t* culled);
 /* normal_loss.wgsl:52-145 (2DGS normal consistency) in FP64. */
 void oref_normal_loss(int w, int h, double focal, double lambda, double alpha_min, const double* color4,
                       const double* aux4, const double* normal4, double* loss_map, double* dl_dnormal,

@@ -137,6 +137,42 @@ def test_prep_f32_agrees_with_fp64_prepare():
     assert counts.sum() > 0
 
 
+@pytest.mark.parametrize("seed,n,size", [(404, 50, 100), (101, 400, 256), (7, 3000, 512)])
+def test_prep_f32_rects_agree_with_fp64_rects(seed, n, size):
+    """The fp32 mirror's tile rects (the HIP projection's, bit-exact) against
+    FP64 forward.rs:104-123 rects: equal except where the FP64 bound
+    (p -+ r) / 16 lies within f32 rounding (1e-5 relative) of an integer tile
+    edge, so truncation may go either way; those are counted and must be
+    rare. Off-screen and near-culled surfels agree exactly (an empty rect in
+    the mirror)."""
+    scene, cam = make_scene(seed, n, 1, size, flavor="parity")
+    rot = oracle.quat_to_mat(cam.quat)
+    c32 = oracle.cam32(rot, cam.center, cam.focal, cam.width, cam.height)
+    pos4 = np.concatenate([scene.pos, np.ones((scene.n, 1))], 1)
+    rects, keys, counts = oracle.prep_f32(pos4, scene.scales, scene.quat, c32)
+    r64, on64, cul64 = oracle.prepare_rects(scene, cam)
+    live = (rects[:, 0] <= rects[:, 2]) & (rects[:, 1] <= rects[:, 3]) & (keys != 0)
+    # FP64 bounds before truncation, for the straddle check.
+    c = (scene.pos - cam.center) @ rot
+    w = -c[:, 2]
+    R = [oracle.quat_to_mat(q) for q in scene.quat]
+    tu = np.array([rot.T @ (Ri[:, 0] * s[0]) for Ri, s in zip(R, scene.scales)])
+    tv = np.array([rot.T @ (Ri[:, 1] * s[1]) for Ri, s in zip(R, scene.scales)])
+    r_px = np.minimum(3.0 * cam.focal * (np.linalg.norm(tu, axis=1) + np.linalg.norm(tv, axis=1)) / w + 3.0, 64.0)
+    px = cam.width * 0.5 + cam.focal * c[:, 0] / w
+    py = cam.height * 0.5 - cam.focal * c[:, 1] / w
+    bounds = np.stack([(px - r_px) / 16, (py - r_px) / 16, (px + r_px) / 16, (py + r_px) / 16], 1)
+    near_edge = np.abs(bounds - np.round(bounds)) <= 1e-5 * np.maximum(np.abs(bounds), 1.0)
+    both = live & on64 & ~cul64
+    diff = np.any(rects.astype(np.int64) != r64, axis=1) & both
+    straddle = np.any(near_edge, axis=1)
+    print(f"  {int(both.sum())} on-screen surfels; rects differing {int(diff.sum())}, all at a tile edge")
+    assert not np.any(diff & ~straddle)
+    assert diff.sum() <= max(2, both.sum() // 1000)
+    # On-screen / culled status agrees (an empty mirror rect = not on screen or culled).
+    assert np.array_equal(live, on64 & ~cul64)
+
+
 # ------------------------------------------------------------------- SSIM
 
 def test_gauss_window_is_normalized_gaussian():
@@ -249,3 +285,36 @@ def test_adam_matches_reference_cpu_loop(mode):
     assert np.abs(raw - p).max() < 1e-5
     want_act = {0: raw, 1: np.exp(raw), 2: 1 / (1 + np.exp(-raw))}[mode]
     np.testing.assert_allclose(act, want_act, rtol=1e-6)
+
+
+def _ssim_sum(w, h, a, b):
+    img = np.full((h, w, 3), a)
+    tgt = np.full((h, w, 3), b)
+    _, _, _, ss = oracle.l1_ssim(img, tgt, 0.2, 1.0)
+    return ss * w * h  # per-channel sum (all channels equal)
+
+
+@pytest.mark.parametrize("a,b", [(0.5, 0.3), (0.9, 0.1), (0.2, 0.2), (0.05, 0.7)])
+def test_ssim_constant_images_closed_form(a, b):
+    """Constant images x = a, y = b: at a pixel 5 or more from every edge the
+    window sums to S = (sum of the 11 taps)^2, so mu_x = a S, E[x^2] = a^2 S,
+    E[xy] = a b S and SSIM = (2abS^2 + C1)(2abS(1-S) + C2) /
+    (((a^2+b^2)S^2 + C1)((a^2+b^2)S(1-S) + C2)) with C1 = 1e-4, C2 = 9e-4
+    (ssim.wgsl:29-34). A pixel's SSIM depends only on its distances to the
+    edges, so the mixed second difference of the image sums over sizes
+    (W, H), (W+1, H), (W, H+1), (W+1, H+1) is exactly one interior pixel."""
+    taps = np.array([0.26601818, 0.21300237, 0.10936069, 0.03599398, 0.00759866, 0.00102838])
+    s1 = taps[0] + 2 * taps[1:].sum()
+    S = s1 * s1
+    C1, C2 = 1e-4, 9e-4
+    want = ((2 * a * b * S * S + C1) * (2 * a * b * S * (1 - S) + C2) /
+            (((a * a + b * b) * S * S + C1) * ((a * a + b * b) * S * (1 - S) + C2)))
+    W, H = 17, 19
+    got = (_ssim_sum(W + 1, H + 1, a, b) - _ssim_sum(W + 1, H, a, b) - _ssim_sum(W, H + 1, a, b)
+           + _ssim_sum(W, H, a, b))
+    assert abs(got - want) < 1e-10, (got, want)
+    # Swapped constants move the value well outside that bound.
+    alt = ((2 * a * b * S * S + C2) * (2 * a * b * S * (1 - S) + C1) /
+           (((a * a + b * b) * S * S + C2) * ((a * a + b * b) * S * (1 - S) + C1)))
+    if a != b:
+        assert abs(alt - want) > 1e-6
