@@ -96,7 +96,7 @@ def lib():
                                        _dp, _dp, _dp]
         L.oref_trace.argtypes = [C.POINTER(_Scene), C.POINTER(_Camera), C.c_int64, _u32p, _u32p]
         L.oref_replay.argtypes = [C.POINTER(_Scene), C.POINTER(_Camera), _u32p, _u32p, C.c_int64, _dp,
-                                  _dp, _dp, _dp, _dp, C.POINTER(_Grads)]
+                                  _dp, _dp, _dp, _dp, C.c_double, C.POINTER(_Grads)]
         L.oref_prepare_order.argtypes = [C.POINTER(_Scene), C.POINTER(_Camera), _i64p]
         L.oref_prepare_order.restype = C.c_int64
         L.oref_prep_f32.argtypes = [C.c_int64, _fp, _fp, _fp, C.POINTER(_Cam32), _u32p, _u32p,
@@ -335,7 +335,8 @@ def trace(scene: Scene, cam: Camera, cap: int = 1024):
     return lists, counts
 
 
-def replay(scene: Scene, cam: Camera, lists, counts, dl_dcolor=None, dl_ddepth=None, dl_dnormal=None):
+def replay(scene: Scene, cam: Camera, lists, counts, dl_dcolor=None, dl_ddepth=None, dl_dnormal=None,
+           lambda_dist: float = 0.0):
     """Decision replay (test aid, no forward.rs counterpart): the FP64
     forward, and with dl_dcolor the backward, along another evaluation's
     per-pixel contribution lists over the camera's window (lists (P, cap) of
@@ -363,7 +364,7 @@ def replay(scene: Scene, cam: Camera, lists, counts, dl_dcolor=None, dl_ddepth=N
     dln = _d(dl_dnormal).reshape(-1) if dl_dnormal is not None else None
     rc = lib().oref_replay(C.byref(s), C.byref(c), _ptr(lists, _u32p), _ptr(counts, _u32p), lists.shape[1],
                            _ptr(color, _dp), _ptr(alpha, _dp), _ptr(dlc, _dp), _ptr(dld, _dp), _ptr(dln, _dp),
-                           C.byref(cg) if cg is not None else None)
+                           float(lambda_dist), C.byref(cg) if cg is not None else None)
     if rc != 0:
         raise RuntimeError(f"oref_replay failed ({rc}: a count above cap or a surfel index out of range)")
     if g is not None:

@@ -1077,15 +1077,15 @@ int oref_trace(const oref_scene* s, const oref_camera* cam, int64_t cap, uint32_
 // (no near-plane or det test: the other evaluation took the branch) and the
 // given clamp (alpha = 0.995 if clamped, else opacity * ghat with no skip
 // test); the walk ends with the list (the transmittance stop is replayed
-// too). The surfel's colour clamp stays FP64's. dl_ddepth (P) and dl_dnormal
-// (3P) optional, as in oref_gradients. Gradients follow the
+// too). The surfel's colour clamp stays FP64's. dl_ddepth (P), dl_dnormal
+// (3P) and lambda_dist as in oref_gradients. Gradients follow the
 // backward.rs:112-245 chain along the replayed walk, so they are the exact
 // derivative of the function the decisions define. color (P x 3) and alpha
 // (P) optional; dl_dcolor + g optional (P x 3). Returns -2 if a count
 // exceeds cap or a surfel index is out of range.
 int oref_replay(const oref_scene* s, const oref_camera* cam, const uint32_t* lists, const uint32_t* counts,
                 int64_t cap, double* color, double* alpha, const double* dl_dcolor, const double* dl_ddepth,
-                const double* dl_dnormal, oref_grads* g) {
+                const double* dl_dnormal, double lambda_dist, oref_grads* g) {
     if (!s || !cam || !lists || !counts || cap <= 0) return -1;
     Scene sc{s, num_coeffs(s->sh_degree)};
     const int64_t n = s->n;
@@ -1160,8 +1160,8 @@ int oref_replay(const oref_scene* s, const oref_camera* cam, const uint32_t* lis
                 V3 dlc(dl_dcolor[3 * pi], dl_dcolor[3 * pi + 1], dl_dcolor[3 * pi + 2]);
                 double dld = dl_ddepth ? dl_ddepth[pi] : 0.0;
                 V3 dln = dl_dnormal ? V3(dl_dnormal[3 * pi], dl_dnormal[3 * pi + 1], dl_dnormal[3 * pi + 2]) : V3();
-                if (!is_zero(dlc) || dld != 0.0 || !is_zero(dln))
-                    reverse_walk(all, cam, d, pix_x, pix_y, w_total, wt_total, dlc, dld, dln, 0.0, w);
+                if (!is_zero(dlc) || dld != 0.0 || !is_zero(dln) || lambda_dist != 0.0)
+                    reverse_walk(all, cam, d, pix_x, pix_y, w_total, wt_total, dlc, dld, dln, lambda_dist, w);
             }
         }
     }

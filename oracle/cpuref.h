@@ -127,7 +127,7 @@ int oref_trace(const oref_scene* s, const oref_camera* cam, int64_t cap, uint32_
  * contribution lists (surfel | ray << 30 | clamped << 31), see cpuref.cpp. */
 int oref_replay(const oref_scene* s, const oref_camera* cam, const uint32_t* lists, const uint32_t* counts,
                 int64_t cap, double* color, double* alpha, const double* dl_dcolor, const double* dl_ddepth,
-                const double* dl_dnormal, oref_grads* g);
+                const double* dl_dnormal, double lambda_dist, oref_grads* g);
 
 int64_t oref_prepare_order(const oref_scene* s, const oref_camera* cam, int64_t* order);
 

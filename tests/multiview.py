@@ -123,28 +123,124 @@ def total_loss(s64, specs, tgts, grad=False, counts=None):
     return total, acc
 
 
-def replay_conditioning(s64, cam, lists, counts, dl, g_ref, img_ref, draws=2, rel=1e-5, seed=0):
-    """Conditioning of the replayed FP64 evaluation: every input is scaled by
-    1 +- 2^-24 (random signs) and replayed along the same decision lists
+def rendered_scene(tr, sc):
+    """The parameters a trainer renders with: its activated buffers hold
+    exp / sigmoid of the raw log-scales and opacity logits (evaluated on the
+    device), a few ulp from the host arrays it was created from."""
+    from splattar_amd import scenes
+    ncoef = (sc.sh_degree + 1) ** 2
+    return scenes.SceneArrays(tr.positions.read()[:, :3].copy(), tr.scales.read().copy(),
+                              tr.quats.read().copy(), tr.opacities.read().copy(),
+                              tr.sh.read()[:, :ncoef * 3].copy(), sc.sh_degree)
+
+
+def trace_views(gpu, sc, specs, counts=None, cap=160):
+    """Per view, the contribution lists of the GPU forward with the
+    trainer's binning (TIGHT) over the whole image: [(lists, counts, image)]."""
+    import splattar_amd as sa
+    n = sc.n
+    counts = counts or [n] * len(specs)
+    out = []
+    for sp, m in zip(specs, counts):
+        r = sa.Rasterizer(gpu, n, sp.width, sp.height, n * 64)
+        try:
+            r.upload_scene(sc.positions, sc.scales, sc.quats, sc.opacities, sc.sh)
+            r.set_binning(sa._lib.GS_BINNING_TIGHT)
+            r.set_trace(0, 0, sp.width, sp.height, cap)
+            r.forward(sa.Camera(np.asarray(sp.center, np.float32), np.asarray(sp.quat, np.float32),
+                                float(np.float32(sp.focal)), sc.sh_degree), m)
+            lists, cnt = r.trace()
+            img = r.out_color.read()[..., :3].astype(np.float64)
+        finally:
+            r.close()
+        assert cnt.max() <= cap, f"a pixel composites {cnt.max()} surfels, above the trace cap {cap}"
+        out.append((lists, cnt, img))
+    return out
+
+
+def _greedy_ill(moves, g_ref, rel):
+    """Per class k, the fewest, largest movers whose exclusion leaves the rest
+    moving by at most rel |g_k| (sqrt of the remaining squared moves)."""
+    n = next(iter(moves.values())).size
+    ill = np.zeros(n, bool)
+    for k, mv in moves.items():
+        budget = rel * np.linalg.norm(getattr(g_ref, k))
+        order = np.argsort(-mv)
+        # rest[m] = sqrt of the squared moves of all but the m largest.
+        rest = np.sqrt(np.concatenate([np.cumsum((mv[order] ** 2)[::-1])[::-1], [0.0]]))
+        m = int(np.argmax(rest <= budget))
+        ill[order[:m]] = True
+    return ill
+
+
+def replay_views(s64, specs, tgts, traces, draws=2, rel=1e-5, seed=0):
+    """The summed multi-view FP64 gradient along the GPU's decisions
+    (trace_views): each view's image replayed, its L1 + D-SSIM gradient taken
+    at the replayed image, the parameter gradient replayed and summed. With
+    it the conditioning mask: scene and per-view loss gradients scaled by
+    1 +- 2^-24 (random signs, one scene draw for all views), the sum replayed
+    again, and per class the fewest, largest movers excluded so that the rest
+    moves by at most rel normwise (what f32 inputs determine to rel).
+    Returns (gradients, ill, replayed images)."""
+    n = s64.n
+    acc, dls, imgs = None, [], []
+    for sp, t, (lists, cnt, _) in zip(specs, tgts, traces):
+        cam = camera64(sp)
+        img, _, _ = oracle.replay(s64, cam, lists, cnt)
+        _, dl, _, _ = oracle.l1_ssim(img, t.astype(np.float64), LAM, DSSIM_SCALE)
+        _, _, g = oracle.replay(s64, cam, lists, cnt, dl)
+        dls.append(dl)
+        imgs.append(img)
+        acc = g if acc is None else _add(acc, g)
+    rng = np.random.default_rng(seed)
+    moves = {k: np.zeros(n) for k in ("pos", "scales", "quat", "opacity", "sh")}
+    for _ in range(draws):
+        sp64 = s64.copy()
+        for arr in (sp64.pos, sp64.scales, sp64.quat, sp64.opacity, sp64.sh):
+            arr *= 1.0 + rng.choice([-1.0, 1.0], arr.shape) * 2.0 ** -24
+        acc2 = None
+        for sp, dl, (lists, cnt, _) in zip(specs, dls, traces):
+            dlj = dl * (1.0 + rng.choice([-1.0, 1.0], dl.shape) * 2.0 ** -24)
+            _, _, g2 = oracle.replay(sp64, camera64(sp), lists, cnt, dlj)
+            acc2 = g2 if acc2 is None else _add(acc2, g2)
+        for k in moves:
+            d = getattr(acc2, k).reshape(n, -1) - getattr(acc, k).reshape(n, -1)
+            moves[k] = np.maximum(moves[k], np.linalg.norm(d, axis=1))
+    return acc, _greedy_ill(moves, acc, rel), imgs
+
+
+def _add(a, b):
+    for f in ("pos", "scales", "quat", "opacity", "sh"):
+        setattr(a, f, getattr(a, f) + getattr(b, f))
+    return a
+
+
+def replay_conditioning(s64, cam, lists, counts, dl, g_ref, img_ref, draws=2, rel=1e-5, seed=0,
+                        dl_ddepth=None, dl_dnormal=None, lambda_dist=0.0):
+    """Conditioning of the replayed FP64 evaluation: every input -- the scene
+    and the per-pixel loss gradients, whose one-ulp perturbation stands for
+    the rounding of each pixel's term in an f32 sum over pixels -- is scaled
+    by 1 +- 2^-24 (random signs) and replayed along the same decision lists
     (oracle.replay), `draws` times. Returns (ill, pixel_sens):
-    - ill[i]: some class of surfel i's gradient moved by more than rel x
-      max(its own norm, 1e-3 x the class's largest surfel norm);
-    - pixel_sens (H, W): the largest change of the pixel's colour.
-    Neither is determined better than this by f32 inputs, whatever
-    evaluates them."""
+    - ill[i]: surfel i is among the fewest, largest movers of some class k
+      whose exclusion leaves that class moving by at most rel normwise
+      (sqrt of the remaining squared moves <= rel |g_k|): what remains is
+      determined by f32 inputs to rel, whatever evaluates it;
+    - pixel_sens (H, W): the largest change of the pixel's colour."""
     n = s64.n
     rng = np.random.default_rng(seed)
-    ill = np.zeros(n, bool)
     sens = np.zeros(img_ref.shape[:2])
+    moves = {k: np.zeros(n) for k in ("pos", "scales", "quat", "opacity", "sh")}
     for _ in range(draws):
         sp = s64.copy()
         for arr in (sp.pos, sp.scales, sp.quat, sp.opacity, sp.sh):
             arr *= 1.0 + rng.choice([-1.0, 1.0], arr.shape) * 2.0 ** -24
-        img2, _, g2 = oracle.replay(sp, cam, lists, counts, dl)
+        def jig(a):
+            return None if a is None else np.asarray(a) * (1.0 + rng.choice([-1.0, 1.0], np.shape(a)) * 2.0 ** -24)
+        img2, _, g2 = oracle.replay(sp, cam, lists, counts, jig(dl), jig(dl_ddepth), jig(dl_dnormal), lambda_dist)
         sens = np.maximum(sens, np.abs(img2 - img_ref).max(axis=2))
-        for k in ("pos", "scales", "quat", "opacity", "sh"):
+        for k in moves:
             a = getattr(g_ref, k).reshape(n, -1)
             b = getattr(g2, k).reshape(n, -1)
-            own = np.linalg.norm(a, axis=1)
-            ill |= np.linalg.norm(b - a, axis=1) > rel * np.maximum(own, 1e-3 * own.max())
-    return ill, sens
+            moves[k] = np.maximum(moves[k], np.linalg.norm(b - a, axis=1))
+    return _greedy_ill(moves, g_ref, rel), sens

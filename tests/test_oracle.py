@@ -445,3 +445,20 @@ def test_replay_gradients_match_central_differences_with_the_decisions_fixed(see
             assert abs(fd - an) <= 1e-7 + 1e-3 * max(abs(fd), abs(an)), (cls, idx, fd, an)
             checked += 1
     assert checked == 30
+
+
+@pytest.mark.parametrize("seed,n,size", [(71, 10, 20), (83, 8, 24)])
+def test_replay_with_aux_losses_is_the_faithful_evaluation(seed, n, size):
+    """The replay of the FP64 walk's own decisions with depth, normal and
+    distortion terms reproduces oracle.gradients (gradcheck.rs:232-326 case)."""
+    scene, cam = make_scene(seed, n, 0, size)
+    P = size * size
+    w_color = make_weights(seed ^ 0x11, P)
+    w_normal = make_weights(seed ^ 0x22, P)
+    w_depth = make_uniform(seed ^ 0x33, P, -0.3, 0.3)
+    g = oracle.gradients(scene, cam, w_color, w_depth, w_normal, 0.35)
+    lists, counts = oracle.trace(scene, cam, 256)
+    _, _, gr = oracle.replay(scene, cam, lists, counts, w_color.reshape(size, size, 3), w_depth, w_normal, 0.35)
+    for k in ("pos", "scales", "quat", "opacity"):
+        np.testing.assert_allclose(getattr(gr, k), getattr(g, k), rtol=1e-11, atol=1e-14, err_msg=k)
+    np.testing.assert_allclose(gr.cam_center, g.cam_center, rtol=1e-10, atol=1e-13)
