@@ -410,7 +410,7 @@ def main():
     # saved before and restored after, so it does not advance the run).
     tr = sa.Trainer(ctx, cfg, scene.as_init(), views)
     schedule = {"digit_bits": a.digit_bits, "blocks_per_cu": None, "tile_size": 16, "fwd_batch": 256,
-                "bwd_batch": 128}
+                "bwd_batch": 64}
     if a.schedule:
         f = [int(x) for x in a.schedule.split(",")]
         schedule.update(digit_bits=f[0], blocks_per_cu=f[1])

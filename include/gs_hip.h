@@ -510,8 +510,8 @@ gs_status gs_trainer_step(gs_trainer t, void* stream, const uint32_t* view_ids, 
                           uint32_t mode);
 /* Blocking. No counterpart in the reference (its tile size, LDS batch,
  * digit width and dispatch shape are fixed). Stage 1 tries every work tiling
- * (tile_size, fwd_batch, bwd_batch) in {(16, 256, 128), (16, 128, 64),
- * (32, 256, 128), (32, 512, 256), (8, 128, 128), (8, 64, 64)} (16-px only
+ * (tile_size, fwd_batch, bwd_batch) in {(16, 256, 64), (16, 256, 128),
+ * (16, 128, 64), (32, 256, 128), (32, 512, 256), (8, 128, 128), (8, 64, 64)} (16-px only
  * unless binning is GS_BINNING_TIGHT) x raster scheduling {persistent tile
  * queue with 8 or 4 workgroups per CU, one workgroup per tile} at 8-bit
  * digits; stage 2 tries radix digit widths {8, 4} at the stage-1 winner.
@@ -537,7 +537,7 @@ gs_status gs_trainer_set_schedule(gs_trainer t, uint32_t digit_bits, uint32_t bl
  * stage fwd_batch / bwd_batch records per LDS batch. Supported:
  * tile 8: fwd 64 | 128, bwd 64 | 128; tile 16: fwd 128 | 256, bwd 64 | 128;
  * tile 32: fwd 256 | 512, bwd 128 | 256. Tiles other than 16 need
- * GS_BINNING_TIGHT. Default (16, 256, 128). */
+ * GS_BINNING_TIGHT. Default (16, 256, 64). */
 gs_status gs_trainer_set_tiling(gs_trainer t, uint32_t tile_size, uint32_t fwd_batch, uint32_t bwd_batch);
 /* Overlapped batched steps (default off; no counterpart in the reference):
  * with two or more views the step splits them in halves and runs each

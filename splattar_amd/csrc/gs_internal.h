@@ -644,12 +644,12 @@ struct TileQueue {
 // Instantiated: forward (tile, batch) in {(8, 64), (8, 128), (16, 128),
 // (16, 256), (32, 256), (32, 512)}, backward in {(8, 64), (8, 128),
 // (16, 64), (16, 128), (32, 128), (32, 256)}; the depth / normal (aux)
-// variants only at (16, 256) / (16, 128). Others return
-// hipErrorInvalidValue.
+// variants on 16-px tiles only (forward batch 256; the aux backward stages
+// 128 records whatever bwd_batch says). Others return hipErrorInvalidValue.
 struct RasterShape {
     uint32_t tile = 16;
     uint32_t fwd_batch = 256;
-    uint32_t bwd_batch = 128;
+    uint32_t bwd_batch = 64;
 };
 // The pass covers total_tiles tiles: queue.order[0 .. total_tiles) with a
 // queue, else the global tiles [tile_begin, tile_begin + total_tiles).

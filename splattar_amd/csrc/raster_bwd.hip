@@ -10,20 +10,21 @@
 //     Nt, px, py, depth, colour, opacity, normal); the linear chain to
 //     camera-space (c, tu, tv) happens once per surfel in geom_bwd instead of
 //     four triple-product gradients per pixel-hit;
-//   * a TW x TW work tile (TW = 16 or 32) is TW^2 / 128 wave64s, each owning
-//     an 8-wide, 16-tall column of two 8x8 blocks with two pixels per lane
-//     (rows y and y + 8); an 8x8 work tile is one wave with one pixel per
-//     lane. Records are staged BATCH at a time, split into per-wave lists
-//     that carry which of the wave's blocks the record's footprint reaches,
-//     so a record is evaluated only where it can contribute and its two
-//     pixels' terms are summed in registers before one reduction;
-//   * per record, each wave64 reduces its lanes' contributions with a
-//     transposed butterfly (17 lane exchanges for 16 values instead of 96,
-//     on permlane swaps and DPP rather than LDS permutes), adds them into an
-//     LDS slot (ds_add_f32), and the batch's slots are flushed with hardware
-//     global FP32 atomics (no CAS loops).
-// AUX = true adds the depth / normal image losses and the in-walk depth
-// distortion loss (backward.rs:126-153).
+//   * a 16x16 work tile is one wave64 with four pixels per lane, one in each
+//     8x8 block (raster_bwd_wave_kernel): a record is evaluated only in the
+//     blocks its footprint reaches, its terms over the tile are summed in
+//     registers, reduced once per (record, tile) and added to the record's
+//     gradient with hardware global FP32 atomics (no CAS loops). A 32x32
+//     tile is eight waves, each owning an 8-wide, 16-tall column of two
+//     blocks (two pixels per lane), whose per-record reductions meet in LDS
+//     slots (ds_add_f32) flushed once per batch; an 8x8 tile is one wave with
+//     one pixel per lane (raster_bwd_kernel). Records are staged BATCH at a
+//     time into order-preserving lists that carry the blocks each reaches;
+//   * each wave reduces its lanes' 16 values with a transposed butterfly (17
+//     lane exchanges instead of 96, on permlane swaps and DPP rather than LDS
+//     permutes).
+// AUX = true (16-px tiles, raster_bwd_kernel) adds the depth / normal image
+// losses and the in-walk depth distortion loss (backward.rs:126-153).
 
 #include "gs_internal.h"
 
@@ -430,6 +431,142 @@ __global__ __launch_bounds__(TW == 8 ? 64 : TW * TW / 2) void raster_bwd_kernel(
     }
 }
 
+// Colour-only backward over 16-px work tiles with one wave per tile: lane l
+// owns pixel (l & 7, l >> 3) of each of the tile's four 8x8 blocks, so a
+// record's terms over the whole tile are summed in registers and reduced
+// once per (record, tile) instead of once per half tile, and the reduced
+// values go straight to the global FP32 atomics (no LDS accumulation slots:
+// one wave is the only contributor of its tile). The records a batch stages
+// form one order-preserving list carrying each record's 4-bit block mask.
+template <int BATCH>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void raster_bwd_wave_kernel(
+    const CamParams* __restrict__ cams, uint32_t n_views, const uint32_t* __restrict__ ranges,
+    const uint32_t* __restrict__ payloads, const Record* __restrict__ recs, BwdIn in,
+    float* __restrict__ grad_coef, bool cull, uint32_t total_tiles, TileQueue queue, uint32_t tile_begin) {
+    constexpr int NS = grad_stride(false);
+    constexpr int SPT = BATCH / 64;          // staging rounds per batch
+    constexpr int MB = 10;                   // list entry: slot | block mask << MB
+    static_assert(BATCH % 64 == 0 && BATCH <= (1 << MB), "batch of whole waves");
+    __shared__ float4 s_m[BATCH], s_a[BATCH], s_b[BATCH], s_c[BATCH];
+    __shared__ float s_lthr[BATCH];
+    __shared__ uint32_t s_pay[BATCH];
+    __shared__ uint16_t s_list[BATCH];
+
+    const int lane = threadIdx.x;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (uint32_t iter = 0;; ++iter) {
+        uint32_t pos;
+        if (queue.cursor != nullptr) {
+            uint32_t claim = 0;
+            if (lane == 0) {
+                const uint32_t q = min(kTileQueues, gridDim.x), x = blockIdx.x % q;
+                claim = atomicAdd(queue.cursor + x, 1u) * q + x;
+            }
+            pos = __builtin_amdgcn_readfirstlane(claim);
+        } else {
+            pos = iter == 0 ? xcd_swizzle(blockIdx.x, gridDim.x) : total_tiles;
+        }
+        if (pos >= total_tiles) break;
+        const uint32_t gtile = queue.order != nullptr ? queue.order[pos] : tile_begin + pos;
+        const uint32_t start = ranges[2 * (size_t)gtile], end = ranges[2 * (size_t)gtile + 1];
+        if (end <= start) continue;
+        const uint32_t v = view_of_tile(cams, n_views, gtile);
+        const CamParams& cam = cams[v];
+        const uint32_t local = gtile - cam.tile_base;
+        const uint32_t tx = local % cam.tiles_x, ty = local / cam.tiles_x;
+        const uint32_t x0 = tx * 16, y0 = ty * 16;
+        PixState p[4];
+        uint32_t mylast = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            load_pixel<false>(p[q], cam, (int)x0 + 8 * (q & 1) + (lane & 7), (int)y0 + 8 * (q >> 1) + (lane >> 3),
+                              in, 0.0f);
+            mylast = max(mylast, p[q].last);
+        }
+        for (int o = 32; o >= 1; o >>= 1) mylast = max(mylast, (uint32_t)__shfl_xor((int)mylast, o));
+        const uint32_t maxlast = __builtin_amdgcn_readfirstlane(mylast);
+        if (maxlast == 0) continue;
+
+        const int nbatch = (int)((maxlast + BATCH - 1) / BATCH);
+        for (int b = nbatch - 1; b >= 0; --b) {
+            const uint32_t base = start + (uint32_t)b * BATCH;
+            const uint32_t cnt = min(min((uint32_t)BATCH, end - base), maxlast - (uint32_t)b * BATCH);
+            __syncthreads();  // the previous batch's walk has read its LDS
+            uint32_t count = 0;
+#pragma unroll
+            for (int m = 0; m < SPT; ++m) {
+                const int slot = lane + m * 64;
+                uint32_t qmask = 0;
+                if ((uint32_t)slot < cnt) {
+                    const uint32_t pay = payloads[base + slot];
+                    Record r = recs[pay];
+                    s_pay[slot] = pay;
+                    s_m[slot] = r.m_nt;
+                    s_a[slot] = r.a_px;
+                    s_b[slot] = r.b_py;
+                    s_c[slot] = r.col_op;
+                    s_lthr[slot] = record_lthr(r.col_op.w);
+                    qmask = cull ? block_mask<16>(r, x0, y0) : 0xFu;
+                }
+                const uint64_t bal = __ballot(qmask != 0u);
+                if (qmask != 0u)
+                    s_list[count + (uint32_t)__popcll(bal & lt_mask)] = (uint16_t)(slot | (qmask << MB));
+                count += (uint32_t)__popcll(bal);
+            }
+            __syncthreads();
+
+            for (int kk = (int)count - 1; kk >= 0; --kk) {
+                const uint32_t e = __builtin_amdgcn_readfirstlane((uint32_t)s_list[kk]);
+                const uint32_t j = e & ((1u << MB) - 1u);
+                const uint32_t qm = e >> MB;
+                const uint32_t rel = (uint32_t)b * BATCH + j;
+                bool need[4];
+                bool any_need = false;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    need[q] = ((qm >> q) & 1u) && __any(rel < p[q].last);
+                    any_need = any_need || need[q];
+                }
+                if (!any_need) continue;
+                Record r;
+                r.m_nt = s_m[j];
+                r.a_px = s_a[j];
+                r.b_py = s_b[j];
+                r.col_op = s_c[j];
+                r.nrm_dep = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                const float lthr = s_lthr[j];
+                float g[16];
+                float g2[4];
+                g[kGNt] = 0.0f;
+                bool first = true;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (!need[q]) continue;
+                    const HitEval h = eval_hit(r, p[q].sx, p[q].sy, p[q].pxc, p[q].pyc, lthr);
+                    const bool act = rel < p[q].last && h.alpha > 0.0f;
+                    if (!__any(act)) continue;
+                    if (first) add_hit<false, false>(p[q], r, h, act, 0.0f, g, g2);
+                    else add_hit<false, true>(p[q], r, h, act, 0.0f, g, g2);
+                    first = false;
+                }
+                if (first) continue;
+                const float red = wave_reduce_transpose<16>(g, lane);
+                if ((lane & 3) == 0 && red != 0.0f) {
+                    const int slot = lane >> 2;
+                    bool keep = true;
+                    if (slot >= kGCol && slot < kGCol + 3) {
+                        // A colour channel clamped at 0 in the forward passes no
+                        // gradient to SH (the record stores it as exactly 0).
+                        const float4 c = s_c[j];
+                        keep = (slot == kGCol ? c.x : (slot == kGCol + 1 ? c.y : c.z)) != 0.0f;
+                    }
+                    if (keep) atomicAdd(&grad_coef[(size_t)s_pay[j] * NS + slot], red);
+                }
+            }
+        }
+    }
+}
+
 template <bool AUX, int TW, int BATCH>
 void launch_bwd(hipStream_t st, uint32_t grid, const CamParams* cams_dev, uint32_t n_views,
                 uint32_t total_tiles, const uint32_t* ranges, const uint32_t* payloads, const Record* recs,
@@ -455,16 +592,24 @@ hipError_t launch_raster_bwd(hipStream_t st, const CamParams* cams_dev, const Ca
            tile_begin);
     };
     if (aux) {
-        if (t != 16 || b != 128) return hipErrorInvalidValue;  // depth / normal losses: 16-px tiles
+        if (t != 16) return hipErrorInvalidValue;  // depth / normal losses: 16-px tiles
         go(launch_bwd<true, 16, 128>);
     } else if (t == 8 && b == 64) {
         go(launch_bwd<false, 8, 64>);
     } else if (t == 8 && b == 128) {
         go(launch_bwd<false, 8, 128>);
-    } else if (t == 16 && b == 64) {
-        go(launch_bwd<false, 16, 64>);
-    } else if (t == 16 && b == 128) {
-        go(launch_bwd<false, 16, 128>);
+    } else if (t == 16) {
+        // One wave per tile: twice the workgroups for the same waves per CU.
+        const uint32_t g2 = queue.cursor != nullptr ? std::min(total_tiles, std::max(2 * queue.grid_blocks, 1u))
+                                                    : total_tiles;
+        if (b == 64)
+            hipLaunchKernelGGL(raster_bwd_wave_kernel<64>, dim3(g2), dim3(64), 0, st, cams_dev, n_views, ranges,
+                               payloads, recs, in, grad_coef, cull, total_tiles, queue, tile_begin);
+        else if (b == 128)
+            hipLaunchKernelGGL(raster_bwd_wave_kernel<128>, dim3(g2), dim3(64), 0, st, cams_dev, n_views, ranges,
+                               payloads, recs, in, grad_coef, cull, total_tiles, queue, tile_begin);
+        else
+            return hipErrorInvalidValue;
     } else if (t == 32 && b == 128) {
         go(launch_bwd<false, 32, 128>);
     } else if (t == 32 && b == 256) {

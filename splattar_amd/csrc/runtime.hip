@@ -2658,7 +2658,7 @@ gs_status gs_trainer_autotune(gs_trainer t, const uint32_t* view_ids, uint32_t n
     };
     std::vector<Cand> cands;
     const bool tight = t->core.binning == GS_BINNING_TIGHT;
-    static const uint32_t tilings[][3] = {{16, 256, 128}, {16, 128, 64}, {32, 256, 128},
+    static const uint32_t tilings[][3] = {{16, 256, 64}, {16, 256, 128}, {16, 128, 64}, {32, 256, 128},
                                           {32, 512, 256}, {8, 128, 128}, {8, 64, 64}};
     static const uint32_t queues[] = {8, 4, 0};
     // Stage 1: work tiling x raster scheduling at 8-bit digits; stage 2: the

@@ -384,11 +384,19 @@ def test_backward_aux_losses_match_fp64_oracle(gpu, seed, n, deg, size):
 
 
 def test_color_only_kernel_matches_full_kernel(gpu):
+    """The colour-only backward (one wave per 16-px tile, four pixels per
+    lane) against the full kernel (two waves, depth / normal terms zero):
+    the same sums in another order, so per class normwise <= 1e-6 and every
+    element within 1e-6 of the class's largest magnitude."""
     import splattar_amd as sa
     _, _, ref, full = _grad_case(gpu, 222, 15, 1, 48, aux=False)
     _, _, _, fast = _grad_case(gpu, 222, 15, 1, 48, aux=False, flags=sa._lib.GS_BWD_COLOR_ONLY)
     for k in ("pos", "scales", "quat", "opacity", "sh"):
-        np.testing.assert_allclose(fast[k], full[k], rtol=1e-5, atol=1e-7)
+        f, g = np.asarray(full[k], np.float64), np.asarray(fast[k], np.float64)
+        nrm = float(np.linalg.norm(g - f) / np.linalg.norm(f))
+        print(f"  {k}: colour-only vs full normwise {nrm:.2e}")
+        assert nrm <= 1e-6, k
+        assert np.abs(g - f).max() <= 1e-6 * np.abs(f).max(), k
 
 
 def test_backward_accumulates(gpu):
