@@ -54,8 +54,8 @@ HBM_PEAK_GBS = 8000.0
 VALU_PEAK_WINST = 256 * 2.4e9
 COUNTERS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "c3_step_counters.json")
 # The main kernel of each stage (its largest launch in the batched step).
-STAGE_KERNEL = {"raster_bwd": "raster_bwd_kernel", "raster_fwd": "raster_fwd_kernel",
-                "sort": "downsweep_kernel", "project": "project_kernel", "l1_ssim": "ssim_bwd_kernel",
+STAGE_KERNEL = {"raster_bwd": "raster_bwd_wave_kernel", "raster_fwd": "raster_fwd_kernel",
+                "sort": "downsweep_kernel", "project": "project_kernel", "l1_ssim": "l1_ssim_fused_kernel",
                 "adam": "adam_kernel", "geom_bwd": "geom_adam_kernel", "emit_keys": "emit_tile_keys_kernel",
                 "scan": "scan_down_kernel", "tile_ranges": "tile_ranges_u32_kernel",
                 "grad_clear": "fill_u32_kernel"}

@@ -693,12 +693,21 @@ hipError_t launch_geom_bwd(hipStream_t st, const CamParams* cams_dev, uint32_t n
 // loss.hip
 // With `view` set, the target is read at target + view->target_base (the
 // per-view descriptor lives in device memory, so a captured graph replays
-// with whatever views the descriptor table holds).
+// with whatever views the descriptor table holds). partial_sums receives
+// l1_ssim_partials(w, h) pairs (sum |x - y|, sum SSIM); dl_out may be null
+// (loss only).
 hipError_t launch_l1_ssim(hipStream_t st, uint32_t w, uint32_t h, float lambda, float dssim_scale,
-                          const float4* img, const float4* target, float4* dl_out,
-                          float* coef_scratch, double* partial_sums, uint32_t n_partials,
+                          const float4* img, const float4* target, float4* dl_out, double* partial_sums,
                           const CamParams* view = nullptr);
 uint32_t l1_ssim_partials(uint32_t w, uint32_t h);
+// The losses of view slots [first, first + k) of a step in one launch: view
+// j's image and dL/dimage start at img / dl_out + pix_base[j], its target at
+// target + cams_dev[first + j].target_base, its partial sums at
+// partials + 2 (first + j) partials_stride. At most 64 views.
+hipError_t launch_l1_ssim_views(hipStream_t st, uint32_t first, uint32_t k, const uint32_t* widths,
+                                const uint32_t* heights, const uint32_t* pix_base, float lambda,
+                                float dssim_scale, const float4* img, const float4* target, float4* dl_out,
+                                double* partials, uint32_t partials_stride, const CamParams* cams_dev);
 
 // mcmc.hip: exploration noise on raw positions (mcmc.wgsl) and the relocation
 // of dead surfels (relocate.wgsl). With state set, the noise takes its seed

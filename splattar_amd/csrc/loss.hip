@@ -1,33 +1,24 @@
-// Fused L1 + D-SSIM loss and its image gradient, two LDS-tiled kernels.
+// Fused L1 + D-SSIM loss and its image gradient in one kernel.
 //
 // Restates first_attempt/crates/gs-train/src/shaders/ssim.wgsl:41-143 and
 // ssim.rs:1-4,84: loss = (1-l) mean|x-y| + l * s * mean(1 - SSIM), 11-tap
 // sigma 1.5 separable Gaussian with zero padding (the reference's f32 taps),
 // C1 = 1e-4, C2 = 9e-4, means over pixels x 3 channels; s = dssim_scale is
-// 0.5 in the reference ((1 - SSIM)/2) and selectable here.
-// The reference runs 19 dispatches over 16 P-sized intermediates; here one
-// workgroup per (16x16 tile, channel) -- the channels are independent, and a
-// channel's working set (x, y staged with a 5 px halo; five blurred rows)
-// fits in ~14 KB of LDS, so many workgroups share a CU:
-//   ssim_fwd: stage x, y; horizontal pass in column quads (one 14-tap window
-//             shared by four outputs, products formed once per input);
-//             vertical pass; SSIM and the three coefficient maps
-//             C_mu = dw dS/dmu_x, C_x2 = dw dS/dE[x^2], C_xy = dw dS/dE[xy];
-//             per-(channel, tile) L1 / SSIM partial sums (deterministic);
-//   ssim_bwd: stage the channel's coefficient maps, blur them (the
-//             zero-padded symmetric blur is self-adjoint) and write
-//             dL/dx = (1-l) sign(x-y)/3P + B(C_mu) + 2x B(C_x2) + y B(C_xy).
-
+// 0.5 in the reference ((1 - SSIM)/2) and selectable here. The gradient
+// blurs the per-pixel coefficient maps C_mu = dw dS/dmu_x, C_x2 = dw
+// dS/dE[x^2], C_xy = dw dS/dE[xy] (the zero-padded symmetric blur is
+// self-adjoint). The reference runs 19 dispatches over 16 P-sized
+// intermediates; here nothing but the image, the target and dL/dimage
+// touches HBM.
 #include "gs_internal.h"
+
+#include <type_traits>
 
 namespace gs {
 
 namespace {
 
-constexpr int kT = 16;          // output tile
-constexpr int kH = 5;           // halo (11-tap window)
-constexpr int kS = kT + 2 * kH; // staged side = 26
-constexpr int kQuads = kS * (kT / 4);  // horizontal tasks: staged rows x output column quads = 104
+constexpr int kH = 5;           // blur half-width (11-tap window)
 __constant__ float c_w[6] = {0.26601818f, 0.21300237f, 0.10936069f,
                              0.03599398f, 0.00759866f, 0.00102838f};
 
@@ -35,223 +26,278 @@ constexpr float kC1 = 0.0001f, kC2 = 0.0009f;
 
 __device__ __forceinline__ float tap(int d) { return c_w[d < 0 ? -d : d]; }
 
-// Coefficient maps between the two kernels: per channel ch and pixel p one
-// float4 (C_mu, C_x2, C_xy, 0) at coef[ch * P + p]. Partial sums: per
-// channel ch and tile t the pair (sum |x - y|, sum SSIM) at partials[2 (ch T + t)].
+// One launch covers the views of a step. A workgroup
+// owns a strip of kSW output columns and SH output rows of one view (all
+// three channels) and walks its input rows top to bottom, one row per step:
+//   load    the input row (kSW + 20 pixels: the blur of the coefficient maps
+//           needs them 5 px beyond the strip, and those need 5 px more) into
+//           LDS as (x, y, x^2, y^2) and xy per channel;
+//   stats   thread (channel, stat column) of kAC = kSW + 10 columns blurs the
+//           five horizontally and keeps the last 11 rows in a register ring;
+//           once 11 are there, the vertical blur gives the window statistics
+//           of the stat row 5 above, then SSIM and the coefficients C_mu,
+//           C_x2, C_xy (0 outside the image: zero padding), which go to an
+//           LDS row;
+//   output  thread (channel, output column) blurs that row horizontally into
+//           a second 11-row ring and, 5 rows later, vertically, giving
+//           dL/dx = (1-l) sign(x-y)/3P + B(C_mu) + 2x B(C_x2) + y B(C_xy)
+//           at the output row 10 above the input row.
+// The coefficient maps never leave the chip; a strip re-blurs a 10-px
+// horizontal halo and its band a 20-row vertical one. Loss partial sums
+// (sum |x - y|, sum SSIM over the workgroup's own outputs) accumulate in
+// double per thread.
+constexpr int kSW = 74;                 // output columns per strip
+constexpr int kAC = kSW + 2 * kH;       // stat columns (84): 3 kAC = 252 threads
+constexpr int kIC = kSW + 4 * kH;       // input columns (94)
+constexpr int kMaxLossViews = 64;
 
-__global__ __launch_bounds__(256) void ssim_fwd_kernel(uint32_t W, uint32_t H, float dw,
-                                                       const float4* __restrict__ img,
-                                                       const float4* __restrict__ tgt,
-                                                       float4* __restrict__ coef,
-                                                       double* __restrict__ partials,
-                                                       const CamParams* __restrict__ view) {
-    if (view) tgt += view->target_base;
-    __shared__ float2 sxy[kS][kS + 2];   // (x, y) of this channel
-    __shared__ float4 hb4[kS][kT];       // horizontally blurred x, y, x^2, y^2
-    __shared__ float hb1[kS][kT];        // horizontally blurred xy
+struct LossViews {
+    uint32_t n;                          // views in this launch
+    uint32_t first;                      // view slot of the first (camera table, partials)
+    uint32_t partials_stride;            // partial-sum pairs per view slot
+    uint32_t block_begin[kMaxLossViews + 1];
+    uint32_t w[kMaxLossViews], h[kMaxLossViews], pix_base[kMaxLossViews];
+    float dw[kMaxLossViews], l1w[kMaxLossViews];
+};
+
+template <int SH>
+__global__ __launch_bounds__(256) void l1_ssim_fused_kernel(LossViews lv, const float4* __restrict__ img_base,
+                                                            const float4* __restrict__ tgt_base,
+                                                            float4* __restrict__ dl_base,
+                                                            double* __restrict__ partials,
+                                                            const CamParams* __restrict__ cams) {
+    __shared__ float4 s_p[3][kIC];    // (x, y, x^2, y^2) of the input row
+    __shared__ float s_q[3][kIC];     // xy
+    __shared__ float s_c[3][3][kAC];  // C_mu, C_x2, C_xy of the stat row
     __shared__ double s_red[2][4];
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-    // Logical block = 3 tile + channel, XCD-contiguous: a tile's three
-    // channel workgroups and its neighbours (shared halos) share an L2, so
-    // the float4 pixels each channel reads one component of come from L2.
-    const uint32_t tiles_x = (W + kT - 1) / kT;
-    const uint32_t tiles = tiles_x * ((H + kT - 1) / kT);
-    const uint32_t lg = xcd_swizzle(blockIdx.x, gridDim.x);
-    const uint32_t lb = lg / 3u;
-    const int ch = (int)(lg - 3u * lb);
-    const int bx = (int)(lb % tiles_x), by = (int)(lb / tiles_x);
-    const int x0 = bx * kT - kH, y0 = by * kT - kH;
-    const float* imgf = reinterpret_cast<const float*>(img) + ch;
-    const float* tgtf = reinterpret_cast<const float*>(tgt) + ch;
-    for (int r = ty; r < kS; r += 16) {
-        for (int c = tx; c < kS; c += 16) {
-            const int gx = x0 + c, gy = y0 + r;
-            float2 v = make_float2(0.f, 0.f);
-            if (gx >= 0 && gy >= 0 && gx < (int)W && gy < (int)H) {
-                const size_t q = 4 * ((size_t)gy * W + gx);
-                v = make_float2(imgf[q], tgtf[q]);
+    const int tid = threadIdx.x;
+    const uint32_t b = xcd_swizzle(blockIdx.x, gridDim.x);
+    uint32_t v = 0;
+    for (uint32_t k = 1; k < lv.n; ++k)
+        if (lv.block_begin[k] <= b) v = k;
+    const uint32_t blk = b - lv.block_begin[v];
+    const uint32_t W = lv.w[v], H = lv.h[v];
+    const float dw = lv.dw[v], l1w = lv.l1w[v];
+    const float4* img = img_base + lv.pix_base[v];
+    float4* dl_out = dl_base != nullptr ? dl_base + lv.pix_base[v] : nullptr;
+    const float4* tgt = tgt_base + (cams != nullptr ? cams[lv.first + v].target_base : 0u);
+    const uint32_t strips = (W + kSW - 1) / kSW;
+    const int x0 = (int)(blk % strips) * kSW, y0 = (int)(blk / strips) * SH;
+    const int iW = (int)W, iH = (int)H;
+    // Stats role: (channel, stat column x0 - 5 + cA); output role: (channel,
+    // output column x0 + cB).
+    const bool roleA = tid < 3 * kAC, roleB = tid < 3 * kSW;
+    const int chA = roleA ? tid / kAC : 0, cA = roleA ? tid - chA * kAC : 0;
+    const int chB = roleB ? tid / kSW : 0, cB = roleB ? tid - chB * kSW : 0;
+    const int gxA = x0 - kH + cA, gxB = x0 + cB;
+    const bool colA_in = roleA && gxA >= 0 && gxA < iW;
+    const bool colA_own = colA_in && cA >= kH && cA < kH + kSW;
+    const bool colB_in = roleB && gxB < iW;
+    const float* imgf = reinterpret_cast<const float*>(img);
+    const float* tgtf = reinterpret_cast<const float*>(tgt);
+    float ra[11][5], rb[11][3];
+    double l1 = 0.0, ss = 0.0;
+    constexpr int nrows = SH + 4 * kH;  // input rows y0 - 10 .. y0 + SH + 9
+    // Step i: input row y0 - 10 + i, stat row y0 - 15 + i, output row
+    // y0 - 20 + i; the ring slot of step i is i % 11 (the template K).
+    auto step = [&](auto kc, int i) {
+        constexpr int K = decltype(kc)::value;
+        const int gy = y0 - 2 * kH + i;
+        if (tid < kIC) {
+            const int gx = x0 - 2 * kH + tid;
+            float4 a = make_float4(0.f, 0.f, 0.f, 0.f), t = a;
+            if (gx >= 0 && gx < iW && gy >= 0 && gy < iH) {
+                const size_t q = (size_t)gy * W + gx;
+                a = img[q];
+                t = tgt[q];
             }
-            sxy[r][c] = v;
+            s_p[0][tid] = make_float4(a.x, t.x, a.x * a.x, t.x * t.x);
+            s_p[1][tid] = make_float4(a.y, t.y, a.y * a.y, t.y * t.y);
+            s_p[2][tid] = make_float4(a.z, t.z, a.z * a.z, t.z * t.z);
+            s_q[0][tid] = a.x * t.x;
+            s_q[1][tid] = a.y * t.y;
+            s_q[2][tid] = a.z * t.z;
         }
-    }
-    __syncthreads();
-    if (tid < kQuads) {
-        // Four outputs per task: a 14-tap window shared by four columns, the
-        // products formed once per input; each output sums its 11 taps in
-        // window order.
-        const int r = tid >> 2, c0 = (tid & 3) * 4;
-        float a[4][5];
+        __syncthreads();
+        if (roleA) {
+            float h0 = 0.f, h1 = 0.f, h2 = 0.f, h3 = 0.f, h4 = 0.f;
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+            for (int j = 0; j <= 2 * kH; ++j) {
+                const float4 p = s_p[chA][cA + j];
+                const float w = tap(j - kH);
+                h0 += w * p.x;
+                h1 += w * p.y;
+                h2 += w * p.z;
+                h3 += w * p.w;
+                h4 += w * s_q[chA][cA + j];
+            }
+            ra[K][0] = h0;
+            ra[K][1] = h1;
+            ra[K][2] = h2;
+            ra[K][3] = h3;
+            ra[K][4] = h4;
+        }
+        if (i >= 2 * kH) {
+            const int sy = gy - kH;
+            if (roleA) {
+                float mu_x = 0.f, mu_y = 0.f, bx2 = 0.f, by2 = 0.f, bxy = 0.f;
 #pragma unroll
-            for (int k = 0; k < 5; ++k) a[q][k] = 0.f;
+                for (int d = -kH; d <= kH; ++d) {
+                    const int s = (K - kH + d + 22) % 11;
+                    const float w = tap(d);
+                    mu_x += w * ra[s][0];
+                    mu_y += w * ra[s][1];
+                    bx2 += w * ra[s][2];
+                    by2 += w * ra[s][3];
+                    bxy += w * ra[s][4];
+                }
+                float cm = 0.f, cx2 = 0.f, cxy = 0.f;
+                if (colA_in && sy >= 0 && sy < iH) {
+                    // ssim.wgsl:95-126 (partials).
+                    const float sx2 = bx2 - mu_x * mu_x, sy2 = by2 - mu_y * mu_y, sxy_ = bxy - mu_x * mu_y;
+                    const float p = mu_x * mu_x + mu_y * mu_y + kC1;
+                    const float q = sx2 + sy2 + kC2;
+                    const float rp = rcp_refined(p), rq = rcp_refined(q);
+                    const float l = (2.0f * mu_x * mu_y + kC1) * rp;
+                    const float cs = (2.0f * sxy_ + kC2) * rq;
+                    const float ds_dmux =
+                        cs * (2.0f * mu_y - 2.0f * mu_x * l) * rp + l * (-2.0f * mu_y + 2.0f * mu_x * cs) * rq;
+                    cm = ds_dmux * dw;
+                    cx2 = (-l * cs * rq) * dw;
+                    cxy = (2.0f * l * rq) * dw;
+                    if (colA_own && sy >= y0 && sy < y0 + SH) ss += (double)(l * cs);
+                }
+                s_c[chA][0][cA] = cm;
+                s_c[chA][1][cA] = cx2;
+                s_c[chA][2][cA] = cxy;
+            }
+            __syncthreads();
+            if (roleB) {
+                float b0 = 0.f, b1 = 0.f, b2 = 0.f;
 #pragma unroll
-        for (int j = 0; j < 2 * kH + 4; ++j) {
-            const float2 v = sxy[r][c0 + j];
-            const float m[5] = {v.x, v.y, v.x * v.x, v.y * v.y, v.x * v.y};
+                for (int j = 0; j <= 2 * kH; ++j) {
+                    const float w = tap(j - kH);
+                    b0 += w * s_c[chB][0][cB + j];
+                    b1 += w * s_c[chB][1][cB + j];
+                    b2 += w * s_c[chB][2][cB + j];
+                }
+                rb[K][0] = b0;
+                rb[K][1] = b1;
+                rb[K][2] = b2;
+            }
+            const int oy = gy - 2 * kH;
+            if (i >= 4 * kH && roleB && colB_in && oy < iH) {
+                float b0 = 0.f, b1 = 0.f, b2 = 0.f;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (j >= q && j - q <= 2 * kH) {
-                    const float w = tap(j - q - kH);
-#pragma unroll
-                    for (int k = 0; k < 5; ++k) a[q][k] += w * m[k];
+                for (int d = -kH; d <= kH; ++d) {
+                    const int s = (K - kH + d + 22) % 11;
+                    const float w = tap(d);
+                    b0 += w * rb[s][0];
+                    b1 += w * rb[s][1];
+                    b2 += w * rb[s][2];
+                }
+                const size_t pix = (size_t)oy * W + gxB;
+                const float x = imgf[4 * pix + chB], y = tgtf[4 * pix + chB];
+                l1 += (double)fabsf(x - y);
+                if (dl_out != nullptr) {
+                    const float sg = x - y > 0.f ? 1.f : (x - y < 0.f ? -1.f : 0.f);
+                    float* o = reinterpret_cast<float*>(dl_out) + 4 * pix;
+                    o[chB] = sg * l1w + b0 + 2.0f * x * b1 + y * b2;
+                    if (chB == 0) o[3] = 0.0f;
                 }
             }
+        } else {
+            __syncthreads();  // the stats pass has read the input row before the next load
         }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            hb4[r][c0 + q] = make_float4(a[q][0], a[q][1], a[q][2], a[q][3]);
-            hb1[r][c0 + q] = a[q][4];
-        }
+    };
+    for (int i0 = 0; i0 < nrows; i0 += 11) {
+        using std::integral_constant;
+        step(integral_constant<int, 0>{}, i0);
+        if (i0 + 1 < nrows) step(integral_constant<int, 1>{}, i0 + 1);
+        if (i0 + 2 < nrows) step(integral_constant<int, 2>{}, i0 + 2);
+        if (i0 + 3 < nrows) step(integral_constant<int, 3>{}, i0 + 3);
+        if (i0 + 4 < nrows) step(integral_constant<int, 4>{}, i0 + 4);
+        if (i0 + 5 < nrows) step(integral_constant<int, 5>{}, i0 + 5);
+        if (i0 + 6 < nrows) step(integral_constant<int, 6>{}, i0 + 6);
+        if (i0 + 7 < nrows) step(integral_constant<int, 7>{}, i0 + 7);
+        if (i0 + 8 < nrows) step(integral_constant<int, 8>{}, i0 + 8);
+        if (i0 + 9 < nrows) step(integral_constant<int, 9>{}, i0 + 9);
+        if (i0 + 10 < nrows) step(integral_constant<int, 10>{}, i0 + 10);
     }
-    __syncthreads();
-    float mu_x = 0.f, mu_y = 0.f, bx2 = 0.f, by2 = 0.f, bxy = 0.f;
-#pragma unroll
-    for (int d = -kH; d <= kH; ++d) {
-        const float w = tap(d);
-        const float4 h = hb4[ty + kH + d][tx];
-        mu_x += w * h.x;
-        mu_y += w * h.y;
-        bx2 += w * h.z;
-        by2 += w * h.w;
-        bxy += w * hb1[ty + kH + d][tx];
-    }
-    const int gx = bx * kT + tx, gy = by * kT + ty;
-    const bool inside = gx < (int)W && gy < (int)H;
-    const size_t P = (size_t)W * H;
-    float l1 = 0.0f, ss = 0.0f;
-    // ssim.wgsl:95-126 (partials).
-    const float sx2 = bx2 - mu_x * mu_x, sy2 = by2 - mu_y * mu_y, sxy_ = bxy - mu_x * mu_y;
-    const float p = mu_x * mu_x + mu_y * mu_y + kC1;
-    const float q = sx2 + sy2 + kC2;
-    // p, q >= C1, C2 > 0: two refined reciprocals (<= 1 ulp) instead of five
-    // divisions.
-    const float rp = rcp_refined(p), rq = rcp_refined(q);
-    const float l = (2.0f * mu_x * mu_y + kC1) * rp;
-    const float cs = (2.0f * sxy_ + kC2) * rq;
-    const float s = l * cs;
-    const float ds_dmux = cs * (2.0f * mu_y - 2.0f * mu_x * l) * rp + l * (-2.0f * mu_y + 2.0f * mu_x * cs) * rq;
-    if (inside) {
-        coef[ch * P + (size_t)gy * W + gx] = make_float4(ds_dmux * dw, (-l * cs * rq) * dw, (2.0f * l * rq) * dw, 0.0f);
-        ss = s;
-        const float2 c = sxy[ty + kH][tx + kH];
-        l1 = fabsf(c.x - c.y);
-    }
-    // Deterministic block reduction of the loss sums: f32 over a wave's 64
-    // pixels (values in [0, 1] and [-1, 1]: relative error <~ 4e-7 per wave
-    // sum), then f64 across waves and tiles.
+    // Deterministic block reduction of the loss sums (double).
     const int lane = tid & 63, wave = tid >> 6;
     for (int o = 32; o >= 1; o >>= 1) {
         l1 += __shfl_xor(l1, o);
         ss += __shfl_xor(ss, o);
     }
     if (lane == 0) {
-        s_red[0][wave] = (double)l1;
-        s_red[1][wave] = (double)ss;
+        s_red[0][wave] = l1;
+        s_red[1][wave] = ss;
     }
     __syncthreads();
     if (tid == 0) {
-        const size_t b = (size_t)ch * tiles + lb;
-        partials[2 * b] = ((s_red[0][0] + s_red[0][1]) + s_red[0][2]) + s_red[0][3];
-        partials[2 * b + 1] = ((s_red[1][0] + s_red[1][1]) + s_red[1][2]) + s_red[1][3];
+        double* pp = partials + ((size_t)(lv.first + v) * lv.partials_stride + blk) * 2;
+        pp[0] = ((s_red[0][0] + s_red[0][1]) + s_red[0][2]) + s_red[0][3];
+        pp[1] = ((s_red[1][0] + s_red[1][1]) + s_red[1][2]) + s_red[1][3];
     }
 }
 
-// One channel per workgroup: stage its three coefficient maps (one float4
-// per pixel), blur them (column quads, then rows) and write that channel of
-// dL/dx = (1-l) sign(x-y)/3P + B(C_mu) + 2x B(C_x2) + y B(C_xy); the
-// channel-0 workgroup also writes w = 0.
-__global__ __launch_bounds__(256) void ssim_bwd_kernel(uint32_t W, uint32_t H, float l1w,
-                                                       const float4* __restrict__ img,
-                                                       const float4* __restrict__ tgt,
-                                                       const float4* __restrict__ coef,
-                                                       float4* __restrict__ dl_out,
-                                                       const CamParams* __restrict__ view) {
-    if (view) tgt += view->target_base;
-    __shared__ float4 sc[kS][kS + 1];
-    __shared__ float4 hb[kS][kT];
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-    const uint32_t tiles_x = (W + kT - 1) / kT;
-    const uint32_t lg = xcd_swizzle(blockIdx.x, gridDim.x);  // 3 tile + channel, as ssim_fwd
-    const uint32_t lb = lg / 3u;
-    const int ch = (int)(lg - 3u * lb);
-    const int bx = (int)(lb % tiles_x), by = (int)(lb / tiles_x);
-    const int x0 = bx * kT - kH, y0 = by * kT - kH;
-    const size_t P = (size_t)W * H;
-    const float4* cch = coef + ch * P;
-    for (int r = ty; r < kS; r += 16) {
-        for (int c = tx; c < kS; c += 16) {
-            const int gx = x0 + c, gy = y0 + r;
-            float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (gx >= 0 && gy >= 0 && gx < (int)W && gy < (int)H) a = cch[(size_t)gy * W + gx];
-            sc[r][c] = a;
-        }
-    }
-    __syncthreads();
-    if (tid < kQuads) {
-        const int r = tid >> 2, c0 = (tid & 3) * 4;  // four columns per task, as ssim_fwd
-        float4 a[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) a[q] = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-        for (int j = 0; j < 2 * kH + 4; ++j) {
-            const float4 v = sc[r][c0 + j];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (j >= q && j - q <= 2 * kH) {
-                    const float w = tap(j - q - kH);
-                    a[q].x += w * v.x;
-                    a[q].y += w * v.y;
-                    a[q].z += w * v.z;
-                }
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) hb[r][c0 + q] = a[q];
-    }
-    __syncthreads();
-    float b0 = 0.f, b1 = 0.f, b2 = 0.f;
-#pragma unroll
-    for (int d = -kH; d <= kH; ++d) {
-        const float w = tap(d);
-        const float4 h = hb[ty + kH + d][tx];
-        b0 += w * h.x;
-        b1 += w * h.y;
-        b2 += w * h.z;
-    }
-    const int gx = bx * kT + tx, gy = by * kT + ty;
-    if (gx >= (int)W || gy >= (int)H) return;
-    const size_t pix = (size_t)gy * W + gx;
-    const float x = reinterpret_cast<const float*>(img)[4 * pix + ch];
-    const float y = reinterpret_cast<const float*>(tgt)[4 * pix + ch];
-    const float sg = x - y > 0.f ? 1.f : (x - y < 0.f ? -1.f : 0.f);
-    float* o = reinterpret_cast<float*>(dl_out) + 4 * pix;
-    o[ch] = sg * l1w + b0 + 2.0f * x * b1 + y * b2;
-    if (ch == 0) o[3] = 0.0f;
+constexpr uint32_t kSH = 64;  // output rows per band
+
+hipError_t launch_fused(hipStream_t st, const LossViews& lv, const float4* img, const float4* target,
+                        float4* dl_out, double* partials, const CamParams* cams) {
+    const dim3 g(lv.block_begin[lv.n]);
+    if (g.x == 0) return hipSuccess;
+    hipLaunchKernelGGL(l1_ssim_fused_kernel<kSH>, g, dim3(256), 0, st, lv, img, target, dl_out, partials, cams);
+    return hipGetLastError();
+}
+
+// Uniform loss weights of a w x h view: dw = -l s / 3P (D-SSIM term), the L1
+// term's (1 - l) / 3P.
+void loss_weights(uint32_t w, uint32_t h, float lambda, float dssim_scale, float* dw, float* l1w) {
+    const float n3 = (float)w * (float)h * 3.0f;
+    *dw = -lambda * dssim_scale * (1.0f / n3);
+    *l1w = (1.0f - lambda) / n3;
 }
 
 }  // namespace
 
-uint32_t l1_ssim_partials(uint32_t w, uint32_t h) { return 3u * div_up(w, kT) * div_up(h, kT); }
+uint32_t l1_ssim_partials(uint32_t w, uint32_t h) { return div_up(w, kSW) * div_up(h, kSH); }
 
 hipError_t launch_l1_ssim(hipStream_t st, uint32_t w, uint32_t h, float lambda, float dssim_scale,
-                          const float4* img, const float4* target, float4* dl_out,
-                          float* coef_scratch, double* partial_sums, uint32_t /*n_partials*/,
+                          const float4* img, const float4* target, float4* dl_out, double* partial_sums,
                           const CamParams* view) {
     if (w == 0 || h == 0) return hipSuccess;
-    dim3 grid(div_up(w, kT) * div_up(h, kT) * 3);
-    float4* coef = reinterpret_cast<float4*>(coef_scratch);
-    // Uniform loss weights, computed once: dw = -l s / 3P (D-SSIM term), the
-    // L1 term's (1 - l) / 3P.
-    const float n3 = (float)w * (float)h * 3.0f;
-    const float dw = -lambda * dssim_scale * (1.0f / n3);
-    const float l1w = (1.0f - lambda) / n3;
-    hipLaunchKernelGGL(ssim_fwd_kernel, grid, dim3(256), 0, st, w, h, dw, img, target, coef, partial_sums,
-                       view);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess || dl_out == nullptr) return e;
-    hipLaunchKernelGGL(ssim_bwd_kernel, grid, dim3(256), 0, st, w, h, l1w, img, target, coef, dl_out, view);
-    return hipGetLastError();
+    LossViews lv{};
+    lv.n = 1;
+    lv.w[0] = w;
+    lv.h[0] = h;
+    loss_weights(w, h, lambda, dssim_scale, &lv.dw[0], &lv.l1w[0]);
+    lv.block_begin[1] = l1_ssim_partials(w, h);
+    return launch_fused(st, lv, img, target, dl_out, partial_sums, view);
+}
+
+hipError_t launch_l1_ssim_views(hipStream_t st, uint32_t first, uint32_t k, const uint32_t* widths,
+                                const uint32_t* heights, const uint32_t* pix_base, float lambda,
+                                float dssim_scale, const float4* img, const float4* target, float4* dl_out,
+                                double* partials, uint32_t partials_stride, const CamParams* cams_dev) {
+    if (k == 0) return hipSuccess;
+    if (k > (uint32_t)kMaxLossViews) return hipErrorInvalidValue;
+    LossViews lv{};
+    lv.n = k;
+    lv.first = first;
+    lv.partials_stride = partials_stride;
+    uint32_t blocks = 0;
+    for (uint32_t j = 0; j < k; ++j) {
+        lv.w[j] = widths[j];
+        lv.h[j] = heights[j];
+        lv.pix_base[j] = pix_base[j];
+        loss_weights(widths[j], heights[j], lambda, dssim_scale, &lv.dw[j], &lv.l1w[j]);
+        lv.block_begin[j] = blocks;
+        blocks += l1_ssim_partials(widths[j], heights[j]);
+    }
+    lv.block_begin[k] = blocks;
+    return launch_fused(st, lv, img, target, dl_out, partials, cams_dev);
 }
 
 }  // namespace gs

@@ -1284,12 +1284,10 @@ struct gs_loss_s {
     const float4* img = nullptr;
     float4* dl_out = nullptr;
     float4* target = nullptr;
-    float* coef = nullptr;
     double* partials = nullptr;
     uint32_t n_partials = 0;
     ~gs_loss_s() {
         dfree(target);
-        dfree(coef);
         dfree(partials);
     }
 };
@@ -1311,7 +1309,6 @@ gs_status gs_loss_create(gs_ctx ctx, uint32_t width, uint32_t height, float lamb
     l->dl_out = reinterpret_cast<float4*>(dl_out);
     const size_t P = (size_t)width * height;
     GS_CHECK(dalloc(&l->target, P, "loss target"));
-    GS_CHECK(dalloc(&l->coef, P * 12, "loss coefficient maps"));
     l->n_partials = l1_ssim_partials(width, height);
     GS_CHECK(dalloc(&l->partials, (size_t)l->n_partials * 2, "loss partials"));
     *out = l.release();
@@ -1331,8 +1328,7 @@ gs_status gs_loss_set_dssim_scale(gs_loss l, float scale) {
 gs_status gs_loss_encode(gs_loss l, void* stream) {
     GS_REQUIRE(l != nullptr, "loss is null");
     GS_HIP(launch_l1_ssim(as_stream(stream, l->ctx->stream), l->width, l->height, l->lambda,
-                          l->dssim_scale, l->img, l->target, l->dl_out, l->coef, l->partials,
-                          l->n_partials));
+                          l->dssim_scale, l->img, l->target, l->dl_out, l->partials));
     return GS_OK;
 }
 
@@ -2150,7 +2146,6 @@ struct gs_trainer_s {
     float* grad_opacity = nullptr;
     float* grad_sh = nullptr;
     float4* dl_dcolor = nullptr;       // cap_pixels (batched layout)
-    float* loss_coef = nullptr;        // 12 floats per pixel of the largest view
     double* loss_partials = nullptr;   // per view slot
     uint32_t partials_per_view = 0;
     gs_optim optim = nullptr;
@@ -2214,7 +2209,6 @@ struct gs_trainer_s {
         dfree(grad_opacity);
         dfree(grad_sh);
         dfree(dl_dcolor);
-        dfree(loss_coef);
         dfree(loss_partials);
         dfree(reloc_pairs);
     }
@@ -2258,9 +2252,28 @@ struct gs_trainer_s {
         const TrainViewDev& tv = views[ids[j]];
         const CamParams& c = core.cams[j];
         GS_HIP(launch_l1_ssim(s, tv.width, tv.height, cfg.lambda, cfg.dssim_scale, core.out_color + c.pix_base,
-                              targets, dl_dcolor + c.pix_base, loss_coef,
-                              loss_partials + (size_t)j * partials_per_view * 2, partials_per_view,
-                              core.cams_dev + j));
+                              targets, dl_dcolor + c.pix_base,
+                              loss_partials + (size_t)j * partials_per_view * 2, core.cams_dev + j));
+        return GS_OK;
+    }
+
+    // L1 + D-SSIM of view slots [j0, j0 + k) of a batched step, in one
+    // launch where the loss kernel supports it.
+    gs_status loss_views(hipStream_t s, const uint32_t* ids, uint32_t j0, uint32_t k) {
+        if (k == 1) {
+            for (uint32_t j = j0; j < j0 + k; ++j) GS_CHECK(loss_view(s, ids, j));
+            return GS_OK;
+        }
+        std::vector<uint32_t> ws(k), hs(k), pb(k);
+        for (uint32_t j = 0; j < k; ++j) {
+            const TrainViewDev& tv = views[ids[j0 + j]];
+            ws[j] = tv.width;
+            hs[j] = tv.height;
+            pb[j] = core.cams[j0 + j].pix_base;
+        }
+        GS_HIP(launch_l1_ssim_views(s, j0, k, ws.data(), hs.data(), pb.data(), cfg.lambda, cfg.dssim_scale,
+                                    core.out_color, targets, dl_dcolor, loss_partials, partials_per_view,
+                                    core.cams_dev));
         return GS_OK;
     }
 
@@ -2321,7 +2334,7 @@ gs_status gs_trainer_s::enqueue_step(hipStream_t st, const uint32_t* ids, uint32
     if (batched) {
         GS_CHECK(core.forward(st, 0, k, n, positions, scales, quats, opacities, sh, P));
         if (P) GS_CHECK(P->mark(st, GS_STAGE_LOSS));
-        for (uint32_t j = 0; j < k; ++j) GS_CHECK(loss_view(st, ids, j));
+        GS_CHECK(loss_views(st, ids, 0, k));
         GS_CHECK(core.backward(st, 0, k, n, positions, scales, quats, sh, dl_dcolor, nullptr, g, false,
                                false, P, false));
         if (P) GS_CHECK(P->mark(st, GS_STAGE_GEOM_BWD));
@@ -2332,8 +2345,7 @@ gs_status gs_trainer_s::enqueue_step(hipStream_t st, const uint32_t* ids, uint32
             GS_CHECK(core.forward(st, j, 1, n, positions, scales, quats, opacities, sh, P));
             if (P) GS_CHECK(P->mark(st, GS_STAGE_LOSS));
             GS_HIP(launch_l1_ssim(st, tv.width, tv.height, cfg.lambda, cfg.dssim_scale, core.out_color,
-                                  targets, dl_dcolor, loss_coef,
-                                  loss_partials + (size_t)j * partials_per_view * 2, partials_per_view,
+                                  targets, dl_dcolor, loss_partials + (size_t)j * partials_per_view * 2,
                                   core.cams_dev + j));
             GS_CHECK(core.backward(st, j, 1, n, positions, scales, quats, sh, dl_dcolor, nullptr, g,
                                    j > 0, false, P));
@@ -2371,10 +2383,10 @@ gs_status gs_trainer_s::enqueue_pipelined(hipStream_t st, const uint32_t* ids, u
     GS_CHECK(core.raster_fwd_range(st, 1, 0, k, ta, tt - ta));
     GS_HIP(hipEventRecord(ev[1], st));
     GS_HIP(hipStreamWaitEvent(side, ev[0], 0));
-    for (uint32_t j = 0; j < ka; ++j) GS_CHECK(loss_view(side, ids, j));
+    GS_CHECK(loss_views(side, ids, 0, ka));
     GS_HIP(hipEventRecord(ev[2], side));
     GS_HIP(hipStreamWaitEvent(side, ev[1], 0));
-    for (uint32_t j = ka; j < k; ++j) GS_CHECK(loss_view(side, ids, j));
+    GS_CHECK(loss_views(side, ids, ka, k - ka));
     GS_HIP(hipEventRecord(ev[3], side));
     GS_HIP(hipStreamWaitEvent(st, ev[2], 0));
     GS_CHECK(core.raster_bwd_range(st, 0, 0, k, 0, ta, dl_dcolor));
@@ -2454,7 +2466,6 @@ gs_status gs_trainer_create(gs_ctx ctx, const gs_train_config* cfg, const gs_sce
     GS_CHECK(dalloc(&t->grad_opacity, n, "grad_opacity"));
     GS_CHECK(dalloc(&t->grad_sh, (size_t)n * kShFloats, "grad_sh"));
     GS_CHECK(dalloc(&t->dl_dcolor, (size_t)max_px * V, "dl_dcolor"));
-    GS_CHECK(dalloc(&t->loss_coef, (size_t)max_px * 12, "loss coefficient maps"));
     t->partials_per_view = l1_ssim_partials(t->max_w, t->max_h);
     GS_CHECK(dalloc(&t->loss_partials, (size_t)t->partials_per_view * 2 * V, "loss partials"));
     GS_CHECK(upload_scene_into(ctx, init, n, t->positions, t->scales, t->quats, t->opacities, t->sh));

@@ -89,6 +89,49 @@ def copy_scale(path, counter):
     return COPY_BYTES / mean
 
 
+VALU_PEAK_WINST = 256 * 2.4e9  # wave instructions / s: 256 CUs, one per SIMD every 4 cycles
+
+
+def markdown_rows(kernels, top=28):
+    """The per-kernel table of a summary: mean duration, HBM bytes, VALU
+    issue rate against the non-packed peak, the fraction of wave cycles spent
+    waiting, LDS instructions per wave and LDS bank-conflict cycles against
+    LDS-active cycles (SQ_LDS_BANK_CONFLICT / SQ_ACTIVE_INST_LDS). A
+    persistent kernel whose launches span several views also gets a row over
+    its large launches (>= 0.6 x its longest)."""
+    head = ("| kernel | grid | launches | mean ms | HBM read MB | HBM write MB | VALU G wave-instr/s "
+            "(frac of 614) | wait frac | LDS instr / wave | bank-conflict / LDS-active cycles |\n"
+            "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|")
+    rows = [head]
+
+    def row(name, grid, launches, k):
+        cs = k.get("counters_per_dispatch", {})
+        ms = k["mean_ms"]
+
+        def f(v, fmt):
+            return fmt.format(v) if v is not None else "-"
+        rd = k.get("hbm_read_bytes")
+        wr = k.get("hbm_write_bytes")
+        valu = cs.get("SQ_INSTS_VALU")
+        rate = valu / (ms * 1e-3) if valu is not None else None
+        wait = (cs["SQ_WAIT_ANY"] / cs["SQ_WAVE_CYCLES"]) if cs.get("SQ_WAVE_CYCLES") and "SQ_WAIT_ANY" in cs \
+            else None
+        lds = (cs["SQ_INSTS_LDS"] / cs["SQ_WAVES"]) if cs.get("SQ_WAVES") and "SQ_INSTS_LDS" in cs else None
+        bank = (cs["SQ_LDS_BANK_CONFLICT"] / cs["SQ_ACTIVE_INST_LDS"]) \
+            if cs.get("SQ_ACTIVE_INST_LDS") and "SQ_LDS_BANK_CONFLICT" in cs else None
+        vstr = f"{rate / 1e9:.0f} ({rate / VALU_PEAK_WINST:.2f})" if rate is not None else "-"
+        rows.append(f"| {name} | {grid} | {launches} | {ms:.4f} | {f(rd and rd / 1e6, '{:.0f}')} | "
+                    f"{f(wr and wr / 1e6, '{:.0f}')} | {vstr} | {f(wait, '{:.2f}')} | {f(lds, '{:.0f}')} | "
+                    f"{f(bank, '{:.3f}')} |")
+
+    for k in kernels[:top]:
+        row(k["kernel"], k["grid"], k["launches"], k)
+        lg = k.get("large")
+        if lg and lg["launches"] < k["launches"] and k["kernel"].startswith("raster_"):
+            row(k["kernel"] + " (large launches)", k["grid"], lg["launches"], lg)
+    return "\n".join(rows) + "\n"
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--trace", required=True)
@@ -97,6 +140,7 @@ def main(argv=None):
     ap.add_argument("--copy-write")
     ap.add_argument("--command", default="")
     ap.add_argument("--out", required=True)
+    ap.add_argument("--markdown", help="also write the per-kernel table (markdown) here")
     a = ap.parse_args(argv)
     trace = read_trace(a.trace)
     counters = collections.defaultdict(dict)
@@ -137,6 +181,9 @@ def main(argv=None):
            "write_bytes_per_unit": write_scale, "kernels": kernels}
     with open(a.out, "w") as f:
         json.dump(out, f, indent=1)
+    if a.markdown:
+        with open(a.markdown, "w") as f:
+            f.write(markdown_rows(kernels))
 
 
 if __name__ == "__main__":
