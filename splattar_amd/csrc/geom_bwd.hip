@@ -125,6 +125,10 @@ __device__ __forceinline__ bool load_coef_grads(const float* __restrict__ grad_c
 // Adds one view's contribution of one surfel (activated position p, scales s,
 // quaternion q, SH row shp) to a; with want_cam also fills the view's 16
 // camera gradient slots cg.
+// VEC: the SH row is in LDS (row stride kShRow float4) and is read as
+// float4; otherwise it is read from global memory one coefficient at a time
+// (fewer registers in flight).
+template <bool VEC>
 __device__ __forceinline__ void add_view(const CamParams& cam, const float gc[kGradStride], float4 p, float2 s,
                                          float4 q, const float* __restrict__ shp, SurfelAcc& a, bool want_cam,
                                          float cg[16]) {
@@ -202,7 +206,25 @@ __device__ __forceinline__ void add_view(const CamParams& cam, const float gc[kG
                 a.sh[3 * k + 0] += dcol.x * basis[k];
                 a.sh[3 * k + 1] += dcol.y * basis[k];
                 a.sh[3 * k + 2] += dcol.z * basis[k];
-                sdot[k] = shp[3 * k] * dcol.x + shp[3 * k + 1] * dcol.y + shp[3 * k + 2] * dcol.z;
+                if (!VEC) sdot[k] = shp[3 * k] * dcol.x + shp[3 * k + 1] * dcol.y + shp[3 * k + 2] * dcol.z;
+            }
+        }
+        if (VEC) {
+            // sdot[k] = SH_k . dcol from the LDS row read as float4, one vector
+            // at a time: at a row stride of 13 float4 these reads are free of
+            // bank conflicts, where scalar reads would be 4-way.
+            const float4* shp4 = reinterpret_cast<const float4*>(shp);
+            const float dc[3] = {dcol.x, dcol.y, dcol.z};
+#pragma unroll
+            for (int c = 0; c < kShFloats / 4; ++c) {
+                if (4 * c >= 3 * nb) break;
+                const float4 v = shp4[c];
+                const float f[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int m = 4 * c + j;
+                    if (m < 3 * nb) sdot[m / 3] += f[j] * dc[m % 3];
+                }
             }
         }
         const float3 dd = sh_dir_grad((int)cam.sh_degree, dir.x, dir.y, dir.z, sdot);
@@ -264,7 +286,7 @@ __global__ __launch_bounds__(kGeomThreads) void geom_bwd_kernel(
         for (int k = 0; k < 16; ++k) cg[k] = 0.f;
         float gc[kGradStride];
         const bool any = valid && load_coef_grads<GS>(grad_coef, v, n, i, gc);
-        if (any) add_view(cams[v], gc, p, s, q, sh + (size_t)i * kShFloats, a, want_cam, cg);
+        if (any) add_view<false>(cams[v], gc, p, s, q, sh + (size_t)i * kShFloats, a, want_cam, cg);
         if (want_cam) {
             // Block reduction of the 16 camera slots (every thread takes part).
 #pragma unroll
@@ -378,7 +400,7 @@ __global__ __launch_bounds__(kFuseThreads, 3) void geom_adam_kernel(
     for (uint32_t v = 0; v < n_views; ++v) {
         float gc[kGradStride];
         const bool any = valid && load_coef_grads<grad_stride(false)>(grad_coef, v, n, i, gc);
-        if (any) add_view(cams[v], gc, p, s, q, reinterpret_cast<const float*>(row), a, false, nullptr);
+        if (any) add_view<true>(cams[v], gc, p, s, q, reinterpret_cast<const float*>(row), a, false, nullptr);
     }
     const float bc1 = state->bc1_inv, bc2 = state->bc2_inv;
     const float* lr = state->lr;

@@ -114,8 +114,10 @@ __global__ __launch_bounds__(256) void project_kernel(
     // Per-tile footprint masks, wave-cooperatively: the (item, tile) pairs of
     // the wave's masked items are dealt 64 at a time over the lanes (owner by
     // binary search over the wave's exclusive scan of tile counts), each lane
-    // tests one tile against its owner's footprint read from LDS and sets the
-    // owner's bit with an LDS atomic OR. Bit t = row-major tile t of the rect.
+    // tests one tile against its owner's footprint read from LDS; an owner's
+    // pairs of a round are consecutive lanes, so its bits are a shifted slice
+    // of the round's ballot, merged into its mask by the run's first lane.
+    // Bit t = row-major tile t of the rect.
     // The cost follows the wave's total tile count, not 64 x its widest item.
     // s_stage holds the footprints for the masks (s_fp), then the block's
     // records for one contiguous store (rows of kRecRow float4, one spare
@@ -124,16 +126,17 @@ __global__ __launch_bounds__(256) void project_kernel(
     __shared__ float4 s_stage[256 * kRecRow];
     __shared__ uint32_t s_bits[256][2];
     __shared__ uint32_t s_write[256];
-    float4 (*s_fp)[4] = reinterpret_cast<float4 (*)[4]>(s_stage);  // box, ell0, ell1, (px, py, rect width, 0)
+    // Field-major (lane-consecutive 16-B accesses): box, ell0, ell1, (px, py, rect width, 0).
+    float4 (*s_fp)[256] = reinterpret_cast<float4 (*)[256]>(s_stage);
     {
         const int tid = threadIdx.x, lane = tid & 63, wbase = tid & ~63;
         const uint32_t mine = masked ? count : 0u;
         const uint32_t rw = rect.z - rect.x + 1u;
         if (masked) {
-            s_fp[tid][0] = rec.box;
-            s_fp[tid][1] = rec.ell0;
-            s_fp[tid][2] = rec.ell1;
-            s_fp[tid][3] = make_float4(rec.a_px.w, rec.b_py.w, (float)rw, 0.0f);
+            s_fp[0][tid] = rec.box;
+            s_fp[1][tid] = rec.ell0;
+            s_fp[2][tid] = rec.ell1;
+            s_fp[3][tid] = make_float4(rec.a_px.w, rec.b_py.w, (float)rw, 0.0f);
         }
         s_bits[tid][0] = 0u;
         s_bits[tid][1] = 0u;
@@ -156,23 +159,35 @@ __global__ __launch_bounds__(256) void project_kernel(
                 if (cand < 64 && ex <= k) owner = cand;
             }
             const uint32_t o_excl = __shfl(excl, owner);
+            const uint32_t o_end = o_excl + __shfl(mine, owner);
             const uint32_t o_x0 = __shfl(rect.x, owner), o_y0 = __shfl(rect.y, owner);
+            const int prev_owner = __shfl_up(owner, 1);
+            const uint32_t t = k - o_excl;
+            bool hit = false;
             if (k < total) {
-                const uint32_t t = k - o_excl;
-                const float4 q3 = s_fp[wbase + owner][3];
+                const float4 q3 = s_fp[3][wbase + owner];
                 const uint32_t w = (uint32_t)q3.z;
                 // t < 64 and w <= 64: the float quotient's floor is exact.
                 const uint32_t ty = (uint32_t)(((float)t + 0.5f) / q3.z);
                 const uint32_t tx = t - ty * w;
                 Record fr;
-                fr.box = s_fp[wbase + owner][0];
-                fr.ell0 = s_fp[wbase + owner][1];
-                fr.ell1 = s_fp[wbase + owner][2];
+                fr.box = s_fp[0][wbase + owner];
+                fr.ell0 = s_fp[1][wbase + owner];
+                fr.ell1 = s_fp[2][wbase + owner];
                 fr.a_px.w = q3.x;
                 fr.b_py.w = q3.y;
                 const float px0 = (float)((o_x0 + tx) * ts), py0 = (float)((o_y0 + ty) * ts);
-                if (footprint_hits(fr, px0 + 0.5f, py0 + 0.5f, px0 + span, py0 + span))
-                    atomicOr(&s_bits[wbase + owner][t >> 5], 1u << (t & 31u));
+                hit = footprint_hits(fr, px0 + 0.5f, py0 + 0.5f, px0 + span, py0 + span);
+            }
+            const uint64_t bal = __ballot(hit);
+            if (k < total && (lane == 0 || prev_owner != owner)) {
+                // This lane starts its owner's run of the round: lanes
+                // [lane, lane + run) hold tiles t .. t + run - 1 (t + run <= 64).
+                const uint32_t run = min(o_end, k0 + 64u) - k;
+                const uint64_t seg = run >= 64u ? ~0ull : ((1ull << run) - 1ull);
+                const uint64_t bits = ((bal >> lane) & seg) << t;
+                s_bits[wbase + owner][0] |= (uint32_t)bits;
+                s_bits[wbase + owner][1] |= (uint32_t)(bits >> 32);
             }
         }
         __syncthreads();

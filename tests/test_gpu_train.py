@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import announce
+from conftest import announce, assert_steps_close
 
 pytestmark = pytest.mark.gpu
 
@@ -247,9 +247,8 @@ def test_batched_and_graph_steps_equal_plain(gpu):
         p, l, e = results[mode]
         assert e == e0
         np.testing.assert_allclose(l, l0, rtol=1e-5)
-        for k in p0:
-            # Same math; float atomics make the order of gradient sums differ.
-            np.testing.assert_allclose(p[k], p0[k], rtol=1e-4, atol=2e-6, err_msg=f"mode {mode} {k}")
+        # Same math; float atomics make the order of gradient sums differ.
+        assert_steps_close(p, p0, f"mode {mode}")
 
 
 def test_fused_geometry_adam_applies_adam_to_the_summed_gradients(gpu):
@@ -333,8 +332,7 @@ def test_every_schedule_gives_the_same_step(gpu):
         p, l, e = results[sched]
         assert e == e0
         np.testing.assert_allclose(l, l0, rtol=1e-5)
-        for k in p0:
-            np.testing.assert_allclose(p[k], p0[k], rtol=1e-4, atol=2e-6, err_msg=f"{sched} {k}")
+        assert_steps_close(p, p0, f"{sched}")
     t = _trainer(gpu, sc, specs, targets, deg, 3)
     t.step([0, 1, 2], mode=t.GRAPH)
     before = _params(t)
@@ -383,8 +381,7 @@ def test_overlapped_step_equals_in_order_step(gpu):
     for key, steps in results.items():
         for it, ((p, l), (p0, l0)) in enumerate(zip(steps, ref)):
             np.testing.assert_allclose(l, l0, rtol=1e-5, err_msg=f"{key} step {it}")
-            for k in p0:
-                np.testing.assert_allclose(p[k], p0[k], rtol=1e-4, atol=2e-6, err_msg=f"{key} step {it} {k}")
+            assert_steps_close(p, p0, f"{key} step {it}")
 
 
 def test_every_work_tiling_gives_the_same_images_and_step(gpu):
@@ -413,8 +410,7 @@ def test_every_work_tiling_gives_the_same_images_and_step(gpu):
             np.testing.assert_array_equal(imgs[v][..., :3], i0[v][..., :3], err_msg=f"{tl} view {v}")
             np.testing.assert_array_equal(imgs[v][..., 3], i0[v][..., 3], err_msg=f"{tl} view {v} alpha")
         np.testing.assert_allclose(l, l0, rtol=1e-5)
-        for k in p0:
-            np.testing.assert_allclose(p[k], p0[k], rtol=1e-4, atol=2e-6, err_msg=f"{tl} {k}")
+        assert_steps_close(p, p0, f"{tl}")
 
 
 def test_training_reduces_loss(gpu):
