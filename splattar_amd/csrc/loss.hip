@@ -31,7 +31,9 @@ __device__ __forceinline__ float tap(int d) { return c_w[d < 0 ? -d : d]; }
 // three channels) and walks its input rows top to bottom, one row per step:
 //   load    the input row (kSW + 20 pixels: the blur of the coefficient maps
 //           needs them 5 px beyond the strip, and those need 5 px more) into
-//           LDS as (x, y, x^2, y^2) and xy per channel;
+//           LDS as (x, y, x^2, y^2) and xy per channel, and (x, y) into a
+//           12-row LDS ring for the output stage (each pixel is read from
+//           HBM once);
 //   stats   thread (channel, stat column) of kAC = kSW + 10 columns blurs the
 //           five horizontally and keeps the last 11 rows in a register ring;
 //           once 11 are there, the vertical blur gives the window statistics
@@ -69,6 +71,9 @@ __global__ __launch_bounds__(256) void l1_ssim_fused_kernel(LossViews lv, const 
     __shared__ float4 s_p[3][kIC];    // (x, y, x^2, y^2) of the input row
     __shared__ float s_q[3][kIC];     // xy
     __shared__ float s_c[3][3][kAC];  // C_mu, C_x2, C_xy of the stat row
+    // (x, y) of the last input rows: the output stage reads its row 10 steps
+    // after the load (12 slots: the next load never meets a slot still read).
+    __shared__ float2 s_xy[12][3][kIC];
     __shared__ double s_red[2][4];
     const int tid = threadIdx.x;
     const uint32_t b = xcd_swizzle(blockIdx.x, gridDim.x);
@@ -93,8 +98,6 @@ __global__ __launch_bounds__(256) void l1_ssim_fused_kernel(LossViews lv, const 
     const bool colA_in = roleA && gxA >= 0 && gxA < iW;
     const bool colA_own = colA_in && cA >= kH && cA < kH + kSW;
     const bool colB_in = roleB && gxB < iW;
-    const float* imgf = reinterpret_cast<const float*>(img);
-    const float* tgtf = reinterpret_cast<const float*>(tgt);
     float ra[11][5], rb[11][3];
     double l1 = 0.0, ss = 0.0;
     constexpr int nrows = SH + 4 * kH;  // input rows y0 - 10 .. y0 + SH + 9
@@ -117,6 +120,10 @@ __global__ __launch_bounds__(256) void l1_ssim_fused_kernel(LossViews lv, const 
             s_q[0][tid] = a.x * t.x;
             s_q[1][tid] = a.y * t.y;
             s_q[2][tid] = a.z * t.z;
+            const int slot = i % 12;
+            s_xy[slot][0][tid] = make_float2(a.x, t.x);
+            s_xy[slot][1][tid] = make_float2(a.y, t.y);
+            s_xy[slot][2][tid] = make_float2(a.z, t.z);
         }
         __syncthreads();
         if (roleA) {
@@ -197,7 +204,8 @@ __global__ __launch_bounds__(256) void l1_ssim_fused_kernel(LossViews lv, const 
                     b2 += w * rb[s][2];
                 }
                 const size_t pix = (size_t)oy * W + gxB;
-                const float x = imgf[4 * pix + chB], y = tgtf[4 * pix + chB];
+                const float2 xy = s_xy[(i - 2 * kH) % 12][chB][cB + 2 * kH];
+                const float x = xy.x, y = xy.y;
                 l1 += (double)fabsf(x - y);
                 if (dl_out != nullptr) {
                     const float sg = x - y > 0.f ? 1.f : (x - y < 0.f ? -1.f : 0.f);

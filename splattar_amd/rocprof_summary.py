@@ -127,7 +127,7 @@ def markdown_rows(kernels, top=28):
     for k in kernels[:top]:
         row(k["kernel"], k["grid"], k["launches"], k)
         lg = k.get("large")
-        if lg and lg["launches"] < k["launches"] and k["kernel"].startswith("raster_"):
+        if lg and lg["launches"] < k["launches"] and "raster_" in k["kernel"]:
             row(k["kernel"] + " (large launches)", k["grid"], lg["launches"], lg)
     return "\n".join(rows) + "\n"
 
