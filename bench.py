@@ -24,7 +24,11 @@ are resident in HBM before timing starts.
 
 Roofline: the dominant stage's algorithmic bytes (per-stage formulas of the
 design notes) divided by its duration measured with HIP events on the step
-stream (profiled batched steps after the timed region).
+stream (profiled batched steps after the timed region). raster_hits: the
+useful hit evaluations of the first step's views (a forward traced over the
+whole image counts every pixel's contributions; untimed) per raster pass
+second, and the backward's wave instructions per useful hit from the
+committed rocprof summary.
 CPU baseline: the FP64 oracle (oracle/, OpenMP tile-binned mode, compiled
 for this host with -O3 -march=native) on a bounded sample -- one view's
 forward + loss + backward over a centred crop, scaled to a full step, median
@@ -546,6 +550,8 @@ def main():
         "losses_l1_dssim": graph_losses.round(6).tolist(),
         "roofline": roofline,
     }
+    result["raster_hits"] = useful_hits(ctx, scene, specs, ids_for(0), counts, W, H, deg, stage_ms,
+                                        roofline.get("profile"))
     if not a.no_cpu_baseline:
         import oracle
         native = native_oracle()
@@ -561,6 +567,38 @@ def main():
         result["cpu_baseline"] = None
     ctx.close()
     print(json.dumps(result), flush=True)
+
+
+def useful_hits(ctx, scene, specs, ids, counts, W, H, deg, stage_ms, prof):
+    """Useful hit evaluations of the first step's views (the (pixel, surfel)
+    pairs that composite with alpha >= 1/255 before the pixel saturates; the
+    backward differentiates exactly these): a forward traced over the whole
+    image counts each pixel's contributions (untimed, initial scene). With
+    the stage times this gives useful hits per second of each raster pass,
+    and with the committed rocprof summary the backward's wave instructions
+    per useful hit."""
+    import splattar_amd as sa
+    from splattar_amd._lib import GS_BINNING_TIGHT
+    n = scene.n
+    r = sa.Rasterizer(ctx, n, W, H, min(64 * n, (1 << 30) // 2))
+    r.upload_scene(scene.positions, scene.scales, scene.quats, scene.opacities, scene.sh)
+    r.set_binning(GS_BINNING_TIGHT)
+    r.set_trace(0, 0, W, H, 1)
+    hits = 0
+    for j in ids:
+        sp = specs[j]
+        r.forward(sa.Camera(sp.center, sp.quat, sp.focal, deg), counts[j])
+        _, c = r.trace()
+        hits += int(c.astype(np.int64).sum())
+    r.close()
+    out = {"useful_hits_per_step": hits,
+           "fwd_useful_hits_per_s": round(hits / (stage_ms["raster_fwd"] * 1e-3), 1),
+           "bwd_useful_hits_per_s": round(hits / (stage_ms["raster_bwd"] * 1e-3), 1)}
+    if prof and prof.get("kernel", "").find("raster_bwd") >= 0 and "valu" in prof:
+        valu = prof["valu"]["achieved"] * 1e9 * prof["profiled_ms"] * 1e-3
+        out["bwd_wave_instr_per_useful_hit"] = round(valu / hits, 3)
+        out["bwd_lane_instr_per_useful_hit"] = round(64 * valu / hits, 1)
+    return out
 
 
 def native_oracle():
