@@ -26,6 +26,11 @@ constexpr float kC1 = 0.0001f, kC2 = 0.0009f;
 
 __device__ __forceinline__ float tap(int d) { return c_w[d < 0 ? -d : d]; }
 
+// Workgroup barrier that waits for this thread's LDS operations only
+// (__syncthreads also drains global loads): the next row's loads, issued
+// before it, stay in flight across the row's barriers.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // One launch covers the views of a step. A workgroup
 // owns a strip of kSW output columns and SH output rows of one view (all
 // three channels) and walks its input rows top to bottom, one row per step:
@@ -103,16 +108,30 @@ __global__ __launch_bounds__(256) void l1_ssim_fused_kernel(LossViews lv, const 
     constexpr int nrows = SH + 4 * kH;  // input rows y0 - 10 .. y0 + SH + 9
     // Step i: input row y0 - 10 + i, stat row y0 - 15 + i, output row
     // y0 - 20 + i; the ring slot of step i is i % 11 (the template K).
+    // The loader threads hold the next input row in registers: its global
+    // loads are issued a step ahead and overlap the row's work. They are
+    // unconditional (clamped addresses; threads past the loaders repeat the
+    // last loader's, which coalesces), so nothing waits on them before the
+    // next step uses them; outside the image the values are replaced by 0.
+    const int lgx = x0 - 2 * kH + min(tid, kIC - 1);
+    const bool lcol = lgx >= 0 && lgx < iW;
+    const int cgx = min(max(lgx, 0), iW - 1);
+    float4 a, t;
+    bool a_ok = false;
+    auto fetch = [&](int gy) {
+        a_ok = lcol && gy >= 0 && gy < iH;
+        const size_t q = (size_t)min(max(gy, 0), iH - 1) * W + cgx;
+        a = img[q];
+        t = tgt[q];
+    };
+    fetch(y0 - 2 * kH);
     auto step = [&](auto kc, int i) {
         constexpr int K = decltype(kc)::value;
         const int gy = y0 - 2 * kH + i;
         if (tid < kIC) {
-            const int gx = x0 - 2 * kH + tid;
-            float4 a = make_float4(0.f, 0.f, 0.f, 0.f), t = a;
-            if (gx >= 0 && gx < iW && gy >= 0 && gy < iH) {
-                const size_t q = (size_t)gy * W + gx;
-                a = img[q];
-                t = tgt[q];
+            if (!a_ok) {
+                a = make_float4(0.f, 0.f, 0.f, 0.f);
+                t = a;
             }
             s_p[0][tid] = make_float4(a.x, t.x, a.x * a.x, t.x * t.x);
             s_p[1][tid] = make_float4(a.y, t.y, a.y * a.y, t.y * t.y);
@@ -125,7 +144,8 @@ __global__ __launch_bounds__(256) void l1_ssim_fused_kernel(LossViews lv, const 
             s_xy[slot][1][tid] = make_float2(a.y, t.y);
             s_xy[slot][2][tid] = make_float2(a.z, t.z);
         }
-        __syncthreads();
+        fetch(gy + 1);  // every thread, every step: no branch for the loads to wait at
+        lds_barrier();
         if (roleA) {
             float h0 = 0.f, h1 = 0.f, h2 = 0.f, h3 = 0.f, h4 = 0.f;
 #pragma unroll
@@ -178,7 +198,7 @@ __global__ __launch_bounds__(256) void l1_ssim_fused_kernel(LossViews lv, const 
                 s_c[chA][1][cA] = cx2;
                 s_c[chA][2][cA] = cxy;
             }
-            __syncthreads();
+            lds_barrier();
             if (roleB) {
                 float b0 = 0.f, b1 = 0.f, b2 = 0.f;
 #pragma unroll
@@ -215,7 +235,7 @@ __global__ __launch_bounds__(256) void l1_ssim_fused_kernel(LossViews lv, const 
                 }
             }
         } else {
-            __syncthreads();  // the stats pass has read the input row before the next load
+            lds_barrier();  // the stats pass has read the input row before the next load
         }
     };
     for (int i0 = 0; i0 < nrows; i0 += 11) {
