@@ -67,18 +67,24 @@ struct LossViews {
     float dw[kMaxLossViews], l1w[kMaxLossViews];
 };
 
+// Four waves per SIMD (at most 128 VGPRs: the two register rings take 88):
+// a fourth workgroup per CU hides the row barriers better than the third's
+// spare registers would.
 template <int SH>
-__global__ __launch_bounds__(256) void l1_ssim_fused_kernel(LossViews lv, const float4* __restrict__ img_base,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l1_ssim_fused_kernel(LossViews lv, const float4* __restrict__ img_base,
                                                             const float4* __restrict__ tgt_base,
                                                             float4* __restrict__ dl_base,
                                                             double* __restrict__ partials,
                                                             const CamParams* __restrict__ cams) {
-    __shared__ float4 s_p[3][kIC];    // (x, y, x^2, y^2) of the input row
-    __shared__ float s_q[3][kIC];     // xy
-    __shared__ float s_c[3][3][kAC];  // C_mu, C_x2, C_xy of the stat row
+    // Channels are the fastest-varying index of every LDS array, as they are
+    // of the thread roles (thread 3c + ch), so a wave's lanes read and write
+    // consecutive elements: no bank conflicts where a wave spans two channels.
+    __shared__ float4 s_p[kIC][3];    // (x, y, x^2, y^2) of the input row
+    __shared__ float s_q[kIC][3];     // xy
+    __shared__ float s_c[3][kAC][3];  // C_mu, C_x2, C_xy of the stat row
     // (x, y) of the last input rows: the output stage reads its row 10 steps
     // after the load (12 slots: the next load never meets a slot still read).
-    __shared__ float2 s_xy[12][3][kIC];
+    __shared__ float2 s_xy[12][kIC][3];
     __shared__ double s_red[2][4];
     const int tid = threadIdx.x;
     const uint32_t b = xcd_swizzle(blockIdx.x, gridDim.x);
@@ -94,11 +100,11 @@ __global__ __launch_bounds__(256) void l1_ssim_fused_kernel(LossViews lv, const 
     const uint32_t strips = (W + kSW - 1) / kSW;
     const int x0 = (int)(blk % strips) * kSW, y0 = (int)(blk / strips) * SH;
     const int iW = (int)W, iH = (int)H;
-    // Stats role: (channel, stat column x0 - 5 + cA); output role: (channel,
-    // output column x0 + cB).
+    // Stats role: thread 3 cA + chA is (stat column x0 - 5 + cA, channel chA);
+    // output role: thread 3 cB + chB is (output column x0 + cB, channel chB).
     const bool roleA = tid < 3 * kAC, roleB = tid < 3 * kSW;
-    const int chA = roleA ? tid / kAC : 0, cA = roleA ? tid - chA * kAC : 0;
-    const int chB = roleB ? tid / kSW : 0, cB = roleB ? tid - chB * kSW : 0;
+    const int cA = roleA ? tid / 3 : 0, chA = roleA ? tid - 3 * cA : 0;
+    const int cB = roleB ? tid / 3 : 0, chB = roleB ? tid - 3 * cB : 0;
     const int gxA = x0 - kH + cA, gxB = x0 + cB;
     const bool colA_in = roleA && gxA >= 0 && gxA < iW;
     const bool colA_own = colA_in && cA >= kH && cA < kH + kSW;
@@ -133,16 +139,16 @@ __global__ __launch_bounds__(256) void l1_ssim_fused_kernel(LossViews lv, const 
                 a = make_float4(0.f, 0.f, 0.f, 0.f);
                 t = a;
             }
-            s_p[0][tid] = make_float4(a.x, t.x, a.x * a.x, t.x * t.x);
-            s_p[1][tid] = make_float4(a.y, t.y, a.y * a.y, t.y * t.y);
-            s_p[2][tid] = make_float4(a.z, t.z, a.z * a.z, t.z * t.z);
-            s_q[0][tid] = a.x * t.x;
-            s_q[1][tid] = a.y * t.y;
-            s_q[2][tid] = a.z * t.z;
+            s_p[tid][0] = make_float4(a.x, t.x, a.x * a.x, t.x * t.x);
+            s_p[tid][1] = make_float4(a.y, t.y, a.y * a.y, t.y * t.y);
+            s_p[tid][2] = make_float4(a.z, t.z, a.z * a.z, t.z * t.z);
+            s_q[tid][0] = a.x * t.x;
+            s_q[tid][1] = a.y * t.y;
+            s_q[tid][2] = a.z * t.z;
             const int slot = i % 12;
-            s_xy[slot][0][tid] = make_float2(a.x, t.x);
-            s_xy[slot][1][tid] = make_float2(a.y, t.y);
-            s_xy[slot][2][tid] = make_float2(a.z, t.z);
+            s_xy[slot][tid][0] = make_float2(a.x, t.x);
+            s_xy[slot][tid][1] = make_float2(a.y, t.y);
+            s_xy[slot][tid][2] = make_float2(a.z, t.z);
         }
         fetch(gy + 1);  // every thread, every step: no branch for the loads to wait at
         lds_barrier();
@@ -150,13 +156,13 @@ __global__ __launch_bounds__(256) void l1_ssim_fused_kernel(LossViews lv, const 
             float h0 = 0.f, h1 = 0.f, h2 = 0.f, h3 = 0.f, h4 = 0.f;
 #pragma unroll
             for (int j = 0; j <= 2 * kH; ++j) {
-                const float4 p = s_p[chA][cA + j];
+                const float4 p = s_p[cA + j][chA];
                 const float w = tap(j - kH);
                 h0 += w * p.x;
                 h1 += w * p.y;
                 h2 += w * p.z;
                 h3 += w * p.w;
-                h4 += w * s_q[chA][cA + j];
+                h4 += w * s_q[cA + j][chA];
             }
             ra[K][0] = h0;
             ra[K][1] = h1;
@@ -194,9 +200,9 @@ __global__ __launch_bounds__(256) void l1_ssim_fused_kernel(LossViews lv, const 
                     cxy = (2.0f * l * rq) * dw;
                     if (colA_own && sy >= y0 && sy < y0 + SH) ss += (double)(l * cs);
                 }
-                s_c[chA][0][cA] = cm;
-                s_c[chA][1][cA] = cx2;
-                s_c[chA][2][cA] = cxy;
+                s_c[0][cA][chA] = cm;
+                s_c[1][cA][chA] = cx2;
+                s_c[2][cA][chA] = cxy;
             }
             lds_barrier();
             if (roleB) {
@@ -204,9 +210,9 @@ __global__ __launch_bounds__(256) void l1_ssim_fused_kernel(LossViews lv, const 
 #pragma unroll
                 for (int j = 0; j <= 2 * kH; ++j) {
                     const float w = tap(j - kH);
-                    b0 += w * s_c[chB][0][cB + j];
-                    b1 += w * s_c[chB][1][cB + j];
-                    b2 += w * s_c[chB][2][cB + j];
+                    b0 += w * s_c[0][cB + j][chB];
+                    b1 += w * s_c[1][cB + j][chB];
+                    b2 += w * s_c[2][cB + j][chB];
                 }
                 rb[K][0] = b0;
                 rb[K][1] = b1;
@@ -224,7 +230,7 @@ __global__ __launch_bounds__(256) void l1_ssim_fused_kernel(LossViews lv, const 
                     b2 += w * rb[s][2];
                 }
                 const size_t pix = (size_t)oy * W + gxB;
-                const float2 xy = s_xy[(i - 2 * kH) % 12][chB][cB + 2 * kH];
+                const float2 xy = s_xy[(i - 2 * kH) % 12][cB + 2 * kH][chB];
                 const float x = xy.x, y = xy.y;
                 l1 += (double)fabsf(x - y);
                 if (dl_out != nullptr) {
