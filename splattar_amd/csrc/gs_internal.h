@@ -695,11 +695,17 @@ hipError_t launch_geom_bwd(hipStream_t st, const CamParams* cams_dev, uint32_t n
 // per-view descriptor lives in device memory, so a captured graph replays
 // with whatever views the descriptor table holds). partial_sums receives
 // l1_ssim_partials(w, h) pairs (sum |x - y|, sum SSIM); dl_out may be null
-// (loss only).
+// (loss only). tstats, when set, holds the target's window mean and second
+// moment per channel (launch_target_stats), at (target pixel) * 3 + channel
+// from the same base offset as the target.
 hipError_t launch_l1_ssim(hipStream_t st, uint32_t w, uint32_t h, float lambda, float dssim_scale,
                           const float4* img, const float4* target, float4* dl_out, double* partial_sums,
-                          const CamParams* view = nullptr);
+                          const CamParams* view = nullptr, const float2* tstats = nullptr);
 uint32_t l1_ssim_partials(uint32_t w, uint32_t h);
+// The 11 x 11 window mean and second moment of each channel of a w x h target
+// (zero padding), as float2 (mean, E[y^2]) at pixel * 3 + channel: the target
+// half of the SSIM statistics, for targets that stay fixed over many steps.
+hipError_t launch_target_stats(hipStream_t st, uint32_t w, uint32_t h, const float4* target, float2* out);
 // The losses of view slots [first, first + k) of a step in one launch: view
 // j's image and dL/dimage start at img / dl_out + pix_base[j], its target at
 // target + cams_dev[first + j].target_base, its partial sums at
@@ -707,7 +713,8 @@ uint32_t l1_ssim_partials(uint32_t w, uint32_t h);
 hipError_t launch_l1_ssim_views(hipStream_t st, uint32_t first, uint32_t k, const uint32_t* widths,
                                 const uint32_t* heights, const uint32_t* pix_base, float lambda,
                                 float dssim_scale, const float4* img, const float4* target, float4* dl_out,
-                                double* partials, uint32_t partials_stride, const CamParams* cams_dev);
+                                double* partials, uint32_t partials_stride, const CamParams* cams_dev,
+                                const float2* tstats = nullptr);
 
 // mcmc.hip: exploration noise on raw positions (mcmc.wgsl) and the relocation
 // of dead surfels (relocate.wgsl). With state set, the noise takes its seed

@@ -9,7 +9,9 @@
 // dS/dE[x^2], C_xy = dw dS/dE[xy] (the zero-padded symmetric blur is
 // self-adjoint). The reference runs 19 dispatches over 16 P-sized
 // intermediates; here nothing but the image, the target and dL/dimage
-// touches HBM.
+// touches HBM, plus, for a training target that never changes, its blurred
+// mean and second moment per channel (computed once by target_stats_kernel;
+// the step then blurs three statistics of the image instead of five).
 #include "gs_internal.h"
 
 #include <type_traits>
@@ -67,19 +69,22 @@ struct LossViews {
     float dw[kMaxLossViews], l1w[kMaxLossViews];
 };
 
-// Four waves per SIMD (at most 128 VGPRs: the two register rings take 88):
-// a fourth workgroup per CU hides the row barriers better than the third's
-// spare registers would.
-template <int SH>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l1_ssim_fused_kernel(LossViews lv, const float4* __restrict__ img_base,
-                                                            const float4* __restrict__ tgt_base,
-                                                            float4* __restrict__ dl_base,
-                                                            double* __restrict__ partials,
-                                                            const CamParams* __restrict__ cams) {
+// Four waves per SIMD (at most 128 VGPRs: the two register rings take up
+// to 88): a fourth workgroup per CU hides the row barriers better than the
+// third's spare registers would. TS: the target's window mean and second
+// moment come from tstats ((target pixel) * 3 + channel) instead of being
+// blurred here.
+template <int SH, bool TS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l1_ssim_fused_kernel(
+    LossViews lv, const float4* __restrict__ img_base, const float4* __restrict__ tgt_base,
+    float4* __restrict__ dl_base, double* __restrict__ partials, const CamParams* __restrict__ cams,
+    const float2* __restrict__ tstats) {
+    constexpr int NR = TS ? 3 : 5;  // statistics blurred per row: (x, x^2, xy) or (x, y, x^2, y^2, xy)
+    using PT = std::conditional_t<TS, float2, float4>;
     // Channels are the fastest-varying index of every LDS array, as they are
     // of the thread roles (thread 3c + ch), so a wave's lanes read and write
     // consecutive elements: no bank conflicts where a wave spans two channels.
-    __shared__ float4 s_p[kIC][3];    // (x, y, x^2, y^2) of the input row
+    __shared__ PT s_p[kIC][3];        // (x, x^2) or (x, y, x^2, y^2) of the input row
     __shared__ float s_q[kIC][3];     // xy
     __shared__ float s_c[3][kAC][3];  // C_mu, C_x2, C_xy of the stat row
     // (x, y) of the last input rows: the output stage reads its row 10 steps
@@ -96,7 +101,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
     const float dw = lv.dw[v], l1w = lv.l1w[v];
     const float4* img = img_base + lv.pix_base[v];
     float4* dl_out = dl_base != nullptr ? dl_base + lv.pix_base[v] : nullptr;
-    const float4* tgt = tgt_base + (cams != nullptr ? cams[lv.first + v].target_base : 0u);
+    const uint32_t tbase = cams != nullptr ? cams[lv.first + v].target_base : 0u;
+    const float4* tgt = tgt_base + tbase;
     const uint32_t strips = (W + kSW - 1) / kSW;
     const int x0 = (int)(blk % strips) * kSW, y0 = (int)(blk / strips) * SH;
     const int iW = (int)W, iH = (int)H;
@@ -109,7 +115,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
     const bool colA_in = roleA && gxA >= 0 && gxA < iW;
     const bool colA_own = colA_in && cA >= kH && cA < kH + kSW;
     const bool colB_in = roleB && gxB < iW;
-    float ra[11][5], rb[11][3];
+    float ra[11][NR], rb[11][3];
     double l1 = 0.0, ss = 0.0;
     constexpr int nrows = SH + 4 * kH;  // input rows y0 - 10 .. y0 + SH + 9
     // Step i: input row y0 - 10 + i, stat row y0 - 15 + i, output row
@@ -119,29 +125,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
     // unconditional (clamped addresses; threads past the loaders repeat the
     // last loader's, which coalesces), so nothing waits on them before the
     // next step uses them; outside the image the values are replaced by 0.
+    // TS: the target statistics of the stat row come the same way, a step
+    // ahead (the stat row of input row gy is gy - 5).
     const int lgx = x0 - 2 * kH + min(tid, kIC - 1);
     const bool lcol = lgx >= 0 && lgx < iW;
     const int cgx = min(max(lgx, 0), iW - 1);
+    const int sgx = min(max(gxA, 0), iW - 1);
+    const float2* ts_row = TS ? tstats + (size_t)tbase * 3 + (size_t)sgx * 3 + chA : nullptr;
     float4 a, t;
+    float2 ts = make_float2(0.f, 0.f);
     bool a_ok = false;
     auto fetch = [&](int gy) {
         a_ok = lcol && gy >= 0 && gy < iH;
         const size_t q = (size_t)min(max(gy, 0), iH - 1) * W + cgx;
         a = img[q];
         t = tgt[q];
+        if constexpr (TS) ts = ts_row[(size_t)min(max(gy - kH, 0), iH - 1) * W * 3];
     };
     fetch(y0 - 2 * kH);
     auto step = [&](auto kc, int i) {
         constexpr int K = decltype(kc)::value;
         const int gy = y0 - 2 * kH + i;
+        const float2 tsc = ts;  // target statistics of this step's stat row
         if (tid < kIC) {
             if (!a_ok) {
                 a = make_float4(0.f, 0.f, 0.f, 0.f);
                 t = a;
             }
-            s_p[tid][0] = make_float4(a.x, t.x, a.x * a.x, t.x * t.x);
-            s_p[tid][1] = make_float4(a.y, t.y, a.y * a.y, t.y * t.y);
-            s_p[tid][2] = make_float4(a.z, t.z, a.z * a.z, t.z * t.z);
+            if constexpr (TS) {
+                s_p[tid][0] = make_float2(a.x, a.x * a.x);
+                s_p[tid][1] = make_float2(a.y, a.y * a.y);
+                s_p[tid][2] = make_float2(a.z, a.z * a.z);
+            } else {
+                s_p[tid][0] = make_float4(a.x, t.x, a.x * a.x, t.x * t.x);
+                s_p[tid][1] = make_float4(a.y, t.y, a.y * a.y, t.y * t.y);
+                s_p[tid][2] = make_float4(a.z, t.z, a.z * a.z, t.z * t.z);
+            }
             s_q[tid][0] = a.x * t.x;
             s_q[tid][1] = a.y * t.y;
             s_q[tid][2] = a.z * t.z;
@@ -153,22 +172,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
         fetch(gy + 1);  // every thread, every step: no branch for the loads to wait at
         lds_barrier();
         if (roleA) {
-            float h0 = 0.f, h1 = 0.f, h2 = 0.f, h3 = 0.f, h4 = 0.f;
+            if constexpr (TS) {
+                float h0 = 0.f, h2 = 0.f, h4 = 0.f;
 #pragma unroll
-            for (int j = 0; j <= 2 * kH; ++j) {
-                const float4 p = s_p[cA + j][chA];
-                const float w = tap(j - kH);
-                h0 += w * p.x;
-                h1 += w * p.y;
-                h2 += w * p.z;
-                h3 += w * p.w;
-                h4 += w * s_q[cA + j][chA];
+                for (int j = 0; j <= 2 * kH; ++j) {
+                    const float2 p = s_p[cA + j][chA];
+                    const float w = tap(j - kH);
+                    h0 += w * p.x;
+                    h2 += w * p.y;
+                    h4 += w * s_q[cA + j][chA];
+                }
+                ra[K][0] = h0;
+                ra[K][1] = h2;
+                ra[K][2] = h4;
+            } else {
+                float h0 = 0.f, h1 = 0.f, h2 = 0.f, h3 = 0.f, h4 = 0.f;
+#pragma unroll
+                for (int j = 0; j <= 2 * kH; ++j) {
+                    const float4 p = s_p[cA + j][chA];
+                    const float w = tap(j - kH);
+                    h0 += w * p.x;
+                    h1 += w * p.y;
+                    h2 += w * p.z;
+                    h3 += w * p.w;
+                    h4 += w * s_q[cA + j][chA];
+                }
+                ra[K][0] = h0;
+                ra[K][1] = h1;
+                ra[K][2] = h2;
+                ra[K][3] = h3;
+                ra[K][4] = h4;
             }
-            ra[K][0] = h0;
-            ra[K][1] = h1;
-            ra[K][2] = h2;
-            ra[K][3] = h3;
-            ra[K][4] = h4;
         }
         if (i >= 2 * kH) {
             const int sy = gy - kH;
@@ -178,11 +212,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
                 for (int d = -kH; d <= kH; ++d) {
                     const int s = (K - kH + d + 22) % 11;
                     const float w = tap(d);
-                    mu_x += w * ra[s][0];
-                    mu_y += w * ra[s][1];
-                    bx2 += w * ra[s][2];
-                    by2 += w * ra[s][3];
-                    bxy += w * ra[s][4];
+                    if constexpr (TS) {
+                        mu_x += w * ra[s][0];
+                        bx2 += w * ra[s][1];
+                        bxy += w * ra[s][2];
+                    } else {
+                        mu_x += w * ra[s][0];
+                        mu_y += w * ra[s][1];
+                        bx2 += w * ra[s][2];
+                        by2 += w * ra[s][3];
+                        bxy += w * ra[s][4];
+                    }
+                }
+                if constexpr (TS) {
+                    mu_y = tsc.x;
+                    by2 = tsc.y;
                 }
                 float cm = 0.f, cx2 = 0.f, cxy = 0.f;
                 if (colA_in && sy >= 0 && sy < iH) {
@@ -279,11 +323,53 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
 constexpr uint32_t kSH = 64;  // output rows per band
 
 hipError_t launch_fused(hipStream_t st, const LossViews& lv, const float4* img, const float4* target,
-                        float4* dl_out, double* partials, const CamParams* cams) {
+                        float4* dl_out, double* partials, const CamParams* cams, const float2* tstats) {
     const dim3 g(lv.block_begin[lv.n]);
     if (g.x == 0) return hipSuccess;
-    hipLaunchKernelGGL(l1_ssim_fused_kernel<kSH>, g, dim3(256), 0, st, lv, img, target, dl_out, partials, cams);
+    if (tstats != nullptr)
+        hipLaunchKernelGGL((l1_ssim_fused_kernel<kSH, true>), g, dim3(256), 0, st, lv, img, target, dl_out,
+                           partials, cams, tstats);
+    else
+        hipLaunchKernelGGL((l1_ssim_fused_kernel<kSH, false>), g, dim3(256), 0, st, lv, img, target, dl_out,
+                           partials, cams, tstats);
     return hipGetLastError();
+}
+
+// The target's window mean and second moment per channel (the statistics
+// l1_ssim_fused_kernel<.., true> reads): the same 11 x 11 zero-padded blur,
+// horizontal then vertical, with the taps summed in the same order as the
+// fused kernel's, one thread per pixel.
+__global__ __launch_bounds__(256) void target_stats_kernel(const float4* __restrict__ tgt, uint32_t W,
+                                                           uint32_t H, float2* __restrict__ out) {
+    const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+    if (p >= W * H) return;
+    const int x = (int)(p % W), y = (int)(p / W), iW = (int)W, iH = (int)H;
+    float mu[3] = {0.f, 0.f, 0.f}, e2[3] = {0.f, 0.f, 0.f};
+    for (int d = -kH; d <= kH; ++d) {
+        const int r = y + d;
+        float h[3] = {0.f, 0.f, 0.f}, h2[3] = {0.f, 0.f, 0.f};
+        if (r >= 0 && r < iH) {
+            for (int j = 0; j <= 2 * kH; ++j) {
+                const int c = x - kH + j;
+                if (c < 0 || c >= iW) continue;
+                const float4 v = tgt[(size_t)r * W + c];
+                const float w = tap(j - kH);
+                const float sx = v.x * v.x, sy = v.y * v.y, sz = v.z * v.z;
+                h[0] += w * v.x;
+                h[1] += w * v.y;
+                h[2] += w * v.z;
+                h2[0] += w * sx;
+                h2[1] += w * sy;
+                h2[2] += w * sz;
+            }
+        }
+        const float w = tap(d);
+        for (int k = 0; k < 3; ++k) {
+            mu[k] += w * h[k];
+            e2[k] += w * h2[k];
+        }
+    }
+    for (int k = 0; k < 3; ++k) out[(size_t)p * 3 + k] = make_float2(mu[k], e2[k]);
 }
 
 // Uniform loss weights of a w x h view: dw = -l s / 3P (D-SSIM term), the L1
@@ -298,9 +384,18 @@ void loss_weights(uint32_t w, uint32_t h, float lambda, float dssim_scale, float
 
 uint32_t l1_ssim_partials(uint32_t w, uint32_t h) { return div_up(w, kSW) * div_up(h, kSH); }
 
+hipError_t launch_target_stats(hipStream_t st, uint32_t w, uint32_t h, const float4* target, float2* out) {
+    const size_t P = (size_t)w * h;
+    if (P == 0) return hipSuccess;
+    if (P >= (1ull << 32)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(target_stats_kernel, dim3((uint32_t)div_up(P, (size_t)256)), dim3(256), 0, st, target, w, h,
+                       out);
+    return hipGetLastError();
+}
+
 hipError_t launch_l1_ssim(hipStream_t st, uint32_t w, uint32_t h, float lambda, float dssim_scale,
                           const float4* img, const float4* target, float4* dl_out, double* partial_sums,
-                          const CamParams* view) {
+                          const CamParams* view, const float2* tstats) {
     if (w == 0 || h == 0) return hipSuccess;
     LossViews lv{};
     lv.n = 1;
@@ -308,13 +403,14 @@ hipError_t launch_l1_ssim(hipStream_t st, uint32_t w, uint32_t h, float lambda, 
     lv.h[0] = h;
     loss_weights(w, h, lambda, dssim_scale, &lv.dw[0], &lv.l1w[0]);
     lv.block_begin[1] = l1_ssim_partials(w, h);
-    return launch_fused(st, lv, img, target, dl_out, partial_sums, view);
+    return launch_fused(st, lv, img, target, dl_out, partial_sums, view, tstats);
 }
 
 hipError_t launch_l1_ssim_views(hipStream_t st, uint32_t first, uint32_t k, const uint32_t* widths,
                                 const uint32_t* heights, const uint32_t* pix_base, float lambda,
                                 float dssim_scale, const float4* img, const float4* target, float4* dl_out,
-                                double* partials, uint32_t partials_stride, const CamParams* cams_dev) {
+                                double* partials, uint32_t partials_stride, const CamParams* cams_dev,
+                                const float2* tstats) {
     if (k == 0) return hipSuccess;
     if (k > (uint32_t)kMaxLossViews) return hipErrorInvalidValue;
     LossViews lv{};
@@ -331,7 +427,7 @@ hipError_t launch_l1_ssim_views(hipStream_t st, uint32_t first, uint32_t k, cons
         blocks += l1_ssim_partials(widths[j], heights[j]);
     }
     lv.block_begin[k] = blocks;
-    return launch_fused(st, lv, img, target, dl_out, partials, cams_dev);
+    return launch_fused(st, lv, img, target, dl_out, partials, cams_dev, tstats);
 }
 
 }  // namespace gs

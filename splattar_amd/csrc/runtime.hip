@@ -2134,6 +2134,9 @@ struct gs_trainer_s {
     uint32_t n = 0;
     std::vector<TrainViewDev> views;
     float4* targets = nullptr;
+    // The targets' window mean and second moment per channel (fixed with the
+    // targets, so computed once at creation; the step's loss reads them).
+    float2* target_stats = nullptr;
     // activated parameters + gradients (layout of gs_raster_buffers)
     float4* positions = nullptr;
     float2* scales = nullptr;
@@ -2198,6 +2201,7 @@ struct gs_trainer_s {
         if (graph) (void)hipGraphDestroy(graph);
         if (optim) gs_optim_destroy(optim);
         dfree(targets);
+        dfree(target_stats);
         dfree(positions);
         dfree(scales);
         dfree(quats);
@@ -2253,7 +2257,8 @@ struct gs_trainer_s {
         const CamParams& c = core.cams[j];
         GS_HIP(launch_l1_ssim(s, tv.width, tv.height, cfg.lambda, cfg.dssim_scale, core.out_color + c.pix_base,
                               targets, dl_dcolor + c.pix_base,
-                              loss_partials + (size_t)j * partials_per_view * 2, core.cams_dev + j));
+                              loss_partials + (size_t)j * partials_per_view * 2, core.cams_dev + j,
+                              target_stats));
         return GS_OK;
     }
 
@@ -2273,7 +2278,7 @@ struct gs_trainer_s {
         }
         GS_HIP(launch_l1_ssim_views(s, j0, k, ws.data(), hs.data(), pb.data(), cfg.lambda, cfg.dssim_scale,
                                     core.out_color, targets, dl_dcolor, loss_partials, partials_per_view,
-                                    core.cams_dev));
+                                    core.cams_dev, target_stats));
         return GS_OK;
     }
 
@@ -2346,7 +2351,7 @@ gs_status gs_trainer_s::enqueue_step(hipStream_t st, const uint32_t* ids, uint32
             if (P) GS_CHECK(P->mark(st, GS_STAGE_LOSS));
             GS_HIP(launch_l1_ssim(st, tv.width, tv.height, cfg.lambda, cfg.dssim_scale, core.out_color,
                                   targets, dl_dcolor, loss_partials + (size_t)j * partials_per_view * 2,
-                                  core.cams_dev + j));
+                                  core.cams_dev + j, target_stats));
             GS_CHECK(core.backward(st, j, 1, n, positions, scales, quats, sh, dl_dcolor, nullptr, g,
                                    j > 0, false, P));
         }
@@ -2455,6 +2460,13 @@ gs_status gs_trainer_create(gs_ctx ctx, const gs_train_config* cfg, const gs_sce
                              hipMemcpyHostToDevice));
         }
     }
+    GS_CHECK(dalloc(&t->target_stats, toff * 3, "target statistics"));
+    for (uint32_t v = 0; v < n_views; ++v) {
+        const TrainViewDev& tv = t->views[v];
+        GS_HIP(launch_target_stats(ctx->stream, tv.width, tv.height, t->targets + tv.target_offset,
+                                   t->target_stats + tv.target_offset * 3));
+    }
+    GS_HIP(hipStreamSynchronize(ctx->stream));
     GS_CHECK(dalloc(&t->positions, n, "positions"));
     GS_CHECK(dalloc(&t->scales, n, "scales"));
     GS_CHECK(dalloc(&t->quats, n, "quats"));
