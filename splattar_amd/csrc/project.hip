@@ -112,16 +112,11 @@ __global__ __launch_bounds__(256) void project_kernel(
     }
 
     // Per-tile footprint masks, wave-cooperatively: the (item, tile) pairs of
-    // the wave's masked items are dealt 64 at a time over the lanes, each lane
+    // the wave's masked items are dealt 64 at a time over the lanes (owner by
+    // binary search over the wave's exclusive scan of tile counts), each lane
     // tests one tile against its owner's footprint read from LDS; an owner's
-    // pairs of a round are consecutive lanes (a run), so its bits are a
-    // shifted slice of the round's ballot, merged into its mask by the run's
-    // first lane. Owners: every item whose run meets the round writes its
-    // lane into the round slot where the run starts; the ballot of the
-    // written slots marks the run starts, and a lane's owner is the item of
-    // the nearest start at or below it (one LDS round trip and a shuffle,
-    // where a binary search over the tile-count scan needs six shuffles in a
-    // dependent chain).
+    // pairs of a round are consecutive lanes, so its bits are a shifted slice
+    // of the round's ballot, merged into its mask by the run's first lane.
     // Bit t = row-major tile t of the rect.
     // The cost follows the wave's total tile count, not 64 x its widest item.
     // s_stage holds the footprints for the masks (s_fp), then the block's
@@ -131,7 +126,6 @@ __global__ __launch_bounds__(256) void project_kernel(
     __shared__ float4 s_stage[256 * kRecRow];
     __shared__ uint32_t s_bits[256][2];
     __shared__ uint32_t s_write[256];
-    __shared__ uint32_t s_own[256];  // per wave: the item whose run starts at each lane of a round, else 64
     // Field-major (lane-consecutive 16-B accesses): box, ell0, ell1, (px, py, rect width, 0).
     float4 (*s_fp)[256] = reinterpret_cast<float4 (*)[256]>(s_stage);
     {
@@ -158,25 +152,16 @@ __global__ __launch_bounds__(256) void project_kernel(
         const float span = (float)ts - 0.5f;
         for (uint32_t k0 = 0; k0 < total; k0 += 64) {
             const uint32_t k = k0 + (uint32_t)lane;
-            // A wave's LDS instructions complete in issue order, so the clear,
-            // the other lanes' claims and the read below see each other; the
-            // wavefront-scope fences keep the compiler to that order (no
-            // forwarding of this lane's own clear to its read).
-            s_own[tid] = 64u;
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            if (mine != 0u && excl < k0 + 64u && excl + mine > k0)
-                s_own[wbase + (excl > k0 ? excl - k0 : 0u)] = (uint32_t)lane;
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            const uint32_t claim = s_own[tid];
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            // Slot 0 is always claimed (the run holding tile k0 starts there).
-            const uint64_t starts = __ballot(claim != 64u);
-            const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-            const int start = 63 - __builtin_clzll(starts & upto);
-            const int owner = __shfl((int)claim, start);
+            int owner = 0;
+            for (int step = 32; step >= 1; step >>= 1) {
+                const int cand = owner + step;
+                const uint32_t ex = __shfl(excl, cand & 63);
+                if (cand < 64 && ex <= k) owner = cand;
+            }
             const uint32_t o_excl = __shfl(excl, owner);
             const uint32_t o_end = o_excl + __shfl(mine, owner);
             const uint32_t o_x0 = __shfl(rect.x, owner), o_y0 = __shfl(rect.y, owner);
+            const int prev_owner = __shfl_up(owner, 1);
             const uint32_t t = k - o_excl;
             bool hit = false;
             if (k < total) {
@@ -195,7 +180,7 @@ __global__ __launch_bounds__(256) void project_kernel(
                 hit = footprint_hits(fr, px0 + 0.5f, py0 + 0.5f, px0 + span, py0 + span);
             }
             const uint64_t bal = __ballot(hit);
-            if (k < total && start == lane) {
+            if (k < total && (lane == 0 || prev_owner != owner)) {
                 // This lane starts its owner's run of the round: lanes
                 // [lane, lane + run) hold tiles t .. t + run - 1 (t + run <= 64).
                 const uint32_t run = min(o_end, k0 + 64u) - k;
