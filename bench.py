@@ -12,15 +12,17 @@ of 500k surfels that grows 10 % per frame (frame k renders its first
 500k + 50k k surfels; 1.25M in all), every frame in one graph launch per
 step, targets rendered from a jittered copy.
 
-A "step" = project, scan, emit keys, 64-bit radix sort, tile ranges,
-rasterize forward, L1 + D-SSIM, rasterize backward, geometry backward and
-fused Adam over the 4 views. `value` is steps/s of the batched step captured
+A "step" = project (with the depth presort keys), scan, emit tile keys,
+32-bit tile radix sort, tile ranges, rasterize forward, L1 + D-SSIM,
+rasterize backward, geometry backward and fused Adam over the 4 views. `value` is steps/s of the batched step captured
 in a hipGraph: 1000 / the median per-step time over steps 10..500 of a
 500-step training run from the initial state (a HIP event on the step stream
 before every step). The plain per-view step (every kernel launched per
 view, no graph) runs the same 500 steps from the same state and is timed the
-same way; both runs end with the PSNR over all views (equal loss). Inputs
-are resident in HBM before timing starts.
+same way; both runs end with the PSNR over all views (equal loss), and two
+more untimed pairs of such runs give the mean PSNR difference (float atomics
+make every run's trajectory differ in the last bits, so one pair alone
+carries run-to-run noise). Inputs are resident in HBM before timing starts.
 
 Roofline: the dominant stage's algorithmic bytes (per-stage formulas of the
 design notes) divided by its duration measured with HIP events on the step
@@ -114,6 +116,9 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-autotune", action="store_true",
                    help="keep --digit-bits and the default tile queue instead of autotuning")
+    p.add_argument("--psnr-repeats", type=int, default=3,
+                   help="pairs of (batched + graph, plain) runs from the initial state for the equal-loss "
+                        "PSNR (the first pair is the timed one); psnr_* are their means")
     p.add_argument("--psnr-steps", type=int, default=500,
                    help="steps of the headline run and of the plain run it is compared with at equal "
                         "loss (value = median step over steps 10..psnr-steps)")
@@ -472,6 +477,29 @@ def main():
     plain_stages, _ = tp.stage_times()
     tp.close()
     log(f"plain: per-step ms median(10..) {np.median(plain_ms[10:]):.3f}; equal loss {equal_loss}")
+    # Float atomics order the gradient sums differently from run to run, so
+    # one pair of 500-step runs differs by run-to-run noise on top of any
+    # batched-vs-plain difference (each form alone moves by a few hundredths
+    # of a dB between runs). Further pairs from the same initial state give
+    # the mean difference.
+    runs = [(equal_loss["psnr_batched_graph_db"], equal_loss["psnr_plain_db"])]
+    for _ in range(max(a.psnr_repeats, 1) - 1):
+        pair = []
+        for mode in ("graph", "plain"):
+            t = sa.Trainer(ctx, cfg, scene.as_init(), views)
+            apply_schedule(t)
+            run(t, t.GRAPH if mode == "graph" else t.PLAIN, total, 0)
+            ctx.synchronize()
+            pair.append(round(psnr(t), 4))
+            t.close()
+        runs.append(tuple(pair))
+    diffs = [b - q for b, q in runs]
+    equal_loss.update(psnr_batched_graph_db=round(float(np.mean([b for b, _ in runs])), 4),
+                      psnr_plain_db=round(float(np.mean([q for _, q in runs])), 4),
+                      psnr_diff_db=round(float(np.mean(diffs)), 4),
+                      runs=[list(r) for r in runs],
+                      psnr_diff_db_per_run=[round(d, 4) for d in diffs])
+    log(f"equal loss over {len(runs)} run pairs: {equal_loss}")
 
     # ---- batched without a graph, then profiled batched steps (stage times)
     tb = sa.Trainer(ctx, cfg, scene.as_init(), views)
