@@ -156,7 +156,9 @@ def stage_bytes(n, V, E, P, T, sort_passes, S=232, nv=None):
         "sort": (8 + 24 * sort_passes) * E,
         "tile_ranges": 8 * E + 8 * T,
         "raster_fwd": 56 * E + 20 * P,
-        "l1_ssim": 36 * P,
+        # image, target (rgb) and dL/dimage, plus the target's window mean and
+        # second moment per channel that the trainer's loss reads.
+        "l1_ssim": 60 * P,
         "grad_clear": 80 * nv,
         "raster_bwd": 56 * E + 20 * P + 52 * nv,
         "geom_bwd": nv * (2 * S + 52),
