@@ -178,9 +178,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
                 for (int j = 0; j <= 2 * kH; ++j) {
                     const float2 p = s_p[cA + j][chA];
                     const float w = tap(j - kH);
-                    h0 += w * p.x;
-                    h2 += w * p.y;
-                    h4 += w * s_q[cA + j][chA];
+                    h0 = fmaf(w, p.x, h0);
+                    h2 = fmaf(w, p.y, h2);
+                    h4 = fmaf(w, s_q[cA + j][chA], h4);
                 }
                 ra[K][0] = h0;
                 ra[K][1] = h2;
@@ -191,11 +191,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
                 for (int j = 0; j <= 2 * kH; ++j) {
                     const float4 p = s_p[cA + j][chA];
                     const float w = tap(j - kH);
-                    h0 += w * p.x;
-                    h1 += w * p.y;
-                    h2 += w * p.z;
-                    h3 += w * p.w;
-                    h4 += w * s_q[cA + j][chA];
+                    h0 = fmaf(w, p.x, h0);
+                    h1 = fmaf(w, p.y, h1);
+                    h2 = fmaf(w, p.z, h2);
+                    h3 = fmaf(w, p.w, h3);
+                    h4 = fmaf(w, s_q[cA + j][chA], h4);
                 }
                 ra[K][0] = h0;
                 ra[K][1] = h1;
@@ -213,15 +213,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
                     const int s = (K - kH + d + 22) % 11;
                     const float w = tap(d);
                     if constexpr (TS) {
-                        mu_x += w * ra[s][0];
-                        bx2 += w * ra[s][1];
-                        bxy += w * ra[s][2];
+                        mu_x = fmaf(w, ra[s][0], mu_x);
+                        bx2 = fmaf(w, ra[s][1], bx2);
+                        bxy = fmaf(w, ra[s][2], bxy);
                     } else {
-                        mu_x += w * ra[s][0];
-                        mu_y += w * ra[s][1];
-                        bx2 += w * ra[s][2];
-                        by2 += w * ra[s][3];
-                        bxy += w * ra[s][4];
+                        mu_x = fmaf(w, ra[s][0], mu_x);
+                        mu_y = fmaf(w, ra[s][1], mu_y);
+                        bx2 = fmaf(w, ra[s][2], bx2);
+                        by2 = fmaf(w, ra[s][3], by2);
+                        bxy = fmaf(w, ra[s][4], bxy);
                     }
                 }
                 if constexpr (TS) {
@@ -231,14 +231,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
                 float cm = 0.f, cx2 = 0.f, cxy = 0.f;
                 if (colA_in && sy >= 0 && sy < iH) {
                     // ssim.wgsl:95-126 (partials).
-                    const float sx2 = bx2 - mu_x * mu_x, sy2 = by2 - mu_y * mu_y, sxy_ = bxy - mu_x * mu_y;
-                    const float p = mu_x * mu_x + mu_y * mu_y + kC1;
-                    const float q = sx2 + sy2 + kC2;
+                    // Every multiply-add is an explicit fmaf: the rounding is then
+                    // the same in both template variants (contraction left to the
+                    // compiler may fuse different products in each).
+                    const float sx2 = fmaf(-mu_x, mu_x, bx2), sy2 = fmaf(-mu_y, mu_y, by2);
+                    const float sxy_ = fmaf(-mu_x, mu_y, bxy);
+                    const float p = fmaf(mu_x, mu_x, fmaf(mu_y, mu_y, kC1));
+                    const float q = (sx2 + sy2) + kC2;
                     const float rp = rcp_refined(p), rq = rcp_refined(q);
-                    const float l = (2.0f * mu_x * mu_y + kC1) * rp;
-                    const float cs = (2.0f * sxy_ + kC2) * rq;
-                    const float ds_dmux =
-                        cs * (2.0f * mu_y - 2.0f * mu_x * l) * rp + l * (-2.0f * mu_y + 2.0f * mu_x * cs) * rq;
+                    const float l = fmaf(2.0f * mu_x, mu_y, kC1) * rp;
+                    const float cs = fmaf(2.0f, sxy_, kC2) * rq;
+                    const float ga = fmaf(-2.0f * mu_x, l, 2.0f * mu_y);
+                    const float gb = fmaf(2.0f * mu_x, cs, -2.0f * mu_y);
+                    const float ds_dmux = fmaf(cs * ga, rp, (l * gb) * rq);
                     cm = ds_dmux * dw;
                     cx2 = (-l * cs * rq) * dw;
                     cxy = (2.0f * l * rq) * dw;
@@ -254,9 +259,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
 #pragma unroll
                 for (int j = 0; j <= 2 * kH; ++j) {
                     const float w = tap(j - kH);
-                    b0 += w * s_c[0][cB + j][chB];
-                    b1 += w * s_c[1][cB + j][chB];
-                    b2 += w * s_c[2][cB + j][chB];
+                    b0 = fmaf(w, s_c[0][cB + j][chB], b0);
+                    b1 = fmaf(w, s_c[1][cB + j][chB], b1);
+                    b2 = fmaf(w, s_c[2][cB + j][chB], b2);
                 }
                 rb[K][0] = b0;
                 rb[K][1] = b1;
@@ -269,9 +274,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
                 for (int d = -kH; d <= kH; ++d) {
                     const int s = (K - kH + d + 22) % 11;
                     const float w = tap(d);
-                    b0 += w * rb[s][0];
-                    b1 += w * rb[s][1];
-                    b2 += w * rb[s][2];
+                    b0 = fmaf(w, rb[s][0], b0);
+                    b1 = fmaf(w, rb[s][1], b1);
+                    b2 = fmaf(w, rb[s][2], b2);
                 }
                 const size_t pix = (size_t)oy * W + gxB;
                 const float2 xy = s_xy[(i - 2 * kH) % 12][cB + 2 * kH][chB];
@@ -280,7 +285,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
                 if (dl_out != nullptr) {
                     const float sg = x - y > 0.f ? 1.f : (x - y < 0.f ? -1.f : 0.f);
                     float* o = reinterpret_cast<float*>(dl_out) + 4 * pix;
-                    o[chB] = sg * l1w + b0 + 2.0f * x * b1 + y * b2;
+                    o[chB] = fmaf(y, b2, fmaf(2.0f * x, b1, fmaf(sg, l1w, b0)));
                     if (chB == 0) o[3] = 0.0f;
                 }
             }
@@ -355,18 +360,18 @@ __global__ __launch_bounds__(256) void target_stats_kernel(const float4* __restr
                 const float4 v = tgt[(size_t)r * W + c];
                 const float w = tap(j - kH);
                 const float sx = v.x * v.x, sy = v.y * v.y, sz = v.z * v.z;
-                h[0] += w * v.x;
-                h[1] += w * v.y;
-                h[2] += w * v.z;
-                h2[0] += w * sx;
-                h2[1] += w * sy;
-                h2[2] += w * sz;
+                h[0] = fmaf(w, v.x, h[0]);
+                h[1] = fmaf(w, v.y, h[1]);
+                h[2] = fmaf(w, v.z, h[2]);
+                h2[0] = fmaf(w, sx, h2[0]);
+                h2[1] = fmaf(w, sy, h2[1]);
+                h2[2] = fmaf(w, sz, h2[2]);
             }
         }
         const float w = tap(d);
         for (int k = 0; k < 3; ++k) {
-            mu[k] += w * h[k];
-            e2[k] += w * h2[k];
+            mu[k] = fmaf(w, h[k], mu[k]);
+            e2[k] = fmaf(w, h2[k], e2[k]);
         }
     }
     for (int k = 0; k < 3; ++k) out[(size_t)p * 3 + k] = make_float2(mu[k], e2[k]);

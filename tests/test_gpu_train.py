@@ -51,6 +51,45 @@ def test_l1_ssim_matches_fp64_oracle(gpu, W, H, scale):
     dl.free()
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_trainer_loss_with_target_statistics_equals_the_loss_op(gpu, mode):
+    """The trainer's loss reads each target's window mean and second moment
+    from a table computed once at creation (target_stats_kernel); the loss op
+    (gs_loss, ssim.rs:25-298), whose target the caller may rewrite at any
+    time, blurs all five statistics itself. Both sum the same taps in the
+    same order, so on the image the trainer renders the two agree exactly:
+    L1 and D-SSIM of every view, plain (one launch per view) and batched
+    (one launch for all), on sizes that are not multiples of the 74 x 64
+    strip (partial strips and bands at the right and bottom edges)."""
+    import splattar_amd as sa
+    from splattar_amd import scenes
+    deg = 2
+    sc = scenes.make_scene(2500, deg, -3.4, 0.4, 41)
+    tgt = scenes.jittered(sc, 0.01, 42)
+    specs = scenes.sphere_views(3, 47, [(150, 97), (96, 64), (37, 29)])
+    targets = _render_targets(gpu, tgt, specs, deg)
+    t = _trainer(gpu, sc, specs, targets, deg, 3)
+    imgs = [t.render(v) for v in range(3)]
+    announce(f"trainer loss vs loss op, mode {mode}")
+    t.step([0, 1, 2], mode=t.BATCHED if mode == 1 else t.PLAIN)
+    got = t.last_losses(3)
+    t.close()
+    for v, sp in enumerate(specs):
+        W, H = sp.width, sp.height
+        img = gpu.alloc(np.float32, (H, W, 4))
+        dl = gpu.alloc(np.float32, (H, W, 4))
+        img.write(imgs[v])
+        loss = sa.SsimLoss(gpu, W, H, 0.2, img, dl, dssim_scale=0.5)
+        loss.target.write(targets[v])
+        loss.encode()
+        l1, dssim = loss.read_losses()
+        loss.close()
+        img.free()
+        dl.free()
+        print(f"view {v} {W}x{H}: trainer ({got[v, 0]:.9f}, {got[v, 1]:.9f}) op ({l1:.9f}, {dssim:.9f})")
+        assert got[v, 0] == l1 and got[v, 1] == dssim
+
+
 def _ref_xorshift(state):
     x = state[0]
     m = (1 << 64) - 1
