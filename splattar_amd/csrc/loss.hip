@@ -69,13 +69,14 @@ struct LossViews {
     float dw[kMaxLossViews], l1w[kMaxLossViews];
 };
 
-// Four waves per SIMD (at most 128 VGPRs: the two register rings take up
-// to 88): a fourth workgroup per CU hides the row barriers better than the
-// third's spare registers would. TS: the target's window mean and second
-// moment come from tstats ((target pixel) * 3 + channel) instead of being
-// blurred here.
+// TS: the target's window mean and second moment come from tstats
+// ((target pixel) * 3 + channel) instead of being blurred here; four waves
+// per SIMD (at most 128 VGPRs: the two register rings take 66; 36 KB of
+// LDS), a fourth workgroup per CU hiding the row barrier better than the
+// third's spare registers would. Without TS the float4 input rows need
+// 40.5 KB of LDS, three workgroups per CU, so the registers are let grow.
 template <int SH, bool TS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l1_ssim_fused_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TS ? 4 : 3))) void l1_ssim_fused_kernel(
     LossViews lv, const float4* __restrict__ img_base, const float4* __restrict__ tgt_base,
     float4* __restrict__ dl_base, double* __restrict__ partials, const CamParams* __restrict__ cams,
     const float2* __restrict__ tstats) {
@@ -84,12 +85,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
     // Channels are the fastest-varying index of every LDS array, as they are
     // of the thread roles (thread 3c + ch), so a wave's lanes read and write
     // consecutive elements: no bank conflicts where a wave spans two channels.
-    __shared__ PT s_p[kIC][3];        // (x, x^2) or (x, y, x^2, y^2) of the input row
-    __shared__ float s_q[kIC][3];     // xy
-    __shared__ float s_c[3][kAC][3];  // C_mu, C_x2, C_xy of the stat row
-    // (x, y) of the last input rows: the output stage reads its row 10 steps
-    // after the load (12 slots: the next load never meets a slot still read).
-    __shared__ float2 s_xy[12][kIC][3];
+    __shared__ PT s_p[2][kIC][3];        // (x, x^2) or (x, y, x^2, y^2) of the input row, double-buffered
+    __shared__ float s_q[2][kIC][3];     // xy
+    __shared__ float s_c[2][3][kAC][3];  // C_mu, C_x2, C_xy of a stat row, double-buffered
+    // (x, y) of the strip's output columns for the last input rows: the
+    // output stage reads its row 11 steps after the load (13 slots: the
+    // loads of the next two steps never meet a slot still being read).
+    __shared__ float2 s_xy[13][kSW][3];
     __shared__ double s_red[2][4];
     const int tid = threadIdx.x;
     const uint32_t b = xcd_swizzle(blockIdx.x, gridDim.x);
@@ -118,8 +120,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
     float ra[11][NR], rb[11][3];
     double l1 = 0.0, ss = 0.0;
     constexpr int nrows = SH + 4 * kH;  // input rows y0 - 10 .. y0 + SH + 9
-    // Step i: input row y0 - 10 + i, stat row y0 - 15 + i, output row
-    // y0 - 20 + i; the ring slot of step i is i % 11 (the template K).
+    // Step i (0 <= i <= nrows): input row y0 - 10 + i and stat row
+    // y0 - 15 + i (i < nrows); the C-map row of the previous step's stat row
+    // is blurred horizontally and output row y0 - 21 + i is finished. The
+    // input ring slot of step i is i % 11 (the template K), the C-map ring
+    // slot of step i - 1 is (i - 1) % 11 (KP). One barrier per step: the
+    // input row and the previous step's C-map row go into alternate buffers
+    // before it, so a buffer is rewritten two steps after it is read, when
+    // every thread has passed the barrier between.
     // The loader threads hold the next input row in registers: its global
     // loads are issued a step ahead and overlap the row's work. They are
     // unconditional (clamped addresses; threads past the loaders repeat the
@@ -143,44 +151,54 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
         if constexpr (TS) ts = ts_row[(size_t)min(max(gy - kH, 0), iH - 1) * W * 3];
     };
     fetch(y0 - 2 * kH);
+    float cm = 0.f, cx2 = 0.f, cxy = 0.f;  // the C-map values of this thread's last stat row
     auto step = [&](auto kc, int i) {
         constexpr int K = decltype(kc)::value;
+        constexpr int KP = (K + 10) % 11;
         const int gy = y0 - 2 * kH + i;
+        const int buf = i & 1;
         const float2 tsc = ts;  // target statistics of this step's stat row
-        if (tid < kIC) {
+        if (tid < kIC && i < nrows) {
             if (!a_ok) {
                 a = make_float4(0.f, 0.f, 0.f, 0.f);
                 t = a;
             }
             if constexpr (TS) {
-                s_p[tid][0] = make_float2(a.x, a.x * a.x);
-                s_p[tid][1] = make_float2(a.y, a.y * a.y);
-                s_p[tid][2] = make_float2(a.z, a.z * a.z);
+                s_p[buf][tid][0] = make_float2(a.x, a.x * a.x);
+                s_p[buf][tid][1] = make_float2(a.y, a.y * a.y);
+                s_p[buf][tid][2] = make_float2(a.z, a.z * a.z);
             } else {
-                s_p[tid][0] = make_float4(a.x, t.x, a.x * a.x, t.x * t.x);
-                s_p[tid][1] = make_float4(a.y, t.y, a.y * a.y, t.y * t.y);
-                s_p[tid][2] = make_float4(a.z, t.z, a.z * a.z, t.z * t.z);
+                s_p[buf][tid][0] = make_float4(a.x, t.x, a.x * a.x, t.x * t.x);
+                s_p[buf][tid][1] = make_float4(a.y, t.y, a.y * a.y, t.y * t.y);
+                s_p[buf][tid][2] = make_float4(a.z, t.z, a.z * a.z, t.z * t.z);
             }
-            s_q[tid][0] = a.x * t.x;
-            s_q[tid][1] = a.y * t.y;
-            s_q[tid][2] = a.z * t.z;
-            const int slot = i % 12;
-            s_xy[slot][tid][0] = make_float2(a.x, t.x);
-            s_xy[slot][tid][1] = make_float2(a.y, t.y);
-            s_xy[slot][tid][2] = make_float2(a.z, t.z);
+            s_q[buf][tid][0] = a.x * t.x;
+            s_q[buf][tid][1] = a.y * t.y;
+            s_q[buf][tid][2] = a.z * t.z;
+            if (tid >= 2 * kH && tid < 2 * kH + kSW) {
+                const int slot = i % 13;
+                s_xy[slot][tid - 2 * kH][0] = make_float2(a.x, t.x);
+                s_xy[slot][tid - 2 * kH][1] = make_float2(a.y, t.y);
+                s_xy[slot][tid - 2 * kH][2] = make_float2(a.z, t.z);
+            }
+        }
+        if (roleA && i > 2 * kH) {  // the previous step's C-map row
+            s_c[buf ^ 1][0][cA][chA] = cm;
+            s_c[buf ^ 1][1][cA][chA] = cx2;
+            s_c[buf ^ 1][2][cA][chA] = cxy;
         }
         fetch(gy + 1);  // every thread, every step: no branch for the loads to wait at
         lds_barrier();
-        if (roleA) {
+        if (roleA && i < nrows) {
             if constexpr (TS) {
                 float h0 = 0.f, h2 = 0.f, h4 = 0.f;
 #pragma unroll
                 for (int j = 0; j <= 2 * kH; ++j) {
-                    const float2 p = s_p[cA + j][chA];
+                    const float2 p = s_p[buf][cA + j][chA];
                     const float w = tap(j - kH);
                     h0 = fmaf(w, p.x, h0);
                     h2 = fmaf(w, p.y, h2);
-                    h4 = fmaf(w, s_q[cA + j][chA], h4);
+                    h4 = fmaf(w, s_q[buf][cA + j][chA], h4);
                 }
                 ra[K][0] = h0;
                 ra[K][1] = h2;
@@ -189,13 +207,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
                 float h0 = 0.f, h1 = 0.f, h2 = 0.f, h3 = 0.f, h4 = 0.f;
 #pragma unroll
                 for (int j = 0; j <= 2 * kH; ++j) {
-                    const float4 p = s_p[cA + j][chA];
+                    const float4 p = s_p[buf][cA + j][chA];
                     const float w = tap(j - kH);
                     h0 = fmaf(w, p.x, h0);
                     h1 = fmaf(w, p.y, h1);
                     h2 = fmaf(w, p.z, h2);
                     h3 = fmaf(w, p.w, h3);
-                    h4 = fmaf(w, s_q[cA + j][chA], h4);
+                    h4 = fmaf(w, s_q[buf][cA + j][chA], h4);
                 }
                 ra[K][0] = h0;
                 ra[K][1] = h1;
@@ -203,10 +221,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
                 ra[K][3] = h3;
                 ra[K][4] = h4;
             }
-        }
-        if (i >= 2 * kH) {
-            const int sy = gy - kH;
-            if (roleA) {
+            if (i >= 2 * kH) {
+                const int sy = gy - kH;
                 float mu_x = 0.f, mu_y = 0.f, bx2 = 0.f, by2 = 0.f, bxy = 0.f;
 #pragma unroll
                 for (int d = -kH; d <= kH; ++d) {
@@ -228,7 +244,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
                     mu_y = tsc.x;
                     by2 = tsc.y;
                 }
-                float cm = 0.f, cx2 = 0.f, cxy = 0.f;
+                cm = cx2 = cxy = 0.f;
                 if (colA_in && sy >= 0 && sy < iH) {
                     // ssim.wgsl:95-126 (partials).
                     // Every multiply-add is an explicit fmaf: the rounding is then
@@ -249,37 +265,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
                     cxy = (2.0f * l * rq) * dw;
                     if (colA_own && sy >= y0 && sy < y0 + SH) ss += (double)(l * cs);
                 }
-                s_c[0][cA][chA] = cm;
-                s_c[1][cA][chA] = cx2;
-                s_c[2][cA][chA] = cxy;
             }
-            lds_barrier();
+        }
+        if (i > 2 * kH) {
             if (roleB) {
                 float b0 = 0.f, b1 = 0.f, b2 = 0.f;
 #pragma unroll
                 for (int j = 0; j <= 2 * kH; ++j) {
                     const float w = tap(j - kH);
-                    b0 = fmaf(w, s_c[0][cB + j][chB], b0);
-                    b1 = fmaf(w, s_c[1][cB + j][chB], b1);
-                    b2 = fmaf(w, s_c[2][cB + j][chB], b2);
+                    b0 = fmaf(w, s_c[buf ^ 1][0][cB + j][chB], b0);
+                    b1 = fmaf(w, s_c[buf ^ 1][1][cB + j][chB], b1);
+                    b2 = fmaf(w, s_c[buf ^ 1][2][cB + j][chB], b2);
                 }
-                rb[K][0] = b0;
-                rb[K][1] = b1;
-                rb[K][2] = b2;
+                rb[KP][0] = b0;
+                rb[KP][1] = b1;
+                rb[KP][2] = b2;
             }
-            const int oy = gy - 2 * kH;
-            if (i >= 4 * kH && roleB && colB_in && oy < iH) {
+            const int oy = gy - 2 * kH - 1;
+            if (i > 4 * kH && roleB && colB_in && oy < iH) {
                 float b0 = 0.f, b1 = 0.f, b2 = 0.f;
 #pragma unroll
                 for (int d = -kH; d <= kH; ++d) {
-                    const int s = (K - kH + d + 22) % 11;
+                    const int s = (KP - kH + d + 22) % 11;
                     const float w = tap(d);
                     b0 = fmaf(w, rb[s][0], b0);
                     b1 = fmaf(w, rb[s][1], b1);
                     b2 = fmaf(w, rb[s][2], b2);
                 }
                 const size_t pix = (size_t)oy * W + gxB;
-                const float2 xy = s_xy[(i - 2 * kH) % 12][cB + 2 * kH][chB];
+                const float2 xy = s_xy[(i - 2 * kH - 1) % 13][cB][chB];
                 const float x = xy.x, y = xy.y;
                 l1 += (double)fabsf(x - y);
                 if (dl_out != nullptr) {
@@ -289,23 +303,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void l
                     if (chB == 0) o[3] = 0.0f;
                 }
             }
-        } else {
-            lds_barrier();  // the stats pass has read the input row before the next load
         }
     };
-    for (int i0 = 0; i0 < nrows; i0 += 11) {
+    for (int i0 = 0; i0 <= nrows; i0 += 11) {
         using std::integral_constant;
         step(integral_constant<int, 0>{}, i0);
-        if (i0 + 1 < nrows) step(integral_constant<int, 1>{}, i0 + 1);
-        if (i0 + 2 < nrows) step(integral_constant<int, 2>{}, i0 + 2);
-        if (i0 + 3 < nrows) step(integral_constant<int, 3>{}, i0 + 3);
-        if (i0 + 4 < nrows) step(integral_constant<int, 4>{}, i0 + 4);
-        if (i0 + 5 < nrows) step(integral_constant<int, 5>{}, i0 + 5);
-        if (i0 + 6 < nrows) step(integral_constant<int, 6>{}, i0 + 6);
-        if (i0 + 7 < nrows) step(integral_constant<int, 7>{}, i0 + 7);
-        if (i0 + 8 < nrows) step(integral_constant<int, 8>{}, i0 + 8);
-        if (i0 + 9 < nrows) step(integral_constant<int, 9>{}, i0 + 9);
-        if (i0 + 10 < nrows) step(integral_constant<int, 10>{}, i0 + 10);
+        if (i0 + 1 <= nrows) step(integral_constant<int, 1>{}, i0 + 1);
+        if (i0 + 2 <= nrows) step(integral_constant<int, 2>{}, i0 + 2);
+        if (i0 + 3 <= nrows) step(integral_constant<int, 3>{}, i0 + 3);
+        if (i0 + 4 <= nrows) step(integral_constant<int, 4>{}, i0 + 4);
+        if (i0 + 5 <= nrows) step(integral_constant<int, 5>{}, i0 + 5);
+        if (i0 + 6 <= nrows) step(integral_constant<int, 6>{}, i0 + 6);
+        if (i0 + 7 <= nrows) step(integral_constant<int, 7>{}, i0 + 7);
+        if (i0 + 8 <= nrows) step(integral_constant<int, 8>{}, i0 + 8);
+        if (i0 + 9 <= nrows) step(integral_constant<int, 9>{}, i0 + 9);
+        if (i0 + 10 <= nrows) step(integral_constant<int, 10>{}, i0 + 10);
     }
     // Deterministic block reduction of the loss sums (double).
     const int lane = tid & 63, wave = tid >> 6;
