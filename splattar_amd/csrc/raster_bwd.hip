@@ -200,18 +200,25 @@ __device__ __forceinline__ void add_hit(PixState& p, const Record& r, const HitE
         p.suf_cw += coef * w;
     }
     p.T = t_before;
-    // Geometry terms exist only for contributing, unclamped hits.
+    // Geometry terms exist only for contributing, unclamped hits, and the
+    // plane terms only on the ray branch: the masks are applied once, to ghat
+    // and to 1/D, so the terms derived from them are exact zeros (possibly
+    // -0, which no sum tells apart) where the terms are absent. dl_dalpha,
+    // u and v are finite (|1/D| <= 1e12 and finite record coefficients), so
+    // no inf * 0 arises.
     const bool geo = act && !h.clamped;
-    put<AUX, ACC>(g[kGOp], geo ? dl_dalpha * h.ghat : 0.0f);
-    const float gd = geo ? h.ghat * (dl_dalpha * r.col_op.w) : 0.0f;  // ghat dL/dghat
+    const float ghat_g = geo ? h.ghat : 0.0f;
+    const float inv_r = h.ray ? h.inv_d : 0.0f;
+    put<AUX, ACC>(g[kGOp], dl_dalpha * ghat_g);
+    const float gd = ghat_g * (dl_dalpha * r.col_op.w);  // ghat dL/dghat
     const float dl_du = -h.u * gd;
     const float dl_dv = -h.v * gd;
-    const float dnu = h.ray ? dl_du * h.inv_d : 0.0f;
-    const float dnv = h.ray ? dl_dv * h.inv_d : 0.0f;
+    const float dnu = dl_du * inv_r;
+    const float dnv = dl_dv * inv_r;
     // Colour-only (AUX = false): dl_dt = 0, and the record's depth is not
     // staged, so t_hit must not enter (it may hold anything).
     const float dD_num = AUX ? h.u * dl_du + h.v * dl_dv + t_hit * dl_dt : h.u * dl_du + h.v * dl_dv;
-    const float dD = h.ray ? -dD_num * h.inv_d : 0.0f;
+    const float dD = -dD_num * inv_r;
     put<AUX, ACC>(g[kGA + 0], dnu * p.sx);
     put<AUX, ACC>(g[kGA + 1], dnu * p.sy);
     put<AUX, ACC>(g[kGA + 2], dnu);
