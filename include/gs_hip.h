@@ -516,8 +516,10 @@ gs_status gs_trainer_step(gs_trainer t, void* stream, const uint32_t* view_ids, 
  * queue with 8 or 4 workgroups per CU, one workgroup per tile} at 8-bit
  * digits; stage 2 tries radix digit widths {8, 4} at the stage-1 winner.
  * Each candidate runs one settling batched step on `view_ids` and `reps`
- * more timed one by one, over two interleaved rounds; the fastest step wins.
- * A candidate whose binning overflows the entry capacity is skipped
+ * more timed one by one, over two interleaved rounds; the fastest step wins,
+ * candidates within 2 % of it counting as ties that go to the earliest in the
+ * order above (16-px tiles, batches 256 / 64, 8 workgroups per CU, 8-bit
+ * digits first). A candidate whose binning overflows the entry capacity is skipped
  * (GS_ERR_OVERFLOW if all do). The steps are real training steps (no
  * candidate changes a step's result beyond float-atomic order), but the
  * parameters, Adam moments and step count are saved first and restored

@@ -2708,6 +2708,11 @@ gs_status gs_trainer_autotune(gs_trainer t, const uint32_t* view_ids, uint32_t n
     // Candidates of a stage interleaved over two rounds (drift hits all
     // alike); each step timed on its own; a candidate's score is its fastest
     // step. A candidate whose binning overflows the entry capacity is out.
+    // Scores within kTie of the fastest are ties, won by the earliest
+    // candidate in the list (the default schedule first): the steps timed
+    // here are the first of a run, and a schedule that wins them by less
+    // (such as no tile queue at C3) can lose over a run whose per-pixel
+    // walks deepen as the scene trains.
     auto run_stage = [&](const std::vector<Cand>& cs, size_t* best) {
         std::vector<float> best_ms(cs.size(), 3.4e38f);
         std::vector<bool> overflowed(cs.size(), false);
@@ -2747,9 +2752,13 @@ gs_status gs_trainer_autotune(gs_trainer t, const uint32_t* view_ids, uint32_t n
                 }
             }
         }
-        size_t b = cs.size();
+        constexpr float kTie = 0.02f;
+        float fastest = 3.4e38f;
         for (size_t i = 0; i < cs.size(); ++i)
-            if (!overflowed[i] && (b == cs.size() || best_ms[i] < best_ms[b])) b = i;
+            if (!overflowed[i]) fastest = std::min(fastest, best_ms[i]);
+        size_t b = cs.size();
+        for (size_t i = 0; i < cs.size() && b == cs.size(); ++i)
+            if (!overflowed[i] && best_ms[i] <= fastest * (1.0f + kTie)) b = i;
         *best = b;
     };
     size_t b1 = 0, b2 = 0;
