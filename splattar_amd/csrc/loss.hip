@@ -37,20 +37,22 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 // owns a strip of kSW output columns and SH output rows of one view (all
 // three channels) and walks its input rows top to bottom, one row per step:
 //   load    the input row (kSW + 20 pixels: the blur of the coefficient maps
-//           needs them 5 px beyond the strip, and those need 5 px more) into
-//           LDS as (x, y, x^2, y^2) and xy per channel, and (x, y) into a
-//           12-row LDS ring for the output stage (each pixel is read from
-//           HBM once);
+//           needs them 5 px beyond the strip, and those need 5 px more) as
+//           (x, y) per channel into a 13-row LDS ring (each pixel is read
+//           from HBM once);
 //   stats   thread (channel, stat column) of kAC = kSW + 10 columns blurs the
-//           five horizontally and keeps the last 11 rows in a register ring;
+//           five statistics x, y, x^2, y^2, xy horizontally (three with the
+//           target's precomputed) and keeps the last 11 rows in a register
+//           ring;
 //           once 11 are there, the vertical blur gives the window statistics
 //           of the stat row 5 above, then SSIM and the coefficients C_mu,
 //           C_x2, C_xy (0 outside the image: zero padding), which go to an
 //           LDS row;
-//   output  thread (channel, output column) blurs that row horizontally into
-//           a second 11-row ring and, 5 rows later, vertically, giving
+//   output  thread (channel, output column) blurs that row horizontally (a
+//           step later, into a second 11-row ring) and, 5 rows later,
+//           vertically, giving
 //           dL/dx = (1-l) sign(x-y)/3P + B(C_mu) + 2x B(C_x2) + y B(C_xy)
-//           at the output row 10 above the input row.
+//           at the output row 11 above the input row.
 // The coefficient maps never leave the chip; a strip re-blurs a 10-px
 // horizontal halo and its band a 20-row vertical one. Loss partial sums
 // (sum |x - y|, sum SSIM over the workgroup's own outputs) accumulate in
@@ -70,28 +72,27 @@ struct LossViews {
 };
 
 // TS: the target's window mean and second moment come from tstats
-// ((target pixel) * 3 + channel) instead of being blurred here; four waves
-// per SIMD (at most 128 VGPRs: the two register rings take 66; 36 KB of
-// LDS), a fourth workgroup per CU hiding the row barrier better than the
-// third's spare registers would. Without TS the float4 input rows need
-// 40.5 KB of LDS, three workgroups per CU, so the registers are let grow.
+// ((target pixel) * 3 + channel) instead of being blurred here. Four waves
+// per SIMD (at most 128 VGPRs; 35 KB of LDS): a fourth workgroup per CU
+// hides the row barrier better than the third's spare registers would. The
+// TS variant (register rings of 66) is held to that; the other (rings of
+// 88) fits it too when left a looser bound (forced, it spills two).
 template <int SH, bool TS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TS ? 4 : 3))) void l1_ssim_fused_kernel(
     LossViews lv, const float4* __restrict__ img_base, const float4* __restrict__ tgt_base,
     float4* __restrict__ dl_base, double* __restrict__ partials, const CamParams* __restrict__ cams,
     const float2* __restrict__ tstats) {
     constexpr int NR = TS ? 3 : 5;  // statistics blurred per row: (x, x^2, xy) or (x, y, x^2, y^2, xy)
-    using PT = std::conditional_t<TS, float2, float4>;
     // Channels are the fastest-varying index of every LDS array, as they are
     // of the thread roles (thread 3c + ch), so a wave's lanes read and write
     // consecutive elements: no bank conflicts where a wave spans two channels.
-    __shared__ PT s_p[2][kIC][3];        // (x, x^2) or (x, y, x^2, y^2) of the input row, double-buffered
-    __shared__ float s_q[2][kIC][3];     // xy
     __shared__ float s_c[2][3][kAC][3];  // C_mu, C_x2, C_xy of a stat row, double-buffered
-    // (x, y) of the strip's output columns for the last input rows: the
-    // output stage reads its row 11 steps after the load (13 slots: the
-    // loads of the next two steps never meet a slot still being read).
-    __shared__ float2 s_xy[13][kSW][3];
+    // (x, y) of the last input rows, the only staged form of the input: the
+    // horizontal statistics blur reads its row in the step that loads it and
+    // forms x^2 and xy in registers (8 B of LDS per tap instead of 12 or 20),
+    // the output stage reads its row 11 steps later (13 slots: the loads of
+    // the next two steps never meet a slot still being read).
+    __shared__ float2 s_xy[13][kIC][3];
     __shared__ double s_red[2][4];
     const int tid = threadIdx.x;
     const uint32_t b = xcd_swizzle(blockIdx.x, gridDim.x);
@@ -163,24 +164,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TS ? 4 : 3)
                 a = make_float4(0.f, 0.f, 0.f, 0.f);
                 t = a;
             }
-            if constexpr (TS) {
-                s_p[buf][tid][0] = make_float2(a.x, a.x * a.x);
-                s_p[buf][tid][1] = make_float2(a.y, a.y * a.y);
-                s_p[buf][tid][2] = make_float2(a.z, a.z * a.z);
-            } else {
-                s_p[buf][tid][0] = make_float4(a.x, t.x, a.x * a.x, t.x * t.x);
-                s_p[buf][tid][1] = make_float4(a.y, t.y, a.y * a.y, t.y * t.y);
-                s_p[buf][tid][2] = make_float4(a.z, t.z, a.z * a.z, t.z * t.z);
-            }
-            s_q[buf][tid][0] = a.x * t.x;
-            s_q[buf][tid][1] = a.y * t.y;
-            s_q[buf][tid][2] = a.z * t.z;
-            if (tid >= 2 * kH && tid < 2 * kH + kSW) {
-                const int slot = i % 13;
-                s_xy[slot][tid - 2 * kH][0] = make_float2(a.x, t.x);
-                s_xy[slot][tid - 2 * kH][1] = make_float2(a.y, t.y);
-                s_xy[slot][tid - 2 * kH][2] = make_float2(a.z, t.z);
-            }
+            const int slot = i % 13;
+            s_xy[slot][tid][0] = make_float2(a.x, t.x);
+            s_xy[slot][tid][1] = make_float2(a.y, t.y);
+            s_xy[slot][tid][2] = make_float2(a.z, t.z);
         }
         if (roleA && i > 2 * kH) {  // the previous step's C-map row
             s_c[buf ^ 1][0][cA][chA] = cm;
@@ -190,15 +177,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TS ? 4 : 3)
         fetch(gy + 1);  // every thread, every step: no branch for the loads to wait at
         lds_barrier();
         if (roleA && i < nrows) {
+            // Per tap: wx = w x, sums wx, wx x, wx y (and wy, wy y without TS;
+            // target_stats_kernel forms the target's the same way).
+            const int slot = i % 13;
             if constexpr (TS) {
                 float h0 = 0.f, h2 = 0.f, h4 = 0.f;
 #pragma unroll
                 for (int j = 0; j <= 2 * kH; ++j) {
-                    const float2 p = s_p[buf][cA + j][chA];
-                    const float w = tap(j - kH);
-                    h0 = fmaf(w, p.x, h0);
-                    h2 = fmaf(w, p.y, h2);
-                    h4 = fmaf(w, s_q[buf][cA + j][chA], h4);
+                    const float2 p = s_xy[slot][cA + j][chA];
+                    const float wx = tap(j - kH) * p.x;
+                    h0 = h0 + wx;
+                    h2 = fmaf(wx, p.x, h2);
+                    h4 = fmaf(wx, p.y, h4);
                 }
                 ra[K][0] = h0;
                 ra[K][1] = h2;
@@ -207,13 +197,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TS ? 4 : 3)
                 float h0 = 0.f, h1 = 0.f, h2 = 0.f, h3 = 0.f, h4 = 0.f;
 #pragma unroll
                 for (int j = 0; j <= 2 * kH; ++j) {
-                    const float4 p = s_p[buf][cA + j][chA];
+                    const float2 p = s_xy[slot][cA + j][chA];
                     const float w = tap(j - kH);
-                    h0 = fmaf(w, p.x, h0);
-                    h1 = fmaf(w, p.y, h1);
-                    h2 = fmaf(w, p.z, h2);
-                    h3 = fmaf(w, p.w, h3);
-                    h4 = fmaf(w, s_q[buf][cA + j][chA], h4);
+                    const float wx = w * p.x, wy = w * p.y;
+                    h0 = h0 + wx;
+                    h1 = h1 + wy;
+                    h2 = fmaf(wx, p.x, h2);
+                    h3 = fmaf(wy, p.y, h3);
+                    h4 = fmaf(wx, p.y, h4);
                 }
                 ra[K][0] = h0;
                 ra[K][1] = h1;
@@ -293,7 +284,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TS ? 4 : 3)
                     b2 = fmaf(w, rb[s][2], b2);
                 }
                 const size_t pix = (size_t)oy * W + gxB;
-                const float2 xy = s_xy[(i - 2 * kH - 1) % 13][cB][chB];
+                const float2 xy = s_xy[(i - 2 * kH - 1) % 13][cB + 2 * kH][chB];
                 const float x = xy.x, y = xy.y;
                 l1 += (double)fabsf(x - y);
                 if (dl_out != nullptr) {
@@ -371,13 +362,13 @@ __global__ __launch_bounds__(256) void target_stats_kernel(const float4* __restr
                 if (c < 0 || c >= iW) continue;
                 const float4 v = tgt[(size_t)r * W + c];
                 const float w = tap(j - kH);
-                const float sx = v.x * v.x, sy = v.y * v.y, sz = v.z * v.z;
-                h[0] = fmaf(w, v.x, h[0]);
-                h[1] = fmaf(w, v.y, h[1]);
-                h[2] = fmaf(w, v.z, h[2]);
-                h2[0] = fmaf(w, sx, h2[0]);
-                h2[1] = fmaf(w, sy, h2[1]);
-                h2[2] = fmaf(w, sz, h2[2]);
+                const float wx = w * v.x, wy = w * v.y, wz = w * v.z;
+                h[0] = h[0] + wx;
+                h[1] = h[1] + wy;
+                h[2] = h[2] + wz;
+                h2[0] = fmaf(wx, v.x, h2[0]);
+                h2[1] = fmaf(wy, v.y, h2[1]);
+                h2[2] = fmaf(wz, v.z, h2[2]);
             }
         }
         const float w = tap(d);
