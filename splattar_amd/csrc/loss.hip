@@ -123,12 +123,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TS ? 4 : 3)
     constexpr int nrows = SH + 4 * kH;  // input rows y0 - 10 .. y0 + SH + 9
     // Step i (0 <= i <= nrows): input row y0 - 10 + i and stat row
     // y0 - 15 + i (i < nrows); the C-map row of the previous step's stat row
-    // is blurred horizontally and output row y0 - 21 + i is finished. The
-    // input ring slot of step i is i % 11 (the template K), the C-map ring
-    // slot of step i - 1 is (i - 1) % 11 (KP). One barrier per step: the
-    // input row and the previous step's C-map row go into alternate buffers
-    // before it, so a buffer is rewritten two steps after it is read, when
-    // every thread has passed the barrier between.
+    // is blurred horizontally and output row y0 - 21 + i is finished. In the
+    // register rings, step i's statistics row sits in slot i % 11 (the
+    // template K) and step i - 1's blurred C-map row in (i - 1) % 11 (KP).
+    // One barrier per step: before it, the input row goes into LDS ring slot
+    // i % 13 (read by this step's statistics blur and, 11 steps later, by
+    // the output) and the previous step's C-map row into buffer (i - 1) & 1;
+    // a slot or buffer is rewritten at the earliest two steps after its last
+    // read, when every thread has passed the barrier between.
     // The loader threads hold the next input row in registers: its global
     // loads are issued a step ahead and overlap the row's work. They are
     // unconditional (clamped addresses; threads past the loaders repeat the
