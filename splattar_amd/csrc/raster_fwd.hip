@@ -86,10 +86,12 @@ __global__ __launch_bounds__((TW / 8) * (TW / 8) * 64) void raster_fwd_kernel(
         const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
         const uint32_t start = ranges[2 * (size_t)gtile], end = ranges[2 * (size_t)gtile + 1];
-        float T = 1.0f;
+        // A pixel is done once T < kTTerminate (forward.rs stops there); T
+        // only falls, so the test on T itself is the done flag, and pixels
+        // outside the image start done (T = 0; nothing of theirs is written).
+        float T = inside ? 1.0f : 0.0f;
         float cr = 0.f, cg = 0.f, cb = 0.f, wsum = 0.f, dsum = 0.f, nx = 0.f, ny = 0.f, nz = 0.f;
         uint32_t last = 0;
-        bool done = !inside;
         // Trace window slot of this pixel (TRACE only).
         const bool traced = TRACE && inside && s_view == trace.view && (uint32_t)px >= trace.x0 &&
                             (uint32_t)px < trace.x0 + trace.w && (uint32_t)py >= trace.y0 &&
@@ -144,7 +146,7 @@ __global__ __launch_bounds__((TW / 8) * (TW / 8) * 64) void raster_fwd_kernel(
 #pragma unroll
             for (int w = 0; w < SW; ++w) my_count += s_qcnt[w][wave];
             __syncthreads();
-            if (__any(!done)) {
+            if (__any(T >= kTTerminate)) {
                 auto load = [&](uint32_t j, Record& r, float& lthr) {
                     r.m_nt = s_m[j];
                     r.a_px = s_a[j];
@@ -155,7 +157,7 @@ __global__ __launch_bounds__((TW / 8) * (TW / 8) * 64) void raster_fwd_kernel(
                 };
                 auto composite = [&](uint32_t j, const Record& r, float lthr) {
                     const HitEval h = eval_hit(r, sx, sy, pxc, pyc, lthr);
-                    if (!done && h.alpha > 0.0f) {
+                    if (T >= kTTerminate && h.alpha > 0.0f) {
                         const float w = T * h.alpha;
                         cr += r.col_op.x * w;
                         cg += r.col_op.y * w;
@@ -169,7 +171,6 @@ __global__ __launch_bounds__((TW / 8) * (TW / 8) * 64) void raster_fwd_kernel(
                         }
                         T *= 1.0f - h.alpha;
                         last = base - start + j + 1u;
-                        if (T < kTTerminate) done = true;
                         if (TRACE && traced) {
                             if (tcount < trace.cap)
                                 trace.lists[tslot * trace.cap + tcount] =
@@ -192,13 +193,13 @@ __global__ __launch_bounds__((TW / 8) * (TW / 8) * 64) void raster_fwd_kernel(
                     load(j0, r0, l0);
                     load(j1, r1, l1);
                     composite(j0, r0, l0);
-                    if (!__any(!done)) break;
+                    if (!__any(T >= kTTerminate)) break;
                     if (!two) break;
                     composite(j1, r1, l1);
-                    if (!__any(!done)) break;
+                    if (!__any(T >= kTTerminate)) break;
                 }
             }
-            if (__syncthreads_count(!done) == 0) break;
+            if (__syncthreads_count(T >= kTTerminate) == 0) break;
         }
         if (inside) {
             const size_t pi = (size_t)cam.pix_base + (size_t)py * cam.width + (size_t)px;
