@@ -500,13 +500,11 @@ __device__ __forceinline__ HitEval eval_hit(const Record& r, float sx, float sy,
     h.ray = e_ray >= e_scr;
     const float e = h.ray ? e_ray : e_scr;
     if (!h.ray) h.t = r.nrm_dep.w;
-    if (e < lthr) {  // cannot reach 1/255 (see record_lthr)
-        h.ghat = 0.0f;
-        h.clamped = false;
-        h.alpha = 0.0f;
-        return h;
-    }
-    h.ghat = __expf(e);
+    // e < lthr cannot reach 1/255 (see record_lthr): ghat = 0, hence alpha
+    // = 0 and no clamp. A select, not a branch: exp runs on every lane (e is
+    // finite, the screen term bounds it below), and no lane's state forks.
+    const float g = __expf(e);
+    h.ghat = e < lthr ? 0.0f : g;
     const float raw = r.col_op.w * h.ghat;
     h.clamped = raw > kAlphaClamp;
     float a = fminf(raw, kAlphaClamp);
