@@ -192,8 +192,9 @@ __global__ __launch_bounds__((TW / 8) * (TW / 8) * 64) void raster_fwd_kernel(
                     float l0, l1;
                     load(j0, r0, l0);
                     load(j1, r1, l1);
+                    // One saturation test per round: a wave done after j0
+                    // walks j1 with every lane masked off (no contribution).
                     composite(j0, r0, l0);
-                    if (!__any(T >= kTTerminate)) break;
                     if (!two) break;
                     composite(j1, r1, l1);
                     if (!__any(T >= kTTerminate)) break;
