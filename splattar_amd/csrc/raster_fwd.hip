@@ -37,13 +37,14 @@ __device__ __forceinline__ uint32_t view_of_tile(const CamParams* cams, uint32_t
 // accumulators: aux = (T_final, 0, 0, 0), normal is not written. TRACE
 // additionally records every contribution of the trace window's pixels
 // (FwdTrace; a verification variant, never used by training steps).
-// The colour-only 16-px variants are held to six waves per SIMD (at most 80
-// VGPRs; four spilled registers, stored before the tile loop and reloaded
-// once per tile, none in the batch or record loops): a sixth workgroup per
-// CU (20 KB of LDS each) hides more of the list walk's LDS latency.
+// The colour-only 16-px variants are held to seven waves per SIMD (at most
+// 72 VGPRs; five spilled registers, stored before the tile loop and
+// reloaded once per tile, none in the batch or record loops): more
+// workgroups per CU (20 KB of LDS each) hide more of the list walk's LDS
+// latency. At eight the spills reach the batch loop.
 template <bool AUX, int TW, int BATCH, bool TRACE = false>
 __global__ __launch_bounds__((TW / 8) * (TW / 8) * 64)
-__attribute__((amdgpu_waves_per_eu((!AUX && TW == 16) ? 6 : 1))) void raster_fwd_kernel(
+__attribute__((amdgpu_waves_per_eu((!AUX && TW == 16) ? 7 : 1))) void raster_fwd_kernel(
     const CamParams* __restrict__ cams, uint32_t n_views, const uint32_t* __restrict__ ranges,
     const uint32_t* __restrict__ payloads, const Record* __restrict__ recs, FwdOut out, bool cull,
     uint32_t total_tiles, TileQueue queue, uint32_t tile_begin, FwdTrace trace) {
