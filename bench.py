@@ -457,12 +457,15 @@ def main():
     run(tr, tr.GRAPH, rest, a.warmup + a.steps, ev, a.warmup + a.steps)
     ctx.synchronize()
     graph_ms = ev.intervals(total)
+    # The last step's entry count, read before the PSNR renders (one view
+    # each) replace it with one view's count.
+    entries_late = tr.entry_count()
     log(f"graph: {a.steps} timed steps in {t_graph:.3f}s -> {a.steps / t_graph:.2f} steps/s; "
-        f"entries/step {entries_step}; per-step ms over {total} steps: "
+        f"entries/step {entries_step} (step {a.warmup + a.steps}), {entries_late} (step {total}); "
+        f"per-step ms over {total} steps: "
         f"median(10..) {np.median(graph_ms[10:]):.3f}, steps 0-9 {np.round(graph_ms[:10], 3).tolist()}, "
         f"by 100-step block {[round(float(np.median(graph_ms[b:b + 100])), 3) for b in range(0, total, 100)]}")
     equal_loss = {"steps": total, "psnr_batched_graph_db": round(psnr(tr), 4)}
-    entries_late = tr.entry_count()
     tr.close()
 
     # ---- plain per-view steps from the same initial state: the same views,
