@@ -1068,7 +1068,9 @@ int oref_trace(const oref_scene* s, const oref_camera* cam, int64_t cap, uint32_
     return 0;
 }
 
-// Decision replay (a test aid with no forward.rs counterpart): the
+// Decision replay (a test aid with no forward.rs counterpart: it separates
+// FP32 rounding from the discrete decisions that FP32 and FP64 can take
+// differently near the alpha skip, clamp, stop and branch thresholds): the
 // FP64 forward and backward along the discrete decisions another evaluation
 // took. For window pixel p (row-major over the camera's crop window),
 // lists[p * cap + k], k < counts[p], is its k-th contribution in compositing
