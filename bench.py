@@ -26,7 +26,9 @@ carries run-to-run noise). Inputs are resident in HBM before timing starts.
 
 Roofline: the dominant stage's algorithmic bytes (per-stage formulas of the
 design notes) divided by its duration measured with HIP events on the step
-stream (profiled batched steps after the timed region). raster_hits: the
+stream (after the timed region: steps 10.. of a batched run from the initial
+state, the scene state of the committed rocprof runs, whose per-launch
+averages these match). raster_hits: the
 useful hit evaluations of the first step's views (a forward traced over the
 whole image counts every pixel's contributions; untimed) per raster pass
 second, and the backward's wave instructions per useful hit from the
@@ -506,17 +508,22 @@ def main():
                       psnr_diff_db_per_run=[round(d, 4) for d in diffs])
     log(f"equal loss over {len(runs)} run pairs: {equal_loss}")
 
-    # ---- batched without a graph, then profiled batched steps (stage times)
+    # ---- batched without a graph from the initial state: steps 10.. are
+    # profiled (stage times; the scene state matches the profiles' short
+    # runs, since per-step work grows as the scene trains), then 60 more
+    # steps timed with events.
     tb = sa.Trainer(ctx, cfg, scene.as_init(), views)
     apply_schedule(tb)
-    nb = min(total, 60)
-    run(tb, tb.BATCHED, nb, 0, ev, 0)
-    ctx.synchronize()
-    batched_ms = ev.intervals(nb)
+    warm = min(total, 10)
+    run(tb, tb.BATCHED, warm, 0)
     tb.set_profiling(True)
-    run(tb, tb.BATCHED, a.profile_steps, nb)
+    run(tb, tb.BATCHED, a.profile_steps, warm)
     stages, psteps = tb.stage_times()
     tb.set_profiling(False)
+    nb = min(total, 60)
+    run(tb, tb.BATCHED, nb, warm + a.profile_steps, ev, 0)
+    ctx.synchronize()
+    batched_ms = ev.intervals(nb)
     tb.close()
     ev.close()
     stage_ms = {k: v / max(psteps, 1) for k, v in stages.items()}
